@@ -1,0 +1,209 @@
+/*
+ * ixg.h -- C ABI of the MI355X spectral-transport library (libixg.so).
+ *
+ * Plain C, POD structs, no exceptions across the boundary.  Every entry point
+ * returns an ixg_status; a non-OK status leaves caller-owned output buffers
+ * untouched (the "written only on Done" rule of the reference's GPU backend).
+ *
+ * Each declaration cites the reference interface it replaces.  Paths are
+ * relative to the reference checkout; IX/ = crates/indicatrix/src/.
+ */
+#ifndef IXG_H
+#define IXG_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------------ */
+/* Status codes.  GpuAccumulate {Done, Declined, Cancelled} plus the        */
+/* GpuFrameError kinds (IX/renderer/gpu_backend/mod.rs:307-326,             */
+/* IX/renderer/gpu/frame/mod.rs:383-418).                                    */
+/* ------------------------------------------------------------------------ */
+typedef enum ixg_status {
+    IXG_OK = 0,            /* Done: results were ADDED into the caller's buffer */
+    IXG_DECLINED = 1,      /* scene not routable to the GPU (caller falls back)  */
+    IXG_CANCELLED = 2,     /* cancel flag observed between chunks                */
+    IXG_DEVICE_LOST = 3,   /* HIP error / injected fault; context marked lost    */
+    IXG_INVALID = 4,       /* malformed arguments (sizes, null pointers)         */
+    IXG_RESOURCE_LIMIT = 5 /* request above a documented limit                   */
+} ixg_status;
+
+/* Why ixg_route declines a scene (scene_routes_to_gpu,
+ * IX/renderer/gpu_backend/mod.rs:287-298). */
+typedef enum ixg_route_reason {
+    IXG_ROUTE_OK = 0,
+    IXG_ROUTE_UNSUPPORTED_MATERIAL = 1,    /* biaxial, scattering, zoned, edge rounding */
+    IXG_ROUTE_UNSUPPORTED_ENVIRONMENT = 2, /* HDR maps, UV lamps, tent/dome/ASET models */
+    IXG_ROUTE_DEVICE_LIMITS = 3,           /* too many planes for the LDS facet set     */
+    IXG_ROUTE_NO_DEVICE = 4
+} ixg_route_reason;
+
+#define IXG_MAX_PLANES 256   /* LDS facet-set capacity (16 B per plane)        */
+#define IXG_MAX_BANDS 8      /* absorption bands per ray (o or e), as the ref  */
+#define IXG_NUM_CHANNELS 8   /* hero-wavelength comb width                     */
+
+/* GpuFacetPlane: n.x + d <= 0 inside, unit n, 16 B (IX/geometry/plane.rs:31-53). */
+typedef struct ixg_facet_plane {
+    float normal[3];
+    float d;
+} ixg_facet_plane;
+
+/* GpuCameraParams, byte for byte (IX/renderer/buffers/camera.rs:72-88). */
+typedef struct ixg_camera {
+    float origin[3];
+    float fov_tan;
+    float forward[3];
+    float width;
+    float right[3];
+    float height;
+    float up[3];
+    uint32_t num_samples;
+} ixg_camera;
+
+/* DispersionModel (IX/optics/dispersion.rs:1-25). */
+typedef enum ixg_dispersion_kind {
+    IXG_DISP_SELLMEIER1 = 0, /* coeff = {b1, c1}                       */
+    IXG_DISP_SELLMEIER3 = 1, /* coeff = {b0, b1, b2, c0, c1, c2}        */
+    IXG_DISP_CAUCHY = 2      /* coeff = {a, b, c}                       */
+} ixg_dispersion_kind;
+
+typedef struct ixg_dispersion {
+    int32_t kind;
+    float coeff[6];
+} ixg_dispersion;
+
+/* AbsorptionBand (IX/optics/absorption.rs). shape 0 = Gaussian in wavelength,
+ * 1 = Gaussian in energy (width is then sigma in cm^-1). */
+typedef struct ixg_band {
+    float center_nm;
+    float width;
+    float peak;
+    int32_t shape;
+} ixg_band;
+
+/* The GemMaterial fields the transport reads (IX/optics/materials/mod.rs:116-190). */
+typedef struct ixg_material {
+    ixg_dispersion dispersion;            /* ordinary / isotropic curve            */
+    float birefringence_delta;            /* n_e - n_o when no e-curve is given    */
+    float c_axis[3];                      /* unit optic axis                       */
+    int32_t cubic;                        /* 1 = CrystalSystem::Cubic              */
+    int32_t has_e_dispersion;             /* uniaxial_extraordinary_dispersion     */
+    ixg_dispersion e_dispersion;
+    int32_t n_bands_o, n_bands_e;
+    ixg_band bands_o[IXG_MAX_BANDS];
+    ixg_band bands_e[IXG_MAX_BANDS];
+    float absorption_path_scale;
+    float edge_rounding_radius;           /* GPU path: must be 0                   */
+    float scattering_sigma_s;             /* GPU path: must be 0                   */
+    int32_t biaxial;                      /* GPU path: must be 0                   */
+} ixg_material;
+
+/* LightingPreset ids, the reference's persisted ids
+ * (IX/optics/raytracer/environment/mod.rs:470-515). */
+typedef enum ixg_preset {
+    IXG_PRESET_LIGHT_TENT = 0,
+    IXG_PRESET_ISO_HEMISPHERE = 1,
+    IXG_PRESET_DAYLIGHT = 7,
+    IXG_PRESET_INCANDESCENT = 8,
+    IXG_PRESET_ILLUMINANT_A = 9,
+    IXG_PRESET_RING_LIGHTS = 10,
+    IXG_PRESET_DARK_SPOTLIGHT = 11
+} ixg_preset;
+
+/* EnvironmentSource::Studio (IX/optics/raytracer/environment/source.rs:20-45). */
+typedef struct ixg_environment {
+    int32_t preset;
+    float exposure;
+    float light_yaw;
+    float light_pitch;
+    float backdrop;        /* lit tray radiance; GPU path: must be 0      */
+    float surface_glare;   /* clamped to [0,1]; 1 = off                   */
+    float head_shadow_deg; /* 16 = default, <= 0 = off                    */
+} ixg_environment;
+
+/* GpuSceneRef (IX/renderer/gpu_backend/mod.rs:217-237). */
+typedef struct ixg_scene {
+    ixg_camera camera;
+    uint32_t width, height;
+    const ixg_facet_plane* planes;
+    uint32_t n_planes;
+    const ixg_material* material;
+    uint32_t max_bounces;
+    ixg_environment environment;
+} ixg_scene;
+
+typedef struct ixg_opts {
+    int32_t device;            /* HIP device ordinal                          */
+    uint32_t chunk_paths;      /* paths per launch (0 = library default)      */
+} ixg_opts;
+
+typedef struct ixg_ctx ixg_ctx;
+
+/* Per-path integer side channel (bit-exact parity with the CPU tracer):
+ * the bounce loop's stop point and reason (RT/transport/mod.rs:68-93), a
+ * hash of the facet sequence and per-event counts used by the FLOP model. */
+typedef struct ixg_path_events {
+    uint32_t bounces;       /* loop iterations executed                    */
+    uint32_t termination;   /* 0 escaped, 2 russian roulette, 3 hit cap    */
+    uint32_t facet_hash;    /* FNV-1a over (facet index, event kind)       */
+    uint16_t n_tir, n_reflect, n_refract, n_probe;
+} ixg_path_events;
+
+/* ---- lifecycle (GpuBackend::new / disabled / is_lost / adapter_label,
+ *      IX/renderer/gpu_backend/backend.rs:125-165) ---- */
+ixg_status ixg_create(const ixg_opts* opts, ixg_ctx** out);
+void ixg_destroy(ixg_ctx* ctx);
+int ixg_is_lost(const ixg_ctx* ctx);
+ixg_status ixg_adapter_label(const ixg_ctx* ctx, char* buf, size_t n);
+ixg_status ixg_last_error(const ixg_ctx* ctx, char* buf, size_t n);
+/* try_recover (IX/renderer/gpu_backend/backend.rs:704): clears the lost flag. */
+ixg_status ixg_try_recover(ixg_ctx* ctx);
+/* fail_on_turn test seam (IX/renderer/gpu_backend/backend.rs:81-85): the
+ * n-th chunk from now reports IXG_DEVICE_LOST without touching the device. */
+ixg_status ixg_inject_fault(ixg_ctx* ctx, int32_t fail_on_chunk);
+
+/* scene_routes_to_gpu (IX/renderer/gpu_backend/mod.rs:287-298). */
+ixg_route_reason ixg_route(const ixg_scene* scene);
+
+/* ---- film ----
+ * GpuBackend::try_accumulate_cancellable (backend.rs:226): traces absolute
+ * sample indices [sample_offset, sample_offset+spp) for every pixel and ADDS
+ * the per-pixel XYZ sums (FP64, non-finite samples dropped but counted) into
+ * accum_xyz[w*h*3] only on IXG_OK.  cancel may be NULL. */
+ixg_status ixg_accumulate(ixg_ctx* ctx, const ixg_scene* scene, uint32_t sample_offset,
+                          uint32_t spp, double* accum_xyz, const volatile uint32_t* cancel);
+/* Same, with accum_xyz a device pointer (inputs resident in HBM), on the
+ * library's stream; no host copy of the result. */
+ixg_status ixg_accumulate_device(ixg_ctx* ctx, const ixg_scene* scene, uint32_t sample_offset,
+                                 uint32_t spp, double* d_accum_xyz);
+/* Per-path diagnostics for the parity tiers: out_xyz[paths*3] (f32, raw
+ * per-sample XYZ) and events[paths] for paths = w*h*spp in (pixel, sample)
+ * order.  Either output may be NULL. */
+ixg_status ixg_trace_paths(ixg_ctx* ctx, const ixg_scene* scene, uint32_t sample_offset,
+                           uint32_t spp, float* out_xyz, ixg_path_events* events);
+/* Event totals of one film call (FLOP model input), summed on the device. */
+ixg_status ixg_count_events(ixg_ctx* ctx, const ixg_scene* scene, uint32_t sample_offset,
+                            uint32_t spp, uint64_t totals[8]);
+/* Average duration (ms) of the last film call's transport kernel launches,
+ * measured with HIP events on the library's stream. */
+double ixg_last_kernel_ms(const ixg_ctx* ctx);
+uint32_t ixg_last_kernel_launches(const ixg_ctx* ctx);
+
+/* ---- host helpers (same f32 arithmetic as the reference, glibc libm) ---- */
+/* Camera::new (RT/camera.rs:108). */
+void ixg_camera_new(float yaw, float pitch, float distance, float fov_deg, uint32_t width,
+                    uint32_t height, uint32_t spp, ixg_camera* out);
+/* StandardGemCuts::standard_round_brilliant (IX/geometry/cuts/round_brilliant.rs:71). */
+uint32_t ixg_cut_standard_round_brilliant(ixg_facet_plane* out /* >= 74 */);
+/* 57-facet round brilliant from crown/pavilion angles (deg) and table ratio. */
+uint32_t ixg_cut_round_brilliant(float crown_deg, float pavilion_deg, float table_ratio,
+                                 ixg_facet_plane* out /* >= 74 */);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* IXG_H */

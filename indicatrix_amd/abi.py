@@ -1,0 +1,102 @@
+"""ctypes mirror of include/ixg.h (the library's C ABI structs and enums).
+
+Field order and types follow the header byte for byte; ``check_layout`` asserts
+the sizes the header implies so a drift fails loudly at import.
+"""
+import ctypes as C
+
+IXG_OK, IXG_DECLINED, IXG_CANCELLED, IXG_DEVICE_LOST, IXG_INVALID, IXG_RESOURCE_LIMIT = range(6)
+STATUS_NAMES = {0: "OK", 1: "DECLINED", 2: "CANCELLED", 3: "DEVICE_LOST", 4: "INVALID",
+                5: "RESOURCE_LIMIT"}
+
+ROUTE_OK, ROUTE_UNSUPPORTED_MATERIAL, ROUTE_UNSUPPORTED_ENVIRONMENT, ROUTE_DEVICE_LIMITS, \
+    ROUTE_NO_DEVICE = range(5)
+
+MAX_PLANES = 256
+MAX_BANDS = 8
+NUM_CHANNELS = 8
+
+DISP_SELLMEIER1, DISP_SELLMEIER3, DISP_CAUCHY = 0, 1, 2
+
+# LightingPreset persisted ids (IX/optics/raytracer/environment/mod.rs:470-515).
+PRESETS = {
+    "LightTent": 0,
+    "IsoHemisphere": 1,
+    "Daylight": 7,
+    "Incandescent": 8,
+    "IlluminantA": 9,
+    "RingLights": 10,
+    "DarkSpotlight": 11,
+}
+
+
+class FacetPlane(C.Structure):
+    _fields_ = [("normal", C.c_float * 3), ("d", C.c_float)]
+
+
+class Camera(C.Structure):
+    _fields_ = [("origin", C.c_float * 3), ("fov_tan", C.c_float),
+                ("forward", C.c_float * 3), ("width", C.c_float),
+                ("right", C.c_float * 3), ("height", C.c_float),
+                ("up", C.c_float * 3), ("num_samples", C.c_uint32)]
+
+
+class Dispersion(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("coeff", C.c_float * 6)]
+
+
+class Band(C.Structure):
+    _fields_ = [("center_nm", C.c_float), ("width", C.c_float), ("peak", C.c_float),
+                ("shape", C.c_int32)]
+
+
+class Material(C.Structure):
+    _fields_ = [("dispersion", Dispersion),
+                ("birefringence_delta", C.c_float),
+                ("c_axis", C.c_float * 3),
+                ("cubic", C.c_int32),
+                ("has_e_dispersion", C.c_int32),
+                ("e_dispersion", Dispersion),
+                ("n_bands_o", C.c_int32), ("n_bands_e", C.c_int32),
+                ("bands_o", Band * MAX_BANDS), ("bands_e", Band * MAX_BANDS),
+                ("absorption_path_scale", C.c_float),
+                ("edge_rounding_radius", C.c_float),
+                ("scattering_sigma_s", C.c_float),
+                ("biaxial", C.c_int32)]
+
+
+class Environment(C.Structure):
+    _fields_ = [("preset", C.c_int32), ("exposure", C.c_float), ("light_yaw", C.c_float),
+                ("light_pitch", C.c_float), ("backdrop", C.c_float),
+                ("surface_glare", C.c_float), ("head_shadow_deg", C.c_float)]
+
+
+class Scene(C.Structure):
+    _fields_ = [("camera", Camera),
+                ("width", C.c_uint32), ("height", C.c_uint32),
+                ("planes", C.POINTER(FacetPlane)), ("n_planes", C.c_uint32),
+                ("material", C.POINTER(Material)),
+                ("max_bounces", C.c_uint32),
+                ("environment", Environment)]
+
+
+class Opts(C.Structure):
+    _fields_ = [("device", C.c_int32), ("chunk_paths", C.c_uint32)]
+
+
+class PathEvents(C.Structure):
+    _fields_ = [("bounces", C.c_uint32), ("termination", C.c_uint32),
+                ("facet_hash", C.c_uint32),
+                ("n_tir", C.c_uint16), ("n_reflect", C.c_uint16),
+                ("n_refract", C.c_uint16), ("n_probe", C.c_uint16)]
+
+
+def check_layout():
+    assert C.sizeof(FacetPlane) == 16
+    assert C.sizeof(Camera) == 64
+    assert Camera.fov_tan.offset == 12 and Camera.width.offset == 28
+    assert Camera.height.offset == 44 and Camera.num_samples.offset == 60
+    assert C.sizeof(PathEvents) == 20
+
+
+check_layout()

@@ -1,0 +1,1029 @@
+/*
+ * ixo.c -- CPU oracle (test infrastructure only; see ixo.h).
+ *
+ * An FP32 restatement of the reference's isotropic spectral tracer, written
+ * against the reference's evaluation order: Rust never contracts a*b+c, so
+ * this file is built with -ffp-contract=off and uses fmaf() exactly where the
+ * reference writes mul_add.  Vector helpers follow glam 0.34's scalar Vec3
+ * order (left-associated dot, multiply-then-add) and Mat4::mul_vec4's
+ * column-by-column accumulation.  Transcendentals are glibc libm, the Linux
+ * binding of Rust's f32 methods.
+ *
+ * Parity status: pinned by the reference's golden tables (trace bits for
+ * Diamond, studio-environment bits) and closed forms -- see tests/.
+ */
+#define _GNU_SOURCE
+#include "ixo.h"
+
+#include <math.h>
+#include <pthread.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define IX_TABLE_QUAL static const
+#include "spectral_tables.h"
+
+#define NCH 8
+#define F32_PI 3.14159265358979323846f
+
+/* ------------------------------------------------------------------ */
+/* glam-order vector helpers                                            */
+/* ------------------------------------------------------------------ */
+typedef struct { float x, y, z; } v3;
+
+static inline v3 mk(float x, float y, float z) { v3 r = {x, y, z}; return r; }
+static inline v3 vadd(v3 a, v3 b) { return mk(a.x + b.x, a.y + b.y, a.z + b.z); }
+static inline v3 vsub(v3 a, v3 b) { return mk(a.x - b.x, a.y - b.y, a.z - b.z); }
+static inline v3 vmul(v3 a, float s) { return mk(a.x * s, a.y * s, a.z * s); }
+static inline v3 vneg(v3 a) { return mk(-a.x, -a.y, -a.z); }
+static inline float vdot(v3 a, v3 b) { return (a.x * b.x + a.y * b.y) + a.z * b.z; }
+static inline v3 vcross(v3 a, v3 b) {
+    return mk(a.y * b.z - b.y * a.z, a.z * b.x - b.z * a.x, a.x * b.y - b.x * a.y);
+}
+static inline float vlen_sq(v3 a) { return vdot(a, a); }
+static inline v3 vnormalize(v3 a) {
+    float r = 1.0f / sqrtf(vdot(a, a));
+    return vmul(a, r);
+}
+static inline v3 vnormalize_or_zero(v3 a) {
+    float r = 1.0f / sqrtf(vdot(a, a));
+    if (isfinite(r) && r > 0.0f) return vmul(a, r);
+    return mk(0.0f, 0.0f, 0.0f);
+}
+static inline v3 vfrom(const float* p) { return mk(p[0], p[1], p[2]); }
+/* Rust f32::clamp: NaN propagates. */
+static inline float clampf(float v, float lo, float hi) {
+    if (v < lo) v = lo;
+    if (v > hi) v = hi;
+    return v;
+}
+/* Rust f32::max/min: the non-NaN operand wins (fmaxf/fminf). */
+static inline float maxf(float a, float b) { return fmaxf(a, b); }
+static inline float minf(float a, float b) { return fminf(a, b); }
+/* powi by binary exponentiation, LLVM's expansion order. */
+static float powi_f(float x, int n) {
+    float res = 0.0f;
+    int have = 0;
+    float sq = x;
+    while (n) {
+        if (n & 1) {
+            if (have) res = res * sq;
+            else { res = sq; have = 1; }
+        }
+        n >>= 1;
+        if (n) sq = sq * sq;
+    }
+    return have ? res : 1.0f;
+}
+
+/* Stokes vectors and Mueller matrices (IX/optics/polarization.rs). */
+typedef struct { float i, q, u, v; } stokes;
+/* Column-major 4x4: m[c][r]. */
+typedef struct { float m[4][4]; } mat4;
+
+static inline stokes st_apply(const mat4* M, stokes s) {
+    /* glam Mat4::mul_vec4: res = c0*x; res += c1*y; res += c2*z; res += c3*w. */
+    float in[4] = {s.i, s.q, s.u, s.v};
+    float r[4];
+    for (int k = 0; k < 4; ++k) r[k] = M->m[0][k] * in[0];
+    for (int c = 1; c < 4; ++c)
+        for (int k = 0; k < 4; ++k) r[k] = r[k] + M->m[c][k] * in[c];
+    stokes o = {r[0], r[1], r[2], r[3]};
+    return o;
+}
+static inline stokes st_scale(stokes s, float f) {
+    stokes o = {s.i * f, s.q * f, s.u * f, s.v * f};
+    return o;
+}
+static inline float st_intensity(stokes s) { return maxf(s.i, 0.0f); }
+
+static mat4 mat_cols(const float a[16]) {
+    mat4 M;
+    for (int c = 0; c < 4; ++c)
+        for (int r = 0; r < 4; ++r) M.m[c][r] = a[c * 4 + r];
+    return M;
+}
+/* f32::midpoint: correctly rounded (a+b)/2; equals 0.5*(a+b) for the
+ * magnitudes reached here, computed in double exactly like the std impl. */
+static inline float midpointf(float a, float b) { return (float)(((double)a + (double)b) / 2.0); }
+
+/* MuellerMatrix::frame_rotation (polarization.rs:138). */
+static mat4 mueller_frame_rotation(float psi) {
+    float c2 = cosf(2.0f * psi), s2 = sinf(2.0f * psi);
+    float a[16] = {1, 0, 0, 0, 0, c2, -s2, 0, 0, s2, c2, 0, 0, 0, 0, 1};
+    return mat_cols(a);
+}
+/* MuellerMatrix::fresnel_reflection (polarization.rs:148-158). */
+static mat4 mueller_fresnel_reflection(float r_s, float r_p) {
+    float rs2 = r_s * r_s, rp2 = r_p * r_p;
+    float a = midpointf(rs2, rp2), b = 0.5f * (rs2 - rp2), c = r_s * r_p;
+    float m[16] = {a, b, 0, 0, b, a, 0, 0, 0, 0, c, 0, 0, 0, 0, c};
+    return mat_cols(m);
+}
+/* MuellerMatrix::fresnel_transmission (polarization.rs:162-180). */
+static mat4 mueller_fresnel_transmission(float n1, float n2, float cos_i, float cos_t, float t_s,
+                                         float t_p) {
+    float factor = (n2 * cos_t) / maxf(n1 * cos_i, 1e-6f);
+    float ts2 = t_s * t_s * factor, tp2 = t_p * t_p * factor;
+    float a = midpointf(ts2, tp2), b = 0.5f * (ts2 - tp2), c = t_s * t_p * factor;
+    float m[16] = {a, b, 0, 0, b, a, 0, 0, 0, 0, c, 0, 0, 0, 0, c};
+    return mat_cols(m);
+}
+/* MuellerMatrix::tir_retardation (polarization.rs:184-190). */
+static mat4 mueller_tir_retardation(float delta) {
+    float c = cosf(delta), s = sinf(delta);
+    float m[16] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, c, -s, 0, 0, s, c};
+    return mat_cols(m);
+}
+
+/* ------------------------------------------------------------------ */
+/* RNG (RT/sampling.rs)                                                 */
+/* ------------------------------------------------------------------ */
+#define FRESNEL_BRANCH_STREAM 0x9E3779B1u
+#define RUSSIAN_ROULETTE_STREAM 0x517CC1B7u
+#define PIXEL_JITTER_X_ROTATION_STREAM 0xA511E9B3u
+#define PIXEL_JITTER_Y_ROTATION_STREAM 0x63D81B23u
+#define HERO_WAVELENGTH_ROTATION_STREAM 0x1B873593u
+
+uint32_t ixo_hash_u32(uint32_t x) { /* sampling.rs:66 */
+    x *= 0x85ebca6bu;
+    x ^= x >> 13;
+    x *= 0xc2b2ae35u;
+    x ^= x >> 16;
+    return x;
+}
+static uint32_t bit_reverse(uint32_t v) {
+    uint32_t r = 0;
+    for (int i = 0; i < 32; ++i) { r = (r << 1) | (v & 1u); v >>= 1; }
+    return r;
+}
+float ixo_low_discrepancy_base2(uint32_t n) { /* sampling.rs:110 */
+    return (float)bit_reverse(n) / 4294967296.0f;
+}
+float ixo_radical_inverse_base(uint32_t n, uint32_t base) { /* sampling.rs:127 */
+    uint32_t remaining = n;
+    float val = 0.0f;
+    float inv_base = 1.0f / (float)base;
+    while (remaining > 0) {
+        uint32_t digit = remaining % base;
+        val = fmaf((float)digit, inv_base, val);
+        inv_base /= (float)base;
+        remaining /= base;
+    }
+    return val;
+}
+static float cp_rotate(float x, float offset) { /* sampling.rs:148 */
+    float sum = x + offset;
+    return sum - floorf(sum);
+}
+void ixo_sample_draws(uint32_t pixel, uint32_t sample, uint32_t* seed, float* jx, float* jy,
+                      float* hero) { /* sampling.rs:176-218 */
+    float rx = ixo_low_discrepancy_base2(ixo_hash_u32(pixel ^ PIXEL_JITTER_X_ROTATION_STREAM));
+    float ry = ixo_low_discrepancy_base2(ixo_hash_u32(pixel ^ PIXEL_JITTER_Y_ROTATION_STREAM));
+    float rh = ixo_low_discrepancy_base2(ixo_hash_u32(pixel ^ HERO_WAVELENGTH_ROTATION_STREAM));
+    *seed = ixo_hash_u32((pixel * 0x9e3779b9u) ^ (sample * 0x85ebca6bu));
+    *jx = cp_rotate(ixo_low_discrepancy_base2(sample), rx) - 0.5f;
+    *jy = cp_rotate(ixo_radical_inverse_base(sample, 3), ry) - 0.5f;
+    *hero = cp_rotate(ixo_radical_inverse_base(sample, 5), rh);
+}
+static inline float unit_draw(uint32_t seed, uint32_t bounce, uint32_t stream) {
+    /* bounce_body.rs:71-73 */
+    return (float)ixo_hash_u32(seed ^ ixo_hash_u32(bounce ^ stream)) / 4294967295.0f;
+}
+void ixo_wrapped_hero_wavelengths(float hero_rand, float out[8]) { /* transport/mod.rs:58-66 */
+    const float span = 780.0f - 380.0f;
+    float channel_width = span / 8.0f;
+    float lambda_hero = fmaf(hero_rand, span, 380.0f);
+    for (int k = 0; k < 8; ++k) {
+        float offset = fmaf((float)k, channel_width, lambda_hero - 380.0f);
+        float r = fmodf(offset, span);
+        if (r < 0.0f) r = r + fabsf(span);
+        out[k] = 380.0f + r;
+    }
+}
+
+/* ------------------------------------------------------------------ */
+/* Optical primitives                                                   */
+/* ------------------------------------------------------------------ */
+static float dispersion_unclamped(const ixg_dispersion* d, float lambda_nm) {
+    /* dispersion.rs:181-210 */
+    float um = lambda_nm * 1e-3f;
+    float l2 = um * um;
+    const float* c = d->coeff;
+    switch (d->kind) {
+    case IXG_DISP_SELLMEIER1: {
+        float n2 = 1.0f + (c[0] * l2) / (l2 - c[1]);
+        return sqrtf(maxf(n2, 1.0f));
+    }
+    case IXG_DISP_SELLMEIER3: {
+        float n2 = 1.0f;
+        n2 += (c[0] * l2) / (l2 - c[3]);
+        n2 += (c[1] * l2) / (l2 - c[4]);
+        n2 += (c[2] * l2) / (l2 - c[5]);
+        return sqrtf(maxf(n2, 1.0f));
+    }
+    default: {
+        float l4 = l2 * l2;
+        return maxf((c[0] + (c[1] / l2)) + (c[2] / l4), 1.0f);
+    }
+    }
+}
+float ixo_dispersion_evaluate(const ixg_dispersion* d, float lambda_nm) {
+    /* dispersion.rs:170-178; the sub-380 nm clamp only serves fluorescence
+     * (not on this path), so wavelengths below 380 evaluate at 380. */
+    if (lambda_nm < 380.0f) lambda_nm = 380.0f;
+    return dispersion_unclamped(d, lambda_nm);
+}
+
+void ixo_cie_cmf(float lambda_nm, float out[3]) { /* color/cie1931.rs:184-209 */
+    const float end = 380.0f + (float)(IX_CMF_ROWS - 1) * 5.0f;
+    if (!(lambda_nm >= 380.0f && lambda_nm <= end)) {
+        out[0] = out[1] = out[2] = 0.0f;
+        return;
+    }
+    float position = (lambda_nm - 380.0f) / 5.0f;
+    size_t i0 = (size_t)floorf(position);
+    if (i0 > IX_CMF_ROWS - 2) i0 = IX_CMF_ROWS - 2;
+    float t = position - (float)i0;
+    for (int c = 0; c < 3; ++c) {
+        float lo = ix_cmf_table[i0][c], hi = ix_cmf_table[i0 + 1][c];
+        out[c] = lo + (hi - lo) * t;
+    }
+}
+float ixo_d65(float lambda_nm) { /* environment/spectral.rs:49-76 (>= 380 nm part) */
+    float clamped = clampf(lambda_nm, 380.0f, fmaf(10.0f, (float)(IX_D65_ROWS - 1), 380.0f));
+    float position = (clamped - 380.0f) / 10.0f;
+    size_t i0 = (size_t)floorf(position);
+    if (i0 > IX_D65_ROWS - 2) i0 = IX_D65_ROWS - 2;
+    float frac = position - (float)i0;
+    float v0 = ix_d65_table[i0], v1 = ix_d65_table[i0 + 1];
+    return fmaf(frac, v1 - v0, v0) / 100.0f;
+}
+typedef struct { float t_k, denom_560; } bb_norm;
+static bb_norm bb_norm_new(float temp_k) { /* spectral.rs:101-107 */
+    bb_norm n;
+    n.t_k = maxf(temp_k, 1000.0f);
+    float e = expf(minf(14388000.0f / (560.0f * n.t_k), 80.0f));
+    n.denom_560 = maxf(e - 1.0f, 1e-6f);
+    return n;
+}
+static float bb_with(bb_norm n, float lambda_nm) { /* spectral.rs:110-116 */
+    float e = expf(minf(14388000.0f / (lambda_nm * n.t_k), 80.0f));
+    float denom = maxf(e - 1.0f, 1e-6f);
+    float ratio = n.denom_560 / denom;
+    return clampf(powi_f(560.0f / lambda_nm, 5) * ratio, 0.01f, 20.0f);
+}
+float ixo_blackbody(float lambda_nm, float temp_k) { return bb_with(bb_norm_new(temp_k), lambda_nm); }
+
+static const float BRADFORD_XYZ_TO_LMS[3][3] = {
+    {0.8951f, 0.2664f, -0.1614f}, {-0.7502f, 1.7135f, 0.0367f}, {0.0389f, -0.0685f, 1.0296f}};
+static const float BRADFORD_LMS_TO_XYZ[3][3] = {{0.986993f, -0.147054f, 0.159963f},
+                                                {0.432305f, 0.518360f, 0.049291f},
+                                                {-0.008529f, 0.040043f, 0.968487f}};
+static v3 mat3_fma(const float m[3][3], v3 a) { /* color.rs:220-240 */
+    return mk(fmaf(m[0][0], a.x, fmaf(m[0][1], a.y, m[0][2] * a.z)),
+              fmaf(m[1][0], a.x, fmaf(m[1][1], a.y, m[1][2] * a.z)),
+              fmaf(m[2][0], a.x, fmaf(m[2][1], a.y, m[2][2] * a.z)));
+}
+void ixo_white_balance(float temp_k, float out[3]) { /* color.rs:268-305 */
+    v3 w = mk(0, 0, 0);
+    for (int step = 0; step <= 400; ++step) {
+        float lambda = 380.0f + (float)step;
+        float cmf[3];
+        ixo_cie_cmf(lambda, cmf);
+        float bb = ixo_blackbody(lambda, temp_k);
+        w = vadd(w, mk(cmf[0] * bb, cmf[1] * bb, cmf[2] * bb));
+    }
+    float ty = maxf(w.y, 1e-6f);
+    const float X = 0.3127f, Y = 0.3290f;
+    v3 target = mk((X / Y) * ty, ty, ((1.0f - X - Y) / Y) * ty);
+    v3 ls = mat3_fma(BRADFORD_XYZ_TO_LMS, w), lt = mat3_fma(BRADFORD_XYZ_TO_LMS, target);
+    out[0] = ls.x > 1e-6f ? lt.x / ls.x : 1.0f;
+    out[1] = ls.y > 1e-6f ? lt.y / ls.y : 1.0f;
+    out[2] = ls.z > 1e-6f ? lt.z / ls.z : 1.0f;
+}
+static v3 von_kries(v3 xyz, v3 scale) { /* color.rs:252-254 */
+    v3 lms = mat3_fma(BRADFORD_XYZ_TO_LMS, xyz);
+    lms = mk(lms.x * scale.x, lms.y * scale.y, lms.z * scale.z);
+    return mat3_fma(BRADFORD_LMS_TO_XYZ, lms);
+}
+
+float ixo_exp_lane(float x) { /* simd/exp_poly.rs:37-60 */
+    const float LOG2E = 1.44269504088896340736f;
+    union { uint32_t u; float f; } c1 = {0x3F318000u};
+    const float C2 = -2.1219444e-4f;
+    const float P0 = 1.9875691e-4f, P1 = 1.3981999e-3f, P2 = 8.333452e-3f, P3 = 4.1665796e-2f,
+                P4 = 1.6666665e-1f, P5 = 5.0000003e-1f;
+    const float HI = 88.02875f, LO = -87.33654f;
+    float upper = x < HI ? x : HI;
+    x = upper > LO ? upper : LO;
+    float n = floorf(fmaf(x, LOG2E, 0.5f));
+    x = fmaf(n, -c1.f, x);
+    x = fmaf(n, -C2, x);
+    float z = x * x;
+    float p = P0;
+    p = fmaf(p, x, P1);
+    p = fmaf(p, x, P2);
+    p = fmaf(p, x, P3);
+    p = fmaf(p, x, P4);
+    p = fmaf(p, x, P5);
+    p = fmaf(p, z, x) + 1.0f;
+    int32_t e = (int32_t)n;
+    union { uint32_t u; float f; } pw = {(uint32_t)((e + 127) << 23)};
+    return p * pw.f;
+}
+
+float ixo_band_evaluate(const ixg_band* b, float lambda_nm) { /* optics/absorption.rs:108-138 */
+    if (b->shape == 0) {
+        float t = (lambda_nm - b->center_nm) / b->width;
+        return b->peak * expf(-0.5f * t * t);
+    }
+    float delta = b->center_nm - lambda_nm;
+    float nu = 1.0e7f * delta / (lambda_nm * b->center_nm);
+    float t = nu / b->width;
+    return b->peak * expf(-0.5f * t * t);
+}
+
+/* ------------------------------------------------------------------ */
+/* Environment (RT/environment/)                                        */
+/* ------------------------------------------------------------------ */
+enum { MODEL_STUDIO = 0, MODEL_ISO = 1 };
+typedef struct {
+    int model;
+    int d65;            /* 1: tabulated D65 illuminant, 0: blackbody */
+    bb_norm bb;
+    float exposure, spot_mult;
+    float shadow[2];
+    v3 key_dir, fill_dir, ring[16];
+    v3 wb;              /* von Kries scale (identity for D65 presets) */
+} env_t;
+
+static int preset_params(int32_t preset, int* model, int* d65, float* temp_k, float* spot_mult,
+                         int* white_balance) { /* environment/mod.rs:321-410,519-590 */
+    switch (preset) {
+    case IXG_PRESET_INCANDESCENT: *model = MODEL_STUDIO; *d65 = 0; *temp_k = 3200.0f; *spot_mult = 1.2f; *white_balance = 1; return 1;
+    case IXG_PRESET_RING_LIGHTS: *model = MODEL_STUDIO; *d65 = 0; *temp_k = 5000.0f; *spot_mult = 1.6f; *white_balance = 1; return 1;
+    case IXG_PRESET_DARK_SPOTLIGHT: *model = MODEL_STUDIO; *d65 = 0; *temp_k = 6000.0f; *spot_mult = 2.4f; *white_balance = 1; return 1;
+    case IXG_PRESET_ILLUMINANT_A: *model = MODEL_STUDIO; *d65 = 0; *temp_k = 2856.0f; *spot_mult = 1.2f; *white_balance = 1; return 1;
+    case IXG_PRESET_DAYLIGHT: *model = MODEL_STUDIO; *d65 = 1; *temp_k = 6500.0f; *spot_mult = 1.0f; *white_balance = 0; return 1;
+    case IXG_PRESET_ISO_HEMISPHERE: *model = MODEL_ISO; *d65 = 1; *temp_k = 6500.0f; *spot_mult = 1.0f; *white_balance = 0; return 1;
+    default: return 0;
+    }
+}
+
+static void head_shadow_cosines(float deg, float out[2]) { /* environment/mod.rs:124-137 */
+    if (isnan(deg) || deg == 16.0f) { out[0] = 0.9510565f; out[1] = 0.9702957f; return; }
+    if (deg <= 0.0f) { out[0] = 2.0f; out[1] = 3.0f; return; }
+    deg = clampf(deg, 3.0f, 88.0f);
+    const float to_rad = F32_PI / 180.0f;
+    out[0] = cosf((deg + 2.0f) * to_rad);
+    out[1] = cosf((deg - 2.0f) * to_rad);
+}
+
+static void studio_rig_new(float yaw, float pitch, env_t* e) { /* optics/studio_rig.rs:47-80 */
+    float cp = cosf(pitch), sp = sinf(pitch), cy = cosf(yaw), sy = sinf(yaw);
+    e->key_dir = vnormalize(mk(cp * sy, sp, cp * cy));
+    float fill_yaw = fmaf(F32_PI, 0.78f, yaw);
+    float fill_pitch = clampf(pitch * 0.65f, 0.15f, 1.2f);
+    e->fill_dir = vnormalize(mk(cosf(fill_pitch) * sinf(fill_yaw), sinf(fill_pitch),
+                                cosf(fill_pitch) * cosf(fill_yaw)));
+    for (int i = 0; i < 16; ++i) {
+        float a = fmaf((float)i, F32_PI * 2.0f / 16.0f, yaw);
+        e->ring[i] = vnormalize(mk(sinf(a) * 0.75f, sp * 0.8f, cosf(a) * 0.75f));
+    }
+}
+
+static int env_build(const ixg_environment* src, env_t* e) {
+    float temp_k, spot;
+    int wb;
+    if (!preset_params(src->preset, &e->model, &e->d65, &temp_k, &spot, &wb)) return 0;
+    e->bb = bb_norm_new(temp_k);
+    e->exposure = src->exposure;
+    e->spot_mult = spot;
+    head_shadow_cosines(src->head_shadow_deg, e->shadow);
+    studio_rig_new(src->light_yaw, src->light_pitch, e);
+    if (wb) {
+        float s[3];
+        ixo_white_balance(temp_k, s);
+        e->wb = mk(s[0], s[1], s[2]);
+    } else {
+        e->wb = mk(1.0f, 1.0f, 1.0f);
+    }
+    return 1;
+}
+
+static float smoothstep_f(float e0, float e1, float x) { /* rig.rs:86-89 */
+    float t = clampf((x - e0) / (e1 - e0), 0.0f, 1.0f);
+    return t * t * fmaf(-2.0f, t, 3.0f);
+}
+
+/* Wavelength-independent lighting of one direction (rig.rs:379-405). */
+typedef struct {
+    int model;
+    float bg, softbox, fill, ring[16];
+    int has_softbox, has_fill, ring_len;
+    float scaled;
+} dir_light;
+
+static dir_light direction_lighting(const env_t* e, v3 dir, v3 observer) {
+    dir_light L;
+    memset(&L, 0, sizeof L);
+    v3 d = vnormalize(dir);
+    L.model = e->model;
+    if (e->model == MODEL_ISO) { /* rig.rs:238-252, tray = 0 */
+        float horizon = smoothstep_f(-0.05f, 0.05f, d.y);
+        float vis = 1.0f - smoothstep_f(e->shadow[0], e->shadow[1], vdot(d, observer));
+        L.scaled = horizon * vis;
+        return L;
+    }
+    /* studio_rig_lighting, rig.rs:322-370 (ambient fill on, tray = 0) */
+    L.bg = maxf(fmaf(0.012f, fmaf(d.y, 0.5f, 0.5f), 0.015f), 0.005f) * e->exposure;
+    float key_dot = maxf(vdot(d, e->key_dir), 0.0f);
+    if (key_dot > 0.0f) {
+        L.has_softbox = 1;
+        L.softbox = powi_f(key_dot, 28) * 12.0f * e->spot_mult * e->exposure;
+    }
+    float fill_dot = maxf(vdot(d, e->fill_dir), 0.0f);
+    if (fill_dot > 0.0f) {
+        L.has_fill = 1;
+        L.fill = powi_f(fill_dot, 18) * 4.5f * e->exposure;
+    }
+    for (int i = 0; i < 16; ++i) {
+        float rd = maxf(vdot(d, e->ring[i]), 0.0f);
+        if (rd > 0.96f) {
+            float spark = (rd - 0.96f) / 0.04f;
+            L.ring[L.ring_len++] = powi_f(spark, 6) * 22.0f * e->spot_mult * e->exposure;
+        }
+    }
+    return L;
+}
+static float spec_power(const env_t* e, float lambda) {
+    return e->d65 ? ixo_d65(lambda) : bb_with(e->bb, lambda);
+}
+static float light_radiance(const dir_light* L, float sp, float exposure) { /* rig.rs:178-200 */
+    if (L->model == MODEL_ISO) return L->scaled * (sp * exposure);
+    float r = L->bg * sp;
+    if (L->has_softbox) r = fmaf(L->softbox, sp, r);
+    if (L->has_fill) r = fmaf(L->fill, sp, r);
+    for (int i = 0; i < L->ring_len; ++i) r = fmaf(L->ring[i], sp, r);
+    return r;
+}
+
+float ixo_sample_studio_environment(const float dir[3], float lambda_nm, int32_t preset,
+                                    float exposure, float light_yaw, float light_pitch) {
+    ixg_environment src = {preset, exposure, light_yaw, light_pitch, 0.0f, 1.0f, 16.0f};
+    env_t e;
+    if (!env_build(&src, &e)) return NAN;
+    dir_light L = direction_lighting(&e, vfrom(dir), mk(0, 0, 0));
+    return light_radiance(&L, spec_power(&e, lambda_nm), exposure);
+}
+
+/* ------------------------------------------------------------------ */
+/* Camera and geometry                                                  */
+/* ------------------------------------------------------------------ */
+void ixo_camera_new(float yaw, float pitch, float distance, float fov_deg, uint32_t width,
+                    uint32_t height, uint32_t spp, ixg_camera* out) { /* camera.rs:108-147 */
+    float cp = cosf(pitch), sp = sinf(pitch), cy = cosf(yaw), sy = sinf(yaw);
+    v3 origin = mk(distance * cp * sy, distance * sp, distance * cp * cy);
+    v3 forward = vnormalize(vneg(origin));
+    v3 world_up;
+    if (fabsf(cp) < 1e-4f) world_up = mk(0, 0, -1);
+    else if (cp < 0.0f) world_up = mk(0, -1, 0);
+    else world_up = mk(0, 1, 0);
+    v3 right = vnormalize(vcross(forward, world_up));
+    v3 up = vnormalize(vcross(right, forward));
+    float fov_tan = tanf(fov_deg * (F32_PI / 180.0f) * 0.5f);
+    out->origin[0] = origin.x; out->origin[1] = origin.y; out->origin[2] = origin.z;
+    out->fov_tan = fov_tan;
+    out->forward[0] = forward.x; out->forward[1] = forward.y; out->forward[2] = forward.z;
+    out->width = (float)width;
+    out->right[0] = right.x; out->right[1] = right.y; out->right[2] = right.z;
+    out->height = (float)height;
+    out->up[0] = up.x; out->up[1] = up.y; out->up[2] = up.z;
+    out->num_samples = spp;
+}
+void ixo_generate_ray(const ixg_camera* c, float sx, float sy, float jx, float jy, float o[3],
+                      float d[3]) { /* camera.rs:149-167 */
+    float width = c->width, height = c->height;
+    float aspect = width / height;
+    float u = ((sx + jx) / width - 0.5f) * 2.0f * aspect * c->fov_tan;
+    float v = (0.5f - (sy + jy) / height) * 2.0f * c->fov_tan;
+    v3 dir = vnormalize(vadd(vadd(vfrom(c->forward), vmul(vfrom(c->right), u)),
+                             vmul(vfrom(c->up), v)));
+    o[0] = c->origin[0]; o[1] = c->origin[1]; o[2] = c->origin[2];
+    d[0] = dir.x; d[1] = dir.y; d[2] = dir.z;
+}
+
+static void plane_new(v3 n, float d, ixg_facet_plane* p) { /* geometry/plane.rs:47-54 */
+    v3 u = vnormalize(n);
+    p->normal[0] = u.x; p->normal[1] = u.y; p->normal[2] = u.z;
+    p->d = d;
+}
+static float index_to_azimuth(float index, float teeth) { /* cuts/round_brilliant.rs:65 */
+    return 2.0f * F32_PI * (index + 0.0f) / teeth;
+}
+static uint32_t tier(ixg_facet_plane* out, uint32_t at, float angle_rad, float sign_y,
+                     const float* idx, int count, float d) {
+    for (int i = 0; i < count; ++i) {
+        float phi = index_to_azimuth(idx[i], 96.0f);
+        v3 n = mk(sinf(angle_rad) * cosf(phi), sign_y * cosf(angle_rad), sinf(angle_rad) * sinf(phi));
+        plane_new(n, d, &out[at++]);
+    }
+    return at;
+}
+static const float STAR_IDX[8] = {6, 18, 30, 42, 54, 66, 78, 90};
+static const float MAIN_IDX[8] = {0, 12, 24, 36, 48, 60, 72, 84};
+static const float BREAK_IDX[16] = {95, 1, 11, 13, 23, 25, 35, 37, 47, 49, 59, 61, 71, 73, 83, 85};
+static const float TO_RAD = F32_PI / 180.0f;
+
+static uint32_t round_brilliant(float table_d, float crown_main_deg, float crown_main_d,
+                                float pav_main_deg, float pav_main_d, ixg_facet_plane* out) {
+    uint32_t n = 0;
+    plane_new(mk(0, 1, 0), table_d, &out[n++]);
+    n = tier(out, n, 15.0f * TO_RAD, 1.0f, STAR_IDX, 8, -0.45f);
+    n = tier(out, n, crown_main_deg * TO_RAD, 1.0f, MAIN_IDX, 8, crown_main_d);
+    n = tier(out, n, 41.0f * TO_RAD, 1.0f, BREAK_IDX, 16, -0.67f);
+    for (int i = 0; i < 16; ++i) {
+        float phi = 2.0f * F32_PI * (float)i / 16.0f;
+        plane_new(mk(cosf(phi), 0.0f, sinf(phi)), -1.0f, &out[n++]);
+    }
+    n = tier(out, n, pav_main_deg * TO_RAD, -1.0f, MAIN_IDX, 8, pav_main_d);
+    n = tier(out, n, 42.5f * TO_RAD, -1.0f, BREAK_IDX, 16, -0.68f);
+    plane_new(mk(0, -1, 0), -0.88f, &out[n++]);
+    return n;
+}
+uint32_t ixo_cut_standard_round_brilliant(ixg_facet_plane* out) { /* round_brilliant.rs:71-155 */
+    return round_brilliant(-0.32f, 34.5f, -0.59f, 41.0f, -0.67f, out);
+}
+uint32_t ixo_cut_round_brilliant(float crown_deg, float pavilion_deg, float table_ratio,
+                                 ixg_facet_plane* out) {
+    /* The standard cut's tiers with the crown/pavilion main angles moved and
+     * the table plane placed where it cuts the crown mains at radius
+     * table_ratio (girdle radius 1).  Mains keep the standard cut's girdle
+     * heights (crown +0.02866, pavilion -0.01847) so the girdle band stays. */
+    double ca = (double)crown_deg * 3.14159265358979323846 / 180.0;
+    double pa = (double)pavilion_deg * 3.14159265358979323846 / 180.0;
+    double crown_d = sin(ca) + cos(ca) * 0.028660;
+    double pav_d = sin(pa) + cos(pa) * 0.018470;
+    double table_y = (crown_d - sin(ca) * (double)table_ratio) / cos(ca);
+    return round_brilliant((float)-table_y, crown_deg, (float)-crown_d, pavilion_deg,
+                           (float)-pav_d, out);
+}
+
+/* slab_scan + intersect_polyhedron_soa (simd/slab.rs:147-160, RT/intersect.rs:101-132). */
+int32_t ixo_intersect(const ixg_facet_plane* planes, uint32_t n, const float o[3], const float d[3],
+                      float* t_out) {
+    v3 origin = vfrom(o), dir = vfrom(d);
+    float t_near = -1e30f, t_far = 1e30f;
+    int32_t near_idx = -1, far_idx = -1;
+    for (uint32_t i = 0; i < n; ++i) {
+        v3 nrm = vfrom(planes[i].normal);
+        float denom = vdot(nrm, dir);
+        float side = planes[i].d + vdot(nrm, origin);
+        float t = (-side) / denom;
+        if (fabsf(denom) > 1e-7f) {
+            if (denom < 0.0f) {
+                if (t > t_near) { t_near = t; near_idx = (int32_t)i; }
+            } else if (t < t_far) {
+                t_far = t;
+                far_idx = (int32_t)i;
+            }
+        } else if (side > 0.0f) {
+            return -1;
+        }
+    }
+    if (t_near > t_far) return -1;
+    if (t_near > 1e-4f) { *t_out = t_near; return near_idx < 0 ? 0 : near_idx; }
+    if (t_far > 1e-4f) { *t_out = t_far; return far_idx < 0 ? 0 : far_idx; }
+    return -1;
+}
+
+/* ------------------------------------------------------------------ */
+/* Transport (RT/transport/, RT/refraction/)                            */
+/* ------------------------------------------------------------------ */
+#define DIRECTION_MATCH_COS_TOL (1.0f - 1e-6f)
+#define R_UNPOL_PDF_MIN 1e-4f
+#define R_UNPOL_PDF_MAX (1.0f - 1e-4f)
+#define R_UNPOL_SELECT_MIN 0.02f
+#define R_UNPOL_SELECT_MAX 0.98f
+
+typedef struct {
+    const ixg_facet_plane* planes;
+    uint32_t n_planes;
+    const ixg_material* mat;
+    uint32_t max_bounces;
+    env_t env;
+    float surface_glare;
+} scene_t;
+
+typedef struct {
+    float lambdas[NCH];
+    stokes st[NCH];
+    float radiance[NCH];
+    float split[NCH];
+    float path_pdf[NCH];
+    float n_o[NCH];
+    uint8_t compat[NCH];
+    v3 org, dir, k;
+    int inside;
+    int has_prev;
+    v3 prev_plane;
+    int escaped;
+    v3 observer;
+} path_t;
+
+static void ev_facet(ixg_path_events* ev, int32_t facet, uint32_t kind) {
+    if (!ev) return;
+    uint32_t h = ev->facet_hash;
+    h = (h ^ (uint32_t)facet) * 16777619u;
+    h = (h ^ kind) * 16777619u;
+    ev->facet_hash = h;
+}
+
+static float tir_phase_delta(float n1k, float cos_i, float sin_i) { /* refraction/tir.rs:33-37 */
+    float tan_half = (cos_i * sqrtf(maxf(fmaf(n1k * n1k * sin_i, sin_i, -1.0f), 0.0f))) /
+                     maxf(n1k * sin_i * sin_i, 1e-6f);
+    return 2.0f * atanf(tan_half);
+}
+
+static float env_channel(const scene_t* S, v3 dir, float lambda, v3 observer) {
+    dir_light L = direction_lighting(&S->env, dir, observer);
+    return light_radiance(&L, spec_power(&S->env, lambda), S->env.exposure);
+}
+
+/* The hit point and geometry of one facet event. */
+typedef struct {
+    float cos_i, sin_i, n1, n2, sin2_t;
+    float n1_ch[NCH], n2_ch[NCH], sin2_t_ch[NCH];
+} geo_t;
+
+static void fresnel_rs_rp(float n1, float n2, float cos_i, float cos_t, float* rs, float* rp) {
+    *rs = fmaf(n2, -cos_t, n1 * cos_i) / fmaf(n2, cos_t, n1 * cos_i);
+    *rp = fmaf(n1, -cos_t, n2 * cos_i) / fmaf(n1, cos_t, n2 * cos_i);
+}
+
+static void facet_event(const scene_t* S, path_t* P, uint32_t bounce, uint32_t seed, float t_hit,
+                        int32_t facet, ixg_path_events* ev, int* stop) {
+    v3 hit_point = vadd(P->org, vmul(P->dir, t_hit));
+    v3 normal = vfrom(S->planes[facet].normal);
+    /* rotate_stokes_to_plane_of_incidence (RT/absorption.rs:69-87) */
+    v3 cur_plane = vnormalize_or_zero(vcross(P->k, normal));
+    if (P->has_prev && vlen_sq(cur_plane) > 1e-6f && vlen_sq(P->prev_plane) > 1e-6f) {
+        float cos_psi = clampf(vdot(P->prev_plane, cur_plane), -1.0f, 1.0f);
+        float sin_psi = vdot(vcross(P->prev_plane, cur_plane), vnormalize_or_zero(P->k));
+        float psi = atan2f(sin_psi, cos_psi);
+        mat4 R = mueller_frame_rotation(psi);
+        for (int c = 0; c < NCH; ++c) P->st[c] = st_apply(&R, P->st[c]);
+    }
+    P->prev_plane = cur_plane;
+    P->has_prev = 1;
+    /* apply_interior_segment (transport/inner.rs:139-155): absorption-free
+     * isotropic materials transmit exp(-0) == 1 exactly, a no-op. */
+    if (P->inside) normal = vneg(normal);
+
+    /* compute_bounce_refraction_geometry (refraction/geometry.rs:306-425), isotropic */
+    geo_t G;
+    G.cos_i = clampf(vdot(vneg(P->k), normal), 0.0f, 1.0f);
+    G.sin_i = sqrtf(maxf(fmaf(G.cos_i, -G.cos_i, 1.0f), 0.0f));
+    float nh = P->n_o[0];
+    G.n1 = P->inside ? nh : 1.0f;
+    G.n2 = P->inside ? 1.0f : nh;
+    float eta = G.n1 / G.n2;
+    G.sin2_t = eta * eta * fmaf(G.cos_i, -G.cos_i, 1.0f);
+    for (int c = 0; c < NCH; ++c) {
+        float n1k = P->inside ? P->n_o[c] : 1.0f;
+        float n2k = P->inside ? 1.0f : P->n_o[c];
+        float etak = n1k / n2k;
+        G.n1_ch[c] = n1k;
+        G.n2_ch[c] = n2k;
+        G.sin2_t_ch[c] = etak * etak * fmaf(G.cos_i, -G.cos_i, 1.0f);
+    }
+
+    v3 new_k, new_s;
+    int new_inside = P->inside;
+    v3 kk = P->k;
+    if (G.sin2_t > 1.0f) {
+        /* apply_tir_bounce (refraction/tir.rs:56-160), scalar branch */
+        if (ev) ev->n_tir++;
+        ev_facet(ev, facet, 1);
+        for (int c = 0; c < NCH; ++c) {
+            float n1k = G.n1_ch[c], n2k = G.n2_ch[c];
+            if (G.sin2_t_ch[c] > 1.0f) {
+                mat4 M = mueller_tir_retardation(tir_phase_delta(n1k, G.cos_i, G.sin_i));
+                P->st[c] = st_apply(&M, P->st[c]);
+            } else {
+                float cos_t = sqrtf(maxf(1.0f - G.sin2_t_ch[c], 0.0f));
+                float rs, rp;
+                fresnel_rs_rp(n1k, n2k, G.cos_i, cos_t, &rs, &rp);
+                mat4 M = mueller_fresnel_reflection(rs, rp);
+                P->st[c] = st_apply(&M, P->st[c]);
+                float ru = clampf(0.5f * fmaf(rp, rp, rs * rs), R_UNPOL_PDF_MIN, R_UNPOL_PDF_MAX);
+                P->path_pdf[c] *= ru;
+            }
+        }
+        new_k = vsub(kk, vmul(normal, 2.0f * vdot(kk, normal)));
+        new_s = new_k;
+    } else {
+        /* apply_partial_fresnel_bounce (refraction/dispatch.rs:307-380) */
+        float cos_t = sqrtf(maxf(1.0f - G.sin2_t, 0.0f));
+        float rs, rp;
+        fresnel_rs_rp(G.n1, G.n2, G.cos_i, cos_t, &rs, &rp);
+        float r_unpol = clampf(0.5f * fmaf(rp, rp, rs * rs), R_UNPOL_SELECT_MIN, R_UNPOL_SELECT_MAX);
+        float rng = unit_draw(seed, bounce, FRESNEL_BRANCH_STREAM);
+        if (rng < r_unpol) {
+            /* apply_partial_reflect_bounce (reflect_refract.rs:33-71) */
+            if (ev) ev->n_reflect++;
+            ev_facet(ev, facet, 2);
+            for (int c = 0; c < NCH; ++c) {
+                float n1k = G.n1_ch[c], n2k = G.n2_ch[c];
+                mat4 M;
+                if (G.sin2_t_ch[c] > 1.0f) {
+                    M = mueller_tir_retardation(tir_phase_delta(n1k, G.cos_i, G.sin_i));
+                } else {
+                    float ct = sqrtf(maxf(1.0f - G.sin2_t_ch[c], 0.0f));
+                    float a, b;
+                    fresnel_rs_rp(n1k, n2k, G.cos_i, ct, &a, &b);
+                    float ru = clampf(0.5f * fmaf(b, b, a * a), R_UNPOL_PDF_MIN, R_UNPOL_PDF_MAX);
+                    P->path_pdf[c] *= ru;
+                    M = mueller_fresnel_reflection(a, b);
+                }
+                P->st[c] = st_scale(st_apply(&M, P->st[c]), 1.0f / r_unpol);
+            }
+            new_k = vsub(kk, vmul(normal, 2.0f * vdot(kk, normal)));
+            new_s = new_k;
+        } else {
+            /* apply_refract_bounce (reflect_refract.rs:121-234), isotropic */
+            if (ev) ev->n_refract++;
+            ev_facet(ev, facet, 3);
+            float eta_dir = G.n1 / G.n2;
+            float s2 = minf(eta_dir * eta_dir * fmaf(G.cos_i, -G.cos_i, 1.0f), 1.0f);
+            float ct_dir = sqrtf(maxf(1.0f - s2, 0.0f));
+            v3 refr = vnormalize(vadd(vmul(kk, eta_dir), vmul(normal, fmaf(eta_dir, G.cos_i, -ct_dir))));
+            v3 dirs[NCH];
+            int has_dir[NCH], hero_match[NCH];
+            for (int c = 0; c < NCH; ++c) {
+                /* apply_refract_channel (reflect_refract.rs:517-591) */
+                stokes orig = P->st[c];
+                float n1k = G.n1_ch[c], n2k = G.n2_ch[c];
+                float r = n1k / n2k;
+                float s2k = r * r * fmaf(G.cos_i, -G.cos_i, 1.0f);
+                if (s2k > 1.0f) {
+                    P->st[c] = st_scale(P->st[c], 0.0f);
+                    P->path_pdf[c] = 0.0f;
+                    has_dir[c] = 0;
+                    hero_match[c] = 0;
+                    continue;
+                }
+                float ctk = sqrtf(maxf(1.0f - s2k, 0.0f));
+                float ek = n1k / n2k;
+                v3 dk = vnormalize(vadd(vmul(kk, ek), vmul(normal, fmaf(ek, G.cos_i, -ctk))));
+                int match = vdot(dk, refr) >= DIRECTION_MATCH_COS_TOL;
+                /* transmission of channel c (exit_split.rs:69-101) */
+                float num = 2.0f * n1k * G.cos_i;
+                float t_s = num / fmaf(n2k, ctk, n1k * G.cos_i);
+                float t_p = num / fmaf(n1k, ctk, n2k * G.cos_i);
+                mat4 T = mueller_fresnel_transmission(n1k, n2k, G.cos_i, ctk, t_s, t_p);
+                stokes transmitted = st_scale(st_apply(&T, orig), 1.0f / (1.0f - r_unpol));
+                float a, b;
+                fresnel_rs_rp(n1k, n2k, G.cos_i, ctk, &a, &b);
+                float ruk = clampf(0.5f * fmaf(b, b, a * a), R_UNPOL_PDF_MIN, R_UNPOL_PDF_MAX);
+                if (match) {
+                    P->st[c] = transmitted;
+                    P->path_pdf[c] *= 1.0f - ruk;
+                } else {
+                    /* apply_channel_chromatic_termination (reflect_refract.rs:444-515) */
+                    float prefix = P->path_pdf[c];
+                    P->st[c] = st_scale(P->st[c], 0.0f);
+                    P->path_pdf[c] = prefix * (1.0f - ruk);
+                    if (P->inside && st_intensity(orig) > 0.0f) {
+                        /* try_split_exit_channel (exit_split.rs:128-163) */
+                        float po[3], pd[3], tt;
+                        v3 probe_o = vadd(hit_point, vmul(dk, 1e-4f));
+                        po[0] = probe_o.x; po[1] = probe_o.y; po[2] = probe_o.z;
+                        pd[0] = dk.x; pd[1] = dk.y; pd[2] = dk.z;
+                        if (ev) ev->n_probe++;
+                        if (ixo_intersect(S->planes, S->n_planes, po, pd, &tt) < 0) {
+                            float env = env_channel(S, dk, P->lambdas[c], P->observer);
+                            P->split[c] = fmaf(maxf(st_intensity(transmitted), 0.0f) * 1.0f, env,
+                                               P->split[c]);
+                        }
+                    }
+                }
+                dirs[c] = dk;
+                has_dir[c] = 1;
+                hero_match[c] = match;
+            }
+            if (!P->inside) { /* narrow_compat (refraction/context.rs:219-242) */
+                for (int a = 0; a < NCH; ++a)
+                    for (int b = a + 1; b < NCH; ++b) {
+                        int m;
+                        if (a == 0) m = hero_match[b];
+                        else if (has_dir[a] && has_dir[b]) m = vdot(dirs[a], dirs[b]) >= DIRECTION_MATCH_COS_TOL;
+                        else m = 1;
+                        if (!m) {
+                            P->compat[a] &= (uint8_t)~(1u << b);
+                            P->compat[b] &= (uint8_t)~(1u << a);
+                        }
+                    }
+            }
+            new_k = refr;
+            new_s = refr;
+            new_inside = !P->inside;
+        }
+    }
+    /* dispatch_bounce tail (transport/bounce.rs:200-238) */
+    P->dir = new_s;
+    P->org = vadd(hit_point, vmul(new_s, 1e-4f));
+    P->k = new_k;
+    P->inside = new_inside;
+    /* surface glare (bounce_body.rs:686-692) */
+    if (bounce == 0 && !P->inside && S->surface_glare < 1.0f)
+        for (int c = 0; c < NCH; ++c) P->st[c] = st_scale(P->st[c], S->surface_glare);
+    /* apply_russian_roulette (transport/bounce.rs:355-375) */
+    if (bounce > 4) {
+        float mx = 0.0f;
+        for (int c = 0; c < NCH; ++c) mx = maxf(mx, st_intensity(P->st[c]));
+        float q = clampf(mx, 0.05f, 1.0f);
+        float rr = unit_draw(seed, bounce, RUSSIAN_ROULETTE_STREAM);
+        if (rr > q) { *stop = 1; return; }
+        for (int c = 0; c < NCH; ++c) P->st[c] = st_scale(P->st[c], 1.0f / q);
+        for (int c = 0; c < NCH; ++c) P->split[c] /= q;
+    }
+}
+
+static int scene_build(const ixg_scene* sc, scene_t* S) {
+    S->planes = sc->planes;
+    S->n_planes = sc->n_planes;
+    S->mat = sc->material;
+    S->max_bounces = sc->max_bounces;
+    float g = sc->environment.surface_glare;
+    S->surface_glare = (g >= 1.0f || isnan(g)) ? 1.0f : (g > 0.0f ? g : 0.0f);
+    return env_build(&sc->environment, &S->env);
+}
+
+static void trace_core(const scene_t* S, v3 origin, v3 dir, uint32_t seed, float hero_rand,
+                       float xyz[3], ixg_path_events* ev) {
+    path_t P;
+    memset(&P, 0, sizeof P);
+    ixo_wrapped_hero_wavelengths(hero_rand, P.lambdas);
+    for (int c = 0; c < NCH; ++c) {
+        P.st[c].i = 1.0f;
+        P.path_pdf[c] = 1.0f;
+        P.compat[c] = 0xFF;
+        P.n_o[c] = ixo_dispersion_evaluate(&S->mat->dispersion, P.lambdas[c]);
+    }
+    P.org = origin;
+    P.dir = dir;
+    P.k = dir;
+    P.observer = vneg(dir);
+    if (ev) { memset(ev, 0, sizeof *ev); ev->facet_hash = 2166136261u; ev->termination = 3; }
+    uint32_t bounce;
+    int stopped = 0;
+    for (bounce = 0; bounce < S->max_bounces; ++bounce) {
+        float o[3] = {P.org.x, P.org.y, P.org.z}, d[3] = {P.dir.x, P.dir.y, P.dir.z}, t;
+        int32_t f = ixo_intersect(S->planes, S->n_planes, o, d, &t);
+        if (f < 0) { /* handle_miss (bounce_body/miss.rs:33-54) */
+            dir_light L = direction_lighting(&S->env, P.dir, P.observer);
+            for (int c = 0; c < NCH; ++c) {
+                float env = light_radiance(&L, spec_power(&S->env, P.lambdas[c]), S->env.exposure);
+                P.radiance[c] = fmaf(st_intensity(P.st[c]) * 1.0f, env, P.radiance[c]);
+            }
+            P.escaped = 1;
+            if (ev) { ev->termination = 0; ev->bounces = bounce + 1; }
+            stopped = 1;
+            break;
+        }
+        int stop = 0;
+        facet_event(S, &P, bounce, seed, t, f, ev, &stop);
+        if (stop) {
+            if (ev) { ev->termination = 2; ev->bounces = bounce + 1; }
+            stopped = 1;
+            break;
+        }
+    }
+    if (!stopped && ev) ev->bounces = S->max_bounces;
+    /* PathState::finish (bounce_body.rs:724-764) */
+    if (P.escaped)
+        for (int c = 0; c < NCH; ++c) P.radiance[c] += P.split[c];
+    const float norm_factor = (400.0f / 8.0f) / 106.856f;
+    v3 acc = mk(0, 0, 0);
+    for (int c = 0; c < NCH; ++c) { /* color.rs:145-172 */
+        float family = -0.0f;
+        for (int j = 0; j < NCH; ++j)
+            if (P.compat[c] & (1u << j)) family += P.path_pdf[j];
+        float w = family <= 1e-12f ? 1.0f : (8.0f * P.path_pdf[0]) / family;
+        float cmf[3];
+        ixo_cie_cmf(P.lambdas[c], cmf);
+        float s = (P.radiance[c] * w) * norm_factor;
+        acc = vadd(acc, mk(cmf[0] * s, cmf[1] * s, cmf[2] * s));
+    }
+    acc = vadd(acc, mk(0, 0, 0)); /* + nee_xyz (zero on this path) */
+    acc = von_kries(acc, S->env.wb);
+    xyz[0] = maxf(acc.x, 0.0f);
+    xyz[1] = maxf(acc.y, 0.0f);
+    xyz[2] = maxf(acc.z, 0.0f);
+}
+
+void ixo_trace_ray(const ixg_scene* sc, const float origin[3], const float dir[3], uint32_t seed,
+                   float hero_rand, float xyz[3], ixg_path_events* ev) {
+    scene_t S;
+    if (!scene_build(sc, &S)) { xyz[0] = xyz[1] = xyz[2] = NAN; return; }
+    trace_core(&S, vfrom(origin), vfrom(dir), seed, hero_rand, xyz, ev);
+}
+
+static void sample_core(const scene_t* S, const ixg_scene* sc, uint32_t pixel, uint32_t sample,
+                        float xyz[3], ixg_path_events* ev) { /* cpu_frame.rs:45-80 */
+    uint32_t seed;
+    float jx, jy, hero;
+    ixo_sample_draws(pixel, sample, &seed, &jx, &jy, &hero);
+    uint32_t x = pixel % sc->width, y = pixel / sc->width;
+    float o[3], d[3];
+    ixg_camera cam = sc->camera;
+    cam.width = (float)sc->width;
+    cam.height = (float)sc->height;
+    ixo_generate_ray(&cam, (float)x, (float)y, jx, jy, o, d);
+    trace_core(S, vfrom(o), vfrom(d), seed, hero, xyz, ev);
+}
+
+void ixo_trace_sample(const ixg_scene* sc, uint32_t pixel, uint32_t sample, float xyz[3],
+                      ixg_path_events* ev) {
+    scene_t S;
+    if (!scene_build(sc, &S)) { xyz[0] = xyz[1] = xyz[2] = NAN; return; }
+    sample_core(&S, sc, pixel, sample, xyz, ev);
+}
+
+typedef struct {
+    const ixg_scene* sc;
+    const scene_t* S;
+    uint32_t sample_offset, spp;
+    double* accum;
+    int t, T;
+    uint64_t first, count;
+    float* out_xyz;
+    ixg_path_events* events;
+} job_t;
+
+static void* accumulate_worker(void* arg) { /* hybrid.rs:337-392: thread t owns t, t+T, ... */
+    job_t* J = (job_t*)arg;
+    uint32_t npix = J->sc->width * J->sc->height;
+    for (uint32_t p = (uint32_t)J->t; p < npix; p += (uint32_t)J->T) {
+        double s0 = 0, s1 = 0, s2 = 0;
+        for (uint32_t s = 0; s < J->spp; ++s) {
+            float xyz[3];
+            sample_core(J->S, J->sc, p, J->sample_offset + s, xyz, NULL);
+            if (isfinite(xyz[0]) && isfinite(xyz[1]) && isfinite(xyz[2])) { /* sampling.rs:237 */
+                s0 += xyz[0];
+                s1 += xyz[1];
+                s2 += xyz[2];
+            }
+        }
+        J->accum[3 * (size_t)p] += s0;
+        J->accum[3 * (size_t)p + 1] += s1;
+        J->accum[3 * (size_t)p + 2] += s2;
+    }
+    return NULL;
+}
+
+void ixo_accumulate(const ixg_scene* sc, uint32_t sample_offset, uint32_t spp, double* accum,
+                    int threads) {
+    scene_t S;
+    if (!scene_build(sc, &S)) return;
+    if (threads < 1) threads = 1;
+    pthread_t th[256];
+    job_t jobs[256];
+    if (threads > 256) threads = 256;
+    for (int t = 0; t < threads; ++t) {
+        jobs[t] = (job_t){sc, &S, sample_offset, spp, accum, t, threads, 0, 0, NULL, NULL};
+        pthread_create(&th[t], NULL, accumulate_worker, &jobs[t]);
+    }
+    for (int t = 0; t < threads; ++t) pthread_join(th[t], NULL);
+}
+
+static void* paths_worker(void* arg) {
+    job_t* J = (job_t*)arg;
+    for (uint64_t i = J->first + (uint64_t)J->t; i < J->first + J->count; i += (uint64_t)J->T) {
+        uint32_t pixel = (uint32_t)(i / J->spp), s = (uint32_t)(i % J->spp);
+        float xyz[3];
+        ixg_path_events ev;
+        sample_core(J->S, J->sc, pixel, J->sample_offset + s, xyz, J->events ? &ev : NULL);
+        uint64_t k = i - J->first;
+        if (J->out_xyz) { J->out_xyz[3 * k] = xyz[0]; J->out_xyz[3 * k + 1] = xyz[1]; J->out_xyz[3 * k + 2] = xyz[2]; }
+        if (J->events) J->events[k] = ev;
+    }
+    return NULL;
+}
+
+void ixo_trace_paths(const ixg_scene* sc, uint32_t sample_offset, uint32_t spp, uint64_t first,
+                     uint64_t count, float* out_xyz, ixg_path_events* events, int threads) {
+    scene_t S;
+    if (!scene_build(sc, &S)) return;
+    if (threads < 1) threads = 1;
+    if (threads > 256) threads = 256;
+    pthread_t th[256];
+    job_t jobs[256];
+    for (int t = 0; t < threads; ++t) {
+        jobs[t] = (job_t){sc, &S, sample_offset, spp, NULL, t, threads, first, count, out_xyz, events};
+        pthread_create(&th[t], NULL, paths_worker, &jobs[t]);
+    }
+    for (int t = 0; t < threads; ++t) pthread_join(th[t], NULL);
+}

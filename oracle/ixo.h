@@ -1,0 +1,74 @@
+/*
+ * ixo.h -- the CPU oracle: a plain-C FP32 restatement of the reference's
+ * spectral tracer, used ONLY as a checker (tests, smoke, the bench's CPU
+ * baseline leg).  Never linked into libixg.so.
+ *
+ * Struct layouts come from the public ABI header so both sides read the same
+ * scene; every function below restates the reference file:line it cites.
+ */
+#ifndef IXO_H
+#define IXO_H
+
+#include <stdint.h>
+#include "../include/ixg.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* RNG and stratified draws (RT/sampling.rs). */
+uint32_t ixo_hash_u32(uint32_t x);
+float ixo_low_discrepancy_base2(uint32_t n);
+float ixo_radical_inverse_base(uint32_t n, uint32_t base);
+void ixo_sample_draws(uint32_t pixel, uint32_t sample, uint32_t* seed, float* jx, float* jy,
+                      float* hero);
+void ixo_wrapped_hero_wavelengths(float hero_rand, float out[8]);
+
+/* Optical primitives. */
+float ixo_dispersion_evaluate(const ixg_dispersion* d, float lambda_nm);
+void ixo_cie_cmf(float lambda_nm, float out[3]);
+float ixo_d65(float lambda_nm);
+float ixo_blackbody(float lambda_nm, float temp_k);
+void ixo_white_balance(float temp_k, float out[3]);
+float ixo_exp_lane(float x);
+float ixo_band_evaluate(const ixg_band* b, float lambda_nm);
+
+/* sample_studio_environment (RT/environment/mod.rs:653): observer = 0. */
+float ixo_sample_studio_environment(const float dir[3], float lambda_nm, int32_t preset,
+                                    float exposure, float light_yaw, float light_pitch);
+
+/* Camera (RT/camera.rs). */
+void ixo_camera_new(float yaw, float pitch, float distance, float fov_deg, uint32_t width,
+                    uint32_t height, uint32_t spp, ixg_camera* out);
+void ixo_generate_ray(const ixg_camera* cam, float sx, float sy, float jx, float jy,
+                      float origin[3], float dir[3]);
+
+/* Geometry. */
+uint32_t ixo_cut_standard_round_brilliant(ixg_facet_plane* out);
+uint32_t ixo_cut_round_brilliant(float crown_deg, float pavilion_deg, float table_ratio,
+                                 ixg_facet_plane* out);
+/* intersect_polyhedron_soa (RT/intersect.rs:101): returns facet index or -1. */
+int32_t ixo_intersect(const ixg_facet_plane* planes, uint32_t n, const float origin[3],
+                      const float dir[3], float* t_out);
+
+/* trace_spectral_ray (RT/transport/mod.rs:118) with an explicit ray; events may be NULL. */
+void ixo_trace_ray(const ixg_scene* scene, const float origin[3], const float dir[3],
+                   uint32_t rng_seed, float hero_rand, float xyz[3], ixg_path_events* ev);
+
+/* One (pixel, absolute sample) path of a frame (IX/renderer/cpu_frame.rs:45). */
+void ixo_trace_sample(const ixg_scene* scene, uint32_t pixel, uint32_t sample, float xyz[3],
+                      ixg_path_events* ev);
+
+/* cpu_accumulate (IX/renderer/gpu/hybrid.rs:305) with FP64 per-pixel sums,
+ * interleaved pixel partition over `threads` threads.  ADDS into accum. */
+void ixo_accumulate(const ixg_scene* scene, uint32_t sample_offset, uint32_t spp,
+                    double* accum_xyz, int threads);
+/* Per-path raw XYZ + events in (pixel, sample) order for paths [first, first+count). */
+void ixo_trace_paths(const ixg_scene* scene, uint32_t sample_offset, uint32_t spp,
+                     uint64_t first, uint64_t count, float* out_xyz, ixg_path_events* events,
+                     int threads);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
