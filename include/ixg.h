@@ -185,6 +185,12 @@ ixg_status ixg_accumulate_device(ixg_ctx* ctx, const ixg_scene* scene, uint32_t 
  * order.  Either output may be NULL. */
 ixg_status ixg_trace_paths(ixg_ctx* ctx, const ixg_scene* scene, uint32_t sample_offset,
                            uint32_t spp, float* out_xyz, ixg_path_events* events);
+/* trace_spectral_ray (RT/transport/mod.rs:118) for explicit rays, the entry
+ * the reference's golden tables pin: rays[8*i..] = {origin xyz, dir xyz,
+ * hero_rand, rng_seed bits as float}.  The scene's camera and frame size are
+ * ignored.  out_xyz[n*3] f32; events[n] may be NULL. */
+ixg_status ixg_trace_rays(ixg_ctx* ctx, const ixg_scene* scene, const float* rays, uint32_t n,
+                          float* out_xyz, ixg_path_events* events);
 /* Event totals of one film call (FLOP model input), summed on the device. */
 ixg_status ixg_count_events(ixg_ctx* ctx, const ixg_scene* scene, uint32_t sample_offset,
                             uint32_t spp, uint64_t totals[8]);

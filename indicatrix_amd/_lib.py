@@ -1,0 +1,78 @@
+"""Loader for the in-tree HIP library (libixg.so).
+
+The product path has no CPU fallback: if the library is missing or cannot be
+loaded this raises, naming the build step.
+"""
+import ctypes as C
+import os
+
+from . import abi
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libixg.so")
+HEADER_PATH = os.path.join(os.path.dirname(_HERE), "include", "ixg.h")
+
+_lib = None
+
+
+class IxgError(RuntimeError):
+    def __init__(self, status, message=""):
+        self.status = status
+        name = abi.STATUS_NAMES.get(status, str(status))
+        super().__init__(f"ixg status {name}: {message}" if message else f"ixg status {name}")
+
+
+def _declare(lib):
+    P = C.POINTER
+    st = C.c_int
+    sig = {
+        "ixg_create": (st, [P(abi.Opts), P(C.c_void_p)]),
+        "ixg_destroy": (None, [C.c_void_p]),
+        "ixg_is_lost": (C.c_int, [C.c_void_p]),
+        "ixg_adapter_label": (st, [C.c_void_p, C.c_char_p, C.c_size_t]),
+        "ixg_last_error": (st, [C.c_void_p, C.c_char_p, C.c_size_t]),
+        "ixg_try_recover": (st, [C.c_void_p]),
+        "ixg_inject_fault": (st, [C.c_void_p, C.c_int32]),
+        "ixg_route": (C.c_int, [P(abi.Scene)]),
+        "ixg_accumulate": (st, [C.c_void_p, P(abi.Scene), C.c_uint32, C.c_uint32,
+                                P(C.c_double), P(C.c_uint32)]),
+        "ixg_accumulate_device": (st, [C.c_void_p, P(abi.Scene), C.c_uint32, C.c_uint32,
+                                       C.c_void_p]),
+        "ixg_trace_paths": (st, [C.c_void_p, P(abi.Scene), C.c_uint32, C.c_uint32,
+                                 P(C.c_float), P(abi.PathEvents)]),
+        "ixg_trace_rays": (st, [C.c_void_p, P(abi.Scene), P(C.c_float), C.c_uint32,
+                                P(C.c_float), P(abi.PathEvents)]),
+        "ixg_count_events": (st, [C.c_void_p, P(abi.Scene), C.c_uint32, C.c_uint32,
+                                  P(C.c_uint64)]),
+        "ixg_last_kernel_ms": (C.c_double, [C.c_void_p]),
+        "ixg_last_kernel_launches": (C.c_uint32, [C.c_void_p]),
+        "ixg_camera_new": (None, [C.c_float, C.c_float, C.c_float, C.c_float, C.c_uint32,
+                                  C.c_uint32, C.c_uint32, P(abi.Camera)]),
+        "ixg_cut_standard_round_brilliant": (C.c_uint32, [P(abi.FacetPlane)]),
+        "ixg_cut_round_brilliant": (C.c_uint32, [C.c_float, C.c_float, C.c_float,
+                                                 P(abi.FacetPlane)]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    return lib
+
+
+def lib():
+    """The loaded libixg.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is missing: run `make -C indicatrix_amd` "
+                              "(or __graft_entry__.build()) first")
+        _lib = _declare(C.CDLL(LIB_PATH))
+    return _lib
+
+
+def header_symbols():
+    """Function names declared in include/ixg.h."""
+    import re
+    text = open(HEADER_PATH).read()
+    return sorted(set(re.findall(r"^[A-Za-z_][A-Za-z0-9_ \*]*?\b(ixg_[a-z0-9_]+)\s*\(", text,
+                                 re.M)))

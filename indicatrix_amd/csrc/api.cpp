@@ -1,0 +1,413 @@
+// api.cpp -- libixg's C ABI: context lifecycle, routing, chunked film launches.
+//
+// Mirrors the reference's GpuBackend contract (IX/renderer/gpu_backend/):
+//  * results are ADDED to the caller's accumulator and only on IXG_OK;
+//  * a HIP error or an injected fault marks the context lost (DeviceLost)
+//    and leaves the caller's buffer untouched; ixg_try_recover clears it;
+//  * the cancel flag is polled between chunks (GPU/frame/mod.rs:43-49).
+// One context is used by one host thread at a time (the reference serialises
+// GpuFrameRenderer behind a Mutex; callers here do the same).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/ixg.h"
+#include "host_scene.h"
+#include "transport_params.h"
+
+extern "C" hipError_t ixg_launch_film(const ixg::FilmParams* P, const float4* d_planes, float* d_xyz,
+                                      hipStream_t stream);
+extern "C" hipError_t ixg_launch_film_events(const ixg::FilmParams* P, const float4* d_planes,
+                                             float* d_xyz, ixg_path_events* d_ev,
+                                             unsigned long long* d_totals, hipStream_t stream);
+extern "C" hipError_t ixg_launch_rays(const ixg::FilmParams* P, const float4* d_planes, const float* d_rays,
+                                      uint32_t n_rays, float* d_xyz, ixg_path_events* d_ev,
+                                      hipStream_t stream);
+extern "C" hipError_t ixg_launch_reduce(const float* d_xyz, uint32_t spp, uint32_t n_pixels,
+                                        uint32_t first_pixel, double* d_accum, hipStream_t stream);
+
+struct ixg_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    bool lost = false;
+    int32_t fail_countdown = -1;  // chunks until an injected DeviceLost (-1 = none)
+    std::string last_error;
+    std::string label;
+    uint64_t chunk_paths = 1ull << 23;
+    // device buffers, grown on demand
+    float4* d_planes = nullptr;
+    float* d_xyz = nullptr;
+    size_t xyz_cap = 0;  // floats
+    double* d_accum = nullptr;
+    size_t accum_cap = 0;  // doubles
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    double last_kernel_ms_total = 0.0;
+    uint32_t last_kernel_launches = 0;
+};
+
+namespace {
+
+ixg_status fail(ixg_ctx* ctx, ixg_status st, const char* what) {
+    if (ctx) ctx->last_error = what;
+    return st;
+}
+
+ixg_status hip_fail(ixg_ctx* ctx, hipError_t e, const char* where) {
+    ctx->lost = true;
+    ctx->last_error = std::string(where) + ": " + hipGetErrorString(e);
+    return IXG_DEVICE_LOST;
+}
+
+#define HIP_TRY(ctx, expr)                                   \
+    do {                                                     \
+        hipError_t _e = (expr);                              \
+        if (_e != hipSuccess) return hip_fail(ctx, _e, #expr); \
+    } while (0)
+
+ixg_status ensure_xyz(ixg_ctx* ctx, size_t floats) {
+    if (floats <= ctx->xyz_cap) return IXG_OK;
+    if (ctx->d_xyz) (void)hipFree(ctx->d_xyz);
+    ctx->d_xyz = nullptr;
+    HIP_TRY(ctx, hipMalloc(&ctx->d_xyz, floats * sizeof(float)));
+    ctx->xyz_cap = floats;
+    return IXG_OK;
+}
+
+ixg_status ensure_accum(ixg_ctx* ctx, size_t doubles) {
+    if (doubles <= ctx->accum_cap) return IXG_OK;
+    if (ctx->d_accum) (void)hipFree(ctx->d_accum);
+    ctx->d_accum = nullptr;
+    HIP_TRY(ctx, hipMalloc(&ctx->d_accum, doubles * sizeof(double)));
+    ctx->accum_cap = doubles;
+    return IXG_OK;
+}
+
+bool material_routable(const ixg_material& m) {
+    // The device path covers isotropic, absorption-free, polished, unzoned,
+    // non-scattering materials (the configurations of this film path).
+    if (m.biaxial || m.scattering_sigma_s > 0.0f || m.edge_rounding_radius > 0.0f) return false;
+    bool anisotropic = !m.cubic && std::fabs(m.birefringence_delta) > 1e-4f;
+    if (anisotropic) return false;
+    if (m.n_bands_o != 0 || m.n_bands_e != 0) return false;
+    return m.dispersion.kind >= IXG_DISP_SELLMEIER1 && m.dispersion.kind <= IXG_DISP_CAUCHY;
+}
+
+ixg_status validate(ixg_ctx* ctx, const ixg_scene* s) {
+    if (!s || !s->planes || !s->material) return fail(ctx, IXG_INVALID, "null scene pointer");
+    if (s->width == 0 || s->height == 0) return fail(ctx, IXG_INVALID, "empty frame");
+    if ((uint64_t)s->width * s->height > (1ull << 31)) return fail(ctx, IXG_RESOURCE_LIMIT, "frame too large");
+    return IXG_OK;
+}
+
+ixg_status build_params(ixg_ctx* ctx, const ixg_scene* s, uint32_t sample_offset, uint32_t spp,
+                        ixg::FilmParams* P) {
+    std::memset(P, 0, sizeof *P);
+    std::memcpy(P->cam_origin, s->camera.origin, sizeof P->cam_origin);
+    P->fov_tan = s->camera.fov_tan;
+    std::memcpy(P->forward, s->camera.forward, sizeof P->forward);
+    std::memcpy(P->right, s->camera.right, sizeof P->right);
+    std::memcpy(P->up, s->camera.up, sizeof P->up);
+    P->width = s->width;
+    P->height = s->height;
+    P->width_f = (float)s->width;
+    P->height_f = (float)s->height;
+    P->spp = spp;
+    P->sample_offset = sample_offset;
+    P->n_planes = s->n_planes;
+    P->max_bounces = s->max_bounces;
+    P->disp_kind = s->material->dispersion.kind;
+    std::memcpy(P->disp, s->material->dispersion.coeff, sizeof P->disp);
+    float g = s->environment.surface_glare;
+    P->surface_glare = (g >= 1.0f || std::isnan(g)) ? 1.0f : (g > 0.0f ? g : 0.0f);
+    if (!ixg::resolve_environment(s->environment, &P->env))
+        return fail(ctx, IXG_DECLINED, "environment not supported on the device path");
+    return IXG_OK;
+}
+
+ixg_status upload_planes(ixg_ctx* ctx, const ixg_scene* s) {
+    std::vector<float4> pl(s->n_planes);
+    for (uint32_t i = 0; i < s->n_planes; ++i)
+        pl[i] = make_float4(s->planes[i].normal[0], s->planes[i].normal[1], s->planes[i].normal[2],
+                            s->planes[i].d);
+    if (s->n_planes)
+        HIP_TRY(ctx, hipMemcpyAsync(ctx->d_planes, pl.data(), pl.size() * sizeof(float4),
+                                    hipMemcpyHostToDevice, ctx->stream));
+    return IXG_OK;
+}
+
+ixg_status begin_call(ixg_ctx* ctx, const ixg_scene* s) {
+    if (!ctx) return IXG_INVALID;
+    if (ctx->lost) return fail(ctx, IXG_DEVICE_LOST, "context is lost; call ixg_try_recover");
+    ixg_status st = validate(ctx, s);
+    if (st != IXG_OK) return st;
+    ixg_route_reason r = ixg_route(s);
+    if (r != IXG_ROUTE_OK) return fail(ctx, IXG_DECLINED, "scene declined by ixg_route");
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    return IXG_OK;
+}
+
+bool take_injected_fault(ixg_ctx* ctx) {
+    if (ctx->fail_countdown < 0) return false;
+    if (ctx->fail_countdown == 0) {
+        ctx->fail_countdown = -1;
+        ctx->lost = true;
+        ctx->last_error = "injected device loss";
+        return true;
+    }
+    --ctx->fail_countdown;
+    return false;
+}
+
+// Runs transport + reduction for every chunk of whole pixels, adding into d_accum.
+ixg_status run_film(ixg_ctx* ctx, const ixg_scene* s, uint32_t sample_offset, uint32_t spp,
+                    double* d_accum, const volatile uint32_t* cancel, bool timed) {
+    ixg::FilmParams P;
+    ixg_status st = build_params(ctx, s, sample_offset, spp, &P);
+    if (st != IXG_OK) return st;
+    st = upload_planes(ctx, s);
+    if (st != IXG_OK) return st;
+    const uint32_t n_pixels = s->width * s->height;
+    uint64_t pix_per_chunk = std::max<uint64_t>(1, ctx->chunk_paths / spp);
+    pix_per_chunk = std::min<uint64_t>(pix_per_chunk, n_pixels);
+    st = ensure_xyz(ctx, (size_t)pix_per_chunk * spp * 3);
+    if (st != IXG_OK) return st;
+    ctx->last_kernel_ms_total = 0.0;
+    ctx->last_kernel_launches = 0;
+    for (uint64_t first = 0; first < n_pixels; first += pix_per_chunk) {
+        if (cancel && *cancel) return fail(ctx, IXG_CANCELLED, "cancelled");
+        if (take_injected_fault(ctx)) return IXG_DEVICE_LOST;
+        uint32_t npx = (uint32_t)std::min<uint64_t>(pix_per_chunk, n_pixels - first);
+        P.first_path = first * spp;
+        P.n_paths = (uint64_t)npx * spp;
+        if (timed) HIP_TRY(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+        HIP_TRY(ctx, ixg_launch_film(&P, ctx->d_planes, ctx->d_xyz, ctx->stream));
+        if (timed) {
+            HIP_TRY(ctx, hipEventRecord(ctx->ev1, ctx->stream));
+            HIP_TRY(ctx, hipEventSynchronize(ctx->ev1));
+            float ms = 0.0f;
+            HIP_TRY(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+            ctx->last_kernel_ms_total += ms;
+            ctx->last_kernel_launches += 1;
+        }
+        HIP_TRY(ctx, ixg_launch_reduce(ctx->d_xyz, spp, npx, (uint32_t)first, d_accum, ctx->stream));
+    }
+    return IXG_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+ixg_status ixg_create(const ixg_opts* opts, ixg_ctx** out) {
+    if (!out) return IXG_INVALID;
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n == 0) return IXG_DECLINED;
+    int dev = opts ? opts->device : 0;
+    if (dev < 0 || dev >= n) return IXG_INVALID;
+    ixg_ctx* ctx = new ixg_ctx();
+    ctx->device = dev;
+    if (opts && opts->chunk_paths) ctx->chunk_paths = opts->chunk_paths;
+    if (hipSetDevice(dev) != hipSuccess || hipStreamCreate(&ctx->stream) != hipSuccess ||
+        hipMalloc(&ctx->d_planes, IXG_MAX_PLANES * sizeof(float4)) != hipSuccess ||
+        hipEventCreate(&ctx->ev0) != hipSuccess || hipEventCreate(&ctx->ev1) != hipSuccess) {
+        ixg_destroy(ctx);
+        return IXG_DEVICE_LOST;
+    }
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, dev) == hipSuccess)
+        ctx->label = std::string(prop.name) + " (" + prop.gcnArchName + ")";
+    *out = ctx;
+    return IXG_OK;
+}
+
+void ixg_destroy(ixg_ctx* ctx) {
+    if (!ctx) return;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    if (ctx->d_planes) (void)hipFree(ctx->d_planes);
+    if (ctx->d_xyz) (void)hipFree(ctx->d_xyz);
+    if (ctx->d_accum) (void)hipFree(ctx->d_accum);
+    if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
+    if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+int ixg_is_lost(const ixg_ctx* ctx) { return ctx ? (ctx->lost ? 1 : 0) : 1; }
+
+ixg_status ixg_adapter_label(const ixg_ctx* ctx, char* buf, size_t n) {
+    if (!ctx || !buf || n == 0) return IXG_INVALID;
+    std::snprintf(buf, n, "%s", ctx->label.c_str());
+    return IXG_OK;
+}
+
+ixg_status ixg_last_error(const ixg_ctx* ctx, char* buf, size_t n) {
+    if (!ctx || !buf || n == 0) return IXG_INVALID;
+    std::snprintf(buf, n, "%s", ctx->last_error.c_str());
+    return IXG_OK;
+}
+
+ixg_status ixg_try_recover(ixg_ctx* ctx) {
+    if (!ctx) return IXG_INVALID;
+    if (!ctx->lost) return IXG_OK;
+    if (hipSetDevice(ctx->device) != hipSuccess) return IXG_DEVICE_LOST;
+    (void)hipGetLastError();
+    if (hipStreamSynchronize(ctx->stream) != hipSuccess) return IXG_DEVICE_LOST;
+    ctx->lost = false;
+    ctx->last_error.clear();
+    return IXG_OK;
+}
+
+ixg_status ixg_inject_fault(ixg_ctx* ctx, int32_t fail_on_chunk) {
+    if (!ctx) return IXG_INVALID;
+    ctx->fail_countdown = fail_on_chunk;
+    return IXG_OK;
+}
+
+ixg_route_reason ixg_route(const ixg_scene* s) {
+    if (!s || !s->material || !s->planes) return IXG_ROUTE_UNSUPPORTED_MATERIAL;
+    if (s->n_planes == 0 || s->n_planes > IXG_MAX_PLANES) return IXG_ROUTE_DEVICE_LIMITS;
+    if (!material_routable(*s->material)) return IXG_ROUTE_UNSUPPORTED_MATERIAL;
+    ixg::PresetInfo pi;
+    if (!ixg::preset_info(s->environment.preset, &pi)) return IXG_ROUTE_UNSUPPORTED_ENVIRONMENT;
+    if (s->environment.backdrop > 0.0f) return IXG_ROUTE_UNSUPPORTED_ENVIRONMENT;
+    return IXG_ROUTE_OK;
+}
+
+ixg_status ixg_accumulate(ixg_ctx* ctx, const ixg_scene* s, uint32_t sample_offset, uint32_t spp,
+                          double* accum_xyz, const volatile uint32_t* cancel) {
+    ixg_status st = begin_call(ctx, s);
+    if (st != IXG_OK) return st;
+    if (!accum_xyz) return fail(ctx, IXG_INVALID, "null accumulator");
+    if (spp == 0) return IXG_OK;
+    const size_t n = (size_t)s->width * s->height * 3;
+    st = ensure_accum(ctx, n);
+    if (st != IXG_OK) return st;
+    HIP_TRY(ctx, hipMemsetAsync(ctx->d_accum, 0, n * sizeof(double), ctx->stream));
+    st = run_film(ctx, s, sample_offset, spp, ctx->d_accum, cancel, false);
+    if (st != IXG_OK) {
+        (void)hipStreamSynchronize(ctx->stream);
+        return st;
+    }
+    std::vector<double> host(n);
+    HIP_TRY(ctx, hipMemcpyAsync(host.data(), ctx->d_accum, n * sizeof(double), hipMemcpyDeviceToHost,
+                                ctx->stream));
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    for (size_t i = 0; i < n; ++i) accum_xyz[i] += host[i];
+    return IXG_OK;
+}
+
+ixg_status ixg_accumulate_device(ixg_ctx* ctx, const ixg_scene* s, uint32_t sample_offset,
+                                 uint32_t spp, double* d_accum_xyz) {
+    ixg_status st = begin_call(ctx, s);
+    if (st != IXG_OK) return st;
+    if (!d_accum_xyz) return fail(ctx, IXG_INVALID, "null accumulator");
+    if (spp == 0) return IXG_OK;
+    st = run_film(ctx, s, sample_offset, spp, d_accum_xyz, nullptr, true);
+    if (st != IXG_OK) return st;
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    return IXG_OK;
+}
+
+ixg_status ixg_trace_paths(ixg_ctx* ctx, const ixg_scene* s, uint32_t sample_offset, uint32_t spp,
+                           float* out_xyz, ixg_path_events* events) {
+    ixg_status st = begin_call(ctx, s);
+    if (st != IXG_OK) return st;
+    const uint64_t paths = (uint64_t)s->width * s->height * spp;
+    if (paths == 0) return IXG_OK;
+    if (paths > (1ull << 26)) return fail(ctx, IXG_RESOURCE_LIMIT, "diagnostic trace limited to 2^26 paths");
+    ixg::FilmParams P;
+    st = build_params(ctx, s, sample_offset, spp, &P);
+    if (st != IXG_OK) return st;
+    st = upload_planes(ctx, s);
+    if (st != IXG_OK) return st;
+    float* d_xyz = nullptr;
+    ixg_path_events* d_ev = nullptr;
+    HIP_TRY(ctx, hipMalloc(&d_xyz, paths * 3 * sizeof(float)));
+    if (hipMalloc(&d_ev, paths * sizeof(ixg_path_events)) != hipSuccess) {
+        (void)hipFree(d_xyz);
+        return hip_fail(ctx, hipErrorOutOfMemory, "hipMalloc(events)");
+    }
+    P.first_path = 0;
+    P.n_paths = paths;
+    hipError_t e = ixg_launch_film_events(&P, ctx->d_planes, d_xyz, d_ev, nullptr, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e == hipSuccess && out_xyz) e = hipMemcpy(out_xyz, d_xyz, paths * 3 * sizeof(float), hipMemcpyDeviceToHost);
+    if (e == hipSuccess && events) e = hipMemcpy(events, d_ev, paths * sizeof(ixg_path_events), hipMemcpyDeviceToHost);
+    (void)hipFree(d_xyz);
+    (void)hipFree(d_ev);
+    if (e != hipSuccess) return hip_fail(ctx, e, "trace_paths");
+    return IXG_OK;
+}
+
+ixg_status ixg_trace_rays(ixg_ctx* ctx, const ixg_scene* s, const float* rays, uint32_t n,
+                          float* out_xyz, ixg_path_events* events) {
+    ixg_status st = begin_call(ctx, s);
+    if (st != IXG_OK) return st;
+    if (!rays || !out_xyz) return fail(ctx, IXG_INVALID, "null rays/out_xyz");
+    if (n == 0) return IXG_OK;
+    if (n > (1u << 24)) return fail(ctx, IXG_RESOURCE_LIMIT, "at most 2^24 rays per call");
+    ixg::FilmParams P;
+    st = build_params(ctx, s, 0, 1, &P);
+    if (st != IXG_OK) return st;
+    st = upload_planes(ctx, s);
+    if (st != IXG_OK) return st;
+    float *d_rays = nullptr, *d_xyz = nullptr;
+    ixg_path_events* d_ev = nullptr;
+    hipError_t e = hipMalloc(&d_rays, (size_t)n * 8 * sizeof(float));
+    if (e == hipSuccess) e = hipMalloc(&d_xyz, (size_t)n * 3 * sizeof(float));
+    if (e == hipSuccess && events) e = hipMalloc(&d_ev, (size_t)n * sizeof(ixg_path_events));
+    if (e == hipSuccess) e = hipMemcpy(d_rays, rays, (size_t)n * 8 * sizeof(float), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = ixg_launch_rays(&P, ctx->d_planes, d_rays, n, d_xyz, d_ev, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e == hipSuccess) e = hipMemcpy(out_xyz, d_xyz, (size_t)n * 3 * sizeof(float), hipMemcpyDeviceToHost);
+    if (e == hipSuccess && events) e = hipMemcpy(events, d_ev, (size_t)n * sizeof(ixg_path_events), hipMemcpyDeviceToHost);
+    if (d_rays) (void)hipFree(d_rays);
+    if (d_xyz) (void)hipFree(d_xyz);
+    if (d_ev) (void)hipFree(d_ev);
+    if (e != hipSuccess) return hip_fail(ctx, e, "trace_rays");
+    return IXG_OK;
+}
+
+ixg_status ixg_count_events(ixg_ctx* ctx, const ixg_scene* s, uint32_t sample_offset, uint32_t spp,
+                            uint64_t totals[8]) {
+    ixg_status st = begin_call(ctx, s);
+    if (st != IXG_OK) return st;
+    if (!totals) return fail(ctx, IXG_INVALID, "null totals");
+    ixg::FilmParams P;
+    st = build_params(ctx, s, sample_offset, spp, &P);
+    if (st != IXG_OK) return st;
+    st = upload_planes(ctx, s);
+    if (st != IXG_OK) return st;
+    unsigned long long* d_tot = nullptr;
+    HIP_TRY(ctx, hipMalloc(&d_tot, ixg::EV_NUM * sizeof(unsigned long long)));
+    hipError_t e = hipMemsetAsync(d_tot, 0, ixg::EV_NUM * sizeof(unsigned long long), ctx->stream);
+    const uint64_t paths = (uint64_t)s->width * s->height * spp;
+    for (uint64_t first = 0; e == hipSuccess && first < paths; first += ctx->chunk_paths) {
+        P.first_path = first;
+        P.n_paths = std::min<uint64_t>(ctx->chunk_paths, paths - first);
+        e = ixg_launch_film_events(&P, ctx->d_planes, nullptr, nullptr, d_tot, ctx->stream);
+    }
+    unsigned long long host[ixg::EV_NUM] = {};
+    if (e == hipSuccess) e = hipMemcpyAsync(host, d_tot, sizeof host, hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    (void)hipFree(d_tot);
+    if (e != hipSuccess) return hip_fail(ctx, e, "count_events");
+    for (int i = 0; i < ixg::EV_NUM; ++i) totals[i] = host[i];
+    return IXG_OK;
+}
+
+double ixg_last_kernel_ms(const ixg_ctx* ctx) {
+    if (!ctx || ctx->last_kernel_launches == 0) return 0.0;
+    return ctx->last_kernel_ms_total / ctx->last_kernel_launches;
+}
+
+uint32_t ixg_last_kernel_launches(const ixg_ctx* ctx) { return ctx ? ctx->last_kernel_launches : 0; }
+
+}  // extern "C"

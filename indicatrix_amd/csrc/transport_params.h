@@ -1,0 +1,64 @@
+// transport_params.h -- the resolved, device-ready form of one film call.
+//
+// The host turns an ixg_scene into this POD once per call (glibc libm for
+// every scene-constant transcendental: camera basis, light rig, white
+// balance), and the kernel reads it as a by-value kernel argument.
+#pragma once
+#include <stdint.h>
+
+namespace ixg {
+
+enum EnvModel : int32_t { ENV_STUDIO = 0, ENV_ISO_HEMISPHERE = 1 };
+
+struct EnvParams {
+    int32_t model;       // EnvModel
+    int32_t d65;         // 1: tabulated D65 illuminant, 0: Planck fit
+    float bb_t_k;        // BlackbodyNorm::t_k
+    float bb_denom_560;  // BlackbodyNorm::denom_560
+    float exposure;
+    float spot_mult;
+    float shadow_outer;  // head-shadow cosines [outer, inner]
+    float shadow_inner;
+    float key[3];
+    float fill[3];
+    float ring[16][3];
+    float wb[3];         // von Kries LMS scale (1,1,1 for D65 presets)
+};
+
+struct FilmParams {
+    // GpuCameraParams fields
+    float cam_origin[3];
+    float fov_tan;
+    float forward[3];
+    float width_f;
+    float right[3];
+    float height_f;
+    float up[3];
+    uint32_t width;
+    uint32_t height;
+    uint32_t spp;
+    uint32_t sample_offset;
+    uint32_t n_planes;
+    uint32_t max_bounces;
+    int32_t disp_kind;
+    float disp[6];
+    float surface_glare;  // clamped to [0,1]
+    uint64_t first_path;  // first (pixel*spp + sample) index of this launch
+    uint64_t n_paths;     // paths in this launch
+    EnvParams env;
+};
+
+// Event-count slots accumulated by the counting kernel.
+enum EventSlot : int {
+    EV_PATHS = 0,
+    EV_BOUNCES = 1,      // loop iterations (one slab scan each)
+    EV_TIR = 2,
+    EV_REFLECT = 3,
+    EV_REFRACT = 4,
+    EV_PROBE = 5,        // exit-split probe slab scans
+    EV_ESCAPED = 6,
+    EV_ROTATIONS = 7,    // Stokes frame rotations
+    EV_NUM = 8
+};
+
+}  // namespace ixg

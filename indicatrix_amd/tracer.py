@@ -1,0 +1,310 @@
+"""Host-side mirror of the reference's film API over libixg's C ABI.
+
+Names follow the reference: ``GpuBackend.try_accumulate`` /
+``try_accumulate_cancellable`` (IX/renderer/gpu_backend/backend.rs:182-300),
+``GpuAccumulate`` {Done, Declined, Cancelled} (IX/renderer/gpu_backend/mod.rs:307-326),
+``FrameScene`` (IX/renderer/frame_scene.rs:22-40), ``Camera.new``
+(IX/optics/raytracer/camera.rs:108), ``GemMaterial`` (IX/optics/materials/mod.rs:116)
+and ``LightingPreset.studio`` (IX/optics/raytracer/environment/mod.rs:620).
+
+Arrays are numpy; device work goes through libixg only -- there is no CPU
+fallback in this module (a declined scene is reported as Declined, exactly
+as the reference's GPU backend reports it to its caller).
+"""
+import ctypes as C
+import enum
+
+import numpy as np
+
+from . import abi
+from ._lib import IxgError, lib
+
+
+class GpuAccumulate(enum.Enum):
+    DONE = "Done"
+    DECLINED = "Declined"
+    CANCELLED = "Cancelled"
+
+
+# --------------------------------------------------------------------------
+# materials
+# --------------------------------------------------------------------------
+class GemMaterial:
+    """The GemMaterial fields the transport reads."""
+
+    def __init__(self, name, dispersion_kind, coeff, birefringence_delta=0.0, cubic=True,
+                 c_axis=(0.0, 1.0, 0.0), absorption_path_scale=1.0):
+        self.name = name
+        self.dispersion_kind = dispersion_kind
+        self.coeff = list(coeff) + [0.0] * (6 - len(coeff))
+        self.birefringence_delta = birefringence_delta
+        self.cubic = cubic
+        self.c_axis = tuple(c_axis)
+        self.absorption_path_scale = absorption_path_scale
+        self.bands_o = []
+        self.bands_e = []
+
+    def to_abi(self):
+        m = abi.Material()
+        m.dispersion.kind = self.dispersion_kind
+        for i, v in enumerate(self.coeff):
+            m.dispersion.coeff[i] = v
+        m.birefringence_delta = self.birefringence_delta
+        for i in range(3):
+            m.c_axis[i] = self.c_axis[i]
+        m.cubic = 1 if self.cubic else 0
+        m.n_bands_o = len(self.bands_o)
+        m.n_bands_e = len(self.bands_e)
+        for i, (c, w, p, s) in enumerate(self.bands_o):
+            m.bands_o[i] = abi.Band(c, w, p, s)
+        for i, (c, w, p, s) in enumerate(self.bands_e):
+            m.bands_e[i] = abi.Band(c, w, p, s)
+        m.absorption_path_scale = self.absorption_path_scale
+        return m
+
+    @staticmethod
+    def diamond():
+        """Diamond (IX/optics/materials/diamond_through_emerald.rs:24-57): Peter (1923)
+        Sellmeier in the reference's term order."""
+        return GemMaterial("Diamond", abi.DISP_SELLMEIER3,
+                           [4.3356, 0.3306, 0.0, 0.011236, 0.030625, 1.0])
+
+    @staticmethod
+    def isotropic_from_nd_abbe(n_d, abbe, name="custom"):
+        """A one-term Sellmeier curve through n(589.3 nm) = n_d with Abbe number
+        V_d = (n_d - 1) / (n_F - n_C), solved in double precision."""
+        b1, c1 = sellmeier1_from_nd_abbe(n_d, abbe)
+        return GemMaterial(name, abi.DISP_SELLMEIER1, [b1, c1])
+
+
+def sellmeier1_from_nd_abbe(n_d, abbe):
+    """Solves n^2 = 1 + b l^2 / (l^2 - c) (l in um) for (b, c) so that n(D) = n_d
+    and (n_d - 1)/(n(F) - n(C)) = abbe."""
+    lD, lF, lC = 0.5893, 0.4861, 0.6563
+
+    def n_of(b, c, l):
+        return (1.0 + b * l * l / (l * l - c)) ** 0.5
+
+    def b_for(c):
+        return (n_d * n_d - 1.0) * (lD * lD - c) / (lD * lD)
+
+    target = (n_d - 1.0) / abbe
+    lo, hi = 1e-6, 0.2
+    for _ in range(200):
+        mid = 0.5 * (lo + hi)
+        b = b_for(mid)
+        disp = n_of(b, mid, lF) - n_of(b, mid, lC)
+        if disp < target:
+            lo = mid
+        else:
+            hi = mid
+    c = 0.5 * (lo + hi)
+    return b_for(c), c
+
+
+# --------------------------------------------------------------------------
+# lighting
+# --------------------------------------------------------------------------
+class LightingPreset:
+    NAMES = dict(abi.PRESETS)
+
+    @staticmethod
+    def studio(preset, exposure=1.0, light_yaw=0.0, light_pitch=0.0, head_shadow_deg=16.0,
+               surface_glare=1.0):
+        pid = abi.PRESETS[preset] if isinstance(preset, str) else int(preset)
+        return abi.Environment(pid, exposure, light_yaw, light_pitch, 0.0, surface_glare,
+                               head_shadow_deg)
+
+
+# --------------------------------------------------------------------------
+# camera and cuts (host arithmetic in libixg, FP32 + glibc libm)
+# --------------------------------------------------------------------------
+def camera_new(yaw, pitch, distance, fov_deg, width, height, spp=1):
+    cam = abi.Camera()
+    lib().ixg_camera_new(yaw, pitch, distance, fov_deg, width, height, spp, C.byref(cam))
+    return cam
+
+
+def planes_array(planes):
+    """numpy (N,4) float32 [nx, ny, nz, d] from an abi.FacetPlane array."""
+    return np.array([[p.normal[0], p.normal[1], p.normal[2], p.d] for p in planes], np.float32)
+
+
+def planes_from_array(arr):
+    arr = np.asarray(arr, np.float32).reshape(-1, 4)
+    out = (abi.FacetPlane * len(arr))()
+    for i, r in enumerate(arr):
+        out[i] = abi.FacetPlane((C.c_float * 3)(*r[:3]), float(r[3]))
+    return out
+
+
+def standard_round_brilliant():
+    buf = (abi.FacetPlane * 80)()
+    n = lib().ixg_cut_standard_round_brilliant(buf)
+    return (abi.FacetPlane * n)(*buf[:n])
+
+
+def round_brilliant(crown_deg, pavilion_deg, table_ratio):
+    buf = (abi.FacetPlane * 80)()
+    n = lib().ixg_cut_round_brilliant(crown_deg, pavilion_deg, table_ratio, buf)
+    return (abi.FacetPlane * n)(*buf[:n])
+
+
+# --------------------------------------------------------------------------
+# scenes and the backend
+# --------------------------------------------------------------------------
+class FrameScene:
+    """Owns the ctypes storage behind one ixg_scene."""
+
+    def __init__(self, camera, width, height, planes, material, max_bounces, environment):
+        self.planes = planes if isinstance(planes, C.Array) else planes_from_array(planes)
+        self.material_abi = material.to_abi() if isinstance(material, GemMaterial) else material
+        self.scene = abi.Scene()
+        self.scene.camera = camera
+        self.scene.width = width
+        self.scene.height = height
+        self.scene.planes = C.cast(self.planes, C.POINTER(abi.FacetPlane))
+        self.scene.n_planes = len(self.planes)
+        self.scene.material = C.pointer(self.material_abi)
+        self.scene.max_bounces = max_bounces
+        self.scene.environment = environment
+
+    @property
+    def width(self):
+        return self.scene.width
+
+    @property
+    def height(self):
+        return self.scene.height
+
+    def ref(self):
+        return C.byref(self.scene)
+
+
+class GpuBackend:
+    """A libixg context (GpuBackend in the reference)."""
+
+    def __init__(self, device=0, chunk_paths=0):
+        self._ctx = C.c_void_p()
+        opts = abi.Opts(device, chunk_paths)
+        st = lib().ixg_create(C.byref(opts), C.byref(self._ctx))
+        if st != abi.IXG_OK:
+            raise IxgError(st, "ixg_create failed (no HIP device?)")
+
+    def close(self):
+        if self._ctx:
+            lib().ixg_destroy(self._ctx)
+            self._ctx = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def handle(self):
+        return self._ctx
+
+    def last_error(self):
+        buf = C.create_string_buffer(512)
+        lib().ixg_last_error(self._ctx, buf, 512)
+        return buf.value.decode()
+
+    def adapter_label(self):
+        buf = C.create_string_buffer(256)
+        lib().ixg_adapter_label(self._ctx, buf, 256)
+        return buf.value.decode()
+
+    def is_lost(self):
+        return bool(lib().ixg_is_lost(self._ctx))
+
+    def try_recover(self):
+        return lib().ixg_try_recover(self._ctx) == abi.IXG_OK
+
+    def inject_fault(self, fail_on_chunk):
+        lib().ixg_inject_fault(self._ctx, fail_on_chunk)
+
+    def _check(self, st):
+        if st not in (abi.IXG_OK, abi.IXG_DECLINED, abi.IXG_CANCELLED):
+            raise IxgError(st, self.last_error())
+        return st
+
+    def try_accumulate_cancellable(self, scene, sample_offset, spp, accum, cancel=None):
+        """ADDS per-pixel FP64 XYZ sums into ``accum`` (float64, (h*w, 3) or
+        (h, w, 3), C-contiguous) only when the result is Done."""
+        if accum.dtype != np.float64 or not accum.flags.c_contiguous:
+            raise ValueError("accum must be a C-contiguous float64 array")
+        if accum.size != scene.width * scene.height * 3:
+            raise ValueError("accum length must be width*height*3")
+        cflag = None if cancel is None else C.cast(cancel, C.POINTER(C.c_uint32))
+        st = lib().ixg_accumulate(self._ctx, scene.ref(), sample_offset, spp,
+                                  accum.ctypes.data_as(C.POINTER(C.c_double)), cflag)
+        st = self._check(st)
+        return {abi.IXG_OK: GpuAccumulate.DONE, abi.IXG_DECLINED: GpuAccumulate.DECLINED,
+                abi.IXG_CANCELLED: GpuAccumulate.CANCELLED}[st]
+
+    def try_accumulate(self, scene, sample_offset, spp, accum):
+        return self.try_accumulate_cancellable(scene, sample_offset, spp, accum) \
+            is GpuAccumulate.DONE
+
+    def accumulate_device(self, scene, sample_offset, spp, d_accum_ptr):
+        st = lib().ixg_accumulate_device(self._ctx, scene.ref(), sample_offset, spp,
+                                         C.c_void_p(d_accum_ptr))
+        if st != abi.IXG_OK:
+            raise IxgError(st, self.last_error())
+
+    def last_kernel_ms(self):
+        return lib().ixg_last_kernel_ms(self._ctx)
+
+    def trace_paths(self, scene, sample_offset, spp, want_xyz=True, want_events=True):
+        n = scene.width * scene.height * spp
+        xyz = np.zeros((n, 3), np.float32) if want_xyz else None
+        ev = (abi.PathEvents * n)() if want_events else None
+        st = lib().ixg_trace_paths(
+            self._ctx, scene.ref(), sample_offset, spp,
+            xyz.ctypes.data_as(C.POINTER(C.c_float)) if want_xyz else None,
+            C.cast(ev, C.POINTER(abi.PathEvents)) if want_events else None)
+        if st != abi.IXG_OK:
+            raise IxgError(st, self.last_error())
+        return xyz, (events_to_numpy(ev) if want_events else None)
+
+    def trace_rays(self, scene, origins, dirs, seeds, heros):
+        """trace_spectral_ray for explicit rays -> ((n,3) float32 XYZ, events)."""
+        origins = np.asarray(origins, np.float32).reshape(-1, 3)
+        dirs = np.asarray(dirs, np.float32).reshape(-1, 3)
+        n = len(origins)
+        rays = np.zeros((n, 8), np.float32)
+        rays[:, 0:3] = origins
+        rays[:, 3:6] = dirs
+        rays[:, 6] = np.asarray(heros, np.float32)
+        rays[:, 7] = np.asarray(seeds, np.uint32).view(np.float32)
+        xyz = np.zeros((n, 3), np.float32)
+        ev = (abi.PathEvents * n)()
+        st = lib().ixg_trace_rays(self._ctx, scene.ref(), rays.ctypes.data_as(C.POINTER(C.c_float)),
+                                  n, xyz.ctypes.data_as(C.POINTER(C.c_float)),
+                                  C.cast(ev, C.POINTER(abi.PathEvents)))
+        if st != abi.IXG_OK:
+            raise IxgError(st, self.last_error())
+        return xyz, events_to_numpy(ev)
+
+    def count_events(self, scene, sample_offset, spp):
+        tot = (C.c_uint64 * 8)()
+        st = lib().ixg_count_events(self._ctx, scene.ref(), sample_offset, spp, tot)
+        if st != abi.IXG_OK:
+            raise IxgError(st, self.last_error())
+        return dict(zip(EVENT_SLOTS, list(tot)))
+
+
+EVENT_SLOTS = ["paths", "bounces", "tir", "reflect", "refract", "probe", "escaped", "rotations"]
+
+
+def route(scene):
+    return lib().ixg_route(scene.ref())
+
+
+def events_to_numpy(ev):
+    dt = np.dtype([("bounces", "<u4"), ("termination", "<u4"), ("facet_hash", "<u4"),
+                   ("n_tir", "<u2"), ("n_reflect", "<u2"), ("n_refract", "<u2"),
+                   ("n_probe", "<u2")])
+    return np.frombuffer(bytes(ev), dtype=dt).copy()

@@ -1,0 +1,90 @@
+"""ctypes binding of the CPU oracle (oracle/_build/libixo.so) for the tests.
+
+The oracle is the checker only; nothing in indicatrix_amd imports it.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+from indicatrix_amd import abi
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_DIR = os.path.join(ROOT, "oracle")
+LIB = os.path.join(ORACLE_DIR, "_build", "libixo.so")
+_lib = None
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", ORACLE_DIR], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            build()
+        L = C.CDLL(LIB)
+        P = C.POINTER
+        f3 = C.c_float * 3
+        L.ixo_hash_u32.restype = C.c_uint32
+        L.ixo_hash_u32.argtypes = [C.c_uint32]
+        L.ixo_low_discrepancy_base2.restype = C.c_float
+        L.ixo_low_discrepancy_base2.argtypes = [C.c_uint32]
+        L.ixo_radical_inverse_base.restype = C.c_float
+        L.ixo_radical_inverse_base.argtypes = [C.c_uint32, C.c_uint32]
+        L.ixo_sample_draws.argtypes = [C.c_uint32, C.c_uint32, P(C.c_uint32), P(C.c_float),
+                                       P(C.c_float), P(C.c_float)]
+        L.ixo_wrapped_hero_wavelengths.argtypes = [C.c_float, P(C.c_float)]
+        L.ixo_dispersion_evaluate.restype = C.c_float
+        L.ixo_dispersion_evaluate.argtypes = [P(abi.Dispersion), C.c_float]
+        L.ixo_cie_cmf.argtypes = [C.c_float, P(C.c_float)]
+        L.ixo_d65.restype = C.c_float
+        L.ixo_d65.argtypes = [C.c_float]
+        L.ixo_exp_lane.restype = C.c_float
+        L.ixo_exp_lane.argtypes = [C.c_float]
+        L.ixo_sample_studio_environment.restype = C.c_float
+        L.ixo_sample_studio_environment.argtypes = [P(C.c_float), C.c_float, C.c_int32,
+                                                    C.c_float, C.c_float, C.c_float]
+        L.ixo_camera_new.argtypes = [C.c_float, C.c_float, C.c_float, C.c_float, C.c_uint32,
+                                     C.c_uint32, C.c_uint32, P(abi.Camera)]
+        L.ixo_cut_standard_round_brilliant.restype = C.c_uint32
+        L.ixo_cut_standard_round_brilliant.argtypes = [P(abi.FacetPlane)]
+        L.ixo_cut_round_brilliant.restype = C.c_uint32
+        L.ixo_cut_round_brilliant.argtypes = [C.c_float, C.c_float, C.c_float, P(abi.FacetPlane)]
+        L.ixo_intersect.restype = C.c_int32
+        L.ixo_intersect.argtypes = [P(abi.FacetPlane), C.c_uint32, P(C.c_float), P(C.c_float),
+                                    P(C.c_float)]
+        L.ixo_trace_ray.argtypes = [P(abi.Scene), f3, f3, C.c_uint32, C.c_float, P(C.c_float),
+                                    P(abi.PathEvents)]
+        L.ixo_trace_sample.argtypes = [P(abi.Scene), C.c_uint32, C.c_uint32, P(C.c_float),
+                                       P(abi.PathEvents)]
+        L.ixo_accumulate.argtypes = [P(abi.Scene), C.c_uint32, C.c_uint32, P(C.c_double), C.c_int]
+        L.ixo_trace_paths.argtypes = [P(abi.Scene), C.c_uint32, C.c_uint32, C.c_uint64,
+                                      C.c_uint64, P(C.c_float), P(abi.PathEvents), C.c_int]
+        _lib = L
+    return _lib
+
+
+def threads():
+    return max(1, min(16, os.cpu_count() or 1))
+
+
+def accumulate(scene, sample_offset, spp, nthreads=None):
+    acc = np.zeros((scene.width * scene.height, 3), np.float64)
+    lib().ixo_accumulate(scene.ref(), sample_offset, spp,
+                         acc.ctypes.data_as(C.POINTER(C.c_double)), nthreads or threads())
+    return acc
+
+
+def trace_paths(scene, sample_offset, spp, first=0, count=None, nthreads=None):
+    from indicatrix_amd.tracer import events_to_numpy
+    total = scene.width * scene.height * spp
+    count = total - first if count is None else count
+    xyz = np.zeros((count, 3), np.float32)
+    ev = (abi.PathEvents * count)()
+    lib().ixo_trace_paths(scene.ref(), sample_offset, spp, first, count,
+                          xyz.ctypes.data_as(C.POINTER(C.c_float)),
+                          C.cast(ev, C.POINTER(abi.PathEvents)), nthreads or threads())
+    return xyz, events_to_numpy(ev)
