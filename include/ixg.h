@@ -199,6 +199,79 @@ ixg_status ixg_count_events(ixg_ctx* ctx, const ixg_scene* scene, uint32_t sampl
 double ixg_last_kernel_ms(const ixg_ctx* ctx);
 uint32_t ixg_last_kernel_launches(const ixg_ctx* ctx);
 
+/* ---- metrics ----
+ * OpticalPixel (IX/color/metrics/image_types.rs:59-75): per-pixel class
+ * counts {windowed, extinct, returned, entry-blocked/invalid} over the
+ * pixel's deterministic sub-pixel samples, the weighted Fire sum, and the
+ * temporal-scintillation pose counts. */
+typedef struct ixg_optical_pixel {
+    uint32_t hits;
+    uint32_t classes[4];
+    float fire_sum_deg;
+    uint32_t fire_qualified;
+    uint32_t fire_rejected;
+    int32_t centre_entry_facet; /* -1 = none */
+    int32_t centre_exit_facet;  /* -1 = none */
+    uint32_t centre_bounces;
+    uint32_t centre_reason;     /* ExtinctionReason ordinal */
+    uint32_t mixed_facets;
+    uint8_t scint_returned;
+    uint8_t scint_poses;
+    uint8_t pad[2];
+} ixg_optical_pixel;
+
+/* OpticalImageRequest (IX/color/metrics/image_types.rs:89-164). */
+typedef struct ixg_metric_request {
+    ixg_camera camera;             /* origin/forward/right/up/fov_tan are read     */
+    uint32_t width, height;        /* <= 1048576 pixels                            */
+    uint32_t samples;              /* 1..64                                        */
+    uint32_t projection[2];        /* 0,0 = width,height                           */
+    int32_t has_tile_origin;
+    uint32_t tile_origin[2];
+    const ixg_facet_plane* planes;
+    uint32_t n_planes;
+    const ixg_material* material;
+    ixg_environment environment;
+    int32_t want_fire;
+    int32_t want_scintillation;
+    int32_t has_pose;
+    float pose[3];                 /* yaw, pitch, distance (needed for scintillation) */
+} ixg_metric_request;
+
+/* Image totals plus the image-domain metrics: shares of the visible area
+ * (OpticalImage::total.fractions), the Fire index over all traced rays
+ * (accumulate.rs:140-152 convention, x275, floor 0.1) and the image
+ * scintillation (image_scint.rs:261-287). */
+typedef struct ixg_metric_totals {
+    ixg_optical_pixel total;
+    float brilliance_pct, windowing_pct, extinction_pct, fire_index, scintillation_pct;
+} ixg_metric_totals;
+
+/* evaluate_optical_image (IX/color/metrics/image.rs:187-262) on the GPU.
+ * pixels[width*height] may be NULL.  Errors: IXG_RESOURCE_LIMIT (pixels,
+ * samples), IXG_INVALID (non-unit planes, bad camera, scintillation without
+ * a pose), IXG_DECLINED (environment without a device lighting model). */
+ixg_status ixg_metric_image(ixg_ctx* ctx, const ixg_metric_request* req, ixg_optical_pixel* pixels,
+                            ixg_metric_totals* totals);
+
+/* One design of a batched metric sweep: its facet set inside a shared plane
+ * table and its dispersion curve. */
+typedef struct ixg_design {
+    uint32_t plane_offset, n_planes;
+    ixg_dispersion dispersion;
+} ixg_design;
+
+/* Batched tilt sweep (MET/sweep.rs, MET/pose_batch.rs:73-199 generalised to
+ * images): every (design, tilt) image of width x height at `samples`, with
+ * Fire and scintillation, in ONE device pass; camera yaw 0 / 180 deg by the
+ * tilt's sign, pitch 90 - |tilt|, at `distance`, fov 42 deg.
+ * out_metrics[(d*n_tilts + t)*5] = {brilliance, fire, scintillation,
+ * windowing, extinction}. */
+ixg_status ixg_metric_sweep(ixg_ctx* ctx, const ixg_facet_plane* planes, uint32_t total_planes,
+                            const ixg_design* designs, uint32_t n_designs, const float* tilts_deg,
+                            uint32_t n_tilts, uint32_t width, uint32_t height, uint32_t samples,
+                            float distance, const ixg_environment* env, float* out_metrics);
+
 /* ---- host helpers (same f32 arithmetic as the reference, glibc libm) ---- */
 /* Camera::new (RT/camera.rs:108). */
 void ixg_camera_new(float yaw, float pitch, float distance, float fov_deg, uint32_t width,

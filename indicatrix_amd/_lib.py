@@ -44,6 +44,11 @@ def _declare(lib):
                                 P(C.c_float), P(abi.PathEvents)]),
         "ixg_count_events": (st, [C.c_void_p, P(abi.Scene), C.c_uint32, C.c_uint32,
                                   P(C.c_uint64)]),
+        "ixg_metric_image": (st, [C.c_void_p, P(abi.MetricRequest), P(abi.OpticalPixel),
+                                  P(abi.MetricTotals)]),
+        "ixg_metric_sweep": (st, [C.c_void_p, P(abi.FacetPlane), C.c_uint32, P(abi.Design),
+                                  C.c_uint32, P(C.c_float), C.c_uint32, C.c_uint32, C.c_uint32,
+                                  C.c_uint32, C.c_float, P(abi.Environment), P(C.c_float)]),
         "ixg_last_kernel_ms": (C.c_double, [C.c_void_p]),
         "ixg_last_kernel_launches": (C.c_uint32, [C.c_void_p]),
         "ixg_camera_new": (None, [C.c_float, C.c_float, C.c_float, C.c_float, C.c_uint32,

@@ -100,3 +100,36 @@ def check_layout():
 
 
 check_layout()
+
+
+class OpticalPixel(C.Structure):
+    _fields_ = [("hits", C.c_uint32), ("classes", C.c_uint32 * 4), ("fire_sum_deg", C.c_float),
+                ("fire_qualified", C.c_uint32), ("fire_rejected", C.c_uint32),
+                ("centre_entry_facet", C.c_int32), ("centre_exit_facet", C.c_int32),
+                ("centre_bounces", C.c_uint32), ("centre_reason", C.c_uint32),
+                ("mixed_facets", C.c_uint32), ("scint_returned", C.c_uint8),
+                ("scint_poses", C.c_uint8), ("pad", C.c_uint8 * 2)]
+
+
+class MetricRequest(C.Structure):
+    _fields_ = [("camera", Camera), ("width", C.c_uint32), ("height", C.c_uint32),
+                ("samples", C.c_uint32), ("projection", C.c_uint32 * 2),
+                ("has_tile_origin", C.c_int32), ("tile_origin", C.c_uint32 * 2),
+                ("planes", C.POINTER(FacetPlane)), ("n_planes", C.c_uint32),
+                ("material", C.POINTER(Material)), ("environment", Environment),
+                ("want_fire", C.c_int32), ("want_scintillation", C.c_int32),
+                ("has_pose", C.c_int32), ("pose", C.c_float * 3)]
+
+
+class MetricTotals(C.Structure):
+    _fields_ = [("total", OpticalPixel), ("brilliance_pct", C.c_float),
+                ("windowing_pct", C.c_float), ("extinction_pct", C.c_float),
+                ("fire_index", C.c_float), ("scintillation_pct", C.c_float)]
+
+
+class Design(C.Structure):
+    _fields_ = [("plane_offset", C.c_uint32), ("n_planes", C.c_uint32),
+                ("dispersion", Dispersion)]
+
+
+assert C.sizeof(OpticalPixel) == 56

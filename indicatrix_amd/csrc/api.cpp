@@ -403,6 +403,13 @@ ixg_status ixg_count_events(ixg_ctx* ctx, const ixg_scene* s, uint32_t sample_of
     return IXG_OK;
 }
 
+// internals shared with the other host translation units of libixg
+hipStream_t ixg_internal_stream(ixg_ctx* ctx) { return ctx->stream; }
+int ixg_internal_device(ixg_ctx* ctx) { return ctx->device; }
+ixg_status ixg_internal_fail(ixg_ctx* ctx, ixg_status st, const char* what) { return fail(ctx, st, what); }
+ixg_status ixg_internal_hip_fail(ixg_ctx* ctx, hipError_t e, const char* where) { return hip_fail(ctx, e, where); }
+int ixg_internal_lost(ixg_ctx* ctx) { return ctx->lost ? 1 : 0; }
+
 double ixg_last_kernel_ms(const ixg_ctx* ctx) {
     if (!ctx || ctx->last_kernel_launches == 0) return 0.0;
     return ctx->last_kernel_ms_total / ctx->last_kernel_launches;

@@ -1,0 +1,245 @@
+// device_common.h -- device arithmetic shared by the film and metric kernels.
+//
+// Everything here follows the reference's f32 evaluation order (glam's
+// left-associated dot, multiply-then-add, explicit FMA only where the
+// reference writes mul_add); callers compile with -ffp-contract=off.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/ixg.h"
+#include "transport_params.h"
+
+#define IX_TABLE_QUAL __constant__ static const
+#include "spectral_tables.h"
+
+namespace ixg {
+
+#define NCH 8
+#define DIRECTION_MATCH_COS_TOL (1.0f - 1e-6f)
+#define R_UNPOL_PDF_MIN 1e-4f
+#define R_UNPOL_PDF_MAX (1.0f - 1e-4f)
+#define R_UNPOL_SELECT_MIN 0.02f
+#define R_UNPOL_SELECT_MAX 0.98f
+#define FRESNEL_BRANCH_STREAM 0x9E3779B1u
+#define RUSSIAN_ROULETTE_STREAM 0x517CC1B7u
+
+// ---------------------------------------------------------------------------
+// scalar / vector helpers (glam order: left-associated dot, no contraction)
+// ---------------------------------------------------------------------------
+struct V3 {
+    float x, y, z;
+};
+__device__ __forceinline__ V3 v3(float x, float y, float z) { return V3{x, y, z}; }
+__device__ __forceinline__ V3 operator+(V3 a, V3 b) { return v3(a.x + b.x, a.y + b.y, a.z + b.z); }
+__device__ __forceinline__ V3 operator-(V3 a, V3 b) { return v3(a.x - b.x, a.y - b.y, a.z - b.z); }
+__device__ __forceinline__ V3 operator*(V3 a, float s) { return v3(a.x * s, a.y * s, a.z * s); }
+__device__ __forceinline__ V3 neg(V3 a) { return v3(-a.x, -a.y, -a.z); }
+__device__ __forceinline__ float dot(V3 a, V3 b) { return (a.x * b.x + a.y * b.y) + a.z * b.z; }
+__device__ __forceinline__ V3 cross(V3 a, V3 b) {
+    return v3(a.y * b.z - b.y * a.z, a.z * b.x - b.z * a.x, a.x * b.y - b.x * a.y);
+}
+__device__ __forceinline__ V3 normalize(V3 a) { return a * (1.0f / sqrtf(dot(a, a))); }
+__device__ __forceinline__ V3 normalize_or_zero(V3 a) {
+    float r = 1.0f / sqrtf(dot(a, a));
+    return (__builtin_isfinite(r) && r > 0.0f) ? a * r : v3(0.0f, 0.0f, 0.0f);
+}
+// Rust f32::clamp (NaN propagates)
+__device__ __forceinline__ float clampf(float v, float lo, float hi) {
+    v = v < lo ? lo : v;
+    return v > hi ? hi : v;
+}
+__device__ __forceinline__ float fma_(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+// powi with LLVM's binary-exponentiation order
+template <int N>
+__device__ __forceinline__ float powi(float x) {
+    float res = 1.0f;
+    bool have = false;
+    float sq = x;
+    int n = N;
+#pragma unroll
+    for (int b = 0; b < 6; ++b) {
+        if (n & 1) {
+            res = have ? res * sq : sq;
+            have = true;
+        }
+        n >>= 1;
+        if (n) sq = sq * sq;
+    }
+    return res;
+}
+
+__device__ __forceinline__ uint32_t hash_u32(uint32_t x) {
+    x *= 0x85ebca6bu;
+    x ^= x >> 13;
+    x *= 0xc2b2ae35u;
+    x ^= x >> 16;
+    return x;
+}
+__device__ __forceinline__ float unit_draw(uint32_t seed, uint32_t bounce, uint32_t stream) {
+    return (float)hash_u32(seed ^ hash_u32(bounce ^ stream)) / 4294967295.0f;
+}
+__device__ __forceinline__ float radical_inverse(uint32_t n, uint32_t base) {
+    float val = 0.0f;
+    float inv_base = 1.0f / (float)base;
+    while (n > 0) {
+        uint32_t digit = n % base;
+        val = fma_((float)digit, inv_base, val);
+        inv_base /= (float)base;
+        n /= base;
+    }
+    return val;
+}
+__device__ __forceinline__ float ld_base2(uint32_t n) {
+    return (float)__builtin_bitreverse32(n) / 4294967296.0f;
+}
+__device__ __forceinline__ float cp_rotate(float x, float off) {
+    float s = x + off;
+    return s - floorf(s);
+}
+
+// ---------------------------------------------------------------------------
+// spectra
+// ---------------------------------------------------------------------------
+// DispersionModel::evaluate (optics/dispersion.rs:170-210), lambda >= 380 nm
+__device__ __forceinline__ float dispersion(int kind, const float* c, float lambda) {
+    float um = lambda * 1e-3f;
+    float l2 = um * um;
+    if (kind == IXG_DISP_SELLMEIER3) {
+        float n2 = 1.0f;
+        n2 += (c[0] * l2) / (l2 - c[3]);
+        n2 += (c[1] * l2) / (l2 - c[4]);
+        n2 += (c[2] * l2) / (l2 - c[5]);
+        return sqrtf(fmaxf(n2, 1.0f));
+    } else if (kind == IXG_DISP_SELLMEIER1) {
+        float n2 = 1.0f + (c[0] * l2) / (l2 - c[1]);
+        return sqrtf(fmaxf(n2, 1.0f));
+    }
+    float l4 = l2 * l2;
+    return fmaxf((c[0] + (c[1] / l2)) + (c[2] / l4), 1.0f);
+}
+
+__device__ __forceinline__ void cie_cmf(float lambda, float& x, float& y, float& z) {
+    const float end = 380.0f + (float)(IX_CMF_ROWS - 1) * 5.0f;
+    if (!(lambda >= 380.0f && lambda <= end)) {
+        x = y = z = 0.0f;
+        return;
+    }
+    float pos = (lambda - 380.0f) / 5.0f;
+    uint32_t i0 = (uint32_t)floorf(pos);
+    i0 = i0 > IX_CMF_ROWS - 2 ? IX_CMF_ROWS - 2 : i0;
+    float t = pos - (float)i0;
+    x = ix_cmf_table[i0][0] + (ix_cmf_table[i0 + 1][0] - ix_cmf_table[i0][0]) * t;
+    y = ix_cmf_table[i0][1] + (ix_cmf_table[i0 + 1][1] - ix_cmf_table[i0][1]) * t;
+    z = ix_cmf_table[i0][2] + (ix_cmf_table[i0 + 1][2] - ix_cmf_table[i0][2]) * t;
+}
+
+__device__ __forceinline__ float d65(float lambda) {
+    float c = clampf(lambda, 380.0f, fma_(10.0f, (float)(IX_D65_ROWS - 1), 380.0f));
+    float pos = (c - 380.0f) / 10.0f;
+    uint32_t i0 = (uint32_t)floorf(pos);
+    i0 = i0 > IX_D65_ROWS - 2 ? IX_D65_ROWS - 2 : i0;
+    float frac = pos - (float)i0;
+    float v0 = ix_d65_table[i0], v1 = ix_d65_table[i0 + 1];
+    return fma_(frac, v1 - v0, v0) / 100.0f;
+}
+
+__device__ __forceinline__ float spec_power(const EnvParams& E, float lambda) {
+    if (E.d65) return d65(lambda);
+    float e = expf(fminf(14388000.0f / (lambda * E.bb_t_k), 80.0f));
+    float denom = fmaxf(e - 1.0f, 1e-6f);
+    float ratio = E.bb_denom_560 / denom;
+    return clampf(powi<5>(560.0f / lambda) * ratio, 0.01f, 20.0f);
+}
+
+__device__ __forceinline__ float smoothstep(float e0, float e1, float x) {
+    float t = clampf((x - e0) / (e1 - e0), 0.0f, 1.0f);
+    return t * t * fma_(-2.0f, t, 3.0f);
+}
+
+// Wavelength-independent lighting of one exit direction (rig.rs:379-405).
+struct DirLight {
+    float bg;       // studio: backdrop term; iso: the scaled factor
+    float lights[18];
+    int n_lights;
+};
+
+__device__ __forceinline__ void direction_lighting(const EnvParams& E, V3 dir, V3 observer,
+                                                   DirLight& L) {
+    V3 d = normalize(dir);
+    L.n_lights = 0;
+    if (E.model == ENV_ISO_HEMISPHERE) {
+        float horizon = smoothstep(-0.05f, 0.05f, d.y);
+        float vis = 1.0f - smoothstep(E.shadow_outer, E.shadow_inner, dot(d, observer));
+        L.bg = horizon * vis;
+        return;
+    }
+    L.bg = fmaxf(fma_(0.012f, fma_(d.y, 0.5f, 0.5f), 0.015f), 0.005f) * E.exposure;
+    float key_dot = fmaxf(dot(d, v3(E.key[0], E.key[1], E.key[2])), 0.0f);
+    if (key_dot > 0.0f) L.lights[L.n_lights++] = powi<28>(key_dot) * 12.0f * E.spot_mult * E.exposure;
+    float fill_dot = fmaxf(dot(d, v3(E.fill[0], E.fill[1], E.fill[2])), 0.0f);
+    if (fill_dot > 0.0f) L.lights[L.n_lights++] = powi<18>(fill_dot) * 4.5f * E.exposure;
+    for (int i = 0; i < 16; ++i) {
+        float rd = fmaxf(dot(d, v3(E.ring[i][0], E.ring[i][1], E.ring[i][2])), 0.0f);
+        if (rd > 0.96f) {
+            float spark = (rd - 0.96f) / 0.04f;
+            L.lights[L.n_lights++] = powi<6>(spark) * 22.0f * E.spot_mult * E.exposure;
+        }
+    }
+}
+
+__device__ __forceinline__ float light_radiance(const EnvParams& E, const DirLight& L, float sp) {
+    if (E.model == ENV_ISO_HEMISPHERE) return L.bg * (sp * E.exposure);
+    float r = L.bg * sp;
+    for (int i = 0; i < L.n_lights; ++i) r = fma_(L.lights[i], sp, r);
+    return r;
+}
+
+// Iso-hemisphere radiance needs no light list: evaluate it without the
+// DirLight scratch so the common config stays in registers.
+__device__ __forceinline__ float iso_factor(const EnvParams& E, V3 dir, V3 observer) {
+    V3 d = normalize(dir);
+    float horizon = smoothstep(-0.05f, 0.05f, d.y);
+    float vis = 1.0f - smoothstep(E.shadow_outer, E.shadow_inner, dot(d, observer));
+    return horizon * vis;
+}
+
+// ---------------------------------------------------------------------------
+// intersection against the LDS facet set
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int intersect(const float4* __restrict__ pl, uint32_t n, V3 o, V3 d,
+                                         float& t_out) {
+    float t_near = -1e30f, t_far = 1e30f;
+    int near_idx = -1, far_idx = -1;
+    for (uint32_t i = 0; i < n; ++i) {
+        float4 p = pl[i];
+        float denom = (p.x * d.x + p.y * d.y) + p.z * d.z;
+        float side = p.w + ((p.x * o.x + p.y * o.y) + p.z * o.z);
+        if (fabsf(denom) > 1e-7f) {
+            float t = (-side) / denom;
+            if (denom < 0.0f) {
+                if (t > t_near) {
+                    t_near = t;
+                    near_idx = (int)i;
+                }
+            } else if (t < t_far) {
+                t_far = t;
+                far_idx = (int)i;
+            }
+        } else if (side > 0.0f) {
+            return -1;
+        }
+    }
+    if (t_near > t_far) return -1;
+    if (t_near > 1e-4f) {
+        t_out = t_near;
+        return near_idx < 0 ? 0 : near_idx;
+    }
+    if (t_far > 1e-4f) {
+        t_out = t_far;
+        return far_idx < 0 ? 0 : far_idx;
+    }
+    return -1;
+}
+
+}  // namespace ixg

@@ -36,6 +36,36 @@ void cie_cmf_host(float lambda, float out[3]) {
         out[c] = ix_cmf_table[i0][c] + (ix_cmf_table[i0 + 1][c] - ix_cmf_table[i0][c]) * t;
 }
 
+float d65_host(float lambda) {
+    float c = hclamp(lambda, 380.0f, std::fma(10.0f, (float)(IX_D65_ROWS - 1), 380.0f));
+    float pos = (c - 380.0f) / 10.0f;
+    size_t i0 = (size_t)std::floor(pos);
+    if (i0 > IX_D65_ROWS - 2) i0 = IX_D65_ROWS - 2;
+    float frac = pos - (float)i0;
+    float v0 = ix_d65_table[i0], v1 = ix_d65_table[i0 + 1];
+    return std::fma(frac, v1 - v0, v0) / 100.0f;
+}
+
+float dispersion_host(const ixg_dispersion& d, float lambda) {
+    if (lambda < 380.0f) lambda = 380.0f;
+    float um = lambda * 1e-3f;
+    float l2 = um * um;
+    const float* c = d.coeff;
+    if (d.kind == IXG_DISP_SELLMEIER3) {
+        float n2 = 1.0f;
+        n2 += (c[0] * l2) / (l2 - c[3]);
+        n2 += (c[1] * l2) / (l2 - c[4]);
+        n2 += (c[2] * l2) / (l2 - c[5]);
+        return std::sqrt(std::fmax(n2, 1.0f));
+    }
+    if (d.kind == IXG_DISP_SELLMEIER1) {
+        float n2 = 1.0f + (c[0] * l2) / (l2 - c[1]);
+        return std::sqrt(std::fmax(n2, 1.0f));
+    }
+    float l4 = l2 * l2;
+    return std::fmax((c[0] + (c[1] / l2)) + (c[2] / l4), 1.0f);
+}
+
 namespace {
 struct BB {
     float t_k, denom_560;

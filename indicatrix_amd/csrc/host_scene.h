@@ -47,6 +47,10 @@ bool resolve_environment(const ixg_environment& env, EnvParams* out);
 float blackbody(float lambda_nm, float temp_k);
 // compute_illuminant_white_balance (color.rs:268-305).
 void white_balance(float temp_k, float out[3]);
+// D65 relative power (environment/spectral.rs:49-76, >= 380 nm), host copy.
+float d65_host(float lambda_nm);
+// DispersionModel::evaluate (optics/dispersion.rs:170-210, >= 380 nm), host copy.
+float dispersion_host(const ixg_dispersion& d, float lambda_nm);
 // CIE 1931 CMF table lookup (color/cie1931.rs:184-209), host copy.
 void cie_cmf_host(float lambda_nm, float out[3]);
 

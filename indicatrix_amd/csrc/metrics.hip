@@ -1,0 +1,397 @@
+// metrics.hip -- brilliance / fire / scintillation images on CDNA4 (gfx950).
+//
+// The reference evaluates these on the CPU only (color/metrics/): per pixel,
+// up to 64 deterministic sub-pixel rays are classified by a single-wavelength
+// scalar-Fresnel trace at the d line (<= 10 internal bounces), the returned
+// ones replayed at the F and C lines for Fire, and the pixel centre re-traced
+// from five yaw-offset cameras for temporal scintillation.
+//
+// Device layout: a 2-D grid, blockIdx.y = image (design x camera pose),
+// blockIdx.x = a 256-pixel tile of it; the image's facet set is staged in LDS
+// once per block.  One lane owns one pixel and writes its OpticalPixel; a
+// second kernel reduces each image in raster order (one lane per image), so
+// every f32 sum is formed in the reference's order.
+//
+// Reference map (paths relative to crates/indicatrix/src/color/metrics/):
+//   offset / sample_pixel      image.rs:21-175
+//   classify_ray / fire_sample classify.rs:109-256
+//   trace_entry / interior     ray_trace.rs:77-425
+//   returned_or_reason         visibility.rs:55-67, lighting.rs:208
+//   temporal sampler           image_scint.rs:35-130
+//   image scintillation        image_scint.rs:261-287, scintillation.rs:146-198
+#include "device_common.h"
+
+namespace ixg {
+
+constexpr int kMBlock = 256;
+
+enum { FATE_ENTRY_BLOCKED = 0, FATE_LEAKED = 1, FATE_UPWARD = 2, FATE_ABSORBED = 3 };
+enum { REASON_NONE = 0, REASON_SHADOW = 1, REASON_ILLUM = 2, REASON_SIDE = 3, REASON_NOHIT = 4,
+       REASON_BUDGET = 5, REASON_ENTRY = 6 };
+enum { CLS_NONE = -1, CLS_WINDOWED = 0, CLS_EXTINCT = 1, CLS_RETURNED = 2, CLS_OTHER = 3 };
+
+struct Fate {
+    int fate;
+    V3 dir;
+    float transmittance, exit_cos;
+    int facet;
+    uint32_t bounces;
+    int exit_facet;
+    uint32_t d_bounces;
+    uint32_t reason;
+};
+
+__device__ __forceinline__ float fresnel_T(float n1, float n2, float ci, float ct) {
+    float ds = fma_(n2, ct, n1 * ci);
+    float dp = fma_(n2, ci, n1 * ct);
+    if (fabsf(ds) < 1e-6f || fabsf(dp) < 1e-6f) return 0.0f;
+    float a = fma_(n2, -ct, n1 * ci) / ds;
+    float b = fma_(n2, -ci, n1 * ct) / dp;
+    return clampf(1.0f - 0.5f * (a * a + b * b), 0.0f, 1.0f);
+}
+
+// trace_entry for a polished entry facet (ray_trace.rs:256-425).
+__device__ __forceinline__ void trace_entry(const float4* __restrict__ pl, uint32_t n, V3 point,
+                                            V3 dir, V3 nrm, float cos_i, float index, Fate& F) {
+    F.fate = FATE_ABSORBED;
+    F.exit_facet = -1;
+    F.d_bounces = 0;
+    F.reason = REASON_BUDGET;
+    F.facet = 0;
+    F.bounces = 0;
+    F.transmittance = 0.0f;
+    F.exit_cos = 0.0f;
+    F.dir = v3(0.0f, 0.0f, 0.0f);
+    const float sin2_t = (1.0f / (index * index)) * fma_(cos_i, -cos_i, 1.0f);
+    if (sin2_t > 1.0f) {
+        F.fate = FATE_ENTRY_BLOCKED;
+        F.reason = REASON_ENTRY;
+        return;
+    }
+    const float cos_t = sqrtf(1.0f - sin2_t);
+    const float inv = 1.0f / index;
+    V3 cur = normalize(dir * inv + nrm * fma_(inv, cos_i, -cos_t));
+    const float entry_T = fresnel_T(1.0f, index, cos_i, cos_t);
+    V3 hp = point + cur * 1e-4f;
+    const float sin_crit = 1.0f / index;
+    const float idx2 = index * index;
+    for (uint32_t b = 0; b < 10; ++b) {
+        F.d_bounces = b;
+        float t;
+        int f = intersect(pl, n, hp, cur, t);
+        if (f < 0) {
+            F.reason = REASON_NOHIT;
+            return;
+        }
+        V3 np = hp + cur * t;
+        float4 pf = pl[f];
+        V3 n_out = v3(pf.x, pf.y, pf.z);
+        float ct = clampf(dot(cur, n_out), 0.0f, 1.0f);
+        float one_m = fma_(ct, -ct, 1.0f);
+        float st = sqrtf(fmaxf(one_m, 0.0f));
+        if (st < sin_crit) {
+            float s2o = idx2 * one_m;
+            if (s2o <= 1.0f) {
+                float co = sqrtf(1.0f - s2o);
+                V3 od = normalize(cur * index + n_out * fma_(index, -ct, co));
+                F.exit_facet = f;
+                F.reason = REASON_SIDE;
+                if (n_out.y < -0.05f) {
+                    F.reason = REASON_NONE;
+                    F.fate = FATE_LEAKED;
+                } else if (od.y > 0.05f) {
+                    F.reason = REASON_NONE;
+                    F.fate = FATE_UPWARD;
+                    F.dir = od;
+                    F.transmittance = entry_T * fresnel_T(index, 1.0f, ct, co);
+                    F.exit_cos = co;
+                    F.facet = f;
+                    F.bounces = b;
+                }
+                return;
+            }
+        }
+        cur = normalize(cur - n_out * (2.0f * ct));
+        hp = np + cur * 1e-4f;
+        F.d_bounces = b + 1;
+    }
+}
+
+__device__ __forceinline__ float scoring_radiance(const MetricShared& S, V3 dir) {
+    if (S.env.model == ENV_ISO_HEMISPHERE) {
+        V3 d = normalize(dir);
+        float horizon = smoothstep(-0.05f, 0.05f, d.y);
+        float vis = 1.0f - smoothstep(S.env.shadow_outer, S.env.shadow_inner, 0.0f);
+        return (horizon * vis) * (S.spec550 * 1.0f);
+    }
+    DirLight L;
+    direction_lighting(S.env, dir, v3(0.0f, 0.0f, 0.0f), L);
+    return light_radiance(S.env, L, S.spec550);
+}
+
+struct Classified {
+    int cls;
+    int has_fire;
+    float angle, weight;
+    int entry_facet;
+};
+
+// classify_ray (classify.rs:109-197) + fire_sample (classify.rs:199-256).
+__device__ __forceinline__ void classify(const MetricShared& S, const MetricImage& I,
+                                         const float4* __restrict__ pl, V3 org, V3 dir,
+                                         bool want_fire, Classified& C, Fate* detail) {
+    C.cls = CLS_NONE;
+    C.has_fire = 0;
+    C.entry_facet = -1;
+    float t;
+    int f = intersect(pl, I.n_planes, org, dir, t);
+    if (f < 0) return;
+    C.entry_facet = f;
+    V3 hp = org + dir * t;
+    float4 pf = pl[f];
+    V3 nrm = v3(pf.x, pf.y, pf.z);
+    float cos_i = clampf(dot(neg(dir), nrm), 0.0f, 1.0f);
+    Fate F;
+    trace_entry(pl, I.n_planes, hp, dir, nrm, cos_i, I.nd, F);
+    if (detail) *detail = F;
+    if (F.fate == FATE_ENTRY_BLOCKED) { C.cls = CLS_OTHER; return; }
+    if (F.fate == FATE_LEAKED) { C.cls = CLS_WINDOWED; return; }
+    if (F.fate == FATE_ABSORBED) { C.cls = CLS_EXTINCT; return; }
+    if (dot(F.dir, neg(dir)) > S.head_cos) {
+        if (detail) detail->reason = REASON_SHADOW;
+        C.cls = CLS_EXTINCT;
+        return;
+    }
+    if (!(scoring_radiance(S, F.dir) > S.threshold)) {
+        if (detail) detail->reason = REASON_ILLUM;
+        C.cls = CLS_EXTINCT;
+        return;
+    }
+    C.cls = CLS_RETURNED;
+    if (want_fire) {
+        Fate A, B;
+        trace_entry(pl, I.n_planes, hp, dir, nrm, cos_i, I.nf, A);
+        trace_entry(pl, I.n_planes, hp, dir, nrm, cos_i, I.nc, B);
+        if (A.fate == FATE_UPWARD && B.fate == FATE_UPWARD && A.facet == B.facet &&
+            A.bounces == B.bounces) {
+            float cs = clampf(dot(A.dir, B.dir), -1.0f, 1.0f);
+            C.angle = acosf(cs) * 57.2957795130823208767981548141051703f;
+            float energy = F.transmittance * A.transmittance * B.transmittance;
+            float radiance = fmaxf(A.exit_cos, 0.0f) * fmaxf(B.exit_cos, 0.0f);
+            C.weight = energy * radiance;
+            C.has_fire = 1;
+        }
+    }
+}
+
+// Camera::generate_ray with zero jitter (camera.rs:149-167).
+__device__ __forceinline__ void cam_ray(const float* fwd, const float* right, const float* up,
+                                        float fov_tan, float sx, float sy, float pw, float ph,
+                                        V3& dir) {
+    float aspect = pw / ph;
+    float u = ((sx + 0.0f) / pw - 0.5f) * 2.0f * aspect * fov_tan;
+    float v = (0.5f - (sy + 0.0f) / ph) * 2.0f * fov_tan;
+    dir = normalize((v3(fwd[0], fwd[1], fwd[2]) + v3(right[0], right[1], right[2]) * u) +
+                    v3(up[0], up[1], up[2]) * v);
+}
+
+// image.rs:21-36: sample 0 is the pixel centre, then base-2 / base-3 radical
+// inverses evaluated in double precision.
+__device__ __forceinline__ void sample_offset(uint32_t s, float& jx, float& jy) {
+    if (s == 0) {
+        jx = 0.5f;
+        jy = 0.5f;
+        return;
+    }
+    double radical = (double)__builtin_bitreverse32(s) / 4294967296.0;
+    uint32_t n = s;
+    double scale = 1.0 / 3.0, y = 0.0;
+    while (n > 0) {
+        y += (double)(n % 3) * scale;
+        n /= 3;
+        scale /= 3.0;
+    }
+    jx = (float)radical;
+    jy = (float)y;
+}
+
+__global__ __launch_bounds__(kMBlock) void metric_pixels_kernel(const MetricShared S,
+                                                                const MetricImage* __restrict__ images,
+                                                                const float4* __restrict__ planes,
+                                                                ixg_optical_pixel* __restrict__ out) {
+    __shared__ float4 sp[IXG_MAX_PLANES];
+    const uint32_t img = blockIdx.y;
+    const MetricImage& I = images[img];
+    for (uint32_t i = threadIdx.x; i < I.n_planes; i += blockDim.x) sp[i] = planes[I.plane_offset + i];
+    __syncthreads();
+    const uint32_t npx = S.width * S.height;
+    const uint32_t p = blockIdx.x * kMBlock + threadIdx.x;
+    if (p >= npx) return;
+    const uint32_t x = p % S.width, y = p / S.width;
+    const float pw = (float)S.proj_w, ph = (float)S.proj_h;
+    const V3 org = v3(I.origin[0], I.origin[1], I.origin[2]);
+
+    ixg_optical_pixel px;
+    px.hits = 0;
+    px.classes[0] = px.classes[1] = px.classes[2] = px.classes[3] = 0;
+    px.fire_sum_deg = 0.0f;
+    px.fire_qualified = px.fire_rejected = 0;
+    px.centre_entry_facet = -1;
+    px.centre_exit_facet = -1;
+    px.centre_bounces = 0;
+    px.centre_reason = 0;
+    px.mixed_facets = 0;
+    px.scint_returned = 0;
+    px.scint_poses = 0;
+    px.pad[0] = px.pad[1] = 0;
+    int first = -1;
+    for (uint32_t s = 0; s < S.samples; ++s) {
+        float jx, jy;
+        sample_offset(s, jx, jy);
+        float sx = fma_((float)x + jx, S.scale_x, (float)S.origin_x - 0.5f);
+        float sy = fma_((float)y + jy, S.scale_y, (float)S.origin_y - 0.5f);
+        V3 dir;
+        cam_ray(I.forward, I.right, I.up, I.fov_tan, sx, sy, pw, ph, dir);
+        Classified C;
+        Fate det;
+        classify(S, I, sp, org, dir, S.want_fire != 0, C, s == 0 ? &det : nullptr);
+        if (C.cls == CLS_NONE) continue;
+        px.hits++;
+        if (s == 0) {
+            px.centre_entry_facet = C.entry_facet;
+            px.centre_exit_facet = det.exit_facet;
+            px.centre_bounces = det.d_bounces;
+            px.centre_reason = det.reason;
+        }
+        if (first >= 0) px.mixed_facets |= (uint32_t)(C.entry_facet != first);
+        else first = C.entry_facet;
+        if (C.cls == CLS_RETURNED && S.want_fire) {
+            if (C.has_fire && __builtin_isfinite(C.angle) && __builtin_isfinite(C.weight)) {
+                px.fire_sum_deg = fma_(I.zero_dispersion ? 0.0f : C.angle, C.weight, px.fire_sum_deg);
+                px.fire_qualified++;
+            } else {
+                px.fire_rejected++;
+            }
+        }
+        px.classes[C.cls]++;
+    }
+    if (px.hits > 0 && S.want_scintillation) {
+        // TemporalSampler::apply (image_scint.rs:95-118): the centre ray from
+        // five yaw-offset cameras; pose 2 (no offset) must hit for any count.
+        float sx = fma_((float)x + 0.5f, S.scale_x, (float)S.origin_x - 0.5f);
+        float sy = fma_((float)y + 0.5f, S.scale_y, (float)S.origin_y - 0.5f);
+        int ret[5];
+#pragma unroll
+        for (int i = 0; i < 5; ++i) {
+            V3 dir;
+            cam_ray(I.t_forward[i], I.t_right[i], I.t_up[i], I.fov_tan, sx, sy, pw, ph, dir);
+            Classified C;
+            classify(S, I, sp, v3(I.t_origin[i][0], I.t_origin[i][1], I.t_origin[i][2]), dir, false, C,
+                     nullptr);
+            ret[i] = C.cls == CLS_NONE ? -1 : (C.cls == CLS_RETURNED ? 1 : 0);
+        }
+        if (ret[2] >= 0) {
+            uint32_t nret = ret[2] == 1 ? 1u : 0u;
+#pragma unroll
+            for (int i = 0; i < 5; ++i)
+                if (i != 2 && ret[i] == 1) ++nret;
+            px.scint_returned = (uint8_t)nret;
+            px.scint_poses = 5;
+        }
+    }
+    out[(size_t)img * npx + p] = px;
+}
+
+// Per-image totals and metrics in raster order (one lane per image).
+__global__ __launch_bounds__(64) void metric_reduce_kernel(const MetricShared S,
+                                                           const ixg_optical_pixel* __restrict__ px,
+                                                           ixg_metric_totals* __restrict__ totals) {
+    const uint32_t img = blockIdx.x * 64 + threadIdx.x;
+    if (img >= S.n_images) return;
+    const uint32_t npx = S.width * S.height;
+    const ixg_optical_pixel* P = px + (size_t)img * npx;
+    ixg_metric_totals T;
+    T.total.hits = 0;
+    T.total.classes[0] = T.total.classes[1] = T.total.classes[2] = T.total.classes[3] = 0;
+    T.total.fire_sum_deg = 0.0f;
+    T.total.fire_qualified = T.total.fire_rejected = 0;
+    T.total.centre_entry_facet = T.total.centre_exit_facet = -1;
+    T.total.centre_bounces = T.total.centre_reason = T.total.mixed_facets = 0;
+    T.total.scint_returned = T.total.scint_poses = 0;
+    T.total.pad[0] = T.total.pad[1] = 0;
+    float sum = 0.0f, sum_sq = 0.0f, var_sum = 0.0f;
+    uint32_t count = 0, poses = 0;
+    for (uint32_t i = 0; i < npx; ++i) {
+        const ixg_optical_pixel q = P[i];
+        T.total.hits += q.hits;
+        for (int c = 0; c < 4; ++c) T.total.classes[c] += q.classes[c];
+        T.total.fire_sum_deg += q.fire_sum_deg;
+        T.total.fire_qualified += q.fire_qualified;
+        T.total.fire_rejected += q.fire_rejected;
+        if (q.hits > 0) {
+            float f = (float)q.classes[2] / (float)(q.hits > 1 ? q.hits : 1);
+            sum += f;
+            sum_sq = fma_(f, f, sum_sq);
+            ++count;
+        }
+    }
+    for (uint32_t i = 0; i < npx; ++i) {
+        const ixg_optical_pixel q = P[i];
+        if (q.scint_poses) {
+            float fr = (float)q.scint_returned / (float)q.scint_poses;
+            var_sum += fma_(fr, -fr, fr);
+            ++poses;
+        }
+    }
+    float spatial = 0.0f;
+    if (count) {
+        float mean = sum / (float)count;
+        if (mean > 1e-4f) {
+            float var = fmaxf(fma_(mean, -mean, sum_sq / (float)count), 0.0f);
+            float cv = sqrtf(var) / mean;
+            spatial = clampf(cv / (1.0f + cv) * 100.0f, 0.0f, 100.0f);
+        }
+    }
+    float temporal = poses ? clampf(((var_sum / (float)poses) / 0.24f) * 100.0f, 0.0f, 100.0f) : 0.0f;
+    const float h = (float)(T.total.hits > 1 ? T.total.hits : 1);
+    T.windowing_pct = clampf((float)T.total.classes[0] / h * 100.0f, 0.0f, 100.0f);
+    T.extinction_pct = clampf((float)T.total.classes[1] / h * 100.0f, 0.0f, 100.0f);
+    T.brilliance_pct = clampf((float)T.total.classes[2] / h * 100.0f, 0.0f, 100.0f);
+    T.fire_index = fmaxf(T.total.fire_sum_deg / h * 275.0f, 0.1f);
+    T.scintillation_pct = clampf(fma_(0.4f, temporal, 0.6f * spatial), 0.0f, 100.0f);
+    totals[img] = T;
+}
+
+// The scoring radiance at explicit directions (the ambient-level probes).
+__global__ void scoring_probe_kernel(const MetricShared S, const float* __restrict__ dirs, uint32_t n,
+                                     float* __restrict__ out) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    out[i] = scoring_radiance(S, v3(dirs[3 * i], dirs[3 * i + 1], dirs[3 * i + 2]));
+}
+
+}  // namespace ixg
+
+extern "C" hipError_t ixg_launch_scoring_probe(const ixg::MetricShared* S, const float* d_dirs, uint32_t n,
+                                               float* d_out, hipStream_t stream) {
+    hipLaunchKernelGGL(ixg::scoring_probe_kernel, dim3((n + 63) / 64), dim3(64), 0, stream, *S, d_dirs, n,
+                       d_out);
+    return hipGetLastError();
+}
+
+extern "C" hipError_t ixg_launch_metric_pixels(const ixg::MetricShared* S, const ixg::MetricImage* d_images,
+                                               const float4* d_planes, ixg_optical_pixel* d_out,
+                                               hipStream_t stream) {
+    uint32_t npx = S->width * S->height;
+    dim3 grid((npx + ixg::kMBlock - 1) / ixg::kMBlock, S->n_images);
+    hipLaunchKernelGGL(ixg::metric_pixels_kernel, grid, dim3(ixg::kMBlock), 0, stream, *S, d_images,
+                       d_planes, d_out);
+    return hipGetLastError();
+}
+
+extern "C" hipError_t ixg_launch_metric_reduce(const ixg::MetricShared* S, const ixg_optical_pixel* d_px,
+                                               ixg_metric_totals* d_totals, hipStream_t stream) {
+    dim3 grid((S->n_images + 63) / 64);
+    hipLaunchKernelGGL(ixg::metric_reduce_kernel, grid, dim3(64), 0, stream, *S, d_px, d_totals);
+    return hipGetLastError();
+}

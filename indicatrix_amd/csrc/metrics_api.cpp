@@ -1,0 +1,282 @@
+// metrics_api.cpp -- host side of the metric images and the batched sweep.
+//
+// Scene constants are resolved once per call on the host (glibc libm):
+// the three line indices, the scoring rig, the head-shadow cone and the
+// temporal cameras.  The ambient level that sets the illumination threshold
+// (96 golden-angle hemisphere probes, lighting.rs:115-190) is evaluated by
+// the device's own radiance function and sorted here.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "../../include/ixg.h"
+#include "host_scene.h"
+#include "transport_params.h"
+
+extern "C" hipError_t ixg_launch_metric_pixels(const ixg::MetricShared* S, const ixg::MetricImage* d_images,
+                                               const float4* d_planes, ixg_optical_pixel* d_out,
+                                               hipStream_t stream);
+extern "C" hipError_t ixg_launch_metric_reduce(const ixg::MetricShared* S, const ixg_optical_pixel* d_px,
+                                               ixg_metric_totals* d_totals, hipStream_t stream);
+extern "C" hipError_t ixg_launch_scoring_probe(const ixg::MetricShared* S, const float* d_dirs, uint32_t n,
+                                               float* d_out, hipStream_t stream);
+
+// libixg internals shared with api.cpp
+extern "C" hipStream_t ixg_internal_stream(ixg_ctx* ctx);
+extern "C" int ixg_internal_device(ixg_ctx* ctx);
+extern "C" ixg_status ixg_internal_fail(ixg_ctx* ctx, ixg_status st, const char* what);
+extern "C" ixg_status ixg_internal_hip_fail(ixg_ctx* ctx, hipError_t e, const char* where);
+extern "C" int ixg_internal_lost(ixg_ctx* ctx);
+
+namespace {
+
+using namespace ixg;
+
+constexpr float kToRad = kPi / 180.0f;
+
+// visibility.rs:22-40
+float head_shadow_cone_cos(bool studio_rig, float deg) {
+    if (studio_rig || std::isnan(deg) || deg == 16.0f) return 0.96f;
+    if (deg <= 0.0f) return 2.0f;
+    float offset = std::acos(0.96f) - 16.0f * kToRad;
+    return std::cos(hclamp(deg, 1.0f, 89.0f) * kToRad + offset);
+}
+
+struct Resolved {
+    MetricShared S;
+};
+
+ixg_status resolve_shared(ixg_ctx* ctx, const ixg_environment& env, uint32_t width, uint32_t height,
+                          uint32_t samples, MetricShared* S) {
+    std::memset(S, 0, sizeof *S);
+    ixg_environment scoring = env;
+    scoring.exposure = 1.0f;
+    scoring.head_shadow_deg = 16.0f;
+    if (!resolve_environment(scoring, &S->env))
+        return ixg_internal_fail(ctx, IXG_DECLINED, "environment has no device lighting model");
+    PresetInfo pi;
+    preset_info(env.preset, &pi);
+    S->spec550 = pi.d65 ? d65_host(550.0f) : blackbody(550.0f, pi.temp_k);
+    S->head_cos = head_shadow_cone_cos(pi.model == ENV_STUDIO, env.head_shadow_deg);
+    S->width = width;
+    S->height = height;
+    S->samples = samples;
+    // ambient level: the device radiance at 96 hemisphere probes, median
+    std::vector<float> dirs(96 * 3);
+    for (int i = 0; i < 96; ++i) {
+        float y = ((float)i + 0.5f) / 96.0f;
+        float ring = std::sqrt(std::fmax(std::fma(y, -y, 1.0f), 0.0f));
+        float a = (float)i * 2.3999632f;
+        dirs[3 * i] = ring * std::cos(a);
+        dirs[3 * i + 1] = y;
+        dirs[3 * i + 2] = ring * std::sin(a);
+    }
+    hipStream_t st = ixg_internal_stream(ctx);
+    float *d_dirs = nullptr, *d_out = nullptr;
+    hipError_t e = hipMalloc(&d_dirs, dirs.size() * sizeof(float));
+    if (e == hipSuccess) e = hipMalloc(&d_out, 96 * sizeof(float));
+    if (e == hipSuccess) e = hipMemcpyAsync(d_dirs, dirs.data(), dirs.size() * sizeof(float), hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = ixg_launch_scoring_probe(S, d_dirs, 96, d_out, st);
+    std::vector<float> rad(96);
+    if (e == hipSuccess) e = hipMemcpyAsync(rad.data(), d_out, 96 * sizeof(float), hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (d_dirs) (void)hipFree(d_dirs);
+    if (d_out) (void)hipFree(d_out);
+    if (e != hipSuccess) return ixg_internal_hip_fail(ctx, e, "scoring probe");
+    std::sort(rad.begin(), rad.end());
+    bool uniform = pi.model != ENV_STUDIO;
+    S->threshold = std::fmax(rad[48] * (uniform ? 0.5f : 4.0f), 1.17549435e-38f);
+    return IXG_OK;
+}
+
+void fill_camera(const ixg_camera& c, MetricImage* I) {
+    std::memcpy(I->origin, c.origin, sizeof I->origin);
+    I->fov_tan = c.fov_tan;
+    std::memcpy(I->forward, c.forward, sizeof I->forward);
+    std::memcpy(I->right, c.right, sizeof I->right);
+    std::memcpy(I->up, c.up, sizeof I->up);
+}
+
+void fill_temporal(const float pose[3], float fov_tan, uint32_t pw, uint32_t ph, MetricImage* I) {
+    static const float offs[5] = {-3.0f, -1.5f, 0.0f, 1.5f, 3.0f};  // scintillation.rs:19
+    for (int i = 0; i < 5; ++i) {
+        ixg_camera c;
+        ixg_camera_new(pose[0] + offs[i] * kToRad, pose[1], pose[2], 42.0f, pw, ph, 1, &c);
+        std::memcpy(I->t_origin[i], c.origin, sizeof I->t_origin[i]);
+        std::memcpy(I->t_forward[i], c.forward, sizeof I->t_forward[i]);
+        std::memcpy(I->t_right[i], c.right, sizeof I->t_right[i]);
+        std::memcpy(I->t_up[i], c.up, sizeof I->t_up[i]);
+    }
+    (void)fov_tan;
+}
+
+void fill_indices(const ixg_dispersion& d, MetricImage* I) {
+    I->nd = std::fmax(dispersion_host(d, 589.3f), 1.1f);
+    I->nf = std::fmax(dispersion_host(d, 486.1f), 1.001f);
+    I->nc = std::fmax(dispersion_host(d, 656.3f), 1.001f);
+    I->zero_dispersion = I->nf == I->nc;
+}
+
+bool planes_valid(const ixg_facet_plane* p, uint32_t n) {
+    for (uint32_t i = 0; i < n; ++i) {
+        float s = p[i].normal[0] * p[i].normal[0] + p[i].normal[1] * p[i].normal[1] +
+                  p[i].normal[2] * p[i].normal[2];
+        if (!std::isfinite(p[i].d) || !std::isfinite(s) || std::fabs(s - 1.0f) > 0.01f) return false;
+    }
+    return true;
+}
+
+// Runs pixels + reduction for n images already described in `images`.
+ixg_status run_images(ixg_ctx* ctx, const MetricShared& S, const std::vector<MetricImage>& images,
+                      const ixg_facet_plane* planes, uint32_t total_planes, ixg_optical_pixel* host_px,
+                      std::vector<ixg_metric_totals>& totals) {
+    hipStream_t st = ixg_internal_stream(ctx);
+    const size_t npx = (size_t)S.width * S.height;
+    std::vector<float4> pl(total_planes);
+    for (uint32_t i = 0; i < total_planes; ++i)
+        pl[i] = make_float4(planes[i].normal[0], planes[i].normal[1], planes[i].normal[2], planes[i].d);
+    float4* d_pl = nullptr;
+    MetricImage* d_img = nullptr;
+    ixg_optical_pixel* d_px = nullptr;
+    ixg_metric_totals* d_tot = nullptr;
+    hipError_t e = hipMalloc(&d_pl, std::max<size_t>(1, pl.size()) * sizeof(float4));
+    if (e == hipSuccess) e = hipMalloc(&d_img, images.size() * sizeof(MetricImage));
+    if (e == hipSuccess) e = hipMalloc(&d_px, images.size() * npx * sizeof(ixg_optical_pixel));
+    if (e == hipSuccess) e = hipMalloc(&d_tot, images.size() * sizeof(ixg_metric_totals));
+    if (e == hipSuccess) e = hipMemcpyAsync(d_pl, pl.data(), pl.size() * sizeof(float4), hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(d_img, images.data(), images.size() * sizeof(MetricImage), hipMemcpyHostToDevice, st);
+    // gridDim.y <= 65535 images per launch
+    for (size_t first = 0; e == hipSuccess && first < images.size(); first += 65535) {
+        MetricShared Sc = S;
+        Sc.n_images = (uint32_t)std::min<size_t>(65535, images.size() - first);
+        e = ixg_launch_metric_pixels(&Sc, d_img + first, d_pl, d_px + first * npx, st);
+        if (e == hipSuccess) e = ixg_launch_metric_reduce(&Sc, d_px + first * npx, d_tot + first, st);
+    }
+    totals.resize(images.size());
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(totals.data(), d_tot, images.size() * sizeof(ixg_metric_totals), hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess && host_px)
+        e = hipMemcpyAsync(host_px, d_px, images.size() * npx * sizeof(ixg_optical_pixel), hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (d_pl) (void)hipFree(d_pl);
+    if (d_img) (void)hipFree(d_img);
+    if (d_px) (void)hipFree(d_px);
+    if (d_tot) (void)hipFree(d_tot);
+    if (e != hipSuccess) return ixg_internal_hip_fail(ctx, e, "metric images");
+    return IXG_OK;
+}
+
+}  // namespace
+
+extern "C" ixg_status ixg_metric_image(ixg_ctx* ctx, const ixg_metric_request* r, ixg_optical_pixel* pixels,
+                                       ixg_metric_totals* totals) {
+    if (!ctx || !r || !r->planes || !r->material) return IXG_INVALID;
+    if (ixg_internal_lost(ctx)) return ixg_internal_fail(ctx, IXG_DEVICE_LOST, "context is lost");
+    const uint64_t count = (uint64_t)r->width * r->height;
+    if (count == 0 || count > 1048576ull || r->samples == 0 || r->samples > 64)
+        return ixg_internal_fail(ctx, IXG_RESOURCE_LIMIT, "pixels or samples out of range");
+    if (r->want_scintillation && !(r->has_pose && std::isfinite(r->pose[0]) && std::isfinite(r->pose[1]) &&
+                                   std::isfinite(r->pose[2]) && r->pose[2] > 0.0f))
+        return ixg_internal_fail(ctx, IXG_INVALID, "scintillation needs a camera pose");
+    if (r->n_planes == 0) return ixg_internal_fail(ctx, IXG_INVALID, "empty geometry");
+    if (r->n_planes > IXG_MAX_PLANES) return ixg_internal_fail(ctx, IXG_DECLINED, "too many planes");
+    if (!planes_valid(r->planes, r->n_planes)) return ixg_internal_fail(ctx, IXG_INVALID, "non-unit plane");
+    if (hipSetDevice(ixg_internal_device(ctx)) != hipSuccess) return IXG_DEVICE_LOST;
+    MetricShared S;
+    ixg_status stt = resolve_shared(ctx, r->environment, r->width, r->height, r->samples, &S);
+    if (stt != IXG_OK) return stt;
+    S.proj_w = r->projection[0] ? r->projection[0] : r->width;
+    S.proj_h = r->projection[1] ? r->projection[1] : r->height;
+    S.origin_x = r->has_tile_origin ? r->tile_origin[0] : 0;
+    S.origin_y = r->has_tile_origin ? r->tile_origin[1] : 0;
+    S.scale_x = r->has_tile_origin ? 1.0f : (float)S.proj_w / (float)r->width;
+    S.scale_y = r->has_tile_origin ? 1.0f : (float)S.proj_h / (float)r->height;
+    S.want_fire = r->want_fire ? 1 : 0;
+    S.want_scintillation = r->want_scintillation ? 1 : 0;
+    S.n_images = 1;
+    std::vector<MetricImage> images(1);
+    MetricImage& I = images[0];
+    std::memset(&I, 0, sizeof I);
+    fill_camera(r->camera, &I);
+    if (r->want_scintillation) fill_temporal(r->pose, r->camera.fov_tan, S.proj_w, S.proj_h, &I);
+    I.plane_offset = 0;
+    I.n_planes = r->n_planes;
+    fill_indices(r->material->dispersion, &I);
+    std::vector<ixg_metric_totals> tot;
+    stt = run_images(ctx, S, images, r->planes, r->n_planes, pixels, tot);
+    if (stt != IXG_OK) return stt;
+    if (totals) *totals = tot[0];
+    return IXG_OK;
+}
+
+extern "C" ixg_status ixg_metric_sweep(ixg_ctx* ctx, const ixg_facet_plane* planes, uint32_t total_planes,
+                                       const ixg_design* designs, uint32_t n_designs, const float* tilts_deg,
+                                       uint32_t n_tilts, uint32_t width, uint32_t height, uint32_t samples,
+                                       float distance, const ixg_environment* env, float* out_metrics) {
+    if (!ctx || !planes || !designs || !tilts_deg || !env || !out_metrics) return IXG_INVALID;
+    if (ixg_internal_lost(ctx)) return ixg_internal_fail(ctx, IXG_DEVICE_LOST, "context is lost");
+    const uint64_t count = (uint64_t)width * height;
+    if (count == 0 || count > 1048576ull || samples == 0 || samples > 64 || n_designs == 0 || n_tilts == 0)
+        return ixg_internal_fail(ctx, IXG_RESOURCE_LIMIT, "sweep shape out of range");
+    if ((uint64_t)n_designs * n_tilts * count * sizeof(ixg_optical_pixel) > (64ull << 30))
+        return ixg_internal_fail(ctx, IXG_RESOURCE_LIMIT, "sweep pixel buffer above 64 GiB");
+    for (uint32_t d = 0; d < n_designs; ++d) {
+        if (designs[d].n_planes == 0 || designs[d].n_planes > IXG_MAX_PLANES ||
+            (uint64_t)designs[d].plane_offset + designs[d].n_planes > total_planes)
+            return ixg_internal_fail(ctx, IXG_INVALID, "design plane range");
+    }
+    if (!planes_valid(planes, total_planes)) return ixg_internal_fail(ctx, IXG_INVALID, "non-unit plane");
+    if (hipSetDevice(ixg_internal_device(ctx)) != hipSuccess) return IXG_DEVICE_LOST;
+    MetricShared S;
+    ixg_status stt = resolve_shared(ctx, *env, width, height, samples, &S);
+    if (stt != IXG_OK) return stt;
+    S.proj_w = width;
+    S.proj_h = height;
+    S.scale_x = S.scale_y = 1.0f;
+    S.want_fire = 1;
+    S.want_scintillation = 1;
+    std::vector<MetricImage> images((size_t)n_designs * n_tilts);
+    // camera per tilt: yaw 0 or 180 deg by sign, pitch 90 - |tilt| (pose_batch.rs:118-131)
+    std::vector<MetricImage> per_tilt(n_tilts);
+    for (uint32_t t = 0; t < n_tilts; ++t) {
+        MetricImage& T = per_tilt[t];
+        std::memset(&T, 0, sizeof T);
+        float tilt = tilts_deg[t];
+        float yaw = tilt < 0.0f ? 180.0f * kToRad : 0.0f;
+        float pitch = (90.0f - std::fabs(tilt)) * kToRad;
+        ixg_camera c;
+        ixg_camera_new(yaw, pitch, distance, 42.0f, width, height, samples, &c);
+        fill_camera(c, &T);
+        float pose[3] = {yaw, pitch, distance};
+        fill_temporal(pose, c.fov_tan, width, height, &T);
+    }
+    for (uint32_t d = 0; d < n_designs; ++d) {
+        MetricImage base;
+        std::memset(&base, 0, sizeof base);
+        fill_indices(designs[d].dispersion, &base);
+        for (uint32_t t = 0; t < n_tilts; ++t) {
+            MetricImage I = per_tilt[t];
+            I.plane_offset = designs[d].plane_offset;
+            I.n_planes = designs[d].n_planes;
+            I.nd = base.nd;
+            I.nf = base.nf;
+            I.nc = base.nc;
+            I.zero_dispersion = base.zero_dispersion;
+            images[(size_t)d * n_tilts + t] = I;
+        }
+    }
+    std::vector<ixg_metric_totals> tot;
+    stt = run_images(ctx, S, images, planes, total_planes, nullptr, tot);
+    if (stt != IXG_OK) return stt;
+    for (size_t i = 0; i < tot.size(); ++i) {
+        out_metrics[5 * i + 0] = tot[i].brilliance_pct;
+        out_metrics[5 * i + 1] = tot[i].fire_index;
+        out_metrics[5 * i + 2] = tot[i].scintillation_pct;
+        out_metrics[5 * i + 3] = tot[i].windowing_pct;
+        out_metrics[5 * i + 4] = tot[i].extinction_pct;
+    }
+    return IXG_OK;
+}

@@ -61,4 +61,38 @@ enum EventSlot : int {
     EV_NUM = 8
 };
 
+// One metric image (a design under one camera pose).
+struct MetricImage {
+    float origin[3];
+    float fov_tan;
+    float forward[3];
+    float pad0;
+    float right[3];
+    float pad1;
+    float up[3];
+    float pad2;
+    float t_origin[5][3];  // temporal cameras (same fov_tan)
+    float t_forward[5][3];
+    float t_right[5][3];
+    float t_up[5][3];
+    uint32_t plane_offset;
+    uint32_t n_planes;
+    float nd, nf, nc;
+    int32_t zero_dispersion;
+};
+
+// What every image of one metric call shares.
+struct MetricShared {
+    EnvParams env;        // scoring rig: exposure 1, observer-free, default cosines
+    float spec550;        // illuminant power at the 550 nm probe wavelength
+    float threshold;      // ExitLighting::threshold
+    float head_cos;       // head-shadow cone cosine
+    uint32_t width, height, samples;
+    uint32_t proj_w, proj_h;
+    uint32_t origin_x, origin_y;
+    float scale_x, scale_y;
+    int32_t want_fire, want_scintillation;
+    uint32_t n_images;
+};
+
 }  // namespace ixg

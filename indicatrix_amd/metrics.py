@@ -1,0 +1,72 @@
+"""Host mirror of the reference's metric API (color/metrics/) over libixg.
+
+``evaluate_optical_image`` follows IX/color/metrics/image.rs:187 (request in,
+per-pixel classes + totals out); ``metric_sweep`` batches many designs and
+tilts into one device pass (the tilt sweeps of MET/sweep.rs, MET/pose_batch.rs).
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import abi
+from ._lib import IxgError, lib
+
+PIXEL_DTYPE = np.dtype([("hits", "<u4"), ("classes", "<u4", (4,)), ("fire_sum_deg", "<f4"),
+                        ("fire_qualified", "<u4"), ("fire_rejected", "<u4"),
+                        ("centre_entry_facet", "<i4"), ("centre_exit_facet", "<i4"),
+                        ("centre_bounces", "<u4"), ("centre_reason", "<u4"),
+                        ("mixed_facets", "<u4"), ("scint_returned", "u1"), ("scint_poses", "u1"),
+                        ("pad", "u1", (2,))])
+METRIC_FIELDS = ["brilliance_pct", "fire_index", "scintillation_pct", "windowing_pct",
+                 "extinction_pct"]
+
+
+def make_request(camera, width, height, samples, planes, material_abi, environment,
+                 want_fire=True, want_scintillation=False, pose=None):
+    r = abi.MetricRequest()
+    r.camera = camera
+    r.width, r.height, r.samples = width, height, samples
+    r.planes = C.cast(planes, C.POINTER(abi.FacetPlane))
+    r.n_planes = len(planes)
+    r.material = C.pointer(material_abi)
+    r.environment = environment
+    r.want_fire = int(want_fire)
+    r.want_scintillation = int(want_scintillation)
+    if pose is not None:
+        r.has_pose = 1
+        r.pose[0], r.pose[1], r.pose[2] = pose
+    r._keep = (planes, material_abi)
+    return r
+
+
+def evaluate_optical_image(backend, request, want_pixels=True):
+    n = request.width * request.height
+    px = (abi.OpticalPixel * n)() if want_pixels else None
+    tot = abi.MetricTotals()
+    st = lib().ixg_metric_image(backend.handle, C.byref(request),
+                                C.cast(px, C.POINTER(abi.OpticalPixel)) if want_pixels else None,
+                                C.byref(tot))
+    if st != abi.IXG_OK:
+        raise IxgError(st, backend.last_error())
+    pixels = np.frombuffer(bytes(px), PIXEL_DTYPE).copy() if want_pixels else None
+    return pixels, tot
+
+
+def metric_sweep(backend, planes, designs, tilts_deg, width, height, samples, distance,
+                 environment):
+    """planes: (P,4) float32 table; designs: list of (offset, count, abi.Dispersion).
+    Returns (n_designs, n_tilts, 5) float32 metrics in METRIC_FIELDS order."""
+    planes = np.ascontiguousarray(planes, np.float32).reshape(-1, 4)
+    pl = (abi.FacetPlane * len(planes)).from_buffer_copy(planes.tobytes())
+    ds = (abi.Design * len(designs))()
+    for i, (off, cnt, disp) in enumerate(designs):
+        ds[i] = abi.Design(off, cnt, disp)
+    tilts = np.ascontiguousarray(tilts_deg, np.float32)
+    out = np.zeros((len(designs), len(tilts), 5), np.float32)
+    st = lib().ixg_metric_sweep(backend.handle, pl, len(planes), ds, len(designs),
+                                tilts.ctypes.data_as(C.POINTER(C.c_float)), len(tilts), width,
+                                height, samples, distance, C.byref(environment),
+                                out.ctypes.data_as(C.POINTER(C.c_float)))
+    if st != abi.IXG_OK:
+        raise IxgError(st, backend.last_error())
+    return out

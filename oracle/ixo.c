@@ -1027,3 +1027,372 @@ void ixo_trace_paths(const ixg_scene* sc, uint32_t sample_offset, uint32_t spp, 
     }
     for (int t = 0; t < threads; ++t) pthread_join(th[t], NULL);
 }
+
+/* ================================================================== */
+/* Metrics: evaluate_optical_image (color/metrics/)                    */
+/* ================================================================== */
+enum { FATE_ENTRY_BLOCKED = 0, FATE_LEAKED = 1, FATE_UPWARD = 2, FATE_ABSORBED = 3 };
+enum { REASON_NONE = 0, REASON_SHADOW = 1, REASON_ILLUM = 2, REASON_SIDE = 3, REASON_NOHIT = 4,
+       REASON_BUDGET = 5, REASON_ENTRY = 6, REASON_INVALID = 7 };
+
+typedef struct {
+    const ixg_facet_plane* planes;
+    uint32_t n;
+    float nd, nf, nc;
+    env_t env;          /* scoring environment (observer-free, default head-shadow cosines) */
+    float spec550;
+    float threshold;
+    float head_cos;
+} mctx_t;
+
+typedef struct {
+    int fate;
+    v3 dir;
+    float transmittance, exit_cos;
+    int32_t facet;
+    uint32_t bounces;
+    /* detail */
+    int32_t exit_facet;
+    uint32_t d_bounces;
+    uint32_t reason;
+} fate_t;
+
+static float fresnel_T(float n1, float n2, float ci, float ct) { /* ray_trace.rs:77-89 */
+    float ds = fmaf(n2, ct, n1 * ci);
+    float dp = fmaf(n2, ci, n1 * ct);
+    if (fabsf(ds) < 1e-6f || fabsf(dp) < 1e-6f) return 0.0f;
+    float a = fmaf(n2, -ct, n1 * ci) / ds;
+    float b = fmaf(n2, -ci, n1 * ct) / dp;
+    float rs = a * a, rp = b * b;
+    return clampf(1.0f - midpointf(rs, rp), 0.0f, 1.0f);
+}
+
+static void trace_entry_m(const mctx_t* M, v3 point, v3 dir, v3 n, float cos_i, float index,
+                          fate_t* F) { /* ray_trace.rs:256-425, polished facets */
+    memset(F, 0, sizeof *F);
+    F->exit_facet = -1;
+    F->reason = REASON_BUDGET;
+    float sin2_t = (1.0f / (index * index)) * fmaf(cos_i, -cos_i, 1.0f);
+    if (sin2_t > 1.0f) { F->fate = FATE_ENTRY_BLOCKED; F->reason = REASON_ENTRY; return; }
+    float cos_t = sqrtf(1.0f - sin2_t);
+    float inv = 1.0f / index;
+    v3 cur = vnormalize(vadd(vmul(dir, inv), vmul(n, fmaf(inv, cos_i, -cos_t))));
+    float entry_T = fresnel_T(1.0f, index, cos_i, cos_t);
+    v3 hp = vadd(point, vmul(cur, 1e-4f));
+    float sin_crit = 1.0f / index;
+    int leaked = 0, upward = 0;
+    v3 e_dir = mk(0, 0, 0);
+    float e_T = 0, e_cos = 0;
+    int32_t e_facet = 0;
+    uint32_t e_b = 0;
+    float path_len = 0.0f;
+    for (uint32_t b = 0; b < 10; ++b) {
+        F->d_bounces = b;
+        float o[3] = {hp.x, hp.y, hp.z}, d[3] = {cur.x, cur.y, cur.z}, t;
+        int32_t f = ixo_intersect(M->planes, M->n, o, d, &t);
+        if (f < 0) { F->reason = REASON_NOHIT; break; }
+        path_len += t;
+        v3 np = vadd(hp, vmul(cur, t));
+        v3 n_out = vfrom(M->planes[f].normal);
+        float ct = clampf(vdot(cur, n_out), 0.0f, 1.0f);
+        float st = sqrtf(maxf(fmaf(ct, -ct, 1.0f), 0.0f));
+        if (st < sin_crit) {
+            float s2o = (index * index) * fmaf(ct, -ct, 1.0f);
+            if (s2o <= 1.0f) {
+                float co = sqrtf(1.0f - s2o);
+                v3 od = vnormalize(vadd(vmul(cur, index), vmul(n_out, fmaf(index, -ct, co))));
+                /* Escape::record (ray_trace.rs:182-208) */
+                F->exit_facet = f;
+                F->d_bounces = b;
+                F->reason = REASON_SIDE;
+                if (n_out.y < -0.05f) {
+                    F->reason = REASON_NONE;
+                    leaked = 1;
+                } else if (od.y > 0.05f) {
+                    F->reason = REASON_NONE;
+                    upward = 1;
+                    e_dir = od;
+                    e_T = fresnel_T(index, 1.0f, ct, co);
+                    e_cos = co;
+                    e_facet = f;
+                    e_b = b;
+                }
+                break;
+            }
+        }
+        cur = vnormalize(vsub(cur, vmul(n_out, 2.0f * ct)));
+        hp = vadd(np, vmul(cur, 1e-4f));
+        F->d_bounces = b + 1;
+    }
+    (void)path_len;
+    if (leaked) F->fate = FATE_LEAKED;
+    else if (upward) {
+        F->fate = FATE_UPWARD;
+        F->dir = e_dir;
+        F->transmittance = entry_T * e_T;
+        F->exit_cos = e_cos;
+        F->facet = e_facet;
+        F->bounces = e_b;
+    } else F->fate = FATE_ABSORBED;
+}
+
+static float scoring_radiance(const mctx_t* M, v3 dir) { /* lighting.rs:80-100 */
+    dir_light L = direction_lighting(&M->env, dir, mk(0, 0, 0));
+    return light_radiance(&L, M->spec550, 1.0f);
+}
+
+enum { CLS_NONE = -1, CLS_WINDOWED = 0, CLS_EXTINCT = 1, CLS_RETURNED = 2, CLS_OTHER = 3 };
+
+/* classify_ray (classify.rs:109-197); returns class or CLS_NONE on a miss. */
+static int classify_m(const mctx_t* M, v3 org, v3 dir, v3 observer, int want_fire, int* has_fire,
+                      float* angle, float* weight, int32_t* entry_facet, fate_t* detail) {
+    float o[3] = {org.x, org.y, org.z}, d[3] = {dir.x, dir.y, dir.z}, t;
+    int32_t f = ixo_intersect(M->planes, M->n, o, d, &t);
+    if (f < 0) return CLS_NONE;
+    *entry_facet = f;
+    v3 hp = vadd(org, vmul(dir, t));
+    v3 n = vfrom(M->planes[f].normal);
+    float cos_i = clampf(vdot(vneg(dir), n), 0.0f, 1.0f);
+    fate_t F;
+    trace_entry_m(M, hp, dir, n, cos_i, M->nd, &F);
+    *has_fire = 0;
+    if (detail) *detail = F;
+    switch (F.fate) {
+    case FATE_ENTRY_BLOCKED: return CLS_OTHER;
+    case FATE_LEAKED: return CLS_WINDOWED;
+    case FATE_ABSORBED: return CLS_EXTINCT;
+    default: break;
+    }
+    /* returned_or_reason (visibility.rs:55-67) */
+    if (vdot(F.dir, observer) > M->head_cos) {
+        if (detail) detail->reason = REASON_SHADOW;
+        return CLS_EXTINCT;
+    }
+    if (!(scoring_radiance(M, F.dir) > M->threshold)) {
+        if (detail) detail->reason = REASON_ILLUM;
+        return CLS_EXTINCT;
+    }
+    if (want_fire) { /* fire_sample (classify.rs:199-256) */
+        fate_t A, B;
+        trace_entry_m(M, hp, dir, n, cos_i, M->nf, &A);
+        trace_entry_m(M, hp, dir, n, cos_i, M->nc, &B);
+        if (A.fate == FATE_UPWARD && B.fate == FATE_UPWARD && A.facet == B.facet &&
+            A.bounces == B.bounces) {
+            float cs = clampf(vdot(A.dir, B.dir), -1.0f, 1.0f);
+            *angle = acosf(cs) * 57.2957795130823208767981548141051703f;
+            float energy = F.transmittance * A.transmittance * B.transmittance;
+            float radiance = maxf(A.exit_cos, 0.0f) * maxf(B.exit_cos, 0.0f);
+            *weight = energy * radiance;
+            *has_fire = 1;
+        }
+    }
+    return CLS_RETURNED;
+}
+
+static void offset_m(uint32_t s, float* jx, float* jy) { /* image.rs:21-36 */
+    if (s == 0) { *jx = 0.5f; *jy = 0.5f; return; }
+    double radical = (double)bit_reverse(s) / 4294967296.0;
+    uint32_t n = s;
+    double scale = 1.0 / 3.0, y = 0.0;
+    while (n > 0) {
+        y += (double)(n % 3) * scale;
+        n /= 3;
+        scale /= 3.0;
+    }
+    *jx = (float)radical;
+    *jy = (float)y;
+}
+
+static float head_shadow_cone_cos(int studio_rig, float deg) { /* visibility.rs:22-40 */
+    if (studio_rig || isnan(deg) || deg == 16.0f) return 0.96f;
+    if (deg <= 0.0f) return 2.0f;
+    float offset = acosf(0.96f) - 16.0f * (F32_PI / 180.0f);
+    return cosf(clampf(deg, 1.0f, 89.0f) * (F32_PI / 180.0f) + offset);
+}
+
+static int cmp_total(const void* a, const void* b) {
+    float x = *(const float*)a, y = *(const float*)b;
+    return (x > y) - (x < y);
+}
+
+
+static int mctx_build(const ixg_facet_plane* planes, uint32_t n, const ixg_dispersion* disp,
+                      const ixg_environment* env, mctx_t* M) { /* prepared.rs:64-94, lighting.rs:133-190 */
+    M->planes = planes;
+    M->n = n;
+    M->nd = maxf(ixo_dispersion_evaluate(disp, 589.3f), 1.1f);
+    M->nf = maxf(ixo_dispersion_evaluate(disp, 486.1f), 1.001f);
+    M->nc = maxf(ixo_dispersion_evaluate(disp, 656.3f), 1.001f);
+    /* the scoring rig: exposure 1, observer-free, default head-shadow cosines */
+    ixg_environment scoring = *env;
+    scoring.exposure = 1.0f;
+    scoring.head_shadow_deg = 16.0f;
+    if (!env_build(&scoring, &M->env)) return 0;
+    M->spec550 = spec_power(&M->env, 550.0f);
+    float s[96];
+    for (int i = 0; i < 96; ++i) { /* hemisphere_direction (lighting.rs:115-121) */
+        float y = ((float)i + 0.5f) / 96.0f;
+        float ring = sqrtf(maxf(fmaf(y, -y, 1.0f), 0.0f));
+        float a = (float)i * 2.3999632f;
+        float sa = sinf(a), ca = cosf(a);
+        s[i] = scoring_radiance(M, mk(ring * ca, y, ring * sa));
+    }
+    qsort(s, 96, sizeof(float), cmp_total);
+    int uniform = M->env.model != MODEL_STUDIO;
+    float t = s[48] * (uniform ? 0.5f : 4.0f);
+    M->threshold = maxf(t, 1.17549435e-38f);
+    M->head_cos = head_shadow_cone_cos(M->env.model == MODEL_STUDIO, env->head_shadow_deg);
+    return 1;
+}
+
+static void camera_ray_m(const ixg_camera* c, float sx, float sy, float pw, float ph, v3* o, v3* d) {
+    ixg_camera cam = *c;
+    cam.width = pw;
+    cam.height = ph;
+    float oo[3], dd[3];
+    ixo_generate_ray(&cam, sx, sy, 0.0f, 0.0f, oo, dd);
+    *o = vfrom(oo);
+    *d = vfrom(dd);
+}
+
+static float fractions2(const ixg_optical_pixel* p) {
+    uint32_t h = p->hits > 1 ? p->hits : 1;
+    return (float)p->classes[2] / (float)h;
+}
+
+int ixo_metric_image(const ixg_metric_request* r, ixg_optical_pixel* pixels, ixg_metric_totals* tot) {
+    if (r->samples == 0 || r->samples > 64 || (uint64_t)r->width * r->height == 0 ||
+        (uint64_t)r->width * r->height > 1048576ull)
+        return IXG_RESOURCE_LIMIT;
+    if (r->want_scintillation && !(r->has_pose && r->pose[2] > 0.0f)) return IXG_INVALID;
+    mctx_t M;
+    if (!mctx_build(r->planes, r->n_planes, &r->material->dispersion, &r->environment, &M))
+        return IXG_DECLINED;
+    uint32_t pw = r->projection[0] ? r->projection[0] : r->width;
+    uint32_t ph = r->projection[1] ? r->projection[1] : r->height;
+    uint32_t ox = r->has_tile_origin ? r->tile_origin[0] : 0, oy = r->has_tile_origin ? r->tile_origin[1] : 0;
+    float sc0 = r->has_tile_origin ? 1.0f : (float)pw / (float)r->width;
+    float sc1 = r->has_tile_origin ? 1.0f : (float)ph / (float)r->height;
+    int zero_disp = M.nf == M.nc;
+    ixg_camera tcams[5];
+    if (r->want_scintillation) { /* temporal_cameras (image_scint.rs:35-48) */
+        static const float offs[5] = {-3.0f, -1.5f, 0.0f, 1.5f, 3.0f};
+        for (int i = 0; i < 5; ++i) {
+            ixo_camera_new(r->pose[0] + offs[i] * (F32_PI / 180.0f), r->pose[1], r->pose[2], 42.0f,
+                           pw, ph, 1, &tcams[i]);
+            tcams[i].fov_tan = r->camera.fov_tan;
+        }
+    }
+    ixg_optical_pixel total;
+    memset(&total, 0, sizeof total);
+    for (uint32_t y = 0; y < r->height; ++y)
+        for (uint32_t x = 0; x < r->width; ++x) {
+            ixg_optical_pixel p;
+            memset(&p, 0, sizeof p);
+            p.centre_entry_facet = -1;
+            p.centre_exit_facet = -1;
+            int have_first = 0;
+            int32_t first = -1;
+            for (uint32_t s = 0; s < r->samples; ++s) { /* PixelSampler::sample_pixel */
+                float jx, jy;
+                offset_m(s, &jx, &jy);
+                v3 o, d;
+                camera_ray_m(&r->camera, fmaf((float)x + jx, sc0, (float)ox - 0.5f),
+                             fmaf((float)y + jy, sc1, (float)oy - 0.5f), (float)pw, (float)ph, &o, &d);
+                int hf;
+                float ang = 0, w = 0;
+                int32_t ef = -1;
+                fate_t det;
+                int cls = classify_m(&M, o, d, vneg(d), r->want_fire, &hf, &ang, &w, &ef, &det);
+                if (cls == CLS_NONE) continue;
+                p.hits++;
+                if (s == 0) {
+                    p.centre_entry_facet = ef;
+                    p.centre_exit_facet = det.exit_facet;
+                    p.centre_bounces = det.d_bounces;
+                    p.centre_reason = det.reason;
+                }
+                if (have_first) p.mixed_facets |= (uint32_t)(ef != first);
+                else { first = ef; have_first = 1; }
+                if (cls == CLS_RETURNED && r->want_fire) {
+                    if (hf) {
+                        if (isfinite(ang) && isfinite(w)) {
+                            p.fire_sum_deg = fmaf(zero_disp ? 0.0f : ang, w, p.fire_sum_deg);
+                            p.fire_qualified++;
+                        } else p.fire_rejected++;
+                    } else p.fire_rejected++;
+                }
+                p.classes[cls]++;
+            }
+            if (p.hits > 0 && r->want_scintillation) { /* TemporalSampler::apply */
+                int ret[5];
+                for (int i = 0; i < 5; ++i) {
+                    v3 o, d;
+                    camera_ray_m(&tcams[i], fmaf((float)x + 0.5f, sc0, (float)ox - 0.5f),
+                                 fmaf((float)y + 0.5f, sc1, (float)oy - 0.5f), (float)pw, (float)ph, &o, &d);
+                    int hf;
+                    float a, w;
+                    int32_t ef;
+                    int cls = classify_m(&M, o, d, vneg(d), 0, &hf, &a, &w, &ef, NULL);
+                    ret[i] = cls == CLS_NONE ? -1 : (cls == CLS_RETURNED);
+                }
+                if (ret[2] >= 0) {
+                    uint8_t n = (uint8_t)(ret[2] == 1);
+                    for (int i = 0; i < 5; ++i)
+                        if (i != 2 && ret[i] == 1) n++;
+                    p.scint_returned = n;
+                    p.scint_poses = 5;
+                }
+            }
+            total.hits += p.hits;
+            for (int c = 0; c < 4; ++c) total.classes[c] += p.classes[c];
+            total.fire_sum_deg += p.fire_sum_deg;
+            total.fire_qualified += p.fire_qualified;
+            total.fire_rejected += p.fire_rejected;
+            if (pixels) pixels[(size_t)y * r->width + x] = p;
+        }
+    if (tot) {
+        memset(tot, 0, sizeof *tot);
+        tot->total = total;
+        uint32_t h = total.hits > 1 ? total.hits : 1;
+        tot->windowing_pct = (float)total.classes[0] / (float)h * 100.0f;
+        tot->extinction_pct = (float)total.classes[1] / (float)h * 100.0f;
+        tot->brilliance_pct = (float)total.classes[2] / (float)h * 100.0f;
+        tot->fire_index = maxf(total.fire_sum_deg / (float)h * 275.0f, 0.1f);
+        /* image_scintillation_pct (image_scint.rs:261-287) needs every pixel */
+        tot->scintillation_pct = NAN;
+    }
+    return IXG_OK;
+}
+
+/* image_scintillation_pct over a finished pixel array (image_scint.rs:261-287). */
+float ixo_image_scintillation_pct(const ixg_optical_pixel* px, uint32_t n) {
+    float sum = 0.0f, sum_sq = 0.0f;
+    uint32_t count = 0;
+    for (uint32_t i = 0; i < n; ++i)
+        if (px[i].hits > 0) {
+            float f = fractions2(&px[i]);
+            sum += f;
+            sum_sq = fmaf(f, f, sum_sq);
+            count++;
+        }
+    float var_sum = 0.0f;
+    uint32_t poses = 0;
+    for (uint32_t i = 0; i < n; ++i)
+        if (px[i].scint_poses) {
+            float fr = (float)px[i].scint_returned / (float)px[i].scint_poses;
+            var_sum += fmaf(fr, -fr, fr);
+            poses++;
+        }
+    float spatial = 0.0f;
+    if (count) { /* spatial_scintillation_pct (scintillation.rs:146-171) */
+        float mean = sum / (float)count;
+        if (mean > 1e-4f) {
+            float var = maxf(fmaf(mean, -mean, sum_sq / (float)count), 0.0f);
+            float cv = sqrtf(var) / mean;
+            spatial = clampf(cv / (1.0f + cv) * 100.0f, 0.0f, 100.0f);
+        }
+    }
+    float temporal = poses ? clampf(((var_sum / (float)poses) / 0.24f) * 100.0f, 0.0f, 100.0f) : 0.0f;
+    return clampf(fmaf(0.4f, temporal, 0.6f * spatial), 0.0f, 100.0f); /* scintillation.rs:194 */
+}

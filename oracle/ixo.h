@@ -68,6 +68,12 @@ void ixo_trace_paths(const ixg_scene* scene, uint32_t sample_offset, uint32_t sp
                      uint64_t first, uint64_t count, float* out_xyz, ixg_path_events* events,
                      int threads);
 
+/* evaluate_optical_image (IX/color/metrics/image.rs:187); totals->scintillation_pct
+ * is left NaN (use ixo_image_scintillation_pct over the pixels). */
+int ixo_metric_image(const ixg_metric_request* req, ixg_optical_pixel* pixels,
+                     ixg_metric_totals* totals);
+float ixo_image_scintillation_pct(const ixg_optical_pixel* px, uint32_t n);
+
 #ifdef __cplusplus
 }
 #endif

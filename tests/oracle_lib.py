@@ -58,13 +58,32 @@ def lib():
                                     P(C.c_float)]
         L.ixo_trace_ray.argtypes = [P(abi.Scene), f3, f3, C.c_uint32, C.c_float, P(C.c_float),
                                     P(abi.PathEvents)]
+        L.ixo_generate_ray.argtypes = [P(abi.Camera), C.c_float, C.c_float, C.c_float, C.c_float,
+                                       P(C.c_float), P(C.c_float)]
         L.ixo_trace_sample.argtypes = [P(abi.Scene), C.c_uint32, C.c_uint32, P(C.c_float),
                                        P(abi.PathEvents)]
         L.ixo_accumulate.argtypes = [P(abi.Scene), C.c_uint32, C.c_uint32, P(C.c_double), C.c_int]
         L.ixo_trace_paths.argtypes = [P(abi.Scene), C.c_uint32, C.c_uint32, C.c_uint64,
                                       C.c_uint64, P(C.c_float), P(abi.PathEvents), C.c_int]
+        L.ixo_metric_image.restype = C.c_int
+        L.ixo_metric_image.argtypes = [P(abi.MetricRequest), P(abi.OpticalPixel), P(abi.MetricTotals)]
+        L.ixo_image_scintillation_pct.restype = C.c_float
+        L.ixo_image_scintillation_pct.argtypes = [P(abi.OpticalPixel), C.c_uint32]
         _lib = L
     return _lib
+
+
+def metric_image(request):
+    """The oracle's evaluate_optical_image -> (pixels ndarray, totals, scintillation %)."""
+    from indicatrix_amd.metrics import PIXEL_DTYPE
+    n = request.width * request.height
+    px = (abi.OpticalPixel * n)()
+    tot = abi.MetricTotals()
+    st = lib().ixo_metric_image(C.byref(request), px, C.byref(tot))
+    if st != 0:
+        raise RuntimeError(f"oracle metric_image status {st}")
+    scint = lib().ixo_image_scintillation_pct(px, n)
+    return np.frombuffer(bytes(px), PIXEL_DTYPE).copy(), tot, scint
 
 
 def threads():

@@ -1,0 +1,52 @@
+"""Metric images on the GPU against the oracle's evaluate_optical_image.
+
+Integer outputs (per-pixel hits, class counts, centre facets/bounces/reason,
+temporal pose counts) must agree exactly on every pixel; the Fire sums agree
+to the device acos (a few ULP); totals and the five image metrics follow.
+"""
+import numpy as np
+import pytest
+
+import oracle_lib
+from indicatrix_amd import metrics, tracer
+
+pytestmark = pytest.mark.gpu
+
+
+def request(preset="IsoHemisphere", w=48, h=40, samples=8, pose=(0.3, 0.9, 2.4), scint=True,
+            material=None, planes=None):
+    planes = planes if planes is not None else tracer.standard_round_brilliant()
+    cam = tracer.camera_new(pose[0], pose[1], pose[2], 42.0, w, h, samples)
+    env = tracer.LightingPreset.studio(preset, 1.0, 0.85, 0.95)
+    mat = (material or tracer.GemMaterial.diamond()).to_abi()
+    return metrics.make_request(cam, w, h, samples, planes, mat, env, want_fire=True,
+                                want_scintillation=scint, pose=pose)
+
+
+@pytest.mark.parametrize("preset", ["IsoHemisphere", "RingLights"])
+def test_metric_image_matches_oracle(backend, preset):
+    r = request(preset)
+    print(f"metric image {preset} 48x40x8", flush=True)
+    g_px, g_tot = metrics.evaluate_optical_image(backend, r)
+    o_px, o_tot, o_scint = oracle_lib.metric_image(r)
+    for f in ("hits", "classes", "fire_qualified", "fire_rejected", "centre_entry_facet",
+              "centre_exit_facet", "centre_bounces", "centre_reason", "mixed_facets",
+              "scint_returned", "scint_poses"):
+        assert np.array_equal(g_px[f], o_px[f]), f
+    assert g_px["hits"].sum() > 0 and g_px["classes"][:, 2].sum() > 0
+    assert np.allclose(g_px["fire_sum_deg"], o_px["fire_sum_deg"], rtol=1e-5, atol=1e-6)
+    assert g_tot.total.hits == o_tot.total.hits
+    assert list(g_tot.total.classes) == list(o_tot.total.classes)
+    assert g_tot.brilliance_pct == o_tot.brilliance_pct
+    assert abs(g_tot.fire_index - o_tot.fire_index) <= 1e-4 * max(1.0, o_tot.fire_index)
+    assert abs(g_tot.scintillation_pct - o_scint) < 1e-4
+
+
+def test_metric_image_request_validation(backend):
+    r = request(samples=65)
+    with pytest.raises(tracer.IxgError):
+        metrics.evaluate_optical_image(backend, r)
+    r = request(scint=True, pose=(0.3, 0.9, 2.4))
+    r.has_pose = 0
+    with pytest.raises(tracer.IxgError):
+        metrics.evaluate_optical_image(backend, r)

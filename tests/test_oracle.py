@@ -232,3 +232,89 @@ def test_interleaved_partition_is_thread_count_invariant():
     b = oracle_lib.accumulate(W(), 0, 4, nthreads=3)
     assert np.array_equal(a, b)
     assert a.sum() > 0
+
+
+# --------------------------------------------------------------------------
+# metric images: structural properties the reference's own tests pin
+# (color/metrics/image_tests.rs, tests/raytracer_tests/optical_metrics.rs)
+# --------------------------------------------------------------------------
+def _metric_request(material=None, w=24, h=20, samples=6, want_fire=True, scint=False,
+                    tile=None, proj=None, preset="IsoHemisphere"):
+    from indicatrix_amd import metrics, tracer
+    planes = (abi.FacetPlane * 80)()
+    n = oracle_lib.lib().ixo_cut_standard_round_brilliant(planes)
+    planes = (abi.FacetPlane * n)(*planes[:n])
+    cam = abi.Camera()
+    pw, ph = proj or (w, h)
+    oracle_lib.lib().ixo_camera_new(0.3, 0.9, 2.4, 42.0, pw, ph, samples, C.byref(cam))
+    env = abi.Environment(abi.PRESETS[preset], 1.0, 0.85, 0.95, 0.0, 1.0, 16.0)
+    mat = material or diamond_material()
+    r = metrics.make_request(cam, w, h, samples, planes, mat, env, want_fire=want_fire,
+                             want_scintillation=scint, pose=(0.3, 0.9, 2.4))
+    if proj:
+        r.projection[0], r.projection[1] = proj
+    if tile:
+        r.has_tile_origin = 1
+        r.tile_origin[0], r.tile_origin[1] = tile
+    return r
+
+
+def test_metric_counts_conserve_hits_and_repeat():
+    """image_tests.rs:10-37."""
+    r = _metric_request()
+    a, ta, _ = oracle_lib.metric_image(r)
+    b, tb, _ = oracle_lib.metric_image(r)
+    assert np.array_equal(a["classes"].sum(axis=1), a["hits"])
+    assert np.array_equal(a.tobytes(), b.tobytes())
+    assert np.all(a["hits"] <= r.samples)
+    assert np.any(a["hits"] == 0) and ta.total.hits > 0
+
+
+def test_zero_dispersion_has_no_fire_and_fire_preserves_counts():
+    """image_tests.rs:70-99: a flat index gives no fire; fire on/off keeps class counts."""
+    flat = diamond_material()
+    flat.dispersion = abi.Dispersion(abi.DISP_CAUCHY, (C.c_float * 6)(2.4, 0.0, 0.0, 0, 0, 0))
+    px, tot, _ = oracle_lib.metric_image(_metric_request(flat))
+    assert tot.total.classes[2] > 0 and np.all(px["fire_sum_deg"] == 0.0)
+    base, tb, _ = oracle_lib.metric_image(_metric_request(want_fire=False))
+    fire, tf, _ = oracle_lib.metric_image(_metric_request(want_fire=True))
+    assert list(tb.total.classes) == list(tf.total.classes)
+    assert tf.total.fire_sum_deg > 0.0
+
+
+def test_row_tiles_match_full_projection():
+    """image_tests.rs:102-137: a tile of the projection classifies exactly like those rows
+    of the full map (screen coordinates are rounded once per path, so Fire sums agree to
+    rounding)."""
+    full, _, _ = oracle_lib.metric_image(_metric_request(w=24, h=20))
+    tile, _, _ = oracle_lib.metric_image(_metric_request(w=24, h=5, tile=(0, 8), proj=(24, 20)))
+    rows = full.reshape(20, 24)[8:13].reshape(-1)
+    for f in ("hits", "classes", "centre_entry_facet", "centre_exit_facet", "centre_bounces"):
+        assert np.array_equal(tile[f], rows[f]), f
+    assert np.allclose(tile["fire_sum_deg"], rows["fire_sum_deg"], rtol=1e-4, atol=1e-6)
+
+
+def test_centre_sample_hits_the_camera_ray_facet():
+    """image_tests.rs:227-262: sample 0 is the undisturbed pixel-centre camera ray."""
+    r = _metric_request(w=16, h=12)
+    px, _, _ = oracle_lib.metric_image(r)
+    L = oracle_lib.lib()
+    o, d, t = (C.c_float * 3)(), (C.c_float * 3)(), C.c_float()
+    for y in range(12):
+        for x in range(16):
+            cam = r.camera
+            cam.width, cam.height = 16.0, 12.0
+            L.ixo_generate_ray(C.byref(cam), float(x), float(y), 0.0, 0.0, o, d)
+            f = L.ixo_intersect(r.planes, r.n_planes, o, d, C.byref(t))
+            assert px["centre_entry_facet"][y * 16 + x] == (f if f >= 0 else -1)
+
+
+def test_high_dispersion_has_more_fire():
+    """tests/raytracer_tests/optical_metrics.rs / metrics_tests.rs:139: fire orders by dispersion."""
+    lo = diamond_material()
+    lo.dispersion = abi.Dispersion(abi.DISP_CAUCHY, (C.c_float * 6)(2.40, 0.004, 0.0, 0, 0, 0))
+    hi = diamond_material()
+    hi.dispersion = abi.Dispersion(abi.DISP_CAUCHY, (C.c_float * 6)(2.40, 0.030, 0.0, 0, 0, 0))
+    _, tl, _ = oracle_lib.metric_image(_metric_request(lo, samples=8))
+    _, th, _ = oracle_lib.metric_image(_metric_request(hi, samples=8))
+    assert th.fire_index > 2.0 * tl.fire_index
