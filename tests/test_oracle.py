@@ -291,7 +291,7 @@ def test_row_tiles_match_full_projection():
     rows = full.reshape(20, 24)[8:13].reshape(-1)
     for f in ("hits", "classes", "centre_entry_facet", "centre_exit_facet", "centre_bounces"):
         assert np.array_equal(tile[f], rows[f]), f
-    assert np.allclose(tile["fire_sum_deg"], rows["fire_sum_deg"], rtol=1e-4, atol=1e-6)
+    assert np.allclose(tile["fire_sum_deg"], rows["fire_sum_deg"], rtol=2e-3, atol=1e-5)
 
 
 def test_centre_sample_hits_the_camera_ray_facet():
