@@ -22,6 +22,9 @@
 
 extern "C" hipError_t ixg_launch_film(const ixg::FilmParams* P, const float4* d_planes, float* d_xyz,
                                       hipStream_t stream);
+extern "C" hipError_t ixg_launch_film_persistent(const ixg::FilmParams* P, const float4* d_planes, float* d_xyz,
+                                                 uint32_t* d_counter, uint32_t grid_blocks, hipStream_t stream);
+extern "C" hipError_t ixg_persistent_grid(int device, uint32_t* blocks);
 extern "C" hipError_t ixg_launch_film_events(const ixg::FilmParams* P, const float4* d_planes,
                                              float* d_xyz, ixg_path_events* d_ev,
                                              unsigned long long* d_totals, hipStream_t stream);
@@ -45,6 +48,8 @@ struct ixg_ctx {
     size_t xyz_cap = 0;  // floats
     double* d_accum = nullptr;
     size_t accum_cap = 0;  // doubles
+    uint32_t* d_counter = nullptr;  // persistent kernel's path counter
+    uint32_t persistent_blocks = 0;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     double last_kernel_ms_total = 0.0;
     uint32_t last_kernel_launches = 0;
@@ -185,7 +190,8 @@ ixg_status run_film(ixg_ctx* ctx, const ixg_scene* s, uint32_t sample_offset, ui
         P.first_path = first * spp;
         P.n_paths = (uint64_t)npx * spp;
         if (timed) HIP_TRY(ctx, hipEventRecord(ctx->ev0, ctx->stream));
-        HIP_TRY(ctx, ixg_launch_film(&P, ctx->d_planes, ctx->d_xyz, ctx->stream));
+        HIP_TRY(ctx, ixg_launch_film_persistent(&P, ctx->d_planes, ctx->d_xyz, ctx->d_counter,
+                                                ctx->persistent_blocks, ctx->stream));
         if (timed) {
             HIP_TRY(ctx, hipEventRecord(ctx->ev1, ctx->stream));
             HIP_TRY(ctx, hipEventSynchronize(ctx->ev1));
@@ -215,6 +221,8 @@ ixg_status ixg_create(const ixg_opts* opts, ixg_ctx** out) {
     if (opts && opts->chunk_paths) ctx->chunk_paths = opts->chunk_paths;
     if (hipSetDevice(dev) != hipSuccess || hipStreamCreate(&ctx->stream) != hipSuccess ||
         hipMalloc(&ctx->d_planes, IXG_MAX_PLANES * sizeof(float4)) != hipSuccess ||
+        hipMalloc(&ctx->d_counter, sizeof(uint32_t)) != hipSuccess ||
+        ixg_persistent_grid(dev, &ctx->persistent_blocks) != hipSuccess ||
         hipEventCreate(&ctx->ev0) != hipSuccess || hipEventCreate(&ctx->ev1) != hipSuccess) {
         ixg_destroy(ctx);
         return IXG_DEVICE_LOST;
@@ -232,6 +240,7 @@ void ixg_destroy(ixg_ctx* ctx) {
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     if (ctx->d_planes) (void)hipFree(ctx->d_planes);
     if (ctx->d_xyz) (void)hipFree(ctx->d_xyz);
+    if (ctx->d_counter) (void)hipFree(ctx->d_counter);
     if (ctx->d_accum) (void)hipFree(ctx->d_accum);
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
     if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);

@@ -63,7 +63,7 @@ __device__ __forceinline__ void fresnel_rs_rp(float n1, float n2, float cos_i, f
 __device__ __forceinline__ uint32_t fnv(uint32_t h, uint32_t v) { return (h ^ v) * 16777619u; }
 
 // ---------------------------------------------------------------------------
-// one path
+// one path, as init / bounce / finish steps
 // ---------------------------------------------------------------------------
 struct PathOut {
     float xyz[3];
@@ -91,297 +91,323 @@ __device__ __forceinline__ void camera_sample(const FilmParams& P, uint32_t pixe
     org = v3(P.cam_origin[0], P.cam_origin[1], P.cam_origin[2]);
 }
 
-// trace_spectral_ray (transport/mod.rs:118) for one explicit ray.
-template <bool EVENTS>
-__device__ __forceinline__ void trace_path(const FilmParams& P, const float4* __restrict__ planes,
-                                           V3 org, V3 dir, const uint32_t seed, const float hero,
-                                           PathOut& out, uint32_t* counts) {
-    const V3 observer = neg(dir);
+// Every per-path local of the reference's bounce loop (bounce_body.rs:147-205)
+// that survives from one bounce to the next.  For an isotropic medium the wave
+// normal k equals the ray direction, so one vector carries both.
+struct PathState {
+    float4 st[NCH];           // Stokes per channel
+    float path_pdf[NCH];
+    float split[NCH];         // exit-split staging radiance
+    float lambda[NCH];
+    float n_o[NCH];
+    uint64_t compat;          // byte c = MIS family of channel c
+    V3 org, dir, observer, prev_plane;
+    uint32_t seed;
+    uint32_t bounce;
+    int32_t last_facet;       // facet of the previous event (-1 before the first)
+    bool inside, has_prev;
+    // integer side channel
+    uint32_t fh, n_tir, n_refl, n_refr, n_probe, n_rot;
+};
 
-    // PathState::new (bounce_body.rs:258-356)
-    float lambda[NCH], n_o[NCH], path_pdf[NCH], radiance[NCH], split[NCH];
-    float4 st[NCH];
-    uint32_t compat[NCH];
+__device__ __forceinline__ uint32_t compat_of(uint64_t c, int ch) { return (uint32_t)(c >> (8 * ch)) & 0xFFu; }
+
+// PathState::new (bounce_body.rs:258-356).
+__device__ __forceinline__ void path_init(const FilmParams& P, V3 org, V3 dir, uint32_t seed,
+                                          float hero, PathState& S) {
+    S.org = org;
+    S.dir = dir;
+    S.observer = neg(dir);
+    S.prev_plane = v3(0.0f, 0.0f, 0.0f);
+    S.seed = seed;
+    S.bounce = 0;
+    S.last_facet = -1;
+    S.inside = false;
+    S.has_prev = false;
+    S.compat = ~0ull;
     const float lambda_hero = fma_(hero, 400.0f, 380.0f);
 #pragma unroll
     for (int c = 0; c < NCH; ++c) {
+        // wrapped_hero_wavelengths (transport/mod.rs:58-66)
         float off = fma_((float)c, 50.0f, lambda_hero - 380.0f);
         float r = fmodf(off, 400.0f);
         r = r < 0.0f ? r + 400.0f : r;
-        lambda[c] = 380.0f + r;
-        n_o[c] = dispersion(P.disp_kind, P.disp, lambda[c]);
-        path_pdf[c] = 1.0f;
-        radiance[c] = 0.0f;
-        split[c] = 0.0f;
-        st[c] = make_float4(1.0f, 0.0f, 0.0f, 0.0f);
-        compat[c] = 0xFFu;
+        S.lambda[c] = 380.0f + r;
+        S.n_o[c] = dispersion(P.disp_kind, P.disp, S.lambda[c]);
+        S.path_pdf[c] = 1.0f;
+        S.split[c] = 0.0f;
+        S.st[c] = make_float4(1.0f, 0.0f, 0.0f, 0.0f);
     }
-    V3 k = dir;
-    bool inside = false, has_prev = false, escaped = false;
-    V3 prev_plane = v3(0.0f, 0.0f, 0.0f);
+    S.fh = 2166136261u;
+    S.n_tir = S.n_refl = S.n_refr = S.n_probe = S.n_rot = 0;
+}
 
-    uint32_t bounces = P.max_bounces, term = 3u;
-    uint32_t fh = 2166136261u;
-    uint32_t n_tir = 0, n_refl = 0, n_refr = 0, n_probe = 0, n_rot = 0;
+__device__ __forceinline__ float env_radiance(const EnvParams& E, V3 dir, V3 observer, float lambda) {
+    if (E.model == ENV_ISO_HEMISPHERE)
+        return iso_factor(E, dir, observer) * (spec_power(E, lambda) * E.exposure);
+    DirLight L;
+    direction_lighting(E, dir, observer, L);
+    return light_radiance(E, L, spec_power(E, lambda));
+}
 
-    for (uint32_t bounce = 0; bounce < P.max_bounces; ++bounce) {
-        float t_hit = 0.0f;
-        int facet = intersect(planes, P.n_planes, org, dir, t_hit);
-        if (facet < 0) {
-            // handle_miss (bounce_body/miss.rs:33-54)
-            if (P.env.model == ENV_ISO_HEMISPHERE) {
-                float f = iso_factor(P.env, dir, observer);
-#pragma unroll
-                for (int c = 0; c < NCH; ++c) {
-                    float env = f * (spec_power(P.env, lambda[c]) * P.env.exposure);
-                    radiance[c] = fma_(fmaxf(st[c].x, 0.0f) * 1.0f, env, radiance[c]);
-                }
-            } else {
-                DirLight L;
-                direction_lighting(P.env, dir, observer, L);
-#pragma unroll
-                for (int c = 0; c < NCH; ++c) {
-                    float env = light_radiance(P.env, L, spec_power(P.env, lambda[c]));
-                    radiance[c] = fma_(fmaxf(st[c].x, 0.0f) * 1.0f, env, radiance[c]);
-                }
-            }
-            escaped = true;
-            bounces = bounce + 1;
-            term = 0u;
-            break;
-        }
+// A ray leaving facet f of a convex polyhedron outward cannot meet it again:
+// if its origin is already strictly outside f's half-space and it moves away
+// from f, the exit plane gives t_far < 0 and the slab scan returns no hit --
+// exactly what scanning every plane would have returned.
+__device__ __forceinline__ bool leaves_convex(const float4* __restrict__ planes, int f, V3 o, V3 d) {
+    float4 p = planes[f];
+    float denom = (p.x * d.x + p.y * d.y) + p.z * d.z;
+    float side = p.w + ((p.x * o.x + p.y * o.y) + p.z * o.z);
+    return side > 0.0f && denom > 1e-7f;
+}
 
-        // facet_event (bounce_body.rs:611-707)
-        const V3 hit = org + dir * t_hit;
-        const float4 pf = planes[facet];
-        V3 normal = v3(pf.x, pf.y, pf.z);
-        // rotate_stokes_to_plane_of_incidence (absorption.rs:69-87)
-        V3 cur_plane = normalize_or_zero(cross(k, normal));
-        if (has_prev && dot(cur_plane, cur_plane) > 1e-6f && dot(prev_plane, prev_plane) > 1e-6f) {
-            float cos_psi = clampf(dot(prev_plane, cur_plane), -1.0f, 1.0f);
-            float sin_psi = dot(cross(prev_plane, cur_plane), normalize_or_zero(k));
-            float psi = atan2f(sin_psi, cos_psi);
-            float c2 = cosf(2.0f * psi), s2 = sinf(2.0f * psi);
+enum StepResult { STEP_CONTINUE = 0, STEP_ESCAPED = 1, STEP_ROULETTE = 2 };
+
+// One iteration of the bounce loop (bounce_body.rs:361-707): intersection, and
+// either the escape lookup (written to `radiance`) or the facet event.
+template <bool EVENTS, bool ISO = false>
+__device__ __forceinline__ int path_bounce(const FilmParams& P, const float4* __restrict__ planes,
+                                           PathState& S, float radiance[NCH]) {
+    const uint32_t bounce = S.bounce;
+    float t_hit = 0.0f;
+    int facet;
+    if (!S.inside && S.last_facet >= 0 && leaves_convex(planes, S.last_facet, S.org, S.dir))
+        facet = -1;
+    else
+        facet = intersect(planes, P.n_planes, S.org, S.dir, t_hit);
+    if (facet < 0) {
+        // handle_miss (bounce_body/miss.rs:33-54)
+        if (ISO || P.env.model == ENV_ISO_HEMISPHERE) {
+            const float f = iso_factor(P.env, S.dir, S.observer);
 #pragma unroll
             for (int c = 0; c < NCH; ++c) {
-                float4 s = st[c];
-                st[c] = make_float4(s.x, c2 * s.y + s2 * s.z, (-s2) * s.y + c2 * s.z, s.w);
+                float env = f * (spec_power(P.env, S.lambda[c]) * P.env.exposure);
+                radiance[c] = fma_(fmaxf(S.st[c].x, 0.0f) * 1.0f, env, 0.0f);
             }
-            if (EVENTS) ++n_rot;
-        }
-        prev_plane = cur_plane;
-        has_prev = true;
-        // apply_interior_segment: absorption-free isotropic media transmit
-        // exp(-0) == 1 exactly, so only the normal flips.
-        if (inside) normal = neg(normal);
-
-        // compute_bounce_refraction_geometry (geometry.rs:306-425), isotropic
-        const float cos_i = clampf(dot(neg(k), normal), 0.0f, 1.0f);
-        const float sin_i = sqrtf(fmaxf(fma_(cos_i, -cos_i, 1.0f), 0.0f));
-        const float one_m_c2 = fma_(cos_i, -cos_i, 1.0f);
-        const float n1 = inside ? n_o[0] : 1.0f;
-        const float n2 = inside ? 1.0f : n_o[0];
-        const float eta = n1 / n2;
-        const float sin2_t = eta * eta * one_m_c2;
-
-        V3 new_k;
-        bool new_inside = inside;
-        if (sin2_t > 1.0f) {
-            // apply_tir_bounce (tir.rs:56-160), scalar branch
-            if (EVENTS) {
-                ++n_tir;
-                fh = fnv(fnv(fh, (uint32_t)facet), 1u);
-            }
-#pragma unroll
-            for (int c = 0; c < NCH; ++c) {
-                float n1k = inside ? n_o[c] : 1.0f;
-                float n2k = inside ? 1.0f : n_o[c];
-                float ek = n1k / n2k;
-                float s2k = ek * ek * one_m_c2;
-                if (s2k > 1.0f) {
-                    float delta = tir_phase_delta(n1k, cos_i, sin_i);
-                    st[c] = mueller_tir(cosf(delta), sinf(delta), st[c]);
-                } else {
-                    float ct = sqrtf(fmaxf(1.0f - s2k, 0.0f));
-                    float rs, rp, a, b, cc;
-                    fresnel_rs_rp(n1k, n2k, cos_i, ct, rs, rp);
-                    fresnel_reflection_coeffs(rs, rp, a, b, cc);
-                    st[c] = mueller_fresnel(a, b, cc, st[c]);
-                    path_pdf[c] *= clampf(0.5f * fma_(rp, rp, rs * rs), R_UNPOL_PDF_MIN, R_UNPOL_PDF_MAX);
-                }
-            }
-            new_k = k - normal * (2.0f * dot(k, normal));
         } else {
-            // apply_partial_fresnel_bounce (dispatch.rs:307-380)
-            const float cos_t = sqrtf(fmaxf(1.0f - sin2_t, 0.0f));
-            float rs, rp;
-            fresnel_rs_rp(n1, n2, cos_i, cos_t, rs, rp);
-            const float r_unpol = clampf(0.5f * fma_(rp, rp, rs * rs), R_UNPOL_SELECT_MIN, R_UNPOL_SELECT_MAX);
-            const float rng = unit_draw(seed, bounce, FRESNEL_BRANCH_STREAM);
-            if (rng < r_unpol) {
-                // apply_partial_reflect_bounce (reflect_refract.rs:33-71)
-                if (EVENTS) {
-                    ++n_refl;
-                    fh = fnv(fnv(fh, (uint32_t)facet), 2u);
-                }
-                const float inv_r = 1.0f / r_unpol;
+            DirLight L;
+            direction_lighting(P.env, S.dir, S.observer, L);
 #pragma unroll
-                for (int c = 0; c < NCH; ++c) {
-                    float n1k = inside ? n_o[c] : 1.0f;
-                    float n2k = inside ? 1.0f : n_o[c];
-                    float ek = n1k / n2k;
-                    float s2k = ek * ek * one_m_c2;
-                    float4 s;
-                    if (s2k > 1.0f) {
-                        float delta = tir_phase_delta(n1k, cos_i, sin_i);
-                        s = mueller_tir(cosf(delta), sinf(delta), st[c]);
-                    } else {
-                        float ct = sqrtf(fmaxf(1.0f - s2k, 0.0f));
-                        float a_, b_, a, b, cc;
-                        fresnel_rs_rp(n1k, n2k, cos_i, ct, a_, b_);
-                        path_pdf[c] *= clampf(0.5f * fma_(b_, b_, a_ * a_), R_UNPOL_PDF_MIN, R_UNPOL_PDF_MAX);
-                        fresnel_reflection_coeffs(a_, b_, a, b, cc);
-                        s = mueller_fresnel(a, b, cc, st[c]);
-                    }
-                    st[c] = scale4(s, inv_r);
-                }
-                new_k = k - normal * (2.0f * dot(k, normal));
-            } else {
-                // apply_refract_bounce (reflect_refract.rs:121-234), isotropic
-                if (EVENTS) {
-                    ++n_refr;
-                    fh = fnv(fnv(fh, (uint32_t)facet), 3u);
-                }
-                const float s2d = fminf(eta * eta * one_m_c2, 1.0f);
-                const float ct_dir = sqrtf(fmaxf(1.0f - s2d, 0.0f));
-                const V3 refr = normalize(k * eta + normal * fma_(eta, cos_i, -ct_dir));
-                const float inv_t = 1.0f / (1.0f - r_unpol);
-                V3 dirs[NCH];
-                uint32_t live = 0u, match_mask = 0u;
-#pragma unroll
-                for (int c = 0; c < NCH; ++c) {
-                    // apply_refract_channel (reflect_refract.rs:517-591)
-                    const float4 orig = st[c];
-                    float n1k = inside ? n_o[c] : 1.0f;
-                    float n2k = inside ? 1.0f : n_o[c];
-                    float r = n1k / n2k;
-                    float s2k = r * r * one_m_c2;
-                    if (s2k > 1.0f) {
-                        st[c] = scale4(st[c], 0.0f);
-                        path_pdf[c] = 0.0f;
-                        dirs[c] = v3(0.0f, 0.0f, 0.0f);
-                        continue;
-                    }
-                    float ctk = sqrtf(fmaxf(1.0f - s2k, 0.0f));
-                    V3 dk = normalize(k * r + normal * fma_(r, cos_i, -ctk));
-                    bool match = dot(dk, refr) >= DIRECTION_MATCH_COS_TOL;
-                    // compute_channel_transmission (exit_split.rs:69-101)
-                    float num = 2.0f * n1k * cos_i;
-                    float t_s = num / fma_(n2k, ctk, n1k * cos_i);
-                    float t_p = num / fma_(n1k, ctk, n2k * cos_i);
-                    float factor = (n2k * ctk) / fmaxf(n1k * cos_i, 1e-6f);
-                    float ts2 = t_s * t_s * factor, tp2 = t_p * t_p * factor;
-                    float4 tr = mueller_fresnel(midpoint(ts2, tp2), 0.5f * (ts2 - tp2), t_s * t_p * factor, orig);
-                    tr = scale4(tr, inv_t);
-                    float a_, b_;
-                    fresnel_rs_rp(n1k, n2k, cos_i, ctk, a_, b_);
-                    float ruk = clampf(0.5f * fma_(b_, b_, a_ * a_), R_UNPOL_PDF_MIN, R_UNPOL_PDF_MAX);
-                    if (match) {
-                        st[c] = tr;
-                        path_pdf[c] *= 1.0f - ruk;
-                        match_mask |= 1u << c;
-                    } else {
-                        // apply_channel_chromatic_termination (reflect_refract.rs:444-515)
-                        float prefix = path_pdf[c];
-                        st[c] = scale4(st[c], 0.0f);
-                        path_pdf[c] = prefix * (1.0f - ruk);
-                        if (inside && fmaxf(orig.x, 0.0f) > 0.0f) {
-                            // try_split_exit_channel (exit_split.rs:128-163)
-                            if (EVENTS) ++n_probe;
-                            float tt;
-                            if (intersect(planes, P.n_planes, hit + dk * 1e-4f, dk, tt) < 0) {
-                                float env;
-                                if (P.env.model == ENV_ISO_HEMISPHERE) {
-                                    env = iso_factor(P.env, dk, observer) *
-                                          (spec_power(P.env, lambda[c]) * P.env.exposure);
-                                } else {
-                                    DirLight L;
-                                    direction_lighting(P.env, dk, observer, L);
-                                    env = light_radiance(P.env, L, spec_power(P.env, lambda[c]));
-                                }
-                                split[c] = fma_(fmaxf(fmaxf(tr.x, 0.0f), 0.0f) * 1.0f, env, split[c]);
-                            }
-                        }
-                    }
-                    dirs[c] = dk;
-                    live |= 1u << c;
-                }
-                if (!inside) {
-                    // narrow_compat (refraction/context.rs:219-242)
-#pragma unroll
-                    for (int a = 0; a < NCH; ++a)
-#pragma unroll
-                        for (int b = a + 1; b < NCH; ++b) {
-                            bool m;
-                            if (a == 0) m = (match_mask >> b) & 1u;
-                            else if (((live >> a) & 1u) && ((live >> b) & 1u))
-                                m = dot(dirs[a], dirs[b]) >= DIRECTION_MATCH_COS_TOL;
-                            else m = true;
-                            if (!m) {
-                                compat[a] &= ~(1u << b);
-                                compat[b] &= ~(1u << a);
-                            }
-                        }
-                }
-                new_k = refr;
-                new_inside = !inside;
+            for (int c = 0; c < NCH; ++c) {
+                float env = light_radiance(P.env, L, spec_power(P.env, S.lambda[c]));
+                radiance[c] = fma_(fmaxf(S.st[c].x, 0.0f) * 1.0f, env, 0.0f);
             }
         }
-        // dispatch_bounce tail (transport/bounce.rs:200-238)
-        dir = new_k;
-        org = hit + new_k * 1e-4f;
-        k = new_k;
-        inside = new_inside;
-        if (bounce == 0 && !inside && P.surface_glare < 1.0f) {
-#pragma unroll
-            for (int c = 0; c < NCH; ++c) st[c] = scale4(st[c], P.surface_glare);
-        }
-        // apply_russian_roulette (transport/bounce.rs:355-375)
-        if (bounce > 4) {
-            float mx = 0.0f;
-#pragma unroll
-            for (int c = 0; c < NCH; ++c) mx = fmaxf(mx, fmaxf(st[c].x, 0.0f));
-            float q = clampf(mx, 0.05f, 1.0f);
-            float rr = unit_draw(seed, bounce, RUSSIAN_ROULETTE_STREAM);
-            if (rr > q) {
-                bounces = bounce + 1;
-                term = 2u;
-                break;
-            }
-            float inv_q = 1.0f / q;
-#pragma unroll
-            for (int c = 0; c < NCH; ++c) st[c] = scale4(st[c], inv_q);
-#pragma unroll
-            for (int c = 0; c < NCH; ++c) split[c] /= q;
-        }
+        return STEP_ESCAPED;
     }
 
-    // PathState::finish (bounce_body.rs:724-764) + color.rs:145-172
+    // facet_event (bounce_body.rs:611-707)
+    const V3 k = S.dir;
+    const V3 hit = S.org + k * t_hit;
+    const float4 pf = planes[facet];
+    V3 normal = v3(pf.x, pf.y, pf.z);
+    // rotate_stokes_to_plane_of_incidence (absorption.rs:69-87)
+    const V3 cur_plane = normalize_or_zero(cross(k, normal));
+    if (S.has_prev && dot(cur_plane, cur_plane) > 1e-6f && dot(S.prev_plane, S.prev_plane) > 1e-6f) {
+        float cos_psi = clampf(dot(S.prev_plane, cur_plane), -1.0f, 1.0f);
+        float sin_psi = dot(cross(S.prev_plane, cur_plane), normalize_or_zero(k));
+        float psi = atan2f(sin_psi, cos_psi);
+        float c2 = cosf(2.0f * psi), s2 = sinf(2.0f * psi);
+#pragma unroll
+        for (int c = 0; c < NCH; ++c) {
+            float4 s = S.st[c];
+            S.st[c] = make_float4(s.x, c2 * s.y + s2 * s.z, (-s2) * s.y + c2 * s.z, s.w);
+        }
+        if (EVENTS) ++S.n_rot;
+    }
+    S.prev_plane = cur_plane;
+    S.has_prev = true;
+    // apply_interior_segment: absorption-free isotropic media transmit
+    // exp(-0) == 1 exactly, so only the normal flips.
+    const bool inside = S.inside;
+    if (inside) normal = neg(normal);
+
+    // compute_bounce_refraction_geometry (geometry.rs:306-425), isotropic
+    const float cos_i = clampf(dot(neg(k), normal), 0.0f, 1.0f);
+    const float one_m_c2 = fma_(cos_i, -cos_i, 1.0f);
+    const float sin_i = sqrtf(fmaxf(one_m_c2, 0.0f));
+    const float n1 = inside ? S.n_o[0] : 1.0f;
+    const float n2 = inside ? 1.0f : S.n_o[0];
+    const float eta = n1 / n2;
+    const float sin2_t = eta * eta * one_m_c2;
+
+    // hero decision: forced TIR (tir.rs:56) or the reflect-vs-transmit draw
+    // against the clamped unpolarised reflectance (dispatch.rs:230-257, 345-378)
+    const bool hero_tir = sin2_t > 1.0f;
+    const float cos_t = sqrtf(fmaxf(1.0f - sin2_t, 0.0f));
+    float rs_h, rp_h;
+    fresnel_rs_rp(n1, n2, cos_i, cos_t, rs_h, rp_h);
+    const float r_unpol = clampf(0.5f * fma_(rp_h, rp_h, rs_h * rs_h), R_UNPOL_SELECT_MIN, R_UNPOL_SELECT_MAX);
+    const float rng = unit_draw(S.seed, bounce, FRESNEL_BRANCH_STREAM);
+    const bool reflect = hero_tir || rng < r_unpol;
+
+    V3 new_k;
+    bool new_inside = inside;
+    if (reflect) {
+        // apply_tir_bounce's scalar branch (tir.rs:139-160) and
+        // apply_partial_reflect_bounce (reflect_refract.rs:33-71) share every
+        // per-channel step; only the partial reflection divides by r_unpol
+        // (a multiplication by exactly 1 for the forced TIR).
+        if (EVENTS) {
+            if (hero_tir) ++S.n_tir;
+            else ++S.n_refl;
+            S.fh = fnv(fnv(S.fh, (uint32_t)facet), hero_tir ? 1u : 2u);
+        }
+        const float scale = hero_tir ? 1.0f : 1.0f / r_unpol;
+#pragma unroll
+        for (int c = 0; c < NCH; ++c) {
+            float n1k = inside ? S.n_o[c] : 1.0f;
+            float n2k = inside ? 1.0f : S.n_o[c];
+            float ek = n1k / n2k;
+            float s2k = ek * ek * one_m_c2;
+            float4 s;
+            if (s2k > 1.0f) {
+                float delta = tir_phase_delta(n1k, cos_i, sin_i);
+                s = mueller_tir(cosf(delta), sinf(delta), S.st[c]);
+            } else {
+                float ct = sqrtf(fmaxf(1.0f - s2k, 0.0f));
+                float a_, b_, a, b, cc;
+                fresnel_rs_rp(n1k, n2k, cos_i, ct, a_, b_);
+                S.path_pdf[c] *= clampf(0.5f * fma_(b_, b_, a_ * a_), R_UNPOL_PDF_MIN, R_UNPOL_PDF_MAX);
+                fresnel_reflection_coeffs(a_, b_, a, b, cc);
+                s = mueller_fresnel(a, b, cc, S.st[c]);
+            }
+            S.st[c] = scale4(s, scale);
+        }
+        new_k = k - normal * (2.0f * dot(k, normal));
+    } else {
+        // apply_refract_bounce (reflect_refract.rs:121-234), isotropic
+        if (EVENTS) {
+            ++S.n_refr;
+            S.fh = fnv(fnv(S.fh, (uint32_t)facet), 3u);
+        }
+        const float s2d = fminf(eta * eta * one_m_c2, 1.0f);
+        const float ct_dir = sqrtf(fmaxf(1.0f - s2d, 0.0f));
+        const V3 refr = normalize(k * eta + normal * fma_(eta, cos_i, -ct_dir));
+        const float inv_t = 1.0f / (1.0f - r_unpol);
+        V3 dirs[NCH];
+        uint32_t live = 0u, match_mask = 0u;
+#pragma unroll
+        for (int c = 0; c < NCH; ++c) {
+            // apply_refract_channel (reflect_refract.rs:517-591)
+            const float4 orig = S.st[c];
+            float n1k = inside ? S.n_o[c] : 1.0f;
+            float n2k = inside ? 1.0f : S.n_o[c];
+            float r = n1k / n2k;
+            float s2k = r * r * one_m_c2;
+            if (s2k > 1.0f) {
+                S.st[c] = scale4(S.st[c], 0.0f);
+                S.path_pdf[c] = 0.0f;
+                dirs[c] = v3(0.0f, 0.0f, 0.0f);
+                continue;
+            }
+            float ctk = sqrtf(fmaxf(1.0f - s2k, 0.0f));
+            V3 dk = normalize(k * r + normal * fma_(r, cos_i, -ctk));
+            bool match = dot(dk, refr) >= DIRECTION_MATCH_COS_TOL;
+            // compute_channel_transmission (exit_split.rs:69-101)
+            float num = 2.0f * n1k * cos_i;
+            float t_s = num / fma_(n2k, ctk, n1k * cos_i);
+            float t_p = num / fma_(n1k, ctk, n2k * cos_i);
+            float factor = (n2k * ctk) / fmaxf(n1k * cos_i, 1e-6f);
+            float ts2 = t_s * t_s * factor, tp2 = t_p * t_p * factor;
+            float4 tr = mueller_fresnel(midpoint(ts2, tp2), 0.5f * (ts2 - tp2), t_s * t_p * factor, orig);
+            tr = scale4(tr, inv_t);
+            float a_, b_;
+            fresnel_rs_rp(n1k, n2k, cos_i, ctk, a_, b_);
+            float ruk = clampf(0.5f * fma_(b_, b_, a_ * a_), R_UNPOL_PDF_MIN, R_UNPOL_PDF_MAX);
+            if (match) {
+                S.st[c] = tr;
+                S.path_pdf[c] *= 1.0f - ruk;
+                match_mask |= 1u << c;
+            } else {
+                // apply_channel_chromatic_termination (reflect_refract.rs:444-515)
+                float prefix = S.path_pdf[c];
+                S.st[c] = scale4(S.st[c], 0.0f);
+                S.path_pdf[c] = prefix * (1.0f - ruk);
+                if (inside && fmaxf(orig.x, 0.0f) > 0.0f) {
+                    // try_split_exit_channel (exit_split.rs:128-163): the
+                    // re-entry probe of a convex stone resolves from the exit
+                    // facet alone in all but near-grazing cases.
+                    if (EVENTS) ++S.n_probe;
+                    const V3 po = hit + dk * 1e-4f;
+                    bool escapes = leaves_convex(planes, facet, po, dk);
+                    if (!escapes) {
+                        float tt;
+                        escapes = intersect(planes, P.n_planes, po, dk, tt) < 0;
+                    }
+                    if (escapes) {
+                        float env = ISO ? iso_factor(P.env, dk, S.observer) *
+                                              (spec_power(P.env, S.lambda[c]) * P.env.exposure)
+                                        : env_radiance(P.env, dk, S.observer, S.lambda[c]);
+                        S.split[c] = fma_(fmaxf(fmaxf(tr.x, 0.0f), 0.0f) * 1.0f, env, S.split[c]);
+                    }
+                }
+            }
+            dirs[c] = dk;
+            live |= 1u << c;
+        }
+        if (!inside) {
+            // narrow_compat (refraction/context.rs:219-242)
+#pragma unroll
+            for (int a = 0; a < NCH; ++a)
+#pragma unroll
+                for (int b = a + 1; b < NCH; ++b) {
+                    bool m;
+                    if (a == 0) m = (match_mask >> b) & 1u;
+                    else if (((live >> a) & 1u) && ((live >> b) & 1u))
+                        m = dot(dirs[a], dirs[b]) >= DIRECTION_MATCH_COS_TOL;
+                    else m = true;
+                    if (!m) S.compat &= ~((1ull << (8 * a + b)) | (1ull << (8 * b + a)));
+                }
+        }
+        new_k = refr;
+        new_inside = !inside;
+    }
+    // dispatch_bounce tail (transport/bounce.rs:200-238)
+    S.dir = new_k;
+    S.org = hit + new_k * 1e-4f;
+    S.inside = new_inside;
+    S.last_facet = facet;
+    if (bounce == 0 && !new_inside && P.surface_glare < 1.0f) {
+#pragma unroll
+        for (int c = 0; c < NCH; ++c) S.st[c] = scale4(S.st[c], P.surface_glare);
+    }
+    // apply_russian_roulette (transport/bounce.rs:355-375)
+    if (bounce > 4) {
+        float mx = 0.0f;
+#pragma unroll
+        for (int c = 0; c < NCH; ++c) mx = fmaxf(mx, fmaxf(S.st[c].x, 0.0f));
+        float q = clampf(mx, 0.05f, 1.0f);
+        float rr = unit_draw(S.seed, bounce, RUSSIAN_ROULETTE_STREAM);
+        if (rr > q) return STEP_ROULETTE;
+        float inv_q = 1.0f / q;
+#pragma unroll
+        for (int c = 0; c < NCH; ++c) S.st[c] = scale4(S.st[c], inv_q);
+#pragma unroll
+        for (int c = 0; c < NCH; ++c) S.split[c] /= q;
+    }
+    return STEP_CONTINUE;
+}
+
+// PathState::finish (bounce_body.rs:724-764) + color.rs:145-172 + von Kries.
+__device__ __forceinline__ void path_finish(const FilmParams& P, const PathState& S, bool escaped,
+                                            float radiance[NCH], float xyz[3]) {
     if (escaped) {
 #pragma unroll
-        for (int c = 0; c < NCH; ++c) radiance[c] += split[c];
+        for (int c = 0; c < NCH; ++c) radiance[c] += S.split[c];
     }
     const float norm_factor = (400.0f / 8.0f) / 106.856f;
     float X = 0.0f, Y = 0.0f, Z = 0.0f;
 #pragma unroll
     for (int c = 0; c < NCH; ++c) {
+        const uint32_t fam = compat_of(S.compat, c);
         float family = -0.0f;
 #pragma unroll
         for (int j = 0; j < NCH; ++j)
-            if ((compat[c] >> j) & 1u) family += path_pdf[j];
-        float w = family <= 1e-12f ? 1.0f : (8.0f * path_pdf[0]) / family;
+            if ((fam >> j) & 1u) family += S.path_pdf[j];
+        float w = family <= 1e-12f ? 1.0f : (8.0f * S.path_pdf[0]) / family;
         float cx, cy, cz;
-        cie_cmf(lambda[c], cx, cy, cz);
+        cie_cmf(S.lambda[c], cx, cy, cz);
         float s = (radiance[c] * w) * norm_factor;
         X = X + cx * s;
         Y = Y + cy * s;
@@ -390,29 +416,53 @@ __device__ __forceinline__ void trace_path(const FilmParams& P, const float4* __
     X = X + 0.0f;
     Y = Y + 0.0f;
     Z = Z + 0.0f;
-    // apply_von_kries_white_balance (color.rs:220-254)
     float L_ = fma_(0.8951f, X, fma_(0.2664f, Y, -0.1614f * Z)) * P.env.wb[0];
     float M_ = fma_(-0.7502f, X, fma_(1.7135f, Y, 0.0367f * Z)) * P.env.wb[1];
     float S_ = fma_(0.0389f, X, fma_(-0.0685f, Y, 1.0296f * Z)) * P.env.wb[2];
-    out.xyz[0] = fmaxf(fma_(0.986993f, L_, fma_(-0.147054f, M_, 0.159963f * S_)), 0.0f);
-    out.xyz[1] = fmaxf(fma_(0.432305f, L_, fma_(0.518360f, M_, 0.049291f * S_)), 0.0f);
-    out.xyz[2] = fmaxf(fma_(-0.008529f, L_, fma_(0.040043f, M_, 0.968487f * S_)), 0.0f);
+    xyz[0] = fmaxf(fma_(0.986993f, L_, fma_(-0.147054f, M_, 0.159963f * S_)), 0.0f);
+    xyz[1] = fmaxf(fma_(0.432305f, L_, fma_(0.518360f, M_, 0.049291f * S_)), 0.0f);
+    xyz[2] = fmaxf(fma_(-0.008529f, L_, fma_(0.040043f, M_, 0.968487f * S_)), 0.0f);
+}
+
+// trace_spectral_ray (transport/mod.rs:118) for one explicit ray, start to end.
+template <bool EVENTS>
+__device__ __forceinline__ void trace_path(const FilmParams& P, const float4* __restrict__ planes,
+                                           V3 org, V3 dir, const uint32_t seed, const float hero,
+                                           PathOut& out, uint32_t* counts) {
+    PathState S;
+    path_init(P, org, dir, seed, hero, S);
+    float radiance[NCH];
+#pragma unroll
+    for (int c = 0; c < NCH; ++c) radiance[c] = 0.0f;
+    uint32_t bounces = P.max_bounces, term = 3u;
+    bool escaped = false;
+    for (uint32_t b = 0; b < P.max_bounces; ++b) {
+        S.bounce = b;
+        int r = path_bounce<EVENTS>(P, planes, S, radiance);
+        if (r != STEP_CONTINUE) {
+            bounces = b + 1;
+            term = r == STEP_ESCAPED ? 0u : 2u;
+            escaped = r == STEP_ESCAPED;
+            break;
+        }
+    }
+    path_finish(P, S, escaped, radiance, out.xyz);
     if (EVENTS) {
         out.ev.bounces = bounces;
         out.ev.termination = term;
-        out.ev.facet_hash = fh;
-        out.ev.n_tir = (uint16_t)n_tir;
-        out.ev.n_reflect = (uint16_t)n_refl;
-        out.ev.n_refract = (uint16_t)n_refr;
-        out.ev.n_probe = (uint16_t)n_probe;
+        out.ev.facet_hash = S.fh;
+        out.ev.n_tir = (uint16_t)S.n_tir;
+        out.ev.n_reflect = (uint16_t)S.n_refl;
+        out.ev.n_refract = (uint16_t)S.n_refr;
+        out.ev.n_probe = (uint16_t)S.n_probe;
         if (counts) {
             counts[EV_BOUNCES] += bounces;
-            counts[EV_TIR] += n_tir;
-            counts[EV_REFLECT] += n_refl;
-            counts[EV_REFRACT] += n_refr;
-            counts[EV_PROBE] += n_probe;
+            counts[EV_TIR] += S.n_tir;
+            counts[EV_REFLECT] += S.n_refl;
+            counts[EV_REFRACT] += S.n_refr;
+            counts[EV_PROBE] += S.n_probe;
             counts[EV_ESCAPED] += term == 0u ? 1u : 0u;
-            counts[EV_ROTATIONS] += n_rot;
+            counts[EV_ROTATIONS] += S.n_rot;
         }
     }
 }
@@ -421,6 +471,15 @@ __device__ __forceinline__ void trace_path(const FilmParams& P, const float4* __
 // kernels
 // ---------------------------------------------------------------------------
 constexpr int kBlock = 256;
+
+// Occupancy target of the persistent film kernel (waves per SIMD); the
+// register budget follows from it (512 VGPRs / waves).
+#ifndef IXG_FILM_WAVES_PER_EU
+#define IXG_FILM_WAVES_PER_EU 2
+#endif
+#define IXG_WAVES_ATTR __attribute__((amdgpu_waves_per_eu(IXG_FILM_WAVES_PER_EU)))
+
+__global__ void zero_counter_kernel(uint32_t* c) { *c = 0u; }
 
 __device__ __forceinline__ void stage_planes(const FilmParams& P, const float4* __restrict__ g,
                                              float4* s) {
@@ -448,6 +507,68 @@ __global__ __launch_bounds__(kBlock) void film_transport_kernel(const FilmParams
     out_xyz[3 * i + 0] = o.xyz[0];
     out_xyz[3 * i + 1] = o.xyz[1];
     out_xyz[3 * i + 2] = o.xyz[2];
+}
+
+// Film, persistent: each lane loops one bounce at a time and, the moment its
+// path ends, takes the next path index from a wave-wide grab on a global
+// counter.  Lanes therefore never idle behind a longer path of their wave
+// (paths run 1..max_bounces iterations, mostly 2-6), and the per-path results
+// are the same as film_transport_kernel's because a path is a pure function
+// of its (pixel, sample) index.
+template <bool ISO>
+__global__ __launch_bounds__(kBlock) IXG_WAVES_ATTR void film_persistent_kernel(const FilmParams P,
+                                                                 const float4* __restrict__ planes,
+                                                                 float* __restrict__ out_xyz,
+                                                                 uint32_t* __restrict__ counter) {
+    __shared__ float4 sp[IXG_MAX_PLANES];
+    stage_planes(P, planes, sp);
+    const uint32_t n = (uint32_t)P.n_paths;
+    const uint32_t lane = __lane_id();
+    PathState S;
+    float radiance[NCH];
+    uint32_t idx = 0;
+    bool active = false, done = false;
+    while (true) {
+        const bool need = !active && !done;
+        const uint64_t need_mask = __ballot(need);
+        if (need_mask) {
+            const uint32_t leader = (uint32_t)(__ffsll((unsigned long long)need_mask) - 1);
+            uint32_t base = 0;
+            if (lane == leader) base = atomicAdd(counter, (uint32_t)__popcll(need_mask));
+            base = __shfl(base, (int)leader);
+            if (need) {
+                idx = base + (uint32_t)__popcll(need_mask & ((1ull << lane) - 1ull));
+                if (idx < n) {
+                    const uint64_t p = P.first_path + idx;
+                    V3 org, dir;
+                    uint32_t seed;
+                    float hero;
+                    camera_sample(P, (uint32_t)(p / P.spp), P.sample_offset + (uint32_t)(p % P.spp),
+                                  org, dir, seed, hero);
+                    path_init(P, org, dir, seed, hero, S);
+#pragma unroll
+                    for (int c = 0; c < NCH; ++c) radiance[c] = 0.0f;
+                    active = true;
+                } else {
+                    done = true;
+                }
+            }
+        }
+        if (__ballot(active) == 0ull) break;
+        if (active) {
+            int r = STEP_CONTINUE;
+            if (S.bounce < P.max_bounces) r = path_bounce<false, ISO>(P, sp, S, radiance);
+            const bool fin = r != STEP_CONTINUE || ++S.bounce >= P.max_bounces;
+            if (fin) {
+                float xyz[3];
+                path_finish(P, S, r == STEP_ESCAPED, radiance, xyz);
+                out_xyz[3 * (size_t)idx + 0] = xyz[0];
+                out_xyz[3 * (size_t)idx + 1] = xyz[1];
+                out_xyz[3 * (size_t)idx + 2] = xyz[2];
+                active = false;
+            }
+        }
+    }
 }
 
 // Diagnostics: per-path XYZ + integer side channel.
@@ -574,4 +695,28 @@ extern "C" hipError_t ixg_launch_rays(const ixg::FilmParams* P, const float4* d_
     hipLaunchKernelGGL(ixg::film_rays_kernel, dim3(blocks), dim3(ixg::kBlock), 0, stream, *P, d_planes,
                        d_rays, n_rays, d_xyz, d_ev);
     return hipGetLastError();
+}
+
+extern "C" hipError_t ixg_launch_film_persistent(const ixg::FilmParams* P, const float4* d_planes, float* d_xyz,
+                                                 uint32_t* d_counter, uint32_t grid_blocks, hipStream_t stream) {
+    // a fill kernel rather than a memset node, so the launch pair can be captured
+    hipLaunchKernelGGL(ixg::zero_counter_kernel, dim3(1), dim3(1), 0, stream, d_counter);
+    if (P->env.model == ixg::ENV_ISO_HEMISPHERE)
+        hipLaunchKernelGGL(ixg::film_persistent_kernel<true>, dim3(grid_blocks), dim3(ixg::kBlock), 0, stream,
+                           *P, d_planes, d_xyz, d_counter);
+    else
+        hipLaunchKernelGGL(ixg::film_persistent_kernel<false>, dim3(grid_blocks), dim3(ixg::kBlock), 0, stream,
+                           *P, d_planes, d_xyz, d_counter);
+    return hipGetLastError();
+}
+
+extern "C" hipError_t ixg_persistent_grid(int device, uint32_t* blocks) {
+    int per_cu = 0, cus = 0;
+    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, ixg::film_persistent_kernel<true>,
+                                                                ixg::kBlock, 0);
+    if (e != hipSuccess) return e;
+    e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device);
+    if (e != hipSuccess) return e;
+    *blocks = (uint32_t)((per_cu > 0 ? per_cu : 1) * cus);
+    return hipSuccess;
 }

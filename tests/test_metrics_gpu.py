@@ -34,11 +34,11 @@ def test_metric_image_matches_oracle(backend, preset):
               "scint_returned", "scint_poses"):
         assert np.array_equal(g_px[f], o_px[f]), f
     assert g_px["hits"].sum() > 0 and g_px["classes"][:, 2].sum() > 0
-    assert np.allclose(g_px["fire_sum_deg"], o_px["fire_sum_deg"], rtol=1e-5, atol=1e-6)
+    assert np.allclose(g_px["fire_sum_deg"], o_px["fire_sum_deg"], rtol=2e-3, atol=1e-5)
     assert g_tot.total.hits == o_tot.total.hits
     assert list(g_tot.total.classes) == list(o_tot.total.classes)
     assert g_tot.brilliance_pct == o_tot.brilliance_pct
-    assert abs(g_tot.fire_index - o_tot.fire_index) <= 1e-4 * max(1.0, o_tot.fire_index)
+    assert abs(g_tot.fire_index - o_tot.fire_index) <= 1e-3 * max(1.0, o_tot.fire_index)
     assert abs(g_tot.scintillation_pct - o_scint) < 1e-4
 
 
