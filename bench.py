@@ -116,7 +116,7 @@ def main():
                      "achieved": round(achieved, 3), "peak": MI355X_FP32_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": round(achieved / MI355X_FP32_PEAK_TFLOPS, 4),
                      "traffic": None,
-                     "kernel": "film_transport_kernel", "kernel_ms": round(avg_kernel_ms, 4),
+                     "kernel": "film_persistent_kernel", "kernel_ms": round(avg_kernel_ms, 4),
                      "flops_per_launch": step_flops, "events": totals},
     }
     if not args.no_cpu_baseline:

@@ -191,9 +191,13 @@ ixg_status ixg_trace_paths(ixg_ctx* ctx, const ixg_scene* scene, uint32_t sample
  * ignored.  out_xyz[n*3] f32; events[n] may be NULL. */
 ixg_status ixg_trace_rays(ixg_ctx* ctx, const ixg_scene* scene, const float* rays, uint32_t n,
                           float* out_xyz, ixg_path_events* events);
-/* Event totals of one film call (FLOP model input), summed on the device. */
+/* Event totals of one film call (FLOP model input), summed on the device:
+ * {paths, loop iterations, TIR, partial reflections, refractions, exit-split
+ * probes, escapes, Stokes frame rotations, bounce slab scans run, probe slab
+ * scans run}. */
+#define IXG_EVENT_SLOTS 10
 ixg_status ixg_count_events(ixg_ctx* ctx, const ixg_scene* scene, uint32_t sample_offset,
-                            uint32_t spp, uint64_t totals[8]);
+                            uint32_t spp, uint64_t totals[IXG_EVENT_SLOTS]);
 /* Average duration (ms) of the last film call's transport kernel launches,
  * measured with HIP events on the library's stream. */
 double ixg_last_kernel_ms(const ixg_ctx* ctx);

@@ -385,7 +385,7 @@ ixg_status ixg_trace_rays(ixg_ctx* ctx, const ixg_scene* s, const float* rays, u
 }
 
 ixg_status ixg_count_events(ixg_ctx* ctx, const ixg_scene* s, uint32_t sample_offset, uint32_t spp,
-                            uint64_t totals[8]) {
+                            uint64_t totals[IXG_EVENT_SLOTS]) {
     ixg_status st = begin_call(ctx, s);
     if (st != IXG_OK) return st;
     if (!totals) return fail(ctx, IXG_INVALID, "null totals");

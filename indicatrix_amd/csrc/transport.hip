@@ -108,6 +108,7 @@ struct PathState {
     bool inside, has_prev;
     // integer side channel
     uint32_t fh, n_tir, n_refl, n_refr, n_probe, n_rot;
+    uint32_t n_scan, n_probe_scan;  // slab scans actually run (bounce, exit probes)
 };
 
 __device__ __forceinline__ uint32_t compat_of(uint64_t c, int ch) { return (uint32_t)(c >> (8 * ch)) & 0xFFu; }
@@ -140,6 +141,7 @@ __device__ __forceinline__ void path_init(const FilmParams& P, V3 org, V3 dir, u
     }
     S.fh = 2166136261u;
     S.n_tir = S.n_refl = S.n_refr = S.n_probe = S.n_rot = 0;
+    S.n_scan = S.n_probe_scan = 0;
 }
 
 __device__ __forceinline__ float env_radiance(const EnvParams& E, V3 dir, V3 observer, float lambda) {
@@ -173,8 +175,10 @@ __device__ __forceinline__ int path_bounce(const FilmParams& P, const float4* __
     int facet;
     if (!S.inside && S.last_facet >= 0 && leaves_convex(planes, S.last_facet, S.org, S.dir))
         facet = -1;
-    else
+    else {
         facet = intersect(planes, P.n_planes, S.org, S.dir, t_hit);
+        if (EVENTS) ++S.n_scan;
+    }
     if (facet < 0) {
         // handle_miss (bounce_body/miss.rs:33-54)
         if (ISO || P.env.model == ENV_ISO_HEMISPHERE) {
@@ -334,6 +338,7 @@ __device__ __forceinline__ int path_bounce(const FilmParams& P, const float4* __
                     if (!escapes) {
                         float tt;
                         escapes = intersect(planes, P.n_planes, po, dk, tt) < 0;
+                        if (EVENTS) ++S.n_probe_scan;
                     }
                     if (escapes) {
                         float env = ISO ? iso_factor(P.env, dk, S.observer) *
@@ -463,6 +468,8 @@ __device__ __forceinline__ void trace_path(const FilmParams& P, const float4* __
             counts[EV_PROBE] += S.n_probe;
             counts[EV_ESCAPED] += term == 0u ? 1u : 0u;
             counts[EV_ROTATIONS] += S.n_rot;
+            counts[EV_SCANS] += S.n_scan;
+            counts[EV_PROBE_SCANS] += S.n_probe_scan;
         }
     }
 }
@@ -475,7 +482,7 @@ constexpr int kBlock = 256;
 // Occupancy target of the persistent film kernel (waves per SIMD); the
 // register budget follows from it (512 VGPRs / waves).
 #ifndef IXG_FILM_WAVES_PER_EU
-#define IXG_FILM_WAVES_PER_EU 2
+#define IXG_FILM_WAVES_PER_EU 3
 #endif
 #define IXG_WAVES_ATTR __attribute__((amdgpu_waves_per_eu(IXG_FILM_WAVES_PER_EU)))
 
@@ -582,7 +589,7 @@ __global__ __launch_bounds__(kBlock) void film_events_kernel(const FilmParams P,
     if (threadIdx.x < EV_NUM) block_tot[threadIdx.x] = 0ull;
     stage_planes(P, planes, sp);
     uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
-    uint32_t counts[EV_NUM] = {0, 0, 0, 0, 0, 0, 0, 0};
+    uint32_t counts[EV_NUM] = {};
     if (i < P.n_paths) {
         uint64_t p = P.first_path + i;
         uint32_t pixel = (uint32_t)(p / P.spp);

@@ -58,7 +58,9 @@ enum EventSlot : int {
     EV_PROBE = 5,        // exit-split probe slab scans
     EV_ESCAPED = 6,
     EV_ROTATIONS = 7,    // Stokes frame rotations
-    EV_NUM = 8
+    EV_SCANS = 8,        // bounce slab scans actually run (convex-exit shortcut elides the rest)
+    EV_PROBE_SCANS = 9,  // exit-probe slab scans actually run
+    EV_NUM = 10
 };
 
 // One metric image (a design under one camera pose).

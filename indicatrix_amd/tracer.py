@@ -289,14 +289,15 @@ class GpuBackend:
         return xyz, events_to_numpy(ev)
 
     def count_events(self, scene, sample_offset, spp):
-        tot = (C.c_uint64 * 8)()
+        tot = (C.c_uint64 * len(EVENT_SLOTS))()
         st = lib().ixg_count_events(self._ctx, scene.ref(), sample_offset, spp, tot)
         if st != abi.IXG_OK:
             raise IxgError(st, self.last_error())
         return dict(zip(EVENT_SLOTS, list(tot)))
 
 
-EVENT_SLOTS = ["paths", "bounces", "tir", "reflect", "refract", "probe", "escaped", "rotations"]
+EVENT_SLOTS = ["paths", "bounces", "tir", "reflect", "refract", "probe", "escaped", "rotations",
+               "scans", "probe_scans"]
 
 
 def route(scene):
