@@ -276,6 +276,30 @@ ixg_status ixg_metric_sweep(ixg_ctx* ctx, const ixg_facet_plane* planes, uint32_
                             uint32_t n_tilts, uint32_t width, uint32_t height, uint32_t samples,
                             float distance, const ixg_environment* env, float* out_metrics);
 
+/* ---- metric surrogate (new work; the reference has no surrogate) ----
+ * A BF16 MLP 16 -> 4 x 256 (GELU) -> 3, FP32 master weights and Adam,
+ * trained and evaluated with MFMA kernels.  x, t, y are device pointers:
+ * x [batch x 16], t and y [batch x 3], f32, batch a multiple of 64 and at
+ * most the max_batch given at creation.  Parameters are exchanged in the
+ * padded FP32 layout of ixg_mlp_param_count() floats:
+ *   W0 [256 x 64] | W1..W3 [256 x 256] | W4 [64 x 256] | b0..b3 [256] | b4 [64]
+ * (row-major, out x in; input columns >= 16 and output rows >= 3 are zero). */
+typedef struct ixg_mlp ixg_mlp;
+uint32_t ixg_mlp_param_count(void);
+ixg_status ixg_mlp_create(ixg_ctx* ctx, uint32_t max_batch, uint64_t seed, ixg_mlp** out);
+void ixg_mlp_destroy(ixg_mlp* mlp);
+ixg_status ixg_mlp_set_params(ixg_mlp* mlp, const float* host, size_t count);
+ixg_status ixg_mlp_get_params(ixg_mlp* mlp, float* host, size_t count);
+ixg_status ixg_mlp_infer(ixg_mlp* mlp, const float* d_x, uint32_t batch, float* d_y);
+/* Loss (mean squared error over batch x 3) and its gradient w.r.t. every
+ * parameter (host, padded layout), without updating. */
+ixg_status ixg_mlp_gradients(ixg_mlp* mlp, const float* d_x, const float* d_t, uint32_t batch,
+                             float* host_grad, size_t count, float* loss);
+/* One Adam step (beta 0.9/0.999, eps 1e-8) at learning rate lr; loss may be NULL. */
+ixg_status ixg_mlp_train_step(ixg_mlp* mlp, const float* d_x, const float* d_t, uint32_t batch,
+                              float lr, float* loss);
+ixg_status ixg_mlp_sync(ixg_mlp* mlp);
+
 /* ---- host helpers (same f32 arithmetic as the reference, glibc libm) ---- */
 /* Camera::new (RT/camera.rs:108). */
 void ixg_camera_new(float yaw, float pitch, float distance, float fov_deg, uint32_t width,

@@ -49,6 +49,17 @@ def _declare(lib):
         "ixg_metric_sweep": (st, [C.c_void_p, P(abi.FacetPlane), C.c_uint32, P(abi.Design),
                                   C.c_uint32, P(C.c_float), C.c_uint32, C.c_uint32, C.c_uint32,
                                   C.c_uint32, C.c_float, P(abi.Environment), P(C.c_float)]),
+        "ixg_mlp_param_count": (C.c_uint32, []),
+        "ixg_mlp_create": (st, [C.c_void_p, C.c_uint32, C.c_uint64, P(C.c_void_p)]),
+        "ixg_mlp_destroy": (None, [C.c_void_p]),
+        "ixg_mlp_set_params": (st, [C.c_void_p, P(C.c_float), C.c_size_t]),
+        "ixg_mlp_get_params": (st, [C.c_void_p, P(C.c_float), C.c_size_t]),
+        "ixg_mlp_infer": (st, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p]),
+        "ixg_mlp_gradients": (st, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, P(C.c_float),
+                                   C.c_size_t, P(C.c_float)]),
+        "ixg_mlp_train_step": (st, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_float,
+                                    P(C.c_float)]),
+        "ixg_mlp_sync": (st, [C.c_void_p]),
         "ixg_last_kernel_ms": (C.c_double, [C.c_void_p]),
         "ixg_last_kernel_launches": (C.c_uint32, [C.c_void_p]),
         "ixg_camera_new": (None, [C.c_float, C.c_float, C.c_float, C.c_float, C.c_uint32,
