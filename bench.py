@@ -1,10 +1,17 @@
-"""Film throughput on one MI355X: Msamples/s of 8-lambda spectral paths.
+"""Throughput on one MI355X: Msamples/s of 8-lambda spectral paths, and
+design evaluations/s of the batched metric sweep and the surrogate-screened
+Optimize.
 
 One step = one pass of the film hot path over one frame of the workload
 (every pixel x spp paths, absolute sample indices advancing each step),
-accumulated in FP64 into a device-resident buffer.  Prints ONE JSON line.
+accumulated in FP64 into a device-resident buffer; `value` is the film rate.
+The same JSON line carries the sweep (512 seeded designs x 21 tilts, 64x64 @
+32 samples) and the screened Optimize (8192 candidates per step through the
+surrogate, top 64 traced at 128x128 @ 64 samples x 21 tilts) as
+`design_evals`.  Prints ONE JSON line.
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
+                  [--film-only] [--copy-calibration]
 """
 import argparse
 import json
@@ -27,7 +34,87 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0,
                     help="approximate CPU work of the bounded CPU-baseline sample")
+    ap.add_argument("--film-only", action="store_true",
+                    help="measure the film only (profiling runs)")
+    ap.add_argument("--optimize-steps", type=int, default=2)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="run only a 1 GiB device copy (scales rocprof FETCH/WRITE_SIZE)")
     return ap.parse_args()
+
+
+SURROGATE_WEIGHTS = os.path.join(ROOT, "indicatrix_amd", "data", "surrogate.npz")
+
+
+def copy_calibration():
+    """A plain device copy of known size, so the HBM byte counters of a
+    profiled run can be scaled (read 1 GiB, write 1 GiB per launch)."""
+    import torch
+    a = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_per_launch": a.numel() * 4, "launches": 3}), flush=True)
+
+
+def sweep_rate(backend, n_designs=512):
+    """C3: seeded designs (32..256 planes, Sellmeier-1 from n_D / Abbe) x 21
+    tilts, 64x64 @ 32 samples, one ixg_metric_sweep call."""
+    from indicatrix_amd import designs, metrics, tracer
+    planes, ds, _ = designs.sweep_batch(n_designs)
+    env = tracer.LightingPreset.studio("IsoHemisphere", 1.0, 0.0, 0.0)
+    metrics.metric_sweep(backend, planes, ds[:16], designs.TILTS_C3, 64, 64, 32, 2.4, env)  # warm
+    t0 = time.perf_counter()
+    out = metrics.metric_sweep(backend, planes, ds, designs.TILTS_C3, 64, 64, 32, 2.4, env)
+    dt = time.perf_counter() - t0
+    return {"designs": n_designs, "tilts": len(designs.TILTS_C3), "seconds": round(dt, 3),
+            "design_evals_per_s": round(n_designs / dt, 2),
+            "mean_metrics": [round(float(v), 4) for v in out.mean(axis=(0, 1))]}, (planes, ds, env)
+
+
+def sweep_cpu_baseline(planes, ds, env):
+    """The oracle's evaluate_optical_image per (design, tilt) on the host's
+    cores for the first design of the sweep (21 images)."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib
+    from indicatrix_amd import designs
+    threads = min(os.cpu_count() or 1, 16)
+    t0 = time.perf_counter()
+    oracle_lib.metric_sweep(planes, ds[:1], designs.TILTS_C3, 64, 64, 32, 2.4, env, nthreads=threads)
+    dt = time.perf_counter() - t0
+    return {"value": round(1.0 / dt, 4), "unit": "design evals/s", "cores": threads, "kind": "port",
+            "sample": f"design 0 of the sweep ({ds[0][1]} planes) x 21 tilts, 64x64 @ 32 samples: "
+                      f"{dt:.2f} s on {threads} threads"}
+
+
+def screened_rate(backend, steps):
+    """C5: the surrogate-screened Optimize loop with the committed surrogate."""
+    import numpy as np
+    from indicatrix_amd import optimize, surrogate
+    if not os.path.exists(SURROGATE_WEIGHTS):
+        return {"skipped": f"{SURROGATE_WEIGHTS} missing"}
+    net = surrogate.SurrogateMLP(backend, max_batch=8192 * 21, seed=0)
+    net.load(SURROGATE_WEIGHTS)
+    opt = optimize.ScreenedOptimizer(backend, net, nd=2.05, abbe=35.0, seed=11)
+    recs = [opt.step() for _ in range(steps)]
+    screen = sum(r["screen_s"] for r in recs)
+    verify = sum(r["verify_s"] for r in recs)
+    total = sum(r["step_s"] for r in recs)
+    cand = sum(r["candidates"] for r in recs)
+    verified = sum(r["verified"] for r in recs)
+    err = np.concatenate([r["rel_err"] for r in recs])
+    tracer_only = verified / verify   # the same tracer work per verified design
+    net.close()
+    return {"steps": steps, "candidates_per_step": recs[0]["candidates"],
+            "verified_per_step": recs[0]["verified"], "verify_shape": list(optimize.VERIFY_SHAPE),
+            "screen_s_per_step": round(screen / steps, 4), "verify_s_per_step": round(verify / steps, 3),
+            "design_evals_per_s": round(cand / total, 1),
+            "tracer_only_design_evals_per_s": round(tracer_only, 2),
+            "gain_vs_tracer_only": round(cand / total / tracer_only, 1),
+            "metrics": ["brilliance_pct", "fire_index", "scintillation_pct"],
+            "rel_err_median": [round(float(v), 5) for v in np.median(err, axis=0)],
+            "rel_err_mean": [round(float(v), 5) for v in err.mean(axis=0)],
+            "best_verified_objective": round(recs[-1]["best"], 4)}
 
 
 def cpu_baseline(scene, target_s):
@@ -56,11 +143,14 @@ def main():
     args = parse()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU; --gpus must be 1 "
-                                   "(the film shards by sample range across GPUs, see DESIGN.md)"}))
+                                   "(the film would shard by sample range across GPUs; only one is measured here)"}))
         sys.exit(2)
     import torch
     from indicatrix_amd import flops, tracer, workloads
 
+    if args.copy_calibration:
+        copy_calibration()
+        return
     if not torch.cuda.is_available():
         print(json.dumps({"error": "no HIP device visible"}))
         sys.exit(1)
@@ -119,8 +209,16 @@ def main():
                      "kernel": "film_persistent_kernel", "kernel_ms": round(avg_kernel_ms, 4),
                      "flops_per_launch": step_flops, "events": totals},
     }
+    if not args.film_only:
+        print("# bench: metric sweep", file=sys.stderr, flush=True)
+        sweep, sweep_inputs = sweep_rate(backend)
+        print("# bench: screened optimize", file=sys.stderr, flush=True)
+        line["design_evals"] = {"sweep": sweep, "screened_optimize": screened_rate(backend, args.optimize_steps)}
     if not args.no_cpu_baseline:
+        print("# bench: cpu baseline", file=sys.stderr, flush=True)
         line["cpu_baseline"] = cpu_baseline(scene, args.cpu_seconds)
+        if not args.film_only:
+            line["design_evals"]["sweep"]["cpu_baseline"] = sweep_cpu_baseline(*sweep_inputs)
     print(json.dumps(line), flush=True)
     backend.close()
 

@@ -173,9 +173,10 @@ ixg_route_reason ixg_route(const ixg_scene* scene);
  * GpuBackend::try_accumulate_cancellable (backend.rs:226): traces absolute
  * sample indices [sample_offset, sample_offset+spp) for every pixel and ADDS
  * the per-pixel XYZ sums (FP64, non-finite samples dropped but counted) into
- * accum_xyz[w*h*3] only on IXG_OK.  cancel may be NULL. */
+ * accum_xyz[w*h*3] only on IXG_OK.  cancel may be NULL; it is one byte,
+ * non-zero = cancel, the in-memory form of Rust's AtomicBool. */
 ixg_status ixg_accumulate(ixg_ctx* ctx, const ixg_scene* scene, uint32_t sample_offset,
-                          uint32_t spp, double* accum_xyz, const volatile uint32_t* cancel);
+                          uint32_t spp, double* accum_xyz, const volatile uint8_t* cancel);
 /* Same, with accum_xyz a device pointer (inputs resident in HBM), on the
  * library's stream; no host copy of the result. */
 ixg_status ixg_accumulate_device(ixg_ctx* ctx, const ixg_scene* scene, uint32_t sample_offset,

@@ -35,7 +35,7 @@ def _declare(lib):
         "ixg_inject_fault": (st, [C.c_void_p, C.c_int32]),
         "ixg_route": (C.c_int, [P(abi.Scene)]),
         "ixg_accumulate": (st, [C.c_void_p, P(abi.Scene), C.c_uint32, C.c_uint32,
-                                P(C.c_double), P(C.c_uint32)]),
+                                P(C.c_double), P(C.c_uint8)]),
         "ixg_accumulate_device": (st, [C.c_void_p, P(abi.Scene), C.c_uint32, C.c_uint32,
                                        C.c_void_p]),
         "ixg_trace_paths": (st, [C.c_void_p, P(abi.Scene), C.c_uint32, C.c_uint32,
@@ -82,6 +82,14 @@ def lib():
         if not os.path.exists(LIB_PATH):
             raise ImportError(f"{LIB_PATH} is missing: run `make -C indicatrix_amd` "
                               "(or __graft_entry__.build()) first")
+        # torch's wheel bundles its own HIP runtime under the same sonames as
+        # /opt/rocm's.  Loading torch first lets libixg bind to that one copy,
+        # so device pointers and streams are shared; loading libixg first would
+        # put two HIP runtimes in the process and torch then finds no device.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         _lib = _declare(C.CDLL(LIB_PATH))
     return _lib
 

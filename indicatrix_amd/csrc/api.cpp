@@ -170,7 +170,7 @@ bool take_injected_fault(ixg_ctx* ctx) {
 
 // Runs transport + reduction for every chunk of whole pixels, adding into d_accum.
 ixg_status run_film(ixg_ctx* ctx, const ixg_scene* s, uint32_t sample_offset, uint32_t spp,
-                    double* d_accum, const volatile uint32_t* cancel, bool timed) {
+                    double* d_accum, const volatile uint8_t* cancel, bool timed) {
     ixg::FilmParams P;
     ixg_status st = build_params(ctx, s, sample_offset, spp, &P);
     if (st != IXG_OK) return st;
@@ -290,7 +290,7 @@ ixg_route_reason ixg_route(const ixg_scene* s) {
 }
 
 ixg_status ixg_accumulate(ixg_ctx* ctx, const ixg_scene* s, uint32_t sample_offset, uint32_t spp,
-                          double* accum_xyz, const volatile uint32_t* cancel) {
+                          double* accum_xyz, const volatile uint8_t* cancel) {
     ixg_status st = begin_call(ctx, s);
     if (st != IXG_OK) return st;
     if (!accum_xyz) return fail(ctx, IXG_INVALID, "null accumulator");

@@ -148,10 +148,7 @@ def trace_dataset(backend, n_designs, seed, tilts=TILTS_C3, width=64, height=64,
     rng = np.random.default_rng(seed)
     u, nd, abbe = sample_candidates(rng, n_designs)
     m = trace_metrics(backend, u, nd, abbe, tilts, width, height, samples, chunk=chunk)
-    nt = len(tilts)
-    x = features(np.repeat(u, nt, axis=0), np.repeat(nd, nt), np.repeat(abbe, nt),
-                 np.tile(np.asarray(tilts, np.float64), len(u)))
-    return x, m.reshape(-1, N_OUT)
+    return design_features(u, nd, abbe, tilts), m.reshape(-1, N_OUT)
 
 
 class SurrogateMLP:
@@ -250,8 +247,9 @@ class SurrogateMLP:
                     "train_step")
         return float(loss.value) if want_loss else None
 
-    def fit(self, x, t, epochs=10, batch=4096, lr=1e-3, seed=0, log=None):
-        """Adam on (x, t) in tracer units; standardises t, keeps the statistics.
+    def fit(self, x, t, epochs=10, batch=4096, lr=1e-3, lr_final=None, seed=0, log=None):
+        """Adam on (x, t) in tracer units; standardises t and keeps the statistics.
+        The learning rate follows a cosine from lr to lr_final (constant when None).
         Returns the mean training loss of each epoch (standardised units)."""
         torch = self._torch
         if batch % 64 or batch > self.max_batch:
@@ -266,7 +264,10 @@ class SurrogateMLP:
             raise ValueError("fewer samples than one batch")
         gen = torch.Generator(device="cuda")
         gen.manual_seed(seed)
+        steps_per_epoch = n // batch
+        total = epochs * steps_per_epoch
         history = []
+        k = 0
         for ep in range(epochs):
             perm = torch.randperm(xd.shape[0], device="cuda", generator=gen)[:n]
             xs = xd[perm].contiguous()
@@ -274,8 +275,64 @@ class SurrogateMLP:
             torch.cuda.synchronize()
             tot = 0.0
             for s in range(0, n, batch):
-                tot += self.train_step(xs[s:s + batch], ts[s:s + batch], lr, want_loss=True)
-            history.append(tot / (n // batch))
+                r = lr if lr_final is None else lr_final + 0.5 * (lr - lr_final) * (1 + math.cos(math.pi * k / total))
+                tot += self.train_step(xs[s:s + batch], ts[s:s + batch], r, want_loss=True)
+                k += 1
+            history.append(tot / steps_per_epoch)
             if log:
                 log(f"epoch {ep}: loss {history[-1]:.5f}")
         return history
+
+    def fine_tune(self, x, t, steps, lr, batch=4096, seed=0):
+        """Adam steps on (x, t) in tracer units, keeping the output statistics
+        of the original fit (used to refine the network on verified designs)."""
+        torch = self._torch
+        t = (np.asarray(t, np.float32) - self.t_mean) / self.t_std
+        n = len(x)
+        b = min(batch, self.max_batch, n) // 64 * 64
+        if b == 0 or steps <= 0:
+            return None
+        xd = self._dev(x, N_IN)
+        td = self._dev(t, N_OUT)
+        gen = torch.Generator(device="cuda")
+        gen.manual_seed(seed)
+        loss = None
+        for k in range(steps):
+            idx = torch.randperm(n, device="cuda", generator=gen)[:b]
+            xs = xd[idx].contiguous()
+            ts = td[idx].contiguous()
+            torch.cuda.synchronize()
+            loss = self.train_step(xs, ts, lr, want_loss=(k == steps - 1))
+        return loss
+
+    def save(self, path, **meta):
+        """Parameters (padded FP32 layout) and output statistics as .npz."""
+        np.savez(path, params=self.get_params(), t_mean=self.t_mean, t_std=self.t_std,
+                 **{k: np.asarray(v) for k, v in meta.items()})
+
+    def load(self, path):
+        with np.load(path, allow_pickle=False) as z:
+            self.set_params(z["params"])
+            self.t_mean = z["t_mean"].astype(np.float32)
+            self.t_std = z["t_std"].astype(np.float32)
+            return {k: z[k][()] for k in z.files if k not in ("params", "t_mean", "t_std")}
+
+
+def design_features(u, nd, abbe, tilts=TILTS_C3):
+    """Network inputs for every (design, tilt) pair, design-major: (n * t, 16)."""
+    nt = len(tilts)
+    return features(np.repeat(u, nt, axis=0), np.repeat(nd, nt), np.repeat(abbe, nt),
+                    np.tile(np.asarray(tilts, np.float64), len(u)))
+
+
+def predict_designs(net, u, nd, abbe, tilts=TILTS_C3):
+    """Surrogate metrics (n, t, 3) for a candidate set, in one batched pass."""
+    y = net.predict(design_features(u, nd, abbe, tilts))
+    return y.cpu().numpy().reshape(len(u), len(tilts), N_OUT)
+
+
+def relative_error(pred, ref):
+    """Per-design relative error of the tilt-averaged metrics: (n, 3)."""
+    p = np.asarray(pred, np.float64).mean(axis=-2)
+    r = np.asarray(ref, np.float64).mean(axis=-2)
+    return np.abs(p - r) / np.maximum(np.abs(r), 1e-6)

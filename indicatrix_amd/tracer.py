@@ -237,7 +237,7 @@ class GpuBackend:
             raise ValueError("accum must be a C-contiguous float64 array")
         if accum.size != scene.width * scene.height * 3:
             raise ValueError("accum length must be width*height*3")
-        cflag = None if cancel is None else C.cast(cancel, C.POINTER(C.c_uint32))
+        cflag = None if cancel is None else C.cast(cancel, C.POINTER(C.c_uint8))
         st = lib().ixg_accumulate(self._ctx, scene.ref(), sample_offset, spp,
                                   accum.ctypes.data_as(C.POINTER(C.c_double)), cflag)
         st = self._check(st)

@@ -107,3 +107,47 @@ def trace_paths(scene, sample_offset, spp, first=0, count=None, nthreads=None):
                           xyz.ctypes.data_as(C.POINTER(C.c_float)),
                           C.cast(ev, C.POINTER(abi.PathEvents)), nthreads or threads())
     return xyz, events_to_numpy(ev)
+
+
+def sweep_pose(tilt_deg, distance):
+    """(yaw, pitch) of a tilt-sweep camera: yaw 0 / 180 deg by the tilt's sign,
+    pitch 90 - |tilt| (MET/pose_batch.rs:118-131), in the f32 arithmetic the
+    library uses."""
+    to_rad = np.float32(np.float32(np.pi) / np.float32(180.0))
+    t = np.float32(tilt_deg)
+    yaw = float(np.float32(180.0) * to_rad) if t < 0 else 0.0
+    pitch = float(np.float32(np.float32(90.0) - np.float32(abs(t))) * to_rad)
+    return yaw, pitch
+
+
+def metric_sweep(planes, designs, tilts, width, height, samples, distance, environment,
+                 nthreads=None):
+    """The oracle's per-image evaluation of every (design, tilt) pair: (n, t, 5)
+    {brilliance, fire, scintillation, windowing, extinction}, images spread over
+    host threads."""
+    from concurrent.futures import ThreadPoolExecutor
+    from indicatrix_amd import metrics, tracer
+    planes = np.ascontiguousarray(planes, np.float32).reshape(-1, 4)
+    jobs = []
+    for i, (off, cnt, disp) in enumerate(designs):
+        pl = (abi.FacetPlane * cnt).from_buffer_copy(planes[off:off + cnt].tobytes())
+        mat = abi.Material()
+        mat.dispersion = disp
+        mat.cubic = 1
+        mat.absorption_path_scale = 1.0
+        for j, t in enumerate(tilts):
+            yaw, pitch = sweep_pose(t, distance)
+            cam = tracer.camera_new(yaw, pitch, distance, 42.0, width, height, samples)
+            req = metrics.make_request(cam, width, height, samples, pl, mat, environment, True, True,
+                                       (yaw, pitch, distance))
+            jobs.append((i, j, req, pl, mat))
+    out = np.zeros((len(designs), len(tilts), 5), np.float32)
+
+    def run(job):
+        i, j, req, _, _ = job
+        _, tot, scint = metric_image(req)
+        out[i, j] = (tot.brilliance_pct, tot.fire_index, scint, tot.windowing_pct, tot.extinction_pct)
+
+    with ThreadPoolExecutor(max_workers=nthreads or threads()) as ex:
+        list(ex.map(run, jobs))
+    return out

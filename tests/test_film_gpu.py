@@ -112,10 +112,25 @@ def test_determinism(backend):
     assert a.tobytes() == b.tobytes()
 
 
+def test_accumulate_into_device_buffer(backend):
+    """ixg_accumulate_device ADDs into a caller-owned device buffer (a torch
+    tensor here), with the same sums as the host entry point."""
+    import torch
+    sc = small_scene(16, 16)
+    host = np.zeros((256, 3))
+    assert backend.try_accumulate(sc, 0, 8, host)
+    d = torch.full((256 * 3,), 0.5, dtype=torch.float64, device="cuda")
+    torch.cuda.synchronize()
+    print("accumulate_device 16x16x8", flush=True)
+    backend.accumulate_device(sc, 0, 8, d.data_ptr())
+    got = d.cpu().numpy().reshape(256, 3)
+    np.testing.assert_array_equal(got, host + 0.5)
+
+
 def test_cancel_and_decline_leave_buffer_untouched(backend):
     sc = small_scene(8, 8)
     acc = np.full((64, 3), 7.0)
-    flag = (C.c_uint32 * 1)(1)
+    flag = (C.c_uint8 * 1)(1)
     assert backend.try_accumulate_cancellable(sc, 0, 4, acc, cancel=flag) \
         is tracer.GpuAccumulate.CANCELLED
     assert np.all(acc == 7.0)
