@@ -93,12 +93,15 @@ ixg_status ensure_accum(ixg_ctx* ctx, size_t doubles) {
 }
 
 bool material_routable(const ixg_material& m) {
-    // The device path covers isotropic, absorption-free, polished, unzoned,
-    // non-scattering materials (the configurations of this film path).
+    // The device path covers isotropic-by-symmetry (cubic, or |n_e - n_o| <=
+    // 1e-4: RayMaterialContext::is_anisotropic, transport/inner.rs:181-198),
+    // polished, unzoned, non-scattering materials, with or without absorption
+    // bands.
     if (m.biaxial || m.scattering_sigma_s > 0.0f || m.edge_rounding_radius > 0.0f) return false;
     bool anisotropic = !m.cubic && std::fabs(m.birefringence_delta) > 1e-4f;
     if (anisotropic) return false;
-    if (m.n_bands_o != 0 || m.n_bands_e != 0) return false;
+    if (m.n_bands_o < 0 || m.n_bands_o > IXG_MAX_BANDS || m.n_bands_e < 0 || m.n_bands_e > IXG_MAX_BANDS)
+        return false;
     return m.dispersion.kind >= IXG_DISP_SELLMEIER1 && m.dispersion.kind <= IXG_DISP_CAUCHY;
 }
 
@@ -107,6 +110,40 @@ ixg_status validate(ixg_ctx* ctx, const ixg_scene* s) {
     if (s->width == 0 || s->height == 0) return fail(ctx, IXG_INVALID, "empty frame");
     if ((uint64_t)s->width * s->height > (1ull << 31)) return fail(ctx, IXG_RESOURCE_LIMIT, "frame too large");
     return IXG_OK;
+}
+
+// glam Vec3 helpers on the host (compiled without contraction, so these
+// round like the reference).
+struct H3 {
+    float x, y, z;
+};
+float hdot(H3 a, H3 b) { return (a.x * b.x + a.y * b.y) + a.z * b.z; }
+H3 hscale(H3 a, float s) { return H3{a.x * s, a.y * s, a.z * s}; }
+H3 hnormalize_or_zero(H3 a) {
+    float r = 1.0f / std::sqrt(hdot(a, a));
+    return (std::isfinite(r) && r > 0.0f) ? hscale(a, r) : H3{0.0f, 0.0f, 0.0f};
+}
+H3 hcross(H3 a, H3 b) { return H3{a.y * b.z - b.y * a.z, a.z * b.x - b.z * a.x, a.x * b.y - b.x * a.y}; }
+
+// AbsorptionTensor3::uniaxial's axes: stable_orthonormal_basis of the unit
+// optic axis (birefringence/absorption_tensor.rs, birefringence/helpers.rs).
+void fill_absorption(const ixg_material& m, ixg::AbsorbParams* A) {
+    A->n_o = m.n_bands_o;
+    A->n_e = m.n_bands_e;
+    A->path_scale = m.absorption_path_scale;
+    H3 c{m.c_axis[0], m.c_axis[1], m.c_axis[2]};
+    H3 n = hnormalize_or_zero(c);
+    H3 a = std::fabs(n.x) > 0.9f ? H3{0.0f, 1.0f, 0.0f} : H3{1.0f, 0.0f, 0.0f};
+    float na = hdot(n, a);
+    H3 t = hnormalize_or_zero(H3{a.x - n.x * na, a.y - n.y * na, a.z - n.z * na});
+    H3 b = hcross(n, t);
+    const float* src[4] = {&c.x, &t.x, &b.x, &n.x};
+    float* dst[4] = {A->c_raw, A->a1, A->a2, A->cn};
+    for (int i = 0; i < 4; ++i) std::memcpy(dst[i], src[i], 3 * sizeof(float));
+    for (int i = 0; i < m.n_bands_o; ++i)
+        A->o[i] = ixg::AbsBand{m.bands_o[i].center_nm, m.bands_o[i].width, m.bands_o[i].peak, m.bands_o[i].shape};
+    for (int i = 0; i < m.n_bands_e; ++i)
+        A->e[i] = ixg::AbsBand{m.bands_e[i].center_nm, m.bands_e[i].width, m.bands_e[i].peak, m.bands_e[i].shape};
 }
 
 ixg_status build_params(ixg_ctx* ctx, const ixg_scene* s, uint32_t sample_offset, uint32_t spp,
@@ -126,6 +163,8 @@ ixg_status build_params(ixg_ctx* ctx, const ixg_scene* s, uint32_t sample_offset
     P->n_planes = s->n_planes;
     P->max_bounces = s->max_bounces;
     P->disp_kind = s->material->dispersion.kind;
+    P->absorb = (s->material->n_bands_o > 0 || s->material->n_bands_e > 0) ? 1 : 0;
+    if (P->absorb) fill_absorption(*s->material, &P->abs);
     std::memcpy(P->disp, s->material->dispersion.coeff, sizeof P->disp);
     float g = s->environment.surface_glare;
     P->surface_glare = (g >= 1.0f || std::isnan(g)) ? 1.0f : (g > 0.0f ? g : 0.0f);

@@ -242,4 +242,40 @@ __device__ __forceinline__ int intersect(const float4* __restrict__ pl, uint32_t
     return -1;
 }
 
+// exp_f32x8's lane polynomial (simd/exp_poly.rs:37-60): Cody-Waite range
+// reduction by ln 2 in two parts, a degree-5 minimax polynomial, and the
+// exponent built in the float's bits.  Few-ULP, and exactly 1 at +-0.
+__device__ __forceinline__ float exp_lane(float x) {
+    const float LOG2E = 1.44269504088896340736f;
+    const float C1 = __uint_as_float(0x3F318000u);
+    const float C2 = -2.1219444e-4f;
+    const float HI = 88.02875f, LO = -87.33654f;
+    float upper = x < HI ? x : HI;
+    x = upper > LO ? upper : LO;
+    const float n = floorf(fma_(x, LOG2E, 0.5f));
+    x = fma_(n, -C1, x);
+    x = fma_(n, -C2, x);
+    const float z = x * x;
+    float p = 1.9875691e-4f;
+    p = fma_(p, x, 1.3981999e-3f);
+    p = fma_(p, x, 8.333452e-3f);
+    p = fma_(p, x, 4.1665796e-2f);
+    p = fma_(p, x, 1.6666665e-1f);
+    p = fma_(p, x, 5.0000003e-1f);
+    p = fma_(p, z, x) + 1.0f;
+    return p * __uint_as_float((uint32_t)(((int32_t)n + 127) << 23));
+}
+
+// AbsorptionBand::evaluate (optics/absorption.rs:108-138).
+__device__ __forceinline__ float band_eval(const AbsBand& b, float lambda) {
+    float t;
+    if (b.shape == 0) {
+        t = (lambda - b.center_nm) / b.width;
+    } else {
+        const float delta = b.center_nm - lambda;
+        t = (1.0e7f * delta / (lambda * b.center_nm)) / b.width;
+    }
+    return b.peak * expf(-0.5f * t * t);
+}
+
 }  // namespace ixg

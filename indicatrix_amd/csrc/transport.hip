@@ -100,6 +100,7 @@ struct PathState {
     float split[NCH];         // exit-split staging radiance
     float lambda[NCH];
     float n_o[NCH];
+    float alpha_o[NCH], alpha_e[NCH];  // spectral_absorption, absorbing materials only
     uint64_t compat;          // byte c = MIS family of channel c
     V3 org, dir, observer, prev_plane;
     uint32_t seed;
@@ -114,6 +115,7 @@ struct PathState {
 __device__ __forceinline__ uint32_t compat_of(uint64_t c, int ch) { return (uint32_t)(c >> (8 * ch)) & 0xFFu; }
 
 // PathState::new (bounce_body.rs:258-356).
+template <bool ABSORB = false>
 __device__ __forceinline__ void path_init(const FilmParams& P, V3 org, V3 dir, uint32_t seed,
                                           float hero, PathState& S) {
     S.org = org;
@@ -135,6 +137,14 @@ __device__ __forceinline__ void path_init(const FilmParams& P, V3 org, V3 dir, u
         r = r < 0.0f ? r + 400.0f : r;
         S.lambda[c] = 380.0f + r;
         S.n_o[c] = dispersion(P.disp_kind, P.disp, S.lambda[c]);
+        if (ABSORB) {
+            // spectral_absorption (absorption.rs:33): the bands' sum in order
+            float so = -0.0f, se = -0.0f;
+            for (int32_t b = 0; b < P.abs.n_o; ++b) so += band_eval(P.abs.o[b], S.lambda[c]);
+            for (int32_t b = 0; b < P.abs.n_e; ++b) se += band_eval(P.abs.e[b], S.lambda[c]);
+            S.alpha_o[c] = so;
+            S.alpha_e[c] = se;
+        }
         S.path_pdf[c] = 1.0f;
         S.split[c] = 0.0f;
         S.st[c] = make_float4(1.0f, 0.0f, 0.0f, 0.0f);
@@ -165,9 +175,45 @@ __device__ __forceinline__ bool leaves_convex(const float4* __restrict__ planes,
 
 enum StepResult { STEP_CONTINUE = 0, STEP_ESCAPED = 1, STEP_ROULETTE = 2 };
 
+// AbsorptionTensor3::quadratic_form (birefringence/absorption_tensor.rs):
+// alpha . (axes^T e)^2, axes = (a1, a2, c), alpha = (ao, ao, ae).
+__device__ __forceinline__ float tensor_qf(const AbsorbParams& A, float ao, float ae, V3 e) {
+    const float l0 = dot(v3(A.a1[0], A.a1[1], A.a1[2]), e);
+    const float l1 = dot(v3(A.a2[0], A.a2[1], A.a2[2]), e);
+    const float l2 = dot(v3(A.cn[0], A.cn[1], A.cn[2]), e);
+    return (ao * (l0 * l0) + ao * (l1 * l1)) + ae * (l2 * l2);
+}
+
+// apply_absorption (absorption.rs:99-122) through
+// channel_absorption_alphas_assigned's isotropic-by-symmetry branch
+// (absorption.rs:205-222): the mean of the two eigen-directions' quadratic
+// forms (f32::midpoint, computed in f64 like the reference), Beer-Lambert
+// through exp_f32x8's lane polynomial over the scaled segment length.
+__device__ __forceinline__ void apply_absorption(const AbsorbParams& A, PathState& S, V3 k, float path_len) {
+    const V3 c = v3(A.c_raw[0], A.c_raw[1], A.c_raw[2]);
+    V3 ea = cross(k, c);
+    if (dot(ea, ea) > 1e-8f) {
+        ea = normalize(ea);
+    } else {
+        // stable_orthonormal_basis(normalize_or_zero(k)).0 (birefringence/helpers.rs)
+        const V3 n = normalize_or_zero(k);
+        const V3 a = fabsf(n.x) > 0.9f ? v3(0.0f, 1.0f, 0.0f) : v3(1.0f, 0.0f, 0.0f);
+        ea = normalize_or_zero(a - n * dot(n, a));
+    }
+    const V3 eb = normalize_or_zero(cross(k, ea));
+    const float scaled = path_len * A.path_scale;
+#pragma unroll
+    for (int ch = 0; ch < NCH; ++ch) {
+        const float qa = tensor_qf(A, S.alpha_o[ch], S.alpha_e[ch], ea);
+        const float qb = tensor_qf(A, S.alpha_o[ch], S.alpha_e[ch], eb);
+        const float alpha = (float)(((double)qa + (double)qb) * 0.5);
+        S.st[ch] = scale4(S.st[ch], exp_lane(-alpha * scaled));
+    }
+}
+
 // One iteration of the bounce loop (bounce_body.rs:361-707): intersection, and
 // either the escape lookup (written to `radiance`) or the facet event.
-template <bool EVENTS, bool ISO = false>
+template <bool EVENTS, bool ISO = false, bool ABSORB = false>
 __device__ __forceinline__ int path_bounce(const FilmParams& P, const float4* __restrict__ planes,
                                            PathState& S, float radiance[NCH]) {
     const uint32_t bounce = S.bounce;
@@ -221,9 +267,10 @@ __device__ __forceinline__ int path_bounce(const FilmParams& P, const float4* __
     }
     S.prev_plane = cur_plane;
     S.has_prev = true;
-    // apply_interior_segment: absorption-free isotropic media transmit
-    // exp(-0) == 1 exactly, so only the normal flips.
+    // apply_interior_segment (transport/inner.rs:161-178); without bands every
+    // channel transmits exp(-0) == 1 exactly, so only the normal flips.
     const bool inside = S.inside;
+    if (ABSORB && inside) apply_absorption(P.abs, S, k, t_hit);
     if (inside) normal = neg(normal);
 
     // compute_bounce_refraction_geometry (geometry.rs:306-425), isotropic
@@ -430,12 +477,12 @@ __device__ __forceinline__ void path_finish(const FilmParams& P, const PathState
 }
 
 // trace_spectral_ray (transport/mod.rs:118) for one explicit ray, start to end.
-template <bool EVENTS>
+template <bool EVENTS, bool ABSORB>
 __device__ __forceinline__ void trace_path(const FilmParams& P, const float4* __restrict__ planes,
                                            V3 org, V3 dir, const uint32_t seed, const float hero,
                                            PathOut& out, uint32_t* counts) {
     PathState S;
-    path_init(P, org, dir, seed, hero, S);
+    path_init<ABSORB>(P, org, dir, seed, hero, S);
     float radiance[NCH];
 #pragma unroll
     for (int c = 0; c < NCH; ++c) radiance[c] = 0.0f;
@@ -443,7 +490,7 @@ __device__ __forceinline__ void trace_path(const FilmParams& P, const float4* __
     bool escaped = false;
     for (uint32_t b = 0; b < P.max_bounces; ++b) {
         S.bounce = b;
-        int r = path_bounce<EVENTS>(P, planes, S, radiance);
+        int r = path_bounce<EVENTS, false, ABSORB>(P, planes, S, radiance);
         if (r != STEP_CONTINUE) {
             bounces = b + 1;
             term = r == STEP_ESCAPED ? 0u : 2u;
@@ -495,6 +542,7 @@ __device__ __forceinline__ void stage_planes(const FilmParams& P, const float4* 
 }
 
 // Film: one lane per path, raw per-sample XYZ to `out_xyz` (path order).
+template <bool ABSORB>
 __global__ __launch_bounds__(kBlock) void film_transport_kernel(const FilmParams P,
                                                                 const float4* __restrict__ planes,
                                                                 float* __restrict__ out_xyz) {
@@ -510,7 +558,7 @@ __global__ __launch_bounds__(kBlock) void film_transport_kernel(const FilmParams
     float hero;
     camera_sample(P, pixel, sample, org, dir, seed, hero);
     PathOut o;
-    trace_path<false>(P, sp, org, dir, seed, hero, o, nullptr);
+    trace_path<false, ABSORB>(P, sp, org, dir, seed, hero, o, nullptr);
     out_xyz[3 * i + 0] = o.xyz[0];
     out_xyz[3 * i + 1] = o.xyz[1];
     out_xyz[3 * i + 2] = o.xyz[2];
@@ -522,7 +570,7 @@ __global__ __launch_bounds__(kBlock) void film_transport_kernel(const FilmParams
 // (paths run 1..max_bounces iterations, mostly 2-6), and the per-path results
 // are the same as film_transport_kernel's because a path is a pure function
 // of its (pixel, sample) index.
-template <bool ISO>
+template <bool ISO, bool ABSORB>
 __global__ __launch_bounds__(kBlock) IXG_WAVES_ATTR void film_persistent_kernel(const FilmParams P,
                                                                  const float4* __restrict__ planes,
                                                                  float* __restrict__ out_xyz,
@@ -552,7 +600,7 @@ __global__ __launch_bounds__(kBlock) IXG_WAVES_ATTR void film_persistent_kernel(
                     float hero;
                     camera_sample(P, (uint32_t)(p / P.spp), P.sample_offset + (uint32_t)(p % P.spp),
                                   org, dir, seed, hero);
-                    path_init(P, org, dir, seed, hero, S);
+                    path_init<ABSORB>(P, org, dir, seed, hero, S);
 #pragma unroll
                     for (int c = 0; c < NCH; ++c) radiance[c] = 0.0f;
                     active = true;
@@ -564,7 +612,7 @@ __global__ __launch_bounds__(kBlock) IXG_WAVES_ATTR void film_persistent_kernel(
         if (__ballot(active) == 0ull) break;
         if (active) {
             int r = STEP_CONTINUE;
-            if (S.bounce < P.max_bounces) r = path_bounce<false, ISO>(P, sp, S, radiance);
+            if (S.bounce < P.max_bounces) r = path_bounce<false, ISO, ABSORB>(P, sp, S, radiance);
             const bool fin = r != STEP_CONTINUE || ++S.bounce >= P.max_bounces;
             if (fin) {
                 float xyz[3];
@@ -579,6 +627,7 @@ __global__ __launch_bounds__(kBlock) IXG_WAVES_ATTR void film_persistent_kernel(
 }
 
 // Diagnostics: per-path XYZ + integer side channel.
+template <bool ABSORB>
 __global__ __launch_bounds__(kBlock) void film_events_kernel(const FilmParams P,
                                                              const float4* __restrict__ planes,
                                                              float* __restrict__ out_xyz,
@@ -599,7 +648,7 @@ __global__ __launch_bounds__(kBlock) void film_events_kernel(const FilmParams P,
         float hero;
         camera_sample(P, pixel, sample, org, dir, seed, hero);
         PathOut o;
-        trace_path<true>(P, sp, org, dir, seed, hero, o, counts);
+        trace_path<true, ABSORB>(P, sp, org, dir, seed, hero, o, counts);
         counts[EV_PATHS] = 1;
         if (out_xyz) {
             out_xyz[3 * i + 0] = o.xyz[0];
@@ -621,6 +670,7 @@ __global__ __launch_bounds__(kBlock) void film_events_kernel(const FilmParams P,
 
 // Explicit rays (the reference's trace_spectral_ray entry): ray i starts at
 // rays[i] = {ox, oy, oz, dx, dy, dz, hero_rand, seed bits}.
+template <bool ABSORB>
 __global__ __launch_bounds__(kBlock) void film_rays_kernel(const FilmParams P,
                                                            const float4* __restrict__ planes,
                                                            const float* __restrict__ rays,
@@ -633,8 +683,8 @@ __global__ __launch_bounds__(kBlock) void film_rays_kernel(const FilmParams P,
     if (i >= n_rays) return;
     const float* r = rays + 8 * (size_t)i;
     PathOut o;
-    trace_path<true>(P, sp, v3(r[0], r[1], r[2]), v3(r[3], r[4], r[5]), __float_as_uint(r[7]), r[6],
-                     o, nullptr);
+    trace_path<true, ABSORB>(P, sp, v3(r[0], r[1], r[2]), v3(r[3], r[4], r[5]), __float_as_uint(r[7]),
+                             r[6], o, nullptr);
     out_xyz[3 * i + 0] = o.xyz[0];
     out_xyz[3 * i + 1] = o.xyz[1];
     out_xyz[3 * i + 2] = o.xyz[2];
@@ -673,8 +723,12 @@ __global__ __launch_bounds__(kBlock) void film_reduce_kernel(const float* __rest
 extern "C" hipError_t ixg_launch_film(const ixg::FilmParams* P, const float4* d_planes, float* d_xyz,
                                       hipStream_t stream) {
     uint64_t blocks = (P->n_paths + ixg::kBlock - 1) / ixg::kBlock;
-    hipLaunchKernelGGL(ixg::film_transport_kernel, dim3((uint32_t)blocks), dim3(ixg::kBlock), 0,
-                       stream, *P, d_planes, d_xyz);
+    if (P->absorb)
+        hipLaunchKernelGGL(ixg::film_transport_kernel<true>, dim3((uint32_t)blocks), dim3(ixg::kBlock), 0,
+                           stream, *P, d_planes, d_xyz);
+    else
+        hipLaunchKernelGGL(ixg::film_transport_kernel<false>, dim3((uint32_t)blocks), dim3(ixg::kBlock), 0,
+                           stream, *P, d_planes, d_xyz);
     return hipGetLastError();
 }
 
@@ -682,8 +736,12 @@ extern "C" hipError_t ixg_launch_film_events(const ixg::FilmParams* P, const flo
                                              float* d_xyz, ixg_path_events* d_ev,
                                              unsigned long long* d_totals, hipStream_t stream) {
     uint64_t blocks = (P->n_paths + ixg::kBlock - 1) / ixg::kBlock;
-    hipLaunchKernelGGL(ixg::film_events_kernel, dim3((uint32_t)blocks), dim3(ixg::kBlock), 0,
-                       stream, *P, d_planes, d_xyz, d_ev, d_totals);
+    if (P->absorb)
+        hipLaunchKernelGGL(ixg::film_events_kernel<true>, dim3((uint32_t)blocks), dim3(ixg::kBlock), 0,
+                           stream, *P, d_planes, d_xyz, d_ev, d_totals);
+    else
+        hipLaunchKernelGGL(ixg::film_events_kernel<false>, dim3((uint32_t)blocks), dim3(ixg::kBlock), 0,
+                           stream, *P, d_planes, d_xyz, d_ev, d_totals);
     return hipGetLastError();
 }
 
@@ -699,8 +757,12 @@ extern "C" hipError_t ixg_launch_rays(const ixg::FilmParams* P, const float4* d_
                                       uint32_t n_rays, float* d_xyz, ixg_path_events* d_ev,
                                       hipStream_t stream) {
     uint32_t blocks = (n_rays + ixg::kBlock - 1) / ixg::kBlock;
-    hipLaunchKernelGGL(ixg::film_rays_kernel, dim3(blocks), dim3(ixg::kBlock), 0, stream, *P, d_planes,
-                       d_rays, n_rays, d_xyz, d_ev);
+    if (P->absorb)
+        hipLaunchKernelGGL(ixg::film_rays_kernel<true>, dim3(blocks), dim3(ixg::kBlock), 0, stream, *P,
+                           d_planes, d_rays, n_rays, d_xyz, d_ev);
+    else
+        hipLaunchKernelGGL(ixg::film_rays_kernel<false>, dim3(blocks), dim3(ixg::kBlock), 0, stream, *P,
+                           d_planes, d_rays, n_rays, d_xyz, d_ev);
     return hipGetLastError();
 }
 
@@ -708,18 +770,21 @@ extern "C" hipError_t ixg_launch_film_persistent(const ixg::FilmParams* P, const
                                                  uint32_t* d_counter, uint32_t grid_blocks, hipStream_t stream) {
     // a fill kernel rather than a memset node, so the launch pair can be captured
     hipLaunchKernelGGL(ixg::zero_counter_kernel, dim3(1), dim3(1), 0, stream, d_counter);
-    if (P->env.model == ixg::ENV_ISO_HEMISPHERE)
-        hipLaunchKernelGGL(ixg::film_persistent_kernel<true>, dim3(grid_blocks), dim3(ixg::kBlock), 0, stream,
-                           *P, d_planes, d_xyz, d_counter);
-    else
-        hipLaunchKernelGGL(ixg::film_persistent_kernel<false>, dim3(grid_blocks), dim3(ixg::kBlock), 0, stream,
-                           *P, d_planes, d_xyz, d_counter);
+    const bool iso = P->env.model == ixg::ENV_ISO_HEMISPHERE;
+#define IXG_PERSISTENT(ISO, ABS)                                                                              \
+    hipLaunchKernelGGL((ixg::film_persistent_kernel<ISO, ABS>), dim3(grid_blocks), dim3(ixg::kBlock), 0, stream, \
+                       *P, d_planes, d_xyz, d_counter)
+    if (iso && !P->absorb) IXG_PERSISTENT(true, false);
+    else if (iso) IXG_PERSISTENT(true, true);
+    else if (!P->absorb) IXG_PERSISTENT(false, false);
+    else IXG_PERSISTENT(false, true);
+#undef IXG_PERSISTENT
     return hipGetLastError();
 }
 
 extern "C" hipError_t ixg_persistent_grid(int device, uint32_t* blocks) {
     int per_cu = 0, cus = 0;
-    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, ixg::film_persistent_kernel<true>,
+    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, ixg::film_persistent_kernel<true, false>,
                                                                 ixg::kBlock, 0);
     if (e != hipSuccess) return e;
     e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device);

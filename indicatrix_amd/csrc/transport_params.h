@@ -25,6 +25,22 @@ struct EnvParams {
     float wb[3];         // von Kries LMS scale (1,1,1 for D65 presets)
 };
 
+struct AbsBand {
+    float center_nm, width, peak;
+    int32_t shape;  // 0 Gaussian in wavelength, 1 in wavenumber
+};
+
+// Absorption of an isotropic-by-symmetry material: the o / e band lists, the
+// optic axis as given (eigen-directions) and the AbsorptionTensor3 axes.
+struct AbsorbParams {
+    int32_t n_o, n_e;
+    float path_scale;
+    float c_raw[3];
+    float a1[3], a2[3], cn[3];
+    AbsBand o[8];
+    AbsBand e[8];
+};
+
 struct FilmParams {
     // GpuCameraParams fields
     float cam_origin[3];
@@ -45,7 +61,9 @@ struct FilmParams {
     float surface_glare;  // clamped to [0,1]
     uint64_t first_path;  // first (pixel*spp + sample) index of this launch
     uint64_t n_paths;     // paths in this launch
+    int32_t absorb;       // 1: the material has absorption bands
     EnvParams env;
+    AbsorbParams abs;
 };
 
 // Event-count slots accumulated by the counting kernel.

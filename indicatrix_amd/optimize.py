@@ -79,7 +79,7 @@ class ScreenedOptimizer(_Loop):
     """Surrogate screening + tracer verification of the top n_verify."""
 
     def __init__(self, backend, net, nd, abbe, seed=0, n_candidates=8192, n_verify=64, tilts=TILTS_C3,
-                 verify_shape=VERIFY_SHAPE, sigma0=0.15, sigma_decay=0.93, refine_steps=0,
+                 verify_shape=VERIFY_SHAPE, sigma0=0.15, sigma_decay=0.93, refine_steps=100,
                  refine_lr=1e-4):
         super().__init__(backend, nd, abbe, seed, n_candidates, tilts, verify_shape, sigma0, sigma_decay)
         self.net = net

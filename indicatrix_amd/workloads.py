@@ -30,3 +30,18 @@ def golden_scene(preset, light_yaw, light_pitch, max_bounces=12, width=1, height
     env = tracer.LightingPreset.studio(preset, 1.0, light_yaw, light_pitch)
     return tracer.FrameScene(cam, width, height, planes, tracer.GemMaterial.diamond(),
                              max_bounces, env)
+
+
+def film_absorbing_scene(width=32, height=24, max_bounces=16):
+    """The C1 geometry and camera with an absorbing cubic material: two
+    wavelength-domain bands and one wavenumber-domain band, different o / e
+    coefficients and a tilted optic axis (the tensor's unpolarised mean over
+    the two eigen-directions then depends on the ray direction)."""
+    base = film_c1_scene(width, height, max_bounces)
+    m = tracer.GemMaterial.diamond()
+    m.c_axis = (0.48, 0.6, 0.64)
+    m.bands_o = [(450.0, 35.0, 0.9, 0), (610.0, 1800.0, 0.6, 1)]
+    m.bands_e = [(520.0, 55.0, 1.4, 0)]
+    m.absorption_path_scale = 1.3
+    return tracer.FrameScene(base.scene.camera, width, height, base.planes, m, max_bounces,
+                             base.scene.environment)

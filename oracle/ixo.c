@@ -614,6 +614,9 @@ typedef struct {
     uint32_t max_bounces;
     env_t env;
     float surface_glare;
+    int absorbing;      /* any absorption band (o or e) */
+    v3 c_raw;           /* material c_axis as given */
+    v3 ta1, ta2, tcn;   /* AbsorptionTensor3 axes: basis of normalize_or_zero(c_axis) */
 } scene_t;
 
 typedef struct {
@@ -623,6 +626,7 @@ typedef struct {
     float split[NCH];
     float path_pdf[NCH];
     float n_o[NCH];
+    float alpha_o[NCH], alpha_e[NCH]; /* spectral_absorption of o / e bands */
     uint8_t compat[NCH];
     v3 org, dir, k;
     int inside;
@@ -638,6 +642,53 @@ static void ev_facet(ixg_path_events* ev, int32_t facet, uint32_t kind) {
     h = (h ^ (uint32_t)facet) * 16777619u;
     h = (h ^ kind) * 16777619u;
     ev->facet_hash = h;
+}
+
+/* stable_orthonormal_basis (optics/birefringence/helpers.rs) */
+static void stable_basis(v3 n, v3* t, v3* b) {
+    v3 a = fabsf(n.x) > 0.9f ? mk(0.0f, 1.0f, 0.0f) : mk(1.0f, 0.0f, 0.0f);
+    *t = vnormalize_or_zero(vsub(a, vmul(n, vdot(n, a))));
+    *b = vcross(n, *t);
+}
+
+/* BirefringenceParams::ordinary_eigen_polarization (optics/birefringence/uniaxial.rs) */
+static v3 ordinary_eigen(v3 k, v3 c) {
+    v3 x = vcross(k, c);
+    if (vlen_sq(x) > 1e-8f) return vnormalize(x);
+    v3 t, b;
+    stable_basis(vnormalize_or_zero(k), &t, &b);
+    return t;
+}
+
+/* spectral_absorption (RT/absorption.rs:33): the bands' sum in order */
+static float band_sum(const ixg_band* bands, int32_t n, float lambda) {
+    float s = -0.0f;
+    for (int32_t i = 0; i < n; ++i) s += ixo_band_evaluate(&bands[i], lambda);
+    return s;
+}
+
+/* AbsorptionTensor3::quadratic_form (optics/birefringence/absorption_tensor.rs):
+ * alpha . (axes^T e)^2 with axes = (a1, a2, c) and alpha = (ao, ao, ae) */
+static float tensor_qf(const scene_t* S, float ao, float ae, v3 e) {
+    float l0 = vdot(S->ta1, e), l1 = vdot(S->ta2, e), l2 = vdot(S->tcn, e);
+    return (ao * (l0 * l0) + ao * (l1 * l1)) + ae * (l2 * l2);
+}
+
+/* apply_absorption (RT/absorption.rs:99-122) with
+ * channel_absorption_alphas_assigned's isotropic-by-symmetry branch
+ * (:205-222): the unpolarised mean of the two eigen-directions' quadratic
+ * forms, Beer-Lambert through exp_f32x8 over the scaled segment length. */
+static void apply_absorption(const scene_t* S, path_t* P, float path_len) {
+    v3 ea = ordinary_eigen(P->k, S->c_raw);
+    v3 eb = vnormalize_or_zero(vcross(P->k, ea));
+    float scaled = path_len * S->mat->absorption_path_scale;
+    for (int c = 0; c < NCH; ++c) {
+        float qa = tensor_qf(S, P->alpha_o[c], P->alpha_e[c], ea);
+        float qb = tensor_qf(S, P->alpha_o[c], P->alpha_e[c], eb);
+        float alpha = (float)(((double)qa + (double)qb) / 2.0); /* f32::midpoint */
+        float tr = ixo_exp_lane(-alpha * scaled);
+        P->st[c] = st_scale(P->st[c], tr);
+    }
 }
 
 static float tir_phase_delta(float n1k, float cos_i, float sin_i) { /* refraction/tir.rs:33-37 */
@@ -677,8 +728,9 @@ static void facet_event(const scene_t* S, path_t* P, uint32_t bounce, uint32_t s
     }
     P->prev_plane = cur_plane;
     P->has_prev = 1;
-    /* apply_interior_segment (transport/inner.rs:139-155): absorption-free
-     * isotropic materials transmit exp(-0) == 1 exactly, a no-op. */
+    /* apply_interior_segment (transport/inner.rs:161-178); without bands
+     * every channel transmits exp(-0) == 1 exactly, a no-op. */
+    if (P->inside && S->absorbing) apply_absorption(S, P, t_hit);
     if (P->inside) normal = vneg(normal);
 
     /* compute_bounce_refraction_geometry (refraction/geometry.rs:306-425), isotropic */
@@ -858,6 +910,10 @@ static int scene_build(const ixg_scene* sc, scene_t* S) {
     S->max_bounces = sc->max_bounces;
     float g = sc->environment.surface_glare;
     S->surface_glare = (g >= 1.0f || isnan(g)) ? 1.0f : (g > 0.0f ? g : 0.0f);
+    S->absorbing = sc->material->n_bands_o > 0 || sc->material->n_bands_e > 0;
+    S->c_raw = vfrom(sc->material->c_axis);
+    S->tcn = vnormalize_or_zero(S->c_raw);
+    stable_basis(S->tcn, &S->ta1, &S->ta2);
     return env_build(&sc->environment, &S->env);
 }
 
@@ -871,6 +927,10 @@ static void trace_core(const scene_t* S, v3 origin, v3 dir, uint32_t seed, float
         P.path_pdf[c] = 1.0f;
         P.compat[c] = 0xFF;
         P.n_o[c] = ixo_dispersion_evaluate(&S->mat->dispersion, P.lambdas[c]);
+        if (S->absorbing) {
+            P.alpha_o[c] = band_sum(S->mat->bands_o, S->mat->n_bands_o, P.lambdas[c]);
+            P.alpha_e[c] = band_sum(S->mat->bands_e, S->mat->n_bands_e, P.lambdas[c]);
+        }
     }
     P.org = origin;
     P.dir = dir;

@@ -44,6 +44,8 @@ def lib():
         L.ixo_d65.argtypes = [C.c_float]
         L.ixo_exp_lane.restype = C.c_float
         L.ixo_exp_lane.argtypes = [C.c_float]
+        L.ixo_band_evaluate.restype = C.c_float
+        L.ixo_band_evaluate.argtypes = [P(abi.Band), C.c_float]
         L.ixo_sample_studio_environment.restype = C.c_float
         L.ixo_sample_studio_environment.argtypes = [P(C.c_float), C.c_float, C.c_int32,
                                                     C.c_float, C.c_float, C.c_float]

@@ -77,6 +77,10 @@ def test_route_predicate():
     sc4 = tracer.FrameScene(sc.scene.camera, 8, 8, big, tracer.GemMaterial.diamond(), 16,
                             sc.scene.environment)
     assert tracer.route(sc4) == abi.ROUTE_DEVICE_LIMITS
+    assert tracer.route(workloads.film_absorbing_scene(8, 8)) == abi.ROUTE_OK
+    sc5 = workloads.film_absorbing_scene(8, 8)
+    sc5.material_abi.n_bands_o = abi.MAX_BANDS + 1      # more bands than a ray carries
+    assert tracer.route(sc5) == abi.ROUTE_UNSUPPORTED_MATERIAL
 
 
 def test_create_reports_status_without_crashing():

@@ -77,6 +77,25 @@ def test_paths_match_oracle(backend):
     assert np.median(ulp_dist(g_xyz[same], o_xyz[same])) <= 2
 
 
+def test_absorbing_paths_match_oracle(backend):
+    """Beer-Lambert along every interior segment (bands in both domains,
+    o != e coefficients, tilted optic axis): same integer side channel, XYZ
+    within the budget of the device's expf inside the band sums."""
+    sc = workloads.film_absorbing_scene(32, 24)
+    spp = 8
+    print("trace_paths absorbing 32x24x8", flush=True)
+    g_xyz, g_ev = backend.trace_paths(sc, 0, spp)
+    o_xyz, o_ev = oracle_lib.trace_paths(sc, 0, spp)
+    same = ((g_ev["bounces"] == o_ev["bounces"]) & (g_ev["termination"] == o_ev["termination"])
+            & (g_ev["facet_hash"] == o_ev["facet_hash"]))
+    assert 1.0 - same.mean() < 1e-3
+    a, b = g_xyz[same].astype(np.float64), o_xyz[same].astype(np.float64)
+    scale = np.maximum(np.abs(b).max(axis=1, keepdims=True), 1e-6)
+    assert (np.abs(a - b) / scale).max() < 1e-4
+    clear_xyz, _ = backend.trace_paths(workloads.film_c1_scene(32, 24), 0, spp)
+    assert g_xyz.sum() < 0.9 * clear_xyz.sum()      # the bands do absorb
+
+
 def test_accumulate_matches_oracle_sums(backend):
     sc = small_scene(24, 16)
     acc = np.zeros((24 * 16, 3), np.float64)

@@ -318,3 +318,106 @@ def test_high_dispersion_has_more_fire():
     _, tl, _ = oracle_lib.metric_image(_metric_request(lo, samples=8))
     _, th, _ = oracle_lib.metric_image(_metric_request(hi, samples=8))
     assert th.fire_index > 2.0 * tl.fire_index
+
+
+# ---------------------------------------------------------------------------
+# absorption (optics/absorption.rs, RT/absorption.rs): bands and Beer-Lambert
+# ---------------------------------------------------------------------------
+def test_band_closed_forms():
+    """AbsorptionBand::evaluate: peak at the centre, exp(-1/2) one width away,
+    in both the wavelength and the energy (wavenumber) domain."""
+    L = oracle_lib.lib()
+    b = abi.Band(550.0, 30.0, 1.7, 0)
+    assert L.ixo_band_evaluate(C.byref(b), 550.0) == f32(1.7)
+    assert L.ixo_band_evaluate(C.byref(b), 580.0) == pytest.approx(1.7 * math.exp(-0.5), rel=2e-7)
+    assert L.ixo_band_evaluate(C.byref(b), 520.0) == pytest.approx(1.7 * math.exp(-0.5), rel=2e-7)
+    e = abi.Band(600.0, 1500.0, 0.9, 1)
+    assert L.ixo_band_evaluate(C.byref(e), 600.0) == f32(0.9)
+    lam = 1e7 / (1e7 / 600.0 + 1500.0)   # one sigma in wavenumber
+    assert L.ixo_band_evaluate(C.byref(e), lam) == pytest.approx(0.9 * math.exp(-0.5), rel=2e-5)
+
+
+CUBE = [((1, 0, 0), -1), ((-1, 0, 0), -1), ((0, 1, 0), -1), ((0, -1, 0), -1), ((0, 0, 1), -1),
+        ((0, 0, -1), -1)]
+
+
+def _cube_scene(mat, max_bounces=12):
+    planes = (abi.FacetPlane * 6)()
+    for i, (n, d) in enumerate(CUBE):
+        planes[i] = abi.FacetPlane((C.c_float * 3)(*n), d)
+    sc = abi.Scene()
+    sc.planes = C.cast(planes, C.POINTER(abi.FacetPlane))
+    sc.n_planes = 6
+    sc.material = C.pointer(mat)
+    sc.max_bounces = max_bounces
+    sc.environment = abi.Environment(abi.PRESETS["IsoHemisphere"], 1.0, 0.0, 0.0, 0.0, 1.0, 0.0)
+    sc._keep = (planes, mat)
+    return sc
+
+
+def _flat_bands(mat, alpha_o, alpha_e, c_axis=(0.0, 1.0, 0.0), scale=1.0):
+    """Bands so wide that every channel's coefficient is exactly the peak."""
+    mat.n_bands_o = 1
+    mat.n_bands_e = 1
+    mat.bands_o[0] = abi.Band(550.0, 1e7, alpha_o, 0)
+    mat.bands_e[0] = abi.Band(550.0, 1e7, alpha_e, 0)
+    for i in range(3):
+        mat.c_axis[i] = c_axis[i]
+    mat.absorption_path_scale = scale
+    return mat
+
+
+def _straight_paths(sc, seeds):
+    """Y of rays sent straight down the z axis through the cube, keyed by seed,
+    for the paths that enter and leave once (no internal reflection)."""
+    out = {}
+    for seed in seeds:
+        xyz = (C.c_float * 3)()
+        ev = abi.PathEvents()
+        oracle_lib.lib().ixo_trace_ray(C.byref(sc), (C.c_float * 3)(0.1, 0.2, 5.0),
+                                       (C.c_float * 3)(0.0, 0.0, -1.0), seed, hero_of(seed), xyz,
+                                       C.byref(ev))
+        if ev.n_refract == 2 and ev.n_reflect == 0 and ev.n_tir == 0 and ev.termination == 0:
+            out[seed] = (xyz[1], ev.facet_hash)
+    return out
+
+
+@pytest.mark.parametrize("c_axis,alpha", [((0.0, 0.0, 1.0), 0.2), ((1.0, 0.0, 0.0), 0.4),
+                                          ((0.0, 1.0, 0.0), 0.4)])
+def test_beer_lambert_through_a_slab(c_axis, alpha):
+    """A ray crossing the 2-unit cube once transmits exp(-alpha L) with
+    L = 2 - 1e-4 (the self-hit offset).  With alpha_o = 0.2, alpha_e = 0.6 in a
+    cubic material the coefficient is the unpolarised mean of the two eigen-
+    directions' quadratic forms: alpha_o along the optic axis, the midpoint
+    (0.4) across it."""
+    clear = _straight_paths(_cube_scene(diamond_material()), range(64))
+    mat = _flat_bands(diamond_material(), 0.2, 0.6, c_axis)
+    dark = _straight_paths(_cube_scene(mat), range(64))
+    assert len(clear) >= 10 and clear.keys() == dark.keys()
+    expect = math.exp(-alpha * (2.0 - 1e-4))
+    for s in clear:
+        assert dark[s][1] == clear[s][1]                 # same facet sequence
+        assert dark[s][0] / clear[s][0] == pytest.approx(expect, rel=2e-6)
+
+
+def test_absorption_path_scale_squares_the_transmittance():
+    clear = _straight_paths(_cube_scene(diamond_material()), range(48))
+    one = _straight_paths(_cube_scene(_flat_bands(diamond_material(), 0.3, 0.3)), range(48))
+    two = _straight_paths(_cube_scene(_flat_bands(diamond_material(), 0.3, 0.3, scale=2.0)), range(48))
+    for s in clear:
+        r1 = one[s][0] / clear[s][0]
+        r2 = two[s][0] / clear[s][0]
+        assert r2 == pytest.approx(r1 * r1, rel=4e-6)
+
+
+def test_zero_peak_bands_are_exactly_transparent():
+    sc0 = _cube_scene(diamond_material())
+    sc1 = _cube_scene(_flat_bands(diamond_material(), 0.0, 0.0))
+    for seed in range(32):
+        a = (C.c_float * 3)()
+        b = (C.c_float * 3)()
+        d = (C.c_float * 3)(0.3, -0.2, -1.0)
+        oracle_lib.lib().ixo_trace_ray(C.byref(sc0), (C.c_float * 3)(0.1, 0.2, 5.0), d, seed, hero_of(seed), a, None)
+        oracle_lib.lib().ixo_trace_ray(C.byref(sc1), (C.c_float * 3)(0.1, 0.2, 5.0), d, seed, hero_of(seed), b, None)
+        assert [bits(v) for v in a] == [bits(v) for v in b]
+
