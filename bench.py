@@ -24,6 +24,10 @@ sys.path.insert(0, ROOT)
 
 MI355X_FP32_PEAK_TFLOPS = 157.3
 MI355X_HBM_PEAK_GBS = 8000.0
+# HBM bytes per film_persistent_kernel launch at this workload: FETCH_SIZE x 2.0
+# + WRITE_SIZE x 1.0 (scales from a 1 GiB copy) from two rocprofv3 --pmc runs,
+# recorded with their commands in profiles/film_persistent_kernel_c1.md.
+FILM_KERNEL_HBM_BYTES = (1633 * 2.0 + 18261) * 1024
 
 
 def parse():
@@ -216,7 +220,7 @@ def main():
         "roofline": {"bound": "mfma", "pipe": "fp32-valu (non-matrix FP32)",
                      "achieved": round(achieved, 3), "peak": MI355X_FP32_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": round(achieved / MI355X_FP32_PEAK_TFLOPS, 4),
-                     "traffic": None,
+                     "traffic": round(FILM_KERNEL_HBM_BYTES),
                      "kernel": "film_persistent_kernel", "kernel_ms": round(avg_kernel_ms, 4),
                      "flops_per_launch": step_flops, "events": totals},
     }

@@ -6,9 +6,9 @@
 // ones replayed at the F and C lines for Fire, and the pixel centre re-traced
 // from five yaw-offset cameras for temporal scintillation.
 //
-// Device layout: a 2-D grid, blockIdx.y = image (design x camera pose),
-// blockIdx.x = a 256-pixel tile of it; the image's facet set is staged in LDS
-// once per block.  One lane owns one pixel and writes its OpticalPixel; a
+// Device layout: either a 2-D grid (blockIdx.y = image = design x camera
+// pose, blockIdx.x = a 256-pixel tile of it) or a persistent work queue over
+// (image, tile) items; the image's facet set is staged in LDS.  One lane owns one pixel and writes its OpticalPixel; a
 // second kernel reduces each image in raster order (one lane per image), so
 // every f32 sum is formed in the reference's order.
 //
@@ -215,18 +215,10 @@ __device__ __forceinline__ void sample_offset(uint32_t s, float& jx, float& jy) 
     jy = (float)y;
 }
 
-__global__ __launch_bounds__(kMBlock) void metric_pixels_kernel(const MetricShared S,
-                                                                const MetricImage* __restrict__ images,
-                                                                const float4* __restrict__ planes,
-                                                                ixg_optical_pixel* __restrict__ out) {
-    __shared__ float4 sp[IXG_MAX_PLANES];
-    const uint32_t img = blockIdx.y;
-    const MetricImage& I = images[img];
-    for (uint32_t i = threadIdx.x; i < I.n_planes; i += blockDim.x) sp[i] = planes[I.plane_offset + i];
-    __syncthreads();
-    const uint32_t npx = S.width * S.height;
-    const uint32_t p = blockIdx.x * kMBlock + threadIdx.x;
-    if (p >= npx) return;
+// One pixel p of image I (sample_pixel, image.rs:110-175, plus the temporal
+// sampler): the OpticalPixel the reference's evaluator produces for it.
+__device__ __forceinline__ ixg_optical_pixel metric_pixel(const MetricShared& S, const MetricImage& I,
+                                                          const float4* __restrict__ sp, uint32_t p) {
     const uint32_t x = p % S.width, y = p / S.width;
     const float pw = (float)S.proj_w, ph = (float)S.proj_h;
     const V3 org = v3(I.origin[0], I.origin[1], I.origin[2]);
@@ -299,8 +291,64 @@ __global__ __launch_bounds__(kMBlock) void metric_pixels_kernel(const MetricShar
             px.scint_poses = 5;
         }
     }
-    out[(size_t)img * npx + p] = px;
+    return px;
 }
+
+// 2-D grid: blockIdx.y = image, blockIdx.x = a kMBlock-pixel tile.
+__global__ __launch_bounds__(kMBlock) void metric_pixels_kernel(const MetricShared S,
+                                                                const MetricImage* __restrict__ images,
+                                                                const float4* __restrict__ planes,
+                                                                ixg_optical_pixel* __restrict__ out) {
+    __shared__ float4 sp[IXG_MAX_PLANES];
+    const uint32_t img = blockIdx.y;
+    const MetricImage& I = images[img];
+    for (uint32_t i = threadIdx.x; i < I.n_planes; i += blockDim.x) sp[i] = planes[I.plane_offset + i];
+    __syncthreads();
+    const uint32_t npx = S.width * S.height;
+    const uint32_t p = blockIdx.x * kMBlock + threadIdx.x;
+    if (p >= npx) return;
+    out[(size_t)img * npx + p] = metric_pixel(S, I, sp, p);
+}
+
+// Persistent work queue over (image, tile) items, image-major: a resident
+// block takes the next item from a global counter, restages the facet set in
+// LDS only when the image changes (consecutive items mostly share a design),
+// and runs the tile's pixels one per lane, TILE / BLOCK passes per item.  The
+// item count bounds the loop, so every block leaves once the queue is empty.
+template <int BLOCK, int TILE>
+__global__ __launch_bounds__(BLOCK) void metric_queue_kernel(const MetricShared S,
+                                                             const MetricImage* __restrict__ images,
+                                                             const float4* __restrict__ planes,
+                                                             ixg_optical_pixel* __restrict__ out,
+                                                             uint32_t* __restrict__ counter, uint32_t n_items) {
+    __shared__ float4 sp[IXG_MAX_PLANES];
+    __shared__ uint32_t s_item;
+    const uint32_t npx = S.width * S.height;
+    const uint32_t tiles = (npx + TILE - 1) / TILE;
+    uint32_t cur = 0xFFFFFFFFu;
+    while (true) {
+        if (threadIdx.x == 0) s_item = atomicAdd(counter, 1u);
+        __syncthreads();
+        const uint32_t item = s_item;
+        if (item >= n_items) break;
+        const uint32_t img = item / tiles, tile = item % tiles;
+        const MetricImage& I = images[img];
+        if (img != cur) {
+            const MetricImage& J = images[img];
+            for (uint32_t i = threadIdx.x; i < J.n_planes; i += BLOCK) sp[i] = planes[J.plane_offset + i];
+            cur = img;
+        }
+        __syncthreads();
+#pragma unroll 1
+        for (uint32_t k = 0; k < TILE; k += BLOCK) {
+            const uint32_t p = tile * TILE + k + threadIdx.x;
+            if (p < npx) out[(size_t)img * npx + p] = metric_pixel(S, I, sp, p);
+        }
+        __syncthreads();  // s_item and sp are rewritten by the next item
+    }
+}
+
+__global__ void metric_zero_counter_kernel(uint32_t* c) { *c = 0u; }
 
 // Per-image totals and metrics in raster order (one lane per image).
 __global__ __launch_bounds__(64) void metric_reduce_kernel(const MetricShared S,
@@ -387,6 +435,54 @@ extern "C" hipError_t ixg_launch_metric_pixels(const ixg::MetricShared* S, const
     hipLaunchKernelGGL(ixg::metric_pixels_kernel, grid, dim3(ixg::kMBlock), 0, stream, *S, d_images,
                        d_planes, d_out);
     return hipGetLastError();
+}
+
+// Queue shapes the autotuner chooses among: (block lanes, pixels per item).
+extern "C" int ixg_metric_queue_shapes(int* block, int* tile, int max_n) {
+    static const int shapes[][2] = {{64, 64}, {64, 256}, {128, 128}, {128, 512}, {256, 256}, {256, 1024}};
+    const int n = (int)(sizeof shapes / sizeof shapes[0]);
+    for (int i = 0; i < n && i < max_n; ++i) {
+        block[i] = shapes[i][0];
+        tile[i] = shapes[i][1];
+    }
+    return n;
+}
+
+extern "C" hipError_t ixg_launch_metric_queue(const ixg::MetricShared* S, const ixg::MetricImage* d_images,
+                                              const float4* d_planes, ixg_optical_pixel* d_out, uint32_t* d_counter,
+                                              int block, int tile, uint32_t grid_blocks, hipStream_t stream) {
+    const uint32_t npx = S->width * S->height;
+    const uint32_t n_items = S->n_images * ((npx + tile - 1) / tile);
+    hipLaunchKernelGGL(ixg::metric_zero_counter_kernel, dim3(1), dim3(1), 0, stream, d_counter);
+#define IXG_QUEUE(B, T)                                                                                        \
+    if (block == B && tile == T) {                                                                             \
+        hipLaunchKernelGGL((ixg::metric_queue_kernel<B, T>), dim3(grid_blocks), dim3(B), 0, stream, *S, d_images, \
+                           d_planes, d_out, d_counter, n_items);                                               \
+        return hipGetLastError();                                                                              \
+    }
+    IXG_QUEUE(64, 64)
+    IXG_QUEUE(64, 256)
+    IXG_QUEUE(128, 128)
+    IXG_QUEUE(128, 512)
+    IXG_QUEUE(256, 256)
+    IXG_QUEUE(256, 1024)
+#undef IXG_QUEUE
+    return hipErrorInvalidValue;
+}
+
+// Resident blocks per CU of a queue shape (the grid is this x CUs x fill).
+extern "C" hipError_t ixg_metric_queue_occupancy(int block, int tile, int* per_cu) {
+#define IXG_OCC(B, T)                                                                                 \
+    if (block == B && tile == T)                                                                      \
+        return hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, ixg::metric_queue_kernel<B, T>, B, 0);
+    IXG_OCC(64, 64)
+    IXG_OCC(64, 256)
+    IXG_OCC(128, 128)
+    IXG_OCC(128, 512)
+    IXG_OCC(256, 256)
+    IXG_OCC(256, 1024)
+#undef IXG_OCC
+    return hipErrorInvalidValue;
 }
 
 extern "C" hipError_t ixg_launch_metric_reduce(const ixg::MetricShared* S, const ixg_optical_pixel* d_px,
