@@ -277,6 +277,19 @@ ixg_status ixg_metric_sweep(ixg_ctx* ctx, const ixg_facet_plane* planes, uint32_
                             uint32_t n_tilts, uint32_t width, uint32_t height, uint32_t samples,
                             float distance, const ixg_environment* env, float* out_metrics);
 
+/* How ixg_metric_sweep launches (no reference counterpart: the reference
+ * evaluates metrics on the CPU).  use_graph = 1 captures the sweep's kernels
+ * in a HIP graph that later calls of the same shape replay; block = 0 lets
+ * the autotuner time the 2-D grid and each persistent work-queue shape once
+ * per image size and keep the fastest, block = -1 forces the 2-D grid, and
+ * block > 0 with tile forces one queue shape (lanes per block, pixels per
+ * queue item: 64/64, 64/256, 128/128, 128/512, 256/256, 256/1024).
+ * Defaults: use_graph 1, block 0. */
+ixg_status ixg_set_metric_mode(ixg_ctx* ctx, int use_graph, int block, int tile);
+/* The shape the last sweep ran with (block -1 = 2-D grid) and whether it was
+ * replayed from a graph. */
+ixg_status ixg_metric_tuning(ixg_ctx* ctx, int* block, int* tile, uint32_t* grid_blocks, int* graphed);
+
 /* ---- metric surrogate (new work; the reference has no surrogate) ----
  * A BF16 MLP 16 -> 4 x 256 (GELU) -> 3, FP32 master weights and Adam,
  * trained and evaluated with MFMA kernels.  x, t, y are device pointers:

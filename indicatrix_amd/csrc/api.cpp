@@ -53,6 +53,9 @@ struct ixg_ctx {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     double last_kernel_ms_total = 0.0;
     uint32_t last_kernel_launches = 0;
+    // the metric sweep's persistent buffers, captured graph and tuning cache
+    void* metric_plan = nullptr;
+    void (*metric_plan_free)(void*) = nullptr;
 };
 
 namespace {
@@ -277,6 +280,7 @@ void ixg_destroy(ixg_ctx* ctx) {
     if (!ctx) return;
     (void)hipSetDevice(ctx->device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    if (ctx->metric_plan && ctx->metric_plan_free) ctx->metric_plan_free(ctx->metric_plan);
     if (ctx->d_planes) (void)hipFree(ctx->d_planes);
     if (ctx->d_xyz) (void)hipFree(ctx->d_xyz);
     if (ctx->d_counter) (void)hipFree(ctx->d_counter);
@@ -457,6 +461,10 @@ int ixg_internal_device(ixg_ctx* ctx) { return ctx->device; }
 ixg_status ixg_internal_fail(ixg_ctx* ctx, ixg_status st, const char* what) { return fail(ctx, st, what); }
 ixg_status ixg_internal_hip_fail(ixg_ctx* ctx, hipError_t e, const char* where) { return hip_fail(ctx, e, where); }
 int ixg_internal_lost(ixg_ctx* ctx) { return ctx->lost ? 1 : 0; }
+void** ixg_internal_metric_plan(ixg_ctx* ctx, void (*free_fn)(void*)) {
+    if (free_fn) ctx->metric_plan_free = free_fn;
+    return &ctx->metric_plan;
+}
 
 double ixg_last_kernel_ms(const ixg_ctx* ctx) {
     if (!ctx || ctx->last_kernel_launches == 0) return 0.0;

@@ -8,8 +8,10 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstring>
+#include <map>
 #include <vector>
 
 #include "../../include/ixg.h"
@@ -21,6 +23,11 @@ extern "C" hipError_t ixg_launch_metric_pixels(const ixg::MetricShared* S, const
                                                hipStream_t stream);
 extern "C" hipError_t ixg_launch_metric_reduce(const ixg::MetricShared* S, const ixg_optical_pixel* d_px,
                                                ixg_metric_totals* d_totals, hipStream_t stream);
+extern "C" hipError_t ixg_launch_metric_queue(const ixg::MetricShared* S, const ixg::MetricImage* d_images,
+                                              const float4* d_planes, ixg_optical_pixel* d_out, uint32_t* d_counter,
+                                              int block, int tile, uint32_t grid_blocks, hipStream_t stream);
+extern "C" hipError_t ixg_metric_queue_occupancy(int block, int tile, int* per_cu);
+extern "C" int ixg_metric_queue_shapes(int* block, int* tile, int max_n);
 extern "C" hipError_t ixg_launch_scoring_probe(const ixg::MetricShared* S, const float* d_dirs, uint32_t n,
                                                float* d_out, hipStream_t stream);
 
@@ -30,6 +37,7 @@ extern "C" int ixg_internal_device(ixg_ctx* ctx);
 extern "C" ixg_status ixg_internal_fail(ixg_ctx* ctx, ixg_status st, const char* what);
 extern "C" ixg_status ixg_internal_hip_fail(ixg_ctx* ctx, hipError_t e, const char* where);
 extern "C" int ixg_internal_lost(ixg_ctx* ctx);
+extern "C" void** ixg_internal_metric_plan(ixg_ctx* ctx, void (*free_fn)(void*));
 
 namespace {
 
@@ -169,6 +177,231 @@ ixg_status run_images(ixg_ctx* ctx, const MetricShared& S, const std::vector<Met
     return IXG_OK;
 }
 
+// ---------------------------------------------------------------------------
+// The sweep plan: device buffers that persist across calls (grown on demand),
+// the launch shape chosen by the autotuner for each image size, and the
+// captured graph of [counter fill, pixel kernel, reduction] that later calls
+// with the same shape replay after copying in their own designs.
+// ---------------------------------------------------------------------------
+struct Shape {
+    int block = 0;  // -1: 2-D grid kernel; otherwise the queue kernel's lanes per block
+    int tile = 0;   // pixels per queue item
+    uint32_t grid = 0;
+};
+
+struct SweepPlan {
+    int cus = 0;
+    int use_graph = 1;
+    Shape forced;  // block 0 = autotune
+    float4* d_pl = nullptr;
+    size_t cap_pl = 0;
+    MetricImage* d_img = nullptr;
+    size_t cap_img = 0;
+    ixg_optical_pixel* d_px = nullptr;
+    size_t cap_px = 0;
+    ixg_metric_totals* d_tot = nullptr;
+    size_t cap_tot = 0;
+    uint32_t* d_counter = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    std::vector<uint8_t> graph_key;
+    std::map<std::array<uint32_t, 3>, Shape> tuned;
+    Shape last;
+    int last_graphed = 0;
+    std::vector<float> tune_ms;  // per candidate, last autotune
+};
+
+void drop_graph(SweepPlan* P) {
+    if (P->exec) (void)hipGraphExecDestroy(P->exec);
+    if (P->graph) (void)hipGraphDestroy(P->graph);
+    P->exec = nullptr;
+    P->graph = nullptr;
+    P->graph_key.clear();
+}
+
+void plan_free(void* p) {
+    SweepPlan* P = static_cast<SweepPlan*>(p);
+    drop_graph(P);
+    void* bufs[] = {P->d_pl, P->d_img, P->d_px, P->d_tot, P->d_counter};
+    for (void* b : bufs)
+        if (b) (void)hipFree(b);
+    delete P;
+}
+
+SweepPlan* get_plan(ixg_ctx* ctx) {
+    void** slot = ixg_internal_metric_plan(ctx, plan_free);
+    if (!*slot) {
+        SweepPlan* P = new SweepPlan();
+        if (hipDeviceGetAttribute(&P->cus, hipDeviceAttributeMultiprocessorCount, ixg_internal_device(ctx)) !=
+            hipSuccess)
+            P->cus = 1;
+        *slot = P;
+    }
+    return static_cast<SweepPlan*>(*slot);
+}
+
+template <class T>
+hipError_t grow(T** buf, size_t* cap, size_t n, SweepPlan* P) {
+    if (n <= *cap) return hipSuccess;
+    drop_graph(P);  // the graph holds the old addresses
+    if (*buf) (void)hipFree(*buf);
+    *buf = nullptr;
+    *cap = 0;
+    hipError_t e = hipMalloc(buf, n * sizeof(T));
+    if (e == hipSuccess) *cap = n;
+    return e;
+}
+
+// Pixel kernel(s) + reduction for n images already in the plan's buffers.
+hipError_t enqueue_sweep(SweepPlan* P, const MetricShared& S, size_t n_images, const Shape& sh, hipStream_t st) {
+    const size_t npx = (size_t)S.width * S.height;
+    hipError_t e = hipSuccess;
+    if (sh.block < 0) {
+        for (size_t first = 0; e == hipSuccess && first < n_images; first += 65535) {
+            MetricShared Sc = S;
+            Sc.n_images = (uint32_t)std::min<size_t>(65535, n_images - first);
+            e = ixg_launch_metric_pixels(&Sc, P->d_img + first, P->d_pl, P->d_px + first * npx, st);
+        }
+    } else {
+        MetricShared Sc = S;
+        Sc.n_images = (uint32_t)n_images;
+        e = ixg_launch_metric_queue(&Sc, P->d_img, P->d_pl, P->d_px, P->d_counter, sh.block, sh.tile, sh.grid, st);
+    }
+    for (size_t first = 0; e == hipSuccess && first < n_images; first += 65535) {
+        MetricShared Sc = S;
+        Sc.n_images = (uint32_t)std::min<size_t>(65535, n_images - first);
+        e = ixg_launch_metric_reduce(&Sc, P->d_px + first * npx, P->d_tot + first, st);
+    }
+    return e;
+}
+
+// Times every candidate shape on the first (up to) 128 images of this call
+// and keeps the fastest for this image size: the 2-D grid and each queue
+// shape at full residency.  A one-image pass over every candidate first
+// loads each kernel's code, so the timed passes measure only the work.
+hipError_t autotune(SweepPlan* P, const MetricShared& S, size_t n_images, hipStream_t st, Shape* best) {
+    const size_t n = std::min<size_t>(n_images, 128);
+    std::vector<Shape> cands;
+    cands.push_back(Shape{-1, 0, 0});
+    int blocks[16], tiles[16];
+    int nq = ixg_metric_queue_shapes(blocks, tiles, 16);
+    for (int i = 0; i < nq; ++i) {
+        int per_cu = 0;
+        if (ixg_metric_queue_occupancy(blocks[i], tiles[i], &per_cu) != hipSuccess || per_cu <= 0) continue;
+        cands.push_back(Shape{blocks[i], tiles[i], (uint32_t)(per_cu * P->cus)});
+    }
+    hipEvent_t a = nullptr, b = nullptr;
+    hipError_t e = hipEventCreate(&a);
+    if (e == hipSuccess) e = hipEventCreate(&b);
+    float best_ms = 1e30f;
+    P->tune_ms.assign(cands.size(), -1.0f);
+    for (size_t c = 0; e == hipSuccess && c < cands.size(); ++c) e = enqueue_sweep(P, S, 1, cands[c], st);
+    for (size_t c = 0; e == hipSuccess && c < cands.size(); ++c) {
+        e = hipEventRecord(a, st);
+        if (e == hipSuccess) e = enqueue_sweep(P, S, n, cands[c], st);
+        if (e == hipSuccess) e = hipEventRecord(b, st);
+        if (e == hipSuccess) e = hipEventSynchronize(b);
+        float ms = 0.0f;
+        if (e == hipSuccess) e = hipEventElapsedTime(&ms, a, b);
+        if (e == hipSuccess) {
+            P->tune_ms[c] = ms;
+            if (ms < best_ms) {
+                best_ms = ms;
+                *best = cands[c];
+            }
+        }
+    }
+    if (a) (void)hipEventDestroy(a);
+    if (b) (void)hipEventDestroy(b);
+    return e;
+}
+
+ixg_status run_sweep(ixg_ctx* ctx, const MetricShared& S, const std::vector<MetricImage>& images,
+                     const ixg_facet_plane* planes, uint32_t total_planes, std::vector<ixg_metric_totals>& totals) {
+    hipStream_t st = ixg_internal_stream(ctx);
+    SweepPlan* P = get_plan(ctx);
+    const size_t npx = (size_t)S.width * S.height;
+    const size_t n = images.size();
+    std::vector<float4> pl(total_planes);
+    for (uint32_t i = 0; i < total_planes; ++i)
+        pl[i] = make_float4(planes[i].normal[0], planes[i].normal[1], planes[i].normal[2], planes[i].d);
+    hipError_t e = grow(&P->d_pl, &P->cap_pl, std::max<size_t>(1, pl.size()), P);
+    if (e == hipSuccess) e = grow(&P->d_img, &P->cap_img, n, P);
+    if (e == hipSuccess) e = grow(&P->d_px, &P->cap_px, n * npx, P);
+    if (e == hipSuccess) e = grow(&P->d_tot, &P->cap_tot, n, P);
+    if (e == hipSuccess && !P->d_counter) e = hipMalloc(&P->d_counter, sizeof(uint32_t));
+    if (e == hipSuccess) e = hipMemcpyAsync(P->d_pl, pl.data(), pl.size() * sizeof(float4), hipMemcpyHostToDevice, st);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(P->d_img, images.data(), n * sizeof(MetricImage), hipMemcpyHostToDevice, st);
+    Shape sh;
+    if (e == hipSuccess) {
+        if (P->forced.block != 0) {
+            sh = P->forced;
+            if (sh.block > 0) {
+                int per_cu = 0;
+                e = ixg_metric_queue_occupancy(sh.block, sh.tile, &per_cu);
+                sh.grid = (uint32_t)std::max(1, per_cu) * (uint32_t)P->cus;
+            }
+        } else {
+            const std::array<uint32_t, 3> key{S.width, S.height, S.samples};
+            auto it = P->tuned.find(key);
+            if (it != P->tuned.end()) {
+                sh = it->second;
+            } else {
+                e = autotune(P, S, n, st, &sh);
+                if (e == hipSuccess) P->tuned[key] = sh;
+            }
+        }
+    }
+    if (e == hipSuccess) {
+        if (P->use_graph) {
+            // the graph bakes in kernel arguments: shared constants, image
+            // count, shape and buffer addresses
+            std::vector<uint8_t> key(sizeof(MetricShared) + sizeof(size_t) + sizeof(Shape) + 4 * sizeof(void*));
+            uint8_t* k = key.data();
+            std::memcpy(k, &S, sizeof S);
+            k += sizeof S;
+            std::memcpy(k, &n, sizeof n);
+            k += sizeof n;
+            std::memcpy(k, &sh, sizeof sh);
+            k += sizeof sh;
+            const void* ptrs[4] = {P->d_pl, P->d_img, P->d_px, P->d_tot};
+            std::memcpy(k, ptrs, sizeof ptrs);
+            if (key != P->graph_key || !P->exec) {
+                drop_graph(P);
+                e = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal);
+                if (e == hipSuccess) {
+                    hipError_t le = enqueue_sweep(P, S, n, sh, st);
+                    hipGraph_t g = nullptr;
+                    e = hipStreamEndCapture(st, &g);
+                    if (e == hipSuccess) e = le;
+                    if (e == hipSuccess) {
+                        P->graph = g;
+                        e = hipGraphInstantiate(&P->exec, g, nullptr, nullptr, 0);
+                    } else if (g) {
+                        (void)hipGraphDestroy(g);
+                    }
+                }
+                if (e == hipSuccess) P->graph_key = key;
+            }
+            if (e == hipSuccess) e = hipGraphLaunch(P->exec, st);
+        } else {
+            e = enqueue_sweep(P, S, n, sh, st);
+        }
+    }
+    totals.resize(n);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(totals.data(), P->d_tot, n * sizeof(ixg_metric_totals), hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) {
+        drop_graph(P);
+        return ixg_internal_hip_fail(ctx, e, "metric sweep");
+    }
+    P->last = sh;
+    P->last_graphed = P->use_graph;
+    return IXG_OK;
+}
+
 }  // namespace
 
 extern "C" ixg_status ixg_metric_image(ixg_ctx* ctx, const ixg_metric_request* r, ixg_optical_pixel* pixels,
@@ -269,7 +502,7 @@ extern "C" ixg_status ixg_metric_sweep(ixg_ctx* ctx, const ixg_facet_plane* plan
         }
     }
     std::vector<ixg_metric_totals> tot;
-    stt = run_images(ctx, S, images, planes, total_planes, nullptr, tot);
+    stt = run_sweep(ctx, S, images, planes, total_planes, tot);
     if (stt != IXG_OK) return stt;
     for (size_t i = 0; i < tot.size(); ++i) {
         out_metrics[5 * i + 0] = tot[i].brilliance_pct;
@@ -278,5 +511,30 @@ extern "C" ixg_status ixg_metric_sweep(ixg_ctx* ctx, const ixg_facet_plane* plan
         out_metrics[5 * i + 3] = tot[i].windowing_pct;
         out_metrics[5 * i + 4] = tot[i].extinction_pct;
     }
+    return IXG_OK;
+}
+
+extern "C" ixg_status ixg_set_metric_mode(ixg_ctx* ctx, int use_graph, int block, int tile) {
+    if (!ctx) return IXG_INVALID;
+    if (block > 0) {
+        int per_cu = 0;
+        if (ixg_metric_queue_occupancy(block, tile, &per_cu) != hipSuccess)
+            return ixg_internal_fail(ctx, IXG_INVALID, "unknown queue shape");
+    } else if (block < -1) {
+        return ixg_internal_fail(ctx, IXG_INVALID, "block must be -1 (grid), 0 (autotune) or a queue shape");
+    }
+    SweepPlan* P = get_plan(ctx);
+    P->use_graph = use_graph ? 1 : 0;
+    P->forced = Shape{block, block > 0 ? tile : 0, 0};
+    return IXG_OK;
+}
+
+extern "C" ixg_status ixg_metric_tuning(ixg_ctx* ctx, int* block, int* tile, uint32_t* grid_blocks, int* graphed) {
+    if (!ctx) return IXG_INVALID;
+    SweepPlan* P = get_plan(ctx);
+    if (block) *block = P->last.block;
+    if (tile) *tile = P->last.tile;
+    if (grid_blocks) *grid_blocks = P->last.grid;
+    if (graphed) *graphed = P->last_graphed;
     return IXG_OK;
 }

@@ -70,3 +70,18 @@ def metric_sweep(backend, planes, designs, tilts_deg, width, height, samples, di
     if st != abi.IXG_OK:
         raise IxgError(st, backend.last_error())
     return out
+
+
+def set_metric_mode(backend, use_graph=True, block=0, tile=0):
+    """How metric_sweep launches: graph capture on/off; block 0 = autotune,
+    -1 = the 2-D grid kernel, or a queue shape (block lanes, tile pixels)."""
+    st = lib().ixg_set_metric_mode(backend.handle, 1 if use_graph else 0, block, tile)
+    if st != abi.IXG_OK:
+        raise IxgError(st, backend.last_error())
+
+
+def metric_tuning(backend):
+    """(block, tile, grid_blocks, graphed) of the last metric_sweep."""
+    b, t, g, gr = C.c_int(), C.c_int(), C.c_uint32(), C.c_int()
+    lib().ixg_metric_tuning(backend.handle, C.byref(b), C.byref(t), C.byref(g), C.byref(gr))
+    return b.value, t.value, g.value, bool(gr.value)

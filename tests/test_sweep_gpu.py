@@ -4,7 +4,7 @@ import numpy as np
 import pytest
 
 import oracle_lib
-from indicatrix_amd import abi, designs, metrics, tracer
+from indicatrix_amd import abi, designs, metrics, tracer, workloads
 
 pytestmark = pytest.mark.gpu
 
@@ -36,3 +36,42 @@ def test_sweep_matches_per_image_oracle(backend):
             assert g[4] == pytest.approx(tot.extinction_pct, abs=1e-5)
             assert g[1] == pytest.approx(tot.fire_index, rel=2e-3, abs=1e-3)
             assert g[2] == pytest.approx(scint, abs=1e-3)
+
+
+def test_sweep_shapes_and_graph_replays_match_the_eager_grid(backend):
+    """Every launch shape computes the same pixels, so the sweep's metrics are
+    bit-identical across the 2-D grid, each persistent-queue shape, the
+    autotuned choice and graph replays (with other device work and other
+    inputs between replays)."""
+    env = tracer.LightingPreset.studio("IsoHemisphere", 1.0, 0.0, 0.0)
+    tilts = np.array([-21.0, 0.0, 9.0, 30.0], np.float32)
+    a_planes, a_ds, _ = designs.sweep_batch(6, first_seed=70)
+    b_planes, b_ds, _ = designs.sweep_batch(6, first_seed=90)
+    w = h = 20
+    spp = 4
+    try:
+        print("sweep eager grid", flush=True)
+        metrics.set_metric_mode(backend, use_graph=False, block=-1)
+        ref_a = metrics.metric_sweep(backend, a_planes, a_ds, tilts, w, h, spp, 2.4, env)
+        ref_b = metrics.metric_sweep(backend, b_planes, b_ds, tilts, w, h, spp, 2.4, env)
+        assert metrics.metric_tuning(backend)[0] == -1
+        for block, tile in [(64, 64), (64, 256), (128, 128), (128, 512), (256, 256), (256, 1024)]:
+            print(f"sweep queue {block}x{tile}", flush=True)
+            metrics.set_metric_mode(backend, use_graph=False, block=block, tile=tile)
+            got = metrics.metric_sweep(backend, a_planes, a_ds, tilts, w, h, spp, 2.4, env)
+            assert np.array_equal(got, ref_a), (block, tile)
+        print("sweep autotuned graph replays", flush=True)
+        metrics.set_metric_mode(backend, use_graph=True, block=0)
+        film = workloads.film_c1_scene(8, 8)
+        for k in range(3):
+            got_a = metrics.metric_sweep(backend, a_planes, a_ds, tilts, w, h, spp, 2.4, env)
+            assert np.array_equal(got_a, ref_a), k
+            assert metrics.metric_tuning(backend)[3]
+            acc = np.zeros((64, 3))
+            assert backend.try_accumulate(film, 0, 2, acc)       # eager work between replays
+            got_b = metrics.metric_sweep(backend, b_planes, b_ds, tilts, w, h, spp, 2.4, env)
+            assert np.array_equal(got_b, ref_b), k
+        block, tile, grid, graphed = metrics.metric_tuning(backend)
+        assert graphed and (block == -1 or (block > 0 and tile >= block and grid > 0))
+    finally:
+        metrics.set_metric_mode(backend, use_graph=True, block=0)
