@@ -7,7 +7,11 @@ Sellmeier, n_D ~ 2.417); a uniform D65 hemispherical dome (IsoHemisphere with
 the observer head shadow off); camera yaw 0.35, pitch 0.28, distance 5, fov 18
 deg (the reference's estimator-check camera, renderer/gpu/estimator_check/mod.rs:142-146).
 """
-from . import tracer
+import math
+
+import numpy as np
+
+from . import abi, designs, tracer
 
 C1 = dict(width=128, height=128, spp=64, max_bounces=16, crown_deg=34.5, pavilion_deg=40.75,
           table_ratio=0.57, camera=(0.35, 0.28, 5.0, 18.0))
@@ -45,3 +49,48 @@ def film_absorbing_scene(width=32, height=24, max_bounces=16):
     m.absorption_path_scale = 1.3
     return tracer.FrameScene(base.scene.camera, width, height, base.planes, m, max_bounces,
                              base.scene.environment)
+
+
+C2 = dict(width=1024, height=1024, spp=256, max_bounces=24, facets=121, seed=2, aspect=1.3,
+          camera=(0.35, 0.28, 5.0, 30.0))
+
+
+def c2_material(seed=C2["seed"], birefringence=0.009, bands=True):
+    """A seeded uniaxial, pleochroic material: a 3-term Sellmeier ordinary curve
+    (n_D ~ U[1.60, 1.80], Abbe ~ U[30, 60], split over a visible, a UV and a
+    weak IR pole), n_e = n_o + 0.009, the optic axis uniform on the sphere, and
+    two Gaussian o-bands and two e-bands with centres U[420, 650] nm, widths
+    U[20, 60] nm and peaks U[0, 2] /cm (= 0.1 x per-mm coefficients; the model
+    unit is taken as 3.25 mm, half a 6.5 mm stone)."""
+    rng = np.random.default_rng(20_000 + seed)
+    nd = float(rng.uniform(1.60, 1.80))
+    abbe = float(rng.uniform(30.0, 60.0))
+    b, c = tracer.sellmeier1_from_nd_abbe(nd, abbe)
+    coeff = [0.6 * b, 0.4 * b, 0.02, c, 0.8 * c, 100.0]
+    axis = rng.normal(size=3)
+    axis /= np.linalg.norm(axis)
+    m = tracer.GemMaterial("uniaxial-pleochroic", abi.DISP_SELLMEIER3, coeff,
+                           birefringence_delta=birefringence, cubic=False,
+                           c_axis=tuple(float(v) for v in axis), absorption_path_scale=3.25)
+    if bands:
+        def band():
+            return (float(rng.uniform(420.0, 650.0)), float(rng.uniform(20.0, 60.0)),
+                    float(rng.uniform(0.0, 2.0)) * 0.1, 0)
+        m.bands_o = [band(), band()]
+        m.bands_e = [band(), band()]
+    return m
+
+
+def c2_planes(seed=C2["seed"]):
+    """The 121-facet oval (tier angles U[30, 50] deg, 1.3:1) from the seed."""
+    return designs.design_planes(designs.seeded_params(seed, n_facets=C2["facets"], angle_range=(30.0, 50.0),
+                                                       aspect=C2["aspect"]))
+
+
+def film_c2_scene(width=None, height=None, spp=None, max_bounces=None, material=None):
+    w = width or C2["width"]
+    h = height or C2["height"]
+    cam = tracer.camera_new(*C2["camera"], w, h, spp or C2["spp"])
+    env = tracer.LightingPreset.studio("IsoHemisphere", 1.0, 0.0, 0.0, head_shadow_deg=0.0)
+    return tracer.FrameScene(cam, w, h, c2_planes(), material or c2_material(),
+                             max_bounces or C2["max_bounces"], env)

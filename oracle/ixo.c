@@ -615,6 +615,7 @@ typedef struct {
     env_t env;
     float surface_glare;
     int absorbing;      /* any absorption band (o or e) */
+    int aniso;          /* RayMaterialContext::is_anisotropic (uniaxial here) */
     v3 c_raw;           /* material c_axis as given */
     v3 ta1, ta2, tcn;   /* AbsorptionTensor3 axes: basis of normalize_or_zero(c_axis) */
 } scene_t;
@@ -628,8 +629,9 @@ typedef struct {
     float n_o[NCH];
     float alpha_o[NCH], alpha_e[NCH]; /* spectral_absorption of o / e bands */
     uint8_t compat[NCH];
-    v3 org, dir, k;
+    v3 org, dir, k;     /* ray origin, ray (Poynting) direction S, wave normal k */
     int inside;
+    int is_extra;       /* eigenmode assigned at the latest entry (uniaxial) */
     int has_prev;
     v3 prev_plane;
     int escaped;
@@ -691,6 +693,8 @@ static void apply_absorption(const scene_t* S, path_t* P, float path_len) {
     }
 }
 
+#include "ixo_uniaxial.h"
+
 static float tir_phase_delta(float n1k, float cos_i, float sin_i) { /* refraction/tir.rs:33-37 */
     float tan_half = (cos_i * sqrtf(maxf(fmaf(n1k * n1k * sin_i, sin_i, -1.0f), 0.0f))) /
                      maxf(n1k * sin_i * sin_i, 1e-6f);
@@ -713,8 +717,15 @@ static void fresnel_rs_rp(float n1, float n2, float cos_i, float cos_t, float* r
     *rp = fmaf(n1, -cos_t, n2 * cos_i) / fmaf(n1, cos_t, n2 * cos_i);
 }
 
+static void facet_event_aniso(const scene_t* S, path_t* P, uint32_t bounce, uint32_t seed, float t_hit,
+                              int32_t facet, ixg_path_events* ev, int* stop);
+
 static void facet_event(const scene_t* S, path_t* P, uint32_t bounce, uint32_t seed, float t_hit,
                         int32_t facet, ixg_path_events* ev, int* stop) {
+    if (S->aniso) {
+        facet_event_aniso(S, P, bounce, seed, t_hit, facet, ev, stop);
+        return;
+    }
     v3 hit_point = vadd(P->org, vmul(P->dir, t_hit));
     v3 normal = vfrom(S->planes[facet].normal);
     /* rotate_stokes_to_plane_of_incidence (RT/absorption.rs:69-87) */
@@ -903,6 +914,553 @@ static void facet_event(const scene_t* S, path_t* P, uint32_t bounce, uint32_t s
     }
 }
 
+/* ------------------------------------------------------------------ */
+/* Uniaxial facet event (RT/refraction/{geometry,dispatch,tir,         */
+/* uniaxial_entry,uniaxial_internal,reflect_refract,exit_split}.rs,     */
+/* RT/transport/bounce.rs)                                              */
+/* ------------------------------------------------------------------ */
+#define BIREFRINGENT_SPLIT_STREAM 0x2545F491u
+#define MODE_COUPLING_STREAM 0xCC9E2D51u
+
+/* GemMaterial::extraordinary_index_at (IX/optics/materials/optics.rs:42-47) */
+static float e_index_at(const scene_t* S, float lambda, float n_o) {
+    return S->mat->has_e_dispersion ? ixo_dispersion_evaluate(&S->mat->e_dispersion, lambda)
+                                    : n_o + S->mat->birefringence_delta;
+}
+
+/* theta_c_for_bounce (refraction/geometry.rs:75-120), uniaxial: a two-step
+ * fixed point for the refracted wave's angle to the optic axis at entry. */
+static float theta_c_for_bounce(const scene_t* S, const path_t* P, v3 normal, v3 k, float cos_i, int inside,
+                                float n_o_seed) {
+    v3 c = S->c_raw;
+    if (!inside) {
+        float n_e_seed = e_index_at(S, P->lambdas[0], n_o_seed);
+        float n_guess = n_o_seed, theta = 0.0f;
+        for (int it = 0; it < 2; ++it) {
+            float eg = 1.0f / n_guess;
+            float s2g = eg * eg * fmaf(cos_i, -cos_i, 1.0f);
+            if (s2g > 1.0f) break;
+            float ctg = sqrtf(maxf(1.0f - s2g, 0.0f));
+            v3 wd = vnormalize(vadd(vmul(k, eg), vmul(normal, fmaf(eg, cos_i, -ctg))));
+            theta = acosf(fabsf(clampf(vdot(wd, c), -1.0f, 1.0f)));
+            n_guess = effective_extraordinary_index(n_o_seed, n_e_seed, theta);
+        }
+        return theta;
+    }
+    return acosf(fabsf(clampf(vdot(k, c), -1.0f, 1.0f)));
+}
+
+/* entry_eigenmode_selection (refraction/uniaxial_entry.rs:50-75): the
+ * Malus-law probability of the ordinary eigenmode and its doubled azimuth. */
+static int entry_eigenmode_selection(v3 c, v3 cpn, v3 k, stokes h, float* p_o, float* c2, float* s2) {
+    if (h.i <= 1e-7f || vlen_sq(cpn) <= 1e-6f) return 0;
+    if (fmaf(h.q, h.q, h.u * h.u) <= 1e-12f) return 0;
+    v3 o_hat = ordinary_eigen(k, c);
+    v3 p_hat = vcross(k, cpn);
+    float sc = vdot(o_hat, cpn), pc = vdot(o_hat, p_hat);
+    *c2 = fmaf(sc, sc, -(pc * pc));
+    *s2 = 2.0f * sc * pc;
+    *p_o = clampf(0.5f + 0.5f * fmaf(h.q, *c2, h.u * *s2) / h.i, 0.0f, 1.0f);
+    return 1;
+}
+
+/* apply_absorption with the assigned-mode coefficient
+ * (RT/absorption.rs:224-249): alpha = e_mode . A . e_mode for the eigenmode
+ * the path entered in. */
+static void apply_absorption_assigned(const scene_t* S, path_t* P, float path_len) {
+    float n_o_h = P->n_o[0];
+    float n_e_h = e_index_at(S, P->lambdas[0], n_o_h);
+    v3 e = assigned_mode_e_field_uniaxial(P->k, S->c_raw, P->is_extra, n_o_h, n_e_h);
+    float scaled = path_len * S->mat->absorption_path_scale;
+    for (int c = 0; c < NCH; ++c) {
+        float alpha = tensor_qf(S, P->alpha_o[c], P->alpha_e[c], e);
+        P->st[c] = st_scale(P->st[c], ixo_exp_lane(-alpha * scaled));
+    }
+}
+
+/* try_split_exit_channel (refraction/exit_split.rs:128-163) */
+static void split_probe(const scene_t* S, path_t* P, v3 hit, int c, v3 dk, float transmitted_i,
+                        ixg_path_events* ev) {
+    v3 o = vadd(hit, vmul(dk, 1e-4f));
+    float po[3] = {o.x, o.y, o.z}, pd[3] = {dk.x, dk.y, dk.z}, tt;
+    if (ev) ev->n_probe++;
+    if (ixo_intersect(S->planes, S->n_planes, po, pd, &tt) >= 0) return;
+    float env = env_channel(S, dk, P->lambdas[c], P->observer);
+    P->split[c] = fmaf(maxf(transmitted_i, 0.0f) * 1.0f, env, P->split[c]);
+}
+
+/* narrow_compat (refraction/context.rs:219-242) */
+static void narrow_compat_dirs(path_t* P, const v3* dirs, const int* has_dir, const int* hero_match) {
+    for (int a = 0; a < NCH; ++a)
+        for (int b = a + 1; b < NCH; ++b) {
+            int m;
+            if (a == 0) m = hero_match[b];
+            else if (has_dir[a] && has_dir[b]) m = vdot(dirs[a], dirs[b]) >= DIRECTION_MATCH_COS_TOL;
+            else m = 1;
+            if (!m) {
+                P->compat[a] &= (uint8_t)~(1u << b);
+                P->compat[b] &= (uint8_t)~(1u << a);
+            }
+        }
+}
+
+typedef struct {
+    float cos_i, sin_i, n1, n2, sin2_t, n_o_hero, n_e_hero;
+    float n1_ch[NCH], n2_ch[NCH], sin2_t_ch[NCH];
+    uframe F;
+} ageo_t;
+
+static v3 refract_dir(v3 k, v3 normal, float eta, float cos_i, float cos_t) {
+    return vnormalize(vadd(vmul(k, eta), vmul(normal, fmaf(eta, cos_i, -cos_t))));
+}
+
+/* poynting_dir_for_mode (refraction/geometry.rs) */
+static v3 poynting_for_mode(const scene_t* S, const ageo_t* G, v3 k, int inside, int is_extra) {
+    if (!inside || !is_extra) return k;
+    return extraordinary_poynting_dir(k, S->c_raw, G->n_o_hero, G->n_e_hero);
+}
+
+/* compute_uniaxial_exit_transmission (refraction/uniaxial_internal.rs) */
+static stokes uniaxial_exit_transmission(const internal_sol* sol, float r_branch, float inc_i, float* i_unit) {
+    float flux_inc = maxf(sol->flux_inc, 1e-12f);
+    cplx ts = cscale(sol->t_s, sqrtf(sol->flux_ts / flux_inc));
+    cplx tp = cscale(sol->t_p, sqrtf(sol->flux_tp / flux_inc));
+    float iu = cnorm2(ts) + cnorm2(tp);
+    float qu = cnorm2(ts) - cnorm2(tp);
+    cplx cr = cmul(ts, cconj(tp));
+    float uu = 2.0f * cr.re, vu = -2.0f * cr.im;
+    stokes t = {inc_i * iu, inc_i * qu, inc_i * uu, inc_i * vu};
+    *i_unit = iu;
+    return st_scale(t, 1.0f / (1.0f - r_branch));
+}
+
+/* One facet event of a uniaxial material: the bounce body's facet_event
+ * (transport/inner/bounce_body.rs:611-707) with compute_bounce_refraction_geometry's
+ * anisotropic indices, dispatch_bounce's TIR / partial-Fresnel arms
+ * (transport/bounce.rs:108-238, refraction/dispatch.rs) and the internal
+ * o<->e re-coupling draw (transport/bounce.rs:439-500). */
+static void facet_event_aniso(const scene_t* S, path_t* P, uint32_t bounce, uint32_t seed, float t_hit,
+                              int32_t facet, ixg_path_events* ev, int* stop) {
+    const ixg_material* M = S->mat;
+    const v3 cax = S->c_raw;
+    v3 hit_point = vadd(P->org, vmul(P->dir, t_hit));
+    v3 normal = vfrom(S->planes[facet].normal);
+    v3 cur_plane = vnormalize_or_zero(vcross(P->k, normal));
+    if (P->has_prev && vlen_sq(cur_plane) > 1e-6f && vlen_sq(P->prev_plane) > 1e-6f) {
+        float cos_psi = clampf(vdot(P->prev_plane, cur_plane), -1.0f, 1.0f);
+        float sin_psi = vdot(vcross(P->prev_plane, cur_plane), vnormalize_or_zero(P->k));
+        mat4 R = mueller_frame_rotation(atan2f(sin_psi, cos_psi));
+        for (int c = 0; c < NCH; ++c) P->st[c] = st_apply(&R, P->st[c]);
+    }
+    P->prev_plane = cur_plane;
+    P->has_prev = 1;
+    if (P->inside && S->absorbing) apply_absorption_assigned(S, P, t_hit);
+    if (P->inside) normal = vneg(normal);
+
+    const v3 k = P->k;
+    const int inside = P->inside, is_extra = P->is_extra;
+    ageo_t G;
+    G.cos_i = clampf(vdot(vneg(k), normal), 0.0f, 1.0f);
+    G.sin_i = sqrtf(maxf(fmaf(G.cos_i, -G.cos_i, 1.0f), 0.0f));
+    float theta_c = theta_c_for_bounce(S, P, normal, k, G.cos_i, inside, P->n_o[0]);
+    float n_eff[NCH], n_e_ch[NCH];
+    for (int c = 0; c < NCH; ++c) {
+        n_e_ch[c] = e_index_at(S, P->lambdas[c], P->n_o[c]);
+        n_eff[c] = effective_extraordinary_index(P->n_o[c], n_e_ch[c], theta_c);
+    }
+    G.n_o_hero = P->n_o[0];
+    G.n_e_hero = G.n_o_hero + M->birefringence_delta;
+    const float* n_med = (inside && !is_extra) ? P->n_o : n_eff;
+    G.n1 = inside ? n_med[0] : 1.0f;
+    G.n2 = inside ? 1.0f : n_med[0];
+    float eta = G.n1 / G.n2;
+    G.sin2_t = eta * eta * fmaf(G.cos_i, -G.cos_i, 1.0f);
+    for (int c = 0; c < NCH; ++c) {
+        float n1k = inside ? n_med[c] : 1.0f, n2k = inside ? 1.0f : n_med[c];
+        float ek = n1k / n2k;
+        G.n1_ch[c] = n1k;
+        G.n2_ch[c] = n2k;
+        G.sin2_t_ch[c] = ek * ek * fmaf(G.cos_i, -G.cos_i, 1.0f);
+    }
+    G.F = uframe_build(k, normal, cax, G.cos_i, G.sin_i);
+    const int nondegenerate = vlen_sq(vcross(k, cax)) > 1e-6f;
+    const float one_m_c2 = fmaf(G.cos_i, -G.cos_i, 1.0f);
+
+    v3 new_k, new_s;
+    int new_inside = inside, upd = -1;
+    float exact_p_o = -1.0f;
+    if (G.sin2_t > 1.0f) {
+        /* apply_tir_bounce (refraction/tir.rs:56-160) */
+        if (ev) ev->n_tir++;
+        ev_facet(ev, facet, 1);
+        if (nondegenerate) {
+            for (int c = 0; c < NCH; ++c) {
+                internal_sol sol = internal_solve(G.n1_ch[c], P->n_o[c], n_e_ch[c], cax, &G.F, !is_extra);
+                float flux_inc = maxf(sol.flux_inc, 1e-12f);
+                float ro = cnorm2(sol.r_o) * sol.flux_ro, re = cnorm2(sol.r_e) * sol.flux_re;
+                float r_total = minf((ro + re) / flux_inc, 1.0f);
+                P->st[c] = st_scale(P->st[c], r_total);
+                if (G.sin2_t_ch[c] <= 1.0f) P->path_pdf[c] *= clampf(r_total, R_UNPOL_PDF_MIN, R_UNPOL_PDF_MAX);
+                if (c == 0) exact_p_o = ro / maxf(ro + re, 1e-12f);
+            }
+        } else {
+            for (int c = 0; c < NCH; ++c) {
+                float n1k = G.n1_ch[c], n2k = G.n2_ch[c];
+                if (G.sin2_t_ch[c] > 1.0f) {
+                    mat4 Mt = mueller_tir_retardation(tir_phase_delta(n1k, G.cos_i, G.sin_i));
+                    P->st[c] = st_apply(&Mt, P->st[c]);
+                } else {
+                    float ct = sqrtf(maxf(1.0f - G.sin2_t_ch[c], 0.0f));
+                    float rs, rp;
+                    fresnel_rs_rp(n1k, n2k, G.cos_i, ct, &rs, &rp);
+                    mat4 Mr = mueller_fresnel_reflection(rs, rp);
+                    P->st[c] = st_apply(&Mr, P->st[c]);
+                    P->path_pdf[c] *= clampf(0.5f * fmaf(rp, rp, rs * rs), R_UNPOL_PDF_MIN, R_UNPOL_PDF_MAX);
+                }
+            }
+        }
+        new_k = vsub(k, vmul(normal, 2.0f * vdot(k, normal)));
+        new_s = poynting_for_mode(S, &G, new_k, inside, is_extra);
+    } else if (!inside && nondegenerate) {
+        /* apply_uniaxial_entry_bounce (refraction/uniaxial_entry.rs:108-198) */
+        entry_inc I = entry_incidence_frame(1.0f, &G.F);
+        entry_sol hs, hp;
+        entry_solve_pair(&I, 1.0f, P->n_o[0], n_e_ch[0], cax, &G.F, &hs, &hp);
+        mat4 Mh = jones_to_mueller(hs.r_s, hp.r_s, hs.r_p, hp.r_p);
+        float r_branch = clampf(Mh.m[0][0], R_UNPOL_SELECT_MIN, R_UNPOL_SELECT_MAX);
+        float inc_flux = 0.5f * G.F.cos_i;
+        float p_o_raw = mode_power(hs.t_o, hp.t_o, hs.flux_o, inc_flux, P->st[0]);
+        float p_e_raw = mode_power(hs.t_e, hp.t_e, hs.flux_e, inc_flux, P->st[0]);
+        float p_o_hero = clampf(p_o_raw / maxf(p_o_raw + p_e_raw, 1e-12f), R_UNPOL_SELECT_MIN, R_UNPOL_SELECT_MAX);
+        float msr = unit_draw(seed, bounce, BIREFRINGENT_SPLIT_STREAM);
+        int use_extra = msr < (1.0f - p_o_hero);
+        float p_mode_frac = use_extra ? 1.0f - p_o_hero : p_o_hero;
+        float n2_dir = use_extra ? G.n2 : G.n_o_hero;
+        float eta_dir = G.n1 / n2_dir;
+        float s2d = minf(eta_dir * eta_dir * one_m_c2, 1.0f);
+        float ctd = sqrtf(maxf(1.0f - s2d, 0.0f));
+        v3 refr = refract_dir(k, normal, eta_dir, G.cos_i, ctd);
+        v3 final_refr = use_extra ? extraordinary_poynting_dir(refr, cax, G.n_o_hero, G.n_e_hero) : refr;
+        v3 p_axis_t = vcross(refr, G.F.s_axis);
+        float rng = unit_draw(seed, bounce, FRESNEL_BRANCH_STREAM);
+        if (rng < r_branch) {
+            if (ev) ev->n_reflect++;
+            ev_facet(ev, facet, 2);
+            for (int c = 0; c < NCH; ++c) {
+                entry_sol ss, sp;
+                entry_solve_pair(&I, 1.0f, P->n_o[c], n_e_ch[c], cax, &G.F, &ss, &sp);
+                mat4 Mk = jones_to_mueller(ss.r_s, sp.r_s, ss.r_p, sp.r_p);
+                float ru = clampf(Mk.m[0][0], R_UNPOL_PDF_MIN, R_UNPOL_PDF_MAX);
+                P->st[c] = st_scale(st_apply(&Mk, P->st[c]), 1.0f / r_branch);
+                P->path_pdf[c] *= ru;
+            }
+            new_k = vsub(k, vmul(normal, 2.0f * vdot(k, normal)));
+            new_s = new_k;
+        } else {
+            if (ev) ev->n_refract++;
+            ev_facet(ev, facet, 3);
+            v3 dirs[NCH];
+            int has_dir[NCH], hero_match[NCH];
+            for (int c = 0; c < NCH; ++c) {
+                float n_o_k = P->n_o[c];
+                float n2k = use_extra ? G.n2_ch[c] : n_o_k;
+                float r = G.n1 / n2k;
+                float s2k = r * r * one_m_c2;
+                has_dir[c] = 0;
+                hero_match[c] = 0;
+                if (s2k > 1.0f) {
+                    P->st[c] = st_scale(P->st[c], 0.0f);
+                    P->path_pdf[c] = 0.0f;
+                    continue;
+                }
+                float ctk = sqrtf(maxf(1.0f - s2k, 0.0f));
+                float ek = G.n1 / n2k;
+                v3 rk = refract_dir(k, normal, ek, G.cos_i, ctk);
+                v3 fk = use_extra ? extraordinary_poynting_dir(rk, cax, n_o_k, n_o_k + M->birefringence_delta) : rk;
+                dirs[c] = fk;
+                has_dir[c] = 1;
+                int match = vdot(fk, final_refr) >= DIRECTION_MATCH_COS_TOL;
+                hero_match[c] = match;
+                entry_sol ss, sp;
+                entry_solve_pair(&I, 1.0f, n_o_k, n_e_ch[c], cax, &G.F, &ss, &sp);
+                if (!match) {
+                    P->st[c] = st_scale(P->st[c], 0.0f);
+                    mat4 Mk = jones_to_mueller(ss.r_s, sp.r_s, ss.r_p, sp.r_p);
+                    P->path_pdf[c] *= clampf(1.0f - Mk.m[0][0], R_UNPOL_PDF_MIN, R_UNPOL_PDF_MAX);
+                    continue;
+                }
+                cplx tsr = use_extra ? ss.t_e : ss.t_o, tpr = use_extra ? sp.t_e : sp.t_o;
+                float flux_k = use_extra ? ss.flux_e : ss.flux_o;
+                v3 mode_dir = use_extra ? ss.e_hat : ss.o_hat;
+                float p_mode_k = mode_power(tsr, tpr, flux_k, inc_flux, P->st[c]);
+                float dep = p_mode_k / p_mode_frac;
+                float c2, s2;
+                azimuth2_in_frame(mode_dir, G.F.s_axis, p_axis_t, &c2, &s2);
+                stokes t = {dep, dep * c2, dep * s2, 0.0f};
+                P->st[c] = st_scale(t, 1.0f / (1.0f - r_branch));
+                mat4 Mk = jones_to_mueller(ss.r_s, sp.r_s, ss.r_p, sp.r_p);
+                P->path_pdf[c] *= clampf(1.0f - Mk.m[0][0], R_UNPOL_PDF_MIN, R_UNPOL_PDF_MAX);
+            }
+            narrow_compat_dirs(P, dirs, has_dir, hero_match);
+            new_k = refr;
+            new_s = final_refr;
+            new_inside = 1;
+            upd = use_extra;
+        }
+    } else if (inside && nondegenerate) {
+        /* apply_uniaxial_internal_bounce (refraction/uniaxial_internal.rs) */
+        internal_sol sh = internal_solve(G.n1_ch[0], P->n_o[0], n_e_ch[0], cax, &G.F, !is_extra);
+        float flux_inc_h = maxf(sh.flux_inc, 1e-12f);
+        float ro = cnorm2(sh.r_o) * sh.flux_ro, re = cnorm2(sh.r_e) * sh.flux_re;
+        float r_branch = clampf((ro + re) / flux_inc_h, R_UNPOL_SELECT_MIN, R_UNPOL_SELECT_MAX);
+        float p_o_exact = ro / maxf(ro + re, 1e-12f);
+        float rng = unit_draw(seed, bounce, FRESNEL_BRANCH_STREAM);
+        if (rng < r_branch) {
+            if (ev) ev->n_reflect++;
+            ev_facet(ev, facet, 2);
+            for (int c = 0; c < NCH; ++c) {
+                internal_sol sol = c == 0 ? sh : internal_solve(G.n1_ch[c], P->n_o[c], n_e_ch[c], cax, &G.F, !is_extra);
+                float flux_inc = maxf(sol.flux_inc, 1e-12f);
+                float r_total = minf(fmaf(cnorm2(sol.r_e), sol.flux_re, cnorm2(sol.r_o) * sol.flux_ro) / flux_inc, 1.0f);
+                P->st[c] = st_scale(P->st[c], r_total / r_branch);
+                P->path_pdf[c] *= clampf(r_total, R_UNPOL_PDF_MIN, R_UNPOL_PDF_MAX);
+            }
+            new_k = vsub(k, vmul(normal, 2.0f * vdot(k, normal)));
+            new_s = new_k;
+            exact_p_o = p_o_exact;
+        } else {
+            if (ev) ev->n_refract++;
+            ev_facet(ev, facet, 3);
+            float eta_dir = G.n1 / G.n2;
+            float s2d = minf(eta_dir * eta_dir * one_m_c2, 1.0f);
+            float ctd = sqrtf(maxf(1.0f - s2d, 0.0f));
+            v3 hero_refr = refract_dir(k, normal, eta_dir, G.cos_i, ctd);
+            for (int c = 0; c < NCH; ++c) {
+                float orig_i = P->st[c].i;
+                float n_inc = G.n1_ch[c], n2k = G.n2_ch[c];
+                float r = n_inc / n2k;
+                float s2k = r * r * one_m_c2;
+                if (s2k > 1.0f) {
+                    P->st[c] = st_scale(P->st[c], 0.0f);
+                    P->path_pdf[c] = 0.0f;
+                    continue;
+                }
+                float ctk = sqrtf(maxf(1.0f - s2k, 0.0f));
+                float ek = n_inc / n2k;
+                v3 rk = refract_dir(k, normal, ek, G.cos_i, ctk);
+                int match = vdot(rk, hero_refr) >= DIRECTION_MATCH_COS_TOL;
+                internal_sol sol = c == 0 ? sh : internal_solve(n_inc, P->n_o[c], n_e_ch[c], cax, &G.F, !is_extra);
+                float iu;
+                if (!match) {
+                    float prefix = P->path_pdf[c];
+                    P->st[c] = st_scale(P->st[c], 0.0f);
+                    stokes tr = uniaxial_exit_transmission(&sol, r_branch, orig_i, &iu);
+                    P->path_pdf[c] = prefix * clampf(iu, R_UNPOL_PDF_MIN, R_UNPOL_PDF_MAX);
+                    if (orig_i > 0.0f) split_probe(S, P, hit_point, c, rk, st_intensity(tr), ev);
+                    continue;
+                }
+                P->st[c] = uniaxial_exit_transmission(&sol, r_branch, orig_i, &iu);
+                P->path_pdf[c] *= clampf(iu, R_UNPOL_PDF_MIN, R_UNPOL_PDF_MAX);
+            }
+            new_k = hero_refr;
+            new_s = hero_refr;
+            new_inside = 0;
+        }
+    } else {
+        /* k along the optic axis: the scalar-Fresnel arms with the eigenmode
+         * draw (refraction/dispatch.rs:143-380, reflect_refract.rs) */
+        const int entering = !inside;
+        float p_o = 0.5f, c2o = 0.0f, s2o = 0.0f;
+        int have_sel = 0;
+        if (entering) {
+            float p;
+            have_sel = entry_eigenmode_selection(cax, cur_plane, k, P->st[0], &p, &c2o, &s2o);
+            if (have_sel) p_o = p;
+        }
+        float msr = unit_draw(seed, bounce, BIREFRINGENT_SPLIT_STREAM);
+        int use_extra = entering ? (msr < (1.0f - p_o)) : is_extra;
+        float c2x = use_extra ? -c2o : c2o, s2x = use_extra ? -s2o : s2o;
+        /* compute_entry_reflect_probability */
+        int ord_sel = entering && !use_extra;
+        float n2h = ord_sel ? G.n_o_hero : G.n2;
+        float s2h;
+        if (ord_sel) {
+            float e2 = G.n1 / n2h;
+            s2h = e2 * e2 * one_m_c2;
+        } else {
+            s2h = G.sin2_t;
+        }
+        float cth = sqrtf(maxf(1.0f - s2h, 0.0f));
+        float rs, rp;
+        fresnel_rs_rp(G.n1, n2h, G.cos_i, cth, &rs, &rp);
+        float r_unpol = clampf(0.5f * fmaf(rp, rp, rs * rs), R_UNPOL_SELECT_MIN, R_UNPOL_SELECT_MAX);
+        float rng = unit_draw(seed, bounce, FRESNEL_BRANCH_STREAM);
+        if (rng < r_unpol) {
+            if (ev) ev->n_reflect++;
+            ev_facet(ev, facet, 2);
+            for (int c = 0; c < NCH; ++c) {
+                float n1k = G.n1_ch[c], n2k = G.n2_ch[c];
+                mat4 Mr;
+                if (G.sin2_t_ch[c] > 1.0f) {
+                    Mr = mueller_tir_retardation(tir_phase_delta(n1k, G.cos_i, G.sin_i));
+                } else {
+                    float ct = sqrtf(maxf(1.0f - G.sin2_t_ch[c], 0.0f));
+                    float a, b;
+                    fresnel_rs_rp(n1k, n2k, G.cos_i, ct, &a, &b);
+                    P->path_pdf[c] *= clampf(0.5f * fmaf(b, b, a * a), R_UNPOL_PDF_MIN, R_UNPOL_PDF_MAX);
+                    Mr = mueller_fresnel_reflection(a, b);
+                }
+                P->st[c] = st_scale(st_apply(&Mr, P->st[c]), 1.0f / r_unpol);
+            }
+            new_k = vsub(k, vmul(normal, 2.0f * vdot(k, normal)));
+            new_s = poynting_for_mode(S, &G, new_k, inside, is_extra);
+        } else {
+            if (ev) ev->n_refract++;
+            ev_facet(ev, facet, 3);
+            float n2d = (entering && !use_extra) ? G.n_o_hero : G.n2;
+            float eta_dir = G.n1 / n2d;
+            float s2d = minf(eta_dir * eta_dir * one_m_c2, 1.0f);
+            float ctd = sqrtf(maxf(1.0f - s2d, 0.0f));
+            v3 refr = refract_dir(k, normal, eta_dir, G.cos_i, ctd);
+            v3 final_refr = (entering && use_extra) ? extraordinary_poynting_dir(refr, cax, G.n_o_hero, G.n_e_hero) : refr;
+            v3 dirs[NCH];
+            int has_dir[NCH], hero_match[NCH];
+            for (int c = 0; c < NCH; ++c) {
+                stokes orig = P->st[c];
+                float n1k = G.n1_ch[c];
+                float n2k = (entering && !use_extra) ? P->n_o[c] : G.n2_ch[c];
+                float r = n1k / n2k;
+                float s2k = r * r * one_m_c2;
+                has_dir[c] = 0;
+                hero_match[c] = 0;
+                if (s2k > 1.0f) {
+                    P->st[c] = st_scale(P->st[c], 0.0f);
+                    P->path_pdf[c] = 0.0f;
+                    continue;
+                }
+                float ctk = sqrtf(maxf(1.0f - s2k, 0.0f));
+                float ek = n1k / n2k;
+                v3 rk = refract_dir(k, normal, ek, G.cos_i, ctk);
+                v3 fk = (entering && use_extra)
+                            ? extraordinary_poynting_dir(rk, cax, P->n_o[c], P->n_o[c] + M->birefringence_delta)
+                            : rk;
+                int match = vdot(fk, final_refr) >= DIRECTION_MATCH_COS_TOL;
+                /* compute_channel_transmission (exit_split.rs:69-101) */
+                float num = 2.0f * n1k * G.cos_i;
+                float t_s = num / fmaf(n2k, ctk, n1k * G.cos_i);
+                float t_p = num / fmaf(n1k, ctk, n2k * G.cos_i);
+                mat4 T = mueller_fresnel_transmission(n1k, n2k, G.cos_i, ctk, t_s, t_p);
+                stokes incident = orig;
+                if (entering && have_sel) {
+                    stokes o2 = {orig.i, orig.i * c2x, orig.i * s2x, 0.0f};
+                    incident = o2;
+                }
+                stokes transmitted = st_scale(st_apply(&T, incident), 1.0f / (1.0f - r_unpol));
+                float a, b;
+                fresnel_rs_rp(n1k, n2k, G.cos_i, ctk, &a, &b);
+                float ruk = clampf(0.5f * fmaf(b, b, a * a), R_UNPOL_PDF_MIN, R_UNPOL_PDF_MAX);
+                if (match) {
+                    P->st[c] = transmitted;
+                    P->path_pdf[c] *= 1.0f - ruk;
+                } else {
+                    float prefix = P->path_pdf[c];
+                    P->st[c] = st_scale(P->st[c], 0.0f);
+                    P->path_pdf[c] = prefix * (1.0f - ruk);
+                    if (inside && !entering && st_intensity(orig) > 0.0f)
+                        split_probe(S, P, hit_point, c, rk, st_intensity(transmitted), ev);
+                }
+                dirs[c] = fk;
+                has_dir[c] = 1;
+                hero_match[c] = match;
+            }
+            if (!inside) narrow_compat_dirs(P, dirs, has_dir, hero_match);
+            new_k = refr;
+            new_s = final_refr;
+            new_inside = !inside;
+            if (entering) upd = use_extra;
+        }
+    }
+    /* dispatch_bounce tail (transport/bounce.rs:200-238) */
+    int was_internal_reflection = inside && upd < 0 && new_inside == inside;
+    P->dir = new_s;
+    P->org = vadd(hit_point, vmul(new_s, 1e-4f));
+    P->k = new_k;
+    P->inside = new_inside;
+    if (upd >= 0) P->is_extra = upd;
+    if (was_internal_reflection) {
+        /* apply_internal_mode_coupling (transport/bounce.rs:439-470) */
+        float p_o = 0.5f;
+        if (exact_p_o >= 0.0f) {
+            p_o = exact_p_o;
+        } else {
+            float p, c2, s2;
+            if (entry_eigenmode_selection(cax, cur_plane, new_k, P->st[0], &p, &c2, &s2)) p_o = p;
+        }
+        P->is_extra = unit_draw(seed, bounce, MODE_COUPLING_STREAM) >= p_o;
+    }
+    if (bounce == 0 && !P->inside && S->surface_glare < 1.0f)
+        for (int c = 0; c < NCH; ++c) P->st[c] = st_scale(P->st[c], S->surface_glare);
+    if (bounce > 4) {
+        float mx = 0.0f;
+        for (int c = 0; c < NCH; ++c) mx = maxf(mx, st_intensity(P->st[c]));
+        float q = clampf(mx, 0.05f, 1.0f);
+        float rr = unit_draw(seed, bounce, RUSSIAN_ROULETTE_STREAM);
+        if (rr > q) { *stop = 1; return; }
+        for (int c = 0; c < NCH; ++c) P->st[c] = st_scale(P->st[c], 1.0f / q);
+        for (int c = 0; c < NCH; ++c) P->split[c] /= q;
+    }
+}
+
+/* Energy bookkeeping of one uniaxial interface solve, for the tests:
+ * out = {R, T} as fractions of the incident Poynting flux along the normal.
+ * kind 0: air -> crystal entry, incident polarisation pol 0 = s, 1 = p;
+ * kind 1: crystal -> air, incident mode pol 0 = ordinary, 1 = extraordinary
+ * (n_inc is that mode's index along k).  k is the incident wave normal and
+ * normal faces the incident side ((-k).normal >= 0). */
+void ixo_uniaxial_energy(int kind, float n_inc, float n_o, float n_e, const float c[3], const float kk[3],
+                         const float nn[3], int pol, float out[2]) {
+    v3 k = vnormalize(vfrom(kk)), normal = vnormalize(vfrom(nn)), cax = vnormalize(vfrom(c));
+    float cos_i = clampf(vdot(vneg(k), normal), 0.0f, 1.0f);
+    float sin_i = sqrtf(maxf(fmaf(cos_i, -cos_i, 1.0f), 0.0f));
+    uframe F = uframe_build(k, normal, cax, cos_i, sin_i);
+    if (kind == 0) {
+        entry_inc I = entry_incidence_frame(1.0f, &F);
+        entry_sol ss, sp;
+        entry_solve_pair(&I, 1.0f, n_o, n_e, cax, &F, &ss, &sp);
+        const entry_sol* so = pol == 0 ? &ss : &sp;
+        float k_tan = F.sin_i;
+        cv3 k_bwd = cv_wave(F.that, F.zhat, k_tan, cx(-F.cos_i, 0.0f));
+        cv3 s_c = cv_real(F.s_axis);
+        cv3 ep_bwd = cv_scale_real(cv_cross(k_bwd, s_c), -1.0f);
+        mfields rs = {k_bwd, s_c, cv_cross(k_bwd, s_c)}, rp = {k_bwd, ep_bwd, cv_cross(k_bwd, ep_bwd)};
+        float inc = 0.5f * F.cos_i;
+        float R = (cnorm2(so->r_s) * fabsf(poynting_z(&rs, F.zhat)) + cnorm2(so->r_p) * fabsf(poynting_z(&rp, F.zhat))) / inc;
+        float T = (cnorm2(so->t_o) * so->flux_o + cnorm2(so->t_e) * so->flux_e) / inc;
+        out[0] = R;
+        out[1] = T;
+    } else {
+        internal_sol sol = internal_solve(n_inc, n_o, n_e, cax, &F, pol == 0);
+        float inc = maxf(sol.flux_inc, 1e-12f);
+        out[0] = (cnorm2(sol.r_o) * sol.flux_ro + cnorm2(sol.r_e) * sol.flux_re) / inc;
+        out[1] = (cnorm2(sol.t_s) * sol.flux_ts + cnorm2(sol.t_p) * sol.flux_tp) / inc;
+    }
+}
+
+/* The closed-form solution's reflection amplitudes at entry for s or p
+ * incidence: out = {re, im} of (r_ss, r_sp) for pol 0, (r_ps, r_pp) for 1. */
+void ixo_uniaxial_entry_r(float n_o, float n_e, const float c[3], const float kk[3], const float nn[3], int pol,
+                          float out[4]) {
+    v3 k = vnormalize(vfrom(kk)), normal = vnormalize(vfrom(nn)), cax = vnormalize(vfrom(c));
+    float cos_i = clampf(vdot(vneg(k), normal), 0.0f, 1.0f);
+    float sin_i = sqrtf(maxf(fmaf(cos_i, -cos_i, 1.0f), 0.0f));
+    uframe F = uframe_build(k, normal, cax, cos_i, sin_i);
+    entry_inc I = entry_incidence_frame(1.0f, &F);
+    entry_sol ss, sp;
+    entry_solve_pair(&I, 1.0f, n_o, n_e, cax, &F, &ss, &sp);
+    const entry_sol* so = pol == 0 ? &ss : &sp;
+    out[0] = so->r_s.re; out[1] = so->r_s.im; out[2] = so->r_p.re; out[3] = so->r_p.im;
+}
+
 static int scene_build(const ixg_scene* sc, scene_t* S) {
     S->planes = sc->planes;
     S->n_planes = sc->n_planes;
@@ -911,6 +1469,7 @@ static int scene_build(const ixg_scene* sc, scene_t* S) {
     float g = sc->environment.surface_glare;
     S->surface_glare = (g >= 1.0f || isnan(g)) ? 1.0f : (g > 0.0f ? g : 0.0f);
     S->absorbing = sc->material->n_bands_o > 0 || sc->material->n_bands_e > 0;
+    S->aniso = !sc->material->cubic && fabsf(sc->material->birefringence_delta) > 1e-4f;
     S->c_raw = vfrom(sc->material->c_axis);
     S->tcn = vnormalize_or_zero(S->c_raw);
     stable_basis(S->tcn, &S->ta1, &S->ta2);

@@ -32,6 +32,10 @@ float ixo_blackbody(float lambda_nm, float temp_k);
 void ixo_white_balance(float temp_k, float out[3]);
 float ixo_exp_lane(float x);
 float ixo_band_evaluate(const ixg_band* b, float lambda_nm);
+void ixo_uniaxial_energy(int kind, float n_inc, float n_o, float n_e, const float c[3], const float k[3],
+                         const float normal[3], int pol, float out[2]);
+void ixo_uniaxial_entry_r(float n_o, float n_e, const float c[3], const float k[3], const float normal[3], int pol,
+                          float out[4]);
 
 /* sample_studio_environment (RT/environment/mod.rs:653): observer = 0. */
 float ixo_sample_studio_environment(const float dir[3], float lambda_nm, int32_t preset,

@@ -44,6 +44,9 @@ def lib():
         L.ixo_d65.argtypes = [C.c_float]
         L.ixo_exp_lane.restype = C.c_float
         L.ixo_exp_lane.argtypes = [C.c_float]
+        L.ixo_uniaxial_energy.argtypes = [C.c_int, C.c_float, C.c_float, C.c_float, f3, f3, f3, C.c_int,
+                                          P(C.c_float)]
+        L.ixo_uniaxial_entry_r.argtypes = [C.c_float, C.c_float, f3, f3, f3, C.c_int, P(C.c_float)]
         L.ixo_band_evaluate.restype = C.c_float
         L.ixo_band_evaluate.argtypes = [P(abi.Band), C.c_float]
         L.ixo_sample_studio_environment.restype = C.c_float

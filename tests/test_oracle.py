@@ -421,3 +421,103 @@ def test_zero_peak_bands_are_exactly_transparent():
         oracle_lib.lib().ixo_trace_ray(C.byref(sc1), (C.c_float * 3)(0.1, 0.2, 5.0), d, seed, hero_of(seed), b, None)
         assert [bits(v) for v in a] == [bits(v) for v in b]
 
+
+
+# ---------------------------------------------------------------------------
+# uniaxial transport (raytracer/uniaxial_fresnel/, refraction/uniaxial_*.rs)
+# ---------------------------------------------------------------------------
+F3 = C.c_float * 3
+
+
+def _neff(n_o, n_e, theta):
+    return n_o * n_e / math.sqrt((n_e * math.cos(theta)) ** 2 + (n_o * math.sin(theta)) ** 2)
+
+
+def test_uniaxial_interfaces_conserve_energy():
+    """Lekner's boundary solution: R + T = 1 (Poynting flux along the normal)
+    at entry for s and p incidence, and inside for an o or e wave meeting the
+    surface, total internal reflection included (T = 0, R = 1)."""
+    L = oracle_lib.lib()
+    rng = np.random.default_rng(8)
+    out = (C.c_float * 2)()
+    worst, n_tir = 0.0, 0
+    for _ in range(400):
+        c = rng.normal(size=3)
+        c /= np.linalg.norm(c)
+        th = rng.uniform(0.0, 1.5)
+        k = (math.sin(th), 0.0, -math.cos(th))
+        for pol in (0, 1):
+            L.ixo_uniaxial_energy(0, 1.0, 1.64, 1.66, F3(*c), F3(*k), F3(0, 0, 1), pol, out)
+            worst = max(worst, abs(out[0] + out[1] - 1.0))
+            n_inc = 2.0 if pol == 0 else _neff(2.0, 2.15, math.acos(abs(float(np.dot(k, c)))))
+            L.ixo_uniaxial_energy(1, n_inc, 2.0, 2.15, F3(*c), F3(*k), F3(0, 0, 1), pol, out)
+            worst = max(worst, abs(out[0] + out[1] - 1.0))
+            n_tir += out[1] == 0.0
+    assert worst < 2e-5, worst
+    assert n_tir > 100
+
+
+def test_uniaxial_entry_isotropic_limit_is_scalar_fresnel():
+    """n_e = n_o: r_ss, r_pp equal the scalar r_s, r_p (the crate's sign
+    convention) and the cross terms vanish, for any optic axis."""
+    L = oracle_lib.lib()
+    r = (C.c_float * 4)()
+    for th in (0.05, 0.4, 0.9, 1.35):
+        k = (math.sin(th), 0.0, -math.cos(th))
+        ci = math.cos(th)
+        ct = math.sqrt(1.0 - (math.sin(th) / 2.0) ** 2)
+        for c in ((0.3, 0.4, 0.866), (1.0, 0.0, 0.0), (0.0, 0.0, 1.0)):
+            L.ixo_uniaxial_entry_r(2.0, 2.0, F3(*c), F3(*k), F3(0, 0, 1), 0, r)
+            assert r[0] == pytest.approx((ci - 2 * ct) / (ci + 2 * ct), abs=2e-6) and abs(r[1]) < 1e-6
+            assert abs(r[2]) < 1e-6 and abs(r[3]) < 1e-6
+            L.ixo_uniaxial_entry_r(2.0, 2.0, F3(*c), F3(*k), F3(0, 0, 1), 1, r)
+            assert r[2] == pytest.approx((2 * ci - ct) / (2 * ci + ct), abs=2e-6) and abs(r[3]) < 1e-6
+            assert abs(r[0]) < 1e-6
+
+
+def test_uniaxial_normal_incidence_axis_in_surface():
+    """Normal incidence, optic axis in the surface: light polarised along the
+    axis sees n_e, across it n_o, so the unpolarised reflectance is the mean
+    of the two normal-incidence reflectances."""
+    L = oracle_lib.lib()
+    out = (C.c_float * 2)()
+    n_o, n_e = 1.544, 1.553
+    rs = []
+    for pol in (0, 1):
+        L.ixo_uniaxial_energy(0, 1.0, n_o, n_e, F3(0.6, 0.8, 0.0), F3(0, 0, -1), F3(0, 0, 1), pol, out)
+        rs.append(out[0])
+    expect = 0.5 * (((n_o - 1) / (n_o + 1)) ** 2 + ((n_e - 1) / (n_e + 1)) ** 2)
+    assert 0.5 * (rs[0] + rs[1]) == pytest.approx(expect, rel=1e-5)
+
+
+def _c2_sums(material, spp, first=0, w=64, h=64):
+    from indicatrix_amd import workloads
+    return oracle_lib.accumulate(workloads.film_c2_scene(w, h, spp, material=material), first, spp).sum(axis=0)
+
+
+def test_uniaxial_film_converges_to_the_isotropic_limit():
+    """The uniaxial estimator (Lekner entry / exit, eigenmode draws, walk-off,
+    o<->e re-coupling) at a birefringence just above the anisotropy
+    threshold renders the same film as the isotropic tracer, within 4 sigma
+    of the Monte Carlo noise measured from independent sample blocks."""
+    from indicatrix_amd import workloads
+    iso = workloads.c2_material(bands=False)
+    iso.cubic = True
+    iso.birefringence_delta = 0.0
+    uni = workloads.c2_material(bands=False, birefringence=2e-4)
+    blocks = np.array([_c2_sums(uni, 32, 32 * b) for b in range(8)])
+    sigma = blocks.std(axis=0, ddof=1) / math.sqrt(8)
+    a = _c2_sums(iso, 256)
+    b = blocks.sum(axis=0)
+    assert (np.abs(b - a) < 4.0 * sigma * 8 + 1e-9).all(), (a, b, sigma)
+
+
+def test_pleochroic_bands_absorb_and_depend_on_the_axis():
+    from indicatrix_amd import workloads
+    clear = _c2_sums(workloads.c2_material(bands=False), 16)
+    dark = _c2_sums(workloads.c2_material(), 16)
+    assert (dark < clear).all()
+    m = workloads.c2_material()
+    m.c_axis = (0.0, 0.0, 1.0)
+    turned = _c2_sums(m, 16)
+    assert np.abs(turned - dark).max() / dark.max() > 1e-3
