@@ -96,13 +96,13 @@ ixg_status ensure_accum(ixg_ctx* ctx, size_t doubles) {
 }
 
 bool material_routable(const ixg_material& m) {
-    // The device path covers isotropic-by-symmetry (cubic, or |n_e - n_o| <=
-    // 1e-4: RayMaterialContext::is_anisotropic, transport/inner.rs:181-198),
-    // polished, unzoned, non-scattering materials, with or without absorption
-    // bands.
+    // The device path covers isotropic and uniaxial (RayMaterialContext::
+    // is_anisotropic, transport/inner.rs:181-198), polished, unzoned,
+    // non-scattering materials, with or without absorption bands; biaxial
+    // materials decline.
     if (m.biaxial || m.scattering_sigma_s > 0.0f || m.edge_rounding_radius > 0.0f) return false;
-    bool anisotropic = !m.cubic && std::fabs(m.birefringence_delta) > 1e-4f;
-    if (anisotropic) return false;
+    if (m.has_e_dispersion && (m.e_dispersion.kind < IXG_DISP_SELLMEIER1 || m.e_dispersion.kind > IXG_DISP_CAUCHY))
+        return false;
     if (m.n_bands_o < 0 || m.n_bands_o > IXG_MAX_BANDS || m.n_bands_e < 0 || m.n_bands_e > IXG_MAX_BANDS)
         return false;
     return m.dispersion.kind >= IXG_DISP_SELLMEIER1 && m.dispersion.kind <= IXG_DISP_CAUCHY;
@@ -168,6 +168,13 @@ ixg_status build_params(ixg_ctx* ctx, const ixg_scene* s, uint32_t sample_offset
     P->disp_kind = s->material->dispersion.kind;
     P->absorb = (s->material->n_bands_o > 0 || s->material->n_bands_e > 0) ? 1 : 0;
     if (P->absorb) fill_absorption(*s->material, &P->abs);
+    const ixg_material& m = *s->material;
+    P->an.on = (!m.cubic && std::fabs(m.birefringence_delta) > 1e-4f) ? 1 : 0;
+    P->an.biref = m.birefringence_delta;
+    P->an.has_e = m.has_e_dispersion ? 1 : 0;
+    P->an.e_kind = m.e_dispersion.kind;
+    std::memcpy(P->an.e_disp, m.e_dispersion.coeff, sizeof P->an.e_disp);
+    std::memcpy(P->an.c_axis, m.c_axis, sizeof P->an.c_axis);
     std::memcpy(P->disp, s->material->dispersion.coeff, sizeof P->disp);
     float g = s->environment.surface_glare;
     P->surface_glare = (g >= 1.0f || std::isnan(g)) ? 1.0f : (g > 0.0f ? g : 0.0f);

@@ -21,6 +21,7 @@
 //   XYZ families            optics/raytracer/color.rs:145-172
 //   per-pixel reduction     renderer/shaders/reduce_xyz.wgsl:55-90, optics/raytracer/sampling.rs:237
 #include "device_common.h"
+#include "uniaxial.h"
 
 namespace ixg {
 
@@ -103,6 +104,8 @@ struct PathState {
     float alpha_o[NCH], alpha_e[NCH];  // spectral_absorption, absorbing materials only
     uint64_t compat;          // byte c = MIS family of channel c
     V3 org, dir, observer, prev_plane;
+    V3 k;                     // wave normal (uniaxial materials; dir is the Poynting direction)
+    bool is_extra;            // eigenmode assigned at the latest entry (uniaxial)
     uint32_t seed;
     uint32_t bounce;
     int32_t last_facet;       // facet of the previous event (-1 before the first)
@@ -120,6 +123,8 @@ __device__ __forceinline__ void path_init(const FilmParams& P, V3 org, V3 dir, u
                                           float hero, PathState& S) {
     S.org = org;
     S.dir = dir;
+    S.k = dir;
+    S.is_extra = false;
     S.observer = neg(dir);
     S.prev_plane = v3(0.0f, 0.0f, 0.0f);
     S.seed = seed;
@@ -211,9 +216,542 @@ __device__ __forceinline__ void apply_absorption(const AbsorbParams& A, PathStat
     }
 }
 
+// ---------------------------------------------------------------------------
+// uniaxial materials (refraction/{geometry,dispatch,tir,uniaxial_entry,
+// uniaxial_internal,reflect_refract}.rs, transport/bounce.rs)
+// ---------------------------------------------------------------------------
+#define BIREFRINGENT_SPLIT_STREAM 0x2545F491u
+#define MODE_COUPLING_STREAM 0xCC9E2D51u
+
+// GemMaterial::extraordinary_index_at (materials/optics.rs:42-47)
+__device__ __forceinline__ float e_index_at(const FilmParams& P, float lambda, float n_o) {
+    return P.an.has_e ? dispersion(P.an.e_kind, P.an.e_disp, lambda) : n_o + P.an.biref;
+}
+
+// entry_eigenmode_selection (refraction/uniaxial_entry.rs:50-75)
+__device__ __forceinline__ bool eigenmode_selection(V3 c, V3 cpn, V3 k, float4 h, float& p_o, float& c2,
+                                                    float& s2) {
+    if (h.x <= 1e-7f || dot(cpn, cpn) <= 1e-6f) return false;
+    if (fma_(h.y, h.y, h.z * h.z) <= 1e-12f) return false;
+    const V3 o_hat = ordinary_eigen(k, c);
+    const V3 p_hat = cross(k, cpn);
+    const float sc = dot(o_hat, cpn), pc = dot(o_hat, p_hat);
+    c2 = fma_(sc, sc, -(pc * pc));
+    s2 = 2.0f * sc * pc;
+    p_o = clampf(0.5f + 0.5f * fma_(h.y, c2, h.z * s2) / h.x, 0.0f, 1.0f);
+    return true;
+}
+
+// try_split_exit_channel (exit_split.rs:128-163) for channel c leaving along dk.
+template <bool EVENTS, bool ISO>
+__device__ __forceinline__ void split_probe(const FilmParams& P, const float4* __restrict__ planes, PathState& S,
+                                            int facet, V3 hit, int c, V3 dk, float transmitted_i) {
+    if (EVENTS) ++S.n_probe;
+    const V3 po = hit + dk * 1e-4f;
+    bool escapes = leaves_convex(planes, facet, po, dk);
+    if (!escapes) {
+        float tt;
+        escapes = intersect(planes, P.n_planes, po, dk, tt) < 0;
+        if (EVENTS) ++S.n_probe_scan;
+    }
+    if (!escapes) return;
+    const float env = ISO ? iso_factor(P.env, dk, S.observer) * (spec_power(P.env, S.lambda[c]) * P.env.exposure)
+                          : env_radiance(P.env, dk, S.observer, S.lambda[c]);
+    S.split[c] = fma_(fmaxf(transmitted_i, 0.0f) * 1.0f, env, S.split[c]);
+}
+
+__device__ __forceinline__ V3 refract_dir(V3 k, V3 normal, float eta, float cos_i, float cos_t) {
+    return normalize(k * eta + normal * fma_(eta, cos_i, -cos_t));
+}
+
+__device__ __forceinline__ V3 reflect_dir(V3 k, V3 normal) { return k - normal * (2.0f * dot(k, normal)); }
+
+// One facet event of a uniaxial material, the same steps as the isotropic
+// arm with compute_bounce_refraction_geometry's anisotropic indices, the
+// closed-form entry / internal arms where the wave is not along the optic
+// axis, the eigenmode-drawing scalar arms where it is, assigned-mode
+// absorption and the internal o<->e re-coupling draw.
+template <bool EVENTS, bool ISO, bool ABSORB>
+__device__ __noinline__ int facet_aniso(const FilmParams& P, const float4* __restrict__ planes, PathState& S,
+                                        float t_hit, int facet) {
+    const uint32_t bounce = S.bounce;
+    const V3 cax = v3(P.an.c_axis[0], P.an.c_axis[1], P.an.c_axis[2]);
+    const V3 hit = S.org + S.dir * t_hit;
+    const float4 pf = planes[facet];
+    V3 normal = v3(pf.x, pf.y, pf.z);
+    const V3 cur_plane = normalize_or_zero(cross(S.k, normal));
+    if (S.has_prev && dot(cur_plane, cur_plane) > 1e-6f && dot(S.prev_plane, S.prev_plane) > 1e-6f) {
+        const float cos_psi = clampf(dot(S.prev_plane, cur_plane), -1.0f, 1.0f);
+        const float sin_psi = dot(cross(S.prev_plane, cur_plane), normalize_or_zero(S.k));
+        const float psi = atan2f(sin_psi, cos_psi);
+        const float c2 = cosf(2.0f * psi), s2 = sinf(2.0f * psi);
+#pragma unroll
+        for (int c = 0; c < NCH; ++c) {
+            const float4 s = S.st[c];
+            S.st[c] = make_float4(s.x, c2 * s.y + s2 * s.z, (-s2) * s.y + c2 * s.z, s.w);
+        }
+        if (EVENTS) ++S.n_rot;
+    }
+    S.prev_plane = cur_plane;
+    S.has_prev = true;
+    const V3 k = S.k;
+    const bool inside = S.inside, is_extra = S.is_extra;
+    if (ABSORB && inside) {
+        // apply_absorption with the assigned-mode coefficient (absorption.rs:224-249)
+        const float n_o_h = S.n_o[0];
+        const float n_e_h = e_index_at(P, S.lambda[0], n_o_h);
+        V3 e;
+        if (!is_extra) {
+            e = ordinary_eigen(k, cax);
+        } else {
+            const V3 kxc = cross(k, cax);
+            if (dot(kxc, kxc) < 1e-8f) e = normalize_or_zero(cross(k, ordinary_eigen(k, cax)));
+            else e = normalize(cross(kxc, extraordinary_poynting_dir(k, cax, n_o_h, n_e_h)));
+        }
+        const float scaled = t_hit * P.abs.path_scale;
+#pragma unroll
+        for (int c = 0; c < NCH; ++c)
+            S.st[c] = scale4(S.st[c], exp_lane(-tensor_qf(P.abs, S.alpha_o[c], S.alpha_e[c], e) * scaled));
+    }
+    if (inside) normal = neg(normal);
+
+    // compute_bounce_refraction_geometry (geometry.rs:306-425), uniaxial
+    const float cos_i = clampf(dot(neg(k), normal), 0.0f, 1.0f);
+    const float one_m_c2 = fma_(cos_i, -cos_i, 1.0f);
+    const float sin_i = sqrtf(fmaxf(one_m_c2, 0.0f));
+    float theta_c;
+    if (!inside) {
+        // theta_c_for_bounce (geometry.rs:75-120): two fixed-point steps
+        const float n_o_seed = S.n_o[0];
+        const float n_e_seed = e_index_at(P, S.lambda[0], n_o_seed);
+        float n_guess = n_o_seed;
+        theta_c = 0.0f;
+        for (int it = 0; it < 2; ++it) {
+            const float eg = 1.0f / n_guess;
+            const float s2g = eg * eg * one_m_c2;
+            if (s2g > 1.0f) break;
+            const float ctg = sqrtf(fmaxf(1.0f - s2g, 0.0f));
+            const V3 wd = refract_dir(k, normal, eg, cos_i, ctg);
+            theta_c = acosf(fabsf(clampf(dot(wd, cax), -1.0f, 1.0f)));
+            n_guess = effective_extraordinary_index(n_o_seed, n_e_seed, theta_c);
+        }
+    } else {
+        theta_c = acosf(fabsf(clampf(dot(k, cax), -1.0f, 1.0f)));
+    }
+    float n_e_ch[NCH], n_med[NCH];
+#pragma unroll
+    for (int c = 0; c < NCH; ++c) {
+        n_e_ch[c] = e_index_at(P, S.lambda[c], S.n_o[c]);
+        const float n_eff = effective_extraordinary_index(S.n_o[c], n_e_ch[c], theta_c);
+        n_med[c] = (inside && !is_extra) ? S.n_o[c] : n_eff;
+    }
+    const float n_o_hero = S.n_o[0];
+    const float n_e_hero = n_o_hero + P.an.biref;
+    const float n1 = inside ? n_med[0] : 1.0f;
+    const float n2 = inside ? 1.0f : n_med[0];
+    const float eta = n1 / n2;
+    const float sin2_t = eta * eta * one_m_c2;
+    const UFrame F = uframe_build(k, normal, cax, cos_i, sin_i);
+    const bool nondegenerate = dot(cross(k, cax), cross(k, cax)) > 1e-6f;
+
+    V3 new_k, new_s;
+    bool new_inside = inside;
+    int upd = -1;
+    float exact_p_o = -1.0f;
+    uint32_t match_bits = 0u;  // channels whose refracted direction matched the hero's
+    if (sin2_t > 1.0f) {
+        // apply_tir_bounce (tir.rs:56-160)
+        if (EVENTS) {
+            ++S.n_tir;
+            S.fh = fnv(fnv(S.fh, (uint32_t)facet), 1u);
+        }
+        if (nondegenerate) {
+#pragma unroll
+            for (int c = 0; c < NCH; ++c) {
+                const InternalSol sol = internal_solve(n_med[c], S.n_o[c], n_e_ch[c], cax, F, !is_extra);
+                const float flux_inc = fmaxf(sol.flux_inc, 1e-12f);
+                const float ro = cnorm2(sol.r_o) * sol.flux_ro, re = cnorm2(sol.r_e) * sol.flux_re;
+                const float r_total = fminf((ro + re) / flux_inc, 1.0f);
+                S.st[c] = scale4(S.st[c], r_total);
+                const float ek = n_med[c] / 1.0f;
+                if (ek * ek * one_m_c2 <= 1.0f) S.path_pdf[c] *= clampf(r_total, R_UNPOL_PDF_MIN, R_UNPOL_PDF_MAX);
+                if (c == 0) exact_p_o = ro / fmaxf(ro + re, 1e-12f);
+            }
+        } else {
+#pragma unroll
+            for (int c = 0; c < NCH; ++c) {
+                const float n1k = n_med[c], n2k = 1.0f;
+                const float ek = n1k / n2k;
+                const float s2k = ek * ek * one_m_c2;
+                if (s2k > 1.0f) {
+                    const float delta = tir_phase_delta(n1k, cos_i, sin_i);
+                    S.st[c] = mueller_tir(cosf(delta), sinf(delta), S.st[c]);
+                } else {
+                    const float ct = sqrtf(fmaxf(1.0f - s2k, 0.0f));
+                    float a_, b_, a, b, cc;
+                    fresnel_rs_rp(n1k, n2k, cos_i, ct, a_, b_);
+                    S.path_pdf[c] *= clampf(0.5f * fma_(b_, b_, a_ * a_), R_UNPOL_PDF_MIN, R_UNPOL_PDF_MAX);
+                    fresnel_reflection_coeffs(a_, b_, a, b, cc);
+                    S.st[c] = mueller_fresnel(a, b, cc, S.st[c]);
+                }
+            }
+        }
+        new_k = reflect_dir(k, normal);
+        new_s = (inside && is_extra) ? extraordinary_poynting_dir(new_k, cax, n_o_hero, n_e_hero) : new_k;
+    } else if (!inside && nondegenerate) {
+        // apply_uniaxial_entry_bounce (uniaxial_entry.rs:108-198)
+        EntryInc I;
+        entry_incidence(F, I);
+        EntrySol hsol;
+        entry_solve(I, S.n_o[0], n_e_ch[0], cax, F, hsol, false);
+        M4 Mh;
+        jones_to_mueller(hsol.x[0][0], hsol.x[1][0], hsol.x[0][1], hsol.x[1][1], Mh);
+        const float r_branch = clampf(Mh.m[0][0], R_UNPOL_SELECT_MIN, R_UNPOL_SELECT_MAX);
+        const float inc_flux = 0.5f * F.cos_i;
+        const float p_o_raw = mode_power(hsol.x[0][2], hsol.x[1][2], hsol.flux_o, inc_flux, S.st[0]);
+        const float p_e_raw = mode_power(hsol.x[0][3], hsol.x[1][3], hsol.flux_e, inc_flux, S.st[0]);
+        const float p_o_hero =
+            clampf(p_o_raw / fmaxf(p_o_raw + p_e_raw, 1e-12f), R_UNPOL_SELECT_MIN, R_UNPOL_SELECT_MAX);
+        const bool use_extra = unit_draw(S.seed, bounce, BIREFRINGENT_SPLIT_STREAM) < (1.0f - p_o_hero);
+        const float p_mode_frac = use_extra ? 1.0f - p_o_hero : p_o_hero;
+        const float n2_dir = use_extra ? n2 : n_o_hero;
+        const float eta_dir = n1 / n2_dir;
+        const float s2d = fminf(eta_dir * eta_dir * one_m_c2, 1.0f);
+        const V3 refr = refract_dir(k, normal, eta_dir, cos_i, sqrtf(fmaxf(1.0f - s2d, 0.0f)));
+        const V3 final_refr = use_extra ? extraordinary_poynting_dir(refr, cax, n_o_hero, n_e_hero) : refr;
+        const V3 p_axis_t = cross(refr, F.s_axis);
+        if (unit_draw(S.seed, bounce, FRESNEL_BRANCH_STREAM) < r_branch) {
+            if (EVENTS) {
+                ++S.n_refl;
+                S.fh = fnv(fnv(S.fh, (uint32_t)facet), 2u);
+            }
+            const float inv_r = 1.0f / r_branch;
+#pragma unroll
+            for (int c = 0; c < NCH; ++c) {
+                EntrySol sol;
+                entry_solve(I, S.n_o[c], n_e_ch[c], cax, F, sol, false);
+                M4 Mk;
+                jones_to_mueller(sol.x[0][0], sol.x[1][0], sol.x[0][1], sol.x[1][1], Mk);
+                const float ru = clampf(Mk.m[0][0], R_UNPOL_PDF_MIN, R_UNPOL_PDF_MAX);
+                S.st[c] = scale4(mat_apply(Mk, S.st[c]), inv_r);
+                S.path_pdf[c] *= ru;
+            }
+            new_k = reflect_dir(k, normal);
+            new_s = new_k;
+        } else {
+            if (EVENTS) {
+                ++S.n_refr;
+                S.fh = fnv(fnv(S.fh, (uint32_t)facet), 3u);
+            }
+            const float inv_t = 1.0f / (1.0f - r_branch);
+            V3 dirs[NCH];
+            uint32_t live = 0u, match_mask = 0u;
+#pragma unroll
+            for (int c = 0; c < NCH; ++c) {
+                const float n_o_k = S.n_o[c];
+                const float n2k = use_extra ? n_med[c] : n_o_k;
+                const float r = n1 / n2k;
+                const float s2k = r * r * one_m_c2;
+                dirs[c] = v3(0.0f, 0.0f, 0.0f);
+                if (s2k > 1.0f) {
+                    S.st[c] = scale4(S.st[c], 0.0f);
+                    S.path_pdf[c] = 0.0f;
+                    continue;
+                }
+                const float ek = n1 / n2k;
+                const V3 rk = refract_dir(k, normal, ek, cos_i, sqrtf(fmaxf(1.0f - s2k, 0.0f)));
+                const V3 fk = use_extra ? extraordinary_poynting_dir(rk, cax, n_o_k, n_o_k + P.an.biref) : rk;
+                dirs[c] = fk;
+                live |= 1u << c;
+                const bool match = dot(fk, final_refr) >= DIRECTION_MATCH_COS_TOL;
+                EntrySol sol;
+                entry_solve(I, n_o_k, n_e_ch[c], cax, F, sol, match);
+                M4 Mk;
+                jones_to_mueller(sol.x[0][0], sol.x[1][0], sol.x[0][1], sol.x[1][1], Mk);
+                const float t_unpol = clampf(1.0f - Mk.m[0][0], R_UNPOL_PDF_MIN, R_UNPOL_PDF_MAX);
+                if (!match) {
+                    S.st[c] = scale4(S.st[c], 0.0f);
+                    S.path_pdf[c] *= t_unpol;
+                    continue;
+                }
+                match_mask |= 1u << c;
+                match_bits |= 1u << c;
+                const Cx tsr = use_extra ? sol.x[0][3] : sol.x[0][2];
+                const Cx tpr = use_extra ? sol.x[1][3] : sol.x[1][2];
+                const float flux_k = use_extra ? sol.flux_e : sol.flux_o;
+                const V3 mode_dir = use_extra ? sol.e_hat : sol.o_hat;
+                const float dep = mode_power(tsr, tpr, flux_k, inc_flux, S.st[c]) / p_mode_frac;
+                float c2, s2;
+                azimuth2_in_frame(mode_dir, F.s_axis, p_axis_t, c2, s2);
+                S.st[c] = scale4(make_float4(dep, dep * c2, dep * s2, 0.0f), inv_t);
+                S.path_pdf[c] *= t_unpol;
+            }
+            // narrow_compat (refraction/context.rs:219-242)
+#pragma unroll
+            for (int a = 0; a < NCH; ++a)
+#pragma unroll
+                for (int b = a + 1; b < NCH; ++b) {
+                    bool m;
+                    if (a == 0) m = (match_mask >> b) & 1u;
+                    else if (((live >> a) & 1u) && ((live >> b) & 1u))
+                        m = dot(dirs[a], dirs[b]) >= DIRECTION_MATCH_COS_TOL;
+                    else m = true;
+                    if (!m) S.compat &= ~((1ull << (8 * a + b)) | (1ull << (8 * b + a)));
+                }
+            new_k = refr;
+            new_s = final_refr;
+            new_inside = true;
+            upd = use_extra ? 1 : 0;
+        }
+    } else if (inside && nondegenerate) {
+        // apply_uniaxial_internal_bounce (uniaxial_internal.rs)
+        const InternalSol sh = internal_solve(n_med[0], S.n_o[0], n_e_ch[0], cax, F, !is_extra);
+        const float flux_inc_h = fmaxf(sh.flux_inc, 1e-12f);
+        const float ro = cnorm2(sh.r_o) * sh.flux_ro, re = cnorm2(sh.r_e) * sh.flux_re;
+        const float r_branch = clampf((ro + re) / flux_inc_h, R_UNPOL_SELECT_MIN, R_UNPOL_SELECT_MAX);
+        const float p_o_exact = ro / fmaxf(ro + re, 1e-12f);
+        if (unit_draw(S.seed, bounce, FRESNEL_BRANCH_STREAM) < r_branch) {
+            if (EVENTS) {
+                ++S.n_refl;
+                S.fh = fnv(fnv(S.fh, (uint32_t)facet), 2u);
+            }
+#pragma unroll
+            for (int c = 0; c < NCH; ++c) {
+                const InternalSol sol = c == 0 ? sh : internal_solve(n_med[c], S.n_o[c], n_e_ch[c], cax, F, !is_extra);
+                const float flux_inc = fmaxf(sol.flux_inc, 1e-12f);
+                const float r_total =
+                    fminf(fma_(cnorm2(sol.r_e), sol.flux_re, cnorm2(sol.r_o) * sol.flux_ro) / flux_inc, 1.0f);
+                S.st[c] = scale4(S.st[c], r_total / r_branch);
+                S.path_pdf[c] *= clampf(r_total, R_UNPOL_PDF_MIN, R_UNPOL_PDF_MAX);
+            }
+            new_k = reflect_dir(k, normal);
+            new_s = new_k;
+            exact_p_o = p_o_exact;
+        } else {
+            if (EVENTS) {
+                ++S.n_refr;
+                S.fh = fnv(fnv(S.fh, (uint32_t)facet), 3u);
+            }
+            const float s2d = fminf(eta * eta * one_m_c2, 1.0f);
+            const V3 hero_refr = refract_dir(k, normal, eta, cos_i, sqrtf(fmaxf(1.0f - s2d, 0.0f)));
+            const float inv_t = 1.0f / (1.0f - r_branch);
+#pragma unroll
+            for (int c = 0; c < NCH; ++c) {
+                const float orig_i = S.st[c].x;
+                const float n_inc = n_med[c];
+                const float r = n_inc / 1.0f;
+                const float s2k = r * r * one_m_c2;
+                if (s2k > 1.0f) {
+                    S.st[c] = scale4(S.st[c], 0.0f);
+                    S.path_pdf[c] = 0.0f;
+                    continue;
+                }
+                const float ek = n_inc / 1.0f;
+                const V3 rk = refract_dir(k, normal, ek, cos_i, sqrtf(fmaxf(1.0f - s2k, 0.0f)));
+                const bool match = dot(rk, hero_refr) >= DIRECTION_MATCH_COS_TOL;
+                if (match) match_bits |= 1u << c;
+                const InternalSol sol = c == 0 ? sh : internal_solve(n_inc, S.n_o[c], n_e_ch[c], cax, F, !is_extra);
+                // compute_uniaxial_exit_transmission (uniaxial_internal.rs)
+                const float flux_inc = fmaxf(sol.flux_inc, 1e-12f);
+                const Cx ts = cscale(sol.t_s, sqrtf(sol.flux_ts / flux_inc));
+                const Cx tp = cscale(sol.t_p, sqrtf(sol.flux_tp / flux_inc));
+                const float iu = cnorm2(ts) + cnorm2(tp);
+                const float qu = cnorm2(ts) - cnorm2(tp);
+                const Cx cr = cmul(ts, cconj(tp));
+                const float4 tr =
+                    scale4(make_float4(orig_i * iu, orig_i * qu, orig_i * (2.0f * cr.re), orig_i * (-2.0f * cr.im)), inv_t);
+                const float t_unpol = clampf(iu, R_UNPOL_PDF_MIN, R_UNPOL_PDF_MAX);
+                if (!match) {
+                    const float prefix = S.path_pdf[c];
+                    S.st[c] = scale4(S.st[c], 0.0f);
+                    S.path_pdf[c] = prefix * t_unpol;
+                    if (orig_i > 0.0f) split_probe<EVENTS, ISO>(P, planes, S, facet, hit, c, rk, fmaxf(tr.x, 0.0f));
+                    continue;
+                }
+                S.st[c] = tr;
+                S.path_pdf[c] *= t_unpol;
+            }
+            new_k = hero_refr;
+            new_s = hero_refr;
+            new_inside = false;
+        }
+    } else {
+        // wave along the optic axis: the scalar arms with the eigenmode draw
+        // (dispatch.rs:143-380, reflect_refract.rs)
+        const bool entering = !inside;
+        float p_o = 0.5f, c2o = 0.0f, s2o = 0.0f;
+        bool have_sel = false;
+        if (entering) {
+            float p;
+            have_sel = eigenmode_selection(cax, cur_plane, k, S.st[0], p, c2o, s2o);
+            if (have_sel) p_o = p;
+        }
+        const float msr = unit_draw(S.seed, bounce, BIREFRINGENT_SPLIT_STREAM);
+        const bool use_extra = entering ? (msr < (1.0f - p_o)) : is_extra;
+        const float c2x = use_extra ? -c2o : c2o, s2x = use_extra ? -s2o : s2o;
+        // compute_entry_reflect_probability (dispatch.rs:200-230)
+        const bool ord_sel = entering && !use_extra;
+        const float n2h = ord_sel ? n_o_hero : n2;
+        float s2h;
+        if (ord_sel) {
+            const float e2 = n1 / n2h;
+            s2h = e2 * e2 * one_m_c2;
+        } else {
+            s2h = sin2_t;
+        }
+        float rs_h, rp_h;
+        fresnel_rs_rp(n1, n2h, cos_i, sqrtf(fmaxf(1.0f - s2h, 0.0f)), rs_h, rp_h);
+        const float r_unpol = clampf(0.5f * fma_(rp_h, rp_h, rs_h * rs_h), R_UNPOL_SELECT_MIN, R_UNPOL_SELECT_MAX);
+        if (unit_draw(S.seed, bounce, FRESNEL_BRANCH_STREAM) < r_unpol) {
+            if (EVENTS) {
+                ++S.n_refl;
+                S.fh = fnv(fnv(S.fh, (uint32_t)facet), 2u);
+            }
+            const float sc = 1.0f / r_unpol;
+#pragma unroll
+            for (int c = 0; c < NCH; ++c) {
+                const float n1k = inside ? n_med[c] : 1.0f, n2k = inside ? 1.0f : n_med[c];
+                const float ek = n1k / n2k;
+                const float s2k = ek * ek * one_m_c2;
+                float4 st;
+                if (s2k > 1.0f) {
+                    const float delta = tir_phase_delta(n1k, cos_i, sin_i);
+                    st = mueller_tir(cosf(delta), sinf(delta), S.st[c]);
+                } else {
+                    const float ct = sqrtf(fmaxf(1.0f - s2k, 0.0f));
+                    float a_, b_, a, b, cc;
+                    fresnel_rs_rp(n1k, n2k, cos_i, ct, a_, b_);
+                    S.path_pdf[c] *= clampf(0.5f * fma_(b_, b_, a_ * a_), R_UNPOL_PDF_MIN, R_UNPOL_PDF_MAX);
+                    fresnel_reflection_coeffs(a_, b_, a, b, cc);
+                    st = mueller_fresnel(a, b, cc, S.st[c]);
+                }
+                S.st[c] = scale4(st, sc);
+            }
+            new_k = reflect_dir(k, normal);
+            new_s = (inside && is_extra) ? extraordinary_poynting_dir(new_k, cax, n_o_hero, n_e_hero) : new_k;
+        } else {
+            if (EVENTS) {
+                ++S.n_refr;
+                S.fh = fnv(fnv(S.fh, (uint32_t)facet), 3u);
+            }
+            const float n2d = (entering && !use_extra) ? n_o_hero : n2;
+            const float eta_dir = n1 / n2d;
+            const float s2d = fminf(eta_dir * eta_dir * one_m_c2, 1.0f);
+            const V3 refr = refract_dir(k, normal, eta_dir, cos_i, sqrtf(fmaxf(1.0f - s2d, 0.0f)));
+            const V3 final_refr =
+                (entering && use_extra) ? extraordinary_poynting_dir(refr, cax, n_o_hero, n_e_hero) : refr;
+            const float inv_t = 1.0f / (1.0f - r_unpol);
+            V3 dirs[NCH];
+            uint32_t live = 0u, match_mask = 0u;
+#pragma unroll
+            for (int c = 0; c < NCH; ++c) {
+                const float4 orig = S.st[c];
+                const float n1k = inside ? n_med[c] : 1.0f;
+                const float n2k = (entering && !use_extra) ? S.n_o[c] : (inside ? 1.0f : n_med[c]);
+                const float r = n1k / n2k;
+                const float s2k = r * r * one_m_c2;
+                dirs[c] = v3(0.0f, 0.0f, 0.0f);
+                if (s2k > 1.0f) {
+                    S.st[c] = scale4(S.st[c], 0.0f);
+                    S.path_pdf[c] = 0.0f;
+                    continue;
+                }
+                const float ctk = sqrtf(fmaxf(1.0f - s2k, 0.0f));
+                const V3 rk = refract_dir(k, normal, n1k / n2k, cos_i, ctk);
+                const V3 fk = (entering && use_extra)
+                                  ? extraordinary_poynting_dir(rk, cax, S.n_o[c], S.n_o[c] + P.an.biref)
+                                  : rk;
+                const bool match = dot(fk, final_refr) >= DIRECTION_MATCH_COS_TOL;
+                // compute_channel_transmission (exit_split.rs:69-101)
+                const float num = 2.0f * n1k * cos_i;
+                const float t_s = num / fma_(n2k, ctk, n1k * cos_i);
+                const float t_p = num / fma_(n1k, ctk, n2k * cos_i);
+                const float factor = (n2k * ctk) / fmaxf(n1k * cos_i, 1e-6f);
+                const float ts2 = t_s * t_s * factor, tp2 = t_p * t_p * factor;
+                const float4 incident =
+                    (entering && have_sel) ? make_float4(orig.x, orig.x * c2x, orig.x * s2x, 0.0f) : orig;
+                const float4 tr =
+                    scale4(mueller_fresnel(midpoint(ts2, tp2), 0.5f * (ts2 - tp2), t_s * t_p * factor, incident), inv_t);
+                float a_, b_;
+                fresnel_rs_rp(n1k, n2k, cos_i, ctk, a_, b_);
+                const float ruk = clampf(0.5f * fma_(b_, b_, a_ * a_), R_UNPOL_PDF_MIN, R_UNPOL_PDF_MAX);
+                if (match) {
+                    S.st[c] = tr;
+                    S.path_pdf[c] *= 1.0f - ruk;
+                    match_mask |= 1u << c;
+                    match_bits |= 1u << c;
+                } else {
+                    const float prefix = S.path_pdf[c];
+                    S.st[c] = scale4(S.st[c], 0.0f);
+                    S.path_pdf[c] = prefix * (1.0f - ruk);
+                    if (inside && fmaxf(orig.x, 0.0f) > 0.0f)
+                        split_probe<EVENTS, ISO>(P, planes, S, facet, hit, c, rk, fmaxf(tr.x, 0.0f));
+                }
+                dirs[c] = fk;
+                live |= 1u << c;
+            }
+            if (!inside) {
+#pragma unroll
+                for (int a = 0; a < NCH; ++a)
+#pragma unroll
+                    for (int b = a + 1; b < NCH; ++b) {
+                        bool m;
+                        if (a == 0) m = (match_mask >> b) & 1u;
+                        else if (((live >> a) & 1u) && ((live >> b) & 1u))
+                            m = dot(dirs[a], dirs[b]) >= DIRECTION_MATCH_COS_TOL;
+                        else m = true;
+                        if (!m) S.compat &= ~((1ull << (8 * a + b)) | (1ull << (8 * b + a)));
+                    }
+            }
+            new_k = refr;
+            new_s = final_refr;
+            new_inside = !inside;
+            if (entering) upd = use_extra ? 1 : 0;
+        }
+    }
+    // dispatch_bounce tail (transport/bounce.rs:200-238)
+    const bool was_internal_reflection = inside && upd < 0 && new_inside == inside;
+    S.dir = new_s;
+    S.org = hit + new_s * 1e-4f;
+    S.k = new_k;
+    S.inside = new_inside;
+    S.last_facet = facet;
+    if (upd >= 0) S.is_extra = upd == 1;
+    if (was_internal_reflection) {
+        // apply_internal_mode_coupling (transport/bounce.rs:439-470)
+        float p_o = 0.5f;
+        if (exact_p_o >= 0.0f) {
+            p_o = exact_p_o;
+        } else {
+            float p, c2, s2;
+            if (eigenmode_selection(cax, cur_plane, new_k, S.st[0], p, c2, s2)) p_o = p;
+        }
+        S.is_extra = unit_draw(S.seed, bounce, MODE_COUPLING_STREAM) >= p_o;
+    }
+    // the side channel also records the eigenmode after every event
+    if (EVENTS) S.fh = fnv(fnv(S.fh, 16u + (S.is_extra ? 1u : 0u)), 256u + match_bits);
+    if (bounce == 0 && !new_inside && P.surface_glare < 1.0f) {
+#pragma unroll
+        for (int c = 0; c < NCH; ++c) S.st[c] = scale4(S.st[c], P.surface_glare);
+    }
+    if (bounce > 4) {
+        float mx = 0.0f;
+#pragma unroll
+        for (int c = 0; c < NCH; ++c) mx = fmaxf(mx, fmaxf(S.st[c].x, 0.0f));
+        const float q = clampf(mx, 0.05f, 1.0f);
+        if (unit_draw(S.seed, bounce, RUSSIAN_ROULETTE_STREAM) > q) return STEP_ROULETTE;
+        const float inv_q = 1.0f / q;
+#pragma unroll
+        for (int c = 0; c < NCH; ++c) S.st[c] = scale4(S.st[c], inv_q);
+#pragma unroll
+        for (int c = 0; c < NCH; ++c) S.split[c] /= q;
+    }
+    return STEP_CONTINUE;
+}
+
 // One iteration of the bounce loop (bounce_body.rs:361-707): intersection, and
 // either the escape lookup (written to `radiance`) or the facet event.
-template <bool EVENTS, bool ISO = false, bool ABSORB = false>
+template <bool EVENTS, bool ISO = false, bool ABSORB = false, bool ANISO = false>
 __device__ __forceinline__ int path_bounce(const FilmParams& P, const float4* __restrict__ planes,
                                            PathState& S, float radiance[NCH]) {
     const uint32_t bounce = S.bounce;
@@ -246,6 +784,7 @@ __device__ __forceinline__ int path_bounce(const FilmParams& P, const float4* __
         return STEP_ESCAPED;
     }
 
+    if (ANISO) return facet_aniso<EVENTS, ISO, ABSORB>(P, planes, S, t_hit, facet);
     // facet_event (bounce_body.rs:611-707)
     const V3 k = S.dir;
     const V3 hit = S.org + k * t_hit;
@@ -477,7 +1016,7 @@ __device__ __forceinline__ void path_finish(const FilmParams& P, const PathState
 }
 
 // trace_spectral_ray (transport/mod.rs:118) for one explicit ray, start to end.
-template <bool EVENTS, bool ABSORB>
+template <bool EVENTS, bool ABSORB, bool ANISO>
 __device__ __forceinline__ void trace_path(const FilmParams& P, const float4* __restrict__ planes,
                                            V3 org, V3 dir, const uint32_t seed, const float hero,
                                            PathOut& out, uint32_t* counts) {
@@ -490,7 +1029,7 @@ __device__ __forceinline__ void trace_path(const FilmParams& P, const float4* __
     bool escaped = false;
     for (uint32_t b = 0; b < P.max_bounces; ++b) {
         S.bounce = b;
-        int r = path_bounce<EVENTS, false, ABSORB>(P, planes, S, radiance);
+        int r = path_bounce<EVENTS, false, ABSORB, ANISO>(P, planes, S, radiance);
         if (r != STEP_CONTINUE) {
             bounces = b + 1;
             term = r == STEP_ESCAPED ? 0u : 2u;
@@ -542,7 +1081,7 @@ __device__ __forceinline__ void stage_planes(const FilmParams& P, const float4* 
 }
 
 // Film: one lane per path, raw per-sample XYZ to `out_xyz` (path order).
-template <bool ABSORB>
+template <bool ABSORB, bool ANISO>
 __global__ __launch_bounds__(kBlock) void film_transport_kernel(const FilmParams P,
                                                                 const float4* __restrict__ planes,
                                                                 float* __restrict__ out_xyz) {
@@ -558,7 +1097,7 @@ __global__ __launch_bounds__(kBlock) void film_transport_kernel(const FilmParams
     float hero;
     camera_sample(P, pixel, sample, org, dir, seed, hero);
     PathOut o;
-    trace_path<false, ABSORB>(P, sp, org, dir, seed, hero, o, nullptr);
+    trace_path<false, ABSORB, ANISO>(P, sp, org, dir, seed, hero, o, nullptr);
     out_xyz[3 * i + 0] = o.xyz[0];
     out_xyz[3 * i + 1] = o.xyz[1];
     out_xyz[3 * i + 2] = o.xyz[2];
@@ -570,7 +1109,7 @@ __global__ __launch_bounds__(kBlock) void film_transport_kernel(const FilmParams
 // (paths run 1..max_bounces iterations, mostly 2-6), and the per-path results
 // are the same as film_transport_kernel's because a path is a pure function
 // of its (pixel, sample) index.
-template <bool ISO, bool ABSORB>
+template <bool ISO, bool ABSORB, bool ANISO>
 __global__ __launch_bounds__(kBlock) IXG_WAVES_ATTR void film_persistent_kernel(const FilmParams P,
                                                                  const float4* __restrict__ planes,
                                                                  float* __restrict__ out_xyz,
@@ -612,7 +1151,7 @@ __global__ __launch_bounds__(kBlock) IXG_WAVES_ATTR void film_persistent_kernel(
         if (__ballot(active) == 0ull) break;
         if (active) {
             int r = STEP_CONTINUE;
-            if (S.bounce < P.max_bounces) r = path_bounce<false, ISO, ABSORB>(P, sp, S, radiance);
+            if (S.bounce < P.max_bounces) r = path_bounce<false, ISO, ABSORB, ANISO>(P, sp, S, radiance);
             const bool fin = r != STEP_CONTINUE || ++S.bounce >= P.max_bounces;
             if (fin) {
                 float xyz[3];
@@ -627,7 +1166,7 @@ __global__ __launch_bounds__(kBlock) IXG_WAVES_ATTR void film_persistent_kernel(
 }
 
 // Diagnostics: per-path XYZ + integer side channel.
-template <bool ABSORB>
+template <bool ABSORB, bool ANISO>
 __global__ __launch_bounds__(kBlock) void film_events_kernel(const FilmParams P,
                                                              const float4* __restrict__ planes,
                                                              float* __restrict__ out_xyz,
@@ -648,7 +1187,7 @@ __global__ __launch_bounds__(kBlock) void film_events_kernel(const FilmParams P,
         float hero;
         camera_sample(P, pixel, sample, org, dir, seed, hero);
         PathOut o;
-        trace_path<true, ABSORB>(P, sp, org, dir, seed, hero, o, counts);
+        trace_path<true, ABSORB, ANISO>(P, sp, org, dir, seed, hero, o, counts);
         counts[EV_PATHS] = 1;
         if (out_xyz) {
             out_xyz[3 * i + 0] = o.xyz[0];
@@ -670,7 +1209,7 @@ __global__ __launch_bounds__(kBlock) void film_events_kernel(const FilmParams P,
 
 // Explicit rays (the reference's trace_spectral_ray entry): ray i starts at
 // rays[i] = {ox, oy, oz, dx, dy, dz, hero_rand, seed bits}.
-template <bool ABSORB>
+template <bool ABSORB, bool ANISO>
 __global__ __launch_bounds__(kBlock) void film_rays_kernel(const FilmParams P,
                                                            const float4* __restrict__ planes,
                                                            const float* __restrict__ rays,
@@ -683,7 +1222,7 @@ __global__ __launch_bounds__(kBlock) void film_rays_kernel(const FilmParams P,
     if (i >= n_rays) return;
     const float* r = rays + 8 * (size_t)i;
     PathOut o;
-    trace_path<true, ABSORB>(P, sp, v3(r[0], r[1], r[2]), v3(r[3], r[4], r[5]), __float_as_uint(r[7]),
+    trace_path<true, ABSORB, ANISO>(P, sp, v3(r[0], r[1], r[2]), v3(r[3], r[4], r[5]), __float_as_uint(r[7]),
                              r[6], o, nullptr);
     out_xyz[3 * i + 0] = o.xyz[0];
     out_xyz[3 * i + 1] = o.xyz[1];
@@ -720,15 +1259,23 @@ __global__ __launch_bounds__(kBlock) void film_reduce_kernel(const float* __rest
 // ---------------------------------------------------------------------------
 // launchers (internal to libixg)
 // ---------------------------------------------------------------------------
+// One instantiation per (absorbing, uniaxial) material class, so the
+// isotropic clear-stone kernel carries none of the other arms' registers.
+#define IXG_DISPATCH2(L, A, N)   \
+    do {                         \
+        if (!(A) && !(N)) L(false, false); \
+        else if ((A) && !(N)) L(true, false); \
+        else if (!(A)) L(false, true); \
+        else L(true, true);      \
+    } while (0)
 extern "C" hipError_t ixg_launch_film(const ixg::FilmParams* P, const float4* d_planes, float* d_xyz,
                                       hipStream_t stream) {
     uint64_t blocks = (P->n_paths + ixg::kBlock - 1) / ixg::kBlock;
-    if (P->absorb)
-        hipLaunchKernelGGL(ixg::film_transport_kernel<true>, dim3((uint32_t)blocks), dim3(ixg::kBlock), 0,
-                           stream, *P, d_planes, d_xyz);
-    else
-        hipLaunchKernelGGL(ixg::film_transport_kernel<false>, dim3((uint32_t)blocks), dim3(ixg::kBlock), 0,
-                           stream, *P, d_planes, d_xyz);
+#define IXG_T(A, N) \
+    hipLaunchKernelGGL((ixg::film_transport_kernel<A, N>), dim3((uint32_t)blocks), dim3(ixg::kBlock), 0, stream, *P, \
+                       d_planes, d_xyz)
+    IXG_DISPATCH2(IXG_T, P->absorb, P->an.on);
+#undef IXG_T
     return hipGetLastError();
 }
 
@@ -736,12 +1283,11 @@ extern "C" hipError_t ixg_launch_film_events(const ixg::FilmParams* P, const flo
                                              float* d_xyz, ixg_path_events* d_ev,
                                              unsigned long long* d_totals, hipStream_t stream) {
     uint64_t blocks = (P->n_paths + ixg::kBlock - 1) / ixg::kBlock;
-    if (P->absorb)
-        hipLaunchKernelGGL(ixg::film_events_kernel<true>, dim3((uint32_t)blocks), dim3(ixg::kBlock), 0,
-                           stream, *P, d_planes, d_xyz, d_ev, d_totals);
-    else
-        hipLaunchKernelGGL(ixg::film_events_kernel<false>, dim3((uint32_t)blocks), dim3(ixg::kBlock), 0,
-                           stream, *P, d_planes, d_xyz, d_ev, d_totals);
+#define IXG_E(A, N) \
+    hipLaunchKernelGGL((ixg::film_events_kernel<A, N>), dim3((uint32_t)blocks), dim3(ixg::kBlock), 0, stream, *P, \
+                       d_planes, d_xyz, d_ev, d_totals)
+    IXG_DISPATCH2(IXG_E, P->absorb, P->an.on);
+#undef IXG_E
     return hipGetLastError();
 }
 
@@ -757,12 +1303,11 @@ extern "C" hipError_t ixg_launch_rays(const ixg::FilmParams* P, const float4* d_
                                       uint32_t n_rays, float* d_xyz, ixg_path_events* d_ev,
                                       hipStream_t stream) {
     uint32_t blocks = (n_rays + ixg::kBlock - 1) / ixg::kBlock;
-    if (P->absorb)
-        hipLaunchKernelGGL(ixg::film_rays_kernel<true>, dim3(blocks), dim3(ixg::kBlock), 0, stream, *P,
-                           d_planes, d_rays, n_rays, d_xyz, d_ev);
-    else
-        hipLaunchKernelGGL(ixg::film_rays_kernel<false>, dim3(blocks), dim3(ixg::kBlock), 0, stream, *P,
-                           d_planes, d_rays, n_rays, d_xyz, d_ev);
+#define IXG_R(A, N) \
+    hipLaunchKernelGGL((ixg::film_rays_kernel<A, N>), dim3(blocks), dim3(ixg::kBlock), 0, stream, *P, d_planes, \
+                       d_rays, n_rays, d_xyz, d_ev)
+    IXG_DISPATCH2(IXG_R, P->absorb, P->an.on);
+#undef IXG_R
     return hipGetLastError();
 }
 
@@ -771,20 +1316,22 @@ extern "C" hipError_t ixg_launch_film_persistent(const ixg::FilmParams* P, const
     // a fill kernel rather than a memset node, so the launch pair can be captured
     hipLaunchKernelGGL(ixg::zero_counter_kernel, dim3(1), dim3(1), 0, stream, d_counter);
     const bool iso = P->env.model == ixg::ENV_ISO_HEMISPHERE;
-#define IXG_PERSISTENT(ISO, ABS)                                                                              \
-    hipLaunchKernelGGL((ixg::film_persistent_kernel<ISO, ABS>), dim3(grid_blocks), dim3(ixg::kBlock), 0, stream, \
+#define IXG_PI(A, N)                                                                                           \
+    hipLaunchKernelGGL((ixg::film_persistent_kernel<true, A, N>), dim3(grid_blocks), dim3(ixg::kBlock), 0, stream, \
                        *P, d_planes, d_xyz, d_counter)
-    if (iso && !P->absorb) IXG_PERSISTENT(true, false);
-    else if (iso) IXG_PERSISTENT(true, true);
-    else if (!P->absorb) IXG_PERSISTENT(false, false);
-    else IXG_PERSISTENT(false, true);
-#undef IXG_PERSISTENT
+#define IXG_PS(A, N)                                                                                            \
+    hipLaunchKernelGGL((ixg::film_persistent_kernel<false, A, N>), dim3(grid_blocks), dim3(ixg::kBlock), 0, stream, \
+                       *P, d_planes, d_xyz, d_counter)
+    if (iso) IXG_DISPATCH2(IXG_PI, P->absorb, P->an.on);
+    else IXG_DISPATCH2(IXG_PS, P->absorb, P->an.on);
+#undef IXG_PI
+#undef IXG_PS
     return hipGetLastError();
 }
 
 extern "C" hipError_t ixg_persistent_grid(int device, uint32_t* blocks) {
     int per_cu = 0, cus = 0;
-    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, ixg::film_persistent_kernel<true, false>,
+    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, ixg::film_persistent_kernel<true, false, false>,
                                                                 ixg::kBlock, 0);
     if (e != hipSuccess) return e;
     e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device);

@@ -41,6 +41,18 @@ struct AbsorbParams {
     AbsBand e[8];
 };
 
+// A uniaxial material (RayMaterialContext::is_anisotropic): the ordinary
+// curve is FilmParams::disp, the extraordinary one e_disp when given, else
+// n_o + biref.
+struct AnisoParams {
+    int32_t on;
+    float biref;
+    int32_t has_e;
+    int32_t e_kind;
+    float e_disp[6];
+    float c_axis[3];  // as given
+};
+
 struct FilmParams {
     // GpuCameraParams fields
     float cam_origin[3];
@@ -64,6 +76,7 @@ struct FilmParams {
     int32_t absorb;       // 1: the material has absorption bands
     EnvParams env;
     AbsorbParams abs;
+    AnisoParams an;
 };
 
 // Event-count slots accumulated by the counting kernel.

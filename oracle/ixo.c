@@ -1089,6 +1089,7 @@ static void facet_event_aniso(const scene_t* S, path_t* P, uint32_t bounce, uint
     v3 new_k, new_s;
     int new_inside = inside, upd = -1;
     float exact_p_o = -1.0f;
+    uint32_t match_bits = 0u; /* channels whose refracted direction matched the hero's */
     if (G.sin2_t > 1.0f) {
         /* apply_tir_bounce (refraction/tir.rs:56-160) */
         if (ev) ev->n_tir++;
@@ -1181,6 +1182,7 @@ static void facet_event_aniso(const scene_t* S, path_t* P, uint32_t bounce, uint
                 has_dir[c] = 1;
                 int match = vdot(fk, final_refr) >= DIRECTION_MATCH_COS_TOL;
                 hero_match[c] = match;
+                if (match) match_bits |= 1u << c;
                 entry_sol ss, sp;
                 entry_solve_pair(&I, 1.0f, n_o_k, n_e_ch[c], cax, &G.F, &ss, &sp);
                 if (!match) {
@@ -1249,6 +1251,7 @@ static void facet_event_aniso(const scene_t* S, path_t* P, uint32_t bounce, uint
                 float ek = n_inc / n2k;
                 v3 rk = refract_dir(k, normal, ek, G.cos_i, ctk);
                 int match = vdot(rk, hero_refr) >= DIRECTION_MATCH_COS_TOL;
+                if (match) match_bits |= 1u << c;
                 internal_sol sol = c == 0 ? sh : internal_solve(n_inc, P->n_o[c], n_e_ch[c], cax, &G.F, !is_extra);
                 float iu;
                 if (!match) {
@@ -1345,6 +1348,7 @@ static void facet_event_aniso(const scene_t* S, path_t* P, uint32_t bounce, uint
                             ? extraordinary_poynting_dir(rk, cax, P->n_o[c], P->n_o[c] + M->birefringence_delta)
                             : rk;
                 int match = vdot(fk, final_refr) >= DIRECTION_MATCH_COS_TOL;
+                if (match) match_bits |= 1u << c;
                 /* compute_channel_transmission (exit_split.rs:69-101) */
                 float num = 2.0f * n1k * G.cos_i;
                 float t_s = num / fmaf(n2k, ctk, n1k * G.cos_i);
@@ -1397,6 +1401,11 @@ static void facet_event_aniso(const scene_t* S, path_t* P, uint32_t bounce, uint
             if (entry_eigenmode_selection(cax, cur_plane, new_k, P->st[0], &p, &c2, &s2)) p_o = p;
         }
         P->is_extra = unit_draw(seed, bounce, MODE_COUPLING_STREAM) >= p_o;
+    }
+    /* the side channel also records the eigenmode after every event */
+    if (ev) {
+        ev->facet_hash = (ev->facet_hash ^ (16u + (uint32_t)P->is_extra)) * 16777619u;
+        ev->facet_hash = (ev->facet_hash ^ (256u + match_bits)) * 16777619u;
     }
     if (bounce == 0 && !P->inside && S->surface_glare < 1.0f)
         for (int c = 0; c < NCH; ++c) P->st[c] = st_scale(P->st[c], S->surface_glare);

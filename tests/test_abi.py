@@ -69,6 +69,8 @@ def test_route_predicate():
     m.cubic = False
     m.birefringence_delta = 0.009
     sc2 = tracer.FrameScene(sc.scene.camera, 8, 8, sc.planes, m, 16, sc.scene.environment)
+    assert tracer.route(sc2) == abi.ROUTE_OK                  # uniaxial
+    sc2.material_abi.biaxial = 1
     assert tracer.route(sc2) == abi.ROUTE_UNSUPPORTED_MATERIAL
     env = tracer.LightingPreset.studio("LightTent")
     sc3 = tracer.FrameScene(sc.scene.camera, 8, 8, sc.planes, tracer.GemMaterial.diamond(), 16, env)
