@@ -61,6 +61,45 @@ def copy_calibration():
     print(json.dumps({"copy_bytes_per_launch": a.numel() * 4, "launches": 3}), flush=True)
 
 
+def film_c2_rate(backend, steps):
+    """C2: the 121-facet seeded oval in a uniaxial pleochroic material,
+    1024x1024 @ 256 spp, depth 24 (one full frame per step)."""
+    import torch
+    from indicatrix_amd import workloads
+    sc = workloads.film_c2_scene()
+    spp = workloads.C2["spp"]
+    paths = sc.width * sc.height * spp
+    accum = torch.zeros(sc.width * sc.height * 3, dtype=torch.float64, device="cuda")
+    backend.accumulate_device(sc, 0, 4, accum.data_ptr())   # warm
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(steps):
+        backend.accumulate_device(sc, 4 + k * spp, spp, accum.data_ptr())
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    return {"workload": "film 1024x1024 @ 256 spp, depth 24, 121-facet seeded oval (tier angles U[30,50]), "
+                        "uniaxial n_e - n_o = 0.009, pleochroic Gaussian o/e bands, random optic axis",
+            "steps": steps, "ms_per_step": round(dt * 1e3 / steps, 2),
+            "value": round(paths * steps / dt / 1e6, 2), "unit": "Msamples/s"}, sc
+
+
+def film_c2_cpu_baseline(sc, target_s=8.0):
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib
+    threads = min(os.cpu_count() or 1, 16)
+    t0 = time.perf_counter()
+    oracle_lib.accumulate(sc, 0, 1, nthreads=threads)
+    one = time.perf_counter() - t0
+    spp = max(1, min(64, int(target_s / max(one, 1e-3))))
+    t0 = time.perf_counter()
+    oracle_lib.accumulate(sc, 1, spp, nthreads=threads)
+    dt = time.perf_counter() - t0
+    paths = sc.width * sc.height * spp
+    return {"value": round(paths / dt / 1e6, 4), "unit": "Msamples/s", "cores": threads, "kind": "port",
+            "sample": f"the C2 frame at {spp} spp (sample indices 1..{spp}): {paths} paths in {dt:.2f} s "
+                      f"on {threads} threads"}
+
+
 def sweep_rate(backend, n_designs=512):
     """C3: seeded designs (32..256 planes, Sellmeier-1 from n_D / Abbe) x 21
     tilts, 64x64 @ 32 samples, one ixg_metric_sweep call."""
@@ -225,6 +264,9 @@ def main():
                      "flops_per_launch": step_flops, "events": totals},
     }
     if not args.film_only:
+        print("# bench: film C2", file=sys.stderr, flush=True)
+        c2, c2_scene = film_c2_rate(backend, 2)
+        line["film_c2"] = c2
         print("# bench: metric sweep", file=sys.stderr, flush=True)
         sweep, sweep_inputs = sweep_rate(backend)
         print("# bench: screened optimize", file=sys.stderr, flush=True)
@@ -234,6 +276,7 @@ def main():
         line["cpu_baseline"] = cpu_baseline(scene, args.cpu_seconds)
         if not args.film_only:
             line["design_evals"]["sweep"]["cpu_baseline"] = sweep_cpu_baseline(*sweep_inputs)
+            line["film_c2"]["cpu_baseline"] = film_c2_cpu_baseline(c2_scene)
     print(json.dumps(line), flush=True)
     backend.close()
 

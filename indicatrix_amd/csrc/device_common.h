@@ -207,8 +207,10 @@ __device__ __forceinline__ float iso_factor(const EnvParams& E, V3 dir, V3 obser
 // ---------------------------------------------------------------------------
 // intersection against the LDS facet set
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ int intersect(const float4* __restrict__ pl, uint32_t n, V3 o, V3 d,
-                                         float& t_out) {
+// The reference scan (RT/intersect.rs:101-132, simd/slab.rs:147-160): one
+// correctly rounded division per plane, t_near the largest entering t and
+// t_far the smallest exiting t, ties to the lowest index.
+__device__ __forceinline__ int intersect(const float4* __restrict__ pl, uint32_t n, V3 o, V3 d, float& t_out) {
     float t_near = -1e30f, t_far = 1e30f;
     int near_idx = -1, far_idx = -1;
     for (uint32_t i = 0; i < n; ++i) {
