@@ -231,3 +231,22 @@ def test_event_totals_match_per_path_events(backend):
     assert tot["refract"] == int(ev["n_refract"].sum())
     assert tot["probe"] == int(ev["n_probe"].sum())
     assert tot["escaped"] == int((ev["termination"] == 0).sum())
+
+
+@pytest.mark.parametrize("scene", ["c1", "absorbing", "uniaxial"])
+def test_statistical_image_equivalence(backend, scene):
+    """Tier 3 of the reference's CPU-vs-GPU harness (renderer/gpu/z_stats.rs):
+    the device renders sample indices [0, 32) and the oracle the independent
+    range [32, 64); per-pixel z-scores of X, Y, Z pass the reference's
+    criteria (3-sigma count and 4-connected cluster size)."""
+    import zstats
+    w = h = 40
+    spp = 32
+    sc = {"c1": lambda: workloads.film_c1_scene(w, h),
+          "absorbing": lambda: workloads.film_absorbing_scene(w, h),
+          "uniaxial": lambda: workloads.film_c2_scene(w, h, spp, max_bounces=24)}[scene]()
+    print(f"z-score {scene} {w}x{h}x{spp}", flush=True)
+    g_xyz, _ = backend.trace_paths(sc, 0, spp)
+    o_xyz, _ = oracle_lib.trace_paths(sc, spp, spp)
+    ok, res = zstats.image_equivalent(o_xyz, g_xyz, w, h, spp)
+    assert ok, res

@@ -521,3 +521,23 @@ def test_pleochroic_bands_absorb_and_depend_on_the_axis():
     m.c_axis = (0.0, 0.0, 1.0)
     turned = _c2_sums(m, 16)
     assert np.abs(turned - dark).max() / dark.max() > 1e-3
+
+
+# ---------------------------------------------------------------------------
+# statistical image equivalence (renderer/gpu/z_stats.rs), pinned on the CPU
+# ---------------------------------------------------------------------------
+def test_z_criteria_accept_independent_samples_and_reject_a_planted_bias():
+    import zstats
+    from indicatrix_amd import workloads
+    w = h = 40
+    spp = 32
+    sc = workloads.film_c1_scene(w, h)
+    a, _ = oracle_lib.trace_paths(sc, 0, spp)
+    b, _ = oracle_lib.trace_paths(sc, spp, spp)
+    ok, res = zstats.image_equivalent(a, b, w, h, spp)
+    assert ok, res
+    # a planted mistake -- half the light of every path kept twice over -- is
+    # caught: per-pixel tests at 32 spp resolve gross or local errors, while
+    # the frame-level sums elsewhere resolve small global ones
+    ok_bad, res_bad = zstats.image_equivalent(a, b * 1.5, w, h, spp)
+    assert not ok_bad, res_bad
