@@ -1071,6 +1071,10 @@ constexpr int kBlock = 256;
 #define IXG_FILM_WAVES_PER_EU 3
 #endif
 #define IXG_WAVES_ATTR __attribute__((amdgpu_waves_per_eu(IXG_FILM_WAVES_PER_EU)))
+// Idle lanes a wave collects before it finishes and regenerates paths.
+#ifndef IXG_FILM_REGEN_MIN
+#define IXG_FILM_REGEN_MIN 24
+#endif
 
 __global__ void zero_counter_kernel(uint32_t* c) { *c = 0u; }
 
@@ -1122,10 +1126,24 @@ __global__ __launch_bounds__(kBlock) IXG_WAVES_ATTR void film_persistent_kernel(
     float radiance[NCH];
     uint32_t idx = 0;
     bool active = false, done = false;
+    bool pending = false, pending_escaped = false;  // finished, XYZ not yet written
     while (true) {
+        // Regenerate (finish the lanes that ended, then take new paths) once
+        // IXG_FILM_REGEN_MIN lanes are idle, or when no lane is active: the
+        // finish / camera / dispersion code then runs once for a batch of
+        // lanes instead of on nearly every bounce.
         const bool need = !active && !done;
         const uint64_t need_mask = __ballot(need);
-        if (need_mask) {
+        const uint64_t active_mask = __ballot(active);
+        if (need_mask && (__popcll(need_mask) >= IXG_FILM_REGEN_MIN || active_mask == 0ull)) {
+            if (pending) {
+                float xyz[3];
+                path_finish(P, S, pending_escaped, radiance, xyz);
+                out_xyz[3 * (size_t)idx + 0] = xyz[0];
+                out_xyz[3 * (size_t)idx + 1] = xyz[1];
+                out_xyz[3 * (size_t)idx + 2] = xyz[2];
+                pending = false;
+            }
             const uint32_t leader = (uint32_t)(__ffsll((unsigned long long)need_mask) - 1);
             uint32_t base = 0;
             if (lane == leader) base = atomicAdd(counter, (uint32_t)__popcll(need_mask));
@@ -1152,14 +1170,10 @@ __global__ __launch_bounds__(kBlock) IXG_WAVES_ATTR void film_persistent_kernel(
         if (active) {
             int r = STEP_CONTINUE;
             if (S.bounce < P.max_bounces) r = path_bounce<false, ISO, ABSORB, ANISO>(P, sp, S, radiance);
-            const bool fin = r != STEP_CONTINUE || ++S.bounce >= P.max_bounces;
-            if (fin) {
-                float xyz[3];
-                path_finish(P, S, r == STEP_ESCAPED, radiance, xyz);
-                out_xyz[3 * (size_t)idx + 0] = xyz[0];
-                out_xyz[3 * (size_t)idx + 1] = xyz[1];
-                out_xyz[3 * (size_t)idx + 2] = xyz[2];
+            if (r != STEP_CONTINUE || ++S.bounce >= P.max_bounces) {
                 active = false;
+                pending = true;
+                pending_escaped = r == STEP_ESCAPED;
             }
         }
     }
