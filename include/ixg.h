@@ -290,6 +290,35 @@ ixg_status ixg_set_metric_mode(ixg_ctx* ctx, int use_graph, int block, int tile)
  * replayed from a graph. */
 ixg_status ixg_metric_tuning(ixg_ctx* ctx, int* block, int* tile, uint32_t* grid_blocks, int* graphed);
 
+/* measure_model_width (IX/render_setup/stone_width.rs:27-42): the short axis
+ * of the solid's x/z vertex bounding box (IX/geometry/stone_metrics/
+ * measure.rs:63-150), in f64 on the host.  IXG_DECLINED when the planes do
+ * not bound a finite solid of positive volume (the reference's None). */
+ixg_status ixg_model_width(const ixg_facet_plane* planes, uint32_t n_planes, double* width);
+
+/* evaluate_gem_optical_metrics (IX/color/metrics/evaluate.rs:158) and its
+ * Fire-less twin evaluate_gem_optical_metrics_geom_core (evaluate.rs:296),
+ * batched: every (design, pose) of the 18 x 18 analytical ray fan in one
+ * device pass, each design's fan scaled to its measured girdle half-width
+ * (fan.rs:49-95).  poses[2p] = yaw, poses[2p+1] = pitch (radians).
+ * out_metrics[(d*n_poses + p)*5] = {brilliance, fire, scintillation,
+ * windowing, extinction}; want_all = 0 skips Fire and scintillation and
+ * reports 25 / 75 for them (placeholder_metrics, evaluate.rs:259); a design
+ * with no planes reports the placeholder metrics. */
+ixg_status ixg_pose_metrics(ixg_ctx* ctx, const ixg_facet_plane* planes, uint32_t total_planes,
+                            const ixg_design* designs, uint32_t n_designs, const float* poses,
+                            uint32_t n_poses, const ixg_environment* env, int want_all, float* out_metrics);
+
+/* sweep_axes (IX/color/metrics/pose_batch.rs:199-217) generalised to many
+ * designs -- evaluate_full_axis_profile_at_azimuth (profile.rs:200) for each
+ * (design, azimuth): the 181 poses of axis_poses (pose_batch.rs:118) in one
+ * device pass, laid out by tilt_slot (pose_batch.rs:133).
+ * out_curves[((d*n_azimuths + a)*3 + k)*181 + t], k = brilliance,
+ * extinction, windowing; t = tilt + 90 deg. */
+ixg_status ixg_axis_profiles(ixg_ctx* ctx, const ixg_facet_plane* planes, uint32_t total_planes,
+                             const ixg_design* designs, uint32_t n_designs, const float* azimuths_deg,
+                             uint32_t n_azimuths, const ixg_environment* env, float* out_curves);
+
 /* ---- metric surrogate (new work; the reference has no surrogate) ----
  * A BF16 MLP 16 -> 4 x 256 (GELU) -> 3, FP32 master weights and Adam,
  * trained and evaluated with MFMA kernels.  x, t, y are device pointers:

@@ -51,6 +51,11 @@ def _declare(lib):
                                   C.c_uint32, C.c_float, P(abi.Environment), P(C.c_float)]),
         "ixg_set_metric_mode": (st, [C.c_void_p, C.c_int, C.c_int, C.c_int]),
         "ixg_metric_tuning": (st, [C.c_void_p, P(C.c_int), P(C.c_int), P(C.c_uint32), P(C.c_int)]),
+        "ixg_model_width": (st, [P(abi.FacetPlane), C.c_uint32, P(C.c_double)]),
+        "ixg_pose_metrics": (st, [C.c_void_p, P(abi.FacetPlane), C.c_uint32, P(abi.Design), C.c_uint32,
+                                  P(C.c_float), C.c_uint32, P(abi.Environment), C.c_int, P(C.c_float)]),
+        "ixg_axis_profiles": (st, [C.c_void_p, P(abi.FacetPlane), C.c_uint32, P(abi.Design), C.c_uint32,
+                                   P(C.c_float), C.c_uint32, P(abi.Environment), P(C.c_float)]),
         "ixg_mlp_param_count": (C.c_uint32, []),
         "ixg_mlp_create": (st, [C.c_void_p, C.c_uint32, C.c_uint64, P(C.c_void_p)]),
         "ixg_mlp_destroy": (None, [C.c_void_p]),

@@ -204,13 +204,18 @@ const float kMain[8] = {0, 12, 24, 36, 48, 60, 72, 84};
 const float kBreak[16] = {95, 1, 11, 13, 23, 25, 35, 37, 47, 49, 59, 61, 71, 73, 83, 85};
 const float kToRad = kPi / 180.0f;
 
+// Correctly rounded f32 sine / cosine (double evaluation, one rounding): the
+// cut geometry the reference's pinned metrics were produced with has
+// sin(42.5 deg) rounded correctly, where glibc's sinf is one ULP high.
+float sin_cr(float x) { return (float)std::sin((double)x); }
+float cos_cr(float x) { return (float)std::cos((double)x); }
+
 uint32_t tier(ixg_facet_plane* out, uint32_t at, float ang, float sy, const float* idx, int count,
               float d) {
     for (int i = 0; i < count; ++i) {
         // index_to_azimuth (cuts/round_brilliant.rs:61-63), 96-tooth wheel
         float phi = 2.0f * kPi * (idx[i] + 0.0f) / 96.0f;
-        plane(h3(std::sin(ang) * std::cos(phi), sy * std::cos(ang), std::sin(ang) * std::sin(phi)), d,
-              &out[at++]);
+        plane(h3(sin_cr(ang) * cos_cr(phi), sy * cos_cr(ang), sin_cr(ang) * sin_cr(phi)), d, &out[at++]);
     }
     return at;
 }
@@ -226,7 +231,7 @@ uint32_t brilliant(float table_d, float crown_deg, float crown_d, float pav_deg,
     n = tier(out, n, 41.0f * kToRad, 1.0f, kBreak, 16, -0.67f);
     for (int i = 0; i < 16; ++i) {
         float phi = 2.0f * kPi * (float)i / 16.0f;
-        plane(h3(std::cos(phi), 0.0f, std::sin(phi)), -1.0f, &out[n++]);
+        plane(h3(cos_cr(phi), 0.0f, sin_cr(phi)), -1.0f, &out[n++]);
     }
     n = tier(out, n, pav_deg * kToRad, -1.0f, kMain, 8, pav_d);
     n = tier(out, n, 42.5f * kToRad, -1.0f, kBreak, 16, -0.68f);

@@ -137,14 +137,16 @@ struct Classified {
 };
 
 // classify_ray (classify.rs:109-197) + fire_sample (classify.rs:199-256).
-__device__ __forceinline__ void classify(const MetricShared& S, const MetricImage& I,
-                                         const float4* __restrict__ pl, V3 org, V3 dir,
-                                         bool want_fire, Classified& C, Fate* detail) {
+// `observer` is the unit direction towards the eye the head-shadow cone is
+// centred on: -dir for a perspective pixel, -forward for the fan.
+__device__ __forceinline__ void classify_ray(const MetricShared& S, const float4* __restrict__ pl, uint32_t n_planes,
+                                             float nd, float nf, float nc, V3 org, V3 dir, V3 observer,
+                                             bool want_fire, Classified& C, Fate* detail) {
     C.cls = CLS_NONE;
     C.has_fire = 0;
     C.entry_facet = -1;
     float t;
-    int f = intersect(pl, I.n_planes, org, dir, t);
+    int f = intersect(pl, n_planes, org, dir, t);
     if (f < 0) return;
     C.entry_facet = f;
     V3 hp = org + dir * t;
@@ -152,12 +154,12 @@ __device__ __forceinline__ void classify(const MetricShared& S, const MetricImag
     V3 nrm = v3(pf.x, pf.y, pf.z);
     float cos_i = clampf(dot(neg(dir), nrm), 0.0f, 1.0f);
     Fate F;
-    trace_entry(pl, I.n_planes, hp, dir, nrm, cos_i, I.nd, F);
+    trace_entry(pl, n_planes, hp, dir, nrm, cos_i, nd, F);
     if (detail) *detail = F;
     if (F.fate == FATE_ENTRY_BLOCKED) { C.cls = CLS_OTHER; return; }
     if (F.fate == FATE_LEAKED) { C.cls = CLS_WINDOWED; return; }
     if (F.fate == FATE_ABSORBED) { C.cls = CLS_EXTINCT; return; }
-    if (dot(F.dir, neg(dir)) > S.head_cos) {
+    if (dot(F.dir, observer) > S.head_cos) {
         if (detail) detail->reason = REASON_SHADOW;
         C.cls = CLS_EXTINCT;
         return;
@@ -170,8 +172,8 @@ __device__ __forceinline__ void classify(const MetricShared& S, const MetricImag
     C.cls = CLS_RETURNED;
     if (want_fire) {
         Fate A, B;
-        trace_entry(pl, I.n_planes, hp, dir, nrm, cos_i, I.nf, A);
-        trace_entry(pl, I.n_planes, hp, dir, nrm, cos_i, I.nc, B);
+        trace_entry(pl, n_planes, hp, dir, nrm, cos_i, nf, A);
+        trace_entry(pl, n_planes, hp, dir, nrm, cos_i, nc, B);
         if (A.fate == FATE_UPWARD && B.fate == FATE_UPWARD && A.facet == B.facet &&
             A.bounces == B.bounces) {
             float cs = clampf(dot(A.dir, B.dir), -1.0f, 1.0f);
@@ -182,6 +184,12 @@ __device__ __forceinline__ void classify(const MetricShared& S, const MetricImag
             C.has_fire = 1;
         }
     }
+}
+
+__device__ __forceinline__ void classify(const MetricShared& S, const MetricImage& I,
+                                         const float4* __restrict__ pl, V3 org, V3 dir,
+                                         bool want_fire, Classified& C, Fate* detail) {
+    classify_ray(S, pl, I.n_planes, I.nd, I.nf, I.nc, org, dir, neg(dir), want_fire, C, detail);
 }
 
 // Camera::generate_ray with zero jitter (camera.rs:149-167).
@@ -418,6 +426,139 @@ __global__ void scoring_probe_kernel(const MetricShared S, const float* __restri
     out[i] = scoring_radiance(S, v3(dirs[3 * i], dirs[3 * i + 1], dirs[3 * i + 2]));
 }
 
+// ---------------------------------------------------------------------------
+// Fan-grid pose metrics (evaluate.rs:327-439): one block per (design, pose)
+// job.  The five aperture rays of every disc cell (and, on the full pass, the
+// cell's five temporal yaw-offset rays) are classified one per lane into LDS;
+// lane 0 then folds them in the reference's (cell, sample) order, so the Fire
+// fma chain, the cell-fraction sums and the temporal variances are formed
+// exactly as the serial grid loop forms them.
+// ---------------------------------------------------------------------------
+constexpr int kPoseBlock = 256;
+enum { CLS_RETURNED_FIRE = 4 };
+
+__device__ __forceinline__ V3 row(const float (*b)[3], int k) { return v3(b[k][0], b[k][1], b[k][2]); }
+
+// FanGeometry::origin (fan.rs:92-94)
+__device__ __forceinline__ V3 fan_origin(V3 f, V3 r, V3 up, float standoff, float lateral, float u, float v) {
+    return (neg(f) * standoff + r * (u * lateral)) + up * (v * lateral);
+}
+
+__global__ __launch_bounds__(kPoseBlock) void pose_kernel(const MetricShared S, const PoseDesign* __restrict__ designs,
+                                                          const PoseJob* __restrict__ jobs,
+                                                          const float4* __restrict__ planes,
+                                                          const float2* __restrict__ cells, uint32_t n_cells,
+                                                          int want_all, float* __restrict__ out) {
+    __shared__ float4 sp[IXG_MAX_PLANES];
+    __shared__ int8_t s_cls[kPoseMaxCells * 5];
+    __shared__ float s_ang[kPoseMaxCells * 5];
+    __shared__ float s_w[kPoseMaxCells * 5];
+    __shared__ uint8_t s_ret[kPoseMaxCells * 5];
+    const PoseJob& J = jobs[blockIdx.x];
+    const PoseDesign D = designs[J.design];
+    float* o = out + (size_t)blockIdx.x * 5;
+    if (D.n_planes == 0) {  // placeholder_metrics (evaluate.rs:259-267)
+        if (threadIdx.x == 0) {
+            o[0] = 85.0f;
+            o[1] = 25.0f;
+            o[2] = 75.0f;
+            o[3] = 5.0f;
+            o[4] = 5.0f;
+        }
+        return;
+    }
+    for (uint32_t i = threadIdx.x; i < D.n_planes; i += kPoseBlock) sp[i] = planes[D.plane_offset + i];
+    __syncthreads();
+    const uint32_t n_main = n_cells * 5;
+    const uint32_t n_items = want_all ? 2 * n_main : n_main;
+    for (uint32_t i = threadIdx.x; i < n_items; i += kPoseBlock) {
+        if (i < n_main) {
+            // classify_aperture_sample (classify.rs:91-106), sub-aperture s
+            const uint32_t c = i / 5, s = i % 5;
+            const float2 uv = cells[c];
+            const V3 f = row(J.basis[0], 0), r = row(J.basis[0], 1), up = row(J.basis[0], 2);
+            const float sr = s == 1 ? 0.08f : (s == 2 ? -0.08f : 0.0f);
+            const float su = s == 3 ? 0.08f : (s == 4 ? -0.08f : 0.0f);
+            const V3 dir = normalize((f + r * sr) + up * su);
+            const V3 org = fan_origin(f, r, up, D.standoff, D.lateral, uv.x, uv.y);
+            Classified C;
+            classify_ray(S, sp, D.n_planes, D.nd, D.nf, D.nc, org, dir, neg(f), want_all != 0, C, nullptr);
+            int8_t code = (int8_t)C.cls;
+            if (C.cls == CLS_RETURNED && C.has_fire) {
+                code = CLS_RETURNED_FIRE;
+                s_ang[i] = C.angle;
+                s_w[i] = C.weight;
+            }
+            s_cls[i] = code;
+        } else {
+            // cell_returned_at_yaw_offset (scintillation.rs:79-101)
+            const uint32_t j = i - n_main, c = j / 5, k = j % 5;
+            const float2 uv = cells[c];
+            const V3 f = row(J.basis[1 + k], 0), r = row(J.basis[1 + k], 1), up = row(J.basis[1 + k], 2);
+            const V3 org = fan_origin(f, r, up, D.standoff, D.lateral, uv.x, uv.y);
+            Classified C;
+            classify_ray(S, sp, D.n_planes, D.nd, D.nf, D.nc, org, f, neg(f), false, C, nullptr);
+            s_ret[j] = (uint8_t)(C.cls == CLS_RETURNED);
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    // run_grid / sample_cell accumulation (evaluate.rs:345-439)
+    uint32_t total = 0, windowed = 0, extinct = 0, returned = 0, counted = 0;
+    float fire = 0.0f, cf = 0.0f, cf2 = 0.0f, tv = 0.0f;
+    for (uint32_t c = 0; c < n_cells; ++c) {
+        uint32_t ct = 0, cr = 0;
+        for (uint32_t s = 0; s < 5; ++s) {
+            const uint32_t k = c * 5 + s;
+            const int code = s_cls[k];
+            if (code < 0) continue;
+            ++total;
+            ++ct;
+            if (code == CLS_WINDOWED) ++windowed;
+            else if (code == CLS_EXTINCT) ++extinct;
+            else if (code == CLS_RETURNED || code == CLS_RETURNED_FIRE) {
+                ++returned;
+                ++cr;
+                if (code == CLS_RETURNED_FIRE) fire = fma_(s_ang[k], s_w[k], fire);
+            }
+        }
+        if (ct > 0) {
+            const float frac = (float)cr / (float)ct;
+            cf += frac;
+            cf2 = fma_(frac, frac, cf2);
+            ++counted;
+            if (want_all) {  // cell_temporal_variance (scintillation.rs:110-128)
+                uint32_t n5 = 0;
+                for (uint32_t k = 0; k < 5; ++k) n5 += s_ret[c * 5 + k];
+                const float p = (float)n5 / 5.0f;
+                tv += fma_(p, -p, p);
+            }
+        }
+    }
+    // finish_metrics (accumulate.rs:112-173, scintillation.rs:146-198)
+    const float nt = (float)(total > 1 ? total : 1);
+    o[3] = clampf((float)windowed / nt * 100.0f, 0.0f, 100.0f);
+    o[4] = clampf((float)extinct / nt * 100.0f, 0.0f, 100.0f);
+    o[0] = clampf((float)returned / nt * 100.0f, 0.0f, 100.0f);
+    if (!want_all) {
+        o[1] = 25.0f;
+        o[2] = 75.0f;
+        return;
+    }
+    o[1] = fmaxf(fire / nt * 275.0f, 0.1f);
+    float spatial = 0.0f, temporal = 0.0f;
+    if (counted) {
+        const float mean = cf / (float)counted;
+        if (mean > 1e-4f) {
+            const float var = fmaxf(fma_(mean, -mean, cf2 / (float)counted), 0.0f);
+            const float cv = sqrtf(var) / mean;
+            spatial = clampf(cv / (1.0f + cv) * 100.0f, 0.0f, 100.0f);
+        }
+        temporal = clampf(((tv / (float)counted) / 0.24f) * 100.0f, 0.0f, 100.0f);
+    }
+    o[2] = clampf(fma_(0.4f, temporal, 0.6f * spatial), 0.0f, 100.0f);
+}
+
 }  // namespace ixg
 
 extern "C" hipError_t ixg_launch_scoring_probe(const ixg::MetricShared* S, const float* d_dirs, uint32_t n,
@@ -489,5 +630,15 @@ extern "C" hipError_t ixg_launch_metric_reduce(const ixg::MetricShared* S, const
                                                ixg_metric_totals* d_totals, hipStream_t stream) {
     dim3 grid((S->n_images + 63) / 64);
     hipLaunchKernelGGL(ixg::metric_reduce_kernel, grid, dim3(64), 0, stream, *S, d_px, d_totals);
+    return hipGetLastError();
+}
+
+extern "C" hipError_t ixg_launch_pose(const ixg::MetricShared* S, const ixg::PoseDesign* d_designs,
+                                      const ixg::PoseJob* d_jobs, uint32_t n_jobs, const float4* d_planes,
+                                      const float2* d_cells, uint32_t n_cells, int want_all, float* d_out,
+                                      hipStream_t stream) {
+    if (n_cells > (uint32_t)ixg::kPoseMaxCells) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(ixg::pose_kernel, dim3(n_jobs), dim3(ixg::kPoseBlock), 0, stream, *S, d_designs, d_jobs,
+                       d_planes, d_cells, n_cells, want_all, d_out);
     return hipGetLastError();
 }

@@ -538,3 +538,187 @@ extern "C" ixg_status ixg_metric_tuning(ixg_ctx* ctx, int* block, int* tile, uin
     if (graphed) *graphed = P->last_graphed;
     return IXG_OK;
 }
+
+// ---------------------------------------------------------------------------
+// Fan-grid pose metrics (evaluate_gem_optical_metrics, evaluate.rs:158-439)
+// and the tilt-axis sweep built on them (pose_batch.rs:73-199).
+// ---------------------------------------------------------------------------
+extern "C" void ixg_internal_half_widths(const ixg_facet_plane* planes, const ixg_design* designs, uint32_t n,
+                                         float* out);
+extern "C" hipError_t ixg_launch_pose(const ixg::MetricShared* S, const ixg::PoseDesign* d_designs,
+                                      const ixg::PoseJob* d_jobs, uint32_t n_jobs, const float4* d_planes,
+                                      const float2* d_cells, uint32_t n_cells, int want_all, float* d_out,
+                                      hipStream_t stream);
+
+namespace {
+
+// run_grid's disc cells in (ix, iz) order (evaluate.rs:351-358, fan.rs:22).
+std::vector<float2> fan_cells() {
+    std::vector<float2> cells;
+    for (int ix = 0; ix < 18; ++ix)
+        for (int iz = 0; iz < 18; ++iz) {
+            const float u = std::fma(((float)ix + 0.5f) / 18.0f, 2.0f, -1.0f);
+            const float v = std::fma(((float)iz + 0.5f) / 18.0f, 2.0f, -1.0f);
+            if (std::fma(v, v, u * u) > 0.70f) continue;
+            cells.push_back(make_float2(u, v));
+        }
+    return cells;
+}
+
+// camera_view_basis (camera.rs:16-21): Camera::new's frame at distance 1.
+void view_basis(float yaw, float pitch, float (*b)[3]) {
+    ixg_camera c;
+    ixg_camera_new(yaw, pitch, 1.0f, 42.0f, 1, 1, 1, &c);
+    std::memcpy(b[0], c.forward, sizeof b[0]);
+    std::memcpy(b[1], c.right, sizeof b[1]);
+    std::memcpy(b[2], c.up, sizeof b[2]);
+}
+
+ixg_status run_poses(ixg_ctx* ctx, const ixg_facet_plane* planes, uint32_t total_planes, const ixg_design* designs,
+                     uint32_t n_designs, const float* poses, uint32_t n_poses, const ixg_environment& env,
+                     int want_all, float* out) {
+    if ((uint64_t)n_designs * n_poses > (1ull << 24))
+        return ixg_internal_fail(ctx, IXG_RESOURCE_LIMIT, "more than 2^24 (design, pose) jobs");
+    for (uint32_t d = 0; d < n_designs; ++d)
+        if (designs[d].n_planes > IXG_MAX_PLANES ||
+            (uint64_t)designs[d].plane_offset + designs[d].n_planes > total_planes)
+            return ixg_internal_fail(ctx, IXG_INVALID, "design plane range");
+    if (!planes_valid(planes, total_planes)) return ixg_internal_fail(ctx, IXG_INVALID, "non-unit plane");
+    for (uint32_t p = 0; p < 2 * n_poses; ++p)
+        if (!std::isfinite(poses[p])) return ixg_internal_fail(ctx, IXG_INVALID, "non-finite pose");
+    if (hipSetDevice(ixg_internal_device(ctx)) != hipSuccess) return IXG_DEVICE_LOST;
+    MetricShared S;
+    ixg_status stt = resolve_shared(ctx, env, 0, 0, 0, &S);
+    if (stt != IXG_OK) return stt;
+
+    // PreparedEval per design (prepared.rs:64-94): indices and fan scale
+    std::vector<float> half(n_designs);
+    ixg_internal_half_widths(planes, designs, n_designs, half.data());
+    std::vector<PoseDesign> pd(n_designs);
+    for (uint32_t d = 0; d < n_designs; ++d) {
+        MetricImage idx;
+        fill_indices(designs[d].dispersion, &idx);
+        PoseDesign& P = pd[d];
+        std::memset(&P, 0, sizeof P);
+        P.plane_offset = designs[d].plane_offset;
+        P.n_planes = designs[d].n_planes;
+        P.nd = idx.nd;
+        P.nf = idx.nf;
+        P.nc = idx.nc;
+        P.standoff = 2.5f * half[d];                            // fan.rs:28,77
+        P.lateral = 0.95f * half[d] / std::sqrt(0.70f);         // fan.rs:25,78
+    }
+    // the frames of each pose, shared by every design (scintillation.rs:47-60)
+    static const float offs[5] = {-3.0f, -1.5f, 0.0f, 1.5f, 3.0f};
+    std::vector<PoseJob> per_pose(n_poses);
+    for (uint32_t p = 0; p < n_poses; ++p) {
+        PoseJob& J = per_pose[p];
+        std::memset(&J, 0, sizeof J);
+        const float yaw = poses[2 * p], pitch = poses[2 * p + 1];
+        view_basis(yaw, pitch, J.basis[0]);
+        for (int k = 0; k < 5; ++k) view_basis(yaw + offs[k] * kToRad, pitch, J.basis[1 + k]);
+    }
+    const size_t n_jobs = (size_t)n_designs * n_poses;
+    std::vector<PoseJob> jobs(n_jobs);
+    for (uint32_t d = 0; d < n_designs; ++d)
+        for (uint32_t p = 0; p < n_poses; ++p) {
+            PoseJob J = per_pose[p];
+            J.design = d;
+            jobs[(size_t)d * n_poses + p] = J;
+        }
+    const std::vector<float2> cells = fan_cells();
+    std::vector<float4> pl(std::max<uint32_t>(1, total_planes));
+    for (uint32_t i = 0; i < total_planes; ++i)
+        pl[i] = make_float4(planes[i].normal[0], planes[i].normal[1], planes[i].normal[2], planes[i].d);
+
+    hipStream_t st = ixg_internal_stream(ctx);
+    float4* d_pl = nullptr;
+    PoseDesign* d_des = nullptr;
+    PoseJob* d_jobs = nullptr;
+    float2* d_cells = nullptr;
+    float* d_out = nullptr;
+    hipError_t e = hipMalloc(&d_pl, pl.size() * sizeof(float4));
+    if (e == hipSuccess) e = hipMalloc(&d_des, pd.size() * sizeof(PoseDesign));
+    if (e == hipSuccess) e = hipMalloc(&d_jobs, n_jobs * sizeof(PoseJob));
+    if (e == hipSuccess) e = hipMalloc(&d_cells, cells.size() * sizeof(float2));
+    if (e == hipSuccess) e = hipMalloc(&d_out, n_jobs * 5 * sizeof(float));
+    if (e == hipSuccess) e = hipMemcpyAsync(d_pl, pl.data(), pl.size() * sizeof(float4), hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(d_des, pd.data(), pd.size() * sizeof(PoseDesign), hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(d_jobs, jobs.data(), n_jobs * sizeof(PoseJob), hipMemcpyHostToDevice, st);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(d_cells, cells.data(), cells.size() * sizeof(float2), hipMemcpyHostToDevice, st);
+    if (e == hipSuccess)
+        e = ixg_launch_pose(&S, d_des, d_jobs, (uint32_t)n_jobs, d_pl, d_cells, (uint32_t)cells.size(), want_all,
+                            d_out, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(out, d_out, n_jobs * 5 * sizeof(float), hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    void* bufs[] = {d_pl, d_des, d_jobs, d_cells, d_out};
+    for (void* b : bufs)
+        if (b) (void)hipFree(b);
+    if (e != hipSuccess) return ixg_internal_hip_fail(ctx, e, "pose metrics");
+    return IXG_OK;
+}
+
+// tilt_slot (pose_batch.rs:133-143): where job k of an axis lands in its
+// 181-point tilt curve.
+uint32_t tilt_slot(uint32_t job) {
+    const uint32_t half = 90;
+    if (job == 0) return half;
+    if (job <= half) return 2 * half - (job - 1);
+    return job - 1 - half;
+}
+
+}  // namespace
+
+extern "C" ixg_status ixg_pose_metrics(ixg_ctx* ctx, const ixg_facet_plane* planes, uint32_t total_planes,
+                                       const ixg_design* designs, uint32_t n_designs, const float* poses,
+                                       uint32_t n_poses, const ixg_environment* env, int want_all,
+                                       float* out_metrics) {
+    if (!ctx || !designs || !poses || !env || !out_metrics || (!planes && total_planes)) return IXG_INVALID;
+    if (ixg_internal_lost(ctx)) return IXG_DEVICE_LOST;
+    if (n_designs == 0 || n_poses == 0) return IXG_OK;
+    return run_poses(ctx, planes, total_planes, designs, n_designs, poses, n_poses, *env, want_all ? 1 : 0,
+                     out_metrics);
+}
+
+extern "C" ixg_status ixg_axis_profiles(ixg_ctx* ctx, const ixg_facet_plane* planes, uint32_t total_planes,
+                                        const ixg_design* designs, uint32_t n_designs, const float* azimuths_deg,
+                                        uint32_t n_azimuths, const ixg_environment* env, float* out_curves) {
+    if (!ctx || !designs || !azimuths_deg || !env || !out_curves || (!planes && total_planes)) return IXG_INVALID;
+    if (ixg_internal_lost(ctx)) return IXG_DEVICE_LOST;
+    if (n_designs == 0 || n_azimuths == 0) return IXG_OK;
+    constexpr uint32_t kPer = 181;  // EVALUATIONS_PER_AXIS (profile.rs:369)
+    // axis_poses (pose_batch.rs:118-131): the table-up pole, then pitch
+    // 0..89 deg at the positive azimuth, then at the positive + 180 deg
+    std::vector<float> poses((size_t)n_azimuths * kPer * 2);
+    for (uint32_t a = 0; a < n_azimuths; ++a) {
+        const float pos = azimuths_deg[a] * kToRad;
+        const float neg = (azimuths_deg[a] + 180.0f) * kToRad;
+        float* P = poses.data() + (size_t)a * kPer * 2;
+        P[0] = pos;
+        P[1] = 90.0f * kToRad;
+        for (uint32_t i = 0; i < 90; ++i) {
+            const float pitch = (float)i * kToRad;
+            P[2 * (1 + i)] = pos;
+            P[2 * (1 + i) + 1] = pitch;
+            P[2 * (91 + i)] = neg;
+            P[2 * (91 + i) + 1] = pitch;
+        }
+    }
+    const uint32_t n_poses = n_azimuths * kPer;
+    std::vector<float> m((size_t)n_designs * n_poses * 5);
+    ixg_status st = run_poses(ctx, planes, total_planes, designs, n_designs, poses.data(), n_poses, *env, 0, m.data());
+    if (st != IXG_OK) return st;
+    // assemble_axis (pose_batch.rs:146-156): [brilliance, extinction, windowing]
+    for (uint32_t d = 0; d < n_designs; ++d)
+        for (uint32_t a = 0; a < n_azimuths; ++a)
+            for (uint32_t j = 0; j < kPer; ++j) {
+                const float* r = m.data() + ((size_t)d * n_poses + (size_t)a * kPer + j) * 5;
+                float* curves = out_curves + ((size_t)d * n_azimuths + a) * 3 * kPer;
+                const uint32_t slot = tilt_slot(j);
+                curves[slot] = r[0];
+                curves[kPer + slot] = r[4];
+                curves[2 * kPer + slot] = r[3];
+            }
+    return IXG_OK;
+}

@@ -128,4 +128,23 @@ struct MetricShared {
     uint32_t n_images;
 };
 
+// One design of a fan-grid pose evaluation (PreparedEval, prepared.rs:64-94):
+// its facet set, the three line indices and the fan scale (fan.rs:71-95).
+struct PoseDesign {
+    uint32_t plane_offset, n_planes;
+    float nd, nf, nc;
+    float standoff, lateral;
+    uint32_t pad;
+};
+
+// One (design, pose) job: the camera frame at the pose and at the five
+// temporal yaw offsets (scintillation.rs:19,47-60), rows forward/right/up.
+struct PoseJob {
+    float basis[6][3][3];  // [0] = the pose, [1..5] = yaw offsets -3 .. +3 deg
+    uint32_t design;
+    uint32_t pad[3];
+};
+
+constexpr int kPoseMaxCells = 256;  // disc cells of the 18 x 18 grid (180 used)
+
 }  // namespace ixg

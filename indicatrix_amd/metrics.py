@@ -72,6 +72,63 @@ def metric_sweep(backend, planes, designs, tilts_deg, width, height, samples, di
     return out
 
 
+def _design_table(planes, designs):
+    planes = np.ascontiguousarray(planes, np.float32).reshape(-1, 4)
+    pl = (abi.FacetPlane * max(1, len(planes))).from_buffer_copy(
+        planes.tobytes() if len(planes) else bytes(C.sizeof(abi.FacetPlane)))
+    ds = (abi.Design * len(designs))()
+    for i, (off, cnt, disp) in enumerate(designs):
+        ds[i] = abi.Design(off, cnt, disp)
+    return pl, len(planes), ds
+
+
+def model_width(planes):
+    """measure_model_width (render_setup/stone_width.rs:27): the width in model
+    units, or None when the planes do not bound a finite solid."""
+    planes = np.ascontiguousarray(planes, np.float32).reshape(-1, 4)
+    pl = (abi.FacetPlane * max(1, len(planes))).from_buffer_copy(
+        planes.tobytes() if len(planes) else bytes(C.sizeof(abi.FacetPlane)))
+    w = C.c_double()
+    st = lib().ixg_model_width(pl, len(planes), C.byref(w))
+    if st == abi.IXG_DECLINED:
+        return None
+    if st != abi.IXG_OK:
+        raise IxgError(st, "model width")
+    return w.value
+
+
+def pose_metrics(backend, planes, designs, poses, environment, want_all=True):
+    """evaluate_gem_optical_metrics (color/metrics/evaluate.rs:158) for every
+    (design, (yaw, pitch)) pair in one device pass.  Returns
+    (n_designs, n_poses, 5) float32 in METRIC_FIELDS order; want_all=False is
+    the Fire-less pass (fire 25, scintillation 75)."""
+    pl, n, ds = _design_table(planes, designs)
+    poses = np.ascontiguousarray(poses, np.float32).reshape(-1, 2)
+    out = np.zeros((len(designs), len(poses), 5), np.float32)
+    st = lib().ixg_pose_metrics(backend.handle, pl, n, ds, len(designs),
+                                poses.ctypes.data_as(C.POINTER(C.c_float)), len(poses),
+                                C.byref(environment), 1 if want_all else 0,
+                                out.ctypes.data_as(C.POINTER(C.c_float)))
+    if st != abi.IXG_OK:
+        raise IxgError(st, backend.last_error())
+    return out
+
+
+def axis_profiles(backend, planes, designs, azimuths_deg, environment):
+    """evaluate_full_axis_profile_at_azimuth (color/metrics/profile.rs:200) per
+    (design, azimuth): (n_designs, n_azimuths, 3, 181) float32 curves of
+    brilliance, extinction and windowing over tilt -90..90 degrees."""
+    pl, n, ds = _design_table(planes, designs)
+    az = np.ascontiguousarray(azimuths_deg, np.float32)
+    out = np.zeros((len(designs), len(az), 3, 181), np.float32)
+    st = lib().ixg_axis_profiles(backend.handle, pl, n, ds, len(designs),
+                                 az.ctypes.data_as(C.POINTER(C.c_float)), len(az),
+                                 C.byref(environment), out.ctypes.data_as(C.POINTER(C.c_float)))
+    if st != abi.IXG_OK:
+        raise IxgError(st, backend.last_error())
+    return out
+
+
 def set_metric_mode(backend, use_graph=True, block=0, tile=0):
     """How metric_sweep launches: graph capture on/off; block 0 = autotune,
     -1 = the 2-D grid kernel, or a queue shape (block lanes, tile pixels)."""

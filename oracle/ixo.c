@@ -519,6 +519,13 @@ static void plane_new(v3 n, float d, ixg_facet_plane* p) { /* geometry/plane.rs:
     p->normal[0] = u.x; p->normal[1] = u.y; p->normal[2] = u.z;
     p->d = d;
 }
+/* The cut's sines and cosines are the correctly rounded f32 values (double
+ * evaluation, one rounding).  The reference's pinned metrics
+ * (golden_regression.rs:207-233) were produced where sin(42.5 deg) rounds
+ * correctly; glibc 2.35's sinf returns the f32 one ULP above it, which moves
+ * the 16 lower-girdle normals and, through them, the Fire index. */
+static float sin_cr(float x) { return (float)sin((double)x); }
+static float cos_cr(float x) { return (float)cos((double)x); }
 static float index_to_azimuth(float index, float teeth) { /* cuts/round_brilliant.rs:65 */
     return 2.0f * F32_PI * (index + 0.0f) / teeth;
 }
@@ -526,7 +533,7 @@ static uint32_t tier(ixg_facet_plane* out, uint32_t at, float angle_rad, float s
                      const float* idx, int count, float d) {
     for (int i = 0; i < count; ++i) {
         float phi = index_to_azimuth(idx[i], 96.0f);
-        v3 n = mk(sinf(angle_rad) * cosf(phi), sign_y * cosf(angle_rad), sinf(angle_rad) * sinf(phi));
+        v3 n = mk(sin_cr(angle_rad) * cos_cr(phi), sign_y * cos_cr(angle_rad), sin_cr(angle_rad) * sin_cr(phi));
         plane_new(n, d, &out[at++]);
     }
     return at;
@@ -545,7 +552,7 @@ static uint32_t round_brilliant(float table_d, float crown_main_deg, float crown
     n = tier(out, n, 41.0f * TO_RAD, 1.0f, BREAK_IDX, 16, -0.67f);
     for (int i = 0; i < 16; ++i) {
         float phi = 2.0f * F32_PI * (float)i / 16.0f;
-        plane_new(mk(cosf(phi), 0.0f, sinf(phi)), -1.0f, &out[n++]);
+        plane_new(mk(cos_cr(phi), 0.0f, sin_cr(phi)), -1.0f, &out[n++]);
     }
     n = tier(out, n, pav_main_deg * TO_RAD, -1.0f, MAIN_IDX, 8, pav_main_d);
     n = tier(out, n, 42.5f * TO_RAD, -1.0f, BREAK_IDX, 16, -0.68f);
@@ -2023,4 +2030,294 @@ float ixo_image_scintillation_pct(const ixg_optical_pixel* px, uint32_t n) {
     }
     float temporal = poses ? clampf(((var_sum / (float)poses) / 0.24f) * 100.0f, 0.0f, 100.0f) : 0.0f;
     return clampf(fmaf(0.4f, temporal, 0.6f * spatial), 0.0f, 100.0f); /* scintillation.rs:194 */
+}
+
+/* ================================================================== */
+/* Pose metrics: evaluate_gem_optical_metrics (color/metrics/evaluate.rs) */
+/* ================================================================== */
+
+/* ---- stone width (render_setup/stone_width.rs:27-42, geometry/stone_metrics/) ----
+ * Plain f64 restatement: every triple of (deduplicated, pow2-normalised)
+ * planes plus the six blank-box planes, no pair pruning (the reference's
+ * pruning is proven not to change the visited triples). */
+typedef struct { double x, y, z; } d3;
+static inline d3 dmk(double x, double y, double z) { d3 r = {x, y, z}; return r; }
+static inline d3 dsub(d3 a, d3 b) { return dmk(a.x - b.x, a.y - b.y, a.z - b.z); }
+static inline d3 dadd(d3 a, d3 b) { return dmk(a.x + b.x, a.y + b.y, a.z + b.z); }
+static inline d3 dscale(d3 a, double s) { return dmk(a.x * s, a.y * s, a.z * s); }
+static inline double ddot(d3 a, d3 b) { return (a.x * b.x + a.y * b.y) + a.z * b.z; }
+static inline d3 dcross(d3 a, d3 b) {
+    return dmk(a.y * b.z - b.y * a.z, a.z * b.x - b.z * a.x, a.x * b.y - b.x * a.y);
+}
+static inline double dabsmax(d3 v) { return fmax(fmax(fabs(v.x), fabs(v.y)), fabs(v.z)); }
+
+enum { MW_MAX_PLANES = 512 };
+#define MW_BLANK 64.0          /* meet_solver/mod.rs:218 */
+#define MW_EPS_FEAS 1e-5       /* meet_solver/mod.rs:223 */
+#define MW_MIN_DET 1e-6        /* meet_solver/mod.rs:235 */
+#define MW_DEDUP 1e-6          /* stone_metrics/mod.rs:112 */
+#define MW_EPS_FACE 2e-5       /* stone_metrics/mod.rs:109 */
+
+static int normals_coincide_d(d3 a, d3 b) { /* cuts/asc_schedule.rs:335-339 */
+    double la = sqrt(ddot(a, a)), lb = sqrt(ddot(b, b));
+    double ra = 1.0 / la, rb = 1.0 / lb;
+    d3 ah = (isfinite(ra) && ra > 0.0) ? dscale(a, ra) : dmk(0, 0, 0);
+    d3 bh = (isfinite(rb) && rb > 0.0) ? dscale(b, rb) : dmk(0, 0, 0);
+    return 1.0 - ddot(ah, bh) <= 2e-7;
+}
+
+static double pow2_scale_norm(double s) { /* stone_metrics/mod.rs:152-174 */
+    if (!(isfinite(s) && s > 0.0)) return 1.0;
+    uint64_t raw;
+    memcpy(&raw, &s, 8);
+    int64_t lift = 0;
+    uint64_t bits = raw;
+    if ((raw >> 52) == 0) {
+        double l = s * 18446744073709551616.0;
+        memcpy(&bits, &l, 8);
+        lift = 64;
+    }
+    int64_t biased = (int64_t)(bits >> 52);
+    uint64_t mb = (bits & ((1ull << 52) - 1)) | (1023ull << 52);
+    double mant;
+    memcpy(&mant, &mb, 8);
+    int64_t e = biased - 1023 - lift;
+    if (mant >= 1.41421356237309504880) e += 1;
+    int64_t be = e + 1023;
+    if (be < 1) be = 1;
+    if (be > 2046) be = 2046;
+    uint64_t ob = (uint64_t)be << 52;
+    double out;
+    memcpy(&out, &ob, 8);
+    return out;
+}
+
+/* The triple solve of simd/triple_solve.rs (cofactor inverse times rhs). */
+static double solve_triple(d3 a, d3 b, d3 c, double ma, double mb, double mc, d3* v) {
+    double t0x = b.y * c.z - b.z * c.y, t0y = c.y * a.z - c.z * a.y, t0z = a.y * b.z - a.z * b.y;
+    double det = (a.x * t0x + b.x * t0y) + c.x * t0z;
+    double t2x = b.x * c.y - b.y * c.x, t2y = c.x * a.y - c.y * a.x, t2z = a.x * b.y - a.y * b.x;
+    double t1x = b.z * c.x - b.x * c.z, t1y = c.z * a.x - c.x * a.z, t1z = a.z * b.x - a.x * b.z;
+    double inv = 1.0 / det;
+    v->x = ((t0x * inv) * ma + (t0y * inv) * mb) + (t0z * inv) * mc;
+    v->y = ((t1x * inv) * ma + (t1y * inv) * mb) + (t1z * inv) * mc;
+    v->z = ((t2x * inv) * ma + (t2y * inv) * mb) + (t2z * inv) * mc;
+    return det;
+}
+
+static int cmp_angle(const void* a, const void* b) {
+    double x = *(const double*)a, y = *(const double*)b;
+    return (x > y) - (x < y);
+}
+
+/* face_area (stone_metrics/mesh.rs:139-203): the face polygon's vertices in
+ * angle order about their centroid, shoelace-summed. */
+static double face_area_d(d3 n, double m, const d3* V, int nv) {
+    d3* on = (d3*)malloc(sizeof(d3) * (size_t)nv);
+    double* ang = (double*)malloc(sizeof(double) * 4 * (size_t)nv);
+    int k = 0;
+    for (int i = 0; i < nv; ++i)
+        if (fabs(ddot(n, V[i]) - m) <= MW_EPS_FACE) on[k++] = V[i];
+    double area = 0.0;
+    if (k >= 3) {
+        d3 seed;
+        if (fabs(n.x) <= fabs(n.y) && fabs(n.x) <= fabs(n.z)) seed = dmk(1, 0, 0);
+        else if (fabs(n.y) <= fabs(n.z)) seed = dmk(0, 1, 0);
+        else seed = dmk(0, 0, 1);
+        d3 bu = dsub(seed, dscale(n, ddot(n, seed)));
+        bu = dscale(bu, 1.0 / sqrt(ddot(bu, bu)));
+        d3 bv = dcross(n, bu);
+        d3 c = dmk(0, 0, 0);
+        for (int i = 0; i < k; ++i) c = dadd(c, on[i]);
+        c = dscale(c, 1.0 / (double)k);
+        for (int i = 0; i < k; ++i) {
+            d3 d = dsub(on[i], c);
+            ang[4 * i] = atan2(ddot(bv, d), ddot(bu, d));
+            ang[4 * i + 1] = on[i].x;
+            ang[4 * i + 2] = on[i].y;
+            ang[4 * i + 3] = on[i].z;
+        }
+        qsort(ang, (size_t)k, 4 * sizeof(double), cmp_angle);
+        d3 s = dmk(0, 0, 0);
+        for (int i = 0; i < k; ++i) {
+            int j = (i + 1) % k;
+            d3 a = dsub(dmk(ang[4 * i + 1], ang[4 * i + 2], ang[4 * i + 3]), c);
+            d3 b = dsub(dmk(ang[4 * j + 1], ang[4 * j + 2], ang[4 * j + 3]), c);
+            s = dadd(s, dcross(a, b));
+        }
+        area = 0.5 * fabs(ddot(n, s));
+    }
+    free(on);
+    free(ang);
+    return area;
+}
+
+/* measure_model_width: the short axis of the feasible vertices' x/z bounding
+ * box (measure.rs:79-150).  Returns 0 when the arrangement does not measure
+ * (no planes, unbounded, fewer than 4 vertices, non-positive volume). */
+int ixo_model_width(const ixg_facet_plane* planes, uint32_t n, double* width) {
+    if (n == 0 || n > MW_MAX_PLANES) return 0;
+    d3 N[MW_MAX_PLANES + 6];
+    double Mo[MW_MAX_PLANES + 6];
+    int p = 0;
+    for (uint32_t i = 0; i < n; ++i) { /* dedup_planes (vertices.rs:67-79) */
+        d3 ni = dmk((double)planes[i].normal[0], (double)planes[i].normal[1], (double)planes[i].normal[2]);
+        double mi = -(double)planes[i].d;
+        int dup = 0;
+        for (int j = 0; j < p && !dup; ++j) dup = normals_coincide_d(ni, N[j]) && fabs(mi - Mo[j]) < 1e-9;
+        if (!dup) { N[p] = ni; Mo[p] = mi; ++p; }
+    }
+    double amax = 0.0; /* max_abs_offset (measure.rs:22-33), NaN-propagating */
+    for (int i = 0; i < p; ++i) amax = (isnan(amax) || isnan(Mo[i])) ? NAN : fmax(amax, fabs(Mo[i]));
+    double scale = pow2_scale_norm(amax);
+    for (int i = 0; i < p; ++i) Mo[i] /= scale;
+    const int real = p;
+    static const double bx[6][3] = {{1, 0, 0}, {-1, 0, 0}, {0, 1, 0}, {0, -1, 0}, {0, 0, 1}, {0, 0, -1}};
+    for (int i = 0; i < 6; ++i) { N[p] = dmk(bx[i][0], bx[i][1], bx[i][2]); Mo[p] = MW_BLANK; ++p; }
+    int cap = 1024, nv = 0;
+    d3* V = (d3*)malloc(sizeof(d3) * (size_t)cap);
+    int ok = 1;
+    for (int a = 0; a < p && ok; ++a)
+        for (int b = a + 1; b < p && ok; ++b)
+            for (int c = b + 1; c < p; ++c) { /* flush_solid_batch (vertices.rs:96-139) */
+                d3 v;
+                double det = solve_triple(N[a], N[b], N[c], Mo[a], Mo[b], Mo[c], &v);
+                double da = fabs(det);
+                if (isnan(da) || da < MW_MIN_DET) continue;
+                if (!(isfinite(v.x) && isfinite(v.y) && isfinite(v.z))) continue;
+                if (dabsmax(v) > MW_BLANK + 1.0) continue;
+                int viol = 0;
+                for (int q = 0; q < p && !viol; ++q) viol = ddot(N[q], v) - Mo[q] > MW_EPS_FEAS;
+                if (viol) continue;
+                if (dabsmax(v) > MW_BLANK - 1.0) { ok = 0; break; }
+                int dup = 0; /* VertexAccumulator::insert_if_new */
+                for (int q = 0; q < nv && !dup; ++q) dup = dabsmax(dsub(V[q], v)) < MW_DEDUP;
+                if (dup) continue;
+                if (nv == cap) { cap *= 2; V = (d3*)realloc(V, sizeof(d3) * (size_t)cap); }
+                V[nv++] = v;
+            }
+    if (ok && nv < 4) ok = 0;
+    if (ok) {
+        double vol = 0.0;
+        for (int i = 0; i < real; ++i) vol += Mo[i] * face_area_d(N[i], Mo[i], V, nv) / 3.0;
+        if (!(isfinite(vol) && vol > 0.0)) ok = 0;
+    }
+    if (ok) {
+        double x0 = INFINITY, x1 = -INFINITY, z0 = INFINITY, z1 = -INFINITY;
+        for (int i = 0; i < nv; ++i) {
+            x0 = fmin(x0, V[i].x); x1 = fmax(x1, V[i].x);
+            z0 = fmin(z0, V[i].z); z1 = fmax(z1, V[i].z);
+        }
+        double dx = x1 - x0, dz = z1 - z0;
+        *width = (dx <= dz ? dx : dz) * scale;
+    }
+    free(V);
+    return ok;
+}
+
+/* girdle_half_width (metrics/fan.rs:49-60). */
+float ixo_girdle_half_width(const ixg_facet_plane* planes, uint32_t n) {
+    double w;
+    if (ixo_model_width(planes, n, &w) && isfinite(w) && w > 0.0) return (float)(w * 0.5);
+    return 1.0f;
+}
+
+/* FanGeometry::origin (fan.rs:92-94). */
+static v3 fan_origin(v3 f, v3 r, v3 u, float standoff, float lateral, float gu, float gv) {
+    return vadd(vadd(vmul(vneg(f), standoff), vmul(r, gu * lateral)), vmul(u, gv * lateral));
+}
+
+/* evaluate_pose (evaluate.rs:327-439 with accumulate.rs:112-173 and
+ * scintillation.rs:104-198).  out = {brilliance, fire, scintillation,
+ * windowing, extinction}; want_all = 0 is the Fire-less pass (fire 25,
+ * scintillation 75). */
+int ixo_pose_metrics(const ixg_facet_plane* planes, uint32_t n, const ixg_material* mat,
+                     const ixg_environment* env, float yaw, float pitch, int want_all, float out[5]) {
+    if (n == 0) { /* placeholder_metrics (evaluate.rs:259-267) */
+        out[0] = 85.0f; out[1] = 25.0f; out[2] = 75.0f; out[3] = 5.0f; out[4] = 5.0f;
+        return IXG_OK;
+    }
+    mctx_t M;
+    if (!mctx_build(planes, n, &mat->dispersion, env, &M)) return IXG_DECLINED;
+    const float hw = ixo_girdle_half_width(planes, n);
+    const float standoff = 2.5f * hw;
+    const float lateral = 0.95f * hw / sqrtf(0.70f);
+    ixg_camera cam;
+    ixo_camera_new(yaw, pitch, 1.0f, 42.0f, 1, 1, 1, &cam);
+    const v3 F = vfrom(cam.forward), R = vfrom(cam.right), U = vfrom(cam.up);
+    v3 TF[5], TR[5], TU[5];
+    static const float offs[5] = {-3.0f, -1.5f, 0.0f, 1.5f, 3.0f};
+    for (int k = 0; k < 5; ++k) {
+        ixg_camera c;
+        ixo_camera_new(yaw + offs[k] * (F32_PI / 180.0f), pitch, 1.0f, 42.0f, 1, 1, 1, &c);
+        TF[k] = vfrom(c.forward); TR[k] = vfrom(c.right); TU[k] = vfrom(c.up);
+    }
+    static const float ap[5][2] = {{0.0f, 0.0f}, {0.08f, 0.0f}, {-0.08f, 0.0f}, {0.0f, 0.08f}, {0.0f, -0.08f}};
+    uint32_t total = 0, windowed = 0, extinct = 0, returned = 0, cells = 0;
+    float fire = 0.0f, cf = 0.0f, cf2 = 0.0f, tv = 0.0f;
+    for (int ix = 0; ix < 18; ++ix)
+        for (int iz = 0; iz < 18; ++iz) {
+            const float gu = fmaf(((float)ix + 0.5f) / 18.0f, 2.0f, -1.0f);
+            const float gv = fmaf(((float)iz + 0.5f) / 18.0f, 2.0f, -1.0f);
+            if (fmaf(gv, gv, gu * gu) > 0.70f) continue;
+            uint32_t ct = 0, cr = 0;
+            for (int s = 0; s < 5; ++s) { /* sample_cell (evaluate.rs:380-439) */
+                v3 d = vnormalize(vadd(vadd(F, vmul(R, ap[s][0])), vmul(U, ap[s][1])));
+                v3 o = fan_origin(F, R, U, standoff, lateral, gu, gv);
+                int hf = 0;
+                float ang = 0.0f, w = 0.0f;
+                int32_t ef;
+                int cls = classify_m(&M, o, d, vneg(F), want_all, &hf, &ang, &w, &ef, NULL);
+                if (cls == CLS_NONE) continue;
+                total++;
+                ct++;
+                if (cls == CLS_WINDOWED) windowed++;
+                else if (cls == CLS_EXTINCT) extinct++;
+                else if (cls == CLS_RETURNED) {
+                    returned++;
+                    cr++;
+                    if (hf) fire = fmaf(ang, w, fire);
+                }
+            }
+            if (ct > 0) {
+                float frac = (float)cr / (float)ct;
+                cf += frac;
+                cf2 = fmaf(frac, frac, cf2);
+                cells++;
+                if (want_all) { /* cell_temporal_variance (scintillation.rs:104-128) */
+                    uint32_t k5 = 0;
+                    for (int k = 0; k < 5; ++k) {
+                        v3 o = fan_origin(TF[k], TR[k], TU[k], standoff, lateral, gu, gv);
+                        int hf;
+                        float a, w;
+                        int32_t ef;
+                        if (classify_m(&M, o, TF[k], vneg(TF[k]), 0, &hf, &a, &w, &ef, NULL) == CLS_RETURNED) k5++;
+                    }
+                    float pr = (float)k5 / 5.0f;
+                    tv += fmaf(pr, -pr, pr);
+                }
+            }
+        }
+    const float nt = (float)(total > 1 ? total : 1);
+    out[3] = clampf((float)windowed / nt * 100.0f, 0.0f, 100.0f);
+    out[4] = clampf((float)extinct / nt * 100.0f, 0.0f, 100.0f);
+    out[0] = clampf((float)returned / nt * 100.0f, 0.0f, 100.0f);
+    if (!want_all) {
+        out[1] = 25.0f;
+        out[2] = 75.0f;
+        return IXG_OK;
+    }
+    out[1] = maxf(fire / nt * 275.0f, 0.1f);
+    float spatial = 0.0f, temporal = 0.0f;
+    if (cells) {
+        float mean = cf / (float)cells;
+        if (mean > 1e-4f) {
+            float var = maxf(fmaf(mean, -mean, cf2 / (float)cells), 0.0f);
+            float cv = sqrtf(var) / mean;
+            spatial = clampf(cv / (1.0f + cv) * 100.0f, 0.0f, 100.0f);
+        }
+        temporal = clampf(((tv / (float)cells) / 0.24f) * 100.0f, 0.0f, 100.0f);
+    }
+    out[2] = clampf(fmaf(0.4f, temporal, 0.6f * spatial), 0.0f, 100.0f);
+    return IXG_OK;
 }

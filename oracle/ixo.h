@@ -78,6 +78,17 @@ int ixo_metric_image(const ixg_metric_request* req, ixg_optical_pixel* pixels,
                      ixg_metric_totals* totals);
 float ixo_image_scintillation_pct(const ixg_optical_pixel* px, uint32_t n);
 
+/* measure_model_width (IX/render_setup/stone_width.rs:27): 1 and the width
+ * when the planes bound a finite solid, else 0. */
+int ixo_model_width(const ixg_facet_plane* planes, uint32_t n, double* width);
+/* girdle_half_width (IX/color/metrics/fan.rs:49): width / 2, or 1.0. */
+float ixo_girdle_half_width(const ixg_facet_plane* planes, uint32_t n);
+/* evaluate_gem_optical_metrics[_geom_core] (IX/color/metrics/evaluate.rs:158,285)
+ * at one pose: out = {brilliance, fire, scintillation, windowing, extinction}. */
+int ixo_pose_metrics(const ixg_facet_plane* planes, uint32_t n, const ixg_material* material,
+                     const ixg_environment* env, float yaw, float pitch, int want_all,
+                     float out[5]);
+
 #ifdef __cplusplus
 }
 #endif

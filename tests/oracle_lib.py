@@ -74,6 +74,14 @@ def lib():
         L.ixo_metric_image.argtypes = [P(abi.MetricRequest), P(abi.OpticalPixel), P(abi.MetricTotals)]
         L.ixo_image_scintillation_pct.restype = C.c_float
         L.ixo_image_scintillation_pct.argtypes = [P(abi.OpticalPixel), C.c_uint32]
+        L.ixo_model_width.restype = C.c_int
+        L.ixo_model_width.argtypes = [P(abi.FacetPlane), C.c_uint32, P(C.c_double)]
+        L.ixo_girdle_half_width.restype = C.c_float
+        L.ixo_girdle_half_width.argtypes = [P(abi.FacetPlane), C.c_uint32]
+        L.ixo_pose_metrics.restype = C.c_int
+        L.ixo_pose_metrics.argtypes = [P(abi.FacetPlane), C.c_uint32, P(abi.Material),
+                                       P(abi.Environment), C.c_float, C.c_float, C.c_int,
+                                       P(C.c_float)]
         _lib = L
     return _lib
 
