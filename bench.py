@@ -130,6 +130,49 @@ def sweep_cpu_baseline(planes, ds, env):
                       f"{dt:.2f} s on {threads} threads"}
 
 
+AXIS_AZIMUTHS_DEG = [0.0, 45.0, 90.0, 135.0]   # PROFILE_AZIMUTHS_DEG (color/metrics/profile.rs:26)
+
+
+def pose_sweep_rate(backend, n_designs=64):
+    """M10: tilt-axis profiles (profile.rs:200, pose_batch.rs:199) -- the 57-facet
+    standard round brilliant in Diamond under RingLights, 4 azimuths x 181 poses
+    (724 Fire-less fan-grid evaluations, the reference's full tilt sweep), then
+    the same sweep over n_designs seeded designs (32..256 planes) in one call.
+    Timed per call: the host's girdle-width measurement is included."""
+    from indicatrix_amd import designs, metrics, tracer
+    srb = tracer.planes_array(tracer.standard_round_brilliant())
+    disp = tracer.GemMaterial.diamond().to_abi().dispersion
+    env = tracer.LightingPreset.studio("RingLights", 1.0, 0.85, 0.95)
+    one = [(0, len(srb), disp)]
+    metrics.axis_profiles(backend, srb, one, AXIS_AZIMUTHS_DEG, env)  # warm
+    t0 = time.perf_counter()
+    curves = metrics.axis_profiles(backend, srb, one, AXIS_AZIMUTHS_DEG, env)
+    dt1 = time.perf_counter() - t0
+    planes, ds, _ = designs.sweep_batch(n_designs)
+    t0 = time.perf_counter()
+    metrics.axis_profiles(backend, planes, ds, AXIS_AZIMUTHS_DEG, env)
+    dt2 = time.perf_counter() - t0
+    poses = 181 * len(AXIS_AZIMUTHS_DEG)
+    return {"srb_axes": {"poses": poses, "seconds": round(dt1, 5), "poses_per_s": round(poses / dt1, 1),
+                         "mean_brilliance_pct": round(float(curves[0, :, 0].mean()), 4)},
+            "designs_axes": {"designs": n_designs, "poses": n_designs * poses, "seconds": round(dt2, 4),
+                             "poses_per_s": round(n_designs * poses / dt2, 1)}}, (srb, disp, env)
+
+
+def pose_sweep_cpu_baseline(srb, disp, env):
+    """The oracle's evaluate_pose over the same 724 poses (Fire-less) on the
+    host's cores, one pose per task."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib
+    threads = min(os.cpu_count() or 1, 16)
+    poses = [p for az in AXIS_AZIMUTHS_DEG for p in oracle_lib.axis_poses(az)]
+    t0 = time.perf_counter()
+    oracle_lib.pose_metrics(srb, disp, env, poses, want_all=False, nthreads=threads)
+    dt = time.perf_counter() - t0
+    return {"value": round(len(poses) / dt, 1), "unit": "poses/s", "cores": threads, "kind": "port",
+            "sample": f"the SRB tilt sweep, {len(poses)} Fire-less poses in {dt:.3f} s on {threads} threads"}
+
+
 def screened_rate(backend, steps):
     """C5: the surrogate-screened Optimize loop with the committed surrogate
     (refined on the designs it has verified, 100 Adam steps per step)."""
@@ -271,12 +314,16 @@ def main():
         sweep, sweep_inputs = sweep_rate(backend)
         print("# bench: screened optimize", file=sys.stderr, flush=True)
         line["design_evals"] = {"sweep": sweep, "screened_optimize": screened_rate(backend, args.optimize_steps)}
+        print("# bench: pose sweep", file=sys.stderr, flush=True)
+        pose_sweep, pose_inputs = pose_sweep_rate(backend)
+        line["pose_sweep"] = pose_sweep
     if not args.no_cpu_baseline:
         print("# bench: cpu baseline", file=sys.stderr, flush=True)
         line["cpu_baseline"] = cpu_baseline(scene, args.cpu_seconds)
         if not args.film_only:
             line["design_evals"]["sweep"]["cpu_baseline"] = sweep_cpu_baseline(*sweep_inputs)
             line["film_c2"]["cpu_baseline"] = film_c2_cpu_baseline(c2_scene)
+            line["pose_sweep"]["cpu_baseline"] = pose_sweep_cpu_baseline(*pose_inputs)
     print(json.dumps(line), flush=True)
     backend.close()
 

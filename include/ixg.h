@@ -297,7 +297,7 @@ ixg_status ixg_metric_tuning(ixg_ctx* ctx, int* block, int* tile, uint32_t* grid
 ixg_status ixg_model_width(const ixg_facet_plane* planes, uint32_t n_planes, double* width);
 
 /* evaluate_gem_optical_metrics (IX/color/metrics/evaluate.rs:158) and its
- * Fire-less twin evaluate_gem_optical_metrics_geom_core (evaluate.rs:296),
+ * Fire-less twin evaluate_gem_optical_metrics_geom_core (evaluate.rs:305),
  * batched: every (design, pose) of the 18 x 18 analytical ray fan in one
  * device pass, each design's fan scaled to its measured girdle half-width
  * (fan.rs:49-95).  poses[2p] = yaw, poses[2p+1] = pitch (radians).

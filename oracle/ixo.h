@@ -83,7 +83,7 @@ float ixo_image_scintillation_pct(const ixg_optical_pixel* px, uint32_t n);
 int ixo_model_width(const ixg_facet_plane* planes, uint32_t n, double* width);
 /* girdle_half_width (IX/color/metrics/fan.rs:49): width / 2, or 1.0. */
 float ixo_girdle_half_width(const ixg_facet_plane* planes, uint32_t n);
-/* evaluate_gem_optical_metrics[_geom_core] (IX/color/metrics/evaluate.rs:158,285)
+/* evaluate_gem_optical_metrics[_geom_core] (IX/color/metrics/evaluate.rs:158,305)
  * at one pose: out = {brilliance, fire, scintillation, windowing, extinction}. */
 int ixo_pose_metrics(const ixg_facet_plane* planes, uint32_t n, const ixg_material* material,
                      const ixg_environment* env, float yaw, float pitch, int want_all,

@@ -164,3 +164,41 @@ def metric_sweep(planes, designs, tilts, width, height, samples, distance, envir
     with ThreadPoolExecutor(max_workers=nthreads or threads()) as ex:
         list(ex.map(run, jobs))
     return out
+
+
+def pose_metrics(planes, dispersion, environment, poses, want_all=True, nthreads=None):
+    """The oracle's evaluate_pose for every (yaw, pitch): (n_poses, 5), poses
+    spread over host threads (each call is independent)."""
+    from concurrent.futures import ThreadPoolExecutor
+    planes = np.ascontiguousarray(planes, np.float32).reshape(-1, 4)
+    pl = (abi.FacetPlane * max(1, len(planes))).from_buffer_copy(
+        planes.tobytes() if len(planes) else bytes(C.sizeof(abi.FacetPlane)))
+    mat = abi.Material()
+    mat.dispersion = dispersion
+    mat.cubic = 1
+    mat.absorption_path_scale = 1.0
+    out = np.zeros((len(poses), 5), np.float32)
+
+    def run(k):
+        o = (C.c_float * 5)()
+        st = lib().ixo_pose_metrics(pl, len(planes), C.byref(mat), C.byref(environment),
+                                    float(poses[k][0]), float(poses[k][1]), 1 if want_all else 0, o)
+        if st != 0:
+            raise RuntimeError(f"oracle pose_metrics status {st}")
+        out[k] = o[:]
+
+    with ThreadPoolExecutor(max_workers=nthreads or threads()) as ex:
+        list(ex.map(run, range(len(poses))))
+    return out
+
+
+def axis_poses(azimuth_deg):
+    """axis_poses (color/metrics/pose_batch.rs:118-131) in f32: the pole, then
+    pitch 0..89 deg at the azimuth, then at the azimuth + 180 deg."""
+    to_rad = np.float32(np.pi) / np.float32(180.0)
+    pos = np.float32(azimuth_deg) * to_rad
+    neg = np.float32(np.float32(azimuth_deg) + np.float32(180.0)) * to_rad
+    poses = [(pos, np.float32(90.0) * to_rad)]
+    poses += [(pos, np.float32(i) * to_rad) for i in range(90)]
+    poses += [(neg, np.float32(i) * to_rad) for i in range(90)]
+    return poses
