@@ -177,6 +177,31 @@ ixg_route_reason ixg_route(const ixg_scene* scene);
  * non-zero = cancel, the in-memory form of Rust's AtomicBool. */
 ixg_status ixg_accumulate(ixg_ctx* ctx, const ixg_scene* scene, uint32_t sample_offset,
                           uint32_t spp, double* accum_xyz, const volatile uint8_t* cancel);
+/* One picture of a batch (GpuBatchItem, IX/renderer/gpu_backend/mod.rs:246-259):
+ * its scene, absolute sample range and its own FP64 output (w*h*3), which
+ * receives exactly what ixg_accumulate would ADD for the scene alone. */
+typedef struct ixg_batch_item {
+    const ixg_scene* scene;
+    uint32_t first_sample;
+    uint32_t samples;
+    double* out;
+} ixg_batch_item;
+/* Called once per item, in input order: IXG_OK (Done), IXG_DECLINED,
+ * IXG_CANCELLED, or IXG_INVALID for a malformed item. */
+typedef void (*ixg_batch_done_fn)(void* user, uint32_t index, ixg_status outcome);
+/* GpuBackend::try_accumulate_batch_cancellable (IX/renderer/gpu_backend/
+ * backend.rs:451): traces the items as one stream on the library's stream,
+ * the next picture's chunks queued before the previous picture is read back,
+ * and reports each through on_done.  Each Done item is bit-identical to
+ * ixg_accumulate on it alone; a declined or cancelled item leaves its out
+ * untouched.  An unsupported material or environment declines only its own
+ * item; a device loss declines every item not yet reported; the cancel flag
+ * is polled once per chunk and, once seen, reports the first unfinished item
+ * and all later ones IXG_CANCELLED (items already fully queued may still
+ * finish Done).  Returns IXG_INVALID (nothing traced or reported) for bad
+ * arguments, else IXG_OK once every item has been reported. */
+ixg_status ixg_accumulate_batch(ixg_ctx* ctx, const ixg_batch_item* items, uint32_t n,
+                                const volatile uint8_t* cancel, ixg_batch_done_fn on_done, void* user);
 /* Same, with accum_xyz a device pointer (inputs resident in HBM), on the
  * library's stream; no host copy of the result. */
 ixg_status ixg_accumulate_device(ixg_ctx* ctx, const ixg_scene* scene, uint32_t sample_offset,

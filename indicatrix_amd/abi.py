@@ -80,6 +80,14 @@ class Scene(C.Structure):
                 ("environment", Environment)]
 
 
+class BatchItem(C.Structure):
+    _fields_ = [("scene", C.POINTER(Scene)), ("first_sample", C.c_uint32),
+                ("samples", C.c_uint32), ("out", C.POINTER(C.c_double))]
+
+
+BatchDoneFn = C.CFUNCTYPE(None, C.c_void_p, C.c_uint32, C.c_int)
+
+
 class Opts(C.Structure):
     _fields_ = [("device", C.c_int32), ("chunk_paths", C.c_uint32)]
 

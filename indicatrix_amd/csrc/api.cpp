@@ -362,6 +362,120 @@ ixg_status ixg_accumulate(ixg_ctx* ctx, const ixg_scene* s, uint32_t sample_offs
     return IXG_OK;
 }
 
+ixg_status ixg_accumulate_batch(ixg_ctx* ctx, const ixg_batch_item* items, uint32_t n,
+                                const volatile uint8_t* cancel, ixg_batch_done_fn on_done, void* user) {
+    if (!ctx || (n && !items) || !on_done) return IXG_INVALID;
+    uint32_t reported = 0;  // items [0, reported) have had their on_done
+    auto report = [&](uint32_t i, ixg_status outcome) {
+        on_done(user, i, outcome);
+        reported = i + 1;
+    };
+    auto report_rest = [&](ixg_status outcome) {
+        while (reported < n) report(reported, outcome);
+    };
+    if (n == 0) return IXG_OK;
+    if (ctx->lost || hipSetDevice(ctx->device) != hipSuccess) {
+        if (!ctx->lost) ctx->last_error = "hipSetDevice failed";
+        report_rest(IXG_DECLINED);
+        return IXG_OK;
+    }
+    // The output ring: two device slots and two pinned host slots, sized once
+    // for the largest picture, so picture i+1 is queued while picture i is
+    // read back.
+    size_t cap = 0;
+    for (uint32_t i = 0; i < n; ++i) {
+        const ixg_scene* s = items[i].scene;
+        if (s && s->width && s->height && (uint64_t)s->width * s->height <= (1ull << 31))
+            cap = std::max(cap, (size_t)s->width * s->height * 3);
+    }
+    double* d_slot[2] = {nullptr, nullptr};
+    double* h_slot[2] = {nullptr, nullptr};
+    hipEvent_t ready[2] = {nullptr, nullptr};
+    auto release = [&]() {
+        (void)hipStreamSynchronize(ctx->stream);
+        for (int k = 0; k < 2; ++k) {
+            if (d_slot[k]) (void)hipFree(d_slot[k]);
+            if (h_slot[k]) (void)hipHostFree(h_slot[k]);
+            if (ready[k]) (void)hipEventDestroy(ready[k]);
+        }
+    };
+    hipError_t e = hipSuccess;
+    for (int k = 0; k < 2 && e == hipSuccess && cap; ++k) {
+        e = hipMalloc(&d_slot[k], cap * sizeof(double));
+        if (e == hipSuccess) e = hipHostMalloc((void**)&h_slot[k], cap * sizeof(double), hipHostMallocDefault);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&ready[k], hipEventDisableTiming);
+    }
+    if (e != hipSuccess) {
+        hip_fail(ctx, e, "batch output ring");
+        release();
+        report_rest(IXG_DECLINED);
+        return IXG_OK;
+    }
+    int pending = -1, pending_slot = 0;
+    size_t pending_n = 0;
+    // Waits for the queued picture's readback and reports it Done.
+    auto finish_pending = [&]() -> bool {
+        if (pending < 0) return true;
+        hipError_t w = hipEventSynchronize(ready[pending_slot]);
+        if (w != hipSuccess) {
+            hip_fail(ctx, w, "batch readback");
+            return false;
+        }
+        double* out = items[pending].out;
+        const double* src = h_slot[pending_slot];
+        for (size_t k = 0; k < pending_n; ++k) out[k] += src[k];
+        report((uint32_t)pending, IXG_OK);
+        pending = -1;
+        return true;
+    };
+    bool stopped = false;
+    for (uint32_t i = 0; i < n && !stopped; ++i) {
+        const ixg_batch_item& it = items[i];
+        ixg_status st = it.out ? begin_call(ctx, it.scene) : fail(ctx, IXG_INVALID, "null batch output");
+        if (st == IXG_DEVICE_LOST) break;
+        if (st == IXG_OK && cancel && *cancel) st = IXG_CANCELLED;
+        if (st == IXG_OK && it.samples == 0) {  // nothing to trace: Done, nothing added
+            if (!finish_pending()) break;
+            report(i, IXG_OK);
+            continue;
+        }
+        const int slot = (int)(i & 1u);
+        const size_t count = st == IXG_OK ? (size_t)it.scene->width * it.scene->height * 3 : 0;
+        if (st == IXG_OK) {
+            hipError_t z = hipMemsetAsync(d_slot[slot], 0, count * sizeof(double), ctx->stream);
+            st = z == hipSuccess ? run_film(ctx, it.scene, it.first_sample, it.samples, d_slot[slot], cancel, false)
+                                 : hip_fail(ctx, z, "batch slot clear");
+        }
+        if (st == IXG_DEVICE_LOST) break;  // declines the queued picture too
+        if (st == IXG_CANCELLED) {
+            // the queued picture was fully enqueued before the flag: it finishes
+            if (finish_pending()) report_rest(IXG_CANCELLED);
+            stopped = true;
+            break;
+        }
+        if (st != IXG_OK) {  // declined or malformed: this picture only
+            if (!finish_pending()) break;
+            report(i, st);
+            continue;
+        }
+        e = hipMemcpyAsync(h_slot[slot], d_slot[slot], count * sizeof(double), hipMemcpyDeviceToHost, ctx->stream);
+        if (e == hipSuccess) e = hipEventRecord(ready[slot], ctx->stream);
+        if (e != hipSuccess) {
+            hip_fail(ctx, e, "batch readback enqueue");
+            break;
+        }
+        // picture i is queued behind picture i-1: now read i-1 back
+        if (!finish_pending()) break;
+        pending = (int)i;
+        pending_slot = slot;
+        pending_n = count;
+    }
+    if (!stopped && reported < n && !ctx->lost) finish_pending();
+    release();
+    report_rest(IXG_DECLINED);  // anything left after a device loss
+    return IXG_OK;
+}
+
 ixg_status ixg_accumulate_device(ixg_ctx* ctx, const ixg_scene* s, uint32_t sample_offset,
                                  uint32_t spp, double* d_accum_xyz) {
     ixg_status st = begin_call(ctx, s);

@@ -244,6 +244,40 @@ class GpuBackend:
         return {abi.IXG_OK: GpuAccumulate.DONE, abi.IXG_DECLINED: GpuAccumulate.DECLINED,
                 abi.IXG_CANCELLED: GpuAccumulate.CANCELLED}[st]
 
+    def try_accumulate_batch_cancellable(self, items, cancel=None, on_done=None):
+        """try_accumulate_batch_cancellable (renderer/gpu_backend/backend.rs:451).
+
+        ``items``: (scene, first_sample, samples, out) with ``out`` a C-contiguous
+        float64 array of width*height*3 that receives exactly what
+        try_accumulate_cancellable would ADD for that scene, only when its
+        outcome is Done.  ``on_done(index, GpuAccumulate)`` runs once per item,
+        in input order.  Returns the outcomes, in item order."""
+        arr = (abi.BatchItem * max(1, len(items)))()
+        for i, (scene, first, samples, out) in enumerate(items):
+            if out.dtype != np.float64 or not out.flags.c_contiguous:
+                raise ValueError("out must be a C-contiguous float64 array")
+            if out.size != scene.width * scene.height * 3:
+                raise ValueError("batch output buffer must have one entry per pixel")
+            arr[i] = abi.BatchItem(C.pointer(scene.scene), first, samples,
+                                   out.ctypes.data_as(C.POINTER(C.c_double)))
+        outcomes = [None] * len(items)
+        names = {abi.IXG_OK: GpuAccumulate.DONE, abi.IXG_DECLINED: GpuAccumulate.DECLINED,
+                 abi.IXG_CANCELLED: GpuAccumulate.CANCELLED}
+
+        def done(_user, index, status):
+            if outcomes[index] is not None:
+                raise RuntimeError(f"picture {index} reported more than once")
+            outcomes[index] = names.get(status, GpuAccumulate.DECLINED)
+            if on_done is not None:
+                on_done(index, outcomes[index])
+
+        cb = abi.BatchDoneFn(done)
+        cflag = None if cancel is None else C.cast(cancel, C.POINTER(C.c_uint8))
+        st = lib().ixg_accumulate_batch(self._ctx, arr, len(items), cflag, cb, None)
+        if st != abi.IXG_OK:
+            raise IxgError(st, self.last_error())
+        return outcomes
+
     def try_accumulate(self, scene, sample_offset, spp, accum):
         return self.try_accumulate_cancellable(scene, sample_offset, spp, accum) \
             is GpuAccumulate.DONE

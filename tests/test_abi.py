@@ -99,3 +99,15 @@ def test_sellmeier_from_nd_abbe_hits_targets():
         n = lambda l: (1 + b * l * l / (l * l - c)) ** 0.5
         assert abs(n(0.5893) - nd) < 1e-9
         assert abs((n(0.5893) - 1) / (n(0.4861) - n(0.6563)) - v) < 1e-6
+
+
+def test_batch_entry_rejects_bad_arguments_without_reporting():
+    """ixg_accumulate_batch: a null context or callback is IXG_INVALID and no
+    item is reported (no device work is started)."""
+    calls = []
+    cb = abi.BatchDoneFn(lambda _u, i, s: calls.append((i, s)))
+    items = (abi.BatchItem * 1)()
+    L = _lib.lib()
+    assert L.ixg_accumulate_batch(None, items, 1, None, cb, None) == abi.IXG_INVALID
+    assert L.ixg_accumulate_batch(None, None, 0, None, cb, None) == abi.IXG_INVALID
+    assert calls == []
