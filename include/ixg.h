@@ -107,6 +107,10 @@ typedef struct ixg_material {
 typedef enum ixg_preset {
     IXG_PRESET_LIGHT_TENT = 0,
     IXG_PRESET_ISO_HEMISPHERE = 1,
+    IXG_PRESET_WHITE_TRAY = 2,
+    IXG_PRESET_SHOP_LIGHTS = 3,
+    IXG_PRESET_WINDOW_DAYLIGHT = 4,
+    IXG_PRESET_DAYLIGHT_DOME = 5,
     IXG_PRESET_DAYLIGHT = 7,
     IXG_PRESET_INCANDESCENT = 8,
     IXG_PRESET_ILLUMINANT_A = 9,
@@ -120,7 +124,7 @@ typedef struct ixg_environment {
     float exposure;
     float light_yaw;
     float light_pitch;
-    float backdrop;        /* lit tray radiance; GPU path: must be 0      */
+    float backdrop;        /* lit tray radiance at exposure 1, >= 0; 0 = as lit */
     float surface_glare;   /* clamped to [0,1]; 1 = off                   */
     float head_shadow_deg; /* 16 = default, <= 0 = off                    */
 } ixg_environment;

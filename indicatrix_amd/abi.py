@@ -22,6 +22,10 @@ DISP_SELLMEIER1, DISP_SELLMEIER3, DISP_CAUCHY = 0, 1, 2
 PRESETS = {
     "LightTent": 0,
     "IsoHemisphere": 1,
+    "WhiteTray": 2,
+    "ShopLights": 3,
+    "WindowDaylight": 4,
+    "DaylightDome": 5,
     "Daylight": 7,
     "Incandescent": 8,
     "IlluminantA": 9,

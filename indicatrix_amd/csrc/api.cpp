@@ -335,7 +335,7 @@ ixg_route_reason ixg_route(const ixg_scene* s) {
     if (!material_routable(*s->material)) return IXG_ROUTE_UNSUPPORTED_MATERIAL;
     ixg::PresetInfo pi;
     if (!ixg::preset_info(s->environment.preset, &pi)) return IXG_ROUTE_UNSUPPORTED_ENVIRONMENT;
-    if (s->environment.backdrop > 0.0f) return IXG_ROUTE_UNSUPPORTED_ENVIRONMENT;
+    if (!std::isfinite(s->environment.backdrop)) return IXG_ROUTE_UNSUPPORTED_ENVIRONMENT;
     return IXG_ROUTE_OK;
 }
 

@@ -144,9 +144,15 @@ __device__ __forceinline__ float d65(float lambda) {
     return fma_(frac, v1 - v0, v0) / 100.0f;
 }
 
+// exp rounded once from double precision: the f32 nearest e^x, as glibc's
+// expf gives for the arguments here (the device library's f32 expf is up to
+// one ULP away).  Used where the value feeds every channel of a path: the
+// Planck illuminant and the absorption band sums.
+__device__ __forceinline__ float exp_rn(float x) { return (float)exp((double)x); }
+
 __device__ __forceinline__ float spec_power(const EnvParams& E, float lambda) {
     if (E.d65) return d65(lambda);
-    float e = expf(fminf(14388000.0f / (lambda * E.bb_t_k), 80.0f));
+    float e = exp_rn(fminf(14388000.0f / (lambda * E.bb_t_k), 80.0f));
     float denom = fmaxf(e - 1.0f, 1e-6f);
     float ratio = E.bb_denom_560 / denom;
     return clampf(powi<5>(560.0f / lambda) * ratio, 0.01f, 20.0f);
@@ -157,21 +163,58 @@ __device__ __forceinline__ float smoothstep(float e0, float e1, float x) {
     return t * t * fma_(-2.0f, t, 3.0f);
 }
 
+// The lit models' geometric factor for a unit direction d (rig.rs:230-320):
+// the iso hemisphere, the light tent (walls, black cards, key softbox, spark)
+// and the daylight dome (sky gradient, sun aureole), each under the observer's
+// head shadow and with the lit backdrop tray replacing its own ground.
+__device__ __forceinline__ float lit_factor_unit(const EnvParams& E, V3 d, V3 observer) {
+    const float horizon = smoothstep(-0.05f, 0.05f, d.y);
+    const float vis = 1.0f - smoothstep(E.shadow_outer, E.shadow_inner, dot(d, observer));
+    if (E.model == ENV_ISO_HEMISPHERE)
+        return E.tray > 0.0f ? fma_((1.0f - horizon) * vis, E.tray, horizon * vis) : horizon * vis;
+    float above, own_ground;
+    if (E.model == ENV_LIGHT_TENT) {
+        float walls = fma_(0.08f, fmaxf(d.y, 0.0f), 0.14f) * E.tent_walls;
+        if (E.tent_flat > 0.0f) walls = fma_(0.18f * E.tent_walls, E.tent_flat, walls * (1.0f - E.tent_flat));
+        float card = 0.0f;
+        card = fmaxf(card, smoothstep(0.898794f, 0.9612617f, dot(d, v3(E.ring[4][0], E.ring[4][1], E.ring[4][2]))));
+        card = fmaxf(card, smoothstep(0.898794f, 0.9612617f, dot(d, v3(E.ring[8][0], E.ring[8][1], E.ring[8][2]))));
+        card = fmaxf(card, smoothstep(0.898794f, 0.9612617f, dot(d, v3(E.ring[12][0], E.ring[12][1], E.ring[12][2]))));
+        walls *= fma_(card * E.tent_cards, -0.9f, 1.0f);
+        const float key = smoothstep(0.7660444f, 0.9396926f, dot(d, v3(E.key[0], E.key[1], E.key[2]))) *
+                          (1.4f * E.spot_mult);
+        const float spark = smoothstep(0.9961947f, 0.9993908f, dot(d, v3(E.fill[0], E.fill[1], E.fill[2]))) *
+                            (5.0f * E.spot_mult) * E.tent_spark;
+        above = ((walls + key) + spark) * (horizon * vis);
+        own_ground = E.tent_ground;
+    } else {  // daylight dome
+        const float sun_dot = dot(d, v3(E.key[0], E.key[1], E.key[2]));
+        const float sky = fma_(0.08f, 1.0f - fmaxf(d.y, 0.0f), 0.10f);
+        const float glow = fmaxf(sun_dot, 0.0f);
+        const float glow2 = glow * glow;
+        const float glow4 = glow2 * glow2;
+        const float aureole = (glow4 * glow4) * 0.30f;
+        above = (sky + aureole) * (horizon * vis);
+        own_ground = 0.04f;
+    }
+    const float ground = E.tray > 0.0f ? ((1.0f - horizon) * vis) * E.tray : own_ground * (1.0f - horizon);
+    return above + ground;
+}
+
 // Wavelength-independent lighting of one exit direction (rig.rs:379-405).
 struct DirLight {
-    float bg;       // studio: backdrop term; iso: the scaled factor
+    float bg;       // studio: backdrop term; lit models: the scaled factor
     float lights[18];
     int n_lights;
+    float horizon, ground;  // studio with a tray: fma(horizon, rig, ground * sp)
 };
 
 __device__ __forceinline__ void direction_lighting(const EnvParams& E, V3 dir, V3 observer,
                                                    DirLight& L) {
     V3 d = normalize(dir);
     L.n_lights = 0;
-    if (E.model == ENV_ISO_HEMISPHERE) {
-        float horizon = smoothstep(-0.05f, 0.05f, d.y);
-        float vis = 1.0f - smoothstep(E.shadow_outer, E.shadow_inner, dot(d, observer));
-        L.bg = horizon * vis;
+    if (E.model != ENV_STUDIO) {
+        L.bg = lit_factor_unit(E, d, observer);
         return;
     }
     L.bg = fmaxf(fma_(0.012f, fma_(d.y, 0.5f, 0.5f), 0.015f), 0.005f) * E.exposure;
@@ -186,22 +229,24 @@ __device__ __forceinline__ void direction_lighting(const EnvParams& E, V3 dir, V
             L.lights[L.n_lights++] = powi<6>(spark) * 22.0f * E.spot_mult * E.exposure;
         }
     }
+    if (E.tray > 0.0f) {  // the tray under the rig (rig.rs:358-364)
+        L.horizon = smoothstep(-0.05f, 0.05f, d.y);
+        L.ground = ((1.0f - L.horizon) * E.tray) * E.exposure;
+    }
 }
 
 __device__ __forceinline__ float light_radiance(const EnvParams& E, const DirLight& L, float sp) {
-    if (E.model == ENV_ISO_HEMISPHERE) return L.bg * (sp * E.exposure);
+    if (E.model != ENV_STUDIO) return L.bg * (sp * E.exposure);
     float r = L.bg * sp;
     for (int i = 0; i < L.n_lights; ++i) r = fma_(L.lights[i], sp, r);
+    if (E.tray > 0.0f) r = fma_(L.horizon, r, L.ground * sp);
     return r;
 }
 
-// Iso-hemisphere radiance needs no light list: evaluate it without the
-// DirLight scratch so the common config stays in registers.
-__device__ __forceinline__ float iso_factor(const EnvParams& E, V3 dir, V3 observer) {
-    V3 d = normalize(dir);
-    float horizon = smoothstep(-0.05f, 0.05f, d.y);
-    float vis = 1.0f - smoothstep(E.shadow_outer, E.shadow_inner, dot(d, observer));
-    return horizon * vis;
+// The lit models need no light list: evaluate them without the DirLight
+// scratch so the common configs stay in registers.
+__device__ __forceinline__ float lit_factor(const EnvParams& E, V3 dir, V3 observer) {
+    return lit_factor_unit(E, normalize(dir), observer);
 }
 
 // ---------------------------------------------------------------------------
@@ -277,7 +322,7 @@ __device__ __forceinline__ float band_eval(const AbsBand& b, float lambda) {
         const float delta = b.center_nm - lambda;
         t = (1.0e7f * delta / (lambda * b.center_nm)) / b.width;
     }
-    return b.peak * expf(-0.5f * t * t);
+    return b.peak * exp_rn(-0.5f * t * t);
 }
 
 }  // namespace ixg

@@ -18,6 +18,11 @@ bool preset_info(int32_t preset, PresetInfo* o) {
     case IXG_PRESET_ILLUMINANT_A: *o = {ENV_STUDIO, 0, 2856.0f, 1.2f, 1}; return true;
     case IXG_PRESET_DAYLIGHT: *o = {ENV_STUDIO, 1, 6500.0f, 1.0f, 0}; return true;
     case IXG_PRESET_ISO_HEMISPHERE: *o = {ENV_ISO_HEMISPHERE, 1, 6500.0f, 1.0f, 0}; return true;
+    case IXG_PRESET_LIGHT_TENT: *o = {ENV_LIGHT_TENT, 0, 5000.0f, 1.0f, 1}; return true;
+    case IXG_PRESET_WHITE_TRAY: *o = {ENV_LIGHT_TENT, 1, 5000.0f, 0.0f, 0, {5.0f, 0.0f, 0.0f, 0.8f, 1.0f}}; return true;
+    case IXG_PRESET_SHOP_LIGHTS: *o = {ENV_LIGHT_TENT, 1, 5000.0f, 1.0f, 0, {3.0f, 0.0f, 1.0f, 0.05f, 0.0f}}; return true;
+    case IXG_PRESET_WINDOW_DAYLIGHT: *o = {ENV_LIGHT_TENT, 1, 5000.0f, 1.0f, 0, {0.4f, 0.0f, 0.0f, 0.03f, 0.0f}}; return true;
+    case IXG_PRESET_DAYLIGHT_DOME: *o = {ENV_DAYLIGHT_DOME, 1, 6500.0f, 1.0f, 0}; return true;
     default: return false;
     }
 }
@@ -129,6 +134,12 @@ bool resolve_environment(const ixg_environment& env, EnvParams* E) {
     E->bb_denom_560 = n.denom_560;
     E->exposure = env.exposure;
     E->spot_mult = pi.spot_mult;
+    E->tray = env.backdrop;
+    E->tent_walls = pi.tent[0];
+    E->tent_cards = pi.tent[1];
+    E->tent_spark = pi.tent[2];
+    E->tent_ground = pi.tent[3];
+    E->tent_flat = pi.tent[4];
     // head_shadow_cosines (environment/mod.rs:124-137)
     float deg = env.head_shadow_deg;
     if (std::isnan(deg) || deg == 16.0f) {

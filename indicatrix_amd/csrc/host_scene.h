@@ -37,6 +37,9 @@ struct PresetInfo {
     float temp_k;
     float spot_mult;
     int white_balance;
+    // TentParams {walls, cards, spark, ground, flat} (environment/mod.rs:185-215);
+    // the light tent's own values for every other preset
+    float tent[5] = {1.0f, 1.0f, 1.0f, 0.02f, 0.0f};
 };
 bool preset_info(int32_t preset, PresetInfo* out);
 

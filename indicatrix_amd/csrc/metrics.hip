@@ -118,12 +118,7 @@ __device__ __forceinline__ void trace_entry(const float4* __restrict__ pl, uint3
 }
 
 __device__ __forceinline__ float scoring_radiance(const MetricShared& S, V3 dir) {
-    if (S.env.model == ENV_ISO_HEMISPHERE) {
-        V3 d = normalize(dir);
-        float horizon = smoothstep(-0.05f, 0.05f, d.y);
-        float vis = 1.0f - smoothstep(S.env.shadow_outer, S.env.shadow_inner, 0.0f);
-        return (horizon * vis) * (S.spec550 * 1.0f);
-    }
+    if (S.env.model != ENV_STUDIO) return lit_factor(S.env, dir, v3(0.0f, 0.0f, 0.0f)) * (S.spec550 * 1.0f);
     DirLight L;
     direction_lighting(S.env, dir, v3(0.0f, 0.0f, 0.0f), L);
     return light_radiance(S.env, L, S.spec550);

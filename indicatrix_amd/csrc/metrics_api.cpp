@@ -63,6 +63,7 @@ ixg_status resolve_shared(ixg_ctx* ctx, const ixg_environment& env, uint32_t wid
     ixg_environment scoring = env;
     scoring.exposure = 1.0f;
     scoring.head_shadow_deg = 16.0f;
+    scoring.backdrop = 0.0f;  // the tray is not part of the scoring radiance (lighting.rs:148-152)
     if (!resolve_environment(scoring, &S->env))
         return ixg_internal_fail(ctx, IXG_DECLINED, "environment has no device lighting model");
     PresetInfo pi;

@@ -160,8 +160,8 @@ __device__ __forceinline__ void path_init(const FilmParams& P, V3 org, V3 dir, u
 }
 
 __device__ __forceinline__ float env_radiance(const EnvParams& E, V3 dir, V3 observer, float lambda) {
-    if (E.model == ENV_ISO_HEMISPHERE)
-        return iso_factor(E, dir, observer) * (spec_power(E, lambda) * E.exposure);
+    if (E.model != ENV_STUDIO)
+        return lit_factor(E, dir, observer) * (spec_power(E, lambda) * E.exposure);
     DirLight L;
     direction_lighting(E, dir, observer, L);
     return light_radiance(E, L, spec_power(E, lambda));
@@ -255,7 +255,7 @@ __device__ __forceinline__ void split_probe(const FilmParams& P, const float4* _
         if (EVENTS) ++S.n_probe_scan;
     }
     if (!escapes) return;
-    const float env = ISO ? iso_factor(P.env, dk, S.observer) * (spec_power(P.env, S.lambda[c]) * P.env.exposure)
+    const float env = ISO ? lit_factor(P.env, dk, S.observer) * (spec_power(P.env, S.lambda[c]) * P.env.exposure)
                           : env_radiance(P.env, dk, S.observer, S.lambda[c]);
     S.split[c] = fma_(fmaxf(transmitted_i, 0.0f) * 1.0f, env, S.split[c]);
 }
@@ -765,8 +765,8 @@ __device__ __forceinline__ int path_bounce(const FilmParams& P, const float4* __
     }
     if (facet < 0) {
         // handle_miss (bounce_body/miss.rs:33-54)
-        if (ISO || P.env.model == ENV_ISO_HEMISPHERE) {
-            const float f = iso_factor(P.env, S.dir, S.observer);
+        if (ISO || P.env.model != ENV_STUDIO) {
+            const float f = lit_factor(P.env, S.dir, S.observer);
 #pragma unroll
             for (int c = 0; c < NCH; ++c) {
                 float env = f * (spec_power(P.env, S.lambda[c]) * P.env.exposure);
@@ -927,7 +927,7 @@ __device__ __forceinline__ int path_bounce(const FilmParams& P, const float4* __
                         if (EVENTS) ++S.n_probe_scan;
                     }
                     if (escapes) {
-                        float env = ISO ? iso_factor(P.env, dk, S.observer) *
+                        float env = ISO ? lit_factor(P.env, dk, S.observer) *
                                               (spec_power(P.env, S.lambda[c]) * P.env.exposure)
                                         : env_radiance(P.env, dk, S.observer, S.lambda[c]);
                         S.split[c] = fma_(fmaxf(fmaxf(tr.x, 0.0f), 0.0f) * 1.0f, env, S.split[c]);
@@ -1329,7 +1329,7 @@ extern "C" hipError_t ixg_launch_film_persistent(const ixg::FilmParams* P, const
                                                  uint32_t* d_counter, uint32_t grid_blocks, hipStream_t stream) {
     // a fill kernel rather than a memset node, so the launch pair can be captured
     hipLaunchKernelGGL(ixg::zero_counter_kernel, dim3(1), dim3(1), 0, stream, d_counter);
-    const bool iso = P->env.model == ixg::ENV_ISO_HEMISPHERE;
+    const bool iso = P->env.model != ixg::ENV_STUDIO;  // the lit models: one scaled factor
 #define IXG_PI(A, N)                                                                                           \
     hipLaunchKernelGGL((ixg::film_persistent_kernel<true, A, N>), dim3(grid_blocks), dim3(ixg::kBlock), 0, stream, \
                        *P, d_planes, d_xyz, d_counter)

@@ -8,7 +8,7 @@
 
 namespace ixg {
 
-enum EnvModel : int32_t { ENV_STUDIO = 0, ENV_ISO_HEMISPHERE = 1 };
+enum EnvModel : int32_t { ENV_STUDIO = 0, ENV_ISO_HEMISPHERE = 1, ENV_LIGHT_TENT = 2, ENV_DAYLIGHT_DOME = 3 };
 
 struct EnvParams {
     int32_t model;       // EnvModel
@@ -23,6 +23,8 @@ struct EnvParams {
     float fill[3];
     float ring[16][3];
     float wb[3];         // von Kries LMS scale (1,1,1 for D65 presets)
+    float tray;          // lit backdrop radiance at exposure 1 (0 = the model's own ground)
+    float tent_walls, tent_cards, tent_spark, tent_ground, tent_flat;  // TentParams
 };
 
 struct AbsBand {

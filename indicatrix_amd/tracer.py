@@ -70,6 +70,26 @@ class GemMaterial:
                            [4.3356, 0.3306, 0.0, 0.011236, 0.030625, 1.0])
 
     @staticmethod
+    def sapphire():
+        """Sapphire (IX/optics/materials/diamond_through_emerald.rs:58-134):
+        Malitson-o Sellmeier, n_e - n_o = -0.0081, optic axis +Y, and the
+        Fe-Ti pleochroic band set -- three energy-domain bands per ray, each
+        given as (centre nm, FWHM cm^-1, peak) and stored with sigma =
+        FWHM / 2.35482 (AbsorptionBand::energy, IX/optics/absorption.rs:95)."""
+        m = GemMaterial("Sapphire", abi.DISP_SELLMEIER3,
+                        [1.431349, 0.650547, 5.341402, 0.00528, 0.01424, 325.015],
+                        birefringence_delta=-0.0081, cubic=False)
+
+        def energy(centre, fwhm, peak):
+            return (centre, float(np.float32(fwhm) / np.float32(2.35482)), peak, 1)
+
+        m.bands_o = [energy(585.0, 4878.0, 2.01), energy(744.0, 3142.0, 0.887),
+                     energy(457.0, 12000.0, 0.433)]
+        m.bands_e = [energy(612.0, 7466.0, 0.371), energy(729.0, 4955.0, 1.03),
+                     energy(467.0, 11169.0, 0.524)]
+        return m
+
+    @staticmethod
     def isotropic_from_nd_abbe(n_d, abbe, name="custom"):
         """A one-term Sellmeier curve through n(589.3 nm) = n_d with Abbe number
         V_d = (n_d - 1) / (n_F - n_C), solved in double precision."""
@@ -110,9 +130,11 @@ class LightingPreset:
 
     @staticmethod
     def studio(preset, exposure=1.0, light_yaw=0.0, light_pitch=0.0, head_shadow_deg=16.0,
-               surface_glare=1.0):
+               surface_glare=1.0, backdrop=0.0):
+        """LightingPreset::studio (environment/mod.rs:619-634); ``backdrop`` is
+        EnvironmentSource::with_backdrop's lit tray radiance (source.rs:53-80)."""
         pid = abi.PRESETS[preset] if isinstance(preset, str) else int(preset)
-        return abi.Environment(pid, exposure, light_yaw, light_pitch, 0.0, surface_glare,
+        return abi.Environment(pid, exposure, light_yaw, light_pitch, backdrop, surface_glare,
                                head_shadow_deg)
 
 

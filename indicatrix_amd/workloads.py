@@ -27,12 +27,13 @@ def film_c1_scene(width=None, height=None, max_bounces=None):
                              max_bounces or C1["max_bounces"], env)
 
 
-def golden_scene(preset, light_yaw, light_pitch, max_bounces=12, width=1, height=1):
-    """The reference golden-regression scene: standard round brilliant, Diamond."""
+def golden_scene(preset, light_yaw, light_pitch, max_bounces=12, width=1, height=1, material=None):
+    """The reference golden-regression scene: standard round brilliant, Diamond
+    (or the given material, e.g. GemMaterial.sapphire())."""
     planes = tracer.standard_round_brilliant()
     cam = tracer.camera_new(0.35, 0.28, 5.0, 18.0, width, height, 1)
     env = tracer.LightingPreset.studio(preset, 1.0, light_yaw, light_pitch)
-    return tracer.FrameScene(cam, width, height, planes, tracer.GemMaterial.diamond(),
+    return tracer.FrameScene(cam, width, height, planes, material or tracer.GemMaterial.diamond(),
                              max_bounces, env)
 
 

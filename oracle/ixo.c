@@ -347,7 +347,8 @@ float ixo_band_evaluate(const ixg_band* b, float lambda_nm) { /* optics/absorpti
 /* ------------------------------------------------------------------ */
 /* Environment (RT/environment/)                                        */
 /* ------------------------------------------------------------------ */
-enum { MODEL_STUDIO = 0, MODEL_ISO = 1 };
+enum { MODEL_STUDIO = 0, MODEL_ISO = 1, MODEL_TENT = 2, MODEL_DOME = 3 };
+typedef struct { float walls, cards, spark, ground, flat; } tent_t;  /* TentParams (mod.rs:185-215) */
 typedef struct {
     int model;
     int d65;            /* 1: tabulated D65 illuminant, 0: blackbody */
@@ -356,10 +357,14 @@ typedef struct {
     float shadow[2];
     v3 key_dir, fill_dir, ring[16];
     v3 wb;              /* von Kries scale (identity for D65 presets) */
+    float tray;         /* lit backdrop radiance at exposure 1 (0 = as lit) */
+    tent_t tent;
 } env_t;
 
 static int preset_params(int32_t preset, int* model, int* d65, float* temp_k, float* spot_mult,
-                         int* white_balance) { /* environment/mod.rs:321-410,519-590 */
+                         int* white_balance, tent_t* tent) { /* environment/mod.rs:321-410,519-590 */
+    static const tent_t tent_default = {1.0f, 1.0f, 1.0f, 0.02f, 0.0f};
+    *tent = tent_default;
     switch (preset) {
     case IXG_PRESET_INCANDESCENT: *model = MODEL_STUDIO; *d65 = 0; *temp_k = 3200.0f; *spot_mult = 1.2f; *white_balance = 1; return 1;
     case IXG_PRESET_RING_LIGHTS: *model = MODEL_STUDIO; *d65 = 0; *temp_k = 5000.0f; *spot_mult = 1.6f; *white_balance = 1; return 1;
@@ -367,6 +372,20 @@ static int preset_params(int32_t preset, int* model, int* d65, float* temp_k, fl
     case IXG_PRESET_ILLUMINANT_A: *model = MODEL_STUDIO; *d65 = 0; *temp_k = 2856.0f; *spot_mult = 1.2f; *white_balance = 1; return 1;
     case IXG_PRESET_DAYLIGHT: *model = MODEL_STUDIO; *d65 = 1; *temp_k = 6500.0f; *spot_mult = 1.0f; *white_balance = 0; return 1;
     case IXG_PRESET_ISO_HEMISPHERE: *model = MODEL_ISO; *d65 = 1; *temp_k = 6500.0f; *spot_mult = 1.0f; *white_balance = 0; return 1;
+    case IXG_PRESET_LIGHT_TENT: *model = MODEL_TENT; *d65 = 0; *temp_k = 5000.0f; *spot_mult = 1.0f; *white_balance = 1; return 1;
+    case IXG_PRESET_WHITE_TRAY:
+        *model = MODEL_TENT; *d65 = 1; *temp_k = 5000.0f; *spot_mult = 0.0f; *white_balance = 0;
+        *tent = (tent_t){5.0f, 0.0f, 0.0f, 0.8f, 1.0f};
+        return 1;
+    case IXG_PRESET_SHOP_LIGHTS:
+        *model = MODEL_TENT; *d65 = 1; *temp_k = 5000.0f; *spot_mult = 1.0f; *white_balance = 0;
+        *tent = (tent_t){3.0f, 0.0f, 1.0f, 0.05f, 0.0f};
+        return 1;
+    case IXG_PRESET_WINDOW_DAYLIGHT:
+        *model = MODEL_TENT; *d65 = 1; *temp_k = 5000.0f; *spot_mult = 1.0f; *white_balance = 0;
+        *tent = (tent_t){0.4f, 0.0f, 0.0f, 0.03f, 0.0f};
+        return 1;
+    case IXG_PRESET_DAYLIGHT_DOME: *model = MODEL_DOME; *d65 = 1; *temp_k = 6500.0f; *spot_mult = 1.0f; *white_balance = 0; return 1;
     default: return 0;
     }
 }
@@ -396,7 +415,8 @@ static void studio_rig_new(float yaw, float pitch, env_t* e) { /* optics/studio_
 static int env_build(const ixg_environment* src, env_t* e) {
     float temp_k, spot;
     int wb;
-    if (!preset_params(src->preset, &e->model, &e->d65, &temp_k, &spot, &wb)) return 0;
+    if (!preset_params(src->preset, &e->model, &e->d65, &temp_k, &spot, &wb, &e->tent)) return 0;
+    e->tray = src->backdrop;
     e->bb = bb_norm_new(temp_k);
     e->exposure = src->exposure;
     e->spot_mult = spot;
@@ -422,21 +442,53 @@ typedef struct {
     int model;
     float bg, softbox, fill, ring[16];
     int has_softbox, has_fill, ring_len;
+    int has_ground;
+    float horizon, ground;
     float scaled;
 } dir_light;
+
+/* lit_ground (rig.rs:230-236): the tray replaces the model's own ground. */
+static float lit_ground(float horizon, float vis, float tray, float own_ground) {
+    return tray > 0.0f ? ((1.0f - horizon) * vis) * tray : own_ground * (1.0f - horizon);
+}
 
 static dir_light direction_lighting(const env_t* e, v3 dir, v3 observer) {
     dir_light L;
     memset(&L, 0, sizeof L);
     v3 d = vnormalize(dir);
     L.model = e->model;
-    if (e->model == MODEL_ISO) { /* rig.rs:238-252, tray = 0 */
-        float horizon = smoothstep_f(-0.05f, 0.05f, d.y);
-        float vis = 1.0f - smoothstep_f(e->shadow[0], e->shadow[1], vdot(d, observer));
-        L.scaled = horizon * vis;
+    if (e->model != MODEL_STUDIO) {
+        const float horizon = smoothstep_f(-0.05f, 0.05f, d.y);             /* rig.rs:98-100 */
+        const float vis = 1.0f - smoothstep_f(e->shadow[0], e->shadow[1], vdot(d, observer));
+        if (e->model == MODEL_ISO) { /* rig.rs:238-252 */
+            L.scaled = e->tray > 0.0f ? fmaf((1.0f - horizon) * vis, e->tray, horizon * vis) : horizon * vis;
+        } else if (e->model == MODEL_TENT) { /* light_tent_lighting, rig.rs:254-294 */
+            const tent_t* T = &e->tent;
+            float walls = fmaf(0.08f, maxf(d.y, 0.0f), 0.14f) * T->walls;
+            if (T->flat > 0.0f) walls = fmaf(0.18f * T->walls, T->flat, walls * (1.0f - T->flat));
+            float card = 0.0f;
+            static const int slots[3] = {4, 8, 12};
+            for (int k = 0; k < 3; ++k)
+                card = maxf(card, smoothstep_f(0.898794f, 0.9612617f, vdot(d, e->ring[slots[k]])));
+            walls *= fmaf(card * T->cards, -0.9f, 1.0f);
+            const float key = smoothstep_f(0.7660444f, 0.9396926f, vdot(d, e->key_dir)) * (1.4f * e->spot_mult);
+            const float spark =
+                smoothstep_f(0.9961947f, 0.9993908f, vdot(d, e->fill_dir)) * (5.0f * e->spot_mult) * T->spark;
+            const float above = ((walls + key) + spark) * (horizon * vis);
+            L.scaled = above + lit_ground(horizon, vis, e->tray, T->ground);
+        } else { /* daylight_dome_lighting, rig.rs:296-320 */
+            const float sun_dot = vdot(d, e->key_dir);
+            const float sky = fmaf(0.08f, 1.0f - maxf(d.y, 0.0f), 0.10f);
+            const float glow = maxf(sun_dot, 0.0f);
+            const float glow2 = glow * glow;
+            const float glow4 = glow2 * glow2;
+            const float aureole = (glow4 * glow4) * 0.30f;
+            const float above = (sky + aureole) * (horizon * vis);
+            L.scaled = above + lit_ground(horizon, vis, e->tray, 0.04f);
+        }
         return L;
     }
-    /* studio_rig_lighting, rig.rs:322-370 (ambient fill on, tray = 0) */
+    /* studio_rig_lighting, rig.rs:322-370 (ambient fill on) */
     L.bg = maxf(fmaf(0.012f, fmaf(d.y, 0.5f, 0.5f), 0.015f), 0.005f) * e->exposure;
     float key_dot = maxf(vdot(d, e->key_dir), 0.0f);
     if (key_dot > 0.0f) {
@@ -455,17 +507,23 @@ static dir_light direction_lighting(const env_t* e, v3 dir, v3 observer) {
             L.ring[L.ring_len++] = powi_f(spark, 6) * 22.0f * e->spot_mult * e->exposure;
         }
     }
+    if (e->tray > 0.0f) { /* the tray under the rig, rig.rs:358-364 */
+        L.has_ground = 1;
+        L.horizon = smoothstep_f(-0.05f, 0.05f, d.y);
+        L.ground = ((1.0f - L.horizon) * e->tray) * e->exposure;
+    }
     return L;
 }
 static float spec_power(const env_t* e, float lambda) {
     return e->d65 ? ixo_d65(lambda) : bb_with(e->bb, lambda);
 }
 static float light_radiance(const dir_light* L, float sp, float exposure) { /* rig.rs:178-200 */
-    if (L->model == MODEL_ISO) return L->scaled * (sp * exposure);
+    if (L->model != MODEL_STUDIO) return L->scaled * (sp * exposure);
     float r = L->bg * sp;
     if (L->has_softbox) r = fmaf(L->softbox, sp, r);
     if (L->has_fill) r = fmaf(L->fill, sp, r);
     for (int i = 0; i < L->ring_len; ++i) r = fmaf(L->ring[i], sp, r);
+    if (L->has_ground) r = fmaf(L->horizon, r, L->ground * sp);
     return r;
 }
 
@@ -525,6 +583,12 @@ static void plane_new(v3 n, float d, ixg_facet_plane* p) { /* geometry/plane.rs:
  * correctly; glibc 2.35's sinf returns the f32 one ULP above it, which moves
  * the 16 lower-girdle normals and, through them, the Fire index. */
 static float sin_cr(float x) { return (float)sin((double)x); }
+/* Correctly rounded f32 atan2 (double evaluation, one rounding) for the
+ * Stokes frame rotation angle.  glibc 2.35's atan2f misrounds some of these
+ * arguments by one ULP; with the correctly rounded angle the reference's
+ * whole-frame pin (renderer/gpu/pin_tests.rs:128-141, offset 0) matches bit
+ * for bit, with glibc's it does not. */
+static float atan2_cr(float y, float x) { return (float)atan2((double)y, (double)x); }
 static float cos_cr(float x) { return (float)cos((double)x); }
 static float index_to_azimuth(float index, float teeth) { /* cuts/round_brilliant.rs:65 */
     return 2.0f * F32_PI * (index + 0.0f) / teeth;
@@ -740,7 +804,7 @@ static void facet_event(const scene_t* S, path_t* P, uint32_t bounce, uint32_t s
     if (P->has_prev && vlen_sq(cur_plane) > 1e-6f && vlen_sq(P->prev_plane) > 1e-6f) {
         float cos_psi = clampf(vdot(P->prev_plane, cur_plane), -1.0f, 1.0f);
         float sin_psi = vdot(vcross(P->prev_plane, cur_plane), vnormalize_or_zero(P->k));
-        float psi = atan2f(sin_psi, cos_psi);
+        float psi = atan2_cr(sin_psi, cos_psi);
         mat4 R = mueller_frame_rotation(psi);
         for (int c = 0; c < NCH; ++c) P->st[c] = st_apply(&R, P->st[c]);
     }
@@ -1056,7 +1120,7 @@ static void facet_event_aniso(const scene_t* S, path_t* P, uint32_t bounce, uint
     if (P->has_prev && vlen_sq(cur_plane) > 1e-6f && vlen_sq(P->prev_plane) > 1e-6f) {
         float cos_psi = clampf(vdot(P->prev_plane, cur_plane), -1.0f, 1.0f);
         float sin_psi = vdot(vcross(P->prev_plane, cur_plane), vnormalize_or_zero(P->k));
-        mat4 R = mueller_frame_rotation(atan2f(sin_psi, cos_psi));
+        mat4 R = mueller_frame_rotation(atan2_cr(sin_psi, cos_psi));
         for (int c = 0; c < NCH; ++c) P->st[c] = st_apply(&R, P->st[c]);
     }
     P->prev_plane = cur_plane;
@@ -1862,6 +1926,7 @@ static int mctx_build(const ixg_facet_plane* planes, uint32_t n, const ixg_dispe
     ixg_environment scoring = *env;
     scoring.exposure = 1.0f;
     scoring.head_shadow_deg = 16.0f;
+    scoring.backdrop = 0.0f; /* the tray is not part of the scoring radiance (lighting.rs:148-152) */
     if (!env_build(&scoring, &M->env)) return 0;
     M->spec550 = spec_power(&M->env, 550.0f);
     float s[96];

@@ -72,7 +72,18 @@ def test_route_predicate():
     assert tracer.route(sc2) == abi.ROUTE_OK                  # uniaxial
     sc2.material_abi.biaxial = 1
     assert tracer.route(sc2) == abi.ROUTE_UNSUPPORTED_MATERIAL
-    env = tracer.LightingPreset.studio("LightTent")
+    for name in ("LightTent", "WhiteTray", "ShopLights", "WindowDaylight", "DaylightDome"):
+        env = tracer.LightingPreset.studio(name)
+        sc3 = tracer.FrameScene(sc.scene.camera, 8, 8, sc.planes, tracer.GemMaterial.diamond(), 16, env)
+        assert tracer.route(sc3) == abi.ROUTE_OK, name           # the lit models
+    env.backdrop = 0.23                                              # a lit backdrop tray
+    sc3 = tracer.FrameScene(sc.scene.camera, 8, 8, sc.planes, tracer.GemMaterial.diamond(), 16, env)
+    assert tracer.route(sc3) == abi.ROUTE_OK
+    for preset in (6, 12, 13, 14):      # DaylightSun (sun NEE), ASET, the UV lamps: CPU only
+        env = abi.Environment(preset, 1.0, 0.0, 0.5, 0.0, 1.0, 16.0)
+        sc3 = tracer.FrameScene(sc.scene.camera, 8, 8, sc.planes, tracer.GemMaterial.diamond(), 16, env)
+        assert tracer.route(sc3) == abi.ROUTE_UNSUPPORTED_ENVIRONMENT, preset
+    env = abi.Environment(0, 1.0, 0.0, 0.5, float("nan"), 1.0, 16.0)
     sc3 = tracer.FrameScene(sc.scene.camera, 8, 8, sc.planes, tracer.GemMaterial.diamond(), 16, env)
     assert tracer.route(sc3) == abi.ROUTE_UNSUPPORTED_ENVIRONMENT
     big = np.tile(tracer.planes_array(sc.planes), (4, 1))

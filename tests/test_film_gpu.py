@@ -40,16 +40,19 @@ def glam_normalize(v):
     return v * (np.float32(1.0) / np.sqrt(np.float32(d)))
 
 
+@pytest.mark.parametrize("material", ["diamond", "sapphire"])
 @pytest.mark.parametrize("row", range(4))
-def test_golden_rays_on_device(backend, row):
+def test_golden_rays_on_device(backend, row, material):
     t = GOLDEN["golden_regression"]["traces"][row]
     yaw = math.pi if t["light_yaw"] == "PI" else t["light_yaw"]
-    print(f"golden row {row} preset={t['preset']} seed={t['seed']}", flush=True)
-    sc = workloads.golden_scene(t["preset"], yaw, t["light_pitch"])
+    print(f"golden row {row} {material} preset={t['preset']} seed={t['seed']}", flush=True)
+    mat = tracer.GemMaterial.sapphire() if material == "sapphire" else None
+    sc = workloads.golden_scene(t["preset"], yaw, t["light_pitch"], material=mat)
     d = glam_normalize(t["dir"]) if t["normalize"] else np.asarray(t["dir"], np.float32)
     hero = np.float32(oracle_lib.lib().ixo_hash_u32(t["seed"])) / np.float32(4294967295.0)
     xyz, ev = backend.trace_rays(sc, [t["origin"]], [d], [t["seed"]], [hero])
-    expect = np.array(t["diamond_bits"], np.uint32).view(np.float32)
+    expect = np.array(t[f"{material}_bits"], np.uint32).view(np.float32)
+    print(f"  ulp {ulp_dist(xyz[0], expect).tolist()}", flush=True)
     assert ulp_dist(xyz[0], expect).max() <= 64, (xyz[0], expect)
 
 
