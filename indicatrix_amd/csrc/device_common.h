@@ -255,28 +255,31 @@ __device__ __forceinline__ float lit_factor(const EnvParams& E, V3 dir, V3 obser
 // The reference scan (RT/intersect.rs:101-132, simd/slab.rs:147-160): one
 // correctly rounded division per plane, t_near the largest entering t and
 // t_far the smallest exiting t, ties to the lowest index.
+//
+// Written without branches: every plane's quotient is formed and the running
+// bests are updated by selects, and a parallel plane with the origin outside
+// it (the reference's early miss) is remembered and reported after the loop,
+// which returns the same result because the early return has no side effects.
+// Straight-line code keeps the wave's lanes (different rays) in lock step.
 __device__ __forceinline__ int intersect(const float4* __restrict__ pl, uint32_t n, V3 o, V3 d, float& t_out) {
     float t_near = -1e30f, t_far = 1e30f;
     int near_idx = -1, far_idx = -1;
+    bool outside_parallel = false;
     for (uint32_t i = 0; i < n; ++i) {
-        float4 p = pl[i];
-        float denom = (p.x * d.x + p.y * d.y) + p.z * d.z;
-        float side = p.w + ((p.x * o.x + p.y * o.y) + p.z * o.z);
-        if (fabsf(denom) > 1e-7f) {
-            float t = (-side) / denom;
-            if (denom < 0.0f) {
-                if (t > t_near) {
-                    t_near = t;
-                    near_idx = (int)i;
-                }
-            } else if (t < t_far) {
-                t_far = t;
-                far_idx = (int)i;
-            }
-        } else if (side > 0.0f) {
-            return -1;
-        }
+        const float4 p = pl[i];
+        const float denom = (p.x * d.x + p.y * d.y) + p.z * d.z;
+        const float side = p.w + ((p.x * o.x + p.y * o.y) + p.z * o.z);
+        const float t = (-side) / denom;
+        const bool valid = fabsf(denom) > 1e-7f;
+        const bool up_near = valid && denom < 0.0f && t > t_near;
+        const bool up_far = valid && !(denom < 0.0f) && t < t_far;
+        t_near = up_near ? t : t_near;
+        near_idx = up_near ? (int)i : near_idx;
+        t_far = up_far ? t : t_far;
+        far_idx = up_far ? (int)i : far_idx;
+        outside_parallel = outside_parallel || (!valid && side > 0.0f);
     }
+    if (outside_parallel) return -1;
     if (t_near > t_far) return -1;
     if (t_near > 1e-4f) {
         t_out = t_near;
