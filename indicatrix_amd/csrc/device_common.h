@@ -256,6 +256,24 @@ __device__ __forceinline__ float lit_factor(const EnvParams& E, V3 dir, V3 obser
 // correctly rounded division per plane, t_near the largest entering t and
 // t_far the smallest exiting t, ties to the lowest index.
 //
+// The quotient is the correctly rounded a / b of IEEE division.  The scan's
+// operands are unit-normal dot products and offsets: |b| <= 1 and, for every
+// plane whose quotient is used, |b| > 1e-7, with |a| far from the overflow
+// and underflow ranges, so the device's division sequence needs neither its
+// operand pre-scaling nor its special-case fix-up; what remains (reciprocal,
+// one Newton step, two residual corrections) is the same arithmetic and
+// rounds identically.  A plane whose |b| <= 1e-7 gets an unused value.
+__device__ __forceinline__ float div_rn_scan(float a, float b) {
+    float y = __builtin_amdgcn_rcpf(b);
+    const float e = __builtin_fmaf(-b, y, 1.0f);
+    y = __builtin_fmaf(e, y, y);
+    float q = a * y;
+    float r = __builtin_fmaf(-b, q, a);
+    q = __builtin_fmaf(r, y, q);
+    r = __builtin_fmaf(-b, q, a);
+    return __builtin_fmaf(r, y, q);
+}
+
 // Written without branches: every plane's quotient is formed and the running
 // bests are updated by selects, and a parallel plane with the origin outside
 // it (the reference's early miss) is remembered and reported after the loop,
@@ -269,7 +287,7 @@ __device__ __forceinline__ int intersect(const float4* __restrict__ pl, uint32_t
         const float4 p = pl[i];
         const float denom = (p.x * d.x + p.y * d.y) + p.z * d.z;
         const float side = p.w + ((p.x * o.x + p.y * o.y) + p.z * o.z);
-        const float t = (-side) / denom;
+        const float t = div_rn_scan(-side, denom);
         const bool valid = fabsf(denom) > 1e-7f;
         const bool up_near = valid && denom < 0.0f && t > t_near;
         const bool up_far = valid && !(denom < 0.0f) && t < t_far;
