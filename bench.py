@@ -27,7 +27,7 @@ MI355X_HBM_PEAK_GBS = 8000.0
 # HBM bytes per film_persistent_kernel launch at this workload: FETCH_SIZE x 2.0
 # + WRITE_SIZE x 1.0 (scales from a 1 GiB copy) from two rocprofv3 --pmc runs,
 # recorded with their commands in profiles/film_persistent_kernel_c1.md.
-FILM_KERNEL_HBM_BYTES = (1633 * 2.0 + 18261) * 1024
+FILM_KERNEL_HBM_BYTES = (5333.05 * 2.0 + 29044.98) * 1024   # PMC FETCH_SIZE x 2 (calibrated) + WRITE_SIZE, per launch
 
 
 def parse():
@@ -43,6 +43,8 @@ def parse():
     ap.add_argument("--optimize-steps", type=int, default=8)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a 1 GiB device copy (scales rocprof FETCH/WRITE_SIZE)")
+    ap.add_argument("--part", choices=["c2", "sweep", "pose", "optimize"],
+                    help="run one secondary workload alone and print its figures (profiling runs)")
     return ap.parse_args()
 
 
@@ -247,6 +249,19 @@ def main():
 
     if args.copy_calibration:
         copy_calibration()
+        return
+    if args.part:
+        backend = tracer.GpuBackend()
+        if args.part == "c2":
+            result = film_c2_rate(backend, 2)[0]
+        elif args.part == "sweep":
+            result = sweep_rate(backend)[0]
+        elif args.part == "pose":
+            result = pose_sweep_rate(backend)[0]
+        else:
+            result = screened_rate(backend, args.optimize_steps)
+        print(json.dumps({"part": args.part, "result": result}), flush=True)
+        backend.close()
         return
     if not torch.cuda.is_available():
         print(json.dumps({"error": "no HIP device visible"}))
