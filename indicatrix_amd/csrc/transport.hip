@@ -272,7 +272,7 @@ __device__ __forceinline__ V3 reflect_dir(V3 k, V3 normal) { return k - normal *
 // axis, the eigenmode-drawing scalar arms where it is, assigned-mode
 // absorption and the internal o<->e re-coupling draw.
 template <bool EVENTS, bool ISO, bool ABSORB>
-__device__ __noinline__ int facet_aniso(const FilmParams& P, const float4* __restrict__ planes, PathState& S,
+__device__ __forceinline__ int facet_aniso(const FilmParams& P, const float4* __restrict__ planes, PathState& S,
                                         float t_hit, int facet) {
     const uint32_t bounce = S.bounce;
     const V3 cax = v3(P.an.c_axis[0], P.an.c_axis[1], P.an.c_axis[2]);

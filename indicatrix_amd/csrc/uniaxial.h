@@ -210,7 +210,7 @@ struct EntryInc {
     Cx inc_s[4], inc_p[4], refl_s[4], refl_p[4];
 };
 
-__device__ __noinline__ void entry_incidence(const UFrame& F, EntryInc& I) {
+__device__ __forceinline__ void entry_incidence(const UFrame& F, EntryInc& I) {
     const float n1 = 1.0f;
     const float k_tan = n1 * F.sin_i;
     const CV3 k_fwd = cv_wave(F.that, F.zhat, k_tan, cx(n1 * F.cos_i, 0.0f));
@@ -232,7 +232,7 @@ struct EntrySol {
     V3 o_hat, e_hat;
 };
 
-__device__ __noinline__ void entry_solve(const EntryInc& I, float n_o, float n_e, V3 c, const UFrame& F,
+__device__ __forceinline__ void entry_solve(const EntryInc& I, float n_o, float n_e, V3 c, const UFrame& F,
                                             EntrySol& out, bool want_axes) {
     const float k_tan = 1.0f * F.sin_i;
     const QRoots R = q_roots(n_o, n_e, F, k_tan);
@@ -265,7 +265,7 @@ struct InternalSol {
     float flux_ro, flux_re, flux_ts, flux_tp, flux_inc;
 };
 
-__device__ __noinline__ InternalSol internal_solve(float n_mode_inc, float n_o, float n_e, V3 c, const UFrame& F,
+__device__ __forceinline__ InternalSol internal_solve(float n_mode_inc, float n_o, float n_e, V3 c, const UFrame& F,
                                                       bool incident_is_ordinary) {
     const float k_tan = n_mode_inc * F.sin_i;
     const QRoots R = q_roots(n_o, n_e, F, k_tan);
