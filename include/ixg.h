@@ -372,6 +372,118 @@ ixg_status ixg_mlp_train_step(ixg_mlp* mlp, const float* d_x, const float* d_t, 
                               float lr, float* loss);
 ixg_status ixg_mlp_sync(ixg_mlp* mlp);
 
+/* ---- GemCAD .asc designs (indicatrix-formats/src/asc, IX/geometry/cuts/asc_schedule.rs) ----
+ * A parsed design is an opaque ixg_asc; its header, tiers and text fields are
+ * read with the accessors below and its facet planes with ixg_asc_planes. */
+#define IXG_ASC_MAX_GEAR_TEETH 720 /* AscParseError::MAX_GEAR_TEETH (asc/error.rs) */
+typedef struct ixg_asc ixg_asc;
+
+/* AscParseError's variants (asc/error.rs), plus text that is not UTF-8. */
+typedef enum ixg_asc_error_kind {
+    IXG_ASC_ERR_NONE = 0,
+    IXG_ASC_ERR_EMPTY_INPUT = 1,
+    IXG_ASC_ERR_GEAR_LINE_MISSING_FIELDS = 2,
+    IXG_ASC_ERR_GEAR_TEETH_NOT_NUMERIC = 3,
+    IXG_ASC_ERR_GEAR_REFERENCE_ANGLE_NOT_NUMERIC = 4,
+    IXG_ASC_ERR_SYMMETRY_LINE_MISSING_FIELDS = 5,
+    IXG_ASC_ERR_SYMMETRY_ORDER_NOT_NUMERIC = 6,
+    IXG_ASC_ERR_REFRACTIVE_INDEX_MISSING = 7,
+    IXG_ASC_ERR_REFRACTIVE_INDEX_NOT_NUMERIC = 8,
+    IXG_ASC_ERR_NOT_WHOLE_NUMBER = 9,
+    IXG_ASC_ERR_GEAR_TEETH_ZERO = 10,
+    IXG_ASC_ERR_GEAR_TEETH_TOO_LARGE = 11,
+    IXG_ASC_ERR_MIRROR_FLAG_INVALID = 12,
+    IXG_ASC_ERR_MISSING_GEAR_LINE = 13,
+    IXG_ASC_ERR_MISSING_SYMMETRY_LINE = 14,
+    IXG_ASC_ERR_MISSING_REFRACTIVE_INDEX_LINE = 15,
+    IXG_ASC_ERR_NO_TIER_RECORDS = 16,
+    IXG_ASC_ERR_TIER_RECORD_TOO_SHORT = 17,
+    IXG_ASC_ERR_ANGLE_NOT_NUMERIC = 18,
+    IXG_ASC_ERR_MAST_NOT_NUMERIC = 19,
+    IXG_ASC_ERR_NON_FINITE_VALUE = 20,
+    IXG_ASC_ERR_SYMMETRY_ORDER_ZERO = 21,
+    IXG_ASC_ERR_REFRACTIVE_INDEX_OUT_OF_RANGE = 22,
+    IXG_ASC_ERR_INVALID_UTF8 = 23
+} ixg_asc_error_kind;
+
+/* A parse failure: the variant, its 1-based line (0 for whole-file errors),
+ * the variant's fields and the reference's Display wording in message. */
+typedef struct ixg_asc_error {
+    int32_t kind;          /* ixg_asc_error_kind */
+    uint32_t line;
+    double value;          /* NotWholeNumber / NonFiniteValue / RefractiveIndexOutOfRange */
+    int32_t teeth;         /* GearTeethTooLarge */
+    uint32_t field_count;  /* TierRecordTooShort */
+    char field[32];        /* e.g. "angle", "gear tooth count" */
+    char token[128];
+    char message[512];
+} ixg_asc_error;
+
+typedef enum ixg_asc_line_ending { IXG_ASC_LF = 0, IXG_ASC_CRLF = 1 } ixg_asc_line_ending;
+
+/* AscSchedule's scalar fields (asc/schedule.rs). */
+typedef struct ixg_asc_info {
+    int32_t gear_teeth;            /* signed: a negative count is a handedness convention */
+    uint32_t symmetry_order;
+    int32_t mirror;
+    int32_t line_ending;           /* ixg_asc_line_ending */
+    double gear_reference_angle;
+    double refractive_index;
+    uint32_t n_tiers, n_headers, n_footnotes, n_warnings;
+    uint32_t gear_line;            /* 1-based line of the gear record kept (parse_asc_with_gear_line) */
+    uint32_t facet_plane_count;    /* AscSchedule::facet_plane_count: planes before dedup */
+} ixg_asc_info;
+
+/* One `a` record (AscTier, asc/tier.rs); a culet is angle_deg == -0.0. */
+typedef struct ixg_asc_tier {
+    double angle_deg;
+    double mast;
+    uint32_t n_indices;
+    uint32_t n_index_names;
+} ixg_asc_tier;
+
+typedef enum ixg_asc_text_field {
+    IXG_ASC_TEXT_VERSION = 0,   /* gemcad_version                 */
+    IXG_ASC_TEXT_HEADER = 1,    /* headers[i]                     */
+    IXG_ASC_TEXT_FOOTNOTE = 2,  /* footnotes[i]                   */
+    IXG_ASC_TEXT_WARNING = 3,   /* warnings[i]                    */
+    IXG_ASC_TEXT_TIER_NAME = 4, /* tiers[i].name ("c/d" joined)   */
+    IXG_ASC_TEXT_TIER_NOTES = 5 /* tiers[i].notes                 */
+} ixg_asc_text_field;
+
+/* parse_asc / parse_asc_with_gear_line (asc/parse.rs:54,68): UTF-8 text, a
+ * leading BOM ignored.  IXG_INVALID with *err filled on a parse error. */
+ixg_status ixg_asc_parse_text(const char* text, size_t len, ixg_asc** out, ixg_asc_error* err);
+/* parse_asc_bytes (asc/decode.rs): raw file bytes, UTF-8 or Windows-1252,
+ * lone CR line ends read as LF. */
+ixg_status ixg_asc_parse_bytes(const uint8_t* bytes, size_t len, ixg_asc** out, ixg_asc_error* err);
+void ixg_asc_free(ixg_asc* schedule);
+ixg_status ixg_asc_info_get(const ixg_asc* schedule, ixg_asc_info* info);
+/* Tier i; indices (may be NULL) receives its n_indices index positions. */
+ixg_status ixg_asc_tier_get(const ixg_asc* schedule, uint32_t i, ixg_asc_tier* tier, double* indices,
+                            uint32_t cap);
+/* A text field (ixg_asc_text_field) as UTF-8, NUL-terminated in buf[cap];
+ * returns its full byte length (call with cap 0 to size the buffer). */
+size_t ixg_asc_text(const ixg_asc* schedule, int32_t field, uint32_t i, char* buf, size_t cap);
+/* AscTier::index_names[k] of tier: the position it binds to, and the name. */
+size_t ixg_asc_index_name(const ixg_asc* schedule, uint32_t tier, uint32_t k, uint32_t* position,
+                          char* buf, size_t cap);
+/* StandardGemCuts::from_asc_schedule (IX/geometry/cuts/asc_schedule.rs:20) on
+ * a parsed design: its deduplicated facet planes, in schedule order. *n_out
+ * is always the plane count; IXG_RESOURCE_LIMIT when cap is below it. */
+ixg_status ixg_asc_planes(const ixg_asc* schedule, ixg_facet_plane* out, uint32_t cap, uint32_t* n_out);
+/* The same on plain tier data (tiers[t].n_indices positions each, in order,
+ * from indices[]; n_index_names is ignored), with the two bookkeeping views
+ * of asc_schedule.rs:29,60: raw_to_plane[facet_plane_count] (may be NULL)
+ * maps every raw plane to its output plane (asc_plane_provenance), and
+ * tier_ends[n_tiers] (may be NULL) is the number of output planes up to the
+ * end of each tier (asc_tier_plane_boundaries at gear_reference_angle).
+ * out may be NULL to count only. */
+ixg_status ixg_asc_schedule_planes(const ixg_asc_tier* tiers, uint32_t n_tiers, const double* indices,
+                                   uint32_t n_indices, int32_t gear_teeth, double gear_reference_angle,
+                                   ixg_facet_plane* out, uint32_t cap, uint32_t* n_out,
+                                   uint32_t* raw_to_plane, uint32_t* tier_ends);
+
 /* ---- host helpers (same f32 arithmetic as the reference, glibc libm) ---- */
 /* Camera::new (RT/camera.rs:108). */
 void ixg_camera_new(float yaw, float pitch, float distance, float fov_deg, uint32_t width,

@@ -58,6 +58,18 @@ def _declare(lib):
                                   P(C.c_float), C.c_uint32, P(abi.Environment), C.c_int, P(C.c_float)]),
         "ixg_axis_profiles": (st, [C.c_void_p, P(abi.FacetPlane), C.c_uint32, P(abi.Design), C.c_uint32,
                                    P(C.c_float), C.c_uint32, P(abi.Environment), P(C.c_float)]),
+        "ixg_asc_parse_text": (st, [C.c_char_p, C.c_size_t, P(C.c_void_p), P(abi.AscError)]),
+        "ixg_asc_parse_bytes": (st, [C.c_char_p, C.c_size_t, P(C.c_void_p), P(abi.AscError)]),
+        "ixg_asc_free": (None, [C.c_void_p]),
+        "ixg_asc_info_get": (st, [C.c_void_p, P(abi.AscInfo)]),
+        "ixg_asc_tier_get": (st, [C.c_void_p, C.c_uint32, P(abi.AscTier), P(C.c_double), C.c_uint32]),
+        "ixg_asc_text": (C.c_size_t, [C.c_void_p, C.c_int32, C.c_uint32, C.c_char_p, C.c_size_t]),
+        "ixg_asc_index_name": (C.c_size_t, [C.c_void_p, C.c_uint32, C.c_uint32, P(C.c_uint32),
+                                            C.c_char_p, C.c_size_t]),
+        "ixg_asc_planes": (st, [C.c_void_p, P(abi.FacetPlane), C.c_uint32, P(C.c_uint32)]),
+        "ixg_asc_schedule_planes": (st, [P(abi.AscTier), C.c_uint32, P(C.c_double), C.c_uint32,
+                                         C.c_int32, C.c_double, P(abi.FacetPlane), C.c_uint32,
+                                         P(C.c_uint32), P(C.c_uint32), P(C.c_uint32)]),
         "ixg_mlp_param_count": (C.c_uint32, []),
         "ixg_mlp_create": (st, [C.c_void_p, C.c_uint32, C.c_uint64, P(C.c_void_p)]),
         "ixg_mlp_destroy": (None, [C.c_void_p]),

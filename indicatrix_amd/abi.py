@@ -144,4 +144,40 @@ class Design(C.Structure):
                 ("dispersion", Dispersion)]
 
 
+class AscError(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("line", C.c_uint32), ("value", C.c_double),
+                ("teeth", C.c_int32), ("field_count", C.c_uint32), ("field", C.c_char * 32),
+                ("token", C.c_char * 128), ("message", C.c_char * 512)]
+
+
+class AscInfo(C.Structure):
+    _fields_ = [("gear_teeth", C.c_int32), ("symmetry_order", C.c_uint32), ("mirror", C.c_int32),
+                ("line_ending", C.c_int32), ("gear_reference_angle", C.c_double),
+                ("refractive_index", C.c_double), ("n_tiers", C.c_uint32),
+                ("n_headers", C.c_uint32), ("n_footnotes", C.c_uint32), ("n_warnings", C.c_uint32),
+                ("gear_line", C.c_uint32), ("facet_plane_count", C.c_uint32)]
+
+
+class AscTier(C.Structure):
+    _fields_ = [("angle_deg", C.c_double), ("mast", C.c_double), ("n_indices", C.c_uint32),
+                ("n_index_names", C.c_uint32)]
+
+
+# ixg_asc_error_kind, named as the reference's AscParseError variants.
+ASC_ERROR_KINDS = {
+    0: "None", 1: "EmptyInput", 2: "GearLineMissingFields", 3: "GearTeethNotNumeric",
+    4: "GearReferenceAngleNotNumeric", 5: "SymmetryLineMissingFields",
+    6: "SymmetryOrderNotNumeric", 7: "RefractiveIndexMissing", 8: "RefractiveIndexNotNumeric",
+    9: "NotWholeNumber", 10: "GearTeethZero", 11: "GearTeethTooLarge", 12: "MirrorFlagInvalid",
+    13: "MissingGearLine", 14: "MissingSymmetryLine", 15: "MissingRefractiveIndexLine",
+    16: "NoTierRecords", 17: "TierRecordTooShort", 18: "AngleNotNumeric", 19: "MastNotNumeric",
+    20: "NonFiniteValue", 21: "SymmetryOrderZero", 22: "RefractiveIndexOutOfRange",
+    23: "InvalidUtf8",
+}
+ASC_MAX_GEAR_TEETH = 720
+
+
 assert C.sizeof(OpticalPixel) == 56
+assert C.sizeof(AscError) == 696
+assert C.sizeof(AscInfo) == 56
+assert C.sizeof(AscTier) == 24

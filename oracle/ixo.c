@@ -15,6 +15,7 @@
 #define _GNU_SOURCE
 #include "ixo.h"
 
+#include <float.h>
 #include <math.h>
 #include <pthread.h>
 #include <stdlib.h>
@@ -2220,10 +2221,16 @@ static double face_area_d(d3 n, double m, const d3* V, int nv) {
 /* measure_model_width: the short axis of the feasible vertices' x/z bounding
  * box (measure.rs:79-150).  Returns 0 when the arrangement does not measure
  * (no planes, unbounded, fewer than 4 vertices, non-positive volume). */
-int ixo_model_width(const ixg_facet_plane* planes, uint32_t n, double* width) {
+/* The vertex set behind measure_solid (measure.rs:79-150): the planes
+ * deduplicated and power-of-two normalised into N / Mo (the first *real
+ * entries, then the six blank faces), every feasible well-conditioned triple
+ * vertex in *V (normalised units; the caller frees it).  Returns 1 when the
+ * planes close a finite solid of positive volume. */
+static int solid_vertices(const ixg_facet_plane* planes, uint32_t n, d3* N, double* Mo, int* real_out,
+                          double* scale_out, d3** V_out, int* nv_out) {
+    *V_out = NULL;
+    *nv_out = 0;
     if (n == 0 || n > MW_MAX_PLANES) return 0;
-    d3 N[MW_MAX_PLANES + 6];
-    double Mo[MW_MAX_PLANES + 6];
     int p = 0;
     for (uint32_t i = 0; i < n; ++i) { /* dedup_planes (vertices.rs:67-79) */
         d3 ni = dmk((double)planes[i].normal[0], (double)planes[i].normal[1], (double)planes[i].normal[2]);
@@ -2267,6 +2274,20 @@ int ixo_model_width(const ixg_facet_plane* planes, uint32_t n, double* width) {
         for (int i = 0; i < real; ++i) vol += Mo[i] * face_area_d(N[i], Mo[i], V, nv) / 3.0;
         if (!(isfinite(vol) && vol > 0.0)) ok = 0;
     }
+    *real_out = real;
+    *scale_out = scale;
+    *V_out = V;
+    *nv_out = nv;
+    return ok;
+}
+
+int ixo_model_width(const ixg_facet_plane* planes, uint32_t n, double* width) {
+    d3 N[MW_MAX_PLANES + 6];
+    double Mo[MW_MAX_PLANES + 6];
+    int real, nv;
+    double scale;
+    d3* V;
+    int ok = solid_vertices(planes, n, N, Mo, &real, &scale, &V, &nv);
     if (ok) {
         double x0 = INFINITY, x1 = -INFINITY, z0 = INFINITY, z1 = -INFINITY;
         for (int i = 0; i < nv; ++i) {
@@ -2278,6 +2299,167 @@ int ixo_model_width(const ixg_facet_plane* planes, uint32_t n, double* width) {
     }
     free(V);
     return ok;
+}
+
+/* ---- .asc schedules and the solids they cut ---- */
+
+/* raw_asc_planes + dedup_planes (cuts/asc_schedule.rs:152-200, 264-322):
+ * one plane per index (one per index-less tier) in schedule order, then the
+ * first plane of every near-duplicate group.  Tier t has counts[t] indices,
+ * consecutive in idx[]. */
+uint32_t ixo_asc_planes(const double* angle_deg, const double* mast, const uint32_t* counts, uint32_t n_tiers,
+                        const double* idx, int32_t gear_teeth, double gear_reference_angle,
+                        ixg_facet_plane* out, uint32_t cap) {
+    uint32_t teeth_abs = gear_teeth < 0 ? 0u - (uint32_t)gear_teeth : (uint32_t)gear_teeth;
+    const float teeth = (float)(teeth_abs > 1u ? teeth_abs : 1u);
+    const float ref = (float)gear_reference_angle;
+    uint32_t raw_n = 0;
+    for (uint32_t t = 0; t < n_tiers; ++t) raw_n += counts[t] ? counts[t] : 1u;
+    ixg_facet_plane* raw = (ixg_facet_plane*)malloc(sizeof(ixg_facet_plane) * (raw_n ? raw_n : 1u));
+    uint32_t r = 0;
+    for (uint32_t t = 0; t < n_tiers; ++t) {
+        const int crown = angle_deg[t] > 0.0 || (angle_deg[t] == 0.0 && !signbit(angle_deg[t])); /* plane.rs:25 */
+        const float theta = (float)fabs(angle_deg[t]) * TO_RAD;
+        const float st = sin_cr(theta), ct = cos_cr(theta);
+        const float d = -(float)fabs(mast[t]);
+        if (counts[t] == 0) {
+            plane_new(mk(0.0f, crown ? ct : -ct, st), d, &raw[r++]);
+            continue;
+        }
+        for (uint32_t k = 0; k < counts[t]; ++k) {
+            const float phi = 2.0f * F32_PI * ((float)*idx++ + ref) / teeth; /* round_brilliant.rs:64-66 */
+            plane_new(mk(st * cos_cr(phi), crown ? ct : -ct, st * sin_cr(phi)), d, &raw[r++]);
+        }
+    }
+    uint32_t kept = 0;
+    for (uint32_t i = 0; i < raw_n; ++i) {
+        int dup = 0;
+        for (uint32_t k = 0; k < kept && !dup; ++k) {
+            const ixg_facet_plane* a = &out[k];
+            const float sc = fmaxf(fmaxf(fabsf(a->d), fabsf(raw[i].d)), 1.0f);
+            dup = normals_coincide_d(dmk(a->normal[0], a->normal[1], a->normal[2]),
+                                     dmk(raw[i].normal[0], raw[i].normal[1], raw[i].normal[2])) &&
+                  fabsf(a->d - raw[i].d) <= (1.0f / 2048.0f) * sc;
+        }
+        if (!dup) {
+            if (kept == cap) { free(raw); return 0; }
+            out[kept++] = raw[i];
+        }
+    }
+    free(raw);
+    return kept;
+}
+
+typedef struct { double x, z; int id; } outline_pt;
+
+static int cmp_outline(const void* a, const void* b) {
+    const outline_pt* p = (const outline_pt*)a;
+    const outline_pt* q = (const outline_pt*)b;
+    if (p->x != q->x) return p->x < q->x ? -1 : 1;
+    if (p->z != q->z) return p->z < q->z ? -1 : 1;
+    return (p->id > q->id) - (p->id < q->id);
+}
+
+static double outline_turn(outline_pt o, outline_pt a, outline_pt b) { /* brep.rs:573-575 */
+    return fma(a.z - o.z, -(b.x - o.x), (a.x - o.x) * (b.z - o.z));
+}
+
+/* Incident-plane lists compared lexicographically (the solid's vertex order). */
+static const uint8_t* g_incident;
+static int g_incident_stride;
+static int cmp_incident(const void* a, const void* b) {
+    const uint8_t* p = g_incident + (size_t)(*(const int*)a) * (size_t)g_incident_stride;
+    const uint8_t* q = g_incident + (size_t)(*(const int*)b) * (size_t)g_incident_stride;
+    for (int k = 0; k < g_incident_stride; ++k) {
+        /* the list holding the lower plane index first sorts first */
+        if (p[k] != q[k]) return p[k] ? -1 : 1;
+    }
+    return 0;
+}
+
+/* The solid's girdle outline (GemPolyhedron::girdle_outline, brep.rs:544-570:
+ * the x/z convex hull of the vertices, points within 1e-6 of the bounding
+ * radius merged keeping the first vertex in incident-set order) and the
+ * length/width ratio of asc_designs.rs:63-88 over it, in f32.  *untouched is
+ * the number of planes that carry no face.  Returns 0 when the planes do not
+ * close a solid. */
+int ixo_solid_outline(const ixg_facet_plane* planes, uint32_t n, double* lw, uint32_t* untouched) {
+    d3 N[MW_MAX_PLANES + 6];
+    double Mo[MW_MAX_PLANES + 6];
+    int real, nv;
+    double scale;
+    d3* V;
+    if (!solid_vertices(planes, n, N, Mo, &real, &scale, &V, &nv)) { free(V); return 0; }
+    uint32_t bare = 0;
+    for (int i = 0; i < real; ++i) bare += !(face_area_d(N[i], Mo[i], V, nv) > 0.0);
+    *untouched = bare;
+    /* vertices in incident-set order */
+    uint8_t* inc = (uint8_t*)calloc((size_t)nv * (size_t)real, 1);
+    int* order = (int*)malloc(sizeof(int) * (size_t)nv);
+    for (int v = 0; v < nv; ++v) {
+        order[v] = v;
+        for (int q = 0; q < real; ++q) inc[(size_t)v * real + q] = fabs(ddot(N[q], V[v]) - Mo[q]) <= MW_EPS_FACE;
+    }
+    g_incident = inc;
+    g_incident_stride = real;
+    qsort(order, (size_t)nv, sizeof(int), cmp_incident);
+    float* P = (float*)malloc(sizeof(float) * 3 * (size_t)nv); /* the solid's f32 vertices */
+    double radius = 0.0;
+    for (int k = 0; k < nv; ++k) {
+        const d3 v = V[order[k]];
+        P[3 * k] = (float)(v.x * scale);
+        P[3 * k + 1] = (float)(v.y * scale);
+        P[3 * k + 2] = (float)(v.z * scale);
+        const double r2 = (double)P[3 * k] * P[3 * k] + (double)P[3 * k + 1] * P[3 * k + 1] +
+                          (double)P[3 * k + 2] * P[3 * k + 2];
+        radius = fmax(radius, sqrt(r2));
+    }
+    const double eps = 1e-6 * (double)(float)radius;
+    outline_pt* pts = (outline_pt*)malloc(sizeof(outline_pt) * (size_t)nv);
+    for (int k = 0; k < nv; ++k) { pts[k].x = P[3 * k]; pts[k].z = P[3 * k + 2]; pts[k].id = k; }
+    qsort(pts, (size_t)nv, sizeof(outline_pt), cmp_outline);
+    int m = 0;
+    for (int k = 0; k < nv; ++k) {
+        if (m > 0 && fabs(pts[k].x - pts[m - 1].x) <= eps && fabs(pts[k].z - pts[m - 1].z) <= eps) continue;
+        pts[m++] = pts[k];
+    }
+    outline_pt* hull = (outline_pt*)malloc(sizeof(outline_pt) * (size_t)(2 * m + 2));
+    int h = 0;
+    for (int k = 0; k < m; ++k) { /* lower chain */
+        while (h >= 2 && outline_turn(hull[h - 2], hull[h - 1], pts[k]) <= 0.0) --h;
+        hull[h++] = pts[k];
+    }
+    --h;
+    const int base = h;
+    for (int k = m - 1; k >= 0; --k) { /* upper chain */
+        while (h - base >= 2 && outline_turn(hull[h - 2], hull[h - 1], pts[k]) <= 0.0) --h;
+        hull[h++] = pts[k];
+    }
+    --h;
+    int bi = 0, bj = 0;
+    float best = 0.0f;
+    for (int i = 0; i < h; ++i)
+        for (int j = i + 1; j < h; ++j) {
+            const float* a = &P[3 * hull[i].id];
+            const float* b = &P[3 * hull[j].id];
+            const float dx = a[0] - b[0], dy = a[1] - b[1], dz = a[2] - b[2];
+            const float d = sqrtf((dx * dx + dy * dy) + dz * dz);
+            if (d > best) { best = d; bi = i; bj = j; }
+        }
+    const float* a = &P[3 * hull[bi].id];
+    const float* b = &P[3 * hull[bj].id];
+    v3 dir = vnormalize(mk(b[0] - a[0], b[1] - a[1], b[2] - a[2]));
+    const v3 perp = mk(-dir.z, 0.0f, dir.x);
+    float lo = FLT_MAX, hi = -FLT_MAX;
+    for (int i = 0; i < h; ++i) {
+        const float* q = &P[3 * hull[i].id];
+        const float proj = vdot(mk(q[0], q[1], q[2]), perp);
+        lo = fminf(lo, proj);
+        hi = fmaxf(hi, proj);
+    }
+    *lw = (double)(best / (hi - lo));
+    free(hull); free(pts); free(P); free(order); free(inc); free(V);
+    return 1;
 }
 
 /* girdle_half_width (metrics/fan.rs:49-60). */

@@ -83,6 +83,16 @@ float ixo_image_scintillation_pct(const ixg_optical_pixel* px, uint32_t n);
 int ixo_model_width(const ixg_facet_plane* planes, uint32_t n, double* width);
 /* girdle_half_width (IX/color/metrics/fan.rs:49): width / 2, or 1.0. */
 float ixo_girdle_half_width(const ixg_facet_plane* planes, uint32_t n);
+/* StandardGemCuts::from_asc_schedule on plain tier data (IX/geometry/cuts/
+ * asc_schedule.rs:20,152-200,264-322): tier t has counts[t] index positions,
+ * consecutive in idx[].  Returns the plane count (0 if above cap). */
+uint32_t ixo_asc_planes(const double* angle_deg, const double* mast, const uint32_t* counts, uint32_t n_tiers,
+                        const double* idx, int32_t gear_teeth, double gear_reference_angle,
+                        ixg_facet_plane* out, uint32_t cap);
+/* The girdle outline's length/width ratio (IX/geometry/brep.rs:544 with
+ * tests/optics_geometry_tests/asc_designs.rs:63-88) and the number of planes
+ * that carry no face; 0 when the planes do not close a solid. */
+int ixo_solid_outline(const ixg_facet_plane* planes, uint32_t n, double* lw, uint32_t* untouched);
 /* evaluate_gem_optical_metrics[_geom_core] (IX/color/metrics/evaluate.rs:158,305)
  * at one pose: out = {brilliance, fire, scintillation, windowing, extinction}. */
 int ixo_pose_metrics(const ixg_facet_plane* planes, uint32_t n, const ixg_material* material,

@@ -82,6 +82,11 @@ def lib():
         L.ixo_pose_metrics.argtypes = [P(abi.FacetPlane), C.c_uint32, P(abi.Material),
                                        P(abi.Environment), C.c_float, C.c_float, C.c_int,
                                        P(C.c_float)]
+        L.ixo_asc_planes.restype = C.c_uint32
+        L.ixo_asc_planes.argtypes = [P(C.c_double), P(C.c_double), P(C.c_uint32), C.c_uint32, P(C.c_double),
+                                     C.c_int32, C.c_double, P(abi.FacetPlane), C.c_uint32]
+        L.ixo_solid_outline.restype = C.c_int
+        L.ixo_solid_outline.argtypes = [P(abi.FacetPlane), C.c_uint32, P(C.c_double), P(C.c_uint32)]
         _lib = L
     return _lib
 
@@ -202,3 +207,26 @@ def axis_poses(azimuth_deg):
     poses += [(pos, np.float32(i) * to_rad) for i in range(90)]
     poses += [(neg, np.float32(i) * to_rad) for i in range(90)]
     return poses
+
+
+def asc_planes(schedule):
+    """The oracle's planes for a parsed schedule (tiers read from its fields)."""
+    n = len(schedule.tiers)
+    ang = (C.c_double * max(1, n))(*[t.angle_deg for t in schedule.tiers])
+    mast = (C.c_double * max(1, n))(*[t.mast for t in schedule.tiers])
+    counts = (C.c_uint32 * max(1, n))(*[len(t.indices) for t in schedule.tiers])
+    flat = [i for t in schedule.tiers for i in t.indices]
+    idx = (C.c_double * max(1, len(flat)))(*flat)
+    cap = max(1, schedule.facet_plane_count())
+    out = (abi.FacetPlane * cap)()
+    k = lib().ixo_asc_planes(ang, mast, counts, n, idx, schedule.gear_teeth,
+                             schedule.gear_reference_angle, out, cap)
+    return (abi.FacetPlane * k)(*out[:k])
+
+
+def solid_outline(planes):
+    """(length/width ratio, planes without a face) of the solid, or None."""
+    lw = C.c_double()
+    bare = C.c_uint32()
+    ok = lib().ixo_solid_outline(planes, len(planes), C.byref(lw), C.byref(bare))
+    return (lw.value, bare.value) if ok else None

@@ -60,6 +60,29 @@ def test_pose_metrics_match_oracle(backend, want_all):
         _check_against_oracle(dev[d], table[off:off + cnt], mat, env, poses, want_all)
 
 
+def test_pose_metrics_of_catalogue_designs_match_oracle(backend):
+    """The four committed GemCAD files, read and cut by the library, as one
+    batch of designs; each must equal the oracle's evaluate_pose."""
+    import os
+    from indicatrix_amd import asc
+    here = os.path.dirname(os.path.abspath(__file__))
+    rows, ds = [], []
+    mat = diamond_material()
+    for name in ["trichecker12", "forfun", "texasstar", "shah"]:
+        with open(os.path.join(here, "golden", "asc", name + ".asc"), "rb") as f:
+            planes = np.frombuffer(bytes(asc.from_asc_schedule(asc.parse_asc_bytes(f.read()))),
+                                   np.float32).reshape(-1, 4)
+        ds.append((sum(len(r) for r in rows), len(planes), mat.dispersion))
+        rows.append(planes)
+    table = np.concatenate(rows)
+    poses = [(0.0, 0.45), (0.9, 1.1), (2.6, 0.2)]
+    env = studio_rings()
+    print("pose vs oracle: 4 catalogue .asc designs", flush=True)
+    dev = metrics.pose_metrics(backend, table, ds, poses, env, True)
+    for d, (off, cnt, _) in enumerate(ds):
+        _check_against_oracle(dev[d], table[off:off + cnt], mat, env, poses, True)
+
+
 def test_pose_metrics_other_lighting(backend):
     env = tracer.LightingPreset.studio("IsoHemisphere", 1.0, 0.0, 0.0)
     rows = workloads.c2_planes()
