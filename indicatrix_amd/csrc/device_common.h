@@ -39,9 +39,43 @@ __device__ __forceinline__ float dot(V3 a, V3 b) { return (a.x * b.x + a.y * b.y
 __device__ __forceinline__ V3 cross(V3 a, V3 b) {
     return v3(a.y * b.z - b.y * a.z, a.z * b.x - b.z * a.x, a.x * b.y - b.x * a.y);
 }
-__device__ __forceinline__ V3 normalize(V3 a) { return a * (1.0f / sqrtf(dot(a, a))); }
+// The correctly rounded f32 quotient a / b for operands whose quotient and
+// residuals stay clear of the overflow and underflow ranges (indices of
+// refraction, cosines, Fresnel terms, unit-vector lengths, path lengths): the
+// reciprocal, one Newton step and two residual corrections of the device's
+// IEEE division, without the operand pre-scaling and special-case fix-up that
+// such operands never need, so the bits are the same.  b == 0 gives NaN (IEEE:
+// +-inf), and a numerator of -0 gives +0; callers pass neither where it
+// matters.
+__device__ __forceinline__ float div_rn(float a, float b) {
+    float y = __builtin_amdgcn_rcpf(b);
+    const float e = __builtin_fmaf(-b, y, 1.0f);
+    y = __builtin_fmaf(e, y, y);
+    float q = a * y;
+    float r = __builtin_fmaf(-b, q, a);
+    q = __builtin_fmaf(r, y, q);
+    r = __builtin_fmaf(-b, q, a);
+    return __builtin_fmaf(r, y, q);
+}
+// The correctly rounded f32 square root for an operand that is zero or a
+// normal number: the hardware root (within one ULP) moved to the neighbour
+// the residual's sign selects, as the device's IEEE square root does after
+// its denormal-range scaling, which these operands never need.  Zero, +inf
+// and NaN pass through unchanged.
+__device__ __forceinline__ float sqrt_rn(float x) {
+    const float s = __builtin_amdgcn_sqrtf(x);
+    const float lo = __int_as_float(__float_as_int(s) - 1);
+    const float hi = __int_as_float(__float_as_int(s) + 1);
+    const float r_lo = __builtin_fmaf(-lo, s, x);
+    const float r_hi = __builtin_fmaf(-hi, s, x);
+    const float t = r_lo <= 0.0f ? lo : s;
+    return r_hi > 0.0f ? hi : t;
+}
+__device__ __forceinline__ V3 normalize(V3 a) { return a * div_rn(1.0f, sqrt_rn(dot(a, a))); }
+// A zero vector gives the zero vector (1 / 0 is NaN here, not inf: both fail
+// the finiteness test).
 __device__ __forceinline__ V3 normalize_or_zero(V3 a) {
-    float r = 1.0f / sqrtf(dot(a, a));
+    float r = div_rn(1.0f, sqrt_rn(dot(a, a)));
     return (__builtin_isfinite(r) && r > 0.0f) ? a * r : v3(0.0f, 0.0f, 0.0f);
 }
 // Rust f32::clamp (NaN propagates)
@@ -77,11 +111,11 @@ __device__ __forceinline__ uint32_t hash_u32(uint32_t x) {
     return x;
 }
 __device__ __forceinline__ float unit_draw(uint32_t seed, uint32_t bounce, uint32_t stream) {
-    return (float)hash_u32(seed ^ hash_u32(bounce ^ stream)) / 4294967295.0f;
+    return div_rn((float)hash_u32(seed ^ hash_u32(bounce ^ stream)), 4294967295.0f);
 }
 __device__ __forceinline__ float radical_inverse(uint32_t n, uint32_t base) {
     float val = 0.0f;
-    float inv_base = 1.0f / (float)base;
+    float inv_base = div_rn(1.0f, (float)base);
     while (n > 0) {
         uint32_t digit = n % base;
         val = fma_((float)digit, inv_base, val);
@@ -107,16 +141,16 @@ __device__ __forceinline__ float dispersion(int kind, const float* c, float lamb
     float l2 = um * um;
     if (kind == IXG_DISP_SELLMEIER3) {
         float n2 = 1.0f;
-        n2 += (c[0] * l2) / (l2 - c[3]);
-        n2 += (c[1] * l2) / (l2 - c[4]);
-        n2 += (c[2] * l2) / (l2 - c[5]);
-        return sqrtf(fmaxf(n2, 1.0f));
+        n2 += div_rn(c[0] * l2, l2 - c[3]);
+        n2 += div_rn(c[1] * l2, l2 - c[4]);
+        n2 += div_rn(c[2] * l2, l2 - c[5]);
+        return sqrt_rn(fmaxf(n2, 1.0f));
     } else if (kind == IXG_DISP_SELLMEIER1) {
-        float n2 = 1.0f + (c[0] * l2) / (l2 - c[1]);
-        return sqrtf(fmaxf(n2, 1.0f));
+        float n2 = 1.0f + div_rn(c[0] * l2, l2 - c[1]);
+        return sqrt_rn(fmaxf(n2, 1.0f));
     }
     float l4 = l2 * l2;
-    return fmaxf((c[0] + (c[1] / l2)) + (c[2] / l4), 1.0f);
+    return fmaxf((c[0] + div_rn(c[1], l2)) + div_rn(c[2], l4), 1.0f);
 }
 
 __device__ __forceinline__ void cie_cmf(float lambda, float& x, float& y, float& z) {
@@ -125,7 +159,7 @@ __device__ __forceinline__ void cie_cmf(float lambda, float& x, float& y, float&
         x = y = z = 0.0f;
         return;
     }
-    float pos = (lambda - 380.0f) / 5.0f;
+    float pos = div_rn(lambda - 380.0f, 5.0f);
     uint32_t i0 = (uint32_t)floorf(pos);
     i0 = i0 > IX_CMF_ROWS - 2 ? IX_CMF_ROWS - 2 : i0;
     float t = pos - (float)i0;
@@ -136,12 +170,12 @@ __device__ __forceinline__ void cie_cmf(float lambda, float& x, float& y, float&
 
 __device__ __forceinline__ float d65(float lambda) {
     float c = clampf(lambda, 380.0f, fma_(10.0f, (float)(IX_D65_ROWS - 1), 380.0f));
-    float pos = (c - 380.0f) / 10.0f;
+    float pos = div_rn(c - 380.0f, 10.0f);
     uint32_t i0 = (uint32_t)floorf(pos);
     i0 = i0 > IX_D65_ROWS - 2 ? IX_D65_ROWS - 2 : i0;
     float frac = pos - (float)i0;
     float v0 = ix_d65_table[i0], v1 = ix_d65_table[i0 + 1];
-    return fma_(frac, v1 - v0, v0) / 100.0f;
+    return div_rn(fma_(frac, v1 - v0, v0), 100.0f);
 }
 
 // exp rounded once from double precision: the f32 nearest e^x, as glibc's
@@ -152,10 +186,10 @@ __device__ __forceinline__ float exp_rn(float x) { return (float)exp((double)x);
 
 __device__ __forceinline__ float spec_power(const EnvParams& E, float lambda) {
     if (E.d65) return d65(lambda);
-    float e = exp_rn(fminf(14388000.0f / (lambda * E.bb_t_k), 80.0f));
+    float e = exp_rn(fminf(div_rn(14388000.0f, lambda * E.bb_t_k), 80.0f));
     float denom = fmaxf(e - 1.0f, 1e-6f);
-    float ratio = E.bb_denom_560 / denom;
-    return clampf(powi<5>(560.0f / lambda) * ratio, 0.01f, 20.0f);
+    float ratio = div_rn(E.bb_denom_560, denom);
+    return clampf(powi<5>(div_rn(560.0f, lambda)) * ratio, 0.01f, 20.0f);
 }
 
 __device__ __forceinline__ float smoothstep(float e0, float e1, float x) {
@@ -256,23 +290,10 @@ __device__ __forceinline__ float lit_factor(const EnvParams& E, V3 dir, V3 obser
 // correctly rounded division per plane, t_near the largest entering t and
 // t_far the smallest exiting t, ties to the lowest index.
 //
-// The quotient is the correctly rounded a / b of IEEE division.  The scan's
-// operands are unit-normal dot products and offsets: |b| <= 1 and, for every
-// plane whose quotient is used, |b| > 1e-7, with |a| far from the overflow
-// and underflow ranges, so the device's division sequence needs neither its
-// operand pre-scaling nor its special-case fix-up; what remains (reciprocal,
-// one Newton step, two residual corrections) is the same arithmetic and
-// rounds identically.  A plane whose |b| <= 1e-7 gets an unused value.
-__device__ __forceinline__ float div_rn_scan(float a, float b) {
-    float y = __builtin_amdgcn_rcpf(b);
-    const float e = __builtin_fmaf(-b, y, 1.0f);
-    y = __builtin_fmaf(e, y, y);
-    float q = a * y;
-    float r = __builtin_fmaf(-b, q, a);
-    q = __builtin_fmaf(r, y, q);
-    r = __builtin_fmaf(-b, q, a);
-    return __builtin_fmaf(r, y, q);
-}
+// The quotient is div_rn: the scan's operands are unit-normal dot products
+// and offsets, |b| <= 1 and, for every plane whose quotient is used,
+// |b| > 1e-7, with |a| far from the overflow and underflow ranges.  A plane
+// whose |b| <= 1e-7 gets an unused value.
 
 // Written without branches: every plane's quotient is formed and the running
 // bests are updated by selects, and a parallel plane with the origin outside
@@ -287,7 +308,7 @@ __device__ __forceinline__ int intersect(const float4* __restrict__ pl, uint32_t
         const float4 p = pl[i];
         const float denom = (p.x * d.x + p.y * d.y) + p.z * d.z;
         const float side = p.w + ((p.x * o.x + p.y * o.y) + p.z * o.z);
-        const float t = div_rn_scan(-side, denom);
+        const float t = div_rn(-side, denom);
         const bool valid = fabsf(denom) > 1e-7f;
         const bool up_near = valid && denom < 0.0f && t > t_near;
         const bool up_far = valid && !(denom < 0.0f) && t < t_far;

@@ -68,7 +68,7 @@ __device__ __forceinline__ void trace_entry(const float4* __restrict__ pl, uint3
         F.reason = REASON_ENTRY;
         return;
     }
-    const float cos_t = sqrtf(1.0f - sin2_t);
+    const float cos_t = sqrt_rn(1.0f - sin2_t);
     const float inv = 1.0f / index;
     V3 cur = normalize(dir * inv + nrm * fma_(inv, cos_i, -cos_t));
     const float entry_T = fresnel_T(1.0f, index, cos_i, cos_t);
@@ -88,11 +88,11 @@ __device__ __forceinline__ void trace_entry(const float4* __restrict__ pl, uint3
         V3 n_out = v3(pf.x, pf.y, pf.z);
         float ct = clampf(dot(cur, n_out), 0.0f, 1.0f);
         float one_m = fma_(ct, -ct, 1.0f);
-        float st = sqrtf(fmaxf(one_m, 0.0f));
+        float st = sqrt_rn(fmaxf(one_m, 0.0f));
         if (st < sin_crit) {
             float s2o = idx2 * one_m;
             if (s2o <= 1.0f) {
-                float co = sqrtf(1.0f - s2o);
+                float co = sqrt_rn(1.0f - s2o);
                 V3 od = normalize(cur * index + n_out * fma_(index, -ct, co));
                 F.exit_facet = f;
                 F.reason = REASON_SIDE;
@@ -399,7 +399,7 @@ __global__ __launch_bounds__(64) void metric_reduce_kernel(const MetricShared S,
         float mean = sum / (float)count;
         if (mean > 1e-4f) {
             float var = fmaxf(fma_(mean, -mean, sum_sq / (float)count), 0.0f);
-            float cv = sqrtf(var) / mean;
+            float cv = sqrt_rn(var) / mean;
             spatial = clampf(cv / (1.0f + cv) * 100.0f, 0.0f, 100.0f);
         }
     }
@@ -546,7 +546,7 @@ __global__ __launch_bounds__(kPoseBlock) void pose_kernel(const MetricShared S, 
         const float mean = cf / (float)counted;
         if (mean > 1e-4f) {
             const float var = fmaxf(fma_(mean, -mean, cf2 / (float)counted), 0.0f);
-            const float cv = sqrtf(var) / mean;
+            const float cv = sqrt_rn(var) / mean;
             spatial = clampf(cv / (1.0f + cv) * 100.0f, 0.0f, 100.0f);
         }
         temporal = clampf(((tv / (float)counted) / 0.24f) * 100.0f, 0.0f, 100.0f);

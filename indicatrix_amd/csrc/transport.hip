@@ -49,16 +49,39 @@ __device__ __forceinline__ void fresnel_reflection_coeffs(float rs, float rp, fl
     c = rs * rp;
 }
 
-__device__ __forceinline__ float tir_phase_delta(float n1k, float cos_i, float sin_i) {
-    float th = (cos_i * sqrtf(fmaxf(fma_(n1k * n1k * sin_i, sin_i, -1.0f), 0.0f))) /
-               fmaxf(n1k * sin_i * sin_i, 1e-6f);
-    return 2.0f * atanf(th);
+// cos and sin of the TIR retardation delta = 2 atan(th) (tir.rs:24-40), as
+// the rational functions of th they equal: (1 - th^2) / (1 + th^2) and
+// 2 th / (1 + th^2).  th is bounded (n sin_i > 1 under TIR), and the values
+// agree with cos / sin of the rounded angle to a few 1e-8 (absolute).
+__device__ __forceinline__ void tir_retardation(float n1k, float cos_i, float sin_i, float& cd, float& sd) {
+    const float th = div_rn(cos_i * sqrt_rn(fmaxf(fma_(n1k * n1k * sin_i, sin_i, -1.0f), 0.0f)),
+                            fmaxf(n1k * sin_i * sin_i, 1e-6f));
+    const float t2 = th * th;
+    const float inv = div_rn(1.0f, 1.0f + t2);
+    cd = (1.0f - t2) * inv;
+    sd = (2.0f * th) * inv;
+}
+
+// cos 2 psi and sin 2 psi for psi = atan2(sin_psi, cos_psi) (the Stokes frame
+// rotation, absorption.rs:69-87), from the double-angle identities on the
+// unnormalised pair: (c^2 - s^2) / r^2 and 2 c s / r^2.  A zero pair is the
+// zero angle, as atan2(0, 0) is.
+__device__ __forceinline__ void rotation_double_angle(float cos_psi, float sin_psi, float& c2, float& s2) {
+    const float r2 = fma_(cos_psi, cos_psi, sin_psi * sin_psi);
+    if (r2 > 1e-30f) {
+        const float inv = div_rn(1.0f, r2);
+        c2 = fma_(cos_psi, cos_psi, -(sin_psi * sin_psi)) * inv;
+        s2 = ((2.0f * cos_psi) * sin_psi) * inv;
+    } else {
+        c2 = 1.0f;
+        s2 = 0.0f;
+    }
 }
 
 __device__ __forceinline__ void fresnel_rs_rp(float n1, float n2, float cos_i, float cos_t,
                                               float& rs, float& rp) {
-    rs = fma_(n2, -cos_t, n1 * cos_i) / fma_(n2, cos_t, n1 * cos_i);
-    rp = fma_(n1, -cos_t, n2 * cos_i) / fma_(n1, cos_t, n2 * cos_i);
+    rs = div_rn(fma_(n2, -cos_t, n1 * cos_i), fma_(n2, cos_t, n1 * cos_i));
+    rp = div_rn(fma_(n1, -cos_t, n2 * cos_i), fma_(n1, cos_t, n2 * cos_i));
 }
 
 __device__ __forceinline__ uint32_t fnv(uint32_t h, uint32_t v) { return (h ^ v) * 16777619u; }
@@ -83,9 +106,9 @@ __device__ __forceinline__ void camera_sample(const FilmParams& P, uint32_t pixe
     float jy = cp_rotate(radical_inverse(sample, 3), ry) - 0.5f;
     hero = cp_rotate(radical_inverse(sample, 5), rh);
     uint32_t px = pixel % P.width, py = pixel / P.width;
-    float aspect = P.width_f / P.height_f;
-    float u = (((float)px + jx) / P.width_f - 0.5f) * 2.0f * aspect * P.fov_tan;
-    float v = (0.5f - ((float)py + jy) / P.height_f) * 2.0f * P.fov_tan;
+    float aspect = div_rn(P.width_f, P.height_f);
+    float u = (div_rn((float)px + jx, P.width_f) - 0.5f) * 2.0f * aspect * P.fov_tan;
+    float v = (0.5f - div_rn((float)py + jy, P.height_f)) * 2.0f * P.fov_tan;
     dir = normalize((v3(P.forward[0], P.forward[1], P.forward[2]) +
                      v3(P.right[0], P.right[1], P.right[2]) * u) +
                     v3(P.up[0], P.up[1], P.up[2]) * v);
@@ -283,8 +306,8 @@ __device__ __forceinline__ int facet_aniso(const FilmParams& P, const float4* __
     if (S.has_prev && dot(cur_plane, cur_plane) > 1e-6f && dot(S.prev_plane, S.prev_plane) > 1e-6f) {
         const float cos_psi = clampf(dot(S.prev_plane, cur_plane), -1.0f, 1.0f);
         const float sin_psi = dot(cross(S.prev_plane, cur_plane), normalize_or_zero(S.k));
-        const float psi = atan2f(sin_psi, cos_psi);
-        const float c2 = cosf(2.0f * psi), s2 = sinf(2.0f * psi);
+        float c2, s2;
+        rotation_double_angle(cos_psi, sin_psi, c2, s2);
 #pragma unroll
         for (int c = 0; c < NCH; ++c) {
             const float4 s = S.st[c];
@@ -318,7 +341,7 @@ __device__ __forceinline__ int facet_aniso(const FilmParams& P, const float4* __
     // compute_bounce_refraction_geometry (geometry.rs:306-425), uniaxial
     const float cos_i = clampf(dot(neg(k), normal), 0.0f, 1.0f);
     const float one_m_c2 = fma_(cos_i, -cos_i, 1.0f);
-    const float sin_i = sqrtf(fmaxf(one_m_c2, 0.0f));
+    const float sin_i = sqrt_rn(fmaxf(one_m_c2, 0.0f));
     float theta_c;
     if (!inside) {
         // theta_c_for_bounce (geometry.rs:75-120): two fixed-point steps
@@ -330,7 +353,7 @@ __device__ __forceinline__ int facet_aniso(const FilmParams& P, const float4* __
             const float eg = 1.0f / n_guess;
             const float s2g = eg * eg * one_m_c2;
             if (s2g > 1.0f) break;
-            const float ctg = sqrtf(fmaxf(1.0f - s2g, 0.0f));
+            const float ctg = sqrt_rn(fmaxf(1.0f - s2g, 0.0f));
             const V3 wd = refract_dir(k, normal, eg, cos_i, ctg);
             theta_c = acosf(fabsf(clampf(dot(wd, cax), -1.0f, 1.0f)));
             n_guess = effective_extraordinary_index(n_o_seed, n_e_seed, theta_c);
@@ -349,7 +372,7 @@ __device__ __forceinline__ int facet_aniso(const FilmParams& P, const float4* __
     const float n_e_hero = n_o_hero + P.an.biref;
     const float n1 = inside ? n_med[0] : 1.0f;
     const float n2 = inside ? 1.0f : n_med[0];
-    const float eta = n1 / n2;
+    const float eta = div_rn(n1, n2);
     const float sin2_t = eta * eta * one_m_c2;
     const UFrame F = uframe_build(k, normal, cax, cos_i, sin_i);
     const bool nondegenerate = dot(cross(k, cax), cross(k, cax)) > 1e-6f;
@@ -381,13 +404,14 @@ __device__ __forceinline__ int facet_aniso(const FilmParams& P, const float4* __
 #pragma unroll
             for (int c = 0; c < NCH; ++c) {
                 const float n1k = n_med[c], n2k = 1.0f;
-                const float ek = n1k / n2k;
+                const float ek = div_rn(n1k, n2k);
                 const float s2k = ek * ek * one_m_c2;
                 if (s2k > 1.0f) {
-                    const float delta = tir_phase_delta(n1k, cos_i, sin_i);
-                    S.st[c] = mueller_tir(cosf(delta), sinf(delta), S.st[c]);
+                    float cd, sd;
+                    tir_retardation(n1k, cos_i, sin_i, cd, sd);
+                    S.st[c] = mueller_tir(cd, sd, S.st[c]);
                 } else {
-                    const float ct = sqrtf(fmaxf(1.0f - s2k, 0.0f));
+                    const float ct = sqrt_rn(fmaxf(1.0f - s2k, 0.0f));
                     float a_, b_, a, b, cc;
                     fresnel_rs_rp(n1k, n2k, cos_i, ct, a_, b_);
                     S.path_pdf[c] *= clampf(0.5f * fma_(b_, b_, a_ * a_), R_UNPOL_PDF_MIN, R_UNPOL_PDF_MAX);
@@ -415,9 +439,9 @@ __device__ __forceinline__ int facet_aniso(const FilmParams& P, const float4* __
         const bool use_extra = unit_draw(S.seed, bounce, BIREFRINGENT_SPLIT_STREAM) < (1.0f - p_o_hero);
         const float p_mode_frac = use_extra ? 1.0f - p_o_hero : p_o_hero;
         const float n2_dir = use_extra ? n2 : n_o_hero;
-        const float eta_dir = n1 / n2_dir;
+        const float eta_dir = div_rn(n1, n2_dir);
         const float s2d = fminf(eta_dir * eta_dir * one_m_c2, 1.0f);
-        const V3 refr = refract_dir(k, normal, eta_dir, cos_i, sqrtf(fmaxf(1.0f - s2d, 0.0f)));
+        const V3 refr = refract_dir(k, normal, eta_dir, cos_i, sqrt_rn(fmaxf(1.0f - s2d, 0.0f)));
         const V3 final_refr = use_extra ? extraordinary_poynting_dir(refr, cax, n_o_hero, n_e_hero) : refr;
         const V3 p_axis_t = cross(refr, F.s_axis);
         if (unit_draw(S.seed, bounce, FRESNEL_BRANCH_STREAM) < r_branch) {
@@ -425,7 +449,7 @@ __device__ __forceinline__ int facet_aniso(const FilmParams& P, const float4* __
                 ++S.n_refl;
                 S.fh = fnv(fnv(S.fh, (uint32_t)facet), 2u);
             }
-            const float inv_r = 1.0f / r_branch;
+            const float inv_r = div_rn(1.0f, r_branch);
 #pragma unroll
             for (int c = 0; c < NCH; ++c) {
                 EntrySol sol;
@@ -443,14 +467,14 @@ __device__ __forceinline__ int facet_aniso(const FilmParams& P, const float4* __
                 ++S.n_refr;
                 S.fh = fnv(fnv(S.fh, (uint32_t)facet), 3u);
             }
-            const float inv_t = 1.0f / (1.0f - r_branch);
+            const float inv_t = div_rn(1.0f, 1.0f - r_branch);
             V3 dirs[NCH];
             uint32_t live = 0u, match_mask = 0u;
 #pragma unroll
             for (int c = 0; c < NCH; ++c) {
                 const float n_o_k = S.n_o[c];
                 const float n2k = use_extra ? n_med[c] : n_o_k;
-                const float r = n1 / n2k;
+                const float r = div_rn(n1, n2k);
                 const float s2k = r * r * one_m_c2;
                 dirs[c] = v3(0.0f, 0.0f, 0.0f);
                 if (s2k > 1.0f) {
@@ -458,8 +482,8 @@ __device__ __forceinline__ int facet_aniso(const FilmParams& P, const float4* __
                     S.path_pdf[c] = 0.0f;
                     continue;
                 }
-                const float ek = n1 / n2k;
-                const V3 rk = refract_dir(k, normal, ek, cos_i, sqrtf(fmaxf(1.0f - s2k, 0.0f)));
+                const float ek = div_rn(n1, n2k);
+                const V3 rk = refract_dir(k, normal, ek, cos_i, sqrt_rn(fmaxf(1.0f - s2k, 0.0f)));
                 const V3 fk = use_extra ? extraordinary_poynting_dir(rk, cax, n_o_k, n_o_k + P.an.biref) : rk;
                 dirs[c] = fk;
                 live |= 1u << c;
@@ -533,8 +557,8 @@ __device__ __forceinline__ int facet_aniso(const FilmParams& P, const float4* __
                 S.fh = fnv(fnv(S.fh, (uint32_t)facet), 3u);
             }
             const float s2d = fminf(eta * eta * one_m_c2, 1.0f);
-            const V3 hero_refr = refract_dir(k, normal, eta, cos_i, sqrtf(fmaxf(1.0f - s2d, 0.0f)));
-            const float inv_t = 1.0f / (1.0f - r_branch);
+            const V3 hero_refr = refract_dir(k, normal, eta, cos_i, sqrt_rn(fmaxf(1.0f - s2d, 0.0f)));
+            const float inv_t = div_rn(1.0f, 1.0f - r_branch);
 #pragma unroll
             for (int c = 0; c < NCH; ++c) {
                 const float orig_i = S.st[c].x;
@@ -547,14 +571,14 @@ __device__ __forceinline__ int facet_aniso(const FilmParams& P, const float4* __
                     continue;
                 }
                 const float ek = n_inc / 1.0f;
-                const V3 rk = refract_dir(k, normal, ek, cos_i, sqrtf(fmaxf(1.0f - s2k, 0.0f)));
+                const V3 rk = refract_dir(k, normal, ek, cos_i, sqrt_rn(fmaxf(1.0f - s2k, 0.0f)));
                 const bool match = dot(rk, hero_refr) >= DIRECTION_MATCH_COS_TOL;
                 if (match) match_bits |= 1u << c;
                 const InternalSol sol = c == 0 ? sh : internal_solve(n_inc, S.n_o[c], n_e_ch[c], cax, F, !is_extra);
                 // compute_uniaxial_exit_transmission (uniaxial_internal.rs)
                 const float flux_inc = fmaxf(sol.flux_inc, 1e-12f);
-                const Cx ts = cscale(sol.t_s, sqrtf(sol.flux_ts / flux_inc));
-                const Cx tp = cscale(sol.t_p, sqrtf(sol.flux_tp / flux_inc));
+                const Cx ts = cscale(sol.t_s, sqrt_rn(sol.flux_ts / flux_inc));
+                const Cx tp = cscale(sol.t_p, sqrt_rn(sol.flux_tp / flux_inc));
                 const float iu = cnorm2(ts) + cnorm2(tp);
                 const float qu = cnorm2(ts) - cnorm2(tp);
                 const Cx cr = cmul(ts, cconj(tp));
@@ -594,31 +618,32 @@ __device__ __forceinline__ int facet_aniso(const FilmParams& P, const float4* __
         const float n2h = ord_sel ? n_o_hero : n2;
         float s2h;
         if (ord_sel) {
-            const float e2 = n1 / n2h;
+            const float e2 = div_rn(n1, n2h);
             s2h = e2 * e2 * one_m_c2;
         } else {
             s2h = sin2_t;
         }
         float rs_h, rp_h;
-        fresnel_rs_rp(n1, n2h, cos_i, sqrtf(fmaxf(1.0f - s2h, 0.0f)), rs_h, rp_h);
+        fresnel_rs_rp(n1, n2h, cos_i, sqrt_rn(fmaxf(1.0f - s2h, 0.0f)), rs_h, rp_h);
         const float r_unpol = clampf(0.5f * fma_(rp_h, rp_h, rs_h * rs_h), R_UNPOL_SELECT_MIN, R_UNPOL_SELECT_MAX);
         if (unit_draw(S.seed, bounce, FRESNEL_BRANCH_STREAM) < r_unpol) {
             if (EVENTS) {
                 ++S.n_refl;
                 S.fh = fnv(fnv(S.fh, (uint32_t)facet), 2u);
             }
-            const float sc = 1.0f / r_unpol;
+            const float sc = div_rn(1.0f, r_unpol);
 #pragma unroll
             for (int c = 0; c < NCH; ++c) {
                 const float n1k = inside ? n_med[c] : 1.0f, n2k = inside ? 1.0f : n_med[c];
-                const float ek = n1k / n2k;
+                const float ek = div_rn(n1k, n2k);
                 const float s2k = ek * ek * one_m_c2;
                 float4 st;
                 if (s2k > 1.0f) {
-                    const float delta = tir_phase_delta(n1k, cos_i, sin_i);
-                    st = mueller_tir(cosf(delta), sinf(delta), S.st[c]);
+                    float cd, sd;
+                    tir_retardation(n1k, cos_i, sin_i, cd, sd);
+                    st = mueller_tir(cd, sd, S.st[c]);
                 } else {
-                    const float ct = sqrtf(fmaxf(1.0f - s2k, 0.0f));
+                    const float ct = sqrt_rn(fmaxf(1.0f - s2k, 0.0f));
                     float a_, b_, a, b, cc;
                     fresnel_rs_rp(n1k, n2k, cos_i, ct, a_, b_);
                     S.path_pdf[c] *= clampf(0.5f * fma_(b_, b_, a_ * a_), R_UNPOL_PDF_MIN, R_UNPOL_PDF_MAX);
@@ -635,12 +660,12 @@ __device__ __forceinline__ int facet_aniso(const FilmParams& P, const float4* __
                 S.fh = fnv(fnv(S.fh, (uint32_t)facet), 3u);
             }
             const float n2d = (entering && !use_extra) ? n_o_hero : n2;
-            const float eta_dir = n1 / n2d;
+            const float eta_dir = div_rn(n1, n2d);
             const float s2d = fminf(eta_dir * eta_dir * one_m_c2, 1.0f);
-            const V3 refr = refract_dir(k, normal, eta_dir, cos_i, sqrtf(fmaxf(1.0f - s2d, 0.0f)));
+            const V3 refr = refract_dir(k, normal, eta_dir, cos_i, sqrt_rn(fmaxf(1.0f - s2d, 0.0f)));
             const V3 final_refr =
                 (entering && use_extra) ? extraordinary_poynting_dir(refr, cax, n_o_hero, n_e_hero) : refr;
-            const float inv_t = 1.0f / (1.0f - r_unpol);
+            const float inv_t = div_rn(1.0f, 1.0f - r_unpol);
             V3 dirs[NCH];
             uint32_t live = 0u, match_mask = 0u;
 #pragma unroll
@@ -648,7 +673,7 @@ __device__ __forceinline__ int facet_aniso(const FilmParams& P, const float4* __
                 const float4 orig = S.st[c];
                 const float n1k = inside ? n_med[c] : 1.0f;
                 const float n2k = (entering && !use_extra) ? S.n_o[c] : (inside ? 1.0f : n_med[c]);
-                const float r = n1k / n2k;
+                const float r = div_rn(n1k, n2k);
                 const float s2k = r * r * one_m_c2;
                 dirs[c] = v3(0.0f, 0.0f, 0.0f);
                 if (s2k > 1.0f) {
@@ -656,17 +681,17 @@ __device__ __forceinline__ int facet_aniso(const FilmParams& P, const float4* __
                     S.path_pdf[c] = 0.0f;
                     continue;
                 }
-                const float ctk = sqrtf(fmaxf(1.0f - s2k, 0.0f));
-                const V3 rk = refract_dir(k, normal, n1k / n2k, cos_i, ctk);
+                const float ctk = sqrt_rn(fmaxf(1.0f - s2k, 0.0f));
+                const V3 rk = refract_dir(k, normal, div_rn(n1k, n2k), cos_i, ctk);
                 const V3 fk = (entering && use_extra)
                                   ? extraordinary_poynting_dir(rk, cax, S.n_o[c], S.n_o[c] + P.an.biref)
                                   : rk;
                 const bool match = dot(fk, final_refr) >= DIRECTION_MATCH_COS_TOL;
                 // compute_channel_transmission (exit_split.rs:69-101)
                 const float num = 2.0f * n1k * cos_i;
-                const float t_s = num / fma_(n2k, ctk, n1k * cos_i);
-                const float t_p = num / fma_(n1k, ctk, n2k * cos_i);
-                const float factor = (n2k * ctk) / fmaxf(n1k * cos_i, 1e-6f);
+                const float t_s = div_rn(num, fma_(n2k, ctk, n1k * cos_i));
+                const float t_p = div_rn(num, fma_(n1k, ctk, n2k * cos_i));
+                const float factor = div_rn(n2k * ctk, fmaxf(n1k * cos_i, 1e-6f));
                 const float ts2 = t_s * t_s * factor, tp2 = t_p * t_p * factor;
                 const float4 incident =
                     (entering && have_sel) ? make_float4(orig.x, orig.x * c2x, orig.x * s2x, 0.0f) : orig;
@@ -740,11 +765,11 @@ __device__ __forceinline__ int facet_aniso(const FilmParams& P, const float4* __
         for (int c = 0; c < NCH; ++c) mx = fmaxf(mx, fmaxf(S.st[c].x, 0.0f));
         const float q = clampf(mx, 0.05f, 1.0f);
         if (unit_draw(S.seed, bounce, RUSSIAN_ROULETTE_STREAM) > q) return STEP_ROULETTE;
-        const float inv_q = 1.0f / q;
+        const float inv_q = div_rn(1.0f, q);
 #pragma unroll
         for (int c = 0; c < NCH; ++c) S.st[c] = scale4(S.st[c], inv_q);
 #pragma unroll
-        for (int c = 0; c < NCH; ++c) S.split[c] /= q;
+        for (int c = 0; c < NCH; ++c) S.split[c] = div_rn(S.split[c], q);
     }
     return STEP_CONTINUE;
 }
@@ -795,8 +820,8 @@ __device__ __forceinline__ int path_bounce(const FilmParams& P, const float4* __
     if (S.has_prev && dot(cur_plane, cur_plane) > 1e-6f && dot(S.prev_plane, S.prev_plane) > 1e-6f) {
         float cos_psi = clampf(dot(S.prev_plane, cur_plane), -1.0f, 1.0f);
         float sin_psi = dot(cross(S.prev_plane, cur_plane), normalize_or_zero(k));
-        float psi = atan2f(sin_psi, cos_psi);
-        float c2 = cosf(2.0f * psi), s2 = sinf(2.0f * psi);
+        float c2, s2;
+        rotation_double_angle(cos_psi, sin_psi, c2, s2);
 #pragma unroll
         for (int c = 0; c < NCH; ++c) {
             float4 s = S.st[c];
@@ -815,16 +840,16 @@ __device__ __forceinline__ int path_bounce(const FilmParams& P, const float4* __
     // compute_bounce_refraction_geometry (geometry.rs:306-425), isotropic
     const float cos_i = clampf(dot(neg(k), normal), 0.0f, 1.0f);
     const float one_m_c2 = fma_(cos_i, -cos_i, 1.0f);
-    const float sin_i = sqrtf(fmaxf(one_m_c2, 0.0f));
+    const float sin_i = sqrt_rn(fmaxf(one_m_c2, 0.0f));
     const float n1 = inside ? S.n_o[0] : 1.0f;
     const float n2 = inside ? 1.0f : S.n_o[0];
-    const float eta = n1 / n2;
+    const float eta = div_rn(n1, n2);
     const float sin2_t = eta * eta * one_m_c2;
 
     // hero decision: forced TIR (tir.rs:56) or the reflect-vs-transmit draw
     // against the clamped unpolarised reflectance (dispatch.rs:230-257, 345-378)
     const bool hero_tir = sin2_t > 1.0f;
-    const float cos_t = sqrtf(fmaxf(1.0f - sin2_t, 0.0f));
+    const float cos_t = sqrt_rn(fmaxf(1.0f - sin2_t, 0.0f));
     float rs_h, rp_h;
     fresnel_rs_rp(n1, n2, cos_i, cos_t, rs_h, rp_h);
     const float r_unpol = clampf(0.5f * fma_(rp_h, rp_h, rs_h * rs_h), R_UNPOL_SELECT_MIN, R_UNPOL_SELECT_MAX);
@@ -843,19 +868,20 @@ __device__ __forceinline__ int path_bounce(const FilmParams& P, const float4* __
             else ++S.n_refl;
             S.fh = fnv(fnv(S.fh, (uint32_t)facet), hero_tir ? 1u : 2u);
         }
-        const float scale = hero_tir ? 1.0f : 1.0f / r_unpol;
+        const float scale = hero_tir ? 1.0f : div_rn(1.0f, r_unpol);
 #pragma unroll
         for (int c = 0; c < NCH; ++c) {
             float n1k = inside ? S.n_o[c] : 1.0f;
             float n2k = inside ? 1.0f : S.n_o[c];
-            float ek = n1k / n2k;
+            float ek = div_rn(n1k, n2k);
             float s2k = ek * ek * one_m_c2;
             float4 s;
             if (s2k > 1.0f) {
-                float delta = tir_phase_delta(n1k, cos_i, sin_i);
-                s = mueller_tir(cosf(delta), sinf(delta), S.st[c]);
+                float cd, sd;
+                tir_retardation(n1k, cos_i, sin_i, cd, sd);
+                s = mueller_tir(cd, sd, S.st[c]);
             } else {
-                float ct = sqrtf(fmaxf(1.0f - s2k, 0.0f));
+                float ct = sqrt_rn(fmaxf(1.0f - s2k, 0.0f));
                 float a_, b_, a, b, cc;
                 fresnel_rs_rp(n1k, n2k, cos_i, ct, a_, b_);
                 S.path_pdf[c] *= clampf(0.5f * fma_(b_, b_, a_ * a_), R_UNPOL_PDF_MIN, R_UNPOL_PDF_MAX);
@@ -872,9 +898,9 @@ __device__ __forceinline__ int path_bounce(const FilmParams& P, const float4* __
             S.fh = fnv(fnv(S.fh, (uint32_t)facet), 3u);
         }
         const float s2d = fminf(eta * eta * one_m_c2, 1.0f);
-        const float ct_dir = sqrtf(fmaxf(1.0f - s2d, 0.0f));
+        const float ct_dir = sqrt_rn(fmaxf(1.0f - s2d, 0.0f));
         const V3 refr = normalize(k * eta + normal * fma_(eta, cos_i, -ct_dir));
-        const float inv_t = 1.0f / (1.0f - r_unpol);
+        const float inv_t = div_rn(1.0f, 1.0f - r_unpol);
         V3 dirs[NCH];
         uint32_t live = 0u, match_mask = 0u;
 #pragma unroll
@@ -883,7 +909,7 @@ __device__ __forceinline__ int path_bounce(const FilmParams& P, const float4* __
             const float4 orig = S.st[c];
             float n1k = inside ? S.n_o[c] : 1.0f;
             float n2k = inside ? 1.0f : S.n_o[c];
-            float r = n1k / n2k;
+            float r = div_rn(n1k, n2k);
             float s2k = r * r * one_m_c2;
             if (s2k > 1.0f) {
                 S.st[c] = scale4(S.st[c], 0.0f);
@@ -891,14 +917,14 @@ __device__ __forceinline__ int path_bounce(const FilmParams& P, const float4* __
                 dirs[c] = v3(0.0f, 0.0f, 0.0f);
                 continue;
             }
-            float ctk = sqrtf(fmaxf(1.0f - s2k, 0.0f));
+            float ctk = sqrt_rn(fmaxf(1.0f - s2k, 0.0f));
             V3 dk = normalize(k * r + normal * fma_(r, cos_i, -ctk));
             bool match = dot(dk, refr) >= DIRECTION_MATCH_COS_TOL;
             // compute_channel_transmission (exit_split.rs:69-101)
             float num = 2.0f * n1k * cos_i;
-            float t_s = num / fma_(n2k, ctk, n1k * cos_i);
-            float t_p = num / fma_(n1k, ctk, n2k * cos_i);
-            float factor = (n2k * ctk) / fmaxf(n1k * cos_i, 1e-6f);
+            float t_s = div_rn(num, fma_(n2k, ctk, n1k * cos_i));
+            float t_p = div_rn(num, fma_(n1k, ctk, n2k * cos_i));
+            float factor = div_rn(n2k * ctk, fmaxf(n1k * cos_i, 1e-6f));
             float ts2 = t_s * t_s * factor, tp2 = t_p * t_p * factor;
             float4 tr = mueller_fresnel(midpoint(ts2, tp2), 0.5f * (ts2 - tp2), t_s * t_p * factor, orig);
             tr = scale4(tr, inv_t);
@@ -971,11 +997,11 @@ __device__ __forceinline__ int path_bounce(const FilmParams& P, const float4* __
         float q = clampf(mx, 0.05f, 1.0f);
         float rr = unit_draw(S.seed, bounce, RUSSIAN_ROULETTE_STREAM);
         if (rr > q) return STEP_ROULETTE;
-        float inv_q = 1.0f / q;
+        float inv_q = div_rn(1.0f, q);
 #pragma unroll
         for (int c = 0; c < NCH; ++c) S.st[c] = scale4(S.st[c], inv_q);
 #pragma unroll
-        for (int c = 0; c < NCH; ++c) S.split[c] /= q;
+        for (int c = 0; c < NCH; ++c) S.split[c] = div_rn(S.split[c], q);
     }
     return STEP_CONTINUE;
 }
@@ -996,7 +1022,7 @@ __device__ __forceinline__ void path_finish(const FilmParams& P, const PathState
 #pragma unroll
         for (int j = 0; j < NCH; ++j)
             if ((fam >> j) & 1u) family += S.path_pdf[j];
-        float w = family <= 1e-12f ? 1.0f : (8.0f * S.path_pdf[0]) / family;
+        float w = family <= 1e-12f ? 1.0f : div_rn(8.0f * S.path_pdf[0], family);
         float cx, cy, cz;
         cie_cmf(S.lambda[c], cx, cy, cz);
         float s = (radiance[c] * w) * norm_factor;

@@ -34,8 +34,8 @@ __device__ __forceinline__ Cx cdiv(Cx a, Cx o) {
 }
 // sqrt_forward_branch: Re >= 0, a pure negative real straight to i*sqrt(-re)
 __device__ __forceinline__ Cx csqrt_fwd(Cx z) {
-    if (z.im == 0.0f && z.re < 0.0f) return cx(0.0f, sqrtf(-z.re));
-    const float r = sqrtf(sqrtf(cnorm2(z)));
+    if (z.im == 0.0f && z.re < 0.0f) return cx(0.0f, sqrt_rn(-z.re));
+    const float r = sqrt_rn(sqrt_rn(cnorm2(z)));
     const float th = atan2f(z.im, z.re) * 0.5f;
     Cx o = cx(r * cosf(th), r * sinf(th));
     if (o.re < 0.0f) o = cx(-o.re, -o.im);
@@ -275,8 +275,8 @@ __device__ __forceinline__ InternalSol internal_solve(float n_mode_inc, float n_
     const MFields f_e = extraordinary_fields(n_o, n_e, c, F, k_tan, R.qe_minus);
     const float n2 = 1.0f;
     const float zeta = k_tan / n2;
-    const Cx cos_t = fabsf(zeta) <= 1.0f ? cx(sqrtf(fmaxf(fma_(zeta, -zeta, 1.0f), 0.0f)), 0.0f)
-                                         : cx(0.0f, sqrtf(fma_(zeta, zeta, -1.0f)));
+    const Cx cos_t = fabsf(zeta) <= 1.0f ? cx(sqrt_rn(fmaxf(fma_(zeta, -zeta, 1.0f), 0.0f)), 0.0f)
+                                         : cx(0.0f, sqrt_rn(fma_(zeta, zeta, -1.0f)));
     const CV3 k_iso = cv_wave(F.that, F.zhat, k_tan, cscale(cos_t, n2));
     const CV3 s_c = cvr(F.s_axis);
     const CV3 ep_t = cv_scale_real(cv_cross(k_iso, s_c), -1.0f / (n2 * n2));
@@ -382,13 +382,13 @@ __device__ __forceinline__ float effective_extraordinary_index(float n_o, float 
     const float nos = n_o * st;
     const float denom_sq = fma_(n_e * ct, n_e * ct, nos * nos);
     if (denom_sq <= 1e-8f) return n_o;
-    return (n_o * n_e) / sqrtf(denom_sq);
+    return (n_o * n_e) / sqrt_rn(denom_sq);
 }
 
 __device__ __forceinline__ V3 extraordinary_poynting_dir(V3 w, V3 c, float n_o, float n_e) {
     const float cos_theta = clampf(dot(w, c), -1.0f, 1.0f);
     const float ac = fabsf(cos_theta);
-    const float st = sqrtf(fmaxf((1.0f - ac) * (1.0f + ac), 0.0f));
+    const float st = sqrt_rn(fmaxf((1.0f - ac) * (1.0f + ac), 0.0f));
     if (fabsf(n_o - n_e) < 1e-5f) return w;
     const float n_o2 = n_o * n_o, n_e2 = n_e * n_e;
     const float tan_rho = ((n_o2 - n_e2) * st * ac) / fmaxf(fma_(n_e2 * ac, ac, n_o2 * st * st), 1e-6f);
