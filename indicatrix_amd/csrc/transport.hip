@@ -49,17 +49,18 @@ __device__ __forceinline__ void fresnel_reflection_coeffs(float rs, float rp, fl
     c = rs * rp;
 }
 
-// cos and sin of the TIR retardation delta = 2 atan(th) (tir.rs:24-40), as
-// the rational functions of th they equal: (1 - th^2) / (1 + th^2) and
-// 2 th / (1 + th^2).  th is bounded (n sin_i > 1 under TIR), and the values
-// agree with cos / sin of the rounded angle to a few 1e-8 (absolute).
+// cos and sin of the TIR retardation delta = 2 atan(a / b) (tir.rs:24-40),
+// as the rational functions they equal: (b^2 - a^2) / (b^2 + a^2) and
+// 2 a b / (b^2 + a^2), one reciprocal for both.  a / b is bounded (n sin_i > 1
+// under TIR), and the values agree with cos / sin of the rounded angle to a
+// few 1e-8 (absolute).
 __device__ __forceinline__ void tir_retardation(float n1k, float cos_i, float sin_i, float& cd, float& sd) {
-    const float th = div_rn(cos_i * sqrt_rn(fmaxf(fma_(n1k * n1k * sin_i, sin_i, -1.0f), 0.0f)),
-                            fmaxf(n1k * sin_i * sin_i, 1e-6f));
-    const float t2 = th * th;
-    const float inv = div_rn(1.0f, 1.0f + t2);
-    cd = (1.0f - t2) * inv;
-    sd = (2.0f * th) * inv;
+    const float a = cos_i * sqrt_rn(fmaxf(fma_(n1k * n1k * sin_i, sin_i, -1.0f), 0.0f));
+    const float b = fmaxf(n1k * sin_i * sin_i, 1e-6f);
+    const float a2 = a * a, b2 = b * b;
+    const float inv = div_rn(1.0f, b2 + a2);
+    cd = (b2 - a2) * inv;
+    sd = ((2.0f * a) * b) * inv;
 }
 
 // cos 2 psi and sin 2 psi for psi = atan2(sin_psi, cos_psi) (the Stokes frame
@@ -873,7 +874,8 @@ __device__ __forceinline__ int path_bounce(const FilmParams& P, const float4* __
         for (int c = 0; c < NCH; ++c) {
             float n1k = inside ? S.n_o[c] : 1.0f;
             float n2k = inside ? 1.0f : S.n_o[c];
-            float ek = div_rn(n1k, n2k);
+            float ek = S.n_o[c];  // n1k / n2k: n / 1 is n, only an entering ray divides
+            if (!inside) ek = div_rn(1.0f, ek);
             float s2k = ek * ek * one_m_c2;
             float4 s;
             if (s2k > 1.0f) {
@@ -909,7 +911,8 @@ __device__ __forceinline__ int path_bounce(const FilmParams& P, const float4* __
             const float4 orig = S.st[c];
             float n1k = inside ? S.n_o[c] : 1.0f;
             float n2k = inside ? 1.0f : S.n_o[c];
-            float r = div_rn(n1k, n2k);
+            float r = S.n_o[c];  // n1k / n2k, as above
+            if (!inside) r = div_rn(1.0f, r);
             float s2k = r * r * one_m_c2;
             if (s2k > 1.0f) {
                 S.st[c] = scale4(S.st[c], 0.0f);
@@ -920,16 +923,21 @@ __device__ __forceinline__ int path_bounce(const FilmParams& P, const float4* __
             float ctk = sqrt_rn(fmaxf(1.0f - s2k, 0.0f));
             V3 dk = normalize(k * r + normal * fma_(r, cos_i, -ctk));
             bool match = dot(dk, refr) >= DIRECTION_MATCH_COS_TOL;
-            // compute_channel_transmission (exit_split.rs:69-101)
+            // compute_channel_transmission (exit_split.rs:69-101).  t_s and
+            // r_s share the denominator n1 cos_i + n2 cos_t, t_p and r_p the
+            // other: one reciprocal each, the four ratios within an ULP of
+            // the quotients.
             float num = 2.0f * n1k * cos_i;
-            float t_s = div_rn(num, fma_(n2k, ctk, n1k * cos_i));
-            float t_p = div_rn(num, fma_(n1k, ctk, n2k * cos_i));
+            const float inv_s = div_rn(1.0f, fma_(n2k, ctk, n1k * cos_i));
+            const float inv_p = div_rn(1.0f, fma_(n1k, ctk, n2k * cos_i));
+            float t_s = num * inv_s;
+            float t_p = num * inv_p;
             float factor = div_rn(n2k * ctk, fmaxf(n1k * cos_i, 1e-6f));
             float ts2 = t_s * t_s * factor, tp2 = t_p * t_p * factor;
             float4 tr = mueller_fresnel(midpoint(ts2, tp2), 0.5f * (ts2 - tp2), t_s * t_p * factor, orig);
             tr = scale4(tr, inv_t);
-            float a_, b_;
-            fresnel_rs_rp(n1k, n2k, cos_i, ctk, a_, b_);
+            const float a_ = fma_(n2k, -ctk, n1k * cos_i) * inv_s;
+            const float b_ = fma_(n1k, -ctk, n2k * cos_i) * inv_p;
             float ruk = clampf(0.5f * fma_(b_, b_, a_ * a_), R_UNPOL_PDF_MIN, R_UNPOL_PDF_MAX);
             if (match) {
                 S.st[c] = tr;
