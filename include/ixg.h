@@ -37,7 +37,7 @@ typedef enum ixg_status {
 typedef enum ixg_route_reason {
     IXG_ROUTE_OK = 0,
     IXG_ROUTE_UNSUPPORTED_MATERIAL = 1,    /* biaxial, scattering, zoned, edge rounding */
-    IXG_ROUTE_UNSUPPORTED_ENVIRONMENT = 2, /* HDR maps, UV lamps, tent/dome/ASET models */
+    IXG_ROUTE_UNSUPPORTED_ENVIRONMENT = 2, /* HDR maps, UV lamps                          */
     IXG_ROUTE_DEVICE_LIMITS = 3,           /* too many planes for the LDS facet set     */
     IXG_ROUTE_NO_DEVICE = 4
 } ixg_route_reason;
@@ -111,11 +111,13 @@ typedef enum ixg_preset {
     IXG_PRESET_SHOP_LIGHTS = 3,
     IXG_PRESET_WINDOW_DAYLIGHT = 4,
     IXG_PRESET_DAYLIGHT_DOME = 5,
+    IXG_PRESET_DAYLIGHT_SUN = 6,
     IXG_PRESET_DAYLIGHT = 7,
     IXG_PRESET_INCANDESCENT = 8,
     IXG_PRESET_ILLUMINANT_A = 9,
     IXG_PRESET_RING_LIGHTS = 10,
-    IXG_PRESET_DARK_SPOTLIGHT = 11
+    IXG_PRESET_DARK_SPOTLIGHT = 11,
+    IXG_PRESET_ASET = 12
 } ixg_preset;
 
 /* EnvironmentSource::Studio (IX/optics/raytracer/environment/source.rs:20-45). */

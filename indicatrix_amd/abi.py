@@ -26,11 +26,13 @@ PRESETS = {
     "ShopLights": 3,
     "WindowDaylight": 4,
     "DaylightDome": 5,
+    "DaylightSun": 6,
     "Daylight": 7,
     "Incandescent": 8,
     "IlluminantA": 9,
     "RingLights": 10,
     "DarkSpotlight": 11,
+    "Aset": 12,
 }
 
 

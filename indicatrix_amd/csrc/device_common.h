@@ -197,10 +197,12 @@ __device__ __forceinline__ float smoothstep(float e0, float e1, float x) {
     return t * t * fma_(-2.0f, t, 3.0f);
 }
 
-// The lit models' geometric factor for a unit direction d (rig.rs:230-320):
-// the iso hemisphere, the light tent (walls, black cards, key softbox, spark)
-// and the daylight dome (sky gradient, sun aureole), each under the observer's
-// head shadow and with the lit backdrop tray replacing its own ground.
+// The lit models' geometric factor for a unit direction d (rig.rs:230-320,
+// 461-489): the iso hemisphere, the light tent (walls, black cards, key
+// softbox, spark) and the daylight dome (sky gradient, sun aureole), each
+// under the observer's head shadow and with the lit backdrop tray replacing
+// its own ground; DaylightSun is the dome plus the 0.27 degree sun disc
+// (environment/sun.rs), which neither the head shadow nor the tray touch.
 __device__ __forceinline__ float lit_factor_unit(const EnvParams& E, V3 d, V3 observer) {
     const float horizon = smoothstep(-0.05f, 0.05f, d.y);
     const float vis = 1.0f - smoothstep(E.shadow_outer, E.shadow_inner, dot(d, observer));
@@ -221,7 +223,7 @@ __device__ __forceinline__ float lit_factor_unit(const EnvParams& E, V3 d, V3 ob
                             (5.0f * E.spot_mult) * E.tent_spark;
         above = ((walls + key) + spark) * (horizon * vis);
         own_ground = E.tent_ground;
-    } else {  // daylight dome
+    } else {  // daylight dome (and the sky of DaylightSun)
         const float sun_dot = dot(d, v3(E.key[0], E.key[1], E.key[2]));
         const float sky = fma_(0.08f, 1.0f - fmaxf(d.y, 0.0f), 0.10f);
         const float glow = fmaxf(sun_dot, 0.0f);
@@ -230,9 +232,38 @@ __device__ __forceinline__ float lit_factor_unit(const EnvParams& E, V3 d, V3 ob
         const float aureole = (glow4 * glow4) * 0.30f;
         above = (sky + aureole) * (horizon * vis);
         own_ground = 0.04f;
+        if (E.model == ENV_DAYLIGHT_SUN) {
+            const float ground = E.tray > 0.0f ? ((1.0f - horizon) * vis) * E.tray : own_ground * (1.0f - horizon);
+            return (above + ground) + (sun_dot >= 0.9999889f ? E.sun : 0.0f);  // SUN_DISC_COS
+        }
     }
     const float ground = E.tray > 0.0f ? ((1.0f - horizon) * vis) * E.tray : own_ground * (1.0f - horizon);
     return above + ground;
+}
+
+// The ASET contrast view (rig.rs:146-176, 205-222, 439-459): green up to 45
+// degrees of exit elevation, red from 45 to 75, blue above, each lit by one
+// 20 nm FWHM band at its own gain; below the horizon a neutral D65 tray when
+// one is set.  The radiance depends on the wavelength itself, not on an
+// illuminant power, so it is its own per-channel evaluation.  d is a unit
+// direction; no head shadow.
+__device__ __forceinline__ float aset_band(float lambda, float centre) {
+    const float z = div_rn(lambda - centre, 8.493218f);
+    return exp_rn(-0.5f * z * z);
+}
+__device__ __forceinline__ float aset_radiance(const EnvParams& E, V3 d, float lambda, float exposure) {
+    const float horizon = smoothstep(-0.05f, 0.05f, d.y);
+    const float s45 = smoothstep(0.6871068f, 0.7271068f, d.y);
+    const float s75 = smoothstep(0.9459258f, 0.9859258f, d.y);
+    const float red = (s45 * (1.0f - s75)) * horizon;
+    const float green = (1.0f - s45) * horizon;
+    const float blue = s75 * horizon;
+    const float tray_ground = E.tray > 0.0f ? (1.0f - horizon) * E.tray : 0.0f;
+    const float r = (red * aset_band(lambda, 610.0f)) * 8.0f;
+    const float g = (green * aset_band(lambda, 540.0f)) * 5.0f;
+    const float b = (blue * aset_band(lambda, 460.0f)) * 16.0f;
+    const float zones = ((r + g) + b) * exposure;
+    return tray_ground > 0.0f ? fma_(tray_ground * d65(lambda), exposure, zones) : zones;
 }
 
 // Wavelength-independent lighting of one exit direction (rig.rs:379-405).

@@ -23,6 +23,8 @@ bool preset_info(int32_t preset, PresetInfo* o) {
     case IXG_PRESET_SHOP_LIGHTS: *o = {ENV_LIGHT_TENT, 1, 5000.0f, 1.0f, 0, {3.0f, 0.0f, 1.0f, 0.05f, 0.0f}}; return true;
     case IXG_PRESET_WINDOW_DAYLIGHT: *o = {ENV_LIGHT_TENT, 1, 5000.0f, 1.0f, 0, {0.4f, 0.0f, 0.0f, 0.03f, 0.0f}}; return true;
     case IXG_PRESET_DAYLIGHT_DOME: *o = {ENV_DAYLIGHT_DOME, 1, 6500.0f, 1.0f, 0}; return true;
+    case IXG_PRESET_DAYLIGHT_SUN: *o = {ENV_DAYLIGHT_SUN, 1, 6500.0f, 1.0f, 0}; return true;
+    case IXG_PRESET_ASET: *o = {ENV_ASET, 1, 6500.0f, 1.0f, 0}; return true;
     default: return false;
     }
 }
@@ -162,6 +164,11 @@ bool resolve_environment(const ixg_environment& env, EnvParams* E) {
     HV3 fill = hnormalize(h3(std::cos(fpitch) * std::sin(fyaw), std::sin(fpitch),
                              std::cos(fpitch) * std::cos(fyaw)));
     E->key[0] = key.x; E->key[1] = key.y; E->key[2] = key.z;
+    // sun_radiance_factor (environment/sun.rs): SUN_RADIANCE * horizon_blend(key)
+    if (pi.model == ENV_DAYLIGHT_SUN) {
+        float t = hclamp((key.y - -0.05f) / (0.05f - -0.05f), 0.0f, 1.0f);
+        E->sun = 40000.0f * (t * t * std::fma(-2.0f, t, 3.0f));
+    }
     E->fill[0] = fill.x; E->fill[1] = fill.y; E->fill[2] = fill.z;
     for (int i = 0; i < 16; ++i) {
         float a = std::fma((float)i, kPi * 2.0f / 16.0f, yaw);

@@ -118,6 +118,8 @@ __device__ __forceinline__ void trace_entry(const float4* __restrict__ pl, uint3
 }
 
 __device__ __forceinline__ float scoring_radiance(const MetricShared& S, V3 dir) {
+    // ASET reads its probe wavelength (spec550 holds 550) in place of a power
+    if (S.env.model == ENV_ASET) return aset_radiance(S.env, normalize(dir), S.spec550, 1.0f);
     if (S.env.model != ENV_STUDIO) return lit_factor(S.env, dir, v3(0.0f, 0.0f, 0.0f)) * (S.spec550 * 1.0f);
     DirLight L;
     direction_lighting(S.env, dir, v3(0.0f, 0.0f, 0.0f), L);

@@ -68,7 +68,8 @@ ixg_status resolve_shared(ixg_ctx* ctx, const ixg_environment& env, uint32_t wid
         return ixg_internal_fail(ctx, IXG_DECLINED, "environment has no device lighting model");
     PresetInfo pi;
     preset_info(env.preset, &pi);
-    S->spec550 = pi.d65 ? d65_host(550.0f) : blackbody(550.0f, pi.temp_k);
+    // studio_spec_power (rig.rs:579-585): the probe wavelength itself for ASET
+    S->spec550 = pi.model == ENV_ASET ? 550.0f : pi.d65 ? d65_host(550.0f) : blackbody(550.0f, pi.temp_k);
     S->head_cos = head_shadow_cone_cos(pi.model == ENV_STUDIO, env.head_shadow_deg);
     S->width = width;
     S->height = height;

@@ -184,6 +184,7 @@ __device__ __forceinline__ void path_init(const FilmParams& P, V3 org, V3 dir, u
 }
 
 __device__ __forceinline__ float env_radiance(const EnvParams& E, V3 dir, V3 observer, float lambda) {
+    if (E.model == ENV_ASET) return aset_radiance(E, normalize(dir), lambda, E.exposure);
     if (E.model != ENV_STUDIO)
         return lit_factor(E, dir, observer) * (spec_power(E, lambda) * E.exposure);
     DirLight L;
@@ -791,7 +792,14 @@ __device__ __forceinline__ int path_bounce(const FilmParams& P, const float4* __
     }
     if (facet < 0) {
         // handle_miss (bounce_body/miss.rs:33-54)
-        if (ISO || P.env.model != ENV_STUDIO) {
+        if (!ISO && P.env.model == ENV_ASET) {
+            const V3 d = normalize(S.dir);
+#pragma unroll
+            for (int c = 0; c < NCH; ++c) {
+                float env = aset_radiance(P.env, d, S.lambda[c], P.env.exposure);
+                radiance[c] = fma_(fmaxf(S.st[c].x, 0.0f) * 1.0f, env, 0.0f);
+            }
+        } else if (ISO || P.env.model != ENV_STUDIO) {
             const float f = lit_factor(P.env, S.dir, S.observer);
 #pragma unroll
             for (int c = 0; c < NCH; ++c) {
@@ -1363,7 +1371,8 @@ extern "C" hipError_t ixg_launch_film_persistent(const ixg::FilmParams* P, const
                                                  uint32_t* d_counter, uint32_t grid_blocks, hipStream_t stream) {
     // a fill kernel rather than a memset node, so the launch pair can be captured
     hipLaunchKernelGGL(ixg::zero_counter_kernel, dim3(1), dim3(1), 0, stream, d_counter);
-    const bool iso = P->env.model != ixg::ENV_STUDIO;  // the lit models: one scaled factor
+    // the lit models: one scaled factor per direction (ASET is per wavelength)
+    const bool iso = P->env.model != ixg::ENV_STUDIO && P->env.model != ixg::ENV_ASET;
 #define IXG_PI(A, N)                                                                                           \
     hipLaunchKernelGGL((ixg::film_persistent_kernel<true, A, N>), dim3(grid_blocks), dim3(ixg::kBlock), 0, stream, \
                        *P, d_planes, d_xyz, d_counter)

@@ -8,7 +8,15 @@
 
 namespace ixg {
 
-enum EnvModel : int32_t { ENV_STUDIO = 0, ENV_ISO_HEMISPHERE = 1, ENV_LIGHT_TENT = 2, ENV_DAYLIGHT_DOME = 3 };
+// LightingModel::gpu_id (environment/mod.rs:281-290)
+enum EnvModel : int32_t {
+    ENV_STUDIO = 0,
+    ENV_ISO_HEMISPHERE = 1,
+    ENV_LIGHT_TENT = 2,
+    ENV_DAYLIGHT_DOME = 3,
+    ENV_ASET = 4,
+    ENV_DAYLIGHT_SUN = 5
+};
 
 struct EnvParams {
     int32_t model;       // EnvModel
@@ -25,6 +33,7 @@ struct EnvParams {
     float wb[3];         // von Kries LMS scale (1,1,1 for D65 presets)
     float tray;          // lit backdrop radiance at exposure 1 (0 = the model's own ground)
     float tent_walls, tent_cards, tent_spark, tent_ground, tent_flat;  // TentParams
+    float sun;           // DaylightSun: the disc's radiance factor (0 below the horizon)
 };
 
 struct AbsBand {

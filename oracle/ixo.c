@@ -348,7 +348,7 @@ float ixo_band_evaluate(const ixg_band* b, float lambda_nm) { /* optics/absorpti
 /* ------------------------------------------------------------------ */
 /* Environment (RT/environment/)                                        */
 /* ------------------------------------------------------------------ */
-enum { MODEL_STUDIO = 0, MODEL_ISO = 1, MODEL_TENT = 2, MODEL_DOME = 3 };
+enum { MODEL_STUDIO = 0, MODEL_ISO = 1, MODEL_TENT = 2, MODEL_DOME = 3, MODEL_ASET = 4, MODEL_SUN = 5 };  /* gpu_id */
 typedef struct { float walls, cards, spark, ground, flat; } tent_t;  /* TentParams (mod.rs:185-215) */
 typedef struct {
     int model;
@@ -360,6 +360,7 @@ typedef struct {
     v3 wb;              /* von Kries scale (identity for D65 presets) */
     float tray;         /* lit backdrop radiance at exposure 1 (0 = as lit) */
     tent_t tent;
+    float sun;          /* sun_radiance_factor (environment/sun.rs) for DaylightSun */
 } env_t;
 
 static int preset_params(int32_t preset, int* model, int* d65, float* temp_k, float* spot_mult,
@@ -387,6 +388,8 @@ static int preset_params(int32_t preset, int* model, int* d65, float* temp_k, fl
         *tent = (tent_t){0.4f, 0.0f, 0.0f, 0.03f, 0.0f};
         return 1;
     case IXG_PRESET_DAYLIGHT_DOME: *model = MODEL_DOME; *d65 = 1; *temp_k = 6500.0f; *spot_mult = 1.0f; *white_balance = 0; return 1;
+    case IXG_PRESET_DAYLIGHT_SUN: *model = MODEL_SUN; *d65 = 1; *temp_k = 6500.0f; *spot_mult = 1.0f; *white_balance = 0; return 1;
+    case IXG_PRESET_ASET: *model = MODEL_ASET; *d65 = 1; *temp_k = 6500.0f; *spot_mult = 1.0f; *white_balance = 0; return 1;
     default: return 0;
     }
 }
@@ -423,6 +426,11 @@ static int env_build(const ixg_environment* src, env_t* e) {
     e->spot_mult = spot;
     head_shadow_cosines(src->head_shadow_deg, e->shadow);
     studio_rig_new(src->light_yaw, src->light_pitch, e);
+    e->sun = 0.0f;
+    if (e->model == MODEL_SUN) { /* SUN_RADIANCE * horizon_blend(key_dir), sun.rs */
+        float t = clampf((e->key_dir.y - -0.05f) / (0.05f - -0.05f), 0.0f, 1.0f);
+        e->sun = 40000.0f * (t * t * fmaf(-2.0f, t, 3.0f));
+    }
     if (wb) {
         float s[3];
         ixo_white_balance(temp_k, s);
@@ -446,6 +454,7 @@ typedef struct {
     int has_ground;
     float horizon, ground;
     float scaled;
+    float red, green, blue, tray_ground; /* ASET zone amplitudes (rig.rs:137-142) */
 } dir_light;
 
 /* lit_ground (rig.rs:230-236): the tray replaces the model's own ground. */
@@ -458,6 +467,16 @@ static dir_light direction_lighting(const env_t* e, v3 dir, v3 observer) {
     memset(&L, 0, sizeof L);
     v3 d = vnormalize(dir);
     L.model = e->model;
+    if (e->model == MODEL_ASET) { /* aset_radiance, rig.rs:439-459 */
+        const float horizon = smoothstep_f(-0.05f, 0.05f, d.y);
+        const float s45 = smoothstep_f(0.6871068f, 0.7271068f, d.y);
+        const float s75 = smoothstep_f(0.9459258f, 0.9859258f, d.y);
+        L.red = (s45 * (1.0f - s75)) * horizon;
+        L.green = (1.0f - s45) * horizon;
+        L.blue = s75 * horizon;
+        L.tray_ground = e->tray > 0.0f ? (1.0f - horizon) * e->tray : 0.0f;
+        return L;
+    }
     if (e->model != MODEL_STUDIO) {
         const float horizon = smoothstep_f(-0.05f, 0.05f, d.y);             /* rig.rs:98-100 */
         const float vis = 1.0f - smoothstep_f(e->shadow[0], e->shadow[1], vdot(d, observer));
@@ -486,6 +505,8 @@ static dir_light direction_lighting(const env_t* e, v3 dir, v3 observer) {
             const float aureole = (glow4 * glow4) * 0.30f;
             const float above = (sky + aureole) * (horizon * vis);
             L.scaled = above + lit_ground(horizon, vis, e->tray, 0.04f);
+            if (e->model == MODEL_SUN) /* daylight_sun_radiance, rig.rs:466-489: SUN_DISC_COS */
+                L.scaled = L.scaled + (sun_dot >= 0.9999889f ? e->sun : 0.0f);
         }
         return L;
     }
@@ -515,10 +536,25 @@ static dir_light direction_lighting(const env_t* e, v3 dir, v3 observer) {
     }
     return L;
 }
+/* studio_spec_power (rig.rs:579-585): the illuminant's power, or for ASET the
+ * wavelength itself, which its banded radiance reads. */
 static float spec_power(const env_t* e, float lambda) {
+    if (e->model == MODEL_ASET) return lambda;
     return e->d65 ? ixo_d65(lambda) : bb_with(e->bb, lambda);
 }
-static float light_radiance(const dir_light* L, float sp, float exposure) { /* rig.rs:178-200 */
+static float aset_band(float lambda, float centre) { /* rig.rs:173-176 */
+    float z = (lambda - centre) / 8.493218f;
+    return expf(-0.5f * z * z);
+}
+static float light_radiance(const dir_light* L, float sp, float exposure) { /* rig.rs:178-222 */
+    if (L->model == MODEL_ASET) {
+        const float lambda = sp;
+        const float r = (L->red * aset_band(lambda, 610.0f)) * 8.0f;
+        const float g = (L->green * aset_band(lambda, 540.0f)) * 5.0f;
+        const float b = (L->blue * aset_band(lambda, 460.0f)) * 16.0f;
+        const float zones = ((r + g) + b) * exposure;
+        return L->tray_ground > 0.0f ? fmaf(L->tray_ground * ixo_d65(lambda), exposure, zones) : zones;
+    }
     if (L->model != MODEL_STUDIO) return L->scaled * (sp * exposure);
     float r = L->bg * sp;
     if (L->has_softbox) r = fmaf(L->softbox, sp, r);

@@ -72,14 +72,14 @@ def test_route_predicate():
     assert tracer.route(sc2) == abi.ROUTE_OK                  # uniaxial
     sc2.material_abi.biaxial = 1
     assert tracer.route(sc2) == abi.ROUTE_UNSUPPORTED_MATERIAL
-    for name in ("LightTent", "WhiteTray", "ShopLights", "WindowDaylight", "DaylightDome"):
+    for name in ("LightTent", "WhiteTray", "ShopLights", "WindowDaylight", "DaylightDome", "DaylightSun", "Aset"):
         env = tracer.LightingPreset.studio(name)
         sc3 = tracer.FrameScene(sc.scene.camera, 8, 8, sc.planes, tracer.GemMaterial.diamond(), 16, env)
         assert tracer.route(sc3) == abi.ROUTE_OK, name           # the lit models
     env.backdrop = 0.23                                              # a lit backdrop tray
     sc3 = tracer.FrameScene(sc.scene.camera, 8, 8, sc.planes, tracer.GemMaterial.diamond(), 16, env)
     assert tracer.route(sc3) == abi.ROUTE_OK
-    for preset in (6, 12, 13, 14):      # DaylightSun (sun NEE), ASET, the UV lamps: CPU only
+    for preset in (13, 14):             # the UV lamps (scene_routes_to_gpu declines them too)
         env = abi.Environment(preset, 1.0, 0.0, 0.5, 0.0, 1.0, 16.0)
         sc3 = tracer.FrameScene(sc.scene.camera, 8, 8, sc.planes, tracer.GemMaterial.diamond(), 16, env)
         assert tracer.route(sc3) == abi.ROUTE_UNSUPPORTED_ENVIRONMENT, preset

@@ -42,7 +42,9 @@ def _compare(backend, sc, spp):
 @pytest.mark.parametrize("preset,backdrop", [("LightTent", 0.23), ("LightTent", 0.0), ("WhiteTray", 0.0),
                                              ("ShopLights", 0.0), ("WindowDaylight", 0.1),
                                              ("DaylightDome", 0.0), ("DaylightDome", 0.23),
-                                             ("IsoHemisphere", 0.23), ("RingLights", 0.23)])
+                                             ("IsoHemisphere", 0.23), ("RingLights", 0.23),
+                                             ("DaylightSun", 0.0), ("DaylightSun", 0.23),
+                                             ("Aset", 0.0), ("Aset", 0.23)])
 def test_lit_models_and_tray_match_oracle(backend, preset, backdrop):
     sc = pin_scene(preset, backdrop)
     print(f"env {preset} backdrop={backdrop} 24x16x8", flush=True)
@@ -71,7 +73,7 @@ def test_light_tent_pin_frame(backend):
     assert np.abs(g.astype(np.float64) - o).max() <= 1e-5 * np.abs(o).max()
 
 
-@pytest.mark.parametrize("preset", ["LightTent", "DaylightDome", "ShopLights"])
+@pytest.mark.parametrize("preset", ["LightTent", "DaylightDome", "ShopLights", "DaylightSun", "Aset"])
 def test_pose_and_image_metrics_under_lit_models(backend, preset):
     """The metrics score the lit models with the uniform rule (half the median
     radiance) and ignore the tray (lighting.rs:148-190)."""
@@ -93,3 +95,43 @@ def test_pose_and_image_metrics_under_lit_models(backend, preset):
     o_px, o_tot, _ = oracle_lib.metric_image(r)
     assert np.array_equal(g_px["classes"], o_px["classes"])
     assert g_tot.brilliance_pct == o_tot.brilliance_pct
+
+
+@pytest.mark.parametrize("preset", ["DaylightSun", "Aset"])
+def test_direct_environment_rays_match_oracle(backend, preset):
+    """Rays that miss the stone take the environment lookup alone: aimed into
+    and just outside the 0.27 degree sun disc, and across the ASET elevation
+    zones (green, red, blue) and below the horizon."""
+    import math
+    import ctypes as C
+    yaw, pitch = 0.4, 0.35
+    sc = pin_scene(preset, 0.0, light=(yaw, pitch))
+    cp, sp, cy, sy = math.cos(pitch), math.sin(pitch), math.cos(yaw), math.sin(yaw)
+    key = np.array([cp * sy, sp, cp * cy], np.float64)
+    t = np.cross(key, [0.0, 1.0, 0.0])
+    t /= np.linalg.norm(t)
+    dirs = []
+    for deg in [0.0, 0.05, 0.1, 0.2, 0.3, 1.0]:
+        r = math.radians(deg)
+        dirs.append(key * math.cos(r) + t * math.sin(r))
+    for elev in [10.0, 30.0, 50.0, 70.0, 80.0, 88.0, -20.0]:
+        e = math.radians(elev)
+        dirs.append([math.cos(e) * 0.6, math.sin(e), math.cos(e) * 0.8])
+    dirs = np.asarray(dirs, np.float32)
+    origins = np.tile(np.array([30.0, 30.0, 30.0], np.float32), (len(dirs), 1))   # every ray misses the stone
+    seeds = np.arange(len(dirs), dtype=np.uint32) * 7919 + 11
+    heros = np.linspace(0.05, 0.95, len(dirs)).astype(np.float32)
+    print(f"direct env rays {preset}", flush=True)
+    g, g_ev = backend.trace_rays(sc, origins, dirs, seeds, heros)
+    o = np.zeros_like(g)
+    for i in range(len(dirs)):
+        out = (C.c_float * 3)()
+        oracle_lib.lib().ixo_trace_ray(C.byref(sc.scene), (C.c_float * 3)(*origins[i]), (C.c_float * 3)(*dirs[i]),
+                                       int(seeds[i]), float(heros[i]), out, None)
+        o[i] = out[:]
+    assert np.all(g_ev["n_refract"] == 0) and np.all(g_ev["n_tir"] == 0)
+    rel = np.abs(g.astype(np.float64) - o) / np.maximum(np.abs(o).max(axis=1, keepdims=True), 1e-12)
+    print(f"  max rel {rel.max():.2e}", flush=True)
+    assert rel.max() < 1e-5
+    if preset == "DaylightSun":   # the disc rays carry the sun, the ones outside do not
+        assert g[0].sum() > 1000 * g[5].sum()
