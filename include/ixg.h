@@ -36,7 +36,7 @@ typedef enum ixg_status {
  * IX/renderer/gpu_backend/mod.rs:287-298). */
 typedef enum ixg_route_reason {
     IXG_ROUTE_OK = 0,
-    IXG_ROUTE_UNSUPPORTED_MATERIAL = 1,    /* biaxial, scattering, zoned, edge rounding */
+    IXG_ROUTE_UNSUPPORTED_MATERIAL = 1,    /* biaxial, scattering, zoned                */
     IXG_ROUTE_UNSUPPORTED_ENVIRONMENT = 2, /* HDR maps, UV lamps                          */
     IXG_ROUTE_DEVICE_LIMITS = 3,           /* too many planes for the LDS facet set     */
     IXG_ROUTE_NO_DEVICE = 4
@@ -97,7 +97,7 @@ typedef struct ixg_material {
     ixg_band bands_o[IXG_MAX_BANDS];
     ixg_band bands_e[IXG_MAX_BANDS];
     float absorption_path_scale;
-    float edge_rounding_radius;           /* GPU path: must be 0                   */
+    float edge_rounding_radius;           /* > 0: rounded meet edges (shading)     */
     float scattering_sigma_s;             /* GPU path: must be 0                   */
     int32_t biaxial;                      /* GPU path: must be 0                   */
 } ixg_material;

@@ -100,7 +100,7 @@ bool material_routable(const ixg_material& m) {
     // is_anisotropic, transport/inner.rs:181-198), polished, unzoned,
     // non-scattering materials, with or without absorption bands; biaxial
     // materials decline.
-    if (m.biaxial || m.scattering_sigma_s > 0.0f || m.edge_rounding_radius > 0.0f) return false;
+    if (m.biaxial || m.scattering_sigma_s > 0.0f || !(m.edge_rounding_radius < 1e30f)) return false;
     if (m.has_e_dispersion && (m.e_dispersion.kind < IXG_DISP_SELLMEIER1 || m.e_dispersion.kind > IXG_DISP_CAUCHY))
         return false;
     if (m.n_bands_o < 0 || m.n_bands_o > IXG_MAX_BANDS || m.n_bands_e < 0 || m.n_bands_e > IXG_MAX_BANDS)
@@ -176,6 +176,7 @@ ixg_status build_params(ixg_ctx* ctx, const ixg_scene* s, uint32_t sample_offset
     std::memcpy(P->an.e_disp, m.e_dispersion.coeff, sizeof P->an.e_disp);
     std::memcpy(P->an.c_axis, m.c_axis, sizeof P->an.c_axis);
     std::memcpy(P->disp, s->material->dispersion.coeff, sizeof P->disp);
+    P->edge_radius = s->material->edge_rounding_radius > 0.0f ? s->material->edge_rounding_radius : 0.0f;
     float g = s->environment.surface_glare;
     P->surface_glare = (g >= 1.0f || std::isnan(g)) ? 1.0f : (g > 0.0f ? g : 0.0f);
     if (!ixg::resolve_environment(s->environment, &P->env))

@@ -87,6 +87,31 @@ __device__ __forceinline__ void fresnel_rs_rp(float n1, float n2, float cos_i, f
 
 __device__ __forceinline__ uint32_t fnv(uint32_t h, uint32_t v) { return (h ^ v) * 16777619u; }
 
+// shading_normal_near_edge (RT/intersect.rs:196-233): within radius of the
+// nearest other plane the facet normal eases (smoothstep) toward the bisector
+// of the two normals.  Called only for radius > 0 (a launch-uniform branch),
+// so sharp stones run none of it.
+__device__ __noinline__ V3 shading_normal_near_edge(const float4* __restrict__ planes, uint32_t n, V3 hit,
+                                                    int facet, V3 hn, float radius) {
+    float nearest = __builtin_inff();
+    V3 nn = hn;
+    for (uint32_t i = 0; i < n; ++i) {
+        if ((int)i == facet) continue;
+        const float4 p = planes[i];
+        const V3 pn = v3(p.x, p.y, p.z);
+        const float dist = -(p.w + dot(pn, hit));
+        if (dist < nearest) {
+            nearest = dist;
+            nn = pn;
+        }
+    }
+    if (nearest >= radius) return hn;
+    const float t = clampf(1.0f - div_rn(nearest, radius), 0.0f, 1.0f);
+    const float st = t * t * fma_(2.0f, -t, 3.0f);
+    const V3 bis = normalize_or_zero(hn + nn);
+    return normalize_or_zero(hn * (1.0f - st) + bis * st);
+}
+
 // ---------------------------------------------------------------------------
 // one path, as init / bounce / finish steps
 // ---------------------------------------------------------------------------
@@ -304,6 +329,7 @@ __device__ __forceinline__ int facet_aniso(const FilmParams& P, const float4* __
     const V3 hit = S.org + S.dir * t_hit;
     const float4 pf = planes[facet];
     V3 normal = v3(pf.x, pf.y, pf.z);
+    if (P.edge_radius > 0.0f) normal = shading_normal_near_edge(planes, P.n_planes, hit, facet, normal, P.edge_radius);
     const V3 cur_plane = normalize_or_zero(cross(S.k, normal));
     if (S.has_prev && dot(cur_plane, cur_plane) > 1e-6f && dot(S.prev_plane, S.prev_plane) > 1e-6f) {
         const float cos_psi = clampf(dot(S.prev_plane, cur_plane), -1.0f, 1.0f);
@@ -824,6 +850,8 @@ __device__ __forceinline__ int path_bounce(const FilmParams& P, const float4* __
     const V3 hit = S.org + k * t_hit;
     const float4 pf = planes[facet];
     V3 normal = v3(pf.x, pf.y, pf.z);
+    // bounce_body.rs:620-629: the edge-rounded shading normal replaces the facet's
+    if (P.edge_radius > 0.0f) normal = shading_normal_near_edge(planes, P.n_planes, hit, facet, normal, P.edge_radius);
     // rotate_stokes_to_plane_of_incidence (absorption.rs:69-87)
     const V3 cur_plane = normalize_or_zero(cross(k, normal));
     if (S.has_prev && dot(cur_plane, cur_plane) > 1e-6f && dot(S.prev_plane, S.prev_plane) > 1e-6f) {

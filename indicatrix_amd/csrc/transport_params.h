@@ -82,6 +82,7 @@ struct FilmParams {
     int32_t disp_kind;
     float disp[6];
     float surface_glare;  // clamped to [0,1]
+    float edge_radius;    // GemMaterial::edge_rounding_radius (<= 0: sharp edges)
     uint64_t first_path;  // first (pixel*spp + sample) index of this launch
     uint64_t n_paths;     // paths in this launch
     int32_t absorb;       // 1: the material has absorption bands

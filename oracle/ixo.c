@@ -828,6 +828,33 @@ static void fresnel_rs_rp(float n1, float n2, float cos_i, float cos_t, float* r
 static void facet_event_aniso(const scene_t* S, path_t* P, uint32_t bounce, uint32_t seed, float t_hit,
                               int32_t facet, ixg_path_events* ev, int* stop);
 
+/* shading_normal_near_edge (RT/intersect.rs:196-233): within rounding_radius
+ * of the nearest other plane the facet normal eases (smoothstep) toward the
+ * bisector of the two normals; radius <= 0 returns it unchanged. */
+static v3 shading_normal_near_edge(const ixg_facet_plane* planes, uint32_t n, v3 hit, int32_t facet, v3 hn,
+                                   float radius) {
+    if (radius <= 0.0f) return hn;
+    float nearest = INFINITY;
+    v3 nn = hn;
+    for (uint32_t i = 0; i < n; ++i) {
+        if ((int32_t)i == facet) continue;
+        const v3 pn = vfrom(planes[i].normal);
+        const float dist = -(planes[i].d + vdot(pn, hit));
+        if (dist < nearest) { nearest = dist; nn = pn; }
+    }
+    if (nearest >= radius) return hn;
+    const float t = clampf(1.0f - nearest / radius, 0.0f, 1.0f);
+    const float st = t * t * fmaf(2.0f, -t, 3.0f);
+    const v3 bis = vnormalize_or_zero(vadd(hn, nn));
+    return vnormalize_or_zero(vadd(vmul(hn, 1.0f - st), vmul(bis, st)));
+}
+
+void ixo_shading_normal_near_edge(const ixg_facet_plane* planes, uint32_t n, const float hit[3], int32_t facet,
+                                  const float hn[3], float radius, float out[3]) {
+    const v3 r = shading_normal_near_edge(planes, n, vfrom(hit), facet, vfrom(hn), radius);
+    out[0] = r.x; out[1] = r.y; out[2] = r.z;
+}
+
 static void facet_event(const scene_t* S, path_t* P, uint32_t bounce, uint32_t seed, float t_hit,
                         int32_t facet, ixg_path_events* ev, int* stop) {
     if (S->aniso) {
@@ -835,7 +862,8 @@ static void facet_event(const scene_t* S, path_t* P, uint32_t bounce, uint32_t s
         return;
     }
     v3 hit_point = vadd(P->org, vmul(P->dir, t_hit));
-    v3 normal = vfrom(S->planes[facet].normal);
+    v3 normal = shading_normal_near_edge(S->planes, S->n_planes, hit_point, facet, vfrom(S->planes[facet].normal),
+                                         S->mat->edge_rounding_radius); /* bounce_body.rs:620-629 */
     /* rotate_stokes_to_plane_of_incidence (RT/absorption.rs:69-87) */
     v3 cur_plane = vnormalize_or_zero(vcross(P->k, normal));
     if (P->has_prev && vlen_sq(cur_plane) > 1e-6f && vlen_sq(P->prev_plane) > 1e-6f) {
@@ -1152,7 +1180,8 @@ static void facet_event_aniso(const scene_t* S, path_t* P, uint32_t bounce, uint
     const ixg_material* M = S->mat;
     const v3 cax = S->c_raw;
     v3 hit_point = vadd(P->org, vmul(P->dir, t_hit));
-    v3 normal = vfrom(S->planes[facet].normal);
+    v3 normal = shading_normal_near_edge(S->planes, S->n_planes, hit_point, facet, vfrom(S->planes[facet].normal),
+                                         M->edge_rounding_radius);
     v3 cur_plane = vnormalize_or_zero(vcross(P->k, normal));
     if (P->has_prev && vlen_sq(cur_plane) > 1e-6f && vlen_sq(P->prev_plane) > 1e-6f) {
         float cos_psi = clampf(vdot(P->prev_plane, cur_plane), -1.0f, 1.0f);

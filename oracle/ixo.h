@@ -41,6 +41,10 @@ void ixo_uniaxial_entry_r(float n_o, float n_e, const float c[3], const float k[
 float ixo_sample_studio_environment(const float dir[3], float lambda_nm, int32_t preset,
                                     float exposure, float light_yaw, float light_pitch);
 
+/* shading_normal_near_edge (RT/intersect.rs:196): the edge-rounded shading normal. */
+void ixo_shading_normal_near_edge(const ixg_facet_plane* planes, uint32_t n, const float hit[3], int32_t facet,
+                                  const float hn[3], float radius, float out[3]);
+
 /* Camera (RT/camera.rs). */
 void ixo_camera_new(float yaw, float pitch, float distance, float fov_deg, uint32_t width,
                     uint32_t height, uint32_t spp, ixg_camera* out);

@@ -253,3 +253,28 @@ def test_statistical_image_equivalence(backend, scene):
     o_xyz, _ = oracle_lib.trace_paths(sc, spp, spp)
     ok, res = zstats.image_equivalent(o_xyz, g_xyz, w, h, spp)
     assert ok, res
+
+
+@pytest.mark.parametrize("radius,aniso", [(0.02, False), (0.05, False), (0.03, True)])
+def test_edge_rounded_paths_match_oracle(backend, radius, aniso):
+    """Edge-rounded shading normals (RT/intersect.rs:196-233): the same
+    integer side channel as the oracle path for path, XYZ within the
+    transcendental budget; a sharp stone (radius 0) is the unrounded film."""
+    if aniso:
+        sc = workloads.film_c2_scene(32, 24, 8, max_bounces=16, material=workloads.c2_material(bands=True))
+    else:
+        sc = small_scene()
+    sc.material_abi.edge_rounding_radius = radius
+    print(f"edge rounding r={radius} aniso={aniso} 32x24x8", flush=True)
+    g_xyz, g_ev = backend.trace_paths(sc, 0, 8)
+    o_xyz, o_ev = oracle_lib.trace_paths(sc, 0, 8)
+    same = _side_channel_agreement(g_ev, o_ev)
+    print(f"  flip {1.0 - same.mean():.2e}", flush=True)
+    assert 1.0 - same.mean() < 2e-3
+    a, b = g_xyz[same].astype(np.float64), o_xyz[same].astype(np.float64)
+    scale = np.maximum(np.abs(b).max(axis=1, keepdims=True), 1e-6)
+    rel = np.abs(a - b) / scale
+    assert np.quantile(rel, 0.999) < 1e-3
+    sc.material_abi.edge_rounding_radius = 0.0
+    sharp, _ = backend.trace_paths(sc, 0, 8)
+    assert not np.array_equal(sharp, g_xyz)     # the rounding does change the image
