@@ -1136,11 +1136,18 @@ __device__ __forceinline__ void trace_path(const FilmParams& P, const float4* __
 constexpr int kBlock = 256;
 
 // Occupancy target of the persistent film kernel (waves per SIMD); the
-// register budget follows from it (512 VGPRs / waves).
+// register budget follows from it (512 VGPRs / waves).  The uniaxial
+// instantiations run at 2 (256 registers): measured on the C2 oval, 2 waves
+// beat 3 by 2 % and 1, 4 and 5 lose 20-35 % (fewer spills do not pay for the
+// lost latency hiding at 1).
 #ifndef IXG_FILM_WAVES_PER_EU
 #define IXG_FILM_WAVES_PER_EU 3
 #endif
-#define IXG_WAVES_ATTR __attribute__((amdgpu_waves_per_eu(IXG_FILM_WAVES_PER_EU)))
+#ifndef IXG_FILM_WAVES_PER_EU_ANISO
+#define IXG_FILM_WAVES_PER_EU_ANISO 2
+#endif
+#define IXG_WAVES_ATTR \
+    __attribute__((amdgpu_waves_per_eu(ANISO ? IXG_FILM_WAVES_PER_EU_ANISO : IXG_FILM_WAVES_PER_EU)))
 // Idle lanes a wave collects before it finishes and regenerates paths.
 #ifndef IXG_FILM_REGEN_MIN
 #define IXG_FILM_REGEN_MIN 24
