@@ -140,6 +140,9 @@ typedef struct ixg_scene {
     const ixg_material* material;
     uint32_t max_bounces;
     ixg_environment environment;
+    /* FacetFinish per plane (GpuSceneRef::facet_finishes, IX/renderer/gpu_backend/
+     * mod.rs:217-237): 0 polished, 1 frosted; NULL = every facet polished. */
+    const uint8_t* facet_finishes;
 } ixg_scene;
 
 typedef struct ixg_opts {

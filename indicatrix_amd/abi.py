@@ -83,7 +83,8 @@ class Scene(C.Structure):
                 ("planes", C.POINTER(FacetPlane)), ("n_planes", C.c_uint32),
                 ("material", C.POINTER(Material)),
                 ("max_bounces", C.c_uint32),
-                ("environment", Environment)]
+                ("environment", Environment),
+                ("facet_finishes", C.POINTER(C.c_uint8))]
 
 
 class BatchItem(C.Structure):

@@ -95,6 +95,13 @@ ixg_status ensure_accum(ixg_ctx* ctx, size_t doubles) {
     return IXG_OK;
 }
 
+bool any_frosted(const ixg_scene* s) {
+    if (!s->facet_finishes) return false;
+    for (uint32_t i = 0; i < s->n_planes; ++i)
+        if (s->facet_finishes[i]) return true;
+    return false;
+}
+
 bool material_routable(const ixg_material& m) {
     // The device path covers isotropic and uniaxial (RayMaterialContext::
     // is_anisotropic, transport/inner.rs:181-198), polished, unzoned,
@@ -177,6 +184,7 @@ ixg_status build_params(ixg_ctx* ctx, const ixg_scene* s, uint32_t sample_offset
     std::memcpy(P->an.c_axis, m.c_axis, sizeof P->an.c_axis);
     std::memcpy(P->disp, s->material->dispersion.coeff, sizeof P->disp);
     P->edge_radius = s->material->edge_rounding_radius > 0.0f ? s->material->edge_rounding_radius : 0.0f;
+    P->frosted = any_frosted(s) ? 1 : 0;
     float g = s->environment.surface_glare;
     P->surface_glare = (g >= 1.0f || std::isnan(g)) ? 1.0f : (g > 0.0f ? g : 0.0f);
     if (!ixg::resolve_environment(s->environment, &P->env))
@@ -192,6 +200,13 @@ ixg_status upload_planes(ixg_ctx* ctx, const ixg_scene* s) {
     if (s->n_planes)
         HIP_TRY(ctx, hipMemcpyAsync(ctx->d_planes, pl.data(), pl.size() * sizeof(float4),
                                     hipMemcpyHostToDevice, ctx->stream));
+    if (any_frosted(s)) {  // the FacetFinish bitmask after the facet set
+        uint32_t mask[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        for (uint32_t i = 0; i < s->n_planes; ++i)
+            if (s->facet_finishes[i]) mask[i >> 5] |= 1u << (i & 31);
+        HIP_TRY(ctx, hipMemcpyAsync(ctx->d_planes + IXG_MAX_PLANES, mask, sizeof mask, hipMemcpyHostToDevice,
+                                    ctx->stream));
+    }
     return IXG_OK;
 }
 
@@ -270,7 +285,7 @@ ixg_status ixg_create(const ixg_opts* opts, ixg_ctx** out) {
     ctx->device = dev;
     if (opts && opts->chunk_paths) ctx->chunk_paths = opts->chunk_paths;
     if (hipSetDevice(dev) != hipSuccess || hipStreamCreate(&ctx->stream) != hipSuccess ||
-        hipMalloc(&ctx->d_planes, IXG_MAX_PLANES * sizeof(float4)) != hipSuccess ||
+        hipMalloc(&ctx->d_planes, (IXG_MAX_PLANES + 2) * sizeof(float4)) != hipSuccess ||
         hipMalloc(&ctx->d_counter, sizeof(uint32_t)) != hipSuccess ||
         ixg_persistent_grid(dev, &ctx->persistent_blocks) != hipSuccess ||
         hipEventCreate(&ctx->ev0) != hipSuccess || hipEventCreate(&ctx->ev1) != hipSuccess) {
@@ -337,6 +352,13 @@ ixg_route_reason ixg_route(const ixg_scene* s) {
     ixg::PresetInfo pi;
     if (!ixg::preset_info(s->environment.preset, &pi)) return IXG_ROUTE_UNSUPPORTED_ENVIRONMENT;
     if (!std::isfinite(s->environment.backdrop)) return IXG_ROUTE_UNSUPPORTED_ENVIRONMENT;
+    if (any_frosted(s)) {
+        // frosted facets: isotropic stones, and no environment whose sun the
+        // reference samples by next-event estimation at a frosted event
+        const ixg_material& m = *s->material;
+        if (!m.cubic && std::fabs(m.birefringence_delta) > 1e-4f) return IXG_ROUTE_UNSUPPORTED_MATERIAL;
+        if (pi.model == ixg::ENV_DAYLIGHT_SUN) return IXG_ROUTE_UNSUPPORTED_ENVIRONMENT;
+    }
     return IXG_ROUTE_OK;
 }
 

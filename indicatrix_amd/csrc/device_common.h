@@ -22,6 +22,8 @@ namespace ixg {
 #define R_UNPOL_SELECT_MIN 0.02f
 #define R_UNPOL_SELECT_MAX 0.98f
 #define FRESNEL_BRANCH_STREAM 0x9E3779B1u
+#define FROSTED_DIR_U_STREAM 0x27D4EB2Fu
+#define FROSTED_DIR_V_STREAM 0x165667B1u
 #define RUSSIAN_ROULETTE_STREAM 0x517CC1B7u
 
 // ---------------------------------------------------------------------------

@@ -87,6 +87,26 @@ __device__ __forceinline__ void fresnel_rs_rp(float n1, float n2, float cos_i, f
 
 __device__ __forceinline__ uint32_t fnv(uint32_t h, uint32_t v) { return (h ^ v) * 16777619u; }
 
+// The frosted-facet bitmask rides after the facet set (IXG_MAX_PLANES float4,
+// then 8 words) in global and LDS memory alike.
+__device__ __forceinline__ bool facet_frosted(const float4* __restrict__ planes, int facet) {
+    const uint32_t* mask = reinterpret_cast<const uint32_t*>(planes + IXG_MAX_PLANES);
+    return (mask[facet >> 5] >> (facet & 31)) & 1u;
+}
+
+// cosine_weighted_hemisphere about unit n with frosted_orthonormal_basis
+// (scattering/frosted.rs, scattering/mod.rs:110-115); the azimuth's sine and
+// cosine correctly rounded.
+__device__ __forceinline__ V3 cosine_hemisphere(float u1, float u2, V3 n) {
+    const float r = sqrt_rn(u1);
+    const float theta = (2.0f * 3.14159265358979323846f) * u2;
+    const float sin_t = (float)sin((double)theta), cos_t = (float)cos((double)theta);
+    const V3 a = fabsf(n.x) > 0.9f ? v3(0.0f, 1.0f, 0.0f) : v3(1.0f, 0.0f, 0.0f);
+    const V3 t = normalize_or_zero(a - n * dot(n, a));
+    const V3 b = cross(n, t);
+    return normalize_or_zero((t * (r * cos_t) + b * (r * sin_t)) + n * sqrt_rn(fmaxf(1.0f - u1, 0.0f)));
+}
+
 // shading_normal_near_edge (RT/intersect.rs:196-233): within radius of the
 // nearest other plane the facet normal eases (smoothstep) toward the bisector
 // of the two normals.  Called only for radius > 0 (a launch-uniform branch),
@@ -895,7 +915,30 @@ __device__ __forceinline__ int path_bounce(const FilmParams& P, const float4* __
 
     V3 new_k;
     bool new_inside = inside;
-    if (reflect) {
+    bool frosted_hit = false;
+    if (P.frosted && facet_frosted(planes, facet)) {
+        // apply_frosted_bounce (scattering/frosted.rs), next-event estimation
+        // off: a cosine-weighted hemisphere on the reflect side (always under
+        // TIR) or, transmitted, the far side; every channel depolarised
+        frosted_hit = true;
+        const float u1 = unit_draw(S.seed, bounce, FROSTED_DIR_U_STREAM);
+        const float u2 = unit_draw(S.seed, bounce, FROSTED_DIR_V_STREAM);
+        bool through = false;
+        float pdf = 1.0f;
+        if (!hero_tir) {
+            const float r_pdf = clampf(0.5f * fma_(rp_h, rp_h, rs_h * rs_h), R_UNPOL_PDF_MIN, R_UNPOL_PDF_MAX);
+            through = !(rng < r_pdf);
+            pdf = through ? 1.0f - r_pdf : r_pdf;
+        }
+        new_k = cosine_hemisphere(u1, u2, through ? neg(normal) : normal);
+#pragma unroll
+        for (int c = 0; c < NCH; ++c) {
+            S.st[c] = make_float4(fmaxf(S.st[c].x, 0.0f), 0.0f, 0.0f, 0.0f);
+            if (!hero_tir) S.path_pdf[c] *= pdf;
+        }
+        if (through) new_inside = !inside;
+        if (EVENTS) S.fh = fnv(fnv(S.fh, (uint32_t)facet), hero_tir ? 4u : (through ? 6u : 5u));
+    } else if (reflect) {
         // apply_tir_bounce's scalar branch (tir.rs:139-160) and
         // apply_partial_reflect_bounce (reflect_refract.rs:33-71) share every
         // per-channel step; only the partial reflection divides by r_unpol
@@ -1029,7 +1072,7 @@ __device__ __forceinline__ int path_bounce(const FilmParams& P, const float4* __
     S.org = hit + new_k * 1e-4f;
     S.inside = new_inside;
     S.last_facet = facet;
-    if (bounce == 0 && !new_inside && P.surface_glare < 1.0f) {
+    if (bounce == 0 && !new_inside && P.surface_glare < 1.0f && !frosted_hit) {
 #pragma unroll
         for (int c = 0; c < NCH; ++c) S.st[c] = scale4(S.st[c], P.surface_glare);
     }
@@ -1158,6 +1201,7 @@ __global__ void zero_counter_kernel(uint32_t* c) { *c = 0u; }
 __device__ __forceinline__ void stage_planes(const FilmParams& P, const float4* __restrict__ g,
                                              float4* s) {
     for (uint32_t i = threadIdx.x; i < P.n_planes; i += blockDim.x) s[i] = g[i];
+    if (P.frosted && threadIdx.x < 2) s[IXG_MAX_PLANES + threadIdx.x] = g[IXG_MAX_PLANES + threadIdx.x];
     __syncthreads();
 }
 
@@ -1166,7 +1210,7 @@ template <bool ABSORB, bool ANISO>
 __global__ __launch_bounds__(kBlock) void film_transport_kernel(const FilmParams P,
                                                                 const float4* __restrict__ planes,
                                                                 float* __restrict__ out_xyz) {
-    __shared__ float4 sp[IXG_MAX_PLANES];
+    __shared__ float4 sp[IXG_MAX_PLANES + 2];  // + the frosted bitmask
     stage_planes(P, planes, sp);
     uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
     if (i >= P.n_paths) return;
@@ -1195,7 +1239,7 @@ __global__ __launch_bounds__(kBlock) IXG_WAVES_ATTR void film_persistent_kernel(
                                                                  const float4* __restrict__ planes,
                                                                  float* __restrict__ out_xyz,
                                                                  uint32_t* __restrict__ counter) {
-    __shared__ float4 sp[IXG_MAX_PLANES];
+    __shared__ float4 sp[IXG_MAX_PLANES + 2];  // + the frosted bitmask
     stage_planes(P, planes, sp);
     const uint32_t n = (uint32_t)P.n_paths;
     const uint32_t lane = __lane_id();
@@ -1263,7 +1307,7 @@ __global__ __launch_bounds__(kBlock) void film_events_kernel(const FilmParams P,
                                                              float* __restrict__ out_xyz,
                                                              ixg_path_events* __restrict__ ev,
                                                              unsigned long long* __restrict__ totals) {
-    __shared__ float4 sp[IXG_MAX_PLANES];
+    __shared__ float4 sp[IXG_MAX_PLANES + 2];  // + the frosted bitmask
     __shared__ unsigned long long block_tot[EV_NUM];
     if (threadIdx.x < EV_NUM) block_tot[threadIdx.x] = 0ull;
     stage_planes(P, planes, sp);
@@ -1307,7 +1351,7 @@ __global__ __launch_bounds__(kBlock) void film_rays_kernel(const FilmParams P,
                                                            uint32_t n_rays,
                                                            float* __restrict__ out_xyz,
                                                            ixg_path_events* __restrict__ ev) {
-    __shared__ float4 sp[IXG_MAX_PLANES];
+    __shared__ float4 sp[IXG_MAX_PLANES + 2];  // + the frosted bitmask
     stage_planes(P, planes, sp);
     uint32_t i = blockIdx.x * kBlock + threadIdx.x;
     if (i >= n_rays) return;

@@ -83,6 +83,7 @@ struct FilmParams {
     float disp[6];
     float surface_glare;  // clamped to [0,1]
     float edge_radius;    // GemMaterial::edge_rounding_radius (<= 0: sharp edges)
+    int32_t frosted;      // 1: some facet is frosted (bitmask after the planes)
     uint64_t first_path;  // first (pixel*spp + sample) index of this launch
     uint64_t n_paths;     // paths in this launch
     int32_t absorb;       // 1: the material has absorption bands

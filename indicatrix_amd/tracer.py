@@ -178,7 +178,9 @@ def round_brilliant(crown_deg, pavilion_deg, table_ratio):
 class FrameScene:
     """Owns the ctypes storage behind one ixg_scene."""
 
-    def __init__(self, camera, width, height, planes, material, max_bounces, environment):
+    def __init__(self, camera, width, height, planes, material, max_bounces, environment,
+                 facet_finishes=None):
+        """facet_finishes: per plane 0 (polished) / 1 (frosted), or None."""
         self.planes = planes if isinstance(planes, C.Array) else planes_from_array(planes)
         self.material_abi = material.to_abi() if isinstance(material, GemMaterial) else material
         self.scene = abi.Scene()
@@ -190,6 +192,19 @@ class FrameScene:
         self.scene.material = C.pointer(self.material_abi)
         self.scene.max_bounces = max_bounces
         self.scene.environment = environment
+        self.set_finishes(facet_finishes)
+
+    def set_finishes(self, facet_finishes):
+        """Sets the FacetFinish of every plane (None: all polished)."""
+        if facet_finishes is None:
+            self.finishes = None
+            self.scene.facet_finishes = None
+        else:
+            vals = [1 if f else 0 for f in facet_finishes]
+            if len(vals) != len(self.planes):
+                raise ValueError("one finish per plane")
+            self.finishes = (C.c_uint8 * len(vals))(*vals)
+            self.scene.facet_finishes = C.cast(self.finishes, C.POINTER(C.c_uint8))
 
     @property
     def width(self):

@@ -97,6 +97,10 @@ static inline stokes st_scale(stokes s, float f) {
     return o;
 }
 static inline float st_intensity(stokes s) { return maxf(s.i, 0.0f); }
+static inline stokes st_unpolarized(stokes s) { /* StokesVector::unpolarized(s.intensity()) */
+    stokes o = {st_intensity(s), 0.0f, 0.0f, 0.0f};
+    return o;
+}
 
 static mat4 mat_cols(const float a[16]) {
     mat4 M;
@@ -141,6 +145,8 @@ static mat4 mueller_tir_retardation(float delta) {
 /* RNG (RT/sampling.rs)                                                 */
 /* ------------------------------------------------------------------ */
 #define FRESNEL_BRANCH_STREAM 0x9E3779B1u
+#define FROSTED_DIR_U_STREAM 0x27D4EB2Fu /* sampling.rs:24-25 */
+#define FROSTED_DIR_V_STREAM 0x165667B1u
 #define RUSSIAN_ROULETTE_STREAM 0x517CC1B7u
 #define PIXEL_JITTER_X_ROTATION_STREAM 0xA511E9B3u
 #define PIXEL_JITTER_Y_ROTATION_STREAM 0x63D81B23u
@@ -726,6 +732,7 @@ typedef struct {
     int aniso;          /* RayMaterialContext::is_anisotropic (uniaxial here) */
     v3 c_raw;           /* material c_axis as given */
     v3 ta1, ta2, tcn;   /* AbsorptionTensor3 axes: basis of normalize_or_zero(c_axis) */
+    const uint8_t* finish; /* FacetFinish per plane, NULL = all polished */
 } scene_t;
 
 typedef struct {
@@ -745,6 +752,24 @@ typedef struct {
     int escaped;
     v3 observer;
 } path_t;
+
+/* frosted_orthonormal_basis (scattering/mod.rs:110-115) and
+ * cosine_weighted_hemisphere (scattering/frosted.rs): the azimuth's sine and
+ * cosine correctly rounded. */
+static v3 cosine_weighted_hemisphere(float u1, float u2, v3 n) {
+    const float r = sqrtf(u1);
+    const float theta = (2.0f * F32_PI) * u2;
+    const float sin_t = (float)sin((double)theta), cos_t = (float)cos((double)theta);
+    const v3 a = fabsf(n.x) > 0.9f ? mk(0.0f, 1.0f, 0.0f) : mk(1.0f, 0.0f, 0.0f);
+    const v3 t = vnormalize_or_zero(vsub(a, vmul(n, vdot(n, a))));
+    const v3 b = vcross(n, t);
+    return vnormalize_or_zero(vadd(vadd(vmul(t, r * cos_t), vmul(b, r * sin_t)), vmul(n, sqrtf(maxf(1.0f - u1, 0.0f)))));
+}
+
+void ixo_cosine_hemisphere(float u1, float u2, const float n[3], float out[3]) {
+    const v3 r = cosine_weighted_hemisphere(u1, u2, vfrom(n));
+    out[0] = r.x; out[1] = r.y; out[2] = r.z;
+}
 
 static void ev_facet(ixg_path_events* ev, int32_t facet, uint32_t kind) {
     if (!ev) return;
@@ -901,7 +926,42 @@ static void facet_event(const scene_t* S, path_t* P, uint32_t bounce, uint32_t s
     v3 new_k, new_s;
     int new_inside = P->inside;
     v3 kk = P->k;
-    if (G.sin2_t > 1.0f) {
+    const int frosted = S->finish != NULL && S->finish[facet] != 0;
+    if (frosted) {
+        /* apply_frosted_bounce (scattering/frosted.rs), next-event estimation off:
+         * a cosine-weighted hemisphere on the reflect (or, transmitted, the far)
+         * side, the Stokes vectors depolarised, the hero Fresnel draw */
+        const float u1 = unit_draw(seed, bounce, FROSTED_DIR_U_STREAM);
+        const float u2 = unit_draw(seed, bounce, FROSTED_DIR_V_STREAM);
+        if (G.sin2_t > 1.0f) {
+            new_k = cosine_weighted_hemisphere(u1, u2, normal);
+            for (int c = 0; c < NCH; ++c) P->st[c] = st_unpolarized(P->st[c]);
+            ev_facet(ev, facet, 4);
+        } else {
+            const float cos_t = sqrtf(maxf(1.0f - G.sin2_t, 0.0f));
+            float rs, rp;
+            fresnel_rs_rp(G.n1, G.n2, G.cos_i, cos_t, &rs, &rp);
+            const float r_unpol = clampf(0.5f * fmaf(rp, rp, rs * rs), R_UNPOL_PDF_MIN, R_UNPOL_PDF_MAX);
+            if (unit_draw(seed, bounce, FRESNEL_BRANCH_STREAM) < r_unpol) {
+                new_k = cosine_weighted_hemisphere(u1, u2, normal);
+                for (int c = 0; c < NCH; ++c) {
+                    P->st[c] = st_unpolarized(P->st[c]);
+                    P->path_pdf[c] *= r_unpol;
+                }
+                ev_facet(ev, facet, 5);
+            } else {
+                new_k = cosine_weighted_hemisphere(u1, u2, vneg(normal));
+                const float t_unpol = 1.0f - r_unpol;
+                for (int c = 0; c < NCH; ++c) {
+                    P->st[c] = st_unpolarized(P->st[c]);
+                    P->path_pdf[c] *= t_unpol;
+                }
+                new_inside = !P->inside;
+                ev_facet(ev, facet, 6);
+            }
+        }
+        new_s = new_k;
+    } else if (G.sin2_t > 1.0f) {
         /* apply_tir_bounce (refraction/tir.rs:56-160), scalar branch */
         if (ev) ev->n_tir++;
         ev_facet(ev, facet, 1);
@@ -1035,8 +1095,8 @@ static void facet_event(const scene_t* S, path_t* P, uint32_t bounce, uint32_t s
     P->org = vadd(hit_point, vmul(new_s, 1e-4f));
     P->k = new_k;
     P->inside = new_inside;
-    /* surface glare (bounce_body.rs:686-692) */
-    if (bounce == 0 && !P->inside && S->surface_glare < 1.0f)
+    /* surface glare (bounce_body.rs:686-692), not after a frosted facet */
+    if (bounce == 0 && !P->inside && S->surface_glare < 1.0f && !frosted)
         for (int c = 0; c < NCH; ++c) P->st[c] = st_scale(P->st[c], S->surface_glare);
     /* apply_russian_roulette (transport/bounce.rs:355-375) */
     if (bounce > 4) {
@@ -1609,6 +1669,7 @@ void ixo_uniaxial_entry_r(float n_o, float n_e, const float c[3], const float kk
 
 static int scene_build(const ixg_scene* sc, scene_t* S) {
     S->planes = sc->planes;
+    S->finish = sc->facet_finishes;
     S->n_planes = sc->n_planes;
     S->mat = sc->material;
     S->max_bounces = sc->max_bounces;

@@ -45,6 +45,9 @@ float ixo_sample_studio_environment(const float dir[3], float lambda_nm, int32_t
 void ixo_shading_normal_near_edge(const ixg_facet_plane* planes, uint32_t n, const float hit[3], int32_t facet,
                                   const float hn[3], float radius, float out[3]);
 
+/* cosine_weighted_hemisphere (RT/scattering/frosted.rs) about unit n. */
+void ixo_cosine_hemisphere(float u1, float u2, const float n[3], float out[3]);
+
 /* Camera (RT/camera.rs). */
 void ixo_camera_new(float yaw, float pitch, float distance, float fov_deg, uint32_t width,
                     uint32_t height, uint32_t spp, ixg_camera* out);
