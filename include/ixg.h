@@ -98,8 +98,9 @@ typedef struct ixg_material {
     ixg_band bands_e[IXG_MAX_BANDS];
     float absorption_path_scale;
     float edge_rounding_radius;           /* > 0: rounded meet edges (shading)     */
-    float scattering_sigma_s;             /* GPU path: must be 0                   */
+    float scattering_sigma_s;             /* > 0: Henyey-Greenstein volume scatter */
     int32_t biaxial;                      /* GPU path: must be 0                   */
+    float scattering_g;                   /* Henyey-Greenstein asymmetry           */
 } ixg_material;
 
 /* LightingPreset ids, the reference's persisted ids

@@ -68,7 +68,7 @@ class Material(C.Structure):
                 ("absorption_path_scale", C.c_float),
                 ("edge_rounding_radius", C.c_float),
                 ("scattering_sigma_s", C.c_float),
-                ("biaxial", C.c_int32)]
+                ("biaxial", C.c_int32), ("scattering_g", C.c_float)]
 
 
 class Environment(C.Structure):

@@ -107,7 +107,12 @@ bool material_routable(const ixg_material& m) {
     // is_anisotropic, transport/inner.rs:181-198), polished, unzoned,
     // non-scattering materials, with or without absorption bands; biaxial
     // materials decline.
-    if (m.biaxial || m.scattering_sigma_s > 0.0f || !(m.edge_rounding_radius < 1e30f)) return false;
+    if (m.biaxial || !(m.edge_rounding_radius < 1e30f)) return false;
+    // Henyey-Greenstein volume scattering: isotropic stones, finite parameters
+    if (!(m.scattering_sigma_s <= 0.0f) &&
+        (!(m.scattering_sigma_s < 1e30f) || !(std::fabs(m.scattering_g) < 1.0f) || !(m.absorption_path_scale > 0.0f) ||
+         (!m.cubic && std::fabs(m.birefringence_delta) > 1e-4f)))
+        return false;
     if (m.has_e_dispersion && (m.e_dispersion.kind < IXG_DISP_SELLMEIER1 || m.e_dispersion.kind > IXG_DISP_CAUCHY))
         return false;
     if (m.n_bands_o < 0 || m.n_bands_o > IXG_MAX_BANDS || m.n_bands_e < 0 || m.n_bands_e > IXG_MAX_BANDS)
@@ -175,6 +180,7 @@ ixg_status build_params(ixg_ctx* ctx, const ixg_scene* s, uint32_t sample_offset
     P->disp_kind = s->material->dispersion.kind;
     P->absorb = (s->material->n_bands_o > 0 || s->material->n_bands_e > 0) ? 1 : 0;
     if (P->absorb) fill_absorption(*s->material, &P->abs);
+    P->abs.path_scale = s->material->absorption_path_scale;  // the scattering step reads it too
     const ixg_material& m = *s->material;
     P->an.on = (!m.cubic && std::fabs(m.birefringence_delta) > 1e-4f) ? 1 : 0;
     P->an.biref = m.birefringence_delta;
@@ -185,6 +191,8 @@ ixg_status build_params(ixg_ctx* ctx, const ixg_scene* s, uint32_t sample_offset
     std::memcpy(P->disp, s->material->dispersion.coeff, sizeof P->disp);
     P->edge_radius = s->material->edge_rounding_radius > 0.0f ? s->material->edge_rounding_radius : 0.0f;
     P->frosted = any_frosted(s) ? 1 : 0;
+    P->sigma_s = s->material->scattering_sigma_s > 0.0f ? s->material->scattering_sigma_s : 0.0f;
+    P->scat_g = s->material->scattering_g;
     float g = s->environment.surface_glare;
     P->surface_glare = (g >= 1.0f || std::isnan(g)) ? 1.0f : (g > 0.0f ? g : 0.0f);
     if (!ixg::resolve_environment(s->environment, &P->env))
@@ -352,6 +360,8 @@ ixg_route_reason ixg_route(const ixg_scene* s) {
     ixg::PresetInfo pi;
     if (!ixg::preset_info(s->environment.preset, &pi)) return IXG_ROUTE_UNSUPPORTED_ENVIRONMENT;
     if (!std::isfinite(s->environment.backdrop)) return IXG_ROUTE_UNSUPPORTED_ENVIRONMENT;
+    if (s->material->scattering_sigma_s > 0.0f && pi.model == ixg::ENV_DAYLIGHT_SUN)
+        return IXG_ROUTE_UNSUPPORTED_ENVIRONMENT;  // the sun is sampled by next-event estimation at scatters
     if (any_frosted(s)) {
         // frosted facets: isotropic stones, and no environment whose sun the
         // reference samples by next-event estimation at a frosted event

@@ -23,6 +23,9 @@ namespace ixg {
 #define R_UNPOL_SELECT_MAX 0.98f
 #define FRESNEL_BRANCH_STREAM 0x9E3779B1u
 #define FROSTED_DIR_U_STREAM 0x27D4EB2Fu
+#define DISTANCE_SAMPLE_STREAM 0xA24BAED4u
+#define PHASE_DIR_U_STREAM 0x9FB21C65u
+#define PHASE_DIR_V_STREAM 0x1CE4E5B9u
 #define FROSTED_DIR_V_STREAM 0x165667B1u
 #define RUSSIAN_ROULETTE_STREAM 0x517CC1B7u
 

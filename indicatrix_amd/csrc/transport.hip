@@ -248,7 +248,7 @@ __device__ __forceinline__ bool leaves_convex(const float4* __restrict__ planes,
     return side > 0.0f && denom > 1e-7f;
 }
 
-enum StepResult { STEP_CONTINUE = 0, STEP_ESCAPED = 1, STEP_ROULETTE = 2 };
+enum StepResult { STEP_CONTINUE = 0, STEP_ESCAPED = 1, STEP_ROULETTE = 2, STEP_SCATTER_ABSORBED = 3 };
 
 // AbsorptionTensor3::quadratic_form (birefringence/absorption_tensor.rs):
 // alpha . (axes^T e)^2, axes = (a1, a2, c), alpha = (ao, ao, ae).
@@ -264,6 +264,28 @@ __device__ __forceinline__ float tensor_qf(const AbsorbParams& A, float ao, floa
 // (absorption.rs:205-222): the mean of the two eigen-directions' quadratic
 // forms (f32::midpoint, computed in f64 like the reference), Beer-Lambert
 // through exp_f32x8's lane polynomial over the scaled segment length.
+// channel_absorption_alphas_assigned, isotropic arm (absorption.rs:197-222):
+// per channel the mean of the tensor's quadratic form over the two
+// eigen-polarisations.
+__device__ __forceinline__ void channel_alphas(const AbsorbParams& A, const PathState& S, V3 k, float alpha[NCH]) {
+    const V3 c = v3(A.c_raw[0], A.c_raw[1], A.c_raw[2]);
+    V3 ea = cross(k, c);
+    if (dot(ea, ea) > 1e-8f) {
+        ea = normalize(ea);
+    } else {
+        const V3 n = normalize_or_zero(k);
+        const V3 a = fabsf(n.x) > 0.9f ? v3(0.0f, 1.0f, 0.0f) : v3(1.0f, 0.0f, 0.0f);
+        ea = normalize_or_zero(a - n * dot(n, a));
+    }
+    const V3 eb = normalize_or_zero(cross(k, ea));
+#pragma unroll
+    for (int ch = 0; ch < NCH; ++ch) {
+        const float qa = tensor_qf(A, S.alpha_o[ch], S.alpha_e[ch], ea);
+        const float qb = tensor_qf(A, S.alpha_o[ch], S.alpha_e[ch], eb);
+        alpha[ch] = (float)(((double)qa + (double)qb) * 0.5);
+    }
+}
+
 __device__ __forceinline__ void apply_absorption(const AbsorbParams& A, PathState& S, V3 k, float path_len) {
     const V3 c = v3(A.c_raw[0], A.c_raw[1], A.c_raw[2]);
     V3 ea = cross(k, c);
@@ -341,7 +363,7 @@ __device__ __forceinline__ V3 reflect_dir(V3 k, V3 normal) { return k - normal *
 // closed-form entry / internal arms where the wave is not along the optic
 // axis, the eigenmode-drawing scalar arms where it is, assigned-mode
 // absorption and the internal o<->e re-coupling draw.
-template <bool EVENTS, bool ISO, bool ABSORB>
+template <bool EVENTS, bool ISO, bool ABSORB, bool EXTRA = true>
 __device__ __forceinline__ int facet_aniso(const FilmParams& P, const float4* __restrict__ planes, PathState& S,
                                         float t_hit, int facet) {
     const uint32_t bounce = S.bounce;
@@ -349,7 +371,8 @@ __device__ __forceinline__ int facet_aniso(const FilmParams& P, const float4* __
     const V3 hit = S.org + S.dir * t_hit;
     const float4 pf = planes[facet];
     V3 normal = v3(pf.x, pf.y, pf.z);
-    if (P.edge_radius > 0.0f) normal = shading_normal_near_edge(planes, P.n_planes, hit, facet, normal, P.edge_radius);
+    if (EXTRA && P.edge_radius > 0.0f)
+        normal = shading_normal_near_edge(planes, P.n_planes, hit, facet, normal, P.edge_radius);
     const V3 cur_plane = normalize_or_zero(cross(S.k, normal));
     if (S.has_prev && dot(cur_plane, cur_plane) > 1e-6f && dot(S.prev_plane, S.prev_plane) > 1e-6f) {
         const float cos_psi = clampf(dot(S.prev_plane, cur_plane), -1.0f, 1.0f);
@@ -824,7 +847,104 @@ __device__ __forceinline__ int facet_aniso(const FilmParams& P, const float4* __
 
 // One iteration of the bounce loop (bounce_body.rs:361-707): intersection, and
 // either the escape lookup (written to `radiance`) or the facet event.
-template <bool EVENTS, bool ISO = false, bool ABSORB = false, bool ANISO = false>
+// apply_russian_roulette (transport/bounce.rs:355-375): false when the path dies.
+__device__ __forceinline__ bool russian_roulette(PathState& S, uint32_t bounce) {
+    float mx = 0.0f;
+#pragma unroll
+    for (int c = 0; c < NCH; ++c) mx = fmaxf(mx, fmaxf(S.st[c].x, 0.0f));
+    const float q = clampf(mx, 0.05f, 1.0f);
+    if (unit_draw(S.seed, bounce, RUSSIAN_ROULETTE_STREAM) > q) return false;
+    const float inv_q = div_rn(1.0f, q);
+#pragma unroll
+    for (int c = 0; c < NCH; ++c) S.st[c] = scale4(S.st[c], inv_q);
+#pragma unroll
+    for (int c = 0; c < NCH; ++c) S.split[c] = div_rn(S.split[c], q);
+    return true;
+}
+
+// sample_henyey_greenstein_direction (scattering/hg.rs:636-651) about fwd,
+// the azimuth's sine and cosine correctly rounded.
+__device__ __forceinline__ V3 hg_direction(float u1, float u2, float g, V3 fwd) {
+    float cos_theta;
+    if (fabsf(g) < 1e-3f) {
+        cos_theta = fma_(2.0f, -u1, 1.0f);
+    } else {
+        const float one_minus_g2 = fma_(g, -g, 1.0f);
+        const float denom = fma_(2.0f * g, u1, 1.0f - g);
+        const float sq = div_rn(one_minus_g2, denom);
+        cos_theta = div_rn(fma_(sq, -sq, fma_(g, g, 1.0f)), 2.0f * g);
+    }
+    cos_theta = clampf(cos_theta, -1.0f, 1.0f);
+    const float sin_theta = sqrt_rn(fmaxf(fma_(cos_theta, -cos_theta, 1.0f), 0.0f));
+    const float phi = (2.0f * 3.14159265358979323846f) * u2;
+    const float sin_p = (float)sin((double)phi), cos_p = (float)cos((double)phi);
+    const V3 a = fabsf(fwd.x) > 0.9f ? v3(0.0f, 1.0f, 0.0f) : v3(1.0f, 0.0f, 0.0f);
+    const V3 t = normalize_or_zero(a - fwd * dot(fwd, a));
+    const V3 b = cross(fwd, t);
+    return normalize_or_zero((t * (sin_theta * cos_p) + b * (sin_theta * sin_p)) + fwd * cos_theta);
+}
+
+// try_scatter_step / maybe_scatter_or_extinguish (scattering/hg.rs:91-160,
+// 435-600) for an isotropic, unzoned stone with next-event estimation off: a
+// free-flight distance against the hero channel's extinction; short of the
+// facet the path scatters (Henyey-Greenstein, every channel depolarised and
+// reweighted by its own transmittance) and the iteration ends there, else it
+// reaches the facet with each channel's survival relative to the hero's.
+// Returns a StepResult, or -1 for "reached the facet".
+template <bool EVENTS, bool ABSORB>
+__device__ __forceinline__ int scatter_step(const FilmParams& P, PathState& S, float hit_t) {
+    const uint32_t bounce = S.bounce;
+    float alpha[NCH];
+    if (ABSORB) {
+        channel_alphas(P.abs, S, S.dir, alpha);
+    } else {
+#pragma unroll
+        for (int c = 0; c < NCH; ++c) alpha[c] = 0.0f;
+    }
+    const float path_scale = P.abs.path_scale;
+    const float sigma_s = div_rn(P.sigma_s, path_scale);
+    const float sigma_t_hero = alpha[0] + sigma_s;
+    const float one_minus_u = fmaxf(1.0f - unit_draw(S.seed, bounce, DISTANCE_SAMPLE_STREAM), 1e-7f);
+    const float t_free = div_rn(-(float)log((double)one_minus_u), sigma_t_hero);
+    const float hit_t_scaled = hit_t * path_scale;
+    if (t_free < hit_t_scaled) {
+        const float pdf_hero = sigma_t_hero * one_minus_u;
+#pragma unroll
+        for (int c = 0; c < NCH; ++c) {
+            const float sigma_t = alpha[c] + sigma_s;
+            const float tr = exp_lane(-(alpha[c] + sigma_s) * t_free);
+            const float weight = div_rn(tr * sigma_s, pdf_hero);
+            S.st[c] = make_float4(fmaxf(S.st[c].x, 0.0f) * weight, 0.0f, 0.0f, 0.0f);
+            S.path_pdf[c] *= sigma_t * tr;
+        }
+        const float u1 = unit_draw(S.seed, bounce, PHASE_DIR_U_STREAM);
+        const float u2 = unit_draw(S.seed, bounce, PHASE_DIR_V_STREAM);
+        const V3 old_dir = S.dir;
+        const V3 new_dir = hg_direction(u1, u2, P.scat_g, old_dir);
+        S.org = S.org + old_dir * div_rn(t_free, path_scale);
+        S.dir = new_dir;
+        S.has_prev = false;  // depolarised: no previous plane of incidence
+        if (EVENTS) S.fh = fnv(fnv(S.fh, 0xFFFFu), 7u);
+        if (bounce > 4 && !russian_roulette(S, bounce)) return STEP_SCATTER_ABSORBED;
+        return STEP_CONTINUE;
+    }
+    float survive[NCH];
+#pragma unroll
+    for (int c = 0; c < NCH; ++c) survive[c] = exp_lane(-(alpha[c] + sigma_s) * hit_t_scaled);
+    const float hero = fmaxf(survive[0], 1e-30f);
+#pragma unroll
+    for (int c = 0; c < NCH; ++c) {
+        S.st[c] = scale4(S.st[c], div_rn(survive[c], hero));
+        S.path_pdf[c] *= survive[c];
+    }
+    return -1;
+}
+
+// EXTRA: the rarely used surface and medium features (edge-rounded normals,
+// frosted facets, volume scattering) are compiled in; the persistent film
+// kernel is also built without them, so a stone that uses none runs the
+// lean bounce.
+template <bool EVENTS, bool ISO = false, bool ABSORB = false, bool ANISO = false, bool EXTRA = true>
 __device__ __forceinline__ int path_bounce(const FilmParams& P, const float4* __restrict__ planes,
                                            PathState& S, float radiance[NCH]) {
     const uint32_t bounce = S.bounce;
@@ -864,14 +984,19 @@ __device__ __forceinline__ int path_bounce(const FilmParams& P, const float4* __
         return STEP_ESCAPED;
     }
 
-    if (ANISO) return facet_aniso<EVENTS, ISO, ABSORB>(P, planes, S, t_hit, facet);
+    if (EXTRA && !ANISO && P.sigma_s > 0.0f && S.inside) {  // run_bounce (bounce_body.rs:410-416)
+        const int r = scatter_step<EVENTS, ABSORB>(P, S, t_hit);
+        if (r >= 0) return r;
+    }
+    if (ANISO) return facet_aniso<EVENTS, ISO, ABSORB, EXTRA>(P, planes, S, t_hit, facet);
     // facet_event (bounce_body.rs:611-707)
     const V3 k = S.dir;
     const V3 hit = S.org + k * t_hit;
     const float4 pf = planes[facet];
     V3 normal = v3(pf.x, pf.y, pf.z);
     // bounce_body.rs:620-629: the edge-rounded shading normal replaces the facet's
-    if (P.edge_radius > 0.0f) normal = shading_normal_near_edge(planes, P.n_planes, hit, facet, normal, P.edge_radius);
+    if (EXTRA && P.edge_radius > 0.0f)
+        normal = shading_normal_near_edge(planes, P.n_planes, hit, facet, normal, P.edge_radius);
     // rotate_stokes_to_plane_of_incidence (absorption.rs:69-87)
     const V3 cur_plane = normalize_or_zero(cross(k, normal));
     if (S.has_prev && dot(cur_plane, cur_plane) > 1e-6f && dot(S.prev_plane, S.prev_plane) > 1e-6f) {
@@ -891,7 +1016,7 @@ __device__ __forceinline__ int path_bounce(const FilmParams& P, const float4* __
     // apply_interior_segment (transport/inner.rs:161-178); without bands every
     // channel transmits exp(-0) == 1 exactly, so only the normal flips.
     const bool inside = S.inside;
-    if (ABSORB && inside) apply_absorption(P.abs, S, k, t_hit);
+    if (ABSORB && inside && !(EXTRA && P.sigma_s > 0.0f)) apply_absorption(P.abs, S, k, t_hit);  // scattering: done above
     if (inside) normal = neg(normal);
 
     // compute_bounce_refraction_geometry (geometry.rs:306-425), isotropic
@@ -916,7 +1041,7 @@ __device__ __forceinline__ int path_bounce(const FilmParams& P, const float4* __
     V3 new_k;
     bool new_inside = inside;
     bool frosted_hit = false;
-    if (P.frosted && facet_frosted(planes, facet)) {
+    if (EXTRA && P.frosted && facet_frosted(planes, facet)) {
         // apply_frosted_bounce (scattering/frosted.rs), next-event estimation
         // off: a cosine-weighted hemisphere on the reflect side (always under
         // TIR) or, transmitted, the far side; every channel depolarised
@@ -1145,7 +1270,7 @@ __device__ __forceinline__ void trace_path(const FilmParams& P, const float4* __
         int r = path_bounce<EVENTS, false, ABSORB, ANISO>(P, planes, S, radiance);
         if (r != STEP_CONTINUE) {
             bounces = b + 1;
-            term = r == STEP_ESCAPED ? 0u : 2u;
+            term = r == STEP_ESCAPED ? 0u : (r == STEP_SCATTER_ABSORBED ? 4u : 2u);
             escaped = r == STEP_ESCAPED;
             break;
         }
@@ -1234,7 +1359,7 @@ __global__ __launch_bounds__(kBlock) void film_transport_kernel(const FilmParams
 // (paths run 1..max_bounces iterations, mostly 2-6), and the per-path results
 // are the same as film_transport_kernel's because a path is a pure function
 // of its (pixel, sample) index.
-template <bool ISO, bool ABSORB, bool ANISO>
+template <bool ISO, bool ABSORB, bool ANISO, bool EXTRA>
 __global__ __launch_bounds__(kBlock) IXG_WAVES_ATTR void film_persistent_kernel(const FilmParams P,
                                                                  const float4* __restrict__ planes,
                                                                  float* __restrict__ out_xyz,
@@ -1290,7 +1415,7 @@ __global__ __launch_bounds__(kBlock) IXG_WAVES_ATTR void film_persistent_kernel(
         if (__ballot(active) == 0ull) break;
         if (active) {
             int r = STEP_CONTINUE;
-            if (S.bounce < P.max_bounces) r = path_bounce<false, ISO, ABSORB, ANISO>(P, sp, S, radiance);
+            if (S.bounce < P.max_bounces) r = path_bounce<false, ISO, ABSORB, ANISO, EXTRA>(P, sp, S, radiance);
             if (r != STEP_CONTINUE || ++S.bounce >= P.max_bounces) {
                 active = false;
                 pending = true;
@@ -1452,22 +1577,36 @@ extern "C" hipError_t ixg_launch_film_persistent(const ixg::FilmParams* P, const
     hipLaunchKernelGGL(ixg::zero_counter_kernel, dim3(1), dim3(1), 0, stream, d_counter);
     // the lit models: one scaled factor per direction (ASET is per wavelength)
     const bool iso = P->env.model != ixg::ENV_STUDIO && P->env.model != ixg::ENV_ASET;
-#define IXG_PI(A, N)                                                                                           \
-    hipLaunchKernelGGL((ixg::film_persistent_kernel<true, A, N>), dim3(grid_blocks), dim3(ixg::kBlock), 0, stream, \
-                       *P, d_planes, d_xyz, d_counter)
-#define IXG_PS(A, N)                                                                                            \
-    hipLaunchKernelGGL((ixg::film_persistent_kernel<false, A, N>), dim3(grid_blocks), dim3(ixg::kBlock), 0, stream, \
-                       *P, d_planes, d_xyz, d_counter)
-    if (iso) IXG_DISPATCH2(IXG_PI, P->absorb, P->an.on);
-    else IXG_DISPATCH2(IXG_PS, P->absorb, P->an.on);
+    const bool extra = P->frosted || P->sigma_s > 0.0f || P->edge_radius > 0.0f;
+#define IXG_PI(A, N)                                                                                                 \
+    hipLaunchKernelGGL((ixg::film_persistent_kernel<true, A, N, false>), dim3(grid_blocks), dim3(ixg::kBlock), 0,     \
+                       stream, *P, d_planes, d_xyz, d_counter)
+#define IXG_PS(A, N)                                                                                                 \
+    hipLaunchKernelGGL((ixg::film_persistent_kernel<false, A, N, false>), dim3(grid_blocks), dim3(ixg::kBlock), 0,    \
+                       stream, *P, d_planes, d_xyz, d_counter)
+#define IXG_PIX(A, N)                                                                                                \
+    hipLaunchKernelGGL((ixg::film_persistent_kernel<true, A, N, true>), dim3(grid_blocks), dim3(ixg::kBlock), 0,      \
+                       stream, *P, d_planes, d_xyz, d_counter)
+#define IXG_PSX(A, N)                                                                                                \
+    hipLaunchKernelGGL((ixg::film_persistent_kernel<false, A, N, true>), dim3(grid_blocks), dim3(ixg::kBlock), 0,     \
+                       stream, *P, d_planes, d_xyz, d_counter)
+    if (extra) {
+        if (iso) IXG_DISPATCH2(IXG_PIX, P->absorb, P->an.on);
+        else IXG_DISPATCH2(IXG_PSX, P->absorb, P->an.on);
+    } else {
+        if (iso) IXG_DISPATCH2(IXG_PI, P->absorb, P->an.on);
+        else IXG_DISPATCH2(IXG_PS, P->absorb, P->an.on);
+    }
 #undef IXG_PI
 #undef IXG_PS
+#undef IXG_PIX
+#undef IXG_PSX
     return hipGetLastError();
 }
 
 extern "C" hipError_t ixg_persistent_grid(int device, uint32_t* blocks) {
     int per_cu = 0, cus = 0;
-    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, ixg::film_persistent_kernel<true, false, false>,
+    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, ixg::film_persistent_kernel<true, false, false, false>,
                                                                 ixg::kBlock, 0);
     if (e != hipSuccess) return e;
     e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device);

