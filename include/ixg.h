@@ -132,6 +132,17 @@ typedef struct ixg_environment {
     float head_shadow_deg; /* 16 = default, <= 0 = off                    */
 } ixg_environment;
 
+/* EnvironmentSource::HdrMap (IX/renderer/env_map.rs): an equirectangular
+ * panorama of linear RGB texels, rgb[(y*width + x)*3 + c], row 0 at the +Y
+ * pole, u = atan2(x, z) / 2 pi.  Seen without exposure, head shadow or white
+ * balance; each channel's radiance is rgb_to_spectral_radiance
+ * (IX/renderer/env_map_spectrum.rs:84) of the bilinear texel. */
+typedef struct ixg_hdr_map {
+    uint32_t width, height;
+    const float* rgb;
+} ixg_hdr_map;
+#define IXG_MAX_HDR_TEXELS (16384u * 8192u)
+
 /* GpuSceneRef (IX/renderer/gpu_backend/mod.rs:217-237). */
 typedef struct ixg_scene {
     ixg_camera camera;
@@ -144,6 +155,9 @@ typedef struct ixg_scene {
     /* FacetFinish per plane (GpuSceneRef::facet_finishes, IX/renderer/gpu_backend/
      * mod.rs:217-237): 0 polished, 1 frosted; NULL = every facet polished. */
     const uint8_t* facet_finishes;
+    /* The environment is this panorama instead of `environment`'s preset
+     * when non-NULL (EnvironmentSource::HdrMap). */
+    const ixg_hdr_map* hdr_map;
 } ixg_scene;
 
 typedef struct ixg_opts {

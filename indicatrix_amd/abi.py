@@ -84,7 +84,12 @@ class Scene(C.Structure):
                 ("material", C.POINTER(Material)),
                 ("max_bounces", C.c_uint32),
                 ("environment", Environment),
-                ("facet_finishes", C.POINTER(C.c_uint8))]
+                ("facet_finishes", C.POINTER(C.c_uint8)),
+                ("hdr_map", C.c_void_p)]
+
+
+class HdrMap(C.Structure):
+    _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("rgb", C.POINTER(C.c_float))]
 
 
 class BatchItem(C.Structure):

@@ -46,6 +46,8 @@ struct ixg_ctx {
     float4* d_planes = nullptr;
     float* d_xyz = nullptr;
     size_t xyz_cap = 0;  // floats
+    float* d_hdr = nullptr;  // the scene's HDR panorama texels
+    size_t hdr_cap = 0;      // floats
     double* d_accum = nullptr;
     size_t accum_cap = 0;  // doubles
     uint32_t* d_counter = nullptr;  // persistent kernel's path counter
@@ -195,6 +197,29 @@ ixg_status build_params(ixg_ctx* ctx, const ixg_scene* s, uint32_t sample_offset
     P->scat_g = s->material->scattering_g;
     float g = s->environment.surface_glare;
     P->surface_glare = (g >= 1.0f || std::isnan(g)) ? 1.0f : (g > 0.0f ? g : 0.0f);
+    if (s->hdr_map) {  // EnvironmentSource::HdrMap: no exposure, observer, tray or white balance
+        // the panorama first, copied in stream order, so the pointer below is
+        // the buffer this call's kernels read
+        const size_t n = (size_t)s->hdr_map->width * s->hdr_map->height * 3;
+        if (n > ctx->hdr_cap) {
+            if (ctx->d_hdr) (void)hipFree(ctx->d_hdr);
+            ctx->d_hdr = nullptr;
+            ctx->hdr_cap = 0;
+            HIP_TRY(ctx, hipMalloc(&ctx->d_hdr, n * sizeof(float)));
+            ctx->hdr_cap = n;
+        }
+        if (!ctx->d_hdr) return fail(ctx, IXG_INVALID, "no device buffer for the panorama");
+        HIP_TRY(ctx, hipMemcpyAsync(ctx->d_hdr, s->hdr_map->rgb, n * sizeof(float), hipMemcpyHostToDevice,
+                                    ctx->stream));
+        std::memset(&P->env, 0, sizeof P->env);
+        P->env.model = ixg::ENV_HDR;
+        P->env.exposure = 1.0f;
+        P->env.wb[0] = P->env.wb[1] = P->env.wb[2] = 1.0f;
+        P->env.hdr_w = s->hdr_map->width;
+        P->env.hdr_h = s->hdr_map->height;
+        P->env.hdr_rgb = ctx->d_hdr;
+        return IXG_OK;
+    }
     if (!ixg::resolve_environment(s->environment, &P->env))
         return fail(ctx, IXG_DECLINED, "environment not supported on the device path");
     return IXG_OK;
@@ -313,6 +338,7 @@ void ixg_destroy(ixg_ctx* ctx) {
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     if (ctx->metric_plan && ctx->metric_plan_free) ctx->metric_plan_free(ctx->metric_plan);
     if (ctx->d_planes) (void)hipFree(ctx->d_planes);
+    if (ctx->d_hdr) (void)hipFree(ctx->d_hdr);
     if (ctx->d_xyz) (void)hipFree(ctx->d_xyz);
     if (ctx->d_counter) (void)hipFree(ctx->d_counter);
     if (ctx->d_accum) (void)hipFree(ctx->d_accum);
@@ -358,6 +384,14 @@ ixg_route_reason ixg_route(const ixg_scene* s) {
     if (s->n_planes == 0 || s->n_planes > IXG_MAX_PLANES) return IXG_ROUTE_DEVICE_LIMITS;
     if (!material_routable(*s->material)) return IXG_ROUTE_UNSUPPORTED_MATERIAL;
     ixg::PresetInfo pi;
+    if (s->hdr_map) {
+        // a panorama: frosted facets and scatters would sample it by next-event estimation
+        const ixg_hdr_map& h = *s->hdr_map;
+        if (!h.rgb || h.width == 0 || h.height == 0 || (uint64_t)h.width * h.height > IXG_MAX_HDR_TEXELS)
+            return IXG_ROUTE_UNSUPPORTED_ENVIRONMENT;
+        if (any_frosted(s) || s->material->scattering_sigma_s > 0.0f) return IXG_ROUTE_UNSUPPORTED_ENVIRONMENT;
+        return IXG_ROUTE_OK;
+    }
     if (!ixg::preset_info(s->environment.preset, &pi)) return IXG_ROUTE_UNSUPPORTED_ENVIRONMENT;
     if (!std::isfinite(s->environment.backdrop)) return IXG_ROUTE_UNSUPPORTED_ENVIRONMENT;
     if (s->material->scattering_sigma_s > 0.0f && pi.model == ixg::ENV_DAYLIGHT_SUN)

@@ -246,6 +246,72 @@ __device__ __forceinline__ float lit_factor_unit(const EnvParams& E, V3 d, V3 ob
     return above + ground;
 }
 
+// EnvironmentMap::radiance_rgb (renderer/env_map.rs:303-417): the direction's
+// equirectangular (u, v), u wrapping and v clamped, bilinear over the four
+// neighbouring texels; the angles correctly rounded.
+__device__ __forceinline__ float rem_euclid1(float x, float m) {
+    const float r = fmodf(x, m);
+    return r < 0.0f ? r + fabsf(m) : r;
+}
+__device__ __forceinline__ void hdr_rgb(const EnvParams& E, V3 dir, float out[3]) {
+    const V3 d = normalize_or_zero(dir);
+    const float theta = (float)acos((double)clampf(d.y, -1.0f, 1.0f));
+    const float phi = (float)atan2((double)d.x, (double)d.z);
+    const float pi = 3.14159265358979323846f;
+    const float v = div_rn(theta, pi);
+    const float u = rem_euclid1(div_rn(phi, 2.0f * pi), 1.0f);
+    const float* px = E.hdr_rgb;
+    if (E.hdr_w == 1 && E.hdr_h == 1) {
+        out[0] = px[0]; out[1] = px[1]; out[2] = px[2];
+        return;
+    }
+    float fx = fma_(rem_euclid1(u, 1.0f), (float)E.hdr_w, -0.5f);
+    float fy = fma_(clampf(v, 0.0f, 1.0f), (float)E.hdr_h, -0.5f);
+    if (!(fx == fx)) fx = 0.0f;  // a non-finite direction reads a valid texel
+    if (!(fy == fy)) fy = 0.0f;
+    const float x0 = floorf(fx), y0 = floorf(fy);
+    const float tx = fx - x0, ty = fy - y0;
+    const int64_t w = E.hdr_w, h = E.hdr_h;
+    const int64_t xc = (int64_t)x0, yc = (int64_t)y0;
+    const int64_t x0i = ((xc % w) + w) % w, x1i = (((xc + 1) % w) + w) % w;
+    const int64_t y0i = yc < 0 ? 0 : (yc > h - 1 ? h - 1 : yc);
+    const int64_t y1i = yc + 1 < 0 ? 0 : (yc + 1 > h - 1 ? h - 1 : yc + 1);
+    const float* p00 = px + (y0i * w + x0i) * 3;
+    const float* p10 = px + (y0i * w + x1i) * 3;
+    const float* p01 = px + (y1i * w + x0i) * 3;
+    const float* p11 = px + (y1i * w + x1i) * 3;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        const float top = fma_(p10[c], tx, p00[c] * (1.0f - tx));
+        const float bottom = fma_(p11[c], tx, p01[c] * (1.0f - tx));
+        out[c] = fma_(bottom, ty, top * (1.0f - ty));
+    }
+}
+// rgb_to_spectral_radiance (renderer/env_map_spectrum.rs:84-125): the grey
+// part on three broad bumps, the chroma on three narrow ones.
+__device__ __forceinline__ float asym_gauss(float x, float mu, float lo, float hi) {
+    const float t = div_rn(x - mu, x < mu ? lo : hi);
+    return exp_rn(-0.5f * t * t);
+}
+__device__ __forceinline__ float rgb_to_spectral(const float rgb[3], float lambda) {
+    const float r = fmaxf(rgb[0], 0.0f), g = fmaxf(rgb[1], 0.0f), b = fmaxf(rgb[2], 0.0f);
+    const float w = fminf(fminf(r, g), b);
+    const float cr = r - w, cg = g - w, cb = b - w;
+    const float n_r = fma_(0.82398987f, w, fma_(-0.30506778f, w, 0.26391557f * w));
+    const float n_g = fma_(-0.2971578f, w, fma_(1.2528949f, w, -0.4611476f * w));
+    const float n_b = fma_(0.078837544f, w, fma_(-0.122806296f, w, 1.2317826f * w));
+    const float neutral = fma_(n_r, asym_gauss(lambda, 615.0f, 45.0f, 65.0f),
+                               fma_(n_g, asym_gauss(lambda, 545.0f, 45.0f, 45.0f),
+                                    n_b * asym_gauss(lambda, 465.0f, 40.0f, 45.0f)));
+    const float k_r = fma_(1.0995291f, cr, fma_(0.08711245f, cg, 0.01771266f * cb));
+    const float k_g = fma_(0.007455747f, cr, fma_(1.3345385f, cg, 0.014020192f * cb));
+    const float k_b = fma_(0.02487475f, cr, fma_(0.058899768f, cg, 1.2609048f * cb));
+    const float saturated = fma_(k_r, asym_gauss(lambda, 635.0f, 28.0f, 39.2f),
+                                 fma_(k_g, asym_gauss(lambda, 540.0f, 28.0f, 28.0f),
+                                      k_b * asym_gauss(lambda, 450.0f, 25.2f, 28.0f)));
+    return fmaxf(neutral + saturated, 0.0f);
+}
+
 // The ASET contrast view (rig.rs:146-176, 205-222, 439-459): green up to 45
 // degrees of exit elevation, red from 45 to 75, blue above, each lit by one
 // 20 nm FWHM band at its own gain; below the horizon a neutral D65 tray when

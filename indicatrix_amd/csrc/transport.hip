@@ -230,6 +230,11 @@ __device__ __forceinline__ void path_init(const FilmParams& P, V3 org, V3 dir, u
 
 __device__ __forceinline__ float env_radiance(const EnvParams& E, V3 dir, V3 observer, float lambda) {
     if (E.model == ENV_ASET) return aset_radiance(E, normalize(dir), lambda, E.exposure);
+    if (E.model == ENV_HDR) {
+        float rgb[3];
+        hdr_rgb(E, dir, rgb);
+        return rgb_to_spectral(rgb, lambda);
+    }
     if (E.model != ENV_STUDIO)
         return lit_factor(E, dir, observer) * (spec_power(E, lambda) * E.exposure);
     DirLight L;
@@ -958,7 +963,15 @@ __device__ __forceinline__ int path_bounce(const FilmParams& P, const float4* __
     }
     if (facet < 0) {
         // handle_miss (bounce_body/miss.rs:33-54)
-        if (!ISO && P.env.model == ENV_ASET) {
+        if (!ISO && P.env.model == ENV_HDR) {  // the panorama: one texel, per-channel spectrum
+            float rgb[3];
+            hdr_rgb(P.env, S.dir, rgb);
+#pragma unroll
+            for (int c = 0; c < NCH; ++c) {
+                float env = rgb_to_spectral(rgb, S.lambda[c]);
+                radiance[c] = fma_(fmaxf(S.st[c].x, 0.0f) * 1.0f, env, 0.0f);
+            }
+        } else if (!ISO && P.env.model == ENV_ASET) {
             const V3 d = normalize(S.dir);
 #pragma unroll
             for (int c = 0; c < NCH; ++c) {
@@ -1576,7 +1589,7 @@ extern "C" hipError_t ixg_launch_film_persistent(const ixg::FilmParams* P, const
     // a fill kernel rather than a memset node, so the launch pair can be captured
     hipLaunchKernelGGL(ixg::zero_counter_kernel, dim3(1), dim3(1), 0, stream, d_counter);
     // the lit models: one scaled factor per direction (ASET is per wavelength)
-    const bool iso = P->env.model != ixg::ENV_STUDIO && P->env.model != ixg::ENV_ASET;
+    const bool iso = P->env.model != ixg::ENV_STUDIO && P->env.model != ixg::ENV_ASET && P->env.model != ixg::ENV_HDR;
     const bool extra = P->frosted || P->sigma_s > 0.0f || P->edge_radius > 0.0f;
 #define IXG_PI(A, N)                                                                                                 \
     hipLaunchKernelGGL((ixg::film_persistent_kernel<true, A, N, false>), dim3(grid_blocks), dim3(ixg::kBlock), 0,     \

@@ -15,7 +15,8 @@ enum EnvModel : int32_t {
     ENV_LIGHT_TENT = 2,
     ENV_DAYLIGHT_DOME = 3,
     ENV_ASET = 4,
-    ENV_DAYLIGHT_SUN = 5
+    ENV_DAYLIGHT_SUN = 5,
+    ENV_HDR = 6  // EnvironmentSource::HdrMap
 };
 
 struct EnvParams {
@@ -34,6 +35,8 @@ struct EnvParams {
     float tray;          // lit backdrop radiance at exposure 1 (0 = the model's own ground)
     float tent_walls, tent_cards, tent_spark, tent_ground, tent_flat;  // TentParams
     float sun;           // DaylightSun: the disc's radiance factor (0 below the horizon)
+    uint32_t hdr_w, hdr_h;  // ENV_HDR: the panorama and its device texels
+    const float* hdr_rgb;
 };
 
 struct AbsBand {

@@ -194,6 +194,21 @@ class FrameScene:
         self.scene.environment = environment
         self.set_finishes(facet_finishes)
 
+    def set_hdr_map(self, rgb):
+        """Sets the environment to an equirectangular panorama ((h, w, 3)
+        linear RGB), or back to the preset with None."""
+        if rgb is None:
+            self.hdr = None
+            self.scene.hdr_map = None
+            return
+        import numpy as np
+        arr = np.ascontiguousarray(rgb, np.float32)
+        if arr.ndim != 3 or arr.shape[2] != 3:
+            raise ValueError("rgb must be (height, width, 3)")
+        self._hdr_pixels = arr
+        self.hdr = abi.HdrMap(arr.shape[1], arr.shape[0], arr.ctypes.data_as(C.POINTER(C.c_float)))
+        self.scene.hdr_map = C.cast(C.pointer(self.hdr), C.c_void_p)
+
     def set_finishes(self, facet_finishes):
         """Sets the FacetFinish of every plane (None: all polished)."""
         if facet_finishes is None:

@@ -736,6 +736,7 @@ typedef struct {
     v3 c_raw;           /* material c_axis as given */
     v3 ta1, ta2, tcn;   /* AbsorptionTensor3 axes: basis of normalize_or_zero(c_axis) */
     const uint8_t* finish; /* FacetFinish per plane, NULL = all polished */
+    const ixg_hdr_map* hdr; /* EnvironmentSource::HdrMap, or NULL */
 } scene_t;
 
 typedef struct {
@@ -923,7 +924,71 @@ static float tir_phase_delta(float n1k, float cos_i, float sin_i) { /* refractio
     return 2.0f * atanf(tan_half);
 }
 
+/* EnvironmentMap::radiance_rgb (renderer/env_map.rs:303-417), the angles
+ * correctly rounded. */
+static float rem_euclid_f(float x, float m) {
+    float r = fmodf(x, m);
+    return r < 0.0f ? r + fabsf(m) : r;
+}
+static void hdr_rgb(const ixg_hdr_map* H, v3 dir, float out[3]) {
+    const v3 d = vnormalize_or_zero(dir);
+    const float theta = (float)acos((double)clampf(d.y, -1.0f, 1.0f));
+    const float phi = (float)atan2((double)d.x, (double)d.z);
+    const float v = theta / F32_PI;
+    const float u = rem_euclid_f(phi / (2.0f * F32_PI), 1.0f);
+    if (H->width == 1 && H->height == 1) {
+        out[0] = H->rgb[0]; out[1] = H->rgb[1]; out[2] = H->rgb[2];
+        return;
+    }
+    const float fx = fmaf(rem_euclid_f(u, 1.0f), (float)H->width, -0.5f);
+    const float fy = fmaf(clampf(v, 0.0f, 1.0f), (float)H->height, -0.5f);
+    const float x0 = floorf(fx), y0 = floorf(fy);
+    const float tx = fx - x0, ty = fy - y0;
+    const int64_t w = H->width, h = H->height, xc = (int64_t)x0, yc = (int64_t)y0;
+    const int64_t x0i = ((xc % w) + w) % w, x1i = (((xc + 1) % w) + w) % w;
+    const int64_t y0i = yc < 0 ? 0 : (yc > h - 1 ? h - 1 : yc);
+    const int64_t y1i = yc + 1 < 0 ? 0 : (yc + 1 > h - 1 ? h - 1 : yc + 1);
+    const float* p00 = H->rgb + (y0i * w + x0i) * 3;
+    const float* p10 = H->rgb + (y0i * w + x1i) * 3;
+    const float* p01 = H->rgb + (y1i * w + x0i) * 3;
+    const float* p11 = H->rgb + (y1i * w + x1i) * 3;
+    for (int c = 0; c < 3; ++c) {
+        const float top = fmaf(p10[c], tx, p00[c] * (1.0f - tx));
+        const float bottom = fmaf(p11[c], tx, p01[c] * (1.0f - tx));
+        out[c] = fmaf(bottom, ty, top * (1.0f - ty));
+    }
+}
+/* rgb_to_spectral_radiance (renderer/env_map_spectrum.rs:84-125). */
+static float asym_gauss(float x, float mu, float lo, float hi) {
+    const float t = (x - mu) / (x < mu ? lo : hi);
+    return expf(-0.5f * t * t);
+}
+static const float NEUTRAL_TO_BUMP[3][3] = {{0.82398987f, -0.30506778f, 0.26391557f},
+                                            {-0.2971578f, 1.2528949f, -0.4611476f},
+                                            {0.078837544f, -0.122806296f, 1.2317826f}};
+static const float CHROMA_TO_BUMP[3][3] = {{1.0995291f, 0.08711245f, 0.01771266f},
+                                           {0.007455747f, 1.3345385f, 0.014020192f},
+                                           {0.02487475f, 0.058899768f, 1.2609048f}};
+static float row_dot(const float m[3], const float v[3]) { return fmaf(m[0], v[0], fmaf(m[1], v[1], m[2] * v[2])); }
+float ixo_rgb_to_spectral(const float rgb_in[3], float lambda) {
+    const float r = maxf(rgb_in[0], 0.0f), g = maxf(rgb_in[1], 0.0f), b = maxf(rgb_in[2], 0.0f);
+    const float w = minf(minf(r, g), b);
+    const float chroma[3] = {r - w, g - w, b - w}, grey[3] = {w, w, w};
+    const float neutral = fmaf(row_dot(NEUTRAL_TO_BUMP[0], grey), asym_gauss(lambda, 615.0f, 45.0f, 65.0f),
+                               fmaf(row_dot(NEUTRAL_TO_BUMP[1], grey), asym_gauss(lambda, 545.0f, 45.0f, 45.0f),
+                                    row_dot(NEUTRAL_TO_BUMP[2], grey) * asym_gauss(lambda, 465.0f, 40.0f, 45.0f)));
+    const float saturated = fmaf(row_dot(CHROMA_TO_BUMP[0], chroma), asym_gauss(lambda, 635.0f, 28.0f, 39.2f),
+                                 fmaf(row_dot(CHROMA_TO_BUMP[1], chroma), asym_gauss(lambda, 540.0f, 28.0f, 28.0f),
+                                      row_dot(CHROMA_TO_BUMP[2], chroma) * asym_gauss(lambda, 450.0f, 25.2f, 28.0f)));
+    return maxf(neutral + saturated, 0.0f);
+}
+
 static float env_channel(const scene_t* S, v3 dir, float lambda, v3 observer) {
+    if (S->hdr) {
+        float rgb[3];
+        hdr_rgb(S->hdr, dir, rgb);
+        return ixo_rgb_to_spectral(rgb, lambda);
+    }
     dir_light L = direction_lighting(&S->env, dir, observer);
     return light_radiance(&L, spec_power(&S->env, lambda), S->env.exposure);
 }
@@ -1769,6 +1834,14 @@ static int scene_build(const ixg_scene* sc, scene_t* S) {
     S->c_raw = vfrom(sc->material->c_axis);
     S->tcn = vnormalize_or_zero(S->c_raw);
     stable_basis(S->tcn, &S->ta1, &S->ta2);
+    S->hdr = sc->hdr_map;
+    if (S->hdr) { /* the panorama: no exposure, observer, tray or white balance */
+        memset(&S->env, 0, sizeof S->env);
+        S->env.exposure = 1.0f;
+        S->env.wb = mk(1.0f, 1.0f, 1.0f);
+        S->env.model = -1;
+        return 1;
+    }
     return env_build(&sc->environment, &S->env);
 }
 
@@ -1798,10 +1871,18 @@ static void trace_core(const scene_t* S, v3 origin, v3 dir, uint32_t seed, float
         float o[3] = {P.org.x, P.org.y, P.org.z}, d[3] = {P.dir.x, P.dir.y, P.dir.z}, t;
         int32_t f = ixo_intersect(S->planes, S->n_planes, o, d, &t);
         if (f < 0) { /* handle_miss (bounce_body/miss.rs:33-54) */
-            dir_light L = direction_lighting(&S->env, P.dir, P.observer);
-            for (int c = 0; c < NCH; ++c) {
-                float env = light_radiance(&L, spec_power(&S->env, P.lambdas[c]), S->env.exposure);
-                P.radiance[c] = fmaf(st_intensity(P.st[c]) * 1.0f, env, P.radiance[c]);
+            if (S->hdr) { /* sample_environment_channels' HdrMap arm (rig.rs:756-765) */
+                float rgb[3];
+                hdr_rgb(S->hdr, P.dir, rgb);
+                for (int c = 0; c < NCH; ++c)
+                    P.radiance[c] = fmaf(st_intensity(P.st[c]) * 1.0f, ixo_rgb_to_spectral(rgb, P.lambdas[c]),
+                                         P.radiance[c]);
+            } else {
+                dir_light L = direction_lighting(&S->env, P.dir, P.observer);
+                for (int c = 0; c < NCH; ++c) {
+                    float env = light_radiance(&L, spec_power(&S->env, P.lambdas[c]), S->env.exposure);
+                    P.radiance[c] = fmaf(st_intensity(P.st[c]) * 1.0f, env, P.radiance[c]);
+                }
             }
             P.escaped = 1;
             if (ev) { ev->termination = 0; ev->bounces = bounce + 1; }

@@ -51,6 +51,9 @@ void ixo_cosine_hemisphere(float u1, float u2, const float n[3], float out[3]);
 /* sample_henyey_greenstein_direction (RT/scattering/hg.rs:636) about unit fwd. */
 void ixo_hg_direction(float u1, float u2, float g, const float fwd[3], float out[3]);
 
+/* rgb_to_spectral_radiance (IX/renderer/env_map_spectrum.rs:84). */
+float ixo_rgb_to_spectral(const float rgb[3], float lambda_nm);
+
 /* Camera (RT/camera.rs). */
 void ixo_camera_new(float yaw, float pitch, float distance, float fov_deg, uint32_t width,
                     uint32_t height, uint32_t spp, ixg_camera* out);

@@ -86,6 +86,8 @@ def lib():
                                                    P(C.c_float), C.c_float, P(C.c_float)]
         L.ixo_cosine_hemisphere.argtypes = [C.c_float, C.c_float, P(C.c_float), P(C.c_float)]
         L.ixo_hg_direction.argtypes = [C.c_float, C.c_float, C.c_float, P(C.c_float), P(C.c_float)]
+        L.ixo_rgb_to_spectral.restype = C.c_float
+        L.ixo_rgb_to_spectral.argtypes = [P(C.c_float), C.c_float]
         L.ixo_asc_planes.restype = C.c_uint32
         L.ixo_asc_planes.argtypes = [P(C.c_double), P(C.c_double), P(C.c_uint32), C.c_uint32, P(C.c_double),
                                      C.c_int32, C.c_double, P(abi.FacetPlane), C.c_uint32]
