@@ -36,8 +36,8 @@ typedef enum ixg_status {
  * IX/renderer/gpu_backend/mod.rs:287-298). */
 typedef enum ixg_route_reason {
     IXG_ROUTE_OK = 0,
-    IXG_ROUTE_UNSUPPORTED_MATERIAL = 1,    /* biaxial, scattering, zoned                */
-    IXG_ROUTE_UNSUPPORTED_ENVIRONMENT = 2, /* HDR maps, UV lamps                          */
+    IXG_ROUTE_UNSUPPORTED_MATERIAL = 1,    /* biaxial, zoned, scattering uniaxial       */
+    IXG_ROUTE_UNSUPPORTED_ENVIRONMENT = 2, /* UV lamps; frosted/scatter under sun or HDR */
     IXG_ROUTE_DEVICE_LIMITS = 3,           /* too many planes for the LDS facet set     */
     IXG_ROUTE_NO_DEVICE = 4
 } ixg_route_reason;
@@ -513,6 +513,68 @@ uint32_t ixg_cut_standard_round_brilliant(ixg_facet_plane* out /* >= 74 */);
 /* 57-facet round brilliant from crown/pavilion angles (deg) and table ratio. */
 uint32_t ixg_cut_round_brilliant(float crown_deg, float pavilion_deg, float table_ratio,
                                  ixg_facet_plane* out /* >= 74 */);
+
+/* ---- radiance payloads: the body of a FRAME / PREVIEW message ----
+ * indicatrix-net/src/radiance: width*height pixels of three little-endian f32
+ * (XYZ running sums, 12 B each, radiance/mod.rs:43); a compressed encoding
+ * first byte-shuffles the 4n bytes into 4 planes (radiance/shuffle.rs). */
+#define IXG_MAX_FRAME_LEN (512u * 1024u * 1024u) /* framing/mod.rs:26 */
+typedef enum ixg_payload_encoding { /* PayloadEncoding, messages/encoding.rs:43 */
+    IXG_PAYLOAD_RAW = 0,
+    IXG_PAYLOAD_SHUFFLE_ZSTD = 1,
+    IXG_PAYLOAD_SHUFFLE_LZ4 = 2
+} ixg_payload_encoding;
+typedef enum ixg_radiance_error_kind { /* RadianceError, radiance/mod.rs:66-119 */
+    IXG_RAD_OK = 0,
+    IXG_RAD_LENGTH_MISMATCH = 1,
+    IXG_RAD_MISALIGNED = 2,
+    IXG_RAD_TOO_LARGE = 3,
+    IXG_RAD_RAW_LEN_MISMATCH = 4,
+    IXG_RAD_UNSUPPORTED_ENCODING = 5,
+    IXG_RAD_DECOMPRESS_FAILED = 6,
+    IXG_RAD_SHORT_OUTPUT = 7,
+    IXG_RAD_FRAME_OUTSIDE_RANGE = 8,
+    IXG_RAD_SAMPLE_COUNT_OVERRUN = 9,
+    IXG_RAD_ZERO_SAMPLE_FRAME = 10
+} ixg_radiance_error_kind;
+/* The variant's fields where it has them; message = its Display text. */
+typedef struct ixg_radiance_error {
+    int32_t kind;
+    int32_t encoding;
+    uint32_t width, height;
+    uint32_t raw_len;
+    uint32_t first_sample, samples, start, end;
+    uint32_t cumulative, range_len;
+    uint32_t pad;
+    uint64_t expected_bytes, got_bytes;
+    char message[256];
+} ixg_radiance_error;
+/* shuffle_into / unshuffle_into (radiance/shuffle.rs:35,60) on device
+ * buffers: n_bytes a multiple of 4 (IXG_INVALID otherwise, where the
+ * reference panics), at most IXG_MAX_FRAME_LEN, src and dst distinct. */
+ixg_status ixg_radiance_shuffle_device(ixg_ctx* ctx, const void* d_src, uint64_t n_bytes, void* d_dst);
+ixg_status ixg_radiance_unshuffle_device(ixg_ctx* ctx, const void* d_src, uint64_t n_bytes, void* d_dst);
+/* The summing arms of PayloadDecoder::decode_and_add (radiance/payload.rs:225-250):
+ * adds the payload (shuffled = 0: raw Vec3 bytes, 4-byte aligned; 1: the
+ * planes, add_unshuffled_valid, shuffle.rs:122) onto d_acc[pixels*3] f32,
+ * skipping every pixel that fails is_valid_sample, and writes the number
+ * skipped to *d_dropped (device u32). */
+ixg_status ixg_radiance_add_device(ixg_ctx* ctx, const void* d_payload, int32_t shuffled, uint64_t pixels,
+                                   float* d_acc, uint32_t* d_dropped);
+/* PayloadDecoder::prepare's header checks (radiance/payload.rs:282-320) in
+ * their order: TooLarge, RawLenMismatch, then for Raw LengthMismatch of the
+ * wire bytes, for a compressed encoding UnsupportedEncoding unless
+ * compression != 0 (this library carries no zstd / LZ4 codec: a caller that
+ * decompresses itself passes 1 and then hands the planes to
+ * ixg_radiance_add_device).  IXG_OK or IXG_INVALID with err filled. */
+ixg_status ixg_radiance_check_payload(int32_t encoding, uint32_t raw_len, uint64_t wire_len, uint32_t width,
+                                      uint32_t height, int32_t compression, ixg_radiance_error* err);
+/* Accumulator::apply_frame's admission checks (client/accumulate/mod.rs:414-465):
+ * ZeroSampleFrame, then with a requested range [start, end) FrameOutsideRange
+ * (frame_within_range, overflow-safe) and SampleCountOverrun of
+ * samples_done + samples (saturating) against end - start. */
+ixg_status ixg_radiance_check_frame(uint32_t first_sample, uint32_t samples, int32_t has_range, uint32_t start,
+                                    uint32_t end, uint32_t samples_done, ixg_radiance_error* err);
 
 #ifdef __cplusplus
 }

@@ -61,6 +61,14 @@ def _declare(lib):
         "ixg_asc_parse_text": (st, [C.c_char_p, C.c_size_t, P(C.c_void_p), P(abi.AscError)]),
         "ixg_asc_parse_bytes": (st, [C.c_char_p, C.c_size_t, P(C.c_void_p), P(abi.AscError)]),
         "ixg_asc_free": (None, [C.c_void_p]),
+        "ixg_radiance_shuffle_device": (st, [C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p]),
+        "ixg_radiance_unshuffle_device": (st, [C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p]),
+        "ixg_radiance_add_device": (st, [C.c_void_p, C.c_void_p, C.c_int32, C.c_uint64, C.c_void_p,
+                                         C.c_void_p]),
+        "ixg_radiance_check_payload": (st, [C.c_int32, C.c_uint32, C.c_uint64, C.c_uint32, C.c_uint32,
+                                            C.c_int32, P(abi.RadianceError)]),
+        "ixg_radiance_check_frame": (st, [C.c_uint32, C.c_uint32, C.c_int32, C.c_uint32, C.c_uint32,
+                                          C.c_uint32, P(abi.RadianceError)]),
         "ixg_asc_info_get": (st, [C.c_void_p, P(abi.AscInfo)]),
         "ixg_asc_tier_get": (st, [C.c_void_p, C.c_uint32, P(abi.AscTier), P(C.c_double), C.c_uint32]),
         "ixg_asc_text": (C.c_size_t, [C.c_void_p, C.c_int32, C.c_uint32, C.c_char_p, C.c_size_t]),

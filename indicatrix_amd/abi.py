@@ -189,3 +189,22 @@ assert C.sizeof(OpticalPixel) == 56
 assert C.sizeof(AscError) == 696
 assert C.sizeof(AscInfo) == 56
 assert C.sizeof(AscTier) == 24
+
+
+# ---- radiance payloads (indicatrix-net/src/radiance) ----
+MAX_FRAME_LEN = 512 * 1024 * 1024
+PAYLOAD_RAW, PAYLOAD_SHUFFLE_ZSTD, PAYLOAD_SHUFFLE_LZ4 = 0, 1, 2
+RADIANCE_ERROR_KINDS = {
+    1: "LengthMismatch", 2: "Misaligned", 3: "TooLarge", 4: "RawLenMismatch",
+    5: "UnsupportedEncoding", 6: "DecompressFailed", 7: "ShortOutput",
+    8: "FrameOutsideRange", 9: "SampleCountOverrun", 10: "ZeroSampleFrame",
+}
+
+
+class RadianceError(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("encoding", C.c_int32), ("width", C.c_uint32),
+                ("height", C.c_uint32), ("raw_len", C.c_uint32), ("first_sample", C.c_uint32),
+                ("samples", C.c_uint32), ("start", C.c_uint32), ("end", C.c_uint32),
+                ("cumulative", C.c_uint32), ("range_len", C.c_uint32), ("pad", C.c_uint32),
+                ("expected_bytes", C.c_uint64), ("got_bytes", C.c_uint64),
+                ("message", C.c_char * 256)]
