@@ -235,6 +235,13 @@ ixg_status ixg_accumulate_device(ixg_ctx* ctx, const ixg_scene* scene, uint32_t 
  * order.  Either output may be NULL. */
 ixg_status ixg_trace_paths(ixg_ctx* ctx, const ixg_scene* scene, uint32_t sample_offset,
                            uint32_t spp, float* out_xyz, ixg_path_events* events);
+/* Diagnostic: the per-path XYZ that the film kernel ixg_accumulate runs
+ * (the persistent, feature-specialised instantiation the timed path
+ * launches) writes before its per-pixel reduction, in (pixel, sample)
+ * order, out_xyz[3 * (pixel * spp + s)]; at most 2^26 paths.  The same
+ * paths as ixg_trace_paths, from the production kernel. */
+ixg_status ixg_film_samples(ixg_ctx* ctx, const ixg_scene* scene, uint32_t sample_offset, uint32_t spp,
+                            float* out_xyz);
 /* trace_spectral_ray (RT/transport/mod.rs:118) for explicit rays, the entry
  * the reference's golden tables pin: rays[8*i..] = {origin xyz, dir xyz,
  * hero_rand, rng_seed bits as float}.  The scene's camera and frame size are
@@ -307,6 +314,29 @@ typedef struct ixg_metric_totals {
  * a pose), IXG_DECLINED (environment without a device lighting model). */
 ixg_status ixg_metric_image(ixg_ctx* ctx, const ixg_metric_request* req, ixg_optical_pixel* pixels,
                             ixg_metric_totals* totals);
+
+/* The per-pixel scintillation maps of a finished image (the pixels an
+ * ixg_metric_image call returned): scintillation_spatial
+ * (IX/color/metrics/image_scint.rs:175-199) into spatial[n_pixels] -- the
+ * squashed CV of the brilliance fraction over the 5x5 window of hit pixels,
+ * 0..100, 0 where hits == 0 and everywhere when width*height != n_pixels --
+ * and scintillation_index (image_scint.rs:234-247) into index[n_pixels] --
+ * 0.6 spatial + 0.4 of the five-pose Bernoulli variance over 0.24, NaN where
+ * the temporal term was not computed (None).  Host buffers; either output
+ * may be NULL.  Computed on the device, equal to the CPU pass bit for bit. */
+ixg_status ixg_scintillation_maps(ixg_ctx* ctx, const ixg_optical_pixel* pixels, uint32_t n_pixels, uint32_t width,
+                                  uint32_t height, float* spatial, float* index);
+/* The same on device buffers (no copies), synchronised before it returns. */
+ixg_status ixg_scintillation_maps_device(ixg_ctx* ctx, const ixg_optical_pixel* d_pixels, uint32_t n_pixels,
+                                         uint32_t width, uint32_t height, float* d_spatial, float* d_index);
+/* The hover read-outs on host pixels, equal to the full maps bit for bit:
+ * scintillation_spatial_at (image_scint.rs:201-232) and
+ * scintillation_index_at (image_scint.rs:249-259), which returns 0 (None)
+ * or 1 with *out set. */
+float ixg_scintillation_spatial_at(const ixg_optical_pixel* pixels, uint32_t n_pixels, uint32_t width,
+                                   uint32_t height, uint32_t i);
+int ixg_scintillation_index_at(const ixg_optical_pixel* pixels, uint32_t n_pixels, uint32_t width, uint32_t height,
+                               uint32_t i, float* out);
 
 /* One design of a batched metric sweep: its facet set inside a shared plane
  * table and its dispersion curve. */
@@ -524,6 +554,10 @@ typedef enum ixg_payload_encoding { /* PayloadEncoding, messages/encoding.rs:43 
     IXG_PAYLOAD_SHUFFLE_ZSTD = 1,
     IXG_PAYLOAD_SHUFFLE_LZ4 = 2
 } ixg_payload_encoding;
+/* PayloadEncoding::ShuffleZstd { level } as one int32: the family in bits
+ * 0-7, the encoder's u8 level in bits 8-15 (zero for the other arms). */
+#define IXG_PAYLOAD_ZSTD(level) ((int32_t)IXG_PAYLOAD_SHUFFLE_ZSTD | (int32_t)(((level) & 0xFF) << 8))
+#define IXG_PAYLOAD_ZSTD_LEVEL_OF(encoding) (((encoding) >> 8) & 0xFF)
 typedef enum ixg_radiance_error_kind { /* RadianceError, radiance/mod.rs:66-119 */
     IXG_RAD_OK = 0,
     IXG_RAD_LENGTH_MISMATCH = 1,
@@ -562,7 +596,8 @@ ixg_status ixg_radiance_unshuffle_device(ixg_ctx* ctx, const void* d_src, uint64
 ixg_status ixg_radiance_add_device(ixg_ctx* ctx, const void* d_payload, int32_t shuffled, uint64_t pixels,
                                    float* d_acc, uint32_t* d_dropped);
 /* PayloadDecoder::prepare's header checks (radiance/payload.rs:282-320) in
- * their order: TooLarge, RawLenMismatch, then for Raw LengthMismatch of the
+ * their order (encoding: IXG_PAYLOAD_RAW, IXG_PAYLOAD_ZSTD(level) or
+ * IXG_PAYLOAD_SHUFFLE_LZ4): TooLarge, RawLenMismatch, then for Raw LengthMismatch of the
  * wire bytes, for a compressed encoding UnsupportedEncoding unless
  * compression != 0 (this library carries no zstd / LZ4 codec: a caller that
  * decompresses itself passes 1 and then hands the planes to

@@ -42,6 +42,7 @@ def _declare(lib):
                                       abi.BatchDoneFn, C.c_void_p]),
         "ixg_trace_paths": (st, [C.c_void_p, P(abi.Scene), C.c_uint32, C.c_uint32,
                                  P(C.c_float), P(abi.PathEvents)]),
+        "ixg_film_samples": (st, [C.c_void_p, P(abi.Scene), C.c_uint32, C.c_uint32, P(C.c_float)]),
         "ixg_trace_rays": (st, [C.c_void_p, P(abi.Scene), P(C.c_float), C.c_uint32,
                                 P(C.c_float), P(abi.PathEvents)]),
         "ixg_count_events": (st, [C.c_void_p, P(abi.Scene), C.c_uint32, C.c_uint32,
@@ -51,6 +52,14 @@ def _declare(lib):
         "ixg_metric_sweep": (st, [C.c_void_p, P(abi.FacetPlane), C.c_uint32, P(abi.Design),
                                   C.c_uint32, P(C.c_float), C.c_uint32, C.c_uint32, C.c_uint32,
                                   C.c_uint32, C.c_float, P(abi.Environment), P(C.c_float)]),
+        "ixg_scintillation_maps": (st, [C.c_void_p, P(abi.OpticalPixel), C.c_uint32, C.c_uint32, C.c_uint32,
+                                        P(C.c_float), P(C.c_float)]),
+        "ixg_scintillation_maps_device": (st, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32,
+                                               C.c_void_p, C.c_void_p]),
+        "ixg_scintillation_spatial_at": (C.c_float, [P(abi.OpticalPixel), C.c_uint32, C.c_uint32, C.c_uint32,
+                                                     C.c_uint32]),
+        "ixg_scintillation_index_at": (C.c_int, [P(abi.OpticalPixel), C.c_uint32, C.c_uint32, C.c_uint32,
+                                                 C.c_uint32, P(C.c_float)]),
         "ixg_set_metric_mode": (st, [C.c_void_p, C.c_int, C.c_int, C.c_int]),
         "ixg_metric_tuning": (st, [C.c_void_p, P(C.c_int), P(C.c_int), P(C.c_uint32), P(C.c_int)]),
         "ixg_model_width": (st, [P(abi.FacetPlane), C.c_uint32, P(C.c_double)]),

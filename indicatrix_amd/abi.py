@@ -194,6 +194,13 @@ assert C.sizeof(AscTier) == 24
 # ---- radiance payloads (indicatrix-net/src/radiance) ----
 MAX_FRAME_LEN = 512 * 1024 * 1024
 PAYLOAD_RAW, PAYLOAD_SHUFFLE_ZSTD, PAYLOAD_SHUFFLE_LZ4 = 0, 1, 2
+
+
+def payload_zstd(level: int) -> int:
+    """PayloadEncoding::ShuffleZstd { level } as the ABI's int32
+    (IXG_PAYLOAD_ZSTD: the family in bits 0-7, the u8 level in bits 8-15)."""
+    return PAYLOAD_SHUFFLE_ZSTD | ((level & 0xFF) << 8)
+
 RADIANCE_ERROR_KINDS = {
     1: "LengthMismatch", 2: "Misaligned", 3: "TooLarge", 4: "RawLenMismatch",
     5: "UnsupportedEncoding", 6: "DecompressFailed", 7: "ShortOutput",

@@ -20,8 +20,6 @@
 #include "host_scene.h"
 #include "transport_params.h"
 
-extern "C" hipError_t ixg_launch_film(const ixg::FilmParams* P, const float4* d_planes, float* d_xyz,
-                                      hipStream_t stream);
 extern "C" hipError_t ixg_launch_film_persistent(const ixg::FilmParams* P, const float4* d_planes, float* d_xyz,
                                                  uint32_t* d_counter, uint32_t grid_blocks, hipStream_t stream);
 extern "C" hipError_t ixg_persistent_grid(int device, uint32_t* blocks);
@@ -583,6 +581,39 @@ ixg_status ixg_trace_paths(ixg_ctx* ctx, const ixg_scene* s, uint32_t sample_off
     (void)hipFree(d_xyz);
     (void)hipFree(d_ev);
     if (e != hipSuccess) return hip_fail(ctx, e, "trace_paths");
+    return IXG_OK;
+}
+
+ixg_status ixg_film_samples(ixg_ctx* ctx, const ixg_scene* s, uint32_t sample_offset, uint32_t spp,
+                            float* out_xyz) {
+    ixg_status st = begin_call(ctx, s);
+    if (st != IXG_OK) return st;
+    if (!out_xyz) return fail(ctx, IXG_INVALID, "null out_xyz");
+    const uint64_t paths = (uint64_t)s->width * s->height * spp;
+    if (paths == 0) return IXG_OK;
+    if (paths > (1ull << 26)) return fail(ctx, IXG_RESOURCE_LIMIT, "diagnostic samples limited to 2^26 paths");
+    ixg::FilmParams P;
+    st = build_params(ctx, s, sample_offset, spp, &P);
+    if (st != IXG_OK) return st;
+    st = upload_planes(ctx, s);
+    if (st != IXG_OK) return st;
+    // the chunking and launch of run_film, with each chunk's path records
+    // copied out instead of reduced
+    const uint32_t n_pixels = s->width * s->height;
+    uint64_t pix_per_chunk = std::max<uint64_t>(1, ctx->chunk_paths / spp);
+    pix_per_chunk = std::min<uint64_t>(pix_per_chunk, n_pixels);
+    st = ensure_xyz(ctx, (size_t)pix_per_chunk * spp * 3);
+    if (st != IXG_OK) return st;
+    for (uint64_t first = 0; first < n_pixels; first += pix_per_chunk) {
+        const uint32_t npx = (uint32_t)std::min<uint64_t>(pix_per_chunk, n_pixels - first);
+        P.first_path = first * spp;
+        P.n_paths = (uint64_t)npx * spp;
+        HIP_TRY(ctx, ixg_launch_film_persistent(&P, ctx->d_planes, ctx->d_xyz, ctx->d_counter,
+                                                ctx->persistent_blocks, ctx->stream));
+        HIP_TRY(ctx, hipMemcpyAsync(out_xyz + 3 * P.first_path, ctx->d_xyz, P.n_paths * 3 * sizeof(float),
+                                    hipMemcpyDeviceToHost, ctx->stream));
+    }
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
     return IXG_OK;
 }
 

@@ -21,6 +21,9 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <memory>
+#include <new>
+#include <stdexcept>
 #include <string>
 #include <string_view>
 #include <utility>
@@ -662,17 +665,32 @@ void report(const AscError& e, ixg_asc_error* err) {
 }
 
 ixg_status parse_into(const std::string& text, ixg_asc** out, ixg_asc_error* err) {
-    ixg_asc* s = new ixg_asc();
+    std::unique_ptr<ixg_asc> s(new ixg_asc());
     try {
-        parse_text(text, s);
+        parse_text(text, s.get());
     } catch (const AscError& e) {
-        delete s;
         report(e, err);
         return IXG_INVALID;
     }
     if (err) std::memset(err, 0, sizeof *err);
-    *out = s;
+    *out = s.release();
     return IXG_OK;
+}
+
+// No C++ exception crosses the C ABI: an allocation failure (a huge or
+// hostile file) is a resource limit, anything else unexpected an invalid
+// input, and the caller's process keeps running.
+template <class F>
+ixg_status guarded(F&& body) {
+    try {
+        return body();
+    } catch (const std::bad_alloc&) {
+        return IXG_RESOURCE_LIMIT;
+    } catch (const std::length_error&) {
+        return IXG_RESOURCE_LIMIT;
+    } catch (...) {
+        return IXG_INVALID;
+    }
 }
 
 size_t copy_text(const std::string& s, char* buf, size_t cap) {
@@ -730,14 +748,14 @@ ixg_status ixg_asc_parse_text(const char* text, size_t len, ixg_asc** out, ixg_a
         report(e, err);
         return IXG_INVALID;
     }
-    return parse_into(std::string(text ? text : "", len), out, err);
+    return guarded([&] { return parse_into(std::string(text ? text : "", len), out, err); });
 }
 
 ixg_status ixg_asc_parse_bytes(const uint8_t* bytes, size_t len, ixg_asc** out, ixg_asc_error* err) {
     if (err) std::memset(err, 0, sizeof *err);
     if (!out || (!bytes && len)) return IXG_INVALID;
     *out = nullptr;
-    return parse_into(decode_bytes(bytes, len), out, err);
+    return guarded([&] { return parse_into(decode_bytes(bytes, len), out, err); });
 }
 
 void ixg_asc_free(ixg_asc* s) { delete s; }
@@ -799,10 +817,14 @@ size_t ixg_asc_index_name(const ixg_asc* s, uint32_t tier, uint32_t k, uint32_t*
     return copy_text(e.second, buf, cap);
 }
 
-ixg_status ixg_asc_schedule_planes(const ixg_asc_tier* tiers, uint32_t n_tiers, const double* indices,
-                                   uint32_t n_indices, int32_t gear_teeth, double gear_reference_angle,
-                                   ixg_facet_plane* out, uint32_t cap, uint32_t* n_out, uint32_t* raw_to_plane,
-                                   uint32_t* tier_ends) {
+}  // extern "C"
+
+namespace {
+
+ixg_status schedule_planes(const ixg_asc_tier* tiers, uint32_t n_tiers, const double* indices,
+                           uint32_t n_indices, int32_t gear_teeth, double gear_reference_angle,
+                           ixg_facet_plane* out, uint32_t cap, uint32_t* n_out, uint32_t* raw_to_plane,
+                           uint32_t* tier_ends) {
     if (n_out) *n_out = 0;
     if (!n_out || (n_tiers && !tiers)) return IXG_INVALID;
     uint64_t total = 0;
@@ -869,9 +891,7 @@ ixg_status ixg_asc_schedule_planes(const ixg_asc_tier* tiers, uint32_t n_tiers, 
     return IXG_OK;
 }
 
-ixg_status ixg_asc_planes(const ixg_asc* s, ixg_facet_plane* out, uint32_t cap, uint32_t* n_out) {
-    if (n_out) *n_out = 0;
-    if (!s || !n_out) return IXG_INVALID;
+ixg_status schedule_planes_of(const ixg_asc* s, ixg_facet_plane* out, uint32_t cap, uint32_t* n_out) {
     std::vector<ixg_asc_tier> tiers(s->tiers.size());
     std::vector<double> indices;
     for (size_t t = 0; t < s->tiers.size(); ++t) {
@@ -881,8 +901,29 @@ ixg_status ixg_asc_planes(const ixg_asc* s, ixg_facet_plane* out, uint32_t cap, 
         tiers[t].n_index_names = 0;
         indices.insert(indices.end(), s->tiers[t].indices.begin(), s->tiers[t].indices.end());
     }
-    return ixg_asc_schedule_planes(tiers.data(), (uint32_t)tiers.size(), indices.data(), (uint32_t)indices.size(),
-                                   s->gear_teeth, s->gear_reference_angle, out, cap, n_out, nullptr, nullptr);
+    return schedule_planes(tiers.data(), (uint32_t)tiers.size(), indices.data(), (uint32_t)indices.size(),
+                           s->gear_teeth, s->gear_reference_angle, out, cap, n_out, nullptr, nullptr);
+}
+
+}  // namespace
+
+extern "C" {
+
+ixg_status ixg_asc_schedule_planes(const ixg_asc_tier* tiers, uint32_t n_tiers, const double* indices,
+                                   uint32_t n_indices, int32_t gear_teeth, double gear_reference_angle,
+                                   ixg_facet_plane* out, uint32_t cap, uint32_t* n_out, uint32_t* raw_to_plane,
+                                   uint32_t* tier_ends) {
+    if (n_out) *n_out = 0;
+    return guarded([&] {
+        return schedule_planes(tiers, n_tiers, indices, n_indices, gear_teeth, gear_reference_angle, out, cap,
+                               n_out, raw_to_plane, tier_ends);
+    });
+}
+
+ixg_status ixg_asc_planes(const ixg_asc* s, ixg_facet_plane* out, uint32_t cap, uint32_t* n_out) {
+    if (n_out) *n_out = 0;
+    if (!s || !n_out) return IXG_INVALID;
+    return guarded([&] { return schedule_planes_of(s, out, cap, n_out); });
 }
 
 }  // extern "C"

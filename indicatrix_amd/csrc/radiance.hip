@@ -17,6 +17,9 @@
 
 extern "C" hipStream_t ixg_internal_stream(ixg_ctx* ctx);
 extern "C" int ixg_internal_device(ixg_ctx* ctx);
+extern "C" int ixg_internal_lost(ixg_ctx* ctx);
+extern "C" ixg_status ixg_internal_fail(ixg_ctx* ctx, ixg_status st, const char* what);
+extern "C" ixg_status ixg_internal_hip_fail(ixg_ctx* ctx, hipError_t e, const char* where);
 
 namespace ixg {
 
@@ -159,15 +162,20 @@ static bool aligned16(const void* p) { return ((uintptr_t)p & 15u) == 0; }
 
 using namespace ixg;
 
-static ixg_status finish(hipError_t e, hipStream_t s) {
-    if (e == hipSuccess) e = hipGetLastError();
+// A HIP error marks the context lost and records where, as every other
+// entry point of the library does; the caller recovers with ixg_try_recover.
+static ixg_status finish(ixg_ctx* ctx, const char* where, hipStream_t s) {
+    hipError_t e = hipGetLastError();
     if (e == hipSuccess) e = hipStreamSynchronize(s);
-    return e == hipSuccess ? IXG_OK : IXG_DEVICE_LOST;
+    return e == hipSuccess ? IXG_OK : ixg_internal_hip_fail(ctx, e, where);
 }
 
 static ixg_status rad_stream(ixg_ctx* ctx, hipStream_t* s) {
     if (!ctx) return IXG_INVALID;
-    if (hipSetDevice(ixg_internal_device(ctx)) != hipSuccess) return IXG_DEVICE_LOST;
+    if (ixg_internal_lost(ctx))
+        return ixg_internal_fail(ctx, IXG_DEVICE_LOST, "context is lost; call ixg_try_recover");
+    const hipError_t e = hipSetDevice(ixg_internal_device(ctx));
+    if (e != hipSuccess) return ixg_internal_hip_fail(ctx, e, "hipSetDevice");
     *s = ixg_internal_stream(ctx);
     return IXG_OK;
 }
@@ -186,7 +194,7 @@ static ixg_status shuffle_common(ixg_ctx* ctx, const void* d_src, uint64_t n_byt
         shuffle_kernel<<<grid, kRadBlock, 0, s>>>((const uint8_t*)d_src, n, (uint8_t*)d_dst, groups);
     else
         unshuffle_kernel<<<grid, kRadBlock, 0, s>>>((const uint8_t*)d_src, n, (uint8_t*)d_dst, groups);
-    return finish(hipSuccess, s);
+    return finish(ctx, forward ? "radiance shuffle" : "radiance unshuffle", s);
 }
 
 static void set_message(ixg_radiance_error* e, const char* fmt, ...) __attribute__((format(printf, 2, 3)));
@@ -197,12 +205,15 @@ static void set_message(ixg_radiance_error* e, const char* fmt, ...) {
     va_end(ap);
 }
 
-static const char* encoding_name(int32_t enc) {
-    switch (enc) {
-    case IXG_PAYLOAD_RAW: return "Raw";
-    case IXG_PAYLOAD_SHUFFLE_ZSTD: return "ShuffleZstd";
-    case IXG_PAYLOAD_SHUFFLE_LZ4: return "ShuffleLz4";
-    default: return "Unknown";
+// PayloadEncoding's Debug text (messages/encoding.rs:43-56), which the
+// reference's Display of UnsupportedEncoding prints: the zstd arm carries
+// its level.
+static void encoding_debug(int32_t enc, char* buf, size_t n) {
+    switch (enc & 0xFF) {
+    case IXG_PAYLOAD_RAW: snprintf(buf, n, "Raw"); break;
+    case IXG_PAYLOAD_SHUFFLE_ZSTD: snprintf(buf, n, "ShuffleZstd { level: %d }", (int)IXG_PAYLOAD_ZSTD_LEVEL_OF(enc)); break;
+    case IXG_PAYLOAD_SHUFFLE_LZ4: snprintf(buf, n, "ShuffleLz4"); break;
+    default: snprintf(buf, n, "Unknown"); break;
     }
 }
 
@@ -234,7 +245,7 @@ ixg_status ixg_radiance_add_device(ixg_ctx* ctx, const void* d_payload, int32_t 
         else
             add_kernel<false><<<grid, kRadBlock, 0, s>>>((const uint8_t*)d_payload, pixels, d_acc, d_dropped, quads);
     }
-    return finish(hipSuccess, s);
+    return finish(ctx, "radiance add", s);
 }
 
 ixg_status ixg_radiance_check_payload(int32_t encoding, uint32_t raw_len, uint64_t wire_len, uint32_t width,
@@ -244,7 +255,11 @@ ixg_status ixg_radiance_check_payload(int32_t encoding, uint32_t raw_len, uint64
     err->width = width;
     err->height = height;
     err->encoding = encoding;
-    if (encoding < IXG_PAYLOAD_RAW || encoding > IXG_PAYLOAD_SHUFFLE_LZ4) return IXG_INVALID;
+    const int32_t family = encoding & 0xFF;
+    // the level rides in bits 8-15 of the zstd arm only (a u8, encoding.rs:51)
+    if (encoding < 0 || family > IXG_PAYLOAD_SHUFFLE_LZ4 || (encoding >> 16) != 0 ||
+        (family != IXG_PAYLOAD_SHUFFLE_ZSTD && (encoding >> 8) != 0))
+        return IXG_INVALID;
     const uint64_t expected = (uint64_t)width * (uint64_t)height * 12u;
     if (expected > IXG_MAX_FRAME_LEN) {
         err->kind = IXG_RAD_TOO_LARGE;
@@ -260,7 +275,7 @@ ixg_status ixg_radiance_check_payload(int32_t encoding, uint32_t raw_len, uint64
                     (unsigned long long)expected);
         return IXG_INVALID;
     }
-    if (encoding == IXG_PAYLOAD_RAW) {
+    if (family == IXG_PAYLOAD_RAW) {
         if (wire_len != expected) {
             err->kind = IXG_RAD_LENGTH_MISMATCH;
             err->got_bytes = wire_len;
@@ -272,7 +287,9 @@ ixg_status ixg_radiance_check_payload(int32_t encoding, uint32_t raw_len, uint64
     }
     if (!compression) {
         err->kind = IXG_RAD_UNSUPPORTED_ENCODING;
-        set_message(err, "payload encoding %s is not supported by this build", encoding_name(encoding));
+        char name[48];
+        encoding_debug(encoding, name, sizeof name);
+        set_message(err, "payload encoding %s is not supported by this build", name);
         return IXG_INVALID;
     }
     return IXG_OK;

@@ -52,6 +52,48 @@ def evaluate_optical_image(backend, request, want_pixels=True):
     return pixels, tot
 
 
+def _pixel_array(pixels):
+    px = np.ascontiguousarray(pixels, PIXEL_DTYPE)
+    arr = (abi.OpticalPixel * max(len(px), 1)).from_buffer_copy(px.tobytes() or bytes(C.sizeof(abi.OpticalPixel)))
+    return arr, len(px)
+
+
+def scintillation_maps(backend, pixels, width, height):
+    """scintillation_spatial and scintillation_index over a finished map
+    (image_scint.rs:175-247) on the device: (spatial, index) float32 arrays,
+    index NaN where the temporal term was not computed."""
+    arr, n = _pixel_array(pixels)
+    spatial = np.zeros(n, np.float32)
+    index = np.zeros(n, np.float32)
+    st = lib().ixg_scintillation_maps(backend.handle, arr, n, width, height,
+                                      spatial.ctypes.data_as(C.POINTER(C.c_float)),
+                                      index.ctypes.data_as(C.POINTER(C.c_float)))
+    if st != abi.IXG_OK:
+        raise IxgError(st, backend.last_error())
+    return spatial, index
+
+
+def scintillation_maps_device(backend, d_pixels_ptr, n, width, height, d_spatial_ptr, d_index_ptr):
+    """The same on device buffers (raw device pointers)."""
+    st = lib().ixg_scintillation_maps_device(backend.handle, C.c_void_p(d_pixels_ptr), n, width, height,
+                                             C.c_void_p(d_spatial_ptr), C.c_void_p(d_index_ptr))
+    if st != abi.IXG_OK:
+        raise IxgError(st, backend.last_error())
+
+
+def scintillation_spatial_at(pixels, width, height, i):
+    """scintillation_spatial_at (image_scint.rs:201-232), the hover read-out."""
+    arr, n = _pixel_array(pixels)
+    return lib().ixg_scintillation_spatial_at(arr, n, width, height, i)
+
+
+def scintillation_index_at(pixels, width, height, i):
+    """scintillation_index_at (image_scint.rs:249-259): a float, or None."""
+    arr, n = _pixel_array(pixels)
+    out = C.c_float()
+    return out.value if lib().ixg_scintillation_index_at(arr, n, width, height, i, C.byref(out)) else None
+
+
 def metric_sweep(backend, planes, designs, tilts_deg, width, height, samples, distance,
                  environment):
     """planes: (P,4) float32 table; designs: list of (offset, count, abi.Dispersion).

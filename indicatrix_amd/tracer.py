@@ -355,6 +355,17 @@ class GpuBackend:
             raise IxgError(st, self.last_error())
         return xyz, (events_to_numpy(ev) if want_events else None)
 
+    def film_samples(self, scene, sample_offset, spp):
+        """The film kernel's own per-path XYZ (the launch ixg_accumulate
+        times), shape (pixels * spp, 3) in (pixel, sample) order."""
+        n = scene.width * scene.height * spp
+        xyz = np.zeros((n, 3), np.float32)
+        st = lib().ixg_film_samples(self._ctx, scene.ref(), sample_offset, spp,
+                                    xyz.ctypes.data_as(C.POINTER(C.c_float)))
+        if st != abi.IXG_OK:
+            raise IxgError(st, self.last_error())
+        return xyz
+
     def trace_rays(self, scene, origins, dirs, seeds, heros):
         """trace_spectral_ray for explicit rays -> ((n,3) float32 XYZ, events)."""
         origins = np.asarray(origins, np.float32).reshape(-1, 3)

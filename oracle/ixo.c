@@ -2403,6 +2403,128 @@ float ixo_image_scintillation_pct(const ixg_optical_pixel* px, uint32_t n) {
     return clampf(fmaf(0.4f, temporal, 0.6f * spatial), 0.0f, 100.0f); /* scintillation.rs:194 */
 }
 
+/* spatial_scintillation_pct (scintillation.rs:146-171) */
+static float spatial_pct(float sum, float sum_sq, uint32_t count) {
+    if (count == 0) return 0.0f;
+    float mean = sum / (float)count;
+    if (mean <= 1e-4f) return 0.0f;
+    float var = maxf(fmaf(mean, -mean, sum_sq / (float)count), 0.0f);
+    float cv = sqrtf(var) / mean;
+    return clampf(cv / (1.0f + cv) * 100.0f, 0.0f, 100.0f);
+}
+
+/* window_spatial (image_scint.rs:150-173): the 5x5 window of map pixels
+ * around (x, y), clipped at the frame, hit pixels only, row-major. */
+static float window_spatial_m(const ixg_optical_pixel* px, uint32_t w, uint32_t h, uint32_t x, uint32_t y) {
+    float sum = 0.0f, sum_sq = 0.0f;
+    uint32_t count = 0;
+    uint32_t y0 = y >= 2 ? y - 2 : 0, y1 = y + 3 < h ? y + 3 : h;
+    uint32_t x0 = x >= 2 ? x - 2 : 0, x1 = x + 3 < w ? x + 3 : w;
+    for (uint32_t wy = y0; wy < y1; ++wy)
+        for (uint32_t wx = x0; wx < x1; ++wx) {
+            const ixg_optical_pixel* q = &px[(size_t)wy * w + wx];
+            if (q->hits > 0) {
+                float f = fractions2(q);
+                sum += f;
+                sum_sq = fmaf(f, f, sum_sq);
+                count++;
+            }
+        }
+    return spatial_pct(sum, sum_sq, count);
+}
+
+/* scintillation_spatial (image_scint.rs:175-199): 0 where hits == 0 and
+ * everywhere when width * height is not the pixel count. */
+void ixo_scintillation_spatial(const ixg_optical_pixel* px, uint32_t n, uint32_t w, uint32_t h, float* out) {
+    for (uint32_t i = 0; i < n; ++i) out[i] = 0.0f;
+    if ((uint64_t)w * h != n) return;
+    for (uint32_t y = 0; y < h; ++y)
+        for (uint32_t x = 0; x < w; ++x)
+            if (px[(size_t)y * w + x].hits > 0) out[(size_t)y * w + x] = window_spatial_m(px, w, h, x, y);
+}
+
+/* scintillation_spatial_at (image_scint.rs:201-232): the window laid out as
+ * its own small frame, read by the same window function. */
+float ixo_scintillation_spatial_at(const ixg_optical_pixel* px, uint32_t n, uint32_t w, uint32_t h, uint32_t i) {
+    if ((uint64_t)w * h != n || i >= n) return 0.0f;
+    uint32_t col = i % w, row = i / w;
+    uint32_t x0 = col >= 2 ? col - 2 : 0, x1 = col + 3 < w ? col + 3 : w;
+    uint32_t y0 = row >= 2 ? row - 2 : 0, y1 = row + 3 < h ? row + 3 : h;
+    ixg_optical_pixel local[25];
+    uint32_t fw = x1 - x0, fh = y1 - y0, k = 0;
+    for (uint32_t wy = y0; wy < y1; ++wy)
+        for (uint32_t wx = x0; wx < x1; ++wx) local[k++] = px[(size_t)wy * w + wx];
+    if (local[(row - y0) * fw + (col - x0)].hits == 0) return 0.0f;
+    return window_spatial_m(local, fw, fh, col - x0, row - y0);
+}
+
+/* OpticalImage::temporal_variance (image_types.rs:105-112): 1 and the
+ * Bernoulli variance of the five poses, or 0 (None). */
+static int temporal_variance_m(const ixg_optical_pixel* px, uint32_t n, uint32_t i, float* v) {
+    if (i >= n || px[i].scint_poses == 0) return 0;
+    float fr = (float)px[i].scint_returned / (float)px[i].scint_poses;
+    *v = fmaf(fr, -fr, fr);
+    return 1;
+}
+
+/* combine_scintillation_pct (scintillation.rs:194) of the spatial term and
+ * temporal_scintillation_pct (scintillation.rs:173-180) of one pixel. */
+static float index_of(float spatial, float variance) {
+    float temporal = clampf(((variance / 1.0f) / 0.24f) * 100.0f, 0.0f, 100.0f);
+    return clampf(fmaf(0.4f, temporal, 0.6f * spatial), 0.0f, 100.0f);
+}
+
+/* scintillation_index (image_scint.rs:234-247): 1 and the index of pixel i
+ * from spatial[] (0 past its end), or 0 (None). */
+int ixo_scintillation_index(const ixg_optical_pixel* px, uint32_t n, const float* spatial, uint32_t n_spatial,
+                            uint32_t i, float* out) {
+    float v;
+    if (!temporal_variance_m(px, n, i, &v)) return 0;
+    *out = index_of(i < n_spatial ? spatial[i] : 0.0f, v);
+    return 1;
+}
+
+/* scintillation_index for every pixel: out[i] = the index, NaN for None. */
+void ixo_scintillation_indices(const ixg_optical_pixel* px, uint32_t n, const float* spatial, float* out) {
+    for (uint32_t i = 0; i < n; ++i) {
+        float v;
+        out[i] = ixo_scintillation_index(px, n, spatial, n, i, &v) ? v : NAN;
+    }
+}
+
+/* scintillation_index_at (image_scint.rs:249-259). */
+int ixo_scintillation_index_at(const ixg_optical_pixel* px, uint32_t n, uint32_t w, uint32_t h, uint32_t i,
+                               float* out) {
+    float v;
+    if (!temporal_variance_m(px, n, i, &v)) return 0;
+    *out = index_of(ixo_scintillation_spatial_at(px, n, w, h, i), v);
+    return 1;
+}
+
+/* offset (image.rs:21-36) and temporal_cameras (image_scint.rs:35-48). */
+void ixo_image_offset(uint32_t s, float* jx, float* jy) { offset_m(s, jx, jy); }
+
+void ixo_temporal_cameras(const float pose[3], float fov_tan, uint32_t width, uint32_t height, ixg_camera out[5]) {
+    static const float offs[5] = {-3.0f, -1.5f, 0.0f, 1.5f, 3.0f}; /* SCINT_TEMPORAL_YAW_OFFSETS_DEG */
+    for (int i = 0; i < 5; ++i) {
+        ixo_camera_new(pose[0] + offs[i] * (F32_PI / 180.0f), pose[1], pose[2], 42.0f, width, height, 1, &out[i]);
+        out[i].fov_tan = fov_tan;
+    }
+}
+
+/* classify_ray (classify.rs:109-197) of one ray against an explicit
+ * observer direction: the class (0 windowed, 1 extinct, 2 returned,
+ * 3 entry-blocked / invalid, -1 no hit). */
+int ixo_classify_ray(const ixg_facet_plane* planes, uint32_t n, const ixg_dispersion* disp,
+                     const ixg_environment* env, const float origin[3], const float dir[3], const float observer[3]) {
+    mctx_t M;
+    if (!mctx_build(planes, n, disp, env, &M)) return -2;
+    int hf;
+    float a = 0, w = 0;
+    int32_t ef;
+    return classify_m(&M, vfrom(origin), vfrom(dir), vfrom(observer), 0, &hf, &a, &w, &ef, NULL);
+}
+
 /* ================================================================== */
 /* Pose metrics: evaluate_gem_optical_metrics (color/metrics/evaluate.rs) */
 /* ================================================================== */

@@ -90,6 +90,24 @@ void ixo_trace_paths(const ixg_scene* scene, uint32_t sample_offset, uint32_t sp
 int ixo_metric_image(const ixg_metric_request* req, ixg_optical_pixel* pixels,
                      ixg_metric_totals* totals);
 float ixo_image_scintillation_pct(const ixg_optical_pixel* px, uint32_t n);
+/* scintillation_spatial / _at (IX/color/metrics/image_scint.rs:175-232) and
+ * scintillation_index / _at (:234-259); the index functions return 0 for
+ * None (no temporal term) and 1 with *out set. */
+void ixo_scintillation_spatial(const ixg_optical_pixel* px, uint32_t n, uint32_t w, uint32_t h, float* out);
+float ixo_scintillation_spatial_at(const ixg_optical_pixel* px, uint32_t n, uint32_t w, uint32_t h, uint32_t i);
+int ixo_scintillation_index(const ixg_optical_pixel* px, uint32_t n, const float* spatial, uint32_t n_spatial,
+                            uint32_t i, float* out);
+int ixo_scintillation_index_at(const ixg_optical_pixel* px, uint32_t n, uint32_t w, uint32_t h, uint32_t i,
+                               float* out);
+void ixo_scintillation_indices(const ixg_optical_pixel* px, uint32_t n, const float* spatial, float* out);
+/* offset (IX/color/metrics/image.rs:21-36): the sub-pixel offset of sample s. */
+void ixo_image_offset(uint32_t s, float* jx, float* jy);
+/* temporal_cameras (IX/color/metrics/image_scint.rs:35-48). */
+void ixo_temporal_cameras(const float pose[3], float fov_tan, uint32_t width, uint32_t height, ixg_camera out[5]);
+/* classify_ray (IX/color/metrics/classify.rs:109-197) with an explicit
+ * observer: class 0..3, -1 no hit, -2 no lighting model. */
+int ixo_classify_ray(const ixg_facet_plane* planes, uint32_t n, const ixg_dispersion* disp,
+                     const ixg_environment* env, const float origin[3], const float dir[3], const float observer[3]);
 
 /* measure_model_width (IX/render_setup/stone_width.rs:27): 1 and the width
  * when the planes bound a finite solid, else 0. */

@@ -74,6 +74,22 @@ def lib():
         L.ixo_metric_image.argtypes = [P(abi.MetricRequest), P(abi.OpticalPixel), P(abi.MetricTotals)]
         L.ixo_image_scintillation_pct.restype = C.c_float
         L.ixo_image_scintillation_pct.argtypes = [P(abi.OpticalPixel), C.c_uint32]
+        PX = P(abi.OpticalPixel)
+        L.ixo_scintillation_spatial.argtypes = [PX, C.c_uint32, C.c_uint32, C.c_uint32, P(C.c_float)]
+        L.ixo_scintillation_spatial_at.restype = C.c_float
+        L.ixo_scintillation_spatial_at.argtypes = [PX, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32]
+        L.ixo_scintillation_index.restype = C.c_int
+        L.ixo_scintillation_index.argtypes = [PX, C.c_uint32, P(C.c_float), C.c_uint32, C.c_uint32,
+                                              P(C.c_float)]
+        L.ixo_scintillation_index_at.restype = C.c_int
+        L.ixo_scintillation_index_at.argtypes = [PX, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32,
+                                                 P(C.c_float)]
+        L.ixo_scintillation_indices.argtypes = [PX, C.c_uint32, P(C.c_float), P(C.c_float)]
+        L.ixo_image_offset.argtypes = [C.c_uint32, P(C.c_float), P(C.c_float)]
+        L.ixo_temporal_cameras.argtypes = [P(C.c_float), C.c_float, C.c_uint32, C.c_uint32, P(abi.Camera)]
+        L.ixo_classify_ray.restype = C.c_int
+        L.ixo_classify_ray.argtypes = [P(abi.FacetPlane), C.c_uint32, P(abi.Dispersion), P(abi.Environment),
+                                       P(C.c_float), P(C.c_float), P(C.c_float)]
         L.ixo_model_width.restype = C.c_int
         L.ixo_model_width.argtypes = [P(abi.FacetPlane), C.c_uint32, P(C.c_double)]
         L.ixo_girdle_half_width.restype = C.c_float
@@ -108,6 +124,70 @@ def metric_image(request):
         raise RuntimeError(f"oracle metric_image status {st}")
     scint = lib().ixo_image_scintillation_pct(px, n)
     return np.frombuffer(bytes(px), PIXEL_DTYPE).copy(), tot, scint
+
+
+def pixels_ctypes(px):
+    """A PIXEL_DTYPE ndarray as a ctypes OpticalPixel array (copied)."""
+    n = len(px)
+    arr = (abi.OpticalPixel * max(n, 1))()
+    C.memmove(arr, np.ascontiguousarray(px).tobytes(), n * C.sizeof(abi.OpticalPixel))
+    return arr
+
+
+def scintillation_spatial(px, width, height):
+    """scintillation_spatial (image_scint.rs:175-199) over a pixel array."""
+    n = len(px)
+    out = (C.c_float * max(n, 1))()
+    lib().ixo_scintillation_spatial(pixels_ctypes(px), n, width, height, out)
+    return np.frombuffer(bytes(out), np.float32)[:n].copy()
+
+
+def scintillation_indices(px, spatial):
+    """scintillation_index of every pixel, NaN for None."""
+    n = len(px)
+    sp = np.ascontiguousarray(spatial, np.float32)
+    out = np.zeros(n, np.float32)
+    lib().ixo_scintillation_indices(pixels_ctypes(px), n, sp.ctypes.data_as(C.POINTER(C.c_float)),
+                                    out.ctypes.data_as(C.POINTER(C.c_float)))
+    return out
+
+
+def scintillation_spatial_at(px, width, height, i):
+    return lib().ixo_scintillation_spatial_at(pixels_ctypes(px), len(px), width, height, i)
+
+
+def scintillation_index(px, spatial, i):
+    """scintillation_index (image_scint.rs:234-247): a float, or None."""
+    out = C.c_float()
+    sp = np.ascontiguousarray(spatial, np.float32)
+    ok = lib().ixo_scintillation_index(pixels_ctypes(px), len(px), sp.ctypes.data_as(C.POINTER(C.c_float)),
+                                       len(sp), i, C.byref(out))
+    return out.value if ok else None
+
+
+def scintillation_index_at(px, width, height, i):
+    out = C.c_float()
+    ok = lib().ixo_scintillation_index_at(pixels_ctypes(px), len(px), width, height, i, C.byref(out))
+    return out.value if ok else None
+
+
+def image_offset(s):
+    jx, jy = C.c_float(), C.c_float()
+    lib().ixo_image_offset(s, C.byref(jx), C.byref(jy))
+    return jx.value, jy.value
+
+
+def temporal_cameras(pose, fov_tan, width, height):
+    out = (abi.Camera * 5)()
+    lib().ixo_temporal_cameras((C.c_float * 3)(*pose), fov_tan, width, height, out)
+    return list(out)
+
+
+def classify_ray(planes, dispersion, environment, origin, direction, observer):
+    arr = (abi.FacetPlane * len(planes))(*planes)
+    f3 = C.c_float * 3
+    return lib().ixo_classify_ray(arr, len(planes), C.byref(dispersion), C.byref(environment),
+                                  f3(*origin), f3(*direction), f3(*observer))
 
 
 def threads():

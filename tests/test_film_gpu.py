@@ -149,8 +149,67 @@ def test_accumulate_matches_oracle_sums(backend):
     ref = oracle_lib.accumulate(sc, 0, 16)
     assert acc.sum() > 0
     assert abs(acc.sum() - ref.sum()) / ref.sum() < 1e-4
+    # every pixel: a pixel whose paths all agree in the side channel sums
+    # within the per-path budget (1e-4 of each path's largest component,
+    # non-negative XYZ: at most 3e-4 of the pixel's largest sum); a pixel
+    # holding a flipped path (a branch draw inside the transcendental ULP
+    # gap, < 0.1 % of paths) is bounded only by that path's share
+    g_xyz, g_ev = backend.trace_paths(sc, 0, 16)
+    o_xyz, o_ev = oracle_lib.trace_paths(sc, 0, 16)
+    flipped = (~_side_channel_agreement(g_ev, o_ev)).reshape(24 * 16, 16).any(axis=1)
     per_pixel = np.abs(acc - ref).max(axis=1) / np.maximum(np.abs(ref).max(axis=1), 1e-6)
-    assert np.mean(per_pixel > 1e-4) < 0.02
+    assert per_pixel[~flipped].max() < 3e-4, per_pixel[~flipped].max()
+    assert flipped.mean() < 0.02
+
+
+def _reduce_like_the_device(xyz, n_pixels, spp):
+    """film_reduce_kernel's per-pixel sums restated: FP64, sample order,
+    a sample with any non-finite component dropped."""
+    x = xyz.reshape(n_pixels, spp, 3)
+    acc = np.zeros((n_pixels, 3), np.float64)
+    for k in range(spp):
+        v = x[:, k, :].astype(np.float64)
+        ok = np.isfinite(x[:, k, :]).all(axis=1, keepdims=True)
+        acc = acc + np.where(ok, v, 0.0)
+    return acc
+
+
+@pytest.mark.parametrize("case", ["c1", "c2"])
+def test_timed_film_kernel_per_path(backend, case):
+    """The instantiation the accumulate path launches and the bench times
+    (the persistent kernel specialised for the lit environment and without
+    the rare surface / medium features: C1 <ISO, clear, isotropic>, C2 <ISO,
+    absorbing, uniaxial>) writes, path for path, the same XYZ bits as the
+    diagnostic events kernel, whose paths are held to the oracle here too
+    (integer side channel and XYZ budget); its per-pixel reduction is the
+    FP64 sample-order sum of those records, bit for bit."""
+    if case == "c1":
+        sc, spp = workloads.film_c1_scene(), 4           # the full C1 frame, 65536 paths
+    else:
+        sc, spp = workloads.film_c2_scene(96, 96, 4), 4  # the C2 oval and material, 36864 paths
+    n_px = sc.width * sc.height
+    print(f"timed film kernel {case} {sc.width}x{sc.height}x{spp}", flush=True)
+    f_xyz = backend.film_samples(sc, 64, spp)
+    g_xyz, g_ev = backend.trace_paths(sc, 64, spp)
+    assert f_xyz.tobytes() == g_xyz.tobytes()
+    acc = np.zeros((n_px, 3), np.float64)
+    assert backend.try_accumulate(sc, 64, spp, acc)
+    assert acc.tobytes() == _reduce_like_the_device(f_xyz, n_px, spp).tobytes()
+    o_xyz, o_ev = oracle_lib.trace_paths(sc, 64, spp)
+    same = _side_channel_agreement(g_ev, o_ev)
+    a, b = f_xyz[same].astype(np.float64), o_xyz[same].astype(np.float64)
+    rel = np.abs(a - b) / np.maximum(np.abs(b).max(axis=1, keepdims=True), 1e-6)
+    print(f"  flip rate {1.0 - same.mean():.2e}, q999 {np.quantile(rel, 0.999):.2e}, max {rel.max():.2e}",
+          flush=True)
+    if case == "c1":
+        assert 1.0 - same.mean() < 1e-3
+        assert rel.max() < 1e-4
+        assert np.median(ulp_dist(f_xyz[same], o_xyz[same])) <= 2
+    else:
+        assert 1.0 - same.mean() < 2e-3
+        assert np.quantile(rel, 0.999) < 1e-3
+        assert (rel.max(axis=1) > 1e-3).mean() < 5e-4
+    assert (g_ev["n_refract"] > 0).mean() > 0.05
 
 
 def test_add_semantics_and_sample_additivity(backend):

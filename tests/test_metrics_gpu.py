@@ -50,3 +50,73 @@ def test_metric_image_request_validation(backend):
     r.has_pose = 0
     with pytest.raises(tracer.IxgError):
         metrics.evaluate_optical_image(backend, r)
+
+
+def _oracle_maps(px, w, h):
+    sp = oracle_lib.scintillation_spatial(px, w, h)
+    return sp, oracle_lib.scintillation_indices(px, sp)
+
+
+def _same_bits(a, b):
+    return np.array_equal(np.asarray(a, np.float32).view(np.uint32), np.asarray(b, np.float32).view(np.uint32))
+
+
+def test_scintillation_maps_match_oracle_bit_for_bit(backend):
+    """scintillation_spatial / scintillation_index (image_scint.rs:175-247) on
+    the device: the reference's hand-built maps, a size mismatch and seeded
+    random maps of ragged sizes, every value's bits equal to the oracle's."""
+    from scint_maps_cases import cases
+    for name, px, w, h in cases():
+        print(f"scintillation maps {name}", flush=True)
+        g_sp, g_ix = metrics.scintillation_maps(backend, px, w, h)
+        o_sp, o_ix = _oracle_maps(px, w, h)
+        assert _same_bits(g_sp, o_sp), name
+        assert _same_bits(g_ix, o_ix), name
+        if name == "checker9":
+            assert np.all(g_sp > 10.0)
+        if name in ("uniform9",):
+            assert np.all(np.abs(g_sp) < 1e-3)
+        if name == "mismatch":
+            assert np.all(g_sp == 0.0)
+
+
+def test_scintillation_maps_of_a_traced_image(backend):
+    """The maps of a device-traced LightTent image (temporal term on) equal
+    the oracle's maps of the oracle's image, the device buffers variant
+    equals the host one, and the image aggregate lies within the map's range."""
+    import torch
+    r = request("LightTent", w=64, h=48, samples=4, pose=(0.6, 0.45, 2.4))
+    print("scintillation maps of a 64x48x4 LightTent image", flush=True)
+    g_px, g_tot = metrics.evaluate_optical_image(backend, r)
+    o_px, _, _ = oracle_lib.metric_image(r)
+    assert np.array_equal(g_px["classes"], o_px["classes"]) and np.array_equal(g_px["scint_returned"],
+                                                                                o_px["scint_returned"])
+    g_sp, g_ix = metrics.scintillation_maps(backend, g_px, 64, 48)
+    o_sp, o_ix = _oracle_maps(o_px, 64, 48)
+    assert _same_bits(g_sp, o_sp) and _same_bits(g_ix, o_ix)
+    assert np.any(g_ix > 0) and np.all(np.isnan(g_ix) == (g_px["scint_poses"] == 0))
+    d_px = torch.from_numpy(g_px.view(np.uint8).copy()).to("cuda:0")
+    d_sp = torch.empty(len(g_px), dtype=torch.float32, device="cuda:0")
+    d_ix = torch.empty_like(d_sp)
+    metrics.scintillation_maps_device(backend, d_px.data_ptr(), len(g_px), 64, 48, d_sp.data_ptr(), d_ix.data_ptr())
+    assert _same_bits(d_sp.cpu().numpy(), g_sp) and _same_bits(d_ix.cpu().numpy(), g_ix)
+
+
+def test_scintillation_maps_at_full_size(backend):
+    """A 1024 x 1024 seeded map (16-pixel tiles, every halo case) equals the
+    oracle bit for bit."""
+    rng = np.random.default_rng(5)
+    w, h = 1024, 1024
+    px = np.zeros(w * h, metrics.PIXEL_DTYPE)
+    hits = np.where(rng.random(w * h) < 0.6, rng.integers(1, 33, w * h), 0)
+    px["hits"] = hits
+    ret = rng.integers(0, hits + 1)
+    px["classes"][:, 2] = ret
+    px["classes"][:, 1] = hits - ret
+    pose = (hits > 0) & (rng.random(w * h) < 0.5)
+    px["scint_poses"] = np.where(pose, 5, 0)
+    px["scint_returned"] = np.where(pose, rng.integers(0, 6, w * h), 0)
+    print("scintillation maps 1024x1024", flush=True)
+    g_sp, g_ix = metrics.scintillation_maps(backend, px, w, h)
+    o_sp, o_ix = _oracle_maps(px, w, h)
+    assert _same_bits(g_sp, o_sp) and _same_bits(g_ix, o_ix)

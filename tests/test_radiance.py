@@ -107,11 +107,23 @@ def test_check_payload_order_and_messages():
     assert str(e.value) == "a 8192x8192 radiance payload would exceed the 536870912 byte frame cap"
     # exactly at the cap is allowed: 44739242 pixels * 12 = 536870904
     chk(abi.PAYLOAD_RAW, 536870904, 536870904, 44739242, 1)
+    # Display of UnsupportedEncoding prints the encoding's Debug form
+    # (radiance/mod.rs:149-151), the zstd arm with its level
+    for level in (1, 3, 22):
+        with pytest.raises(radiance.RadianceError) as e:
+            chk(abi.payload_zstd(level), 48, 20, 2, 2)
+        assert e.value.kind == "UnsupportedEncoding"
+        assert str(e.value) == f"payload encoding ShuffleZstd {{ level: {level} }} is not supported by this build"
+        assert e.value.fields["encoding"] == abi.payload_zstd(level)
     with pytest.raises(radiance.RadianceError) as e:
-        chk(abi.PAYLOAD_SHUFFLE_ZSTD, 48, 20, 2, 2)
-    assert e.value.kind == "UnsupportedEncoding"
-    assert str(e.value) == "payload encoding ShuffleZstd is not supported by this build"
+        chk(abi.PAYLOAD_SHUFFLE_LZ4, 48, 20, 2, 2)
+    assert str(e.value) == "payload encoding ShuffleLz4 is not supported by this build"
     chk(abi.PAYLOAD_SHUFFLE_LZ4, 48, 20, 2, 2, compression=True)
+    chk(abi.payload_zstd(1), 48, 20, 2, 2, compression=True)
+    # a level on an arm that has none, or beyond u8, is not an encoding
+    for bad in (abi.PAYLOAD_RAW | (1 << 8), abi.PAYLOAD_SHUFFLE_LZ4 | (2 << 8), 1 << 16, -1, 3):
+        with pytest.raises(ValueError):
+            chk(bad, 48, 48, 2, 2)
 
 
 def test_planted_mistake_in_the_oracle_is_caught():

@@ -105,3 +105,36 @@ def test_payload_throughput_4k_frame(backend):
         print(f"{name}: {dt * 1e3:.3f} ms/call, {traffic / dt / 1e9:.0f} GB/s algorithmic", flush=True)
     assert torch.equal(back, src)
     assert int(dropped.item()) == 0
+
+
+def test_radiance_calls_honour_a_lost_context():
+    """A context marked lost (forced from the host by an injected device loss
+    on the film path) declines the payload kernels with IXG_DEVICE_LOST and
+    stays lost; after ixg_try_recover they run again."""
+    import torch
+    from indicatrix_amd import radiance, tracer, workloads
+    from indicatrix_amd._lib import IxgError
+    print("radiance on a lost context", flush=True)
+    be = tracer.GpuBackend(0)
+    try:
+        src = torch.arange(12, dtype=torch.int32, device="cuda:0")
+        dst = torch.empty_like(src)
+        radiance.shuffle(be, src, dst)
+        be.inject_fault(0)
+        acc = np.zeros((4 * 4, 3), np.float64)
+        with pytest.raises(IxgError):
+            be.try_accumulate(workloads.film_c1_scene(4, 4), 0, 1, acc)
+        assert be.is_lost()
+        for call in (lambda: radiance.shuffle(be, src, dst), lambda: radiance.unshuffle(be, src, dst),
+                     lambda: radiance.add_valid(be, src, torch.zeros(3, dtype=torch.float32, device="cuda:0")
+                                                .repeat(4), False,
+                                                torch.zeros(1, dtype=torch.int32, device="cuda:0"))):
+            with pytest.raises(IxgError) as e:
+                call()
+            assert e.value.status == 3  # IXG_DEVICE_LOST
+            assert be.is_lost()
+        assert be.try_recover()
+        radiance.shuffle(be, src, dst)
+        assert dst.cpu().numpy().tobytes() == ro.shuffle(src.cpu().numpy().astype("<u4").tobytes())
+    finally:
+        be.close()
