@@ -1,8 +1,9 @@
 """Film parity on the GPU, through the C ABI, against the CPU oracle.
 
 Tiers (after the reference's CPU-vs-GPU harness, renderer/gpu/*_check):
-  * golden rays: the reference's own golden traces, <= 64 ULP per XYZ
-    component (device atan2/sin/cos/atan differ from glibc by a few ULP);
+  * golden rays: the reference's own golden traces, <= 4 ULP per XYZ
+    component (the device's f32 atan2 and exp differ from glibc's by an ULP
+    here and there; the measured worst is 3);
   * integer side channel: bounce count, stop reason and the facet-sequence
     hash agree with the oracle path for path; the flip rate (a branch draw
     landing inside the transcendental ULP gap) must stay below 0.1 %;
@@ -53,7 +54,7 @@ def test_golden_rays_on_device(backend, row, material):
     xyz, ev = backend.trace_rays(sc, [t["origin"]], [d], [t["seed"]], [hero])
     expect = np.array(t[f"{material}_bits"], np.uint32).view(np.float32)
     print(f"  ulp {ulp_dist(xyz[0], expect).tolist()}", flush=True)
-    assert ulp_dist(xyz[0], expect).max() <= 64, (xyz[0], expect)
+    assert ulp_dist(xyz[0], expect).max() <= 4, (xyz[0], expect)
 
 
 def small_scene(w=32, h=24):
