@@ -230,6 +230,23 @@ ixg_status ixg_accumulate_batch(ixg_ctx* ctx, const ixg_batch_item* items, uint3
  * library's stream; no host copy of the result. */
 ixg_status ixg_accumulate_device(ixg_ctx* ctx, const ixg_scene* scene, uint32_t sample_offset,
                                  uint32_t spp, double* d_accum_xyz);
+/* ixg_accumulate_device without any device allocation.  No counterpart in
+ * the reference: for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  The film
+ * runs on a fixed scratch area that the library keeps in device memory, one
+ * per GPU, for the life of the process (about 18.1 MiB: the path counter,
+ * the facet set, IXG_NOALLOC_MAX_PATHS path records of 12 B and a panorama
+ * slot of IXG_NOALLOC_MAX_HDR_TEXELS texels).  The frame is traced in chunks
+ * of whole pixels that fit the path records, so any frame size works; spp
+ * above IXG_NOALLOC_MAX_PATHS or a larger panorama is IXG_RESOURCE_LIMIT.
+ * Results equal ixg_accumulate_device's bit for bit.  Every call on a GPU
+ * shares that GPU's scratch: calls that overlap on one GPU race and must be
+ * serialised by the caller. */
+#define IXG_NOALLOC_MAX_PATHS (1u << 20)
+#define IXG_NOALLOC_MAX_HDR_TEXELS (1024u * 512u)
+ixg_status ixg_accumulate_device_noalloc(ixg_ctx* ctx, const ixg_scene* scene, uint32_t sample_offset,
+                                         uint32_t spp, double* d_accum_xyz);
+
 /* Per-path diagnostics for the parity tiers: out_xyz[paths*3] (f32, raw
  * per-sample XYZ) and events[paths] for paths = w*h*spp in (pixel, sample)
  * order.  Either output may be NULL. */

@@ -38,6 +38,8 @@ def _declare(lib):
                                 P(C.c_double), P(C.c_uint8)]),
         "ixg_accumulate_device": (st, [C.c_void_p, P(abi.Scene), C.c_uint32, C.c_uint32,
                                        C.c_void_p]),
+        "ixg_accumulate_device_noalloc": (st, [C.c_void_p, P(abi.Scene), C.c_uint32, C.c_uint32,
+                                               C.c_void_p]),
         "ixg_accumulate_batch": (st, [C.c_void_p, P(abi.BatchItem), C.c_uint32, P(C.c_uint8),
                                       abi.BatchDoneFn, C.c_void_p]),
         "ixg_trace_paths": (st, [C.c_void_p, P(abi.Scene), C.c_uint32, C.c_uint32,

@@ -13,6 +13,8 @@ ROUTE_OK, ROUTE_UNSUPPORTED_MATERIAL, ROUTE_UNSUPPORTED_ENVIRONMENT, ROUTE_DEVIC
     ROUTE_NO_DEVICE = range(5)
 
 MAX_PLANES = 256
+NOALLOC_MAX_PATHS = 1 << 20
+NOALLOC_MAX_HDR_TEXELS = 1024 * 512
 MAX_BANDS = 8
 NUM_CHANNELS = 8
 

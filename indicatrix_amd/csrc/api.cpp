@@ -161,8 +161,10 @@ void fill_absorption(const ixg_material& m, ixg::AbsorbParams* A) {
         A->e[i] = ixg::AbsBand{m.bands_e[i].center_nm, m.bands_e[i].width, m.bands_e[i].peak, m.bands_e[i].shape};
 }
 
+// hdr_dst / hdr_cap (floats): where the panorama is uploaded; null = the
+// context's own buffer, grown on demand.
 ixg_status build_params(ixg_ctx* ctx, const ixg_scene* s, uint32_t sample_offset, uint32_t spp,
-                        ixg::FilmParams* P) {
+                        ixg::FilmParams* P, float* hdr_dst = nullptr, size_t hdr_cap = 0) {
     std::memset(P, 0, sizeof *P);
     std::memcpy(P->cam_origin, s->camera.origin, sizeof P->cam_origin);
     P->fov_tan = s->camera.fov_tan;
@@ -199,15 +201,20 @@ ixg_status build_params(ixg_ctx* ctx, const ixg_scene* s, uint32_t sample_offset
         // the panorama first, copied in stream order, so the pointer below is
         // the buffer this call's kernels read
         const size_t n = (size_t)s->hdr_map->width * s->hdr_map->height * 3;
-        if (n > ctx->hdr_cap) {
-            if (ctx->d_hdr) (void)hipFree(ctx->d_hdr);
-            ctx->d_hdr = nullptr;
-            ctx->hdr_cap = 0;
-            HIP_TRY(ctx, hipMalloc(&ctx->d_hdr, n * sizeof(float)));
-            ctx->hdr_cap = n;
+        if (hdr_dst) {
+            if (n > hdr_cap) return fail(ctx, IXG_RESOURCE_LIMIT, "panorama larger than the scratch panorama slot");
+        } else {
+            if (n > ctx->hdr_cap) {
+                if (ctx->d_hdr) (void)hipFree(ctx->d_hdr);
+                ctx->d_hdr = nullptr;
+                ctx->hdr_cap = 0;
+                HIP_TRY(ctx, hipMalloc(&ctx->d_hdr, n * sizeof(float)));
+                ctx->hdr_cap = n;
+            }
+            if (!ctx->d_hdr) return fail(ctx, IXG_INVALID, "no device buffer for the panorama");
+            hdr_dst = ctx->d_hdr;
         }
-        if (!ctx->d_hdr) return fail(ctx, IXG_INVALID, "no device buffer for the panorama");
-        HIP_TRY(ctx, hipMemcpyAsync(ctx->d_hdr, s->hdr_map->rgb, n * sizeof(float), hipMemcpyHostToDevice,
+        HIP_TRY(ctx, hipMemcpyAsync(hdr_dst, s->hdr_map->rgb, n * sizeof(float), hipMemcpyHostToDevice,
                                     ctx->stream));
         std::memset(&P->env, 0, sizeof P->env);
         P->env.model = ixg::ENV_HDR;
@@ -215,7 +222,7 @@ ixg_status build_params(ixg_ctx* ctx, const ixg_scene* s, uint32_t sample_offset
         P->env.wb[0] = P->env.wb[1] = P->env.wb[2] = 1.0f;
         P->env.hdr_w = s->hdr_map->width;
         P->env.hdr_h = s->hdr_map->height;
-        P->env.hdr_rgb = ctx->d_hdr;
+        P->env.hdr_rgb = hdr_dst;
         return IXG_OK;
     }
     if (!ixg::resolve_environment(s->environment, &P->env))
@@ -223,20 +230,26 @@ ixg_status build_params(ixg_ctx* ctx, const ixg_scene* s, uint32_t sample_offset
     return IXG_OK;
 }
 
-ixg_status upload_planes(ixg_ctx* ctx, const ixg_scene* s) {
-    std::vector<float4> pl(s->n_planes);
+// The facet set (+ the frosted bitmask) into d_planes[IXG_MAX_PLANES + 2]
+// (the context's table unless another is given); ixg_route has bounded
+// n_planes by IXG_MAX_PLANES.  The staging copy lives on the stack: a copy
+// from pageable memory has read its source when hipMemcpyAsync returns.
+ixg_status upload_planes(ixg_ctx* ctx, const ixg_scene* s, float4* d_planes = nullptr) {
+    if (!d_planes) d_planes = ctx->d_planes;
+    float4 pl[IXG_MAX_PLANES + 2];
     for (uint32_t i = 0; i < s->n_planes; ++i)
         pl[i] = make_float4(s->planes[i].normal[0], s->planes[i].normal[1], s->planes[i].normal[2],
                             s->planes[i].d);
     if (s->n_planes)
-        HIP_TRY(ctx, hipMemcpyAsync(ctx->d_planes, pl.data(), pl.size() * sizeof(float4),
-                                    hipMemcpyHostToDevice, ctx->stream));
+        HIP_TRY(ctx, hipMemcpyAsync(d_planes, pl, s->n_planes * sizeof(float4), hipMemcpyHostToDevice,
+                                    ctx->stream));
     if (any_frosted(s)) {  // the FacetFinish bitmask after the facet set
         uint32_t mask[8] = {0, 0, 0, 0, 0, 0, 0, 0};
         for (uint32_t i = 0; i < s->n_planes; ++i)
             if (s->facet_finishes[i]) mask[i >> 5] |= 1u << (i & 31);
-        HIP_TRY(ctx, hipMemcpyAsync(ctx->d_planes + IXG_MAX_PLANES, mask, sizeof mask, hipMemcpyHostToDevice,
-                                    ctx->stream));
+        std::memcpy(&pl[IXG_MAX_PLANES], mask, sizeof mask);
+        HIP_TRY(ctx, hipMemcpyAsync(d_planes + IXG_MAX_PLANES, &pl[IXG_MAX_PLANES], sizeof mask,
+                                    hipMemcpyHostToDevice, ctx->stream));
     }
     return IXG_OK;
 }
@@ -265,18 +278,28 @@ bool take_injected_fault(ixg_ctx* ctx) {
 }
 
 // Runs transport + reduction for every chunk of whole pixels, adding into d_accum.
-ixg_status run_film(ixg_ctx* ctx, const ixg_scene* s, uint32_t sample_offset, uint32_t spp,
-                    double* d_accum, const volatile uint8_t* cancel, bool timed) {
+// The device buffers one film call runs on: the facet table, the chunk's
+// path records (room for xyz_paths paths), the work counter and the
+// panorama slot (null = the context's own, grown on demand).
+struct FilmBuffers {
+    float4* planes;
+    float* xyz;
+    uint64_t xyz_paths;
+    uint32_t* counter;
+    float* hdr;
+    size_t hdr_cap;  // floats
+};
+
+ixg_status run_film_on(ixg_ctx* ctx, const ixg_scene* s, uint32_t sample_offset, uint32_t spp, double* d_accum,
+                       const volatile uint8_t* cancel, bool timed, const FilmBuffers& B) {
     ixg::FilmParams P;
-    ixg_status st = build_params(ctx, s, sample_offset, spp, &P);
+    ixg_status st = build_params(ctx, s, sample_offset, spp, &P, B.hdr, B.hdr_cap);
     if (st != IXG_OK) return st;
-    st = upload_planes(ctx, s);
+    st = upload_planes(ctx, s, B.planes);
     if (st != IXG_OK) return st;
     const uint32_t n_pixels = s->width * s->height;
-    uint64_t pix_per_chunk = std::max<uint64_t>(1, ctx->chunk_paths / spp);
-    pix_per_chunk = std::min<uint64_t>(pix_per_chunk, n_pixels);
-    st = ensure_xyz(ctx, (size_t)pix_per_chunk * spp * 3);
-    if (st != IXG_OK) return st;
+    const uint64_t pix_per_chunk = std::min<uint64_t>(std::max<uint64_t>(1, B.xyz_paths / spp), n_pixels);
+    if (pix_per_chunk * spp > B.xyz_paths) return fail(ctx, IXG_RESOURCE_LIMIT, "spp beyond the path buffer");
     ctx->last_kernel_ms_total = 0.0;
     ctx->last_kernel_launches = 0;
     for (uint64_t first = 0; first < n_pixels; first += pix_per_chunk) {
@@ -286,8 +309,8 @@ ixg_status run_film(ixg_ctx* ctx, const ixg_scene* s, uint32_t sample_offset, ui
         P.first_path = first * spp;
         P.n_paths = (uint64_t)npx * spp;
         if (timed) HIP_TRY(ctx, hipEventRecord(ctx->ev0, ctx->stream));
-        HIP_TRY(ctx, ixg_launch_film_persistent(&P, ctx->d_planes, ctx->d_xyz, ctx->d_counter,
-                                                ctx->persistent_blocks, ctx->stream));
+        HIP_TRY(ctx, ixg_launch_film_persistent(&P, B.planes, B.xyz, B.counter, ctx->persistent_blocks,
+                                                ctx->stream));
         if (timed) {
             HIP_TRY(ctx, hipEventRecord(ctx->ev1, ctx->stream));
             HIP_TRY(ctx, hipEventSynchronize(ctx->ev1));
@@ -296,9 +319,20 @@ ixg_status run_film(ixg_ctx* ctx, const ixg_scene* s, uint32_t sample_offset, ui
             ctx->last_kernel_ms_total += ms;
             ctx->last_kernel_launches += 1;
         }
-        HIP_TRY(ctx, ixg_launch_reduce(ctx->d_xyz, spp, npx, (uint32_t)first, d_accum, ctx->stream));
+        HIP_TRY(ctx, ixg_launch_reduce(B.xyz, spp, npx, (uint32_t)first, d_accum, ctx->stream));
     }
     return IXG_OK;
+}
+
+// run_film_on with the context's own buffers, the path buffer grown to one chunk.
+ixg_status run_film(ixg_ctx* ctx, const ixg_scene* s, uint32_t sample_offset, uint32_t spp,
+                    double* d_accum, const volatile uint8_t* cancel, bool timed) {
+    const uint32_t n_pixels = s->width * s->height;
+    const uint64_t pix_per_chunk = std::min<uint64_t>(std::max<uint64_t>(1, ctx->chunk_paths / spp), n_pixels);
+    ixg_status st = ensure_xyz(ctx, (size_t)pix_per_chunk * spp * 3);
+    if (st != IXG_OK) return st;
+    const FilmBuffers B{ctx->d_planes, ctx->d_xyz, pix_per_chunk * spp, ctx->d_counter, nullptr, 0};
+    return run_film_on(ctx, s, sample_offset, spp, d_accum, cancel, timed, B);
 }
 
 }  // namespace
@@ -671,6 +705,24 @@ ixg_status ixg_count_events(ixg_ctx* ctx, const ixg_scene* s, uint32_t sample_of
     (void)hipFree(d_tot);
     if (e != hipSuccess) return hip_fail(ctx, e, "count_events");
     for (int i = 0; i < ixg::EV_NUM; ++i) totals[i] = host[i];
+    return IXG_OK;
+}
+
+// ixg_accumulate_device on caller-provided device buffers (the static
+// scratch of the allocation-free entry point, film_noalloc.hip): no
+// allocation on any path that returns IXG_OK.
+ixg_status ixg_internal_accumulate_on(ixg_ctx* ctx, const ixg_scene* s, uint32_t sample_offset, uint32_t spp,
+                                      double* d_accum_xyz, float4* d_planes, float* d_xyz, uint64_t xyz_paths,
+                                      uint32_t* d_counter, float* d_hdr, size_t hdr_cap) {
+    ixg_status st = begin_call(ctx, s);
+    if (st != IXG_OK) return st;
+    if (!d_accum_xyz) return fail(ctx, IXG_INVALID, "null accumulator");
+    if (spp == 0) return IXG_OK;
+    if (spp > xyz_paths) return fail(ctx, IXG_RESOURCE_LIMIT, "spp larger than the scratch path capacity");
+    const FilmBuffers B{d_planes, d_xyz, xyz_paths, d_counter, d_hdr, hdr_cap};
+    st = run_film_on(ctx, s, sample_offset, spp, d_accum_xyz, nullptr, true, B);
+    if (st != IXG_OK) return st;
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
     return IXG_OK;
 }
 

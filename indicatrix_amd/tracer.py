@@ -340,6 +340,14 @@ class GpuBackend:
         if st != abi.IXG_OK:
             raise IxgError(st, self.last_error())
 
+    def accumulate_device_noalloc(self, scene, sample_offset, spp, d_accum_ptr):
+        """ixg_accumulate_device_noalloc: the same sums on the library's fixed
+        device scratch, with no allocation (spp <= abi.NOALLOC_MAX_PATHS)."""
+        st = lib().ixg_accumulate_device_noalloc(self._ctx, scene.ref(), sample_offset, spp,
+                                                 C.c_void_p(d_accum_ptr))
+        if st != abi.IXG_OK:
+            raise IxgError(st, self.last_error())
+
     def last_kernel_ms(self):
         return lib().ixg_last_kernel_ms(self._ctx)
 
