@@ -429,6 +429,11 @@ ixg_status ixg_mlp_create(ixg_ctx* ctx, uint32_t max_batch, uint64_t seed, ixg_m
 void ixg_mlp_destroy(ixg_mlp* mlp);
 ixg_status ixg_mlp_set_params(ixg_mlp* mlp, const float* host, size_t count);
 ixg_status ixg_mlp_get_params(ixg_mlp* mlp, float* host, size_t count);
+/* The optimiser state, for a checkpoint that resumes training exactly: the
+ * Adam moments m, v (count = ixg_mlp_param_count() each, FP32) and the step
+ * count behind the bias corrections.  set_params resets them to zero. */
+ixg_status ixg_mlp_get_state(ixg_mlp* mlp, float* m, float* v, size_t count, uint64_t* step);
+ixg_status ixg_mlp_set_state(ixg_mlp* mlp, const float* m, const float* v, size_t count, uint64_t step);
 ixg_status ixg_mlp_infer(ixg_mlp* mlp, const float* d_x, uint32_t batch, float* d_y);
 /* Loss (mean squared error over batch x 3) and its gradient w.r.t. every
  * parameter (host, padded layout), without updating. */

@@ -94,6 +94,8 @@ def _declare(lib):
         "ixg_mlp_destroy": (None, [C.c_void_p]),
         "ixg_mlp_set_params": (st, [C.c_void_p, P(C.c_float), C.c_size_t]),
         "ixg_mlp_get_params": (st, [C.c_void_p, P(C.c_float), C.c_size_t]),
+        "ixg_mlp_get_state": (st, [C.c_void_p, P(C.c_float), P(C.c_float), C.c_size_t, P(C.c_uint64)]),
+        "ixg_mlp_set_state": (st, [C.c_void_p, P(C.c_float), P(C.c_float), C.c_size_t, C.c_uint64]),
         "ixg_mlp_infer": (st, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p]),
         "ixg_mlp_gradients": (st, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, P(C.c_float),
                                    C.c_size_t, P(C.c_float)]),

@@ -196,6 +196,23 @@ class SurrogateMLP:
         self._check(lib().ixg_mlp_set_params(self._h, p.ctypes.data_as(C.POINTER(C.c_float)),
                                              p.size), "set_params")
 
+    def get_state(self):
+        """The Adam moments and step count: (m, v, step)."""
+        m = np.empty(self.n_params, np.float32)
+        v = np.empty(self.n_params, np.float32)
+        step = C.c_uint64()
+        self._check(lib().ixg_mlp_get_state(self._h, m.ctypes.data_as(C.POINTER(C.c_float)),
+                                            v.ctypes.data_as(C.POINTER(C.c_float)), self.n_params,
+                                            C.byref(step)), "get_state")
+        return m, v, int(step.value)
+
+    def set_state(self, m, v, step):
+        m = np.ascontiguousarray(m, np.float32)
+        v = np.ascontiguousarray(v, np.float32)
+        self._check(lib().ixg_mlp_set_state(self._h, m.ctypes.data_as(C.POINTER(C.c_float)),
+                                            v.ctypes.data_as(C.POINTER(C.c_float)), m.size, int(step)),
+                    "set_state")
+
     def _dev(self, a, cols):
         torch = self._torch
         t = torch.as_tensor(a, dtype=torch.float32, device="cuda").contiguous()
@@ -264,6 +281,7 @@ class SurrogateMLP:
             raise ValueError("fewer samples than one batch")
         gen = torch.Generator(device="cuda")
         gen.manual_seed(seed)
+        self.rng_state = None
         steps_per_epoch = n // batch
         total = epochs * steps_per_epoch
         history = []
@@ -279,6 +297,7 @@ class SurrogateMLP:
                 tot += self.train_step(xs[s:s + batch], ts[s:s + batch], r, want_loss=True)
                 k += 1
             history.append(tot / steps_per_epoch)
+            self.rng_state = gen.get_state().cpu().numpy()   # the batch shuffler, for a resume
             if log:
                 log(f"epoch {ep}: loss {history[-1]:.5f}")
         return history
@@ -305,17 +324,26 @@ class SurrogateMLP:
             loss = self.train_step(xs, ts, lr, want_loss=(k == steps - 1))
         return loss
 
+    STATE_KEYS = ("params", "t_mean", "t_std", "adam_m", "adam_v", "adam_step", "rng_state")
+
     def save(self, path, **meta):
-        """Parameters (padded FP32 layout) and output statistics as .npz."""
-        np.savez(path, params=self.get_params(), t_mean=self.t_mean, t_std=self.t_std,
-                 **{k: np.asarray(v) for k, v in meta.items()})
+        """A resumable checkpoint as .npz: parameters (padded FP32 layout),
+        output statistics, the Adam moments and step, and the state of the
+        batch shuffler of the last fit (when there was one)."""
+        m, v, step = self.get_state()
+        extra = {} if getattr(self, "rng_state", None) is None else {"rng_state": self.rng_state}
+        np.savez(path, params=self.get_params(), t_mean=self.t_mean, t_std=self.t_std, adam_m=m, adam_v=v,
+                 adam_step=np.uint64(step), **extra, **{k: np.asarray(v) for k, v in meta.items()})
 
     def load(self, path):
         with np.load(path, allow_pickle=False) as z:
             self.set_params(z["params"])
             self.t_mean = z["t_mean"].astype(np.float32)
             self.t_std = z["t_std"].astype(np.float32)
-            return {k: z[k][()] for k in z.files if k not in ("params", "t_mean", "t_std")}
+            if "adam_m" in z.files:
+                self.set_state(z["adam_m"], z["adam_v"], int(z["adam_step"]))
+            self.rng_state = z["rng_state"] if "rng_state" in z.files else None
+            return {k: z[k][()] for k in z.files if k not in self.STATE_KEYS}
 
 
 def design_features(u, nd, abbe, tilts=TILTS_C3):

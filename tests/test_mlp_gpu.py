@@ -126,3 +126,34 @@ def test_training_fits_a_smooth_target(net):
     assert hist[-1] < 0.1 * hist[0], hist
     pred = net.predict(x[:1024]).cpu().numpy()
     assert np.abs(pred - t[:1024]).mean() < 0.1
+
+
+def test_checkpoint_resumes_adam_exactly(net, tmp_path):
+    """save() keeps the Adam moments and step: two steps, save, two more, and
+    a fresh network loaded from the checkpoint taking the same two steps lands
+    on the same parameters bit for bit."""
+    import torch
+    from indicatrix_amd.surrogate import SurrogateMLP
+    p, x, t = _problem(27, 256)
+    xd = torch.as_tensor(x, device="cuda")
+    td = torch.as_tensor(t, device="cuda")
+    net.set_params(p)
+    torch.cuda.synchronize()
+    for _ in range(2):
+        net.train_step(xd, td, 1e-3, want_loss=False)
+    path = str(tmp_path / "ckpt.npz")
+    net.save(path)
+    m, v, step = net.get_state()
+    assert step == 2 and np.abs(m).sum() > 0 and np.abs(v).sum() > 0
+    for _ in range(2):
+        net.train_step(xd, td, 1e-3, want_loss=False)
+    want = net.get_params()
+    other = SurrogateMLP(net.backend, max_batch=256, seed=99)
+    try:
+        other.load(path)
+        assert other.get_state()[2] == 2
+        for _ in range(2):
+            other.train_step(xd, td, 1e-3, want_loss=False)
+        np.testing.assert_array_equal(other.get_params(), want)
+    finally:
+        other.close()
