@@ -51,7 +51,8 @@ static inline v3 vnormalize_or_zero(v3 a) {
     if (isfinite(r) && r > 0.0f) return vmul(a, r);
     return mk(0.0f, 0.0f, 0.0f);
 }
-static inline v3 vfrom(const float* p) { return mk(p[0], p[1], p[2]); }
+static inline v3 vfrom(const ixo_abi_f32* p) { return mk(p[0], p[1], p[2]); }  /* an ABI float[3] */
+static inline v3 vfromr(const float* p) { return mk(p[0], p[1], p[2]); }      /* an arithmetic float[3] */
 /* Rust f32::clamp: NaN propagates. */
 static inline float clampf(float v, float lo, float hi) {
     if (v < lo) v = lo;
@@ -219,7 +220,7 @@ static float dispersion_unclamped(const ixg_dispersion* d, float lambda_nm) {
     /* dispersion.rs:181-210 */
     float um = lambda_nm * 1e-3f;
     float l2 = um * um;
-    const float* c = d->coeff;
+    const ixo_abi_f32* c = d->coeff;
     switch (d->kind) {
     case IXG_DISP_SELLMEIER1: {
         float n2 = 1.0f + (c[0] * l2) / (l2 - c[1]);
@@ -319,6 +320,9 @@ static v3 von_kries(v3 xyz, v3 scale) { /* color.rs:252-254 */
 }
 
 float ixo_exp_lane(float x) { /* simd/exp_poly.rs:37-60 */
+#ifdef IXO_FP64_TWIN
+    return exp(x);  /* the twin: the function the lane polynomial approximates */
+#endif
     const float LOG2E = 1.44269504088896340736f;
     union { uint32_t u; float f; } c1 = {0x3F318000u};
     const float C2 = -2.1219444e-4f;
@@ -578,7 +582,7 @@ float ixo_sample_studio_environment(const float dir[3], float lambda_nm, int32_t
     ixg_environment src = {preset, exposure, light_yaw, light_pitch, 0.0f, 1.0f, 16.0f};
     env_t e;
     if (!env_build(&src, &e)) return NAN;
-    dir_light L = direction_lighting(&e, vfrom(dir), mk(0, 0, 0));
+    dir_light L = direction_lighting(&e, vfromr(dir), mk(0, 0, 0));
     return light_radiance(&L, spec_power(&e, lambda_nm), exposure);
 }
 
@@ -690,7 +694,7 @@ uint32_t ixo_cut_round_brilliant(float crown_deg, float pavilion_deg, float tabl
 /* slab_scan + intersect_polyhedron_soa (simd/slab.rs:147-160, RT/intersect.rs:101-132). */
 int32_t ixo_intersect(const ixg_facet_plane* planes, uint32_t n, const float o[3], const float d[3],
                       float* t_out) {
-    v3 origin = vfrom(o), dir = vfrom(d);
+    v3 origin = vfromr(o), dir = vfromr(d);
     float t_near = -1e30f, t_far = 1e30f;
     int32_t near_idx = -1, far_idx = -1;
     for (uint32_t i = 0; i < n; ++i) {
@@ -771,7 +775,7 @@ static v3 cosine_weighted_hemisphere(float u1, float u2, v3 n) {
 }
 
 void ixo_cosine_hemisphere(float u1, float u2, const float n[3], float out[3]) {
-    const v3 r = cosine_weighted_hemisphere(u1, u2, vfrom(n));
+    const v3 r = cosine_weighted_hemisphere(u1, u2, vfromr(n));
     out[0] = r.x; out[1] = r.y; out[2] = r.z;
 }
 
@@ -855,7 +859,7 @@ static v3 hg_direction(float u1, float u2, float g, v3 fwd) {
 }
 
 void ixo_hg_direction(float u1, float u2, float g, const float fwd[3], float out[3]) {
-    const v3 r = hg_direction(u1, u2, g, vfrom(fwd));
+    const v3 r = hg_direction(u1, u2, g, vfromr(fwd));
     out[0] = r.x; out[1] = r.y; out[2] = r.z;
 }
 
@@ -948,10 +952,10 @@ static void hdr_rgb(const ixg_hdr_map* H, v3 dir, float out[3]) {
     const int64_t x0i = ((xc % w) + w) % w, x1i = (((xc + 1) % w) + w) % w;
     const int64_t y0i = yc < 0 ? 0 : (yc > h - 1 ? h - 1 : yc);
     const int64_t y1i = yc + 1 < 0 ? 0 : (yc + 1 > h - 1 ? h - 1 : yc + 1);
-    const float* p00 = H->rgb + (y0i * w + x0i) * 3;
-    const float* p10 = H->rgb + (y0i * w + x1i) * 3;
-    const float* p01 = H->rgb + (y1i * w + x0i) * 3;
-    const float* p11 = H->rgb + (y1i * w + x1i) * 3;
+    const ixo_abi_f32* p00 = H->rgb + (y0i * w + x0i) * 3;
+    const ixo_abi_f32* p10 = H->rgb + (y0i * w + x1i) * 3;
+    const ixo_abi_f32* p01 = H->rgb + (y1i * w + x0i) * 3;
+    const ixo_abi_f32* p11 = H->rgb + (y1i * w + x1i) * 3;
     for (int c = 0; c < 3; ++c) {
         const float top = fmaf(p10[c], tx, p00[c] * (1.0f - tx));
         const float bottom = fmaf(p11[c], tx, p01[c] * (1.0f - tx));
@@ -1030,7 +1034,7 @@ static v3 shading_normal_near_edge(const ixg_facet_plane* planes, uint32_t n, v3
 
 void ixo_shading_normal_near_edge(const ixg_facet_plane* planes, uint32_t n, const float hit[3], int32_t facet,
                                   const float hn[3], float radius, float out[3]) {
-    const v3 r = shading_normal_near_edge(planes, n, vfrom(hit), facet, vfrom(hn), radius);
+    const v3 r = shading_normal_near_edge(planes, n, vfromr(hit), facet, vfromr(hn), radius);
     out[0] = r.x; out[1] = r.y; out[2] = r.z;
 }
 
@@ -1779,7 +1783,7 @@ static void facet_event_aniso(const scene_t* S, path_t* P, uint32_t bounce, uint
  * normal faces the incident side ((-k).normal >= 0). */
 void ixo_uniaxial_energy(int kind, float n_inc, float n_o, float n_e, const float c[3], const float kk[3],
                          const float nn[3], int pol, float out[2]) {
-    v3 k = vnormalize(vfrom(kk)), normal = vnormalize(vfrom(nn)), cax = vnormalize(vfrom(c));
+    v3 k = vnormalize(vfromr(kk)), normal = vnormalize(vfromr(nn)), cax = vnormalize(vfromr(c));
     float cos_i = clampf(vdot(vneg(k), normal), 0.0f, 1.0f);
     float sin_i = sqrtf(maxf(fmaf(cos_i, -cos_i, 1.0f), 0.0f));
     uframe F = uframe_build(k, normal, cax, cos_i, sin_i);
@@ -1810,7 +1814,7 @@ void ixo_uniaxial_energy(int kind, float n_inc, float n_o, float n_e, const floa
  * incidence: out = {re, im} of (r_ss, r_sp) for pol 0, (r_ps, r_pp) for 1. */
 void ixo_uniaxial_entry_r(float n_o, float n_e, const float c[3], const float kk[3], const float nn[3], int pol,
                           float out[4]) {
-    v3 k = vnormalize(vfrom(kk)), normal = vnormalize(vfrom(nn)), cax = vnormalize(vfrom(c));
+    v3 k = vnormalize(vfromr(kk)), normal = vnormalize(vfromr(nn)), cax = vnormalize(vfromr(c));
     float cos_i = clampf(vdot(vneg(k), normal), 0.0f, 1.0f);
     float sin_i = sqrtf(maxf(fmaf(cos_i, -cos_i, 1.0f), 0.0f));
     uframe F = uframe_build(k, normal, cax, cos_i, sin_i);
@@ -1933,7 +1937,7 @@ void ixo_trace_ray(const ixg_scene* sc, const float origin[3], const float dir[3
                    float hero_rand, float xyz[3], ixg_path_events* ev) {
     scene_t S;
     if (!scene_build(sc, &S)) { xyz[0] = xyz[1] = xyz[2] = NAN; return; }
-    trace_core(&S, vfrom(origin), vfrom(dir), seed, hero_rand, xyz, ev);
+    trace_core(&S, vfromr(origin), vfromr(dir), seed, hero_rand, xyz, ev);
 }
 
 static void sample_core(const scene_t* S, const ixg_scene* sc, uint32_t pixel, uint32_t sample,
@@ -1947,7 +1951,7 @@ static void sample_core(const scene_t* S, const ixg_scene* sc, uint32_t pixel, u
     cam.width = (float)sc->width;
     cam.height = (float)sc->height;
     ixo_generate_ray(&cam, (float)x, (float)y, jx, jy, o, d);
-    trace_core(S, vfrom(o), vfrom(d), seed, hero, xyz, ev);
+    trace_core(S, vfromr(o), vfromr(d), seed, hero, xyz, ev);
 }
 
 void ixo_trace_sample(const ixg_scene* sc, uint32_t pixel, uint32_t sample, float xyz[3],
@@ -2257,8 +2261,8 @@ static void camera_ray_m(const ixg_camera* c, float sx, float sy, float pw, floa
     cam.height = ph;
     float oo[3], dd[3];
     ixo_generate_ray(&cam, sx, sy, 0.0f, 0.0f, oo, dd);
-    *o = vfrom(oo);
-    *d = vfrom(dd);
+    *o = vfromr(oo);
+    *d = vfromr(dd);
 }
 
 static float fractions2(const ixg_optical_pixel* p) {
@@ -2522,7 +2526,7 @@ int ixo_classify_ray(const ixg_facet_plane* planes, uint32_t n, const ixg_disper
     int hf;
     float a = 0, w = 0;
     int32_t ef;
-    return classify_m(&M, vfrom(origin), vfrom(dir), vfrom(observer), 0, &hf, &a, &w, &ef, NULL);
+    return classify_m(&M, vfromr(origin), vfromr(dir), vfromr(observer), 0, &hf, &a, &w, &ef, NULL);
 }
 
 /* ================================================================== */

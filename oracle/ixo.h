@@ -16,6 +16,11 @@
 extern "C" {
 #endif
 
+/* The element type of the ABI structs' float arrays.  The FP64 twin
+ * (ixo64.c) compiles ixo.c with its arithmetic in double; the scenes it reads
+ * keep f32 fields, which it reads through this type. */
+typedef float ixo_abi_f32;
+
 /* RNG and stratified draws (RT/sampling.rs). */
 uint32_t ixo_hash_u32(uint32_t x);
 float ixo_low_discrepancy_base2(uint32_t n);

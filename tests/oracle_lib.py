@@ -13,7 +13,9 @@ from indicatrix_amd import abi
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 ORACLE_DIR = os.path.join(ROOT, "oracle")
 LIB = os.path.join(ORACLE_DIR, "_build", "libixo.so")
+LIB64 = os.path.join(ORACLE_DIR, "_build", "libixo64.so")
 _lib = None
+_lib64 = None
 
 
 def build():
@@ -111,6 +113,42 @@ def lib():
         L.ixo_solid_outline.argtypes = [P(abi.FacetPlane), C.c_uint32, P(C.c_double), P(C.c_uint32)]
         _lib = L
     return _lib
+
+
+def lib64():
+    """The FP64 twin (oracle/ixo64.c): the same estimator in double precision."""
+    global _lib64
+    if _lib64 is None:
+        if not os.path.exists(LIB64):
+            build()
+        L = C.CDLL(LIB64)
+        P = C.POINTER
+        L.ixo64_accumulate.argtypes = [P(abi.Scene), C.c_uint32, C.c_uint32, P(C.c_double), C.c_int]
+        L.ixo64_trace_paths.argtypes = [P(abi.Scene), C.c_uint32, C.c_uint32, C.c_uint64, C.c_uint64,
+                                        P(C.c_double), P(abi.PathEvents), C.c_int]
+        _lib64 = L
+    return _lib64
+
+
+def accumulate64(scene, sample_offset, spp, nthreads=None):
+    """Per-pixel FP64 sums of the FP64 twin, (pixels, 3)."""
+    n = scene.width * scene.height
+    acc = np.zeros((n, 3), np.float64)
+    lib64().ixo64_accumulate(scene.ref(), sample_offset, spp, acc.ctypes.data_as(C.POINTER(C.c_double)),
+                             nthreads or threads())
+    return acc
+
+
+def trace_paths64(scene, sample_offset, spp, first=0, count=None, nthreads=None):
+    """Per-path XYZ (float64) and events of the FP64 twin."""
+    from indicatrix_amd import tracer
+    total = scene.width * scene.height * spp
+    count = total - first if count is None else count
+    xyz = np.zeros((count, 3), np.float64)
+    ev = (abi.PathEvents * count)()
+    lib64().ixo64_trace_paths(scene.ref(), sample_offset, spp, first, count,
+                              xyz.ctypes.data_as(C.POINTER(C.c_double)), ev, nthreads or threads())
+    return xyz, tracer.events_to_numpy(ev)
 
 
 def metric_image(request):

@@ -338,3 +338,26 @@ def test_edge_rounded_paths_match_oracle(backend, radius, aniso):
     sc.material_abi.edge_rounding_radius = 0.0
     sharp, _ = backend.trace_paths(sc, 0, 8)
     assert not np.array_equal(sharp, g_xyz)     # the rounding does change the image
+
+
+@pytest.mark.parametrize("case", ["c1", "c2"])
+def test_film_error_against_the_fp64_twin(backend, case):
+    """The device film against the FP64 twin of the oracle (the same estimator
+    in double precision), beside the FP32 oracle's own error: C1 the full
+    128 x 128 @ 64 frame, C2 a 128 x 128 @ 16 sub-frame.  The device makes
+    the FP32 oracle's decisions (path for path, above), so its error against
+    FP64 is the FP32 algorithm's: per pixel a few percent at these sample
+    counts, the frame total within 1e-3."""
+    from test_fp64_twin import pixel_error
+    sc, spp = (workloads.film_c1_scene(), 64) if case == "c1" else (workloads.film_c2_scene(128, 128, 16), 16)
+    print(f"film vs FP64 twin {case}", flush=True)
+    g = np.zeros((sc.width * sc.height, 3))
+    assert backend.try_accumulate(sc, 0, spp, g)
+    a32 = oracle_lib.accumulate(sc, 0, spp)
+    a64 = oracle_lib.accumulate64(sc, 0, spp)
+    gm, gp, gx, gf = pixel_error(g, a64)
+    om, op, ox, of = pixel_error(a32, a64)
+    print(f"  GPU  vs FP64: per-pixel Y mean {gm:.3e} p99 {gp:.3e} max {gx:.3e}, frame {gf:.3e}", flush=True)
+    print(f"  FP32 vs FP64: per-pixel Y mean {om:.3e} p99 {op:.3e} max {ox:.3e}, frame {of:.3e}", flush=True)
+    assert gf < 1e-3
+    assert gm <= 1.1 * om + 1e-3 and gp <= 1.2 * op + 1e-2
