@@ -24,6 +24,26 @@
 #define IX_TABLE_QUAL static const
 #include "spectral_tables.h"
 
+/* The reference's AVX2 kernels (simd/slab.rs:381, simd/exp_poly.rs:122): an
+ * 8-wide slab scan over an SoA copy of the facet set and the 8-lane
+ * exponential, each the scalar function's IEEE operations lane by lane (no
+ * contraction; FMA exactly where the scalar writes fmaf), so every result is
+ * bit-identical to the scalar path.  The FP64 twin and the FLOP-counting
+ * build use the scalar path. */
+#if defined(__AVX2__) && defined(__FMA__) && !defined(IXO_SCALAR_ONLY)
+#include <immintrin.h>
+#define IXO_SIMD 1
+#endif
+
+/* The FLOP-counting build (ixo_flops.cpp) counts the f32 operations this
+ * file performs; IXO_FLOPS(n) books the ones it writes through double libm
+ * calls (correctly rounded sin / cos / atan2, the f64 midpoints). */
+#ifndef IXO_FLOPS
+#define IXO_FLOPS(n) ((void)0)
+#endif
+/* a value stored into an ABI struct's f32 field */
+#define ABI_F32(x) ((ixo_abi_f32)(x))
+
 #define NCH 8
 #define F32_PI 3.14159265358979323846f
 
@@ -111,7 +131,7 @@ static mat4 mat_cols(const float a[16]) {
 }
 /* f32::midpoint: correctly rounded (a+b)/2; equals 0.5*(a+b) for the
  * magnitudes reached here, computed in double exactly like the std impl. */
-static inline float midpointf(float a, float b) { return (float)(((double)a + (double)b) / 2.0); }
+static inline float midpointf(float a, float b) { IXO_FLOPS(2); return (float)(((double)a + (double)b) / 2.0); }
 
 /* MuellerMatrix::frame_rotation (polarization.rs:138). */
 static mat4 mueller_frame_rotation(float psi) {
@@ -347,6 +367,33 @@ float ixo_exp_lane(float x) { /* simd/exp_poly.rs:37-60 */
     return p * pw.f;
 }
 
+/* exp_f32x8_avx2 (simd/exp_poly.rs:71, dispatched from :122): ixo_exp_lane on eight lanes. */
+static void exp_lane8(const float x[8], float out[8]) {
+#ifdef IXO_SIMD
+    const __m256 HI = _mm256_set1_ps(88.02875f), LO = _mm256_set1_ps(-87.33654f);
+    __m256 v = _mm256_loadu_ps(x);
+    v = _mm256_max_ps(_mm256_min_ps(v, HI), LO);  /* NaN -> HI, as the scalar selects */
+    const __m256 n = _mm256_floor_ps(_mm256_fmadd_ps(v, _mm256_set1_ps(1.44269504088896340736f), _mm256_set1_ps(0.5f)));
+    const __m256 c1 = _mm256_castsi256_ps(_mm256_set1_epi32(0x3F318000));
+    v = _mm256_fmadd_ps(n, _mm256_xor_ps(c1, _mm256_set1_ps(-0.0f)), v);
+    v = _mm256_fmadd_ps(n, _mm256_set1_ps(2.1219444e-4f), v);
+    const __m256 z = _mm256_mul_ps(v, v);
+    __m256 p = _mm256_set1_ps(1.9875691e-4f);
+    p = _mm256_fmadd_ps(p, v, _mm256_set1_ps(1.3981999e-3f));
+    p = _mm256_fmadd_ps(p, v, _mm256_set1_ps(8.333452e-3f));
+    p = _mm256_fmadd_ps(p, v, _mm256_set1_ps(4.1665796e-2f));
+    p = _mm256_fmadd_ps(p, v, _mm256_set1_ps(1.6666665e-1f));
+    p = _mm256_fmadd_ps(p, v, _mm256_set1_ps(5.0000003e-1f));
+    p = _mm256_add_ps(_mm256_fmadd_ps(p, z, v), _mm256_set1_ps(1.0f));
+    const __m256i e = _mm256_slli_epi32(_mm256_add_epi32(_mm256_cvttps_epi32(n), _mm256_set1_epi32(127)), 23);
+    _mm256_storeu_ps(out, _mm256_mul_ps(p, _mm256_castsi256_ps(e)));
+#else
+    for (int c = 0; c < 8; ++c) out[c] = ixo_exp_lane(x[c]);
+#endif
+}
+
+void ixo_exp_lane8(const float x[8], float out[8]) { exp_lane8(x, out); }
+
 float ixo_band_evaluate(const ixg_band* b, float lambda_nm) { /* optics/absorption.rs:108-138 */
     if (b->shape == 0) {
         float t = (lambda_nm - b->center_nm) / b->width;
@@ -579,7 +626,7 @@ static float light_radiance(const dir_light* L, float sp, float exposure) { /* r
 
 float ixo_sample_studio_environment(const float dir[3], float lambda_nm, int32_t preset,
                                     float exposure, float light_yaw, float light_pitch) {
-    ixg_environment src = {preset, exposure, light_yaw, light_pitch, 0.0f, 1.0f, 16.0f};
+    ixg_environment src = {preset, ABI_F32(exposure), ABI_F32(light_yaw), ABI_F32(light_pitch), 0.0f, 1.0f, 16.0f};
     env_t e;
     if (!env_build(&src, &e)) return NAN;
     dir_light L = direction_lighting(&e, vfromr(dir), mk(0, 0, 0));
@@ -601,13 +648,13 @@ void ixo_camera_new(float yaw, float pitch, float distance, float fov_deg, uint3
     v3 right = vnormalize(vcross(forward, world_up));
     v3 up = vnormalize(vcross(right, forward));
     float fov_tan = tanf(fov_deg * (F32_PI / 180.0f) * 0.5f);
-    out->origin[0] = origin.x; out->origin[1] = origin.y; out->origin[2] = origin.z;
-    out->fov_tan = fov_tan;
-    out->forward[0] = forward.x; out->forward[1] = forward.y; out->forward[2] = forward.z;
-    out->width = (float)width;
-    out->right[0] = right.x; out->right[1] = right.y; out->right[2] = right.z;
-    out->height = (float)height;
-    out->up[0] = up.x; out->up[1] = up.y; out->up[2] = up.z;
+    out->origin[0] = ABI_F32(origin.x); out->origin[1] = ABI_F32(origin.y); out->origin[2] = ABI_F32(origin.z);
+    out->fov_tan = ABI_F32(fov_tan);
+    out->forward[0] = ABI_F32(forward.x); out->forward[1] = ABI_F32(forward.y); out->forward[2] = ABI_F32(forward.z);
+    out->width = ABI_F32((float)width);
+    out->right[0] = ABI_F32(right.x); out->right[1] = ABI_F32(right.y); out->right[2] = ABI_F32(right.z);
+    out->height = ABI_F32((float)height);
+    out->up[0] = ABI_F32(up.x); out->up[1] = ABI_F32(up.y); out->up[2] = ABI_F32(up.z);
     out->num_samples = spp;
 }
 void ixo_generate_ray(const ixg_camera* c, float sx, float sy, float jx, float jy, float o[3],
@@ -624,22 +671,22 @@ void ixo_generate_ray(const ixg_camera* c, float sx, float sy, float jx, float j
 
 static void plane_new(v3 n, float d, ixg_facet_plane* p) { /* geometry/plane.rs:47-54 */
     v3 u = vnormalize(n);
-    p->normal[0] = u.x; p->normal[1] = u.y; p->normal[2] = u.z;
-    p->d = d;
+    p->normal[0] = ABI_F32(u.x); p->normal[1] = ABI_F32(u.y); p->normal[2] = ABI_F32(u.z);
+    p->d = ABI_F32(d);
 }
 /* The cut's sines and cosines are the correctly rounded f32 values (double
  * evaluation, one rounding).  The reference's pinned metrics
  * (golden_regression.rs:207-233) were produced where sin(42.5 deg) rounds
  * correctly; glibc 2.35's sinf returns the f32 one ULP above it, which moves
  * the 16 lower-girdle normals and, through them, the Fire index. */
-static float sin_cr(float x) { return (float)sin((double)x); }
+static float sin_cr(float x) { IXO_FLOPS(1); return (float)sin((double)x); }
 /* Correctly rounded f32 atan2 (double evaluation, one rounding) for the
  * Stokes frame rotation angle.  glibc 2.35's atan2f misrounds some of these
  * arguments by one ULP; with the correctly rounded angle the reference's
  * whole-frame pin (renderer/gpu/pin_tests.rs:128-141, offset 0) matches bit
  * for bit, with glibc's it does not. */
-static float atan2_cr(float y, float x) { return (float)atan2((double)y, (double)x); }
-static float cos_cr(float x) { return (float)cos((double)x); }
+static float atan2_cr(float y, float x) { IXO_FLOPS(1); return (float)atan2((double)y, (double)x); }
+static float cos_cr(float x) { IXO_FLOPS(1); return (float)cos((double)x); }
 static float index_to_azimuth(float index, float teeth) { /* cuts/round_brilliant.rs:65 */
     return 2.0f * F32_PI * (index + 0.0f) / teeth;
 }
@@ -728,6 +775,36 @@ int32_t ixo_intersect(const ixg_facet_plane* planes, uint32_t n, const float o[3
 #define R_UNPOL_SELECT_MIN 0.02f
 #define R_UNPOL_SELECT_MAX 0.98f
 
+/* A facet set laid out for the 8-wide slab scan: planes rounded up to 8,
+ * structure of arrays; padding planes (n = 0, d = -1) are parallel to every
+ * ray and seen from inside, so they never hit and never reject.  n_pad = 0
+ * (the scalar builds, or more planes than the ABI's maximum) scans the
+ * array-of-structs planes with ixo_intersect. */
+typedef struct {
+    uint32_t n_pad;
+#ifdef IXO_SIMD
+    float soa[4][IXG_MAX_PLANES + 8];
+#endif
+} slab_t;
+
+static void slab_fill(slab_t* L, const ixg_facet_plane* planes, uint32_t n) {
+    L->n_pad = 0;
+#ifdef IXO_SIMD
+    if (n > IXG_MAX_PLANES) return;
+    L->n_pad = (n + 7u) & ~7u;
+    for (uint32_t i = 0; i < L->n_pad; ++i) {
+        const int real = i < n;
+        L->soa[0][i] = real ? planes[i].normal[0] : 0.0f;
+        L->soa[1][i] = real ? planes[i].normal[1] : 0.0f;
+        L->soa[2][i] = real ? planes[i].normal[2] : 0.0f;
+        L->soa[3][i] = real ? planes[i].d : -1.0f;
+    }
+#else
+    (void)planes;
+    (void)n;
+#endif
+}
+
 typedef struct {
     const ixg_facet_plane* planes;
     uint32_t n_planes;
@@ -741,7 +818,88 @@ typedef struct {
     v3 ta1, ta2, tcn;   /* AbsorptionTensor3 axes: basis of normalize_or_zero(c_axis) */
     const uint8_t* finish; /* FacetFinish per plane, NULL = all polished */
     const ixg_hdr_map* hdr; /* EnvironmentSource::HdrMap, or NULL */
+    slab_t slab;            /* the facet set for the 8-wide scan */
 } scene_t;
+
+#ifdef IXO_SIMD
+/* slab_scan_avx2 (simd/slab.rs:193, dispatched from :381) over 8 planes at a time: per lane the scalar
+ * scan's operations and strict updates, so each lane keeps its lowest-index
+ * best; the lanes then combine by the same rule (largest entering t, smallest
+ * exiting t, ties to the lowest index).  A parallel plane seen from outside
+ * misses wherever it sits, as the scalar early return does. */
+static int32_t intersect_simd(const slab_t* S, const float o[3], const float d[3], float* t_out) {
+    const __m256 ox = _mm256_set1_ps(o[0]), oy = _mm256_set1_ps(o[1]), oz = _mm256_set1_ps(o[2]);
+    const __m256 dx = _mm256_set1_ps(d[0]), dy = _mm256_set1_ps(d[1]), dz = _mm256_set1_ps(d[2]);
+    const __m256 sign = _mm256_set1_ps(-0.0f), zero = _mm256_setzero_ps(), eps = _mm256_set1_ps(1e-7f);
+    __m256 tn = _mm256_set1_ps(-1e30f), tf = _mm256_set1_ps(1e30f);
+    __m256i in = _mm256_set1_epi32(-1), iff = _mm256_set1_epi32(-1);
+    __m256i idx = _mm256_setr_epi32(0, 1, 2, 3, 4, 5, 6, 7);
+    const __m256i eight = _mm256_set1_epi32(8);
+    int outside = 0;
+    for (uint32_t i = 0; i < S->n_pad; i += 8) {
+        const __m256 nx = _mm256_loadu_ps(&S->soa[0][i]), ny = _mm256_loadu_ps(&S->soa[1][i]);
+        const __m256 nz = _mm256_loadu_ps(&S->soa[2][i]), w = _mm256_loadu_ps(&S->soa[3][i]);
+        const __m256 denom = _mm256_add_ps(_mm256_add_ps(_mm256_mul_ps(nx, dx), _mm256_mul_ps(ny, dy)), _mm256_mul_ps(nz, dz));
+        const __m256 side = _mm256_add_ps(w, _mm256_add_ps(_mm256_add_ps(_mm256_mul_ps(nx, ox), _mm256_mul_ps(ny, oy)),
+                                                           _mm256_mul_ps(nz, oz)));
+        const __m256 t = _mm256_div_ps(_mm256_xor_ps(side, sign), denom);
+        const __m256 valid = _mm256_cmp_ps(_mm256_andnot_ps(sign, denom), eps, _CMP_GT_OQ);
+        outside |= _mm256_movemask_ps(_mm256_andnot_ps(valid, _mm256_cmp_ps(side, zero, _CMP_GT_OQ)));
+        const __m256 entering = _mm256_and_ps(valid, _mm256_cmp_ps(denom, zero, _CMP_LT_OQ));
+        const __m256 up_n = _mm256_and_ps(entering, _mm256_cmp_ps(t, tn, _CMP_GT_OQ));
+        const __m256 up_f = _mm256_andnot_ps(entering, _mm256_and_ps(valid, _mm256_cmp_ps(t, tf, _CMP_LT_OQ)));
+        tn = _mm256_blendv_ps(tn, t, up_n);
+        tf = _mm256_blendv_ps(tf, t, up_f);
+        in = _mm256_castps_si256(_mm256_blendv_ps(_mm256_castsi256_ps(in), _mm256_castsi256_ps(idx), up_n));
+        iff = _mm256_castps_si256(_mm256_blendv_ps(_mm256_castsi256_ps(iff), _mm256_castsi256_ps(idx), up_f));
+        idx = _mm256_add_epi32(idx, eight);
+    }
+    if (outside) return -1;
+    float btn[8], btf[8];
+    int32_t bin[8], bif[8];
+    _mm256_storeu_ps(btn, tn);
+    _mm256_storeu_ps(btf, tf);
+    _mm256_storeu_si256((__m256i*)bin, in);
+    _mm256_storeu_si256((__m256i*)bif, iff);
+    float t_near = -1e30f, t_far = 1e30f;
+    int32_t near_idx = -1, far_idx = -1;
+    for (int k = 0; k < 8; ++k) {
+        if (bin[k] >= 0 && (btn[k] > t_near || (btn[k] == t_near && near_idx >= 0 && bin[k] < near_idx))) {
+            t_near = btn[k];
+            near_idx = bin[k];
+        }
+        if (bif[k] >= 0 && (btf[k] < t_far || (btf[k] == t_far && far_idx >= 0 && bif[k] < far_idx))) {
+            t_far = btf[k];
+            far_idx = bif[k];
+        }
+    }
+    if (t_near > t_far) return -1;
+    if (t_near > 1e-4f) { *t_out = t_near; return near_idx < 0 ? 0 : near_idx; }
+    if (t_far > 1e-4f) { *t_out = t_far; return far_idx < 0 ? 0 : far_idx; }
+    return -1;
+}
+#endif
+
+/* The slab scan: the 8-wide twin where it is built, else the scalar one. */
+static int32_t slab_scan(const slab_t* L, const ixg_facet_plane* planes, uint32_t n, const float o[3],
+                         const float d[3], float* t_out) {
+#ifdef IXO_SIMD
+    if (L->n_pad) return intersect_simd(L, o, d, t_out);
+#else
+    (void)L;
+#endif
+    return ixo_intersect(planes, n, o, d, t_out);
+}
+
+int32_t ixo_slab_scan(const ixg_facet_plane* planes, uint32_t n, const float o[3], const float d[3], float* t_out) {
+    static slab_t L; /* single-threaded test entry; the scene and metric paths own theirs */
+    slab_fill(&L, planes, n);
+    return slab_scan(&L, planes, n, o, d, t_out);
+}
+
+static int32_t scene_intersect(const scene_t* S, const float o[3], const float d[3], float* t_out) {
+    return slab_scan(&S->slab, S->planes, S->n_planes, o, d, t_out);
+}
 
 typedef struct {
     float lambdas[NCH];
@@ -767,6 +925,7 @@ typedef struct {
 static v3 cosine_weighted_hemisphere(float u1, float u2, v3 n) {
     const float r = sqrtf(u1);
     const float theta = (2.0f * F32_PI) * u2;
+    IXO_FLOPS(2);
     const float sin_t = (float)sin((double)theta), cos_t = (float)cos((double)theta);
     const v3 a = fabsf(n.x) > 0.9f ? mk(0.0f, 1.0f, 0.0f) : mk(1.0f, 0.0f, 0.0f);
     const v3 t = vnormalize_or_zero(vsub(a, vmul(n, vdot(n, a))));
@@ -825,13 +984,14 @@ static void apply_absorption(const scene_t* S, path_t* P, float path_len) {
     v3 ea = ordinary_eigen(P->k, S->c_raw);
     v3 eb = vnormalize_or_zero(vcross(P->k, ea));
     float scaled = path_len * S->mat->absorption_path_scale;
+    float arg[NCH], tr[NCH];
     for (int c = 0; c < NCH; ++c) {
         float qa = tensor_qf(S, P->alpha_o[c], P->alpha_e[c], ea);
         float qb = tensor_qf(S, P->alpha_o[c], P->alpha_e[c], eb);
-        float alpha = (float)(((double)qa + (double)qb) / 2.0); /* f32::midpoint */
-        float tr = ixo_exp_lane(-alpha * scaled);
-        P->st[c] = st_scale(P->st[c], tr);
+        arg[c] = -midpointf(qa, qb) * scaled;
     }
+    exp_lane8(arg, tr);
+    for (int c = 0; c < NCH; ++c) P->st[c] = st_scale(P->st[c], tr[c]);
 }
 
 #include "ixo_uniaxial.h"
@@ -851,6 +1011,7 @@ static v3 hg_direction(float u1, float u2, float g, v3 fwd) {
     cos_theta = clampf(cos_theta, -1.0f, 1.0f);
     const float sin_theta = sqrtf(maxf(fmaf(cos_theta, -cos_theta, 1.0f), 0.0f));
     const float phi = (2.0f * F32_PI) * u2;
+    IXO_FLOPS(2);
     const float sin_p = (float)sin((double)phi), cos_p = (float)cos((double)phi);
     const v3 a = fabsf(fwd.x) > 0.9f ? mk(0.0f, 1.0f, 0.0f) : mk(1.0f, 0.0f, 0.0f);
     const v3 t = vnormalize_or_zero(vsub(a, vmul(fwd, vdot(fwd, a))));
@@ -876,6 +1037,7 @@ static int scatter_step(const scene_t* S, path_t* P, uint32_t bounce, uint32_t s
         if (!S->absorbing) { alpha[c] = 0.0f; continue; }
         const float qa = tensor_qf(S, P->alpha_o[c], P->alpha_e[c], ea);
         const float qb = tensor_qf(S, P->alpha_o[c], P->alpha_e[c], eb);
+        IXO_FLOPS(2);
         alpha[c] = (float)(((double)qa + (double)qb) / 2.0);
     }
     const float path_scale = S->mat->absorption_path_scale;
@@ -936,6 +1098,7 @@ static float rem_euclid_f(float x, float m) {
 }
 static void hdr_rgb(const ixg_hdr_map* H, v3 dir, float out[3]) {
     const v3 d = vnormalize_or_zero(dir);
+    IXO_FLOPS(2);
     const float theta = (float)acos((double)clampf(d.y, -1.0f, 1.0f));
     const float phi = (float)atan2((double)d.x, (double)d.z);
     const float v = theta / F32_PI;
@@ -1219,7 +1382,7 @@ static void facet_event(const scene_t* S, path_t* P, uint32_t bounce, uint32_t s
                         po[0] = probe_o.x; po[1] = probe_o.y; po[2] = probe_o.z;
                         pd[0] = dk.x; pd[1] = dk.y; pd[2] = dk.z;
                         if (ev) ev->n_probe++;
-                        if (ixo_intersect(S->planes, S->n_planes, po, pd, &tt) < 0) {
+                        if (scene_intersect(S, po, pd, &tt) < 0) {
                             float env = env_channel(S, dk, P->lambdas[c], P->observer);
                             P->split[c] = fmaf(maxf(st_intensity(transmitted), 0.0f) * 1.0f, env,
                                                P->split[c]);
@@ -1326,10 +1489,10 @@ static void apply_absorption_assigned(const scene_t* S, path_t* P, float path_le
     float n_e_h = e_index_at(S, P->lambdas[0], n_o_h);
     v3 e = assigned_mode_e_field_uniaxial(P->k, S->c_raw, P->is_extra, n_o_h, n_e_h);
     float scaled = path_len * S->mat->absorption_path_scale;
-    for (int c = 0; c < NCH; ++c) {
-        float alpha = tensor_qf(S, P->alpha_o[c], P->alpha_e[c], e);
-        P->st[c] = st_scale(P->st[c], ixo_exp_lane(-alpha * scaled));
-    }
+    float arg[NCH], tr[NCH];
+    for (int c = 0; c < NCH; ++c) arg[c] = -tensor_qf(S, P->alpha_o[c], P->alpha_e[c], e) * scaled;
+    exp_lane8(arg, tr);
+    for (int c = 0; c < NCH; ++c) P->st[c] = st_scale(P->st[c], tr[c]);
 }
 
 /* try_split_exit_channel (refraction/exit_split.rs:128-163) */
@@ -1338,7 +1501,7 @@ static void split_probe(const scene_t* S, path_t* P, v3 hit, int c, v3 dk, float
     v3 o = vadd(hit, vmul(dk, 1e-4f));
     float po[3] = {o.x, o.y, o.z}, pd[3] = {dk.x, dk.y, dk.z}, tt;
     if (ev) ev->n_probe++;
-    if (ixo_intersect(S->planes, S->n_planes, po, pd, &tt) >= 0) return;
+    if (scene_intersect(S, po, pd, &tt) >= 0) return;
     float env = env_channel(S, dk, P->lambdas[c], P->observer);
     P->split[c] = fmaf(maxf(transmitted_i, 0.0f) * 1.0f, env, P->split[c]);
 }
@@ -1826,6 +1989,7 @@ void ixo_uniaxial_entry_r(float n_o, float n_e, const float c[3], const float kk
 }
 
 static int scene_build(const ixg_scene* sc, scene_t* S) {
+    slab_fill(&S->slab, sc->planes, sc->n_planes);
     S->planes = sc->planes;
     S->finish = sc->facet_finishes;
     S->n_planes = sc->n_planes;
@@ -1873,7 +2037,7 @@ static void trace_core(const scene_t* S, v3 origin, v3 dir, uint32_t seed, float
     int stopped = 0;
     for (bounce = 0; bounce < S->max_bounces; ++bounce) {
         float o[3] = {P.org.x, P.org.y, P.org.z}, d[3] = {P.dir.x, P.dir.y, P.dir.z}, t;
-        int32_t f = ixo_intersect(S->planes, S->n_planes, o, d, &t);
+        int32_t f = scene_intersect(S, o, d, &t);
         if (f < 0) { /* handle_miss (bounce_body/miss.rs:33-54) */
             if (S->hdr) { /* sample_environment_channels' HdrMap arm (rig.rs:756-765) */
                 float rgb[3];
@@ -1948,8 +2112,8 @@ static void sample_core(const scene_t* S, const ixg_scene* sc, uint32_t pixel, u
     uint32_t x = pixel % sc->width, y = pixel / sc->width;
     float o[3], d[3];
     ixg_camera cam = sc->camera;
-    cam.width = (float)sc->width;
-    cam.height = (float)sc->height;
+    cam.width = ABI_F32((float)sc->width);
+    cam.height = ABI_F32((float)sc->height);
     ixo_generate_ray(&cam, (float)x, (float)y, jx, jy, o, d);
     trace_core(S, vfromr(o), vfromr(d), seed, hero, xyz, ev);
 }
@@ -1981,9 +2145,9 @@ static void* accumulate_worker(void* arg) { /* hybrid.rs:337-392: thread t owns 
             float xyz[3];
             sample_core(J->S, J->sc, p, J->sample_offset + s, xyz, NULL);
             if (isfinite(xyz[0]) && isfinite(xyz[1]) && isfinite(xyz[2])) { /* sampling.rs:237 */
-                s0 += xyz[0];
-                s1 += xyz[1];
-                s2 += xyz[2];
+                s0 += (double)xyz[0];
+                s1 += (double)xyz[1];
+                s2 += (double)xyz[2];
             }
         }
         J->accum[3 * (size_t)p] += s0;
@@ -2047,6 +2211,7 @@ enum { REASON_NONE = 0, REASON_SHADOW = 1, REASON_ILLUM = 2, REASON_SIDE = 3, RE
 typedef struct {
     const ixg_facet_plane* planes;
     uint32_t n;
+    slab_t slab;
     float nd, nf, nc;
     env_t env;          /* scoring environment (observer-free, default head-shadow cosines) */
     float spec550;
@@ -2098,7 +2263,7 @@ static void trace_entry_m(const mctx_t* M, v3 point, v3 dir, v3 n, float cos_i, 
     for (uint32_t b = 0; b < 10; ++b) {
         F->d_bounces = b;
         float o[3] = {hp.x, hp.y, hp.z}, d[3] = {cur.x, cur.y, cur.z}, t;
-        int32_t f = ixo_intersect(M->planes, M->n, o, d, &t);
+        int32_t f = slab_scan(&M->slab, M->planes, M->n, o, d, &t);
         if (f < 0) { F->reason = REASON_NOHIT; break; }
         path_len += t;
         v3 np = vadd(hp, vmul(cur, t));
@@ -2156,7 +2321,7 @@ enum { CLS_NONE = -1, CLS_WINDOWED = 0, CLS_EXTINCT = 1, CLS_RETURNED = 2, CLS_O
 static int classify_m(const mctx_t* M, v3 org, v3 dir, v3 observer, int want_fire, int* has_fire,
                       float* angle, float* weight, int32_t* entry_facet, fate_t* detail) {
     float o[3] = {org.x, org.y, org.z}, d[3] = {dir.x, dir.y, dir.z}, t;
-    int32_t f = ixo_intersect(M->planes, M->n, o, d, &t);
+    int32_t f = slab_scan(&M->slab, M->planes, M->n, o, d, &t);
     if (f < 0) return CLS_NONE;
     *entry_facet = f;
     v3 hp = vadd(org, vmul(dir, t));
@@ -2229,6 +2394,7 @@ static int mctx_build(const ixg_facet_plane* planes, uint32_t n, const ixg_dispe
                       const ixg_environment* env, mctx_t* M) { /* prepared.rs:64-94, lighting.rs:133-190 */
     M->planes = planes;
     M->n = n;
+    slab_fill(&M->slab, planes, n);
     M->nd = maxf(ixo_dispersion_evaluate(disp, 589.3f), 1.1f);
     M->nf = maxf(ixo_dispersion_evaluate(disp, 486.1f), 1.001f);
     M->nc = maxf(ixo_dispersion_evaluate(disp, 656.3f), 1.001f);
@@ -2257,8 +2423,8 @@ static int mctx_build(const ixg_facet_plane* planes, uint32_t n, const ixg_dispe
 
 static void camera_ray_m(const ixg_camera* c, float sx, float sy, float pw, float ph, v3* o, v3* d) {
     ixg_camera cam = *c;
-    cam.width = pw;
-    cam.height = ph;
+    cam.width = ABI_F32(pw);
+    cam.height = ABI_F32(ph);
     float oo[3], dd[3];
     ixo_generate_ray(&cam, sx, sy, 0.0f, 0.0f, oo, dd);
     *o = vfromr(oo);
@@ -2327,7 +2493,7 @@ int ixo_metric_image(const ixg_metric_request* r, ixg_optical_pixel* pixels, ixg
                 if (cls == CLS_RETURNED && r->want_fire) {
                     if (hf) {
                         if (isfinite(ang) && isfinite(w)) {
-                            p.fire_sum_deg = fmaf(zero_disp ? 0.0f : ang, w, p.fire_sum_deg);
+                            p.fire_sum_deg = ABI_F32(fmaf(zero_disp ? 0.0f : ang, w, p.fire_sum_deg));
                             p.fire_qualified++;
                         } else p.fire_rejected++;
                     } else p.fire_rejected++;
@@ -2365,10 +2531,10 @@ int ixo_metric_image(const ixg_metric_request* r, ixg_optical_pixel* pixels, ixg
         memset(tot, 0, sizeof *tot);
         tot->total = total;
         uint32_t h = total.hits > 1 ? total.hits : 1;
-        tot->windowing_pct = (float)total.classes[0] / (float)h * 100.0f;
-        tot->extinction_pct = (float)total.classes[1] / (float)h * 100.0f;
-        tot->brilliance_pct = (float)total.classes[2] / (float)h * 100.0f;
-        tot->fire_index = maxf(total.fire_sum_deg / (float)h * 275.0f, 0.1f);
+        tot->windowing_pct = ABI_F32((float)total.classes[0] / (float)h * 100.0f);
+        tot->extinction_pct = ABI_F32((float)total.classes[1] / (float)h * 100.0f);
+        tot->brilliance_pct = ABI_F32((float)total.classes[2] / (float)h * 100.0f);
+        tot->fire_index = ABI_F32(maxf(total.fire_sum_deg / (float)h * 275.0f, 0.1f));
         /* image_scintillation_pct (image_scint.rs:261-287) needs every pixel */
         tot->scintillation_pct = NAN;
     }
@@ -2512,7 +2678,7 @@ void ixo_temporal_cameras(const float pose[3], float fov_tan, uint32_t width, ui
     static const float offs[5] = {-3.0f, -1.5f, 0.0f, 1.5f, 3.0f}; /* SCINT_TEMPORAL_YAW_OFFSETS_DEG */
     for (int i = 0; i < 5; ++i) {
         ixo_camera_new(pose[0] + offs[i] * (F32_PI / 180.0f), pose[1], pose[2], 42.0f, width, height, 1, &out[i]);
-        out[i].fov_tan = fov_tan;
+        out[i].fov_tan = ABI_F32(fov_tan);
     }
 }
 
@@ -2841,13 +3007,13 @@ int ixo_solid_outline(const ixg_facet_plane* planes, uint32_t n, double* lw, uin
         P[3 * k] = (float)(v.x * scale);
         P[3 * k + 1] = (float)(v.y * scale);
         P[3 * k + 2] = (float)(v.z * scale);
-        const double r2 = (double)P[3 * k] * P[3 * k] + (double)P[3 * k + 1] * P[3 * k + 1] +
-                          (double)P[3 * k + 2] * P[3 * k + 2];
+        const double r2 = (double)P[3 * k] * (double)P[3 * k] + (double)P[3 * k + 1] * (double)P[3 * k + 1] +
+                          (double)P[3 * k + 2] * (double)P[3 * k + 2];
         radius = fmax(radius, sqrt(r2));
     }
     const double eps = 1e-6 * (double)(float)radius;
     outline_pt* pts = (outline_pt*)malloc(sizeof(outline_pt) * (size_t)nv);
-    for (int k = 0; k < nv; ++k) { pts[k].x = P[3 * k]; pts[k].z = P[3 * k + 2]; pts[k].id = k; }
+    for (int k = 0; k < nv; ++k) { pts[k].x = (double)P[3 * k]; pts[k].z = (double)P[3 * k + 2]; pts[k].id = k; }
     qsort(pts, (size_t)nv, sizeof(outline_pt), cmp_outline);
     int m = 0;
     for (int k = 0; k < nv; ++k) {

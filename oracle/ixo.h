@@ -36,6 +36,8 @@ float ixo_d65(float lambda_nm);
 float ixo_blackbody(float lambda_nm, float temp_k);
 void ixo_white_balance(float temp_k, float out[3]);
 float ixo_exp_lane(float x);
+/* exp_f32x8 (IX/simd/exp_poly.rs:122): ixo_exp_lane on eight lanes (AVX2). */
+void ixo_exp_lane8(const float x[8], float out[8]);
 float ixo_band_evaluate(const ixg_band* b, float lambda_nm);
 void ixo_uniaxial_energy(int kind, float n_inc, float n_o, float n_e, const float c[3], const float k[3],
                          const float normal[3], int pol, float out[2]);
@@ -72,6 +74,10 @@ uint32_t ixo_cut_round_brilliant(float crown_deg, float pavilion_deg, float tabl
 /* intersect_polyhedron_soa (RT/intersect.rs:101): returns facet index or -1. */
 int32_t ixo_intersect(const ixg_facet_plane* planes, uint32_t n, const float origin[3],
                       const float dir[3], float* t_out);
+/* The facet scan the transport uses: the 8-wide slab scan (IX/simd/slab.rs:381)
+ * in the AVX2 build, ixo_intersect in the scalar ones; bit-identical to
+ * ixo_intersect.  Single-threaded (one static SoA table). */
+int32_t ixo_slab_scan(const ixg_facet_plane* planes, uint32_t n, const float o[3], const float d[3], float* t_out);
 
 /* trace_spectral_ray (RT/transport/mod.rs:118) with an explicit ray; events may be NULL. */
 void ixo_trace_ray(const ixg_scene* scene, const float origin[3], const float dir[3],

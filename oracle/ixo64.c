@@ -23,6 +23,7 @@
 #include "../include/ixg.h"
 #include "ixo.h"
 
+#define IXO_SCALAR_ONLY 1
 #define IXO_FP64_TWIN 1
 #define float double
 #define fmaf fma
@@ -56,6 +57,7 @@
 #define ixo_d65 ixo64_d65
 #define ixo_dispersion_evaluate ixo64_dispersion_evaluate
 #define ixo_exp_lane ixo64_exp_lane
+#define ixo_exp_lane8 ixo64_exp_lane8
 #define ixo_generate_ray ixo64_generate_ray
 #define ixo_girdle_half_width ixo64_girdle_half_width
 #define ixo_hash_u32 ixo64_hash_u32
@@ -76,6 +78,7 @@
 #define ixo_scintillation_indices ixo64_scintillation_indices
 #define ixo_scintillation_spatial ixo64_scintillation_spatial
 #define ixo_scintillation_spatial_at ixo64_scintillation_spatial_at
+#define ixo_slab_scan ixo64_slab_scan
 #define ixo_shading_normal_near_edge ixo64_shading_normal_near_edge
 #define ixo_solid_outline ixo64_solid_outline
 #define ixo_temporal_cameras ixo64_temporal_cameras

@@ -14,8 +14,12 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 ORACLE_DIR = os.path.join(ROOT, "oracle")
 LIB = os.path.join(ORACLE_DIR, "_build", "libixo.so")
 LIB64 = os.path.join(ORACLE_DIR, "_build", "libixo64.so")
+LIB_SCALAR = os.path.join(ORACLE_DIR, "_build", "libixo_scalar.so")
+LIB_FLOPS = os.path.join(ORACLE_DIR, "_build", "libixo_flops.so")
 _lib = None
 _lib64 = None
+_lib_scalar = None
+_lib_flops = None
 
 
 def build():
@@ -63,6 +67,10 @@ def lib():
         L.ixo_intersect.restype = C.c_int32
         L.ixo_intersect.argtypes = [P(abi.FacetPlane), C.c_uint32, P(C.c_float), P(C.c_float),
                                     P(C.c_float)]
+        L.ixo_slab_scan.restype = C.c_int32
+        L.ixo_slab_scan.argtypes = [P(abi.FacetPlane), C.c_uint32, P(C.c_float), P(C.c_float),
+                                    P(C.c_float)]
+        L.ixo_exp_lane8.argtypes = [P(C.c_float), P(C.c_float)]
         L.ixo_trace_ray.argtypes = [P(abi.Scene), f3, f3, C.c_uint32, C.c_float, P(C.c_float),
                                     P(abi.PathEvents)]
         L.ixo_generate_ray.argtypes = [P(abi.Camera), C.c_float, C.c_float, C.c_float, C.c_float,
@@ -128,6 +136,51 @@ def lib64():
                                         P(C.c_double), P(abi.PathEvents), C.c_int]
         _lib64 = L
     return _lib64
+
+
+def lib_scalar():
+    """The FP32 oracle built without the AVX2 scan and exponential."""
+    global _lib_scalar
+    if _lib_scalar is None:
+        if not os.path.exists(LIB_SCALAR):
+            build()
+        L = C.CDLL(LIB_SCALAR)
+        P = C.POINTER
+        L.ixo_accumulate.argtypes = [P(abi.Scene), C.c_uint32, C.c_uint32, P(C.c_double), C.c_int]
+        L.ixo_metric_image.restype = C.c_int
+        L.ixo_metric_image.argtypes = [P(abi.MetricRequest), P(abi.OpticalPixel), P(abi.MetricTotals)]
+        L.ixo_exp_lane8.argtypes = [P(C.c_float), P(C.c_float)]
+        _lib_scalar = L
+    return _lib_scalar
+
+
+def lib_flops():
+    """The FLOP-counting build (oracle/ixo_flops.cpp)."""
+    global _lib_flops
+    if _lib_flops is None:
+        if not os.path.exists(LIB_FLOPS):
+            build()
+        L = C.CDLL(LIB_FLOPS)
+        P = C.POINTER
+        L.ixo_flops_accumulate.argtypes = [P(abi.Scene), C.c_uint32, C.c_uint32, P(C.c_double),
+                                           P(C.c_uint64)]
+        L.ixo_flops_metric_image.restype = C.c_int
+        L.ixo_flops_metric_image.argtypes = [P(abi.MetricRequest), P(abi.OpticalPixel), P(abi.MetricTotals),
+                                             P(C.c_uint64)]
+        L.ixo_flops_intersect.restype = C.c_int32
+        L.ixo_flops_intersect.argtypes = [P(abi.FacetPlane), C.c_uint32, P(C.c_float), P(C.c_float),
+                                          P(C.c_float), P(C.c_uint64)]
+        _lib_flops = L
+    return _lib_flops
+
+
+def accumulate_flops(scene, sample_offset, spp):
+    """(per-pixel sums, FP operations) of the counting build, one thread."""
+    acc = np.zeros((scene.width * scene.height, 3), np.float64)
+    n = C.c_uint64(0)
+    lib_flops().ixo_flops_accumulate(scene.ref(), sample_offset, spp,
+                                     acc.ctypes.data_as(C.POINTER(C.c_double)), C.byref(n))
+    return acc, n.value
 
 
 def accumulate64(scene, sample_offset, spp, nthreads=None):
