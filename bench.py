@@ -5,13 +5,16 @@ Optimize.
 One step = one pass of the film hot path over one frame of the workload
 (every pixel x spp paths, absolute sample indices advancing each step),
 accumulated in FP64 into a device-resident buffer; `value` is the film rate.
-The same JSON line carries the sweep (512 seeded designs x 21 tilts, 64x64 @
-32 samples) and the screened Optimize (8192 candidates per step through the
-surrogate, top 64 traced at 128x128 @ 64 samples x 21 tilts) as
-`design_evals`.  Prints ONE JSON line.
+The same JSON line carries the uniaxial 1024x1024 @ 256 frame (`film_c2`),
+the sweep (512 seeded designs x 21 tilts, 64x64 @ 32 samples), the surrogate
+training step (`surrogate_training`) and the screened Optimize (8192
+candidates per step through the surrogate, top 64 traced at 128x128 @ 64
+samples x 21 tilts), each with the roofline of its dominant kernel: exact
+algorithmic FLOPs (indicatrix_amd/flops.py) over that kernel's device time
+from HIP events on the library's stream.  Prints ONE JSON line.
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
-                  [--film-only] [--copy-calibration]
+                  [--film-only] [--copy-calibration] [--part c2|sweep|pose|mlp|optimize]
 """
 import argparse
 import json
@@ -22,8 +25,11 @@ import time
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-MI355X_FP32_PEAK_TFLOPS = 157.3
+MI355X_FP32_PEAK_TFLOPS = 157.3    # FP32 vector ALU (the film and metric kernels)
+MI355X_BF16_PEAK_TFLOPS = 2500.0   # dense BF16 MFMA (the surrogate GEMMs)
 MI355X_HBM_PEAK_GBS = 8000.0
+LIBRARY_CHUNK_PATHS = 1 << 23      # paths per film launch (the library's default chunk)
+HOST_CPU_SHARE = 16                # host threads one GPU's share of the box allows
 # HBM bytes per film_persistent_kernel launch at this workload: FETCH_SIZE x 2.0
 # + WRITE_SIZE x 1.0 (scales from a 1 GiB copy) from two rocprofv3 --pmc runs,
 # recorded with their commands in profiles/film_persistent_kernel_c1.md.
@@ -43,12 +49,52 @@ def parse():
     ap.add_argument("--optimize-steps", type=int, default=8)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a 1 GiB device copy (scales rocprof FETCH/WRITE_SIZE)")
-    ap.add_argument("--part", choices=["c2", "sweep", "pose", "optimize"],
+    ap.add_argument("--mlp-steps", type=int, default=200)
+    ap.add_argument("--part", choices=["c2", "sweep", "pose", "mlp", "optimize"],
                     help="run one secondary workload alone and print its figures (profiling runs)")
     return ap.parse_args()
 
 
 SURROGATE_WEIGHTS = os.path.join(ROOT, "indicatrix_amd", "data", "surrogate.npz")
+
+
+def host_threads():
+    """Threads for the CPU baselines: every core this process may run on, up
+    to one GPU's share of the host."""
+    return max(1, min(len(os.sched_getaffinity(0)), HOST_CPU_SHARE))
+
+
+def host_cpu():
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"model": model, "cpu_count": os.cpu_count(), "affinity": len(os.sched_getaffinity(0)),
+            "share_cap": HOST_CPU_SHARE}
+
+
+def film_roofline(entry, totals, kernel_ms, paths_per_launch, traffic=None):
+    """Roofline of film_persistent_kernel from an exact count (flop_counts.json
+    entry) scaled per path to the paths one launch processes."""
+    from indicatrix_amd import flops
+    dev, algo = flops.film_device_flops(entry, totals)
+    per_path = dev / entry["paths"]
+    launch = per_path * paths_per_launch
+    achieved = launch / (kernel_ms * 1e-3) / 1e12
+    return {"bound": "valu", "pipe": "fp32 vector ALU", "achieved": round(achieved, 3),
+            "peak": MI355X_FP32_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": round(achieved / MI355X_FP32_PEAK_TFLOPS, 4),
+            "traffic": traffic, "kernel": "film_persistent_kernel", "kernel_ms": round(kernel_ms, 4),
+            "flops_per_launch": round(launch), "flops_per_path": round(per_path, 1),
+            "algorithm_flops_per_path": round(algo / entry["paths"], 1),
+            "paths_per_launch": paths_per_launch,
+            "flops_counted_on": f"{entry['width']}x{entry['height']}, sample indices "
+                                f"{entry['sample_offset']}..{entry['sample_offset'] + entry['spp'] - 1}",
+            "events": totals}
 
 
 def copy_calibration():
@@ -67,7 +113,7 @@ def film_c2_rate(backend, steps):
     """C2: the 121-facet seeded oval in a uniaxial pleochroic material,
     1024x1024 @ 256 spp, depth 24 (one full frame per step)."""
     import torch
-    from indicatrix_amd import workloads
+    from indicatrix_amd import flops, workloads
     sc = workloads.film_c2_scene()
     spp = workloads.C2["spp"]
     paths = sc.width * sc.height * spp
@@ -75,20 +121,28 @@ def film_c2_rate(backend, steps):
     backend.accumulate_device(sc, 0, 4, accum.data_ptr())   # warm
     torch.cuda.synchronize()
     t0 = time.perf_counter()
+    launch_ms = []
     for k in range(steps):
         backend.accumulate_device(sc, 4 + k * spp, spp, accum.data_ptr())
+        launch_ms.append(backend.last_kernel_ms())
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
+    entry = flops.counts()["film_c2"]
+    if (entry["width"], entry["height"], entry["planes"]) != (sc.width, sc.height, sc.scene.n_planes):
+        raise RuntimeError("flop_counts.json film_c2 does not describe this frame")
+    totals = backend.count_events(sc, entry["sample_offset"], entry["spp"])
+    px_per_launch = min(max(1, LIBRARY_CHUNK_PATHS // spp), sc.width * sc.height)
+    roof = film_roofline(entry, totals, sum(launch_ms) / len(launch_ms), px_per_launch * spp)
     return {"workload": "film 1024x1024 @ 256 spp, depth 24, 121-facet seeded oval (tier angles U[30,50]), "
                         "uniaxial n_e - n_o = 0.009, pleochroic Gaussian o/e bands, random optic axis",
             "steps": steps, "ms_per_step": round(dt * 1e3 / steps, 2),
-            "value": round(paths * steps / dt / 1e6, 2), "unit": "Msamples/s"}, sc
+            "value": round(paths * steps / dt / 1e6, 2), "unit": "Msamples/s", "roofline": roof}, sc
 
 
 def film_c2_cpu_baseline(sc, target_s=8.0):
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib
-    threads = min(os.cpu_count() or 1, 16)
+    threads = host_threads()
     t0 = time.perf_counter()
     oracle_lib.accumulate(sc, 0, 1, nthreads=threads)
     one = time.perf_counter() - t0
@@ -105,31 +159,57 @@ def film_c2_cpu_baseline(sc, target_s=8.0):
 def sweep_rate(backend, n_designs=512):
     """C3: seeded designs (32..256 planes, Sellmeier-1 from n_D / Abbe) x 21
     tilts, 64x64 @ 32 samples, one ixg_metric_sweep call."""
-    from indicatrix_amd import designs, metrics, tracer
+    from indicatrix_amd import designs, flops, metrics, tracer
     planes, ds, _ = designs.sweep_batch(n_designs)
     env = tracer.LightingPreset.studio("IsoHemisphere", 1.0, 0.0, 0.0)
     metrics.metric_sweep(backend, planes, ds[:16], designs.TILTS_C3, 64, 64, 32, 2.4, env)  # warm
     t0 = time.perf_counter()
     out = metrics.metric_sweep(backend, planes, ds, designs.TILTS_C3, 64, 64, 32, 2.4, env)
     dt = time.perf_counter() - t0
-    return {"designs": n_designs, "tilts": len(designs.TILTS_C3), "seconds": round(dt, 3),
-            "design_evals_per_s": round(n_designs / dt, 2),
-            "mean_metrics": [round(float(v), 4) for v in out.mean(axis=(0, 1))]}, (planes, ds, env)
+    dev_ms = metrics.metric_last_ms(backend)
+    block, tile, grid, graphed = metrics.metric_tuning(backend)
+    result = {"designs": n_designs, "tilts": len(designs.TILTS_C3), "seconds": round(dt, 3),
+              "design_evals_per_s": round(n_designs / dt, 2),
+              "mean_metrics": [round(float(v), 4) for v in out.mean(axis=0).mean(axis=0)],
+              "shape": {"block": block, "tile": tile, "grid_blocks": grid, "graphed": graphed}}
+    cnt = flops.counts()["sweep_c3"]
+    if n_designs == cnt["n_designs"] and [d[1] for d in ds] == cnt["design_planes"]:
+        fl = flops.sweep_flops()
+        achieved = fl / (dev_ms * 1e-3) / 1e12
+        result["roofline"] = {
+            "bound": "valu", "pipe": "fp32 vector ALU", "achieved": round(achieved, 3),
+            "peak": MI355X_FP32_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": round(achieved / MI355X_FP32_PEAK_TFLOPS, 4),
+            "traffic": None, "kernel": f"metric_queue_kernel<{block},{tile}>" if block > 0 else "metric_pixels_kernel",
+            "kernel_ms": round(dev_ms, 3), "flops_per_launch": fl,
+            "note": "one launch covers the whole sweep; kernel_ms includes the per-image reductions"}
+    return result, (planes, ds, env)
 
 
-def sweep_cpu_baseline(planes, ds, env):
-    """The oracle's evaluate_optical_image per (design, tilt) on the host's
-    cores for the first design of the sweep (21 images)."""
+def sweep_cpu_baseline(planes, ds, env, n_sample=8):
+    """The oracle's evaluate_optical_image on the host's cores for a stated
+    subset of the sweep: n_sample designs at evenly spaced ranks of their plane
+    count, x 21 tilts.  The rate is the subset's; scaled by the exact FLOP
+    counts of the subset and of the whole sweep it gives the whole-sweep
+    estimate beside it."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib
-    from indicatrix_amd import designs
-    threads = min(os.cpu_count() or 1, 16)
+    from indicatrix_amd import designs, flops
+    threads = host_threads()
+    order = sorted(range(len(ds)), key=lambda i: (ds[i][1], i))
+    pick = [order[int((k + 0.5) * len(ds) / n_sample)] for k in range(n_sample)]
     t0 = time.perf_counter()
-    oracle_lib.metric_sweep(planes, ds[:1], designs.TILTS_C3, 64, 64, 32, 2.4, env, nthreads=threads)
+    oracle_lib.metric_sweep(planes, [ds[i] for i in pick], designs.TILTS_C3, 64, 64, 32, 2.4, env,
+                            nthreads=threads)
     dt = time.perf_counter() - t0
-    return {"value": round(1.0 / dt, 4), "unit": "design evals/s", "cores": threads, "kind": "port",
-            "sample": f"design 0 of the sweep ({ds[0][1]} planes) x 21 tilts, 64x64 @ 32 samples: "
-                      f"{dt:.2f} s on {threads} threads"}
+    out = {"value": round(n_sample / dt, 4), "unit": "design evals/s", "cores": threads, "kind": "port",
+           "sample": f"{n_sample} designs at evenly spaced plane-count ranks "
+                     f"({', '.join(str(ds[i][1]) for i in pick)} planes) x 21 tilts, 64x64 @ 32 samples: "
+                     f"{dt:.2f} s on {threads} threads", "host": host_cpu()}
+    cnt = flops.counts()["sweep_c3"]
+    if len(ds) == cnt["n_designs"]:
+        sub = sum(cnt["design_flops"][i] for i in pick)
+        out["whole_sweep_estimate"] = round(len(ds) / (dt * cnt["sweep_flops"] / sub), 4)
+    return out
 
 
 AXIS_AZIMUTHS_DEG = [0.0, 45.0, 90.0, 135.0]   # PROFILE_AZIMUTHS_DEG (color/metrics/profile.rs:26)
@@ -166,13 +246,55 @@ def pose_sweep_cpu_baseline(srb, disp, env):
     host's cores, one pose per task."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib
-    threads = min(os.cpu_count() or 1, 16)
+    threads = host_threads()
     poses = [p for az in AXIS_AZIMUTHS_DEG for p in oracle_lib.axis_poses(az)]
     t0 = time.perf_counter()
     oracle_lib.pose_metrics(srb, disp, env, poses, want_all=False, nthreads=threads)
     dt = time.perf_counter() - t0
     return {"value": round(len(poses) / dt, 1), "unit": "poses/s", "cores": threads, "kind": "port",
             "sample": f"the SRB tilt sweep, {len(poses)} Fire-less poses in {dt:.3f} s on {threads} threads"}
+
+
+def mlp_train_rate(backend, steps, batch=4096):
+    """C4: surrogate training steps -- BF16 MFMA forward and backward, FP32
+    master weights and Adam -- at batch 4096 on a fixed synthetic batch,
+    timed with HIP events on the library's stream around `steps` steps
+    launched back to back."""
+    import torch
+    from indicatrix_amd import flops, surrogate
+    net = surrogate.SurrogateMLP(backend, max_batch=batch, seed=0)
+    g = torch.Generator(device="cuda").manual_seed(0)
+    x = torch.randn(batch, surrogate.N_IN, device="cuda", generator=g)
+    t = torch.randn(batch, surrogate.N_OUT, device="cuda", generator=g)
+    torch.cuda.synchronize()
+    for _ in range(10):
+        net.train_step(x, t, 1e-3, want_loss=False)
+    stream = torch.cuda.ExternalStream(backend.stream())
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    net.sync()
+    t0 = time.perf_counter()
+    e0.record(stream)
+    for _ in range(steps):
+        net.train_step(x, t, 1e-3, want_loss=False)
+    e1.record(stream)
+    e1.synchronize()
+    wall = time.perf_counter() - t0
+    ms = e0.elapsed_time(e1) / steps
+    loss = net.train_step(x, t, 1e-3, want_loss=True)
+    net.close()
+    fl = flops.mlp_train_step_flops(batch)
+    achieved = fl / (ms * 1e-3) / 1e12
+    return {"workload": "MLP 16 -> 4 x 256 (GELU) -> 3, BF16 weights/activations, FP32 accumulation, "
+                        "master weights and Adam, batch 4096",
+            "steps": steps, "ms_per_step": round(ms, 4), "wall_ms_per_step": round(wall * 1e3 / steps, 4),
+            "samples_per_s": round(batch / (ms * 1e-3)), "final_loss": round(float(loss), 5),
+            "roofline": {"bound": "mfma", "pipe": "bf16 dense MFMA", "achieved": round(achieved, 3),
+                         "peak": MI355X_BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
+                         "frac": round(achieved / MI355X_BF16_PEAK_TFLOPS, 4), "traffic": None,
+                         "kernel": "mlp_forward/backward/dw/reduce (one training step)",
+                         "kernel_ms": round(ms, 4), "flops_per_launch": fl,
+                         "note": "flops = 2 M N K of the forward, backward-data and weight-gradient GEMMs "
+                                 "over the logical layer shapes; time = the step's four launches"}}
 
 
 def screened_rate(backend, steps):
@@ -223,7 +345,7 @@ def cpu_baseline(scene, target_s):
     0..spp-1), sized from a timed 1-spp pass."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib
-    threads = min(os.cpu_count() or 1, 16)
+    threads = host_threads()
     t0 = time.perf_counter()
     oracle_lib.accumulate(scene, 0, 1, nthreads=threads)
     one = time.perf_counter() - t0
@@ -235,7 +357,8 @@ def cpu_baseline(scene, target_s):
     return {"value": round(paths / dt / 1e6, 4), "unit": "Msamples/s", "cores": threads,
             "kind": "port",
             "sample": f"the bench frame at {spp} spp (sample indices 0..{spp - 1}): {paths} paths "
-                      f"in {dt:.2f} s on {threads} threads"}
+                      f"in {dt:.2f} s on {threads} threads (AVX2 slab scan and exponential, -O3)",
+            "host": host_cpu()}
 
 
 def main():
@@ -258,6 +381,8 @@ def main():
             result = sweep_rate(backend)[0]
         elif args.part == "pose":
             result = pose_sweep_rate(backend)[0]
+        elif args.part == "mlp":
+            result = mlp_train_rate(backend, args.mlp_steps)
         else:
             result = screened_rate(backend, args.optimize_steps)
         print(json.dumps({"part": args.part, "result": result}), flush=True)
@@ -280,9 +405,12 @@ def main():
         backend.accumulate_device(scene, i * spp, spp, ptr)
     torch.cuda.synchronize()
 
-    # algorithmic FLOPs of one step (event counts are exact per path; untimed)
-    totals = backend.count_events(scene, 0, spp)
-    step_flops = flops.film_flops(totals, scene.scene.n_planes)
+    # exact algorithmic FLOPs of this frame (sample indices 0..63) with the
+    # device's own event totals for the same paths (untimed)
+    entry = flops.counts()["film_c1"]
+    if (entry["width"], entry["height"], entry["spp"], entry["planes"]) != (w, h, spp, scene.scene.n_planes):
+        raise RuntimeError("flop_counts.json film_c1 does not describe this frame")
+    totals = backend.count_events(scene, entry["sample_offset"], entry["spp"])
 
     kernel_ms = []
     torch.cuda.synchronize()
@@ -296,7 +424,7 @@ def main():
     ms_per_step = dt * 1e3 / args.steps
     value = paths_per_step * args.steps / dt / 1e6
     avg_kernel_ms = sum(kernel_ms) / len(kernel_ms)
-    achieved = step_flops / (avg_kernel_ms * 1e-3) / 1e12
+    roofline = film_roofline(entry, totals, avg_kernel_ms, paths_per_step, round(FILM_KERNEL_HBM_BYTES))
     line = {
         "metric": "Msamples/s (8-lambda spectral paths)",
         "value": round(value, 3),
@@ -314,12 +442,7 @@ def main():
                                "(crown 34.5, pavilion 40.75, table 57%), Diamond Sellmeier-3, "
                                "uniform D65 hemisphere, full Mueller transport",
                    "paths_per_step": paths_per_step, "planes": scene.scene.n_planes},
-        "roofline": {"bound": "mfma", "pipe": "fp32-valu (non-matrix FP32)",
-                     "achieved": round(achieved, 3), "peak": MI355X_FP32_PEAK_TFLOPS,
-                     "unit": "TFLOP/s", "frac": round(achieved / MI355X_FP32_PEAK_TFLOPS, 4),
-                     "traffic": round(FILM_KERNEL_HBM_BYTES),
-                     "kernel": "film_persistent_kernel", "kernel_ms": round(avg_kernel_ms, 4),
-                     "flops_per_launch": step_flops, "events": totals},
+        "roofline": roofline,
     }
     if not args.film_only:
         print("# bench: film C2", file=sys.stderr, flush=True)
@@ -327,6 +450,8 @@ def main():
         line["film_c2"] = c2
         print("# bench: metric sweep", file=sys.stderr, flush=True)
         sweep, sweep_inputs = sweep_rate(backend)
+        print("# bench: surrogate training", file=sys.stderr, flush=True)
+        line["surrogate_training"] = mlp_train_rate(backend, args.mlp_steps)
         print("# bench: screened optimize", file=sys.stderr, flush=True)
         line["design_evals"] = {"sweep": sweep, "screened_optimize": screened_rate(backend, args.optimize_steps)}
         print("# bench: pose sweep", file=sys.stderr, flush=True)

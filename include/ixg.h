@@ -275,6 +275,11 @@ ixg_status ixg_count_events(ixg_ctx* ctx, const ixg_scene* scene, uint32_t sampl
 /* Average duration (ms) of the last film call's transport kernel launches,
  * measured with HIP events on the library's stream. */
 double ixg_last_kernel_ms(const ixg_ctx* ctx);
+/* The HIP stream (a hipStream_t) every call on this context launches on, so
+ * a caller can order its own work and timing events against the library's.
+ * No counterpart in the reference, whose GPU queue is internal to its
+ * renderer; it is for hosts that share the device with other HIP work. */
+void* ixg_stream(ixg_ctx* ctx);
 uint32_t ixg_last_kernel_launches(const ixg_ctx* ctx);
 
 /* ---- metrics ----
@@ -385,6 +390,11 @@ ixg_status ixg_set_metric_mode(ixg_ctx* ctx, int use_graph, int block, int tile)
 /* The shape the last sweep ran with (block -1 = 2-D grid) and whether it was
  * replayed from a graph. */
 ixg_status ixg_metric_tuning(ixg_ctx* ctx, int* block, int* tile, uint32_t* grid_blocks, int* graphed);
+/* Device time (ms) of the last ixg_metric_sweep's launches -- the pixel
+ * kernel (or its graph) and the per-image reductions, autotuning excluded --
+ * measured with HIP events on the library's stream (no reference
+ * counterpart). */
+double ixg_metric_last_ms(ixg_ctx* ctx);
 
 /* measure_model_width (IX/render_setup/stone_width.rs:27-42): the short axis
  * of the solid's x/z vertex bounding box (IX/geometry/stone_metrics/

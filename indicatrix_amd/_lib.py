@@ -47,6 +47,7 @@ def _declare(lib):
         "ixg_film_samples": (st, [C.c_void_p, P(abi.Scene), C.c_uint32, C.c_uint32, P(C.c_float)]),
         "ixg_trace_rays": (st, [C.c_void_p, P(abi.Scene), P(C.c_float), C.c_uint32,
                                 P(C.c_float), P(abi.PathEvents)]),
+        "ixg_stream": (C.c_void_p, [C.c_void_p]),
         "ixg_count_events": (st, [C.c_void_p, P(abi.Scene), C.c_uint32, C.c_uint32,
                                   P(C.c_uint64)]),
         "ixg_metric_image": (st, [C.c_void_p, P(abi.MetricRequest), P(abi.OpticalPixel),
@@ -63,6 +64,7 @@ def _declare(lib):
         "ixg_scintillation_index_at": (C.c_int, [P(abi.OpticalPixel), C.c_uint32, C.c_uint32, C.c_uint32,
                                                  C.c_uint32, P(C.c_float)]),
         "ixg_set_metric_mode": (st, [C.c_void_p, C.c_int, C.c_int, C.c_int]),
+        "ixg_metric_last_ms": (C.c_double, [C.c_void_p]),
         "ixg_metric_tuning": (st, [C.c_void_p, P(C.c_int), P(C.c_int), P(C.c_uint32), P(C.c_int)]),
         "ixg_model_width": (st, [P(abi.FacetPlane), C.c_uint32, P(C.c_double)]),
         "ixg_pose_metrics": (st, [C.c_void_p, P(abi.FacetPlane), C.c_uint32, P(abi.Design), C.c_uint32,

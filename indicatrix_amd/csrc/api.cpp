@@ -737,6 +737,8 @@ void** ixg_internal_metric_plan(ixg_ctx* ctx, void (*free_fn)(void*)) {
     return &ctx->metric_plan;
 }
 
+void* ixg_stream(ixg_ctx* ctx) { return ctx ? static_cast<void*>(ctx->stream) : nullptr; }
+
 double ixg_last_kernel_ms(const ixg_ctx* ctx) {
     if (!ctx || ctx->last_kernel_launches == 0) return 0.0;
     return ctx->last_kernel_ms_total / ctx->last_kernel_launches;

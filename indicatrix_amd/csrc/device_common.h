@@ -338,31 +338,44 @@ __device__ __forceinline__ float aset_radiance(const EnvParams& E, V3 d, float l
 }
 
 // Wavelength-independent lighting of one exit direction (rig.rs:379-405).
+// The reference appends the lit spots to a list (key, fill, then the ring in
+// order) and folds them into each channel in that order; here every spot has
+// a fixed slot (0 key, 1 fill, 2 + i ring light i) and a presence bit, so
+// all indices are compile-time constants and the list lives in registers.
+// Folding the present slots in slot order is the list's order.
 struct DirLight {
     float bg;       // studio: backdrop term; lit models: the scaled factor
     float lights[18];
-    int n_lights;
+    uint32_t mask;  // bit k: slot k is lit
     float horizon, ground;  // studio with a tray: fma(horizon, rig, ground * sp)
 };
 
 __device__ __forceinline__ void direction_lighting(const EnvParams& E, V3 dir, V3 observer,
                                                    DirLight& L) {
     V3 d = normalize(dir);
-    L.n_lights = 0;
+    L.mask = 0u;
     if (E.model != ENV_STUDIO) {
         L.bg = lit_factor_unit(E, d, observer);
         return;
     }
     L.bg = fmaxf(fma_(0.012f, fma_(d.y, 0.5f, 0.5f), 0.015f), 0.005f) * E.exposure;
     float key_dot = fmaxf(dot(d, v3(E.key[0], E.key[1], E.key[2])), 0.0f);
-    if (key_dot > 0.0f) L.lights[L.n_lights++] = powi<28>(key_dot) * 12.0f * E.spot_mult * E.exposure;
+    if (key_dot > 0.0f) {
+        L.lights[0] = powi<28>(key_dot) * 12.0f * E.spot_mult * E.exposure;
+        L.mask |= 1u;
+    }
     float fill_dot = fmaxf(dot(d, v3(E.fill[0], E.fill[1], E.fill[2])), 0.0f);
-    if (fill_dot > 0.0f) L.lights[L.n_lights++] = powi<18>(fill_dot) * 4.5f * E.exposure;
+    if (fill_dot > 0.0f) {
+        L.lights[1] = powi<18>(fill_dot) * 4.5f * E.exposure;
+        L.mask |= 2u;
+    }
+#pragma unroll
     for (int i = 0; i < 16; ++i) {
         float rd = fmaxf(dot(d, v3(E.ring[i][0], E.ring[i][1], E.ring[i][2])), 0.0f);
         if (rd > 0.96f) {
             float spark = (rd - 0.96f) / 0.04f;
-            L.lights[L.n_lights++] = powi<6>(spark) * 22.0f * E.spot_mult * E.exposure;
+            L.lights[2 + i] = powi<6>(spark) * 22.0f * E.spot_mult * E.exposure;
+            L.mask |= 4u << i;
         }
     }
     if (E.tray > 0.0f) {  // the tray under the rig (rig.rs:358-364)
@@ -374,7 +387,9 @@ __device__ __forceinline__ void direction_lighting(const EnvParams& E, V3 dir, V
 __device__ __forceinline__ float light_radiance(const EnvParams& E, const DirLight& L, float sp) {
     if (E.model != ENV_STUDIO) return L.bg * (sp * E.exposure);
     float r = L.bg * sp;
-    for (int i = 0; i < L.n_lights; ++i) r = fma_(L.lights[i], sp, r);
+#pragma unroll
+    for (int i = 0; i < 18; ++i)
+        if (L.mask & (1u << i)) r = fma_(L.lights[i], sp, r);
     if (E.tray > 0.0f) r = fma_(L.horizon, r, L.ground * sp);
     return r;
 }
@@ -402,24 +417,46 @@ __device__ __forceinline__ float lit_factor(const EnvParams& E, V3 dir, V3 obser
 // it (the reference's early miss) is remembered and reported after the loop,
 // which returns the same result because the early return has no side effects.
 // Straight-line code keeps the wave's lanes (different rays) in lock step.
+// One plane of the scan, folded into the running bests.
+struct ScanState {
+    float t_near, t_far;
+    int near_idx, far_idx;
+    bool outside_parallel;
+};
+__device__ __forceinline__ void scan_plane(ScanState& R, float4 p, int i, V3 o, V3 d) {
+    const float denom = (p.x * d.x + p.y * d.y) + p.z * d.z;
+    const float side = p.w + ((p.x * o.x + p.y * o.y) + p.z * o.z);
+    const float t = div_rn(-side, denom);
+    const bool valid = fabsf(denom) > 1e-7f;
+    const bool up_near = valid && denom < 0.0f && t > R.t_near;
+    const bool up_far = valid && !(denom < 0.0f) && t < R.t_far;
+    R.t_near = up_near ? t : R.t_near;
+    R.near_idx = up_near ? i : R.near_idx;
+    R.t_far = up_far ? t : R.t_far;
+    R.far_idx = up_far ? i : R.far_idx;
+    R.outside_parallel = R.outside_parallel || (!valid && side > 0.0f);
+}
+
+// UNROLL planes per iteration, their loads issued together (the facet set is
+// wave-uniform: from LDS one broadcast read each, from memory one scalar load
+// each), then folded in plane order -- the same updates in the same order.
+template <int UNROLL = 1>
 __device__ __forceinline__ int intersect(const float4* __restrict__ pl, uint32_t n, V3 o, V3 d, float& t_out) {
-    float t_near = -1e30f, t_far = 1e30f;
-    int near_idx = -1, far_idx = -1;
-    bool outside_parallel = false;
-    for (uint32_t i = 0; i < n; ++i) {
-        const float4 p = pl[i];
-        const float denom = (p.x * d.x + p.y * d.y) + p.z * d.z;
-        const float side = p.w + ((p.x * o.x + p.y * o.y) + p.z * o.z);
-        const float t = div_rn(-side, denom);
-        const bool valid = fabsf(denom) > 1e-7f;
-        const bool up_near = valid && denom < 0.0f && t > t_near;
-        const bool up_far = valid && !(denom < 0.0f) && t < t_far;
-        t_near = up_near ? t : t_near;
-        near_idx = up_near ? (int)i : near_idx;
-        t_far = up_far ? t : t_far;
-        far_idx = up_far ? (int)i : far_idx;
-        outside_parallel = outside_parallel || (!valid && side > 0.0f);
+    ScanState R{-1e30f, 1e30f, -1, -1, false};
+    uint32_t i = 0;
+    if (UNROLL > 1) {
+        for (; i + UNROLL <= n; i += UNROLL) {
+            float4 q[UNROLL];
+#pragma unroll
+            for (int u = 0; u < UNROLL; ++u) q[u] = pl[i + u];
+#pragma unroll
+            for (int u = 0; u < UNROLL; ++u) scan_plane(R, q[u], (int)(i + u), o, d);
+        }
     }
+    for (; i < n; ++i) scan_plane(R, pl[i], (int)i, o, d);
+    const float t_near = R.t_near, t_far = R.t_far;
+    const int near_idx = R.near_idx, far_idx = R.far_idx;
+    const bool outside_parallel = R.outside_parallel;
     if (outside_parallel) return -1;
     if (t_near > t_far) return -1;
     if (t_near > 1e-4f) {

@@ -25,6 +25,22 @@ namespace ixg {
 
 constexpr int kMBlock = 256;
 
+// Occupancy target of the work-queue and pose kernels (waves per SIMD; 0 =
+// the compiler's choice).  Measured on the 512-design sweep (64x64 @ 32, 21
+// tilts): 4 waves (no spills, 97-122 VGPRs) 2.94 s, 5 waves 2.59 s, 6 waves
+// 2.44 s, 7 waves (72 VGPRs, 80 B/lane spilled) 2.37-2.43 s -- the scan is
+// latency-bound on its LDS reads, so occupancy beats spill-free code.  Reading
+// the facet set with wave-uniform scalar loads in place of LDS, and loading 4
+// planes per scan step, were both slower (2.74-3.28 s).
+#ifndef IXG_METRIC_WAVES_PER_EU
+#define IXG_METRIC_WAVES_PER_EU 7
+#endif
+#if IXG_METRIC_WAVES_PER_EU > 0
+#define IXG_METRIC_WAVES_ATTR __attribute__((amdgpu_waves_per_eu(IXG_METRIC_WAVES_PER_EU)))
+#else
+#define IXG_METRIC_WAVES_ATTR
+#endif
+
 enum { FATE_ENTRY_BLOCKED = 0, FATE_LEAKED = 1, FATE_UPWARD = 2, FATE_ABSORBED = 3 };
 enum { REASON_NONE = 0, REASON_SHADOW = 1, REASON_ILLUM = 2, REASON_SIDE = 3, REASON_NOHIT = 4,
        REASON_BUDGET = 5, REASON_ENTRY = 6 };
@@ -138,7 +154,7 @@ struct Classified {
 // centred on: -dir for a perspective pixel, -forward for the fan.
 __device__ __forceinline__ void classify_ray(const MetricShared& S, const float4* __restrict__ pl, uint32_t n_planes,
                                              float nd, float nf, float nc, V3 org, V3 dir, V3 observer,
-                                             bool want_fire, Classified& C, Fate* detail) {
+                                             bool want_fire, Classified& C, Fate& F) {
     C.cls = CLS_NONE;
     C.has_fire = 0;
     C.entry_facet = -1;
@@ -150,19 +166,17 @@ __device__ __forceinline__ void classify_ray(const MetricShared& S, const float4
     float4 pf = pl[f];
     V3 nrm = v3(pf.x, pf.y, pf.z);
     float cos_i = clampf(dot(neg(dir), nrm), 0.0f, 1.0f);
-    Fate F;
     trace_entry(pl, n_planes, hp, dir, nrm, cos_i, nd, F);
-    if (detail) *detail = F;
     if (F.fate == FATE_ENTRY_BLOCKED) { C.cls = CLS_OTHER; return; }
     if (F.fate == FATE_LEAKED) { C.cls = CLS_WINDOWED; return; }
     if (F.fate == FATE_ABSORBED) { C.cls = CLS_EXTINCT; return; }
     if (dot(F.dir, observer) > S.head_cos) {
-        if (detail) detail->reason = REASON_SHADOW;
+        F.reason = REASON_SHADOW;
         C.cls = CLS_EXTINCT;
         return;
     }
     if (!(scoring_radiance(S, F.dir) > S.threshold)) {
-        if (detail) detail->reason = REASON_ILLUM;
+        F.reason = REASON_ILLUM;
         C.cls = CLS_EXTINCT;
         return;
     }
@@ -185,8 +199,8 @@ __device__ __forceinline__ void classify_ray(const MetricShared& S, const float4
 
 __device__ __forceinline__ void classify(const MetricShared& S, const MetricImage& I,
                                          const float4* __restrict__ pl, V3 org, V3 dir,
-                                         bool want_fire, Classified& C, Fate* detail) {
-    classify_ray(S, pl, I.n_planes, I.nd, I.nf, I.nc, org, dir, neg(dir), want_fire, C, detail);
+                                         bool want_fire, Classified& C, Fate& F) {
+    classify_ray(S, pl, I.n_planes, I.nd, I.nf, I.nc, org, dir, neg(dir), want_fire, C, F);
 }
 
 // Camera::generate_ray with zero jitter (camera.rs:149-167).
@@ -230,7 +244,7 @@ __device__ __forceinline__ ixg_optical_pixel metric_pixel(const MetricShared& S,
 
     ixg_optical_pixel px;
     px.hits = 0;
-    px.classes[0] = px.classes[1] = px.classes[2] = px.classes[3] = 0;
+    uint32_t n_cls[4] = {0u, 0u, 0u, 0u};  // summed by selects: no dynamic index
     px.fire_sum_deg = 0.0f;
     px.fire_qualified = px.fire_rejected = 0;
     px.centre_entry_facet = -1;
@@ -251,7 +265,7 @@ __device__ __forceinline__ ixg_optical_pixel metric_pixel(const MetricShared& S,
         cam_ray(I.forward, I.right, I.up, I.fov_tan, sx, sy, pw, ph, dir);
         Classified C;
         Fate det;
-        classify(S, I, sp, org, dir, S.want_fire != 0, C, s == 0 ? &det : nullptr);
+        classify(S, I, sp, org, dir, S.want_fire != 0, C, det);
         if (C.cls == CLS_NONE) continue;
         px.hits++;
         if (s == 0) {
@@ -270,8 +284,11 @@ __device__ __forceinline__ ixg_optical_pixel metric_pixel(const MetricShared& S,
                 px.fire_rejected++;
             }
         }
-        px.classes[C.cls]++;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) n_cls[k] += C.cls == k ? 1u : 0u;
     }
+#pragma unroll
+    for (int k = 0; k < 4; ++k) px.classes[k] = n_cls[k];
     if (px.hits > 0 && S.want_scintillation) {
         // TemporalSampler::apply (image_scint.rs:95-118): the centre ray from
         // five yaw-offset cameras; pose 2 (no offset) must hit for any count.
@@ -283,8 +300,9 @@ __device__ __forceinline__ ixg_optical_pixel metric_pixel(const MetricShared& S,
             V3 dir;
             cam_ray(I.t_forward[i], I.t_right[i], I.t_up[i], I.fov_tan, sx, sy, pw, ph, dir);
             Classified C;
+            Fate unused;
             classify(S, I, sp, v3(I.t_origin[i][0], I.t_origin[i][1], I.t_origin[i][2]), dir, false, C,
-                     nullptr);
+                     unused);
             ret[i] = C.cls == CLS_NONE ? -1 : (C.cls == CLS_RETURNED ? 1 : 0);
         }
         if (ret[2] >= 0) {
@@ -320,8 +338,9 @@ __global__ __launch_bounds__(kMBlock) void metric_pixels_kernel(const MetricShar
 // LDS only when the image changes (consecutive items mostly share a design),
 // and runs the tile's pixels one per lane, TILE / BLOCK passes per item.  The
 // item count bounds the loop, so every block leaves once the queue is empty.
+
 template <int BLOCK, int TILE>
-__global__ __launch_bounds__(BLOCK) void metric_queue_kernel(const MetricShared S,
+__global__ __launch_bounds__(BLOCK) IXG_METRIC_WAVES_ATTR void metric_queue_kernel(const MetricShared S,
                                                              const MetricImage* __restrict__ images,
                                                              const float4* __restrict__ planes,
                                                              ixg_optical_pixel* __restrict__ out,
@@ -334,7 +353,8 @@ __global__ __launch_bounds__(BLOCK) void metric_queue_kernel(const MetricShared 
     while (true) {
         if (threadIdx.x == 0) s_item = atomicAdd(counter, 1u);
         __syncthreads();
-        const uint32_t item = s_item;
+        // uniform across the block: scalar loads of the image record, in SGPRs
+        const uint32_t item = (uint32_t)__builtin_amdgcn_readfirstlane((int)s_item);
         if (item >= n_items) break;
         const uint32_t img = item / tiles, tile = item % tiles;
         const MetricImage& I = images[img];
@@ -441,7 +461,7 @@ __device__ __forceinline__ V3 fan_origin(V3 f, V3 r, V3 up, float standoff, floa
     return (neg(f) * standoff + r * (u * lateral)) + up * (v * lateral);
 }
 
-__global__ __launch_bounds__(kPoseBlock) void pose_kernel(const MetricShared S, const PoseDesign* __restrict__ designs,
+__global__ __launch_bounds__(kPoseBlock) IXG_METRIC_WAVES_ATTR void pose_kernel(const MetricShared S, const PoseDesign* __restrict__ designs,
                                                           const PoseJob* __restrict__ jobs,
                                                           const float4* __restrict__ planes,
                                                           const float2* __restrict__ cells, uint32_t n_cells,
@@ -479,7 +499,8 @@ __global__ __launch_bounds__(kPoseBlock) void pose_kernel(const MetricShared S, 
             const V3 dir = normalize((f + r * sr) + up * su);
             const V3 org = fan_origin(f, r, up, D.standoff, D.lateral, uv.x, uv.y);
             Classified C;
-            classify_ray(S, sp, D.n_planes, D.nd, D.nf, D.nc, org, dir, neg(f), want_all != 0, C, nullptr);
+            Fate unused;
+            classify_ray(S, sp, D.n_planes, D.nd, D.nf, D.nc, org, dir, neg(f), want_all != 0, C, unused);
             int8_t code = (int8_t)C.cls;
             if (C.cls == CLS_RETURNED && C.has_fire) {
                 code = CLS_RETURNED_FIRE;
@@ -494,7 +515,8 @@ __global__ __launch_bounds__(kPoseBlock) void pose_kernel(const MetricShared S, 
             const V3 f = row(J.basis[1 + k], 0), r = row(J.basis[1 + k], 1), up = row(J.basis[1 + k], 2);
             const V3 org = fan_origin(f, r, up, D.standoff, D.lateral, uv.x, uv.y);
             Classified C;
-            classify_ray(S, sp, D.n_planes, D.nd, D.nf, D.nc, org, f, neg(f), false, C, nullptr);
+            Fate unused;
+            classify_ray(S, sp, D.n_planes, D.nd, D.nf, D.nc, org, f, neg(f), false, C, unused);
             s_ret[j] = (uint8_t)(C.cls == CLS_RETURNED);
         }
     }

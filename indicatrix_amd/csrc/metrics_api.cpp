@@ -211,6 +211,8 @@ struct SweepPlan {
     Shape last;
     int last_graphed = 0;
     std::vector<float> tune_ms;  // per candidate, last autotune
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;  // around the sweep's launches
+    double last_ms = 0.0;                     // their device time, last sweep
 };
 
 void drop_graph(SweepPlan* P) {
@@ -227,6 +229,8 @@ void plan_free(void* p) {
     void* bufs[] = {P->d_pl, P->d_img, P->d_px, P->d_tot, P->d_counter};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
+    if (P->ev0) (void)hipEventDestroy(P->ev0);
+    if (P->ev1) (void)hipEventDestroy(P->ev1);
     delete P;
 }
 
@@ -355,6 +359,9 @@ ixg_status run_sweep(ixg_ctx* ctx, const MetricShared& S, const std::vector<Metr
             }
         }
     }
+    if (e == hipSuccess && !P->ev0) e = hipEventCreate(&P->ev0);
+    if (e == hipSuccess && !P->ev1) e = hipEventCreate(&P->ev1);
+    if (e == hipSuccess) e = hipEventRecord(P->ev0, st);
     if (e == hipSuccess) {
         if (P->use_graph) {
             // the graph bakes in kernel arguments: shared constants, image
@@ -391,10 +398,14 @@ ixg_status run_sweep(ixg_ctx* ctx, const MetricShared& S, const std::vector<Metr
             e = enqueue_sweep(P, S, n, sh, st);
         }
     }
+    if (e == hipSuccess) e = hipEventRecord(P->ev1, st);
     totals.resize(n);
     if (e == hipSuccess)
         e = hipMemcpyAsync(totals.data(), P->d_tot, n * sizeof(ixg_metric_totals), hipMemcpyDeviceToHost, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);
+    float ms = 0.0f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, P->ev0, P->ev1);
+    if (e == hipSuccess) P->last_ms = ms;
     if (e != hipSuccess) {
         drop_graph(P);
         return ixg_internal_hip_fail(ctx, e, "metric sweep");
@@ -514,6 +525,11 @@ extern "C" ixg_status ixg_metric_sweep(ixg_ctx* ctx, const ixg_facet_plane* plan
         out_metrics[5 * i + 4] = tot[i].extinction_pct;
     }
     return IXG_OK;
+}
+
+extern "C" double ixg_metric_last_ms(ixg_ctx* ctx) {
+    if (!ctx) return 0.0;
+    return get_plan(ctx)->last_ms;
 }
 
 extern "C" ixg_status ixg_set_metric_mode(ixg_ctx* ctx, int use_graph, int block, int tile) {

@@ -184,3 +184,9 @@ def metric_tuning(backend):
     b, t, g, gr = C.c_int(), C.c_int(), C.c_uint32(), C.c_int()
     lib().ixg_metric_tuning(backend.handle, C.byref(b), C.byref(t), C.byref(g), C.byref(gr))
     return b.value, t.value, g.value, bool(gr.value)
+
+
+def metric_last_ms(backend):
+    """Device time (ms) of the last metric_sweep's launches (HIP events on the
+    library's stream; autotuning excluded)."""
+    return float(lib().ixg_metric_last_ms(backend.handle))

@@ -257,6 +257,10 @@ class SurrogateMLP:
                                             self.n_params, C.byref(loss)), "gradients")
         return float(loss.value), g
 
+    def sync(self):
+        """Wait for the network's queued work (ixg_mlp_sync)."""
+        self._check(lib().ixg_mlp_sync(self._h), "sync")
+
     def train_step(self, x, t, lr, want_loss=True):
         loss = C.c_float()
         self._check(lib().ixg_mlp_train_step(self._h, C.c_void_p(x.data_ptr()), C.c_void_p(t.data_ptr()),

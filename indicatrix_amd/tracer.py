@@ -351,6 +351,10 @@ class GpuBackend:
     def last_kernel_ms(self):
         return lib().ixg_last_kernel_ms(self._ctx)
 
+    def stream(self):
+        """The hipStream_t (as an int) the context launches its work on."""
+        return int(lib().ixg_stream(self._ctx) or 0)
+
     def trace_paths(self, scene, sample_offset, spp, want_xyz=True, want_events=True):
         n = scene.width * scene.height * spp
         xyz = np.zeros((n, 3), np.float32) if want_xyz else None

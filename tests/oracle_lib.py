@@ -315,12 +315,10 @@ def sweep_pose(tilt_deg, distance):
     return yaw, pitch
 
 
-def metric_sweep(planes, designs, tilts, width, height, samples, distance, environment,
-                 nthreads=None):
-    """The oracle's per-image evaluation of every (design, tilt) pair: (n, t, 5)
-    {brilliance, fire, scintillation, windowing, extinction}, images spread over
-    host threads."""
-    from concurrent.futures import ThreadPoolExecutor
+def sweep_requests(planes, designs, tilts, width, height, samples, distance, environment):
+    """The metric requests of a sweep, one per (design, tilt), as
+    (design index, tilt index, request); each request keeps its planes and
+    material alive."""
     from indicatrix_amd import metrics, tracer
     planes = np.ascontiguousarray(planes, np.float32).reshape(-1, 4)
     jobs = []
@@ -335,11 +333,35 @@ def metric_sweep(planes, designs, tilts, width, height, samples, distance, envir
             cam = tracer.camera_new(yaw, pitch, distance, 42.0, width, height, samples)
             req = metrics.make_request(cam, width, height, samples, pl, mat, environment, True, True,
                                        (yaw, pitch, distance))
-            jobs.append((i, j, req, pl, mat))
+            req._keep = (pl, mat)
+            jobs.append((i, j, req))
+    return jobs
+
+
+def metric_image_flops(request):
+    """FP operations of the oracle's evaluate_optical_image on one request
+    (the counting build, one thread)."""
+    n = request.width * request.height
+    px = (abi.OpticalPixel * n)()
+    tot = abi.MetricTotals()
+    count = C.c_uint64(0)
+    st = lib_flops().ixo_flops_metric_image(C.byref(request), px, C.byref(tot), C.byref(count))
+    if st != 0:
+        raise RuntimeError(f"ixo_flops_metric_image returned {st}")
+    return count.value
+
+
+def metric_sweep(planes, designs, tilts, width, height, samples, distance, environment,
+                 nthreads=None):
+    """The oracle's per-image evaluation of every (design, tilt) pair: (n, t, 5)
+    {brilliance, fire, scintillation, windowing, extinction}, images spread over
+    host threads."""
+    from concurrent.futures import ThreadPoolExecutor
+    jobs = sweep_requests(planes, designs, tilts, width, height, samples, distance, environment)
     out = np.zeros((len(designs), len(tilts), 5), np.float32)
 
     def run(job):
-        i, j, req, _, _ = job
+        i, j, req = job
         _, tot, scint = metric_image(req)
         out[i, j] = (tot.brilliance_pct, tot.fire_index, scint, tot.windowing_pct, tot.extinction_pct)
 
