@@ -268,16 +268,25 @@ class SurrogateMLP:
                     "train_step")
         return float(loss.value) if want_loss else None
 
-    def fit(self, x, t, epochs=10, batch=4096, lr=1e-3, lr_final=None, seed=0, log=None):
-        """Adam on (x, t) in tracer units; standardises t and keeps the statistics.
-        The learning rate follows a cosine from lr to lr_final (constant when None).
-        Returns the mean training loss of each epoch (standardised units)."""
+    def fit(self, x, t, epochs=10, batch=4096, lr=1e-3, lr_final=None, seed=0, log=None,
+            weights=None, restandardise=True):
+        """Adam on (x, t) in tracer units.  With restandardise (a fresh fit) the
+        targets are standardised by their mean / std and the statistics kept;
+        without it (a fine-tune of a loaded network) the stored statistics are
+        used as they are.  weights (3,) weights each output's squared error: the
+        standardising std of output j is divided by sqrt(weights[j]), which is
+        the same loss up to a constant.  The learning rate follows a cosine from
+        lr to lr_final (constant when None).  Returns the mean training loss of
+        each epoch (standardised units)."""
         torch = self._torch
         if batch % 64 or batch > self.max_batch:
             raise ValueError("batch must be a multiple of 64 and at most max_batch")
         t = np.asarray(t, np.float32)
-        self.t_mean = t.mean(axis=0).astype(np.float32)
-        self.t_std = (t.std(axis=0) + 1e-6).astype(np.float32)
+        if restandardise:
+            self.t_mean = t.mean(axis=0).astype(np.float32)
+            self.t_std = (t.std(axis=0) + 1e-6).astype(np.float32)
+            if weights is not None:
+                self.t_std = (self.t_std / np.sqrt(np.asarray(weights, np.float32))).astype(np.float32)
         xd = self._dev(x, N_IN)
         td = self._dev((t - self.t_mean) / self.t_std, N_OUT)
         n = (xd.shape[0] // batch) * batch
@@ -327,6 +336,31 @@ class SurrogateMLP:
             torch.cuda.synchronize()
             loss = self.train_step(xs, ts, lr, want_loss=(k == steps - 1))
         return loss
+
+    # output layer in the padded parameter layout (mlp_api.cpp): W4 [64 x 256]
+    # after W0 [256 x 64] and W1..W3 [256 x 256]; b4 [64] after b0..b3 [256]
+    _W4_OFF = 256 * 64 + 3 * 256 * 256
+    _B4_OFF = _W4_OFF + 64 * 256 + 4 * 256
+
+    def restandardise(self, t_mean, t_std):
+        """Move the network to new output statistics without changing what it
+        predicts: output j's weights and bias are scaled by old_std / new_std
+        and the bias shifted by (old_mean - new_mean) / new_std.  The Adam
+        moments restart (they belonged to the old scale)."""
+        t_mean = np.asarray(t_mean, np.float32)
+        t_std = np.asarray(t_std, np.float32)
+        p = self.get_params().astype(np.float64)
+        for j in range(N_OUT):
+            a = float(self.t_std[j]) / float(t_std[j])
+            c = (float(self.t_mean[j]) - float(t_mean[j])) / float(t_std[j])
+            w = slice(self._W4_OFF + 256 * j, self._W4_OFF + 256 * (j + 1))
+            p[w] *= a
+            p[self._B4_OFF + j] = p[self._B4_OFF + j] * a + c
+        self.set_params(p.astype(np.float32))
+        self.t_mean = t_mean.copy()
+        self.t_std = t_std.copy()
+        z = np.zeros(self.n_params, np.float32)
+        self.set_state(z, z, 0)
 
     STATE_KEYS = ("params", "t_mean", "t_std", "adam_m", "adam_v", "adam_step", "rng_state")
 

@@ -1,17 +1,29 @@
 """Train the metric surrogate on the tracer's own GPU metric sweep.
 
-    python -m indicatrix_amd.train_surrogate --designs 95239 --epochs 60 \\
-        --out indicatrix_amd/data/surrogate.npz
+Two stages, both on the GPU with libixg's MFMA kernels:
+
+  1. pretrain on the cheap sweep shape (64 x 64, 32 samples):
+       python -m indicatrix_amd.train_surrogate --designs 95239 --epochs 60 \\
+           --out pre.npz
+  2. fine-tune at the verification shape of the screened Optimize
+     (128 x 128, 64 samples), with the scintillation error weighted up:
+       python -m indicatrix_amd.train_surrogate --init pre.npz --shape 128 128 64 \\
+           --designs 16384 --epochs 40 --lr 3e-4 --weights 1 1 4 \\
+           --out indicatrix_amd/data/surrogate.npz --report indicatrix_amd/data/surrogate_report.json
 
 Data: seeded candidates from the parametric family (surrogate.sample_candidates)
-traced by ixg_metric_sweep at 64 x 64, 32 samples, 21 tilts (-30..30 deg), one
-sample per (design, tilt): 95 239 designs give 2.0 M samples.  A held-out set
-of designs (other seed) measures the tilt-averaged relative error against the
-same tracer, and a subset of it against the 128 x 128, 64-sample verification
-render the screened optimiser uses.
+traced by ixg_metric_sweep at the given shape and 21 tilts (-30..30 deg), one
+sample per (design, tilt).  Scintillation at 64 x 64 differs from its value at
+128 x 128 by more than the screened loop's error budget (the spatial term
+depends on the pixel grid), which is why the second stage trains on the
+verification shape itself.  A held-out set of designs (another seed) measures
+the relative error against the tracer at the training shape, and a subset of
+it against the 128 x 128, 64-sample verification render.  --data caches the
+traced designs in an .npz (read when present, written otherwise).
 """
 import argparse
 import json
+import os
 import time
 
 import numpy as np
@@ -20,7 +32,7 @@ from . import surrogate as S
 from .tracer import GpuBackend
 
 
-def build_dataset(backend, n_designs, seed, width=64, height=64, samples=32, chunk=2048, log=print):
+def build_dataset(backend, n_designs, seed, width=64, height=64, samples=32, chunk=512, log=print):
     rng = np.random.default_rng(seed)
     u, nd, abbe = S.sample_candidates(rng, n_designs)
     m = np.empty((n_designs, len(S.TILTS_C3), S.N_OUT), np.float32)
@@ -29,8 +41,24 @@ def build_dataset(backend, n_designs, seed, width=64, height=64, samples=32, chu
         e = min(n_designs, s + chunk)
         m[s:e] = S.trace_metrics(backend, u[s:e], nd[s:e], abbe[s:e], S.TILTS_C3, width, height, samples)
         dt = time.perf_counter() - t0
-        log(f"traced {e}/{n_designs} designs, {dt:.1f} s, {e / dt:.1f} design evals/s")
+        log(f"traced {e}/{n_designs} designs at {width}x{height}@{samples}, {dt:.1f} s, "
+            f"{e / dt:.1f} design evals/s")
     return u, nd, abbe, m, time.perf_counter() - t0
+
+
+def cached_dataset(backend, path, n_designs, seed, shape, log):
+    """build_dataset, read from / written to an .npz cache when path is given."""
+    if path and os.path.exists(path):
+        with np.load(path, allow_pickle=False) as z:
+            if (int(z["n_designs"]), int(z["seed"]), list(z["shape"])) != (n_designs, seed, list(shape)):
+                raise ValueError(f"{path} holds another dataset")
+            log(f"dataset read from {path}")
+            return z["u"], z["nd"], z["abbe"], z["m"], float(z["seconds"])
+    u, nd, abbe, m, dt = build_dataset(backend, n_designs, seed, *shape, log=log)
+    if path:
+        np.savez(path, u=u, nd=nd, abbe=abbe, m=m, seconds=dt, n_designs=n_designs, seed=seed,
+                 shape=np.asarray(shape))
+    return u, nd, abbe, m, dt
 
 
 def main(argv=None):
@@ -39,10 +67,17 @@ def main(argv=None):
     ap.add_argument("--val-designs", type=int, default=2048)
     ap.add_argument("--verify-designs", type=int, default=256)
     ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--shape", type=int, nargs=3, default=[64, 64, 32], metavar=("W", "H", "SAMPLES"),
+                    help="metric render of the training data")
+    ap.add_argument("--init", default=None,
+                    help="checkpoint to fine-tune (its output statistics are re-fitted to the new data)")
+    ap.add_argument("--weights", type=float, nargs=3, default=None,
+                    help="squared-error weight of brilliance, fire and scintillation")
     ap.add_argument("--epochs", type=int, default=60)
     ap.add_argument("--batch", type=int, default=4096)
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--lr-final", type=float, default=1e-5)
+    ap.add_argument("--data", default=None, help="cache of the traced training designs (.npz)")
     ap.add_argument("--out", required=True)
     ap.add_argument("--report", default=None)
     a = ap.parse_args(argv)
@@ -50,17 +85,26 @@ def main(argv=None):
     def log(msg):
         print(msg, flush=True)
 
+    shape = tuple(a.shape)
     b = GpuBackend()
-    u, nd, abbe, m, t_data = build_dataset(b, a.designs, a.seed, log=log)
+    u, nd, abbe, m, t_data = cached_dataset(b, a.data, a.designs, a.seed, shape, log)
     x = S.design_features(u, nd, abbe)
     t = m.reshape(-1, S.N_OUT)
     log(f"dataset {x.shape[0]} samples in {t_data:.1f} s")
-    uv, ndv, abv, mv, _ = build_dataset(b, a.val_designs, a.seed + 1_000_003, log=log)
+    val_seed = a.seed + 1_000_003
+    uv, ndv, abv, mv, _ = build_dataset(b, a.val_designs, val_seed, *shape, log=log)
 
     net = S.SurrogateMLP(b, max_batch=max(a.batch, 64 * ((a.val_designs * 21 + 63) // 64)), seed=a.seed)
+    prior = None
+    if a.init:
+        prior = net.load(a.init).get("report")
+        std = t.std(axis=0) + 1e-6
+        if a.weights is not None:
+            std = std / np.sqrt(np.asarray(a.weights))
+        net.restandardise(t.mean(axis=0), std)
     t0 = time.perf_counter()
     hist = net.fit(x, t, epochs=a.epochs, batch=a.batch, lr=a.lr, lr_final=a.lr_final, seed=a.seed,
-                   log=log)
+                   log=log, weights=a.weights, restandardise=a.init is None)
     t_fit = time.perf_counter() - t0
     steps = a.epochs * (len(x) // a.batch)
     log(f"fit {a.epochs} epochs ({steps} Adam steps) in {t_fit:.1f} s, {1e3 * t_fit / steps:.3f} ms/step")
@@ -68,21 +112,28 @@ def main(argv=None):
     pv = S.predict_designs(net, uv, ndv, abv)
     err_val = S.relative_error(pv, mv)
     nv = a.verify_designs
-    m128 = S.trace_metrics(b, uv[:nv], ndv[:nv], abv[:nv], S.TILTS_C3, 128, 128, 64)
+    if shape == (128, 128, 64):
+        m128 = mv[:nv]
+    else:
+        m128 = S.trace_metrics(b, uv[:nv], ndv[:nv], abv[:nv], S.TILTS_C3, 128, 128, 64)
     err_128 = S.relative_error(pv[:nv], m128)
     err_res = S.relative_error(mv[:nv], m128)
+    err_tilt = np.abs(pv[:nv] - m128) / np.maximum(np.abs(m128), 1e-6)
     rep = dict(
-        designs=a.designs, samples=int(x.shape[0]), epochs=a.epochs, batch=a.batch, lr=a.lr,
-        lr_final=a.lr_final, seed=a.seed, data_seconds=t_data, fit_seconds=t_fit,
-        ms_per_step=1e3 * t_fit / steps, final_loss=hist[-1],
+        designs=a.designs, samples=int(x.shape[0]), shape=list(shape), epochs=a.epochs, batch=a.batch,
+        lr=a.lr, lr_final=a.lr_final, seed=a.seed, weights=a.weights, init=os.path.basename(a.init) if a.init else None,
+        data_seconds=t_data, fit_seconds=t_fit, ms_per_step=1e3 * t_fit / steps, final_loss=hist[-1],
         metrics=list(S.METRIC_COLUMNS),
         val_rel_err_median=np.median(err_val, axis=0).tolist(),
         val_rel_err_mean=err_val.mean(axis=0).tolist(),
         val_rel_err_p90=np.percentile(err_val, 90, axis=0).tolist(),
         verify128_rel_err_median=np.median(err_128, axis=0).tolist(),
         verify128_rel_err_mean=err_128.mean(axis=0).tolist(),
-        tracer_64_vs_128_rel_err_median=np.median(err_res, axis=0).tolist(),
+        verify128_per_tilt_rel_err_median=np.median(err_tilt.reshape(-1, S.N_OUT), axis=0).tolist(),
+        tracer_shape_vs_128_rel_err_median=np.median(err_res, axis=0).tolist(),
     )
+    if prior is not None:
+        rep["pretrain"] = json.loads(str(prior))
     log(json.dumps(rep))
     net.save(a.out, report=json.dumps(rep))
     if a.report:
