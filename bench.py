@@ -46,7 +46,7 @@ def parse():
                     help="approximate CPU work of the bounded CPU-baseline sample")
     ap.add_argument("--film-only", action="store_true",
                     help="measure the film only (profiling runs)")
-    ap.add_argument("--optimize-steps", type=int, default=8)
+    ap.add_argument("--optimize-steps", type=int, default=50)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a 1 GiB device copy (scales rocprof FETCH/WRITE_SIZE)")
     ap.add_argument("--mlp-steps", type=int, default=200)
@@ -162,7 +162,7 @@ def sweep_rate(backend, n_designs=512):
     from indicatrix_amd import designs, flops, metrics, tracer
     planes, ds, _ = designs.sweep_batch(n_designs)
     env = tracer.LightingPreset.studio("IsoHemisphere", 1.0, 0.0, 0.0)
-    metrics.metric_sweep(backend, planes, ds[:16], designs.TILTS_C3, 64, 64, 32, 2.4, env)  # warm
+    metrics.metric_sweep(backend, planes, ds[:16], designs.TILTS_C3, 64, 64, 32, 2.4, env)  # warm + autotune
     t0 = time.perf_counter()
     out = metrics.metric_sweep(backend, planes, ds, designs.TILTS_C3, 64, 64, 32, 2.4, env)
     dt = time.perf_counter() - t0
@@ -179,7 +179,9 @@ def sweep_rate(backend, n_designs=512):
         result["roofline"] = {
             "bound": "valu", "pipe": "fp32 vector ALU", "achieved": round(achieved, 3),
             "peak": MI355X_FP32_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": round(achieved / MI355X_FP32_PEAK_TFLOPS, 4),
-            "traffic": None, "kernel": f"metric_queue_kernel<{block},{tile}>" if block > 0 else "metric_pixels_kernel",
+            "traffic": None, "kernel": (f"metric_queue_kernel<{block},{tile}>" if block > 0 else
+                                        f"metric_flow_kernel<{-block}> ({tile} px per item)" if block < -1 else
+                                        "metric_pixels_kernel"),
             "kernel_ms": round(dev_ms, 3), "flops_per_launch": fl,
             "note": "one launch covers the whole sweep; kernel_ms includes the per-image reductions"}
     return result, (planes, ds, env)
@@ -318,6 +320,9 @@ def screened_rate(backend, steps):
     tracer_only = verified / verify   # the same tracer work per verified design
     later = recs[2:] if len(recs) > 2 else recs
     err = np.concatenate([r["rel_err"] for r in later])
+    # per (design, tilt), before the tilt average
+    err_tilt = np.concatenate([(np.abs(r["pred"] - r["traced"]) / np.maximum(np.abs(r["traced"]), 1e-6))
+                               .reshape(-1, 3) for r in later])
 
     def r5(v):
         return [round(float(x), 5) for x in v]
@@ -333,6 +338,9 @@ def screened_rate(backend, steps):
             "rel_err_median_by_step": [r5(np.median(r["rel_err"], axis=0)) for r in recs],
             "rel_err_median_from_step2": r5(np.median(err, axis=0)),
             "rel_err_mean_from_step2": r5(err.mean(axis=0)),
+            "rel_err_p90_from_step2": r5(np.percentile(err, 90, axis=0)),
+            "per_tilt_rel_err_median_from_step2": r5(np.median(err_tilt, axis=0)),
+            "per_tilt_rel_err_mean_from_step2": r5(err_tilt.mean(axis=0)),
             "objective_rel_err_median_from_step2": round(float(np.median(
                 np.concatenate([r["obj_rel_err"] for r in later]))), 5),
             "best_verified_objective": round(recs[-1]["best"], 4)}

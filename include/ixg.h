@@ -384,11 +384,14 @@ ixg_status ixg_metric_sweep(ixg_ctx* ctx, const ixg_facet_plane* planes, uint32_
  * the autotuner time the 2-D grid and each persistent work-queue shape once
  * per image size and keep the fastest, block = -1 forces the 2-D grid, and
  * block > 0 with tile forces one queue shape (lanes per block, pixels per
- * queue item: 64/64, 64/256, 128/128, 128/512, 256/256, 256/1024).
- * Defaults: use_graph 1, block 0. */
+ * queue item: 64/64, 64/256, 128/128, 128/512, 256/256, 256/1024), and
+ * block = -64, -128 or -256 forces the flow kernel (every lane advancing one
+ * slab scan of its pixel's traces per step) with that many lanes per block
+ * and tile pixels per item (0 = the whole image); the autotuner also times
+ * the flow shapes.  Defaults: use_graph 1, block 0. */
 ixg_status ixg_set_metric_mode(ixg_ctx* ctx, int use_graph, int block, int tile);
-/* The shape the last sweep ran with (block -1 = 2-D grid) and whether it was
- * replayed from a graph. */
+/* The shape the last sweep ran with (block -1 = 2-D grid, -64/-128/-256 =
+ * the flow kernel) and whether it was replayed from a graph. */
 ixg_status ixg_metric_tuning(ixg_ctx* ctx, int* block, int* tile, uint32_t* grid_blocks, int* graphed);
 /* Device time (ms) of the last ixg_metric_sweep's launches -- the pixel
  * kernel (or its graph) and the per-image reductions, autotuning excluded --

@@ -118,14 +118,33 @@ __device__ __forceinline__ uint32_t hash_u32(uint32_t x) {
 __device__ __forceinline__ float unit_draw(uint32_t seed, uint32_t bounce, uint32_t stream) {
     return div_rn((float)hash_u32(seed ^ hash_u32(bounce ^ stream)), 4294967295.0f);
 }
-__device__ __forceinline__ float radical_inverse(uint32_t n, uint32_t base) {
+// radical_inverse (sampling.rs): digit k of n in base B weighted by the
+// digit scale 1/B divided by B k more times, each step correctly rounded --
+// the scales are the same f32 sequence for every n, so they are a table
+// (folded by the compiler, whose f32 division is the correctly rounded one):
+// one fma per digit instead of a division.
+template <uint32_t B, int N>
+struct DigitScales {
+    float v[N];
+    constexpr DigitScales() : v() {
+        float x = 1.0f / (float)B;
+        for (int i = 0; i < N; ++i) {
+            v[i] = x;
+            x = x / (float)B;
+        }
+    }
+};
+__constant__ const DigitScales<3, 21> kDigitScale3{};  // 3^21 > 2^32: every u32 index
+__constant__ const DigitScales<5, 14> kDigitScale5{};  // 5^14 > 2^32
+template <uint32_t B>
+__device__ __forceinline__ float radical_inverse(uint32_t n) {
+    static_assert(B == 3 || B == 5, "digit-scale tables exist for bases 3 and 5");
+    const float* scale = B == 3 ? kDigitScale3.v : kDigitScale5.v;
     float val = 0.0f;
-    float inv_base = div_rn(1.0f, (float)base);
-    while (n > 0) {
-        uint32_t digit = n % base;
-        val = fma_((float)digit, inv_base, val);
-        inv_base /= (float)base;
-        n /= base;
+    for (int k = 0; n > 0; ++k) {
+        const uint32_t digit = n % B;
+        val = fma_((float)digit, scale[k], val);
+        n /= B;
     }
     return val;
 }

@@ -154,8 +154,8 @@ __device__ __forceinline__ void camera_sample(const FilmParams& P, uint32_t pixe
     float ry = ld_base2(hash_u32(pixel ^ 0x63D81B23u));
     float rh = ld_base2(hash_u32(pixel ^ 0x1B873593u));
     float jx = cp_rotate(ld_base2(sample), rx) - 0.5f;
-    float jy = cp_rotate(radical_inverse(sample, 3), ry) - 0.5f;
-    hero = cp_rotate(radical_inverse(sample, 5), rh);
+    float jy = cp_rotate(radical_inverse<3>(sample), ry) - 0.5f;
+    hero = cp_rotate(radical_inverse<5>(sample), rh);
     uint32_t px = pixel % P.width, py = pixel / P.width;
     float aspect = div_rn(P.width_f, P.height_f);
     float u = (div_rn((float)px + jx, P.width_f) - 0.5f) * 2.0f * aspect * P.fov_tan;
@@ -212,7 +212,10 @@ __device__ __forceinline__ void path_init(const FilmParams& P, V3 org, V3 dir, u
     for (int c = 0; c < NCH; ++c) {
         // wrapped_hero_wavelengths (transport/mod.rs:58-66)
         float off = fma_((float)c, 50.0f, lambda_hero - 380.0f);
-        float r = fmodf(off, 400.0f);
+        // rem_euclid(400): for 0 <= off < 800 this is off or off - 400, both
+        // exact (Sterbenz), which is what fmodf returns; a hero draw outside
+        // [0, 1) (explicit rays) takes the library remainder
+        float r = off >= 0.0f && off < 800.0f ? (off >= 400.0f ? off - 400.0f : off) : fmodf(off, 400.0f);
         r = r < 0.0f ? r + 400.0f : r;
         S.lambda[c] = 380.0f + r;
         S.n_o[c] = dispersion(P.disp_kind, P.disp, S.lambda[c]);

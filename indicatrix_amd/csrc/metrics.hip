@@ -373,6 +373,412 @@ __global__ __launch_bounds__(BLOCK) IXG_METRIC_WAVES_ATTR void metric_queue_kern
     }
 }
 
+// ---------------------------------------------------------------------------
+// Flow kernel: the same pixels with the lanes kept together in the slab scan.
+//
+// metric_pixel nests three loops per pixel (samples; <= 10 bounces per trace;
+// the F / C replays of a returned ray and the five temporal rays), and across
+// a wave those trip counts differ, so most lanes sit masked while the longest
+// trace runs.  Here a lane runs its pixel as a chain of traces -- per sample
+// the camera ray's entry scan, the d-line trace and, for a returned ray, the
+// F and C replays; then the five temporal camera rays -- and each iteration
+// of the loop advances every lane by exactly one scan of its current trace.
+// A lane whose pixel is complete writes it and takes the next pixel of the
+// block's item (one LDS atomic per wave).  Every lane computes its pixel with
+// the operations of metric_pixel in the same order, so the records are the
+// same bits.
+// ---------------------------------------------------------------------------
+enum FlowTask : uint32_t {
+    FT_CAM = 0,     // the camera ray of sample s: its entry scan
+    FT_DLINE = 1,   // the sample's d-line trace
+    FT_FLINE = 2,   // the F-line replay of a returned ray
+    FT_CLINE = 3,   // the C-line replay
+    FT_TCAM = 4,    // temporal camera ti: its entry scan
+    FT_TDLINE = 5,  // its d-line trace
+};
+
+// One trace_entry in progress: what its bounce loop carries.
+struct TraceRun {
+    V3 hp, cur;
+    float index, sin_crit, idx2, entry_T;
+    uint32_t b;
+};
+
+// trace_entry up to its bounce loop; false when the trace ends without a
+// scan (entry blocked).
+__device__ __forceinline__ bool trace_begin(V3 point, V3 dir, V3 nrm, float cos_i, float index, Fate& F,
+                                            TraceRun& R) {
+    F.fate = FATE_ABSORBED;
+    F.exit_facet = -1;
+    F.d_bounces = 0;
+    F.reason = REASON_BUDGET;
+    F.facet = 0;
+    F.bounces = 0;
+    F.transmittance = 0.0f;
+    F.exit_cos = 0.0f;
+    F.dir = v3(0.0f, 0.0f, 0.0f);
+    const float sin2_t = (1.0f / (index * index)) * fma_(cos_i, -cos_i, 1.0f);
+    if (sin2_t > 1.0f) {
+        F.fate = FATE_ENTRY_BLOCKED;
+        F.reason = REASON_ENTRY;
+        return false;
+    }
+    const float cos_t = sqrt_rn(1.0f - sin2_t);
+    const float inv = 1.0f / index;
+    R.cur = normalize(dir * inv + nrm * fma_(inv, cos_i, -cos_t));
+    R.entry_T = fresnel_T(1.0f, index, cos_i, cos_t);
+    R.hp = point + R.cur * 1e-4f;
+    R.sin_crit = 1.0f / index;
+    R.idx2 = index * index;
+    R.index = index;
+    R.b = 0;
+    return true;
+}
+
+// One iteration of trace_entry's bounce loop after its scan gave (f, t);
+// false when the trace has ended.
+__device__ __forceinline__ bool trace_step(const float4* __restrict__ pl, int f, float t, Fate& F, TraceRun& R) {
+    if (f < 0) {
+        F.reason = REASON_NOHIT;
+        return false;
+    }
+    const V3 np = R.hp + R.cur * t;
+    const float4 pf = pl[f];
+    const V3 n_out = v3(pf.x, pf.y, pf.z);
+    const float ct = clampf(dot(R.cur, n_out), 0.0f, 1.0f);
+    const float one_m = fma_(ct, -ct, 1.0f);
+    const float st = sqrt_rn(fmaxf(one_m, 0.0f));
+    if (st < R.sin_crit) {
+        const float s2o = R.idx2 * one_m;
+        if (s2o <= 1.0f) {
+            const float index = R.index;
+            const float co = sqrt_rn(1.0f - s2o);
+            const V3 od = normalize(R.cur * index + n_out * fma_(index, -ct, co));
+            F.exit_facet = f;
+            F.reason = REASON_SIDE;
+            if (n_out.y < -0.05f) {
+                F.reason = REASON_NONE;
+                F.fate = FATE_LEAKED;
+            } else if (od.y > 0.05f) {
+                F.reason = REASON_NONE;
+                F.fate = FATE_UPWARD;
+                F.dir = od;
+                F.transmittance = R.entry_T * fresnel_T(index, 1.0f, ct, co);
+                F.exit_cos = co;
+                F.facet = f;
+                F.bounces = R.b;
+            }
+            return false;
+        }
+    }
+    R.cur = normalize(R.cur - n_out * (2.0f * ct));
+    R.hp = np + R.cur * 1e-4f;
+    F.d_bounces = R.b + 1;
+    ++R.b;
+    return R.b < 10u;
+}
+
+// classify_ray's verdict on a finished d-line trace (classify.rs:140-183).
+__device__ __forceinline__ int classify_fate(const MetricShared& S, Fate& F, V3 observer) {
+    if (F.fate == FATE_ENTRY_BLOCKED) return CLS_OTHER;
+    if (F.fate == FATE_LEAKED) return CLS_WINDOWED;
+    if (F.fate == FATE_ABSORBED) return CLS_EXTINCT;
+    if (dot(F.dir, observer) > S.head_cos) {
+        F.reason = REASON_SHADOW;
+        return CLS_EXTINCT;
+    }
+    if (!(scoring_radiance(S, F.dir) > S.threshold)) {
+        F.reason = REASON_ILLUM;
+        return CLS_EXTINCT;
+    }
+    return CLS_RETURNED;
+}
+
+// What one lane carries between scans.
+struct FlowLane {
+    uint32_t p, s, task, ti;  // pixel (within the image), sample, FlowTask, temporal camera
+    uint32_t hits, n0, n1, n2, n3, fire_q, fire_r, mixed;
+    float fire_sum;
+    int32_t c_entry, c_exit, first;
+    uint32_t c_bounces, c_reason;
+    uint32_t t_hit, t_ret;    // temporal cameras that met the stone / came back returned (bit i)
+    V3 hp0, dir0, nrm0;       // the traced ray's entry point, direction and entry-facet normal
+    float cos_i0;
+    int32_t entry_facet;
+    int32_t d_exit;           // the sample's d-line fate, as the sample's tail reads it
+    uint32_t d_bounces, d_reason;
+    float d_T;
+    int32_t a_fate, a_facet;  // the F-line fate, while the C line runs
+    uint32_t a_bounces;
+    V3 a_dir;
+    float a_T, a_exit;
+    TraceRun R;               // the running trace; R.hp / R.cur also hold a camera ray
+    Fate F;                   // its fate
+};
+
+// The next camera ray of the lane's pixel: sample L.s of the pixel's
+// sub-pixel pattern, or (temporal) the pixel centre from temporal camera L.ti
+// (Camera::generate_ray with zero jitter, camera.rs:149-167).
+__device__ __forceinline__ void flow_camera(const MetricShared& S, const MetricImage& I, FlowLane& L, bool temporal) {
+    const uint32_t x = L.p % S.width, y = L.p / S.width;
+    float jx = 0.5f, jy = 0.5f;
+    if (!temporal) sample_offset(L.s, jx, jy);
+    const float sx = fma_((float)x + jx, S.scale_x, (float)S.origin_x - 0.5f);
+    const float sy = fma_((float)y + jy, S.scale_y, (float)S.origin_y - 0.5f);
+    const uint32_t i = L.ti;
+    const float* fwd = temporal ? I.t_forward[i] : I.forward;
+    const float* right = temporal ? I.t_right[i] : I.right;
+    const float* up = temporal ? I.t_up[i] : I.up;
+    const float* org = temporal ? I.t_origin[i] : I.origin;
+    cam_ray(fwd, right, up, I.fov_tan, sx, sy, (float)S.proj_w, (float)S.proj_h, L.R.cur);
+    L.R.hp = v3(org[0], org[1], org[2]);
+    L.task = temporal ? FT_TCAM : FT_CAM;
+}
+
+__device__ __forceinline__ void flow_pixel_start(const MetricShared& S, const MetricImage& I, FlowLane& L,
+                                                 uint32_t p) {
+    L.p = p;
+    L.s = 0;
+    L.ti = 0;
+    L.hits = L.n0 = L.n1 = L.n2 = L.n3 = L.fire_q = L.fire_r = L.mixed = 0u;
+    L.fire_sum = 0.0f;
+    L.c_entry = L.c_exit = L.first = -1;
+    L.c_bounces = L.c_reason = 0u;
+    L.t_hit = L.t_ret = 0u;
+    flow_camera(S, I, L, false);
+}
+
+// The pixel's record, as metric_pixel returns it.
+__device__ __forceinline__ ixg_optical_pixel flow_record(const FlowLane& L) {
+    ixg_optical_pixel px;
+    px.hits = L.hits;
+    px.classes[0] = L.n0;
+    px.classes[1] = L.n1;
+    px.classes[2] = L.n2;
+    px.classes[3] = L.n3;
+    px.fire_sum_deg = L.fire_sum;
+    px.fire_qualified = L.fire_q;
+    px.fire_rejected = L.fire_r;
+    px.centre_entry_facet = L.c_entry;
+    px.centre_exit_facet = L.c_exit;
+    px.centre_bounces = L.c_bounces;
+    px.centre_reason = L.c_reason;
+    px.mixed_facets = L.mixed;
+    px.scint_returned = 0;
+    px.scint_poses = 0;
+    if (L.t_hit & 4u) {  // pose 2 (no offset) must meet the stone for any count
+        px.scint_returned = (uint8_t)__popc(L.t_ret);
+        px.scint_poses = 5;
+    }
+    px.pad[0] = px.pad[1] = 0;
+    return px;
+}
+
+// The camera ray (L.R.hp, L.R.cur) met facet f at t: the entry of
+// classify_ray (classify.rs:118-131).
+__device__ __forceinline__ void flow_entry(const float4* __restrict__ sp, FlowLane& L, int f, float t) {
+    L.entry_facet = f;
+    L.dir0 = L.R.cur;
+    L.hp0 = L.R.hp + L.R.cur * t;
+    const float4 pf = sp[f];
+    L.nrm0 = v3(pf.x, pf.y, pf.z);
+    L.cos_i0 = clampf(dot(neg(L.dir0), L.nrm0), 0.0f, 1.0f);
+}
+
+// The F-line fate (in L.F) kept while the C line runs.
+__device__ __forceinline__ void flow_keep_fline(FlowLane& L) {
+    L.a_fate = L.F.fate;
+    L.a_facet = L.F.facet;
+    L.a_bounces = L.F.bounces;
+    L.a_dir = L.F.dir;
+    L.a_T = L.F.transmittance;
+    L.a_exit = L.F.exit_cos;
+}
+
+// fire_sample's tail (classify.rs:226-256) with the C-line fate in L.F.
+__device__ __forceinline__ bool flow_fire(const FlowLane& L, float& angle, float& weight) {
+    const Fate& B = L.F;
+    if (L.a_fate == FATE_UPWARD && B.fate == FATE_UPWARD && L.a_facet == B.facet && L.a_bounces == B.bounces) {
+        const float cs = clampf(dot(L.a_dir, B.dir), -1.0f, 1.0f);
+        angle = acosf(cs) * 57.2957795130823208767981548141051703f;
+        const float energy = L.d_T * L.a_T * B.transmittance;
+        const float radiance = fmaxf(L.a_exit, 0.0f) * fmaxf(B.exit_cos, 0.0f);
+        weight = energy * radiance;
+        return true;
+    }
+    return false;
+}
+
+// Advance the lane by the scan result (f, t) of its current trace or camera
+// ray; false once its pixel is complete.  Each heavy step (classification,
+// trace set-up, camera ray) has one call site, so the inlined code stays
+// small.
+__device__ __forceinline__ bool flow_advance(const MetricShared& S, const MetricImage& I,
+                                             const float4* __restrict__ sp, FlowLane& L, int f, float t) {
+    const uint32_t task = L.task;
+    const bool temporal = task >= FT_TCAM;
+    bool cam_miss = false;
+    if (task == FT_CAM || task == FT_TCAM) {
+        if (f < 0) {
+            cam_miss = true;
+        } else {
+            flow_entry(sp, L, f, t);
+            if (trace_begin(L.hp0, L.dir0, L.nrm0, L.cos_i0, I.nd, L.F, L.R)) {
+                L.task = task + 1;  // FT_DLINE / FT_TDLINE
+                return true;
+            }
+            // entry blocked: the d-line trace has ended
+        }
+    } else if (trace_step(sp, f, t, L.F, L.R)) {
+        return true;
+    }
+    // a camera ray missed, or a trace ended with its fate in L.F
+    int cls = CLS_NONE;
+    const bool dline = task == FT_CAM || task == FT_DLINE || task == FT_TCAM || task == FT_TDLINE;
+    if (dline && !cam_miss) cls = classify_fate(S, L.F, neg(L.dir0));
+    if (temporal) {
+        L.t_hit |= cam_miss ? 0u : (1u << L.ti);
+        L.t_ret |= cls == CLS_RETURNED ? (1u << L.ti) : 0u;
+        if (++L.ti < 5u) {
+            flow_camera(S, I, L, true);
+            return true;
+        }
+        return false;
+    }
+    bool has_fire = false;
+    float angle = 0.0f, weight = 0.0f;
+    int line = 0;  // the replay to set up next: 1 F, 2 C
+    if (dline) {
+        L.d_exit = L.F.exit_facet;
+        L.d_bounces = L.F.d_bounces;
+        L.d_reason = L.F.reason;
+        L.d_T = L.F.transmittance;
+        if (cls == CLS_RETURNED && S.want_fire) line = 1;
+    } else if (task == FT_FLINE) {
+        flow_keep_fline(L);
+        line = 2;
+    } else {  // FT_CLINE
+        cls = CLS_RETURNED;
+        has_fire = flow_fire(L, angle, weight);
+    }
+    while (line) {
+        if (trace_begin(L.hp0, L.dir0, L.nrm0, L.cos_i0, line == 1 ? I.nf : I.nc, L.F, L.R)) {
+            L.task = line == 1 ? FT_FLINE : FT_CLINE;
+            return true;
+        }
+        // entry blocked at this index: the replay has ended at once
+        if (line == 1) {
+            flow_keep_fline(L);
+            line = 2;
+        } else {
+            cls = CLS_RETURNED;
+            has_fire = flow_fire(L, angle, weight);
+            line = 0;
+        }
+    }
+    // the sample's tail (metric_pixel's sample loop body after classify)
+    if (cls != CLS_NONE) {
+        L.hits++;
+        if (L.s == 0) {
+            L.c_entry = L.entry_facet;
+            L.c_exit = L.d_exit;
+            L.c_bounces = L.d_bounces;
+            L.c_reason = L.d_reason;
+        }
+        if (L.first >= 0) L.mixed |= (uint32_t)(L.entry_facet != L.first);
+        else L.first = L.entry_facet;
+        if (cls == CLS_RETURNED && S.want_fire) {
+            if (has_fire && __builtin_isfinite(angle) && __builtin_isfinite(weight)) {
+                L.fire_sum = fma_(I.zero_dispersion ? 0.0f : angle, weight, L.fire_sum);
+                L.fire_q++;
+            } else {
+                L.fire_r++;
+            }
+        }
+        L.n0 += cls == 0 ? 1u : 0u;
+        L.n1 += cls == 1 ? 1u : 0u;
+        L.n2 += cls == 2 ? 1u : 0u;
+        L.n3 += cls == 3 ? 1u : 0u;
+    }
+    const bool more_samples = ++L.s < S.samples;
+    if (more_samples || (L.hits > 0 && S.want_scintillation)) {
+        L.ti = 0;
+        flow_camera(S, I, L, !more_samples);
+        return true;
+    }
+    return false;
+}
+
+// Occupancy target of the flow kernel (waves per SIMD).
+#ifndef IXG_METRIC_FLOW_WAVES_PER_EU
+#define IXG_METRIC_FLOW_WAVES_PER_EU 5
+#endif
+
+// Persistent over (image, tile) items as metric_queue_kernel; inside an item
+// the lanes take pixels one at a time and flow through their traces.
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(IXG_METRIC_FLOW_WAVES_PER_EU))) void
+metric_flow_kernel(const MetricShared S, const MetricImage* __restrict__ images, const float4* __restrict__ planes,
+                   ixg_optical_pixel* __restrict__ out, uint32_t* __restrict__ counter, uint32_t n_items,
+                   uint32_t tile) {
+    __shared__ float4 sp[IXG_MAX_PLANES];
+    __shared__ uint32_t s_item, s_next;
+    const uint32_t npx = S.width * S.height;
+    const uint32_t tiles = (npx + tile - 1) / tile;
+    const uint32_t lane = __lane_id();
+    uint32_t cur = 0xFFFFFFFFu;
+    while (true) {
+        if (threadIdx.x == 0) {
+            s_item = atomicAdd(counter, 1u);
+            s_next = 0u;
+        }
+        __syncthreads();
+        const uint32_t item = (uint32_t)__builtin_amdgcn_readfirstlane((int)s_item);
+        if (item >= n_items) break;
+        const uint32_t img = item / tiles;
+        const uint32_t first_px = (item % tiles) * tile;
+        const uint32_t n_px = min(tile, npx - first_px);
+        const MetricImage& I = images[img];
+        if (img != cur) {
+            for (uint32_t i = threadIdx.x; i < I.n_planes; i += BLOCK) sp[i] = planes[I.plane_offset + i];
+            cur = img;
+        }
+        __syncthreads();
+        ixg_optical_pixel* dst = out + (size_t)img * npx;
+        FlowLane L;
+        bool busy = false, dry = false;
+        while (true) {
+            if (!dry) {
+                // lanes without a pixel take the next ones: one LDS atomic per wave
+                const uint64_t need = __ballot(!busy);
+                if (need) {
+                    const uint32_t leader = (uint32_t)(__ffsll((unsigned long long)need) - 1);
+                    const uint32_t cnt = (uint32_t)__popcll(need);
+                    uint32_t base = 0;
+                    if (lane == leader) base = atomicAdd(&s_next, cnt);
+                    base = (uint32_t)__shfl((int)base, (int)leader);
+                    if (!busy) {
+                        const uint32_t k = base + (uint32_t)__popcll(need & ((1ull << lane) - 1ull));
+                        if (k < n_px) {
+                            flow_pixel_start(S, I, L, first_px + k);
+                            busy = true;
+                        }
+                    }
+                    dry = base + cnt >= n_px;
+                }
+            }
+            if (__ballot(busy) == 0ull) break;
+            if (busy) {
+                float t = 0.0f;
+                const int f = intersect(sp, I.n_planes, L.R.hp, L.R.cur, t);
+                busy = flow_advance(S, I, sp, L, f, t);
+                if (!busy) dst[L.p] = flow_record(L);
+            }
+        }
+        __syncthreads();  // s_item, s_next and sp are rewritten by the next item
+    }
+}
+
 __global__ void metric_zero_counter_kernel(uint32_t* c) { *c = 0u; }
 
 // Per-image totals and metrics in raster order (one lane per image).
@@ -627,6 +1033,34 @@ extern "C" hipError_t ixg_launch_metric_queue(const ixg::MetricShared* S, const 
     IXG_QUEUE(256, 256)
     IXG_QUEUE(256, 1024)
 #undef IXG_QUEUE
+    return hipErrorInvalidValue;
+}
+
+// The flow kernel: block lanes, pixels per item.
+extern "C" hipError_t ixg_launch_metric_flow(const ixg::MetricShared* S, const ixg::MetricImage* d_images,
+                                             const float4* d_planes, ixg_optical_pixel* d_out, uint32_t* d_counter,
+                                             int block, int tile, uint32_t grid_blocks, hipStream_t stream) {
+    const uint32_t npx = S->width * S->height;
+    if (tile <= 0) return hipErrorInvalidValue;
+    const uint32_t n_items = S->n_images * ((npx + (uint32_t)tile - 1) / (uint32_t)tile);
+    hipLaunchKernelGGL(ixg::metric_zero_counter_kernel, dim3(1), dim3(1), 0, stream, d_counter);
+#define IXG_FLOW(B)                                                                                          \
+    if (block == B) {                                                                                        \
+        hipLaunchKernelGGL((ixg::metric_flow_kernel<B>), dim3(grid_blocks), dim3(B), 0, stream, *S, d_images, \
+                           d_planes, d_out, d_counter, n_items, (uint32_t)tile);                             \
+        return hipGetLastError();                                                                            \
+    }
+    IXG_FLOW(64)
+    IXG_FLOW(128)
+    IXG_FLOW(256)
+#undef IXG_FLOW
+    return hipErrorInvalidValue;
+}
+
+extern "C" hipError_t ixg_metric_flow_occupancy(int block, int* per_cu) {
+    if (block == 64) return hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, ixg::metric_flow_kernel<64>, 64, 0);
+    if (block == 128) return hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, ixg::metric_flow_kernel<128>, 128, 0);
+    if (block == 256) return hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, ixg::metric_flow_kernel<256>, 256, 0);
     return hipErrorInvalidValue;
 }
 

@@ -27,6 +27,10 @@ extern "C" hipError_t ixg_launch_metric_queue(const ixg::MetricShared* S, const 
                                               const float4* d_planes, ixg_optical_pixel* d_out, uint32_t* d_counter,
                                               int block, int tile, uint32_t grid_blocks, hipStream_t stream);
 extern "C" hipError_t ixg_metric_queue_occupancy(int block, int tile, int* per_cu);
+extern "C" hipError_t ixg_launch_metric_flow(const ixg::MetricShared* S, const ixg::MetricImage* d_images,
+                                             const float4* d_planes, ixg_optical_pixel* d_out, uint32_t* d_counter,
+                                             int block, int tile, uint32_t grid_blocks, hipStream_t stream);
+extern "C" hipError_t ixg_metric_flow_occupancy(int block, int* per_cu);
 extern "C" int ixg_metric_queue_shapes(int* block, int* tile, int max_n);
 extern "C" hipError_t ixg_launch_scoring_probe(const ixg::MetricShared* S, const float* d_dirs, uint32_t n,
                                                float* d_out, hipStream_t stream);
@@ -186,10 +190,29 @@ ixg_status run_images(ixg_ctx* ctx, const MetricShared& S, const std::vector<Met
 // with the same shape replay after copying in their own designs.
 // ---------------------------------------------------------------------------
 struct Shape {
-    int block = 0;  // -1: 2-D grid kernel; otherwise the queue kernel's lanes per block
+    int block = 0;  // -1: 2-D grid kernel; > 0: the queue kernel's lanes per block;
+                    // -64 / -128 / -256: the flow kernel with that many lanes per block
     int tile = 0;   // pixels per queue item
     uint32_t grid = 0;
 };
+
+bool is_flow(int block) { return block == -64 || block == -128 || block == -256; }
+
+// The flow kernel's pixels per item when the shape leaves it open (tile 0):
+// whole images while there are at least two items per resident block, else
+// images cut into equal parts until there are (at least 64 pixels each).
+int flow_tile(int tile, size_t n_images, uint32_t npx, uint32_t grid) {
+    if (tile > 0) return tile;
+    const size_t want = 2 * (size_t)std::max<uint32_t>(grid, 1);
+    const size_t parts = std::max<size_t>(1, (want + n_images - 1) / std::max<size_t>(n_images, 1));
+    const size_t t = (npx + parts - 1) / parts;
+    return (int)std::min<size_t>(npx, std::max<size_t>(64, t));
+}
+
+// Resident blocks per CU of a queue or flow shape.
+hipError_t shape_occupancy(int block, int tile, int* per_cu) {
+    return is_flow(block) ? ixg_metric_flow_occupancy(-block, per_cu) : ixg_metric_queue_occupancy(block, tile, per_cu);
+}
 
 struct SweepPlan {
     int cus = 0;
@@ -262,7 +285,7 @@ hipError_t grow(T** buf, size_t* cap, size_t n, SweepPlan* P) {
 hipError_t enqueue_sweep(SweepPlan* P, const MetricShared& S, size_t n_images, const Shape& sh, hipStream_t st) {
     const size_t npx = (size_t)S.width * S.height;
     hipError_t e = hipSuccess;
-    if (sh.block < 0) {
+    if (sh.block == -1) {
         for (size_t first = 0; e == hipSuccess && first < n_images; first += 65535) {
             MetricShared Sc = S;
             Sc.n_images = (uint32_t)std::min<size_t>(65535, n_images - first);
@@ -271,7 +294,11 @@ hipError_t enqueue_sweep(SweepPlan* P, const MetricShared& S, size_t n_images, c
     } else {
         MetricShared Sc = S;
         Sc.n_images = (uint32_t)n_images;
-        e = ixg_launch_metric_queue(&Sc, P->d_img, P->d_pl, P->d_px, P->d_counter, sh.block, sh.tile, sh.grid, st);
+        e = is_flow(sh.block)
+                ? ixg_launch_metric_flow(&Sc, P->d_img, P->d_pl, P->d_px, P->d_counter, -sh.block,
+                                         flow_tile(sh.tile, n_images, (uint32_t)npx, sh.grid), sh.grid, st)
+                : ixg_launch_metric_queue(&Sc, P->d_img, P->d_pl, P->d_px, P->d_counter, sh.block, sh.tile, sh.grid,
+                                          st);
     }
     for (size_t first = 0; e == hipSuccess && first < n_images; first += 65535) {
         MetricShared Sc = S;
@@ -281,20 +308,26 @@ hipError_t enqueue_sweep(SweepPlan* P, const MetricShared& S, size_t n_images, c
     return e;
 }
 
-// Times every candidate shape on the first (up to) 128 images of this call
-// and keeps the fastest for this image size: the 2-D grid and each queue
-// shape at full residency.  A one-image pass over every candidate first
-// loads each kernel's code, so the timed passes measure only the work.
+// Times the candidate shapes at full residency on the first images of this
+// call -- about 2^26 sub-pixel rays' worth, at least 16 images -- and keeps
+// the fastest for this image size: the flow kernel at 64, 128 and 256 lanes
+// per block (items sized per call) and three work-queue shapes.  A one-image
+// pass over every candidate first loads each kernel's code, so the timed
+// passes measure only the work.
 hipError_t autotune(SweepPlan* P, const MetricShared& S, size_t n_images, hipStream_t st, Shape* best) {
-    const size_t n = std::min<size_t>(n_images, 128);
+    const size_t rays_per_image = std::max<size_t>(1, (size_t)S.width * S.height * S.samples);
+    const size_t n = std::min<size_t>(n_images, std::max<size_t>(16, ((size_t)1 << 26) / rays_per_image));
     std::vector<Shape> cands;
-    cands.push_back(Shape{-1, 0, 0});
-    int blocks[16], tiles[16];
-    int nq = ixg_metric_queue_shapes(blocks, tiles, 16);
-    for (int i = 0; i < nq; ++i) {
+    for (int lanes : {64, 128, 256}) {
         int per_cu = 0;
-        if (ixg_metric_queue_occupancy(blocks[i], tiles[i], &per_cu) != hipSuccess || per_cu <= 0) continue;
-        cands.push_back(Shape{blocks[i], tiles[i], (uint32_t)(per_cu * P->cus)});
+        if (ixg_metric_flow_occupancy(lanes, &per_cu) != hipSuccess || per_cu <= 0) continue;
+        cands.push_back(Shape{-lanes, 0, (uint32_t)(per_cu * P->cus)});
+    }
+    const int queue[][2] = {{64, 64}, {128, 128}, {256, 1024}};
+    for (const auto& q : queue) {
+        int per_cu = 0;
+        if (ixg_metric_queue_occupancy(q[0], q[1], &per_cu) != hipSuccess || per_cu <= 0) continue;
+        cands.push_back(Shape{q[0], q[1], (uint32_t)(per_cu * P->cus)});
     }
     hipEvent_t a = nullptr, b = nullptr;
     hipError_t e = hipEventCreate(&a);
@@ -343,9 +376,9 @@ ixg_status run_sweep(ixg_ctx* ctx, const MetricShared& S, const std::vector<Metr
     if (e == hipSuccess) {
         if (P->forced.block != 0) {
             sh = P->forced;
-            if (sh.block > 0) {
+            if (sh.block != -1) {
                 int per_cu = 0;
-                e = ixg_metric_queue_occupancy(sh.block, sh.tile, &per_cu);
+                e = shape_occupancy(sh.block, sh.tile, &per_cu);
                 sh.grid = (uint32_t)std::max(1, per_cu) * (uint32_t)P->cus;
             }
         } else {
@@ -411,6 +444,7 @@ ixg_status run_sweep(ixg_ctx* ctx, const MetricShared& S, const std::vector<Metr
         return ixg_internal_hip_fail(ctx, e, "metric sweep");
     }
     P->last = sh;
+    if (is_flow(sh.block)) P->last.tile = flow_tile(sh.tile, n, (uint32_t)npx, sh.grid);
     P->last_graphed = P->use_graph;
     return IXG_OK;
 }
@@ -538,12 +572,15 @@ extern "C" ixg_status ixg_set_metric_mode(ixg_ctx* ctx, int use_graph, int block
         int per_cu = 0;
         if (ixg_metric_queue_occupancy(block, tile, &per_cu) != hipSuccess)
             return ixg_internal_fail(ctx, IXG_INVALID, "unknown queue shape");
+    } else if (is_flow(block)) {
+        if (tile < 0) return ixg_internal_fail(ctx, IXG_INVALID, "flow tile must be >= 0 (0 = whole image)");
     } else if (block < -1) {
-        return ixg_internal_fail(ctx, IXG_INVALID, "block must be -1 (grid), 0 (autotune) or a queue shape");
+        return ixg_internal_fail(ctx, IXG_INVALID,
+                                 "block must be -1 (grid), 0 (autotune), a queue shape or -64/-128/-256 (flow)");
     }
     SweepPlan* P = get_plan(ctx);
     P->use_graph = use_graph ? 1 : 0;
-    P->forced = Shape{block, block > 0 ? tile : 0, 0};
+    P->forced = Shape{block, block != -1 ? tile : 0, 0};
     return IXG_OK;
 }
 

@@ -173,7 +173,9 @@ def axis_profiles(backend, planes, designs, azimuths_deg, environment):
 
 def set_metric_mode(backend, use_graph=True, block=0, tile=0):
     """How metric_sweep launches: graph capture on/off; block 0 = autotune,
-    -1 = the 2-D grid kernel, or a queue shape (block lanes, tile pixels)."""
+    -1 = the 2-D grid kernel, a queue shape (block lanes, tile pixels), or
+    -64 / -128 / -256 = the flow kernel with that many lanes per block and
+    tile pixels per item (0 = the whole image)."""
     st = lib().ixg_set_metric_mode(backend.handle, 1 if use_graph else 0, block, tile)
     if st != abi.IXG_OK:
         raise IxgError(st, backend.last_error())

@@ -55,8 +55,9 @@ def test_sweep_shapes_and_graph_replays_match_the_eager_grid(backend):
         ref_a = metrics.metric_sweep(backend, a_planes, a_ds, tilts, w, h, spp, 2.4, env)
         ref_b = metrics.metric_sweep(backend, b_planes, b_ds, tilts, w, h, spp, 2.4, env)
         assert metrics.metric_tuning(backend)[0] == -1
-        for block, tile in [(64, 64), (64, 256), (128, 128), (128, 512), (256, 256), (256, 1024)]:
-            print(f"sweep queue {block}x{tile}", flush=True)
+        for block, tile in [(64, 64), (64, 256), (128, 128), (128, 512), (256, 256), (256, 1024),
+                            (-64, 0), (-128, 0), (-256, 0), (-128, 100), (-64, 7)]:
+            print(f"sweep queue/flow {block}x{tile}", flush=True)
             metrics.set_metric_mode(backend, use_graph=False, block=block, tile=tile)
             got = metrics.metric_sweep(backend, a_planes, a_ds, tilts, w, h, spp, 2.4, env)
             assert np.array_equal(got, ref_a), (block, tile)
@@ -72,6 +73,29 @@ def test_sweep_shapes_and_graph_replays_match_the_eager_grid(backend):
             got_b = metrics.metric_sweep(backend, b_planes, b_ds, tilts, w, h, spp, 2.4, env)
             assert np.array_equal(got_b, ref_b), k
         block, tile, grid, graphed = metrics.metric_tuning(backend)
-        assert graphed and (block == -1 or (block > 0 and tile >= block and grid > 0))
+        assert graphed and grid > 0 and (block == -1 or (block > 0 and tile >= block) or
+                                         (block in (-64, -128, -256) and tile > 0))
+    finally:
+        metrics.set_metric_mode(backend, use_graph=True, block=0)
+
+
+def test_flow_kernel_matches_grid_on_studio_rig_and_c3_shape(backend):
+    """The flow kernel against the 2-D grid kernel on the lit studio rig
+    (light list, head shadow, Fire replays) and on a slice of the C3 sweep
+    shape (64x64 at 32 samples): the metrics of every image are bit-identical."""
+    tilts = np.array([-30.0, -3.0, 15.0], np.float32)
+    planes, ds, _ = designs.sweep_batch(4, first_seed=7)
+    cases = [(tracer.LightingPreset.studio("RingLights", 1.0, 0.85, 0.95), 32, 32, 9),
+             (tracer.LightingPreset.studio("IsoHemisphere", 1.0, 0.0, 0.0), 64, 64, 32)]
+    try:
+        for env, w, h, spp in cases:
+            print(f"flow vs grid {w}x{h}@{spp}", flush=True)
+            metrics.set_metric_mode(backend, use_graph=False, block=-1)
+            ref = metrics.metric_sweep(backend, planes, ds, tilts, w, h, spp, 2.4, env)
+            for block, tile in [(-128, 0), (-256, 333)]:
+                metrics.set_metric_mode(backend, use_graph=False, block=block, tile=tile)
+                got = metrics.metric_sweep(backend, planes, ds, tilts, w, h, spp, 2.4, env)
+                assert metrics.metric_tuning(backend)[0] == block
+                assert np.array_equal(got, ref), (block, tile, w)
     finally:
         metrics.set_metric_mode(backend, use_graph=True, block=0)
