@@ -191,10 +191,43 @@ struct PathState {
 
 __device__ __forceinline__ uint32_t compat_of(uint64_t c, int ch) { return (uint32_t)(c >> (8 * ch)) & 0xFFu; }
 
-// PathState::new (bounce_body.rs:258-356).
-template <bool ABSORB = false>
-__device__ __forceinline__ void path_init(const FilmParams& P, V3 org, V3 dir, uint32_t seed,
-                                          float hero, PathState& S) {
+// wrapped_hero_wavelengths (transport/mod.rs:58-66): the 8-lambda comb at
+// 50 nm from the hero draw, wrapped into [380, 780).
+__device__ __forceinline__ void hero_comb(float hero, float lambda[NCH]) {
+    const float lambda_hero = fma_(hero, 400.0f, 380.0f);
+#pragma unroll
+    for (int c = 0; c < NCH; ++c) {
+        float off = fma_((float)c, 50.0f, lambda_hero - 380.0f);
+        // rem_euclid(400): for 0 <= off < 800 this is off or off - 400, both
+        // exact (Sterbenz), which is what fmodf returns; a hero draw outside
+        // [0, 1) (explicit rays) takes the library remainder
+        float r = off >= 0.0f && off < 800.0f ? (off >= 400.0f ? off - 400.0f : off) : fmodf(off, 400.0f);
+        r = r < 0.0f ? r + 400.0f : r;
+        lambda[c] = 380.0f + r;
+    }
+}
+
+// The per-channel optics of a new path: n(lambda) and, for absorbing
+// materials, spectral_absorption (absorption.rs:33), the bands' sum in order.
+template <bool ABSORB>
+__device__ __forceinline__ void path_optics(const FilmParams& P, const float lambda[NCH], float n_o[NCH],
+                                            float alpha_o[NCH], float alpha_e[NCH]) {
+#pragma unroll
+    for (int c = 0; c < NCH; ++c) {
+        n_o[c] = dispersion(P.disp_kind, P.disp, lambda[c]);
+        if (ABSORB) {
+            float so = -0.0f, se = -0.0f;
+            for (int32_t b = 0; b < P.abs.n_o; ++b) so += band_eval(P.abs.o[b], lambda[c]);
+            for (int32_t b = 0; b < P.abs.n_e; ++b) se += band_eval(P.abs.e[b], lambda[c]);
+            alpha_o[c] = so;
+            alpha_e[c] = se;
+        }
+    }
+}
+
+// PathState::new (bounce_body.rs:258-356) with the comb and the optics
+// already formed.
+__device__ __forceinline__ void path_start(V3 org, V3 dir, uint32_t seed, PathState& S) {
     S.org = org;
     S.dir = dir;
     S.k = dir;
@@ -207,26 +240,8 @@ __device__ __forceinline__ void path_init(const FilmParams& P, V3 org, V3 dir, u
     S.inside = false;
     S.has_prev = false;
     S.compat = ~0ull;
-    const float lambda_hero = fma_(hero, 400.0f, 380.0f);
 #pragma unroll
     for (int c = 0; c < NCH; ++c) {
-        // wrapped_hero_wavelengths (transport/mod.rs:58-66)
-        float off = fma_((float)c, 50.0f, lambda_hero - 380.0f);
-        // rem_euclid(400): for 0 <= off < 800 this is off or off - 400, both
-        // exact (Sterbenz), which is what fmodf returns; a hero draw outside
-        // [0, 1) (explicit rays) takes the library remainder
-        float r = off >= 0.0f && off < 800.0f ? (off >= 400.0f ? off - 400.0f : off) : fmodf(off, 400.0f);
-        r = r < 0.0f ? r + 400.0f : r;
-        S.lambda[c] = 380.0f + r;
-        S.n_o[c] = dispersion(P.disp_kind, P.disp, S.lambda[c]);
-        if (ABSORB) {
-            // spectral_absorption (absorption.rs:33): the bands' sum in order
-            float so = -0.0f, se = -0.0f;
-            for (int32_t b = 0; b < P.abs.n_o; ++b) so += band_eval(P.abs.o[b], S.lambda[c]);
-            for (int32_t b = 0; b < P.abs.n_e; ++b) se += band_eval(P.abs.e[b], S.lambda[c]);
-            S.alpha_o[c] = so;
-            S.alpha_e[c] = se;
-        }
         S.path_pdf[c] = 1.0f;
         S.split[c] = 0.0f;
         S.st[c] = make_float4(1.0f, 0.0f, 0.0f, 0.0f);
@@ -234,6 +249,15 @@ __device__ __forceinline__ void path_init(const FilmParams& P, V3 org, V3 dir, u
     S.fh = 2166136261u;
     S.n_tir = S.n_refl = S.n_refr = S.n_probe = S.n_rot = 0;
     S.n_scan = S.n_probe_scan = 0;
+}
+
+// PathState::new (bounce_body.rs:258-356).
+template <bool ABSORB = false>
+__device__ __forceinline__ void path_init(const FilmParams& P, V3 org, V3 dir, uint32_t seed,
+                                          float hero, PathState& S) {
+    path_start(org, dir, seed, S);
+    hero_comb(hero, S.lambda);
+    path_optics<ABSORB>(P, S.lambda, S.n_o, S.alpha_o, S.alpha_e);
 }
 
 __device__ __forceinline__ float env_radiance(const EnvParams& E, V3 dir, V3 observer, float lambda) {
@@ -1239,25 +1263,23 @@ __device__ __forceinline__ int path_bounce(const FilmParams& P, const float4* __
     return STEP_CONTINUE;
 }
 
-// PathState::finish (bounce_body.rs:724-764) + color.rs:145-172 + von Kries.
-__device__ __forceinline__ void path_finish(const FilmParams& P, const PathState& S, bool escaped,
-                                            float radiance[NCH], float xyz[3]) {
-    if (escaped) {
-#pragma unroll
-        for (int c = 0; c < NCH; ++c) radiance[c] += S.split[c];
-    }
+// PathState::finish (bounce_body.rs:724-764) + color.rs:145-172 + von Kries,
+// from the path's final radiance (the exit-split staging already added for
+// an escaped path), MIS families, pdfs and wavelengths.
+__device__ __forceinline__ void finish_xyz(const FilmParams& P, uint64_t compat, const float path_pdf[NCH],
+                                           const float lambda[NCH], const float radiance[NCH], float xyz[3]) {
     const float norm_factor = (400.0f / 8.0f) / 106.856f;
     float X = 0.0f, Y = 0.0f, Z = 0.0f;
 #pragma unroll
     for (int c = 0; c < NCH; ++c) {
-        const uint32_t fam = compat_of(S.compat, c);
+        const uint32_t fam = compat_of(compat, c);
         float family = -0.0f;
 #pragma unroll
         for (int j = 0; j < NCH; ++j)
-            if ((fam >> j) & 1u) family += S.path_pdf[j];
-        float w = family <= 1e-12f ? 1.0f : div_rn(8.0f * S.path_pdf[0], family);
+            if ((fam >> j) & 1u) family += path_pdf[j];
+        float w = family <= 1e-12f ? 1.0f : div_rn(8.0f * path_pdf[0], family);
         float cx, cy, cz;
-        cie_cmf(S.lambda[c], cx, cy, cz);
+        cie_cmf(lambda[c], cx, cy, cz);
         float s = (radiance[c] * w) * norm_factor;
         X = X + cx * s;
         Y = Y + cy * s;
@@ -1272,6 +1294,15 @@ __device__ __forceinline__ void path_finish(const FilmParams& P, const PathState
     xyz[0] = fmaxf(fma_(0.986993f, L_, fma_(-0.147054f, M_, 0.159963f * S_)), 0.0f);
     xyz[1] = fmaxf(fma_(0.432305f, L_, fma_(0.518360f, M_, 0.049291f * S_)), 0.0f);
     xyz[2] = fmaxf(fma_(-0.008529f, L_, fma_(0.040043f, M_, 0.968487f * S_)), 0.0f);
+}
+
+__device__ __forceinline__ void path_finish(const FilmParams& P, const PathState& S, bool escaped,
+                                            float radiance[NCH], float xyz[3]) {
+    if (escaped) {
+#pragma unroll
+        for (int c = 0; c < NCH; ++c) radiance[c] += S.split[c];
+    }
+    finish_xyz(P, S.compat, S.path_pdf, S.lambda, radiance, xyz);
 }
 
 // trace_spectral_ray (transport/mod.rs:118) for one explicit ray, start to end.

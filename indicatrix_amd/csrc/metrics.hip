@@ -709,9 +709,12 @@ __device__ __forceinline__ bool flow_advance(const MetricShared& S, const Metric
     return false;
 }
 
-// Occupancy target of the flow kernel (waves per SIMD).
+// Occupancy target of the flow kernel (waves per SIMD).  Its lanes carry a
+// whole pixel's trace state; on the 512-design sweep (64x64 @ 32, 21 tilts)
+// 4 waves (128 VGPRs, some spilled) took 1.30 s, 3 waves (168, none) 1.34 s,
+// 5 waves 1.62 s and 7 waves 2.41 s.
 #ifndef IXG_METRIC_FLOW_WAVES_PER_EU
-#define IXG_METRIC_FLOW_WAVES_PER_EU 5
+#define IXG_METRIC_FLOW_WAVES_PER_EU 4
 #endif
 
 // Persistent over (image, tile) items as metric_queue_kernel; inside an item
