@@ -397,30 +397,46 @@ enum FlowTask : uint32_t {
     FT_TDLINE = 5,  // its d-line trace
 };
 
-// One trace_entry in progress: what its bounce loop carries.
+// One trace_entry in progress: what its bounce loop carries (the index and
+// its square are re-read from the image for the line being traced).
 struct TraceRun {
     V3 hp, cur;
-    float index, sin_crit, idx2, entry_T;
-    uint32_t b;
+    float sin_crit, entry_T;
+    uint32_t b;  // bounce; also the fate's d_bounces, and its bounces at an upward exit
 };
+
+// The fate of a flow trace: Fate less what the trace itself holds (the
+// bounce counts are TraceRun::b), its small integers packed in one word:
+// bits 0-1 fate, 2-10 exit facet + 1, 11-18 exit facet of an upward exit
+// (Fate::facet), 19-21 reason.
+struct FlowFate {
+    uint32_t bits;
+    V3 dir;
+    float transmittance, exit_cos;
+};
+__device__ __forceinline__ uint32_t ff_fate(uint32_t b) { return b & 3u; }
+__device__ __forceinline__ int32_t ff_exit(uint32_t b) { return (int32_t)((b >> 2) & 511u) - 1; }
+__device__ __forceinline__ uint32_t ff_facet(uint32_t b) { return (b >> 11) & 255u; }
+__device__ __forceinline__ uint32_t ff_reason(uint32_t b) { return (b >> 19) & 7u; }
+__device__ __forceinline__ uint32_t ff_pack(uint32_t fate, int32_t exit_facet, uint32_t facet, uint32_t reason) {
+    return fate | ((uint32_t)(exit_facet + 1) << 2) | (facet << 11) | (reason << 19);
+}
+__device__ __forceinline__ uint32_t ff_with_reason(uint32_t b, uint32_t reason) {
+    return (b & ~(7u << 19)) | (reason << 19);
+}
 
 // trace_entry up to its bounce loop; false when the trace ends without a
 // scan (entry blocked).
-__device__ __forceinline__ bool trace_begin(V3 point, V3 dir, V3 nrm, float cos_i, float index, Fate& F,
+__device__ __forceinline__ bool trace_begin(V3 point, V3 dir, V3 nrm, float cos_i, float index, FlowFate& F,
                                             TraceRun& R) {
-    F.fate = FATE_ABSORBED;
-    F.exit_facet = -1;
-    F.d_bounces = 0;
-    F.reason = REASON_BUDGET;
-    F.facet = 0;
-    F.bounces = 0;
+    F.bits = ff_pack(FATE_ABSORBED, -1, 0u, REASON_BUDGET);
     F.transmittance = 0.0f;
     F.exit_cos = 0.0f;
     F.dir = v3(0.0f, 0.0f, 0.0f);
+    R.b = 0;
     const float sin2_t = (1.0f / (index * index)) * fma_(cos_i, -cos_i, 1.0f);
     if (sin2_t > 1.0f) {
-        F.fate = FATE_ENTRY_BLOCKED;
-        F.reason = REASON_ENTRY;
+        F.bits = ff_pack(FATE_ENTRY_BLOCKED, -1, 0u, REASON_ENTRY);
         return false;
     }
     const float cos_t = sqrt_rn(1.0f - sin2_t);
@@ -429,17 +445,15 @@ __device__ __forceinline__ bool trace_begin(V3 point, V3 dir, V3 nrm, float cos_
     R.entry_T = fresnel_T(1.0f, index, cos_i, cos_t);
     R.hp = point + R.cur * 1e-4f;
     R.sin_crit = 1.0f / index;
-    R.idx2 = index * index;
-    R.index = index;
-    R.b = 0;
     return true;
 }
 
 // One iteration of trace_entry's bounce loop after its scan gave (f, t);
 // false when the trace has ended.
-__device__ __forceinline__ bool trace_step(const float4* __restrict__ pl, int f, float t, Fate& F, TraceRun& R) {
+__device__ __forceinline__ bool trace_step(const float4* __restrict__ pl, int f, float t, float index, FlowFate& F,
+                                           TraceRun& R) {
     if (f < 0) {
-        F.reason = REASON_NOHIT;
+        F.bits = ff_with_reason(F.bits, REASON_NOHIT);
         return false;
     }
     const V3 np = R.hp + R.cur * t;
@@ -449,125 +463,127 @@ __device__ __forceinline__ bool trace_step(const float4* __restrict__ pl, int f,
     const float one_m = fma_(ct, -ct, 1.0f);
     const float st = sqrt_rn(fmaxf(one_m, 0.0f));
     if (st < R.sin_crit) {
-        const float s2o = R.idx2 * one_m;
+        const float s2o = (index * index) * one_m;
         if (s2o <= 1.0f) {
-            const float index = R.index;
             const float co = sqrt_rn(1.0f - s2o);
             const V3 od = normalize(R.cur * index + n_out * fma_(index, -ct, co));
-            F.exit_facet = f;
-            F.reason = REASON_SIDE;
             if (n_out.y < -0.05f) {
-                F.reason = REASON_NONE;
-                F.fate = FATE_LEAKED;
+                F.bits = ff_pack(FATE_LEAKED, f, 0u, REASON_NONE);
             } else if (od.y > 0.05f) {
-                F.reason = REASON_NONE;
-                F.fate = FATE_UPWARD;
+                F.bits = ff_pack(FATE_UPWARD, f, (uint32_t)f, REASON_NONE);
                 F.dir = od;
                 F.transmittance = R.entry_T * fresnel_T(index, 1.0f, ct, co);
                 F.exit_cos = co;
-                F.facet = f;
-                F.bounces = R.b;
+            } else {
+                F.bits = ff_pack(ff_fate(F.bits), f, 0u, REASON_SIDE);
             }
             return false;
         }
     }
     R.cur = normalize(R.cur - n_out * (2.0f * ct));
     R.hp = np + R.cur * 1e-4f;
-    F.d_bounces = R.b + 1;
     ++R.b;
     return R.b < 10u;
 }
 
 // classify_ray's verdict on a finished d-line trace (classify.rs:140-183).
-__device__ __forceinline__ int classify_fate(const MetricShared& S, Fate& F, V3 observer) {
-    if (F.fate == FATE_ENTRY_BLOCKED) return CLS_OTHER;
-    if (F.fate == FATE_LEAKED) return CLS_WINDOWED;
-    if (F.fate == FATE_ABSORBED) return CLS_EXTINCT;
+__device__ __forceinline__ int classify_fate(const MetricShared& S, FlowFate& F, V3 observer) {
+    const uint32_t fate = ff_fate(F.bits);
+    if (fate == FATE_ENTRY_BLOCKED) return CLS_OTHER;
+    if (fate == FATE_LEAKED) return CLS_WINDOWED;
+    if (fate == FATE_ABSORBED) return CLS_EXTINCT;
     if (dot(F.dir, observer) > S.head_cos) {
-        F.reason = REASON_SHADOW;
+        F.bits = ff_with_reason(F.bits, REASON_SHADOW);
         return CLS_EXTINCT;
     }
     if (!(scoring_radiance(S, F.dir) > S.threshold)) {
-        F.reason = REASON_ILLUM;
+        F.bits = ff_with_reason(F.bits, REASON_ILLUM);
         return CLS_EXTINCT;
     }
     return CLS_RETURNED;
 }
 
-// What one lane carries between scans.
+// What one lane carries between scans.  The per-pixel counts (each <= 64
+// samples) are 8-bit fields:
+//   counts  classes[0..3]            (byte k = class k)
+//   tally   hits | fire_qualified << 8 | fire_rejected << 16 | mixed << 24
+//   centre  entry + 1 (bits 0-8) | exit + 1 (9-17) | bounces (18-21) | reason (22-24)
+//   step    sample (0-7) | task (8-10) | temporal camera (12-14) |
+//           temporal hits (16-20) | temporal returns (24-28)
+//   facets  first entry facet + 1 (0-8) | this sample's entry facet (16-23)
+//   a_bits  the F-line fate's fate (0-1) | facet (2-9) | bounces (10-13)
 struct FlowLane {
-    uint32_t p, s, task, ti;  // pixel (within the image), sample, FlowTask, temporal camera
-    uint32_t hits, n0, n1, n2, n3, fire_q, fire_r, mixed;
+    uint32_t p;
+    uint32_t step, counts, tally, centre, facets;
     float fire_sum;
-    int32_t c_entry, c_exit, first;
-    uint32_t c_bounces, c_reason;
-    uint32_t t_hit, t_ret;    // temporal cameras that met the stone / came back returned (bit i)
-    V3 hp0, dir0, nrm0;       // the traced ray's entry point, direction and entry-facet normal
-    float cos_i0;
-    int32_t entry_facet;
-    int32_t d_exit;           // the sample's d-line fate, as the sample's tail reads it
-    uint32_t d_bounces, d_reason;
-    float d_T;
-    int32_t a_fate, a_facet;  // the F-line fate, while the C line runs
-    uint32_t a_bounces;
+    V3 hp0, dir0;   // the traced ray's entry point and direction (its facet is in `facets`)
+    float d_T;      // d-line transmittance (Fire energy)
+    uint32_t a_bits;
     V3 a_dir;
     float a_T, a_exit;
-    TraceRun R;               // the running trace; R.hp / R.cur also hold a camera ray
-    Fate F;                   // its fate
+    TraceRun R;     // the running trace; R.hp / R.cur also hold a camera ray
+    FlowFate F;     // its fate
 };
+__device__ __forceinline__ uint32_t fl_sample(const FlowLane& L) { return L.step & 255u; }
+__device__ __forceinline__ uint32_t fl_task(const FlowLane& L) { return (L.step >> 8) & 7u; }
+__device__ __forceinline__ uint32_t fl_ti(const FlowLane& L) { return (L.step >> 12) & 7u; }
+__device__ __forceinline__ void fl_set_task(FlowLane& L, uint32_t task) { L.step = (L.step & ~(7u << 8)) | (task << 8); }
+__device__ __forceinline__ int32_t fl_entry(const FlowLane& L) { return (int32_t)((L.facets >> 16) & 255u); }
 
-// The next camera ray of the lane's pixel: sample L.s of the pixel's
-// sub-pixel pattern, or (temporal) the pixel centre from temporal camera L.ti
+// The index of the line a trace runs at: F and C for the replays, else d.
+__device__ __forceinline__ float fl_index(const MetricImage& I, uint32_t task) {
+    return task == FT_FLINE ? I.nf : (task == FT_CLINE ? I.nc : I.nd);
+}
+
+// The next camera ray of the lane's pixel: sample `sample` of the pixel's
+// sub-pixel pattern, or (temporal) the pixel centre from temporal camera ti
 // (Camera::generate_ray with zero jitter, camera.rs:149-167).
 __device__ __forceinline__ void flow_camera(const MetricShared& S, const MetricImage& I, FlowLane& L, bool temporal) {
     const uint32_t x = L.p % S.width, y = L.p / S.width;
     float jx = 0.5f, jy = 0.5f;
-    if (!temporal) sample_offset(L.s, jx, jy);
+    if (!temporal) sample_offset(fl_sample(L), jx, jy);
     const float sx = fma_((float)x + jx, S.scale_x, (float)S.origin_x - 0.5f);
     const float sy = fma_((float)y + jy, S.scale_y, (float)S.origin_y - 0.5f);
-    const uint32_t i = L.ti;
+    const uint32_t i = fl_ti(L);
     const float* fwd = temporal ? I.t_forward[i] : I.forward;
     const float* right = temporal ? I.t_right[i] : I.right;
     const float* up = temporal ? I.t_up[i] : I.up;
     const float* org = temporal ? I.t_origin[i] : I.origin;
     cam_ray(fwd, right, up, I.fov_tan, sx, sy, (float)S.proj_w, (float)S.proj_h, L.R.cur);
     L.R.hp = v3(org[0], org[1], org[2]);
-    L.task = temporal ? FT_TCAM : FT_CAM;
+    fl_set_task(L, temporal ? FT_TCAM : FT_CAM);
 }
 
 __device__ __forceinline__ void flow_pixel_start(const MetricShared& S, const MetricImage& I, FlowLane& L,
                                                  uint32_t p) {
     L.p = p;
-    L.s = 0;
-    L.ti = 0;
-    L.hits = L.n0 = L.n1 = L.n2 = L.n3 = L.fire_q = L.fire_r = L.mixed = 0u;
+    L.step = 0u;
+    L.counts = L.tally = L.centre = L.facets = 0u;  // centre 0: entry -1, exit -1, bounces 0, reason 0
     L.fire_sum = 0.0f;
-    L.c_entry = L.c_exit = L.first = -1;
-    L.c_bounces = L.c_reason = 0u;
-    L.t_hit = L.t_ret = 0u;
     flow_camera(S, I, L, false);
 }
 
 // The pixel's record, as metric_pixel returns it.
 __device__ __forceinline__ ixg_optical_pixel flow_record(const FlowLane& L) {
     ixg_optical_pixel px;
-    px.hits = L.hits;
-    px.classes[0] = L.n0;
-    px.classes[1] = L.n1;
-    px.classes[2] = L.n2;
-    px.classes[3] = L.n3;
+    px.hits = L.tally & 255u;
+    px.classes[0] = L.counts & 255u;
+    px.classes[1] = (L.counts >> 8) & 255u;
+    px.classes[2] = (L.counts >> 16) & 255u;
+    px.classes[3] = L.counts >> 24;
     px.fire_sum_deg = L.fire_sum;
-    px.fire_qualified = L.fire_q;
-    px.fire_rejected = L.fire_r;
-    px.centre_entry_facet = L.c_entry;
-    px.centre_exit_facet = L.c_exit;
-    px.centre_bounces = L.c_bounces;
-    px.centre_reason = L.c_reason;
-    px.mixed_facets = L.mixed;
+    px.fire_qualified = (L.tally >> 8) & 255u;
+    px.fire_rejected = (L.tally >> 16) & 255u;
+    px.centre_entry_facet = (int32_t)(L.centre & 511u) - 1;
+    px.centre_exit_facet = (int32_t)((L.centre >> 9) & 511u) - 1;
+    px.centre_bounces = (L.centre >> 18) & 15u;
+    px.centre_reason = (L.centre >> 22) & 7u;
+    px.mixed_facets = L.tally >> 24;
     px.scint_returned = 0;
     px.scint_poses = 0;
-    if (L.t_hit & 4u) {  // pose 2 (no offset) must meet the stone for any count
-        px.scint_returned = (uint8_t)__popc(L.t_ret);
+    const uint32_t t_hit = (L.step >> 16) & 31u, t_ret = (L.step >> 24) & 31u;
+    if (t_hit & 4u) {  // pose 2 (no offset) must meet the stone for any count
+        px.scint_returned = (uint8_t)__popc(t_ret);
         px.scint_poses = 5;
     }
     px.pad[0] = px.pad[1] = 0;
@@ -576,20 +592,23 @@ __device__ __forceinline__ ixg_optical_pixel flow_record(const FlowLane& L) {
 
 // The camera ray (L.R.hp, L.R.cur) met facet f at t: the entry of
 // classify_ray (classify.rs:118-131).
-__device__ __forceinline__ void flow_entry(const float4* __restrict__ sp, FlowLane& L, int f, float t) {
-    L.entry_facet = f;
+__device__ __forceinline__ void flow_entry(FlowLane& L, int f, float t) {
+    L.facets = (L.facets & 0xFFFFu) | ((uint32_t)f << 16);
     L.dir0 = L.R.cur;
     L.hp0 = L.R.hp + L.R.cur * t;
-    const float4 pf = sp[f];
-    L.nrm0 = v3(pf.x, pf.y, pf.z);
-    L.cos_i0 = clampf(dot(neg(L.dir0), L.nrm0), 0.0f, 1.0f);
 }
 
-// The F-line fate (in L.F) kept while the C line runs.
+// The entry facet's normal and the cosine of incidence (classify.rs:125-126).
+__device__ __forceinline__ void flow_entry_geometry(const float4* __restrict__ sp, const FlowLane& L, V3& nrm,
+                                                    float& cos_i) {
+    const float4 pf = sp[fl_entry(L)];
+    nrm = v3(pf.x, pf.y, pf.z);
+    cos_i = clampf(dot(neg(L.dir0), nrm), 0.0f, 1.0f);
+}
+
+// The F-line fate (in L.F, bounces in L.R.b) kept while the C line runs.
 __device__ __forceinline__ void flow_keep_fline(FlowLane& L) {
-    L.a_fate = L.F.fate;
-    L.a_facet = L.F.facet;
-    L.a_bounces = L.F.bounces;
+    L.a_bits = ff_fate(L.F.bits) | (ff_facet(L.F.bits) << 2) | (L.R.b << 10);
     L.a_dir = L.F.dir;
     L.a_T = L.F.transmittance;
     L.a_exit = L.F.exit_cos;
@@ -597,12 +616,13 @@ __device__ __forceinline__ void flow_keep_fline(FlowLane& L) {
 
 // fire_sample's tail (classify.rs:226-256) with the C-line fate in L.F.
 __device__ __forceinline__ bool flow_fire(const FlowLane& L, float& angle, float& weight) {
-    const Fate& B = L.F;
-    if (L.a_fate == FATE_UPWARD && B.fate == FATE_UPWARD && L.a_facet == B.facet && L.a_bounces == B.bounces) {
-        const float cs = clampf(dot(L.a_dir, B.dir), -1.0f, 1.0f);
+    const uint32_t a_fate = L.a_bits & 3u, a_facet = (L.a_bits >> 2) & 255u, a_bounces = L.a_bits >> 10;
+    if (a_fate == FATE_UPWARD && ff_fate(L.F.bits) == FATE_UPWARD && a_facet == ff_facet(L.F.bits) &&
+        a_bounces == L.R.b) {
+        const float cs = clampf(dot(L.a_dir, L.F.dir), -1.0f, 1.0f);
         angle = acosf(cs) * 57.2957795130823208767981548141051703f;
-        const float energy = L.d_T * L.a_T * B.transmittance;
-        const float radiance = fmaxf(L.a_exit, 0.0f) * fmaxf(B.exit_cos, 0.0f);
+        const float energy = L.d_T * L.a_T * L.F.transmittance;
+        const float radiance = fmaxf(L.a_exit, 0.0f) * fmaxf(L.F.exit_cos, 0.0f);
         weight = energy * radiance;
         return true;
     }
@@ -615,21 +635,22 @@ __device__ __forceinline__ bool flow_fire(const FlowLane& L, float& angle, float
 // small.
 __device__ __forceinline__ bool flow_advance(const MetricShared& S, const MetricImage& I,
                                              const float4* __restrict__ sp, FlowLane& L, int f, float t) {
-    const uint32_t task = L.task;
+    const uint32_t task = fl_task(L);
     const bool temporal = task >= FT_TCAM;
     bool cam_miss = false;
     if (task == FT_CAM || task == FT_TCAM) {
         if (f < 0) {
             cam_miss = true;
         } else {
-            flow_entry(sp, L, f, t);
-            if (trace_begin(L.hp0, L.dir0, L.nrm0, L.cos_i0, I.nd, L.F, L.R)) {
-                L.task = task + 1;  // FT_DLINE / FT_TDLINE
-                return true;
-            }
+            flow_entry(L, f, t);
+            V3 nrm;
+            float cos_i;
+            flow_entry_geometry(sp, L, nrm, cos_i);
+            fl_set_task(L, task + 1);  // FT_DLINE / FT_TDLINE
+            if (trace_begin(L.hp0, L.dir0, nrm, cos_i, I.nd, L.F, L.R)) return true;
             // entry blocked: the d-line trace has ended
         }
-    } else if (trace_step(sp, f, t, L.F, L.R)) {
+    } else if (trace_step(sp, f, t, fl_index(I, task), L.F, L.R)) {
         return true;
     }
     // a camera ray missed, or a trace ended with its fate in L.F
@@ -637,9 +658,11 @@ __device__ __forceinline__ bool flow_advance(const MetricShared& S, const Metric
     const bool dline = task == FT_CAM || task == FT_DLINE || task == FT_TCAM || task == FT_TDLINE;
     if (dline && !cam_miss) cls = classify_fate(S, L.F, neg(L.dir0));
     if (temporal) {
-        L.t_hit |= cam_miss ? 0u : (1u << L.ti);
-        L.t_ret |= cls == CLS_RETURNED ? (1u << L.ti) : 0u;
-        if (++L.ti < 5u) {
+        const uint32_t ti = fl_ti(L);
+        L.step |= cam_miss ? 0u : (1u << (16 + ti));
+        L.step |= cls == CLS_RETURNED ? (1u << (24 + ti)) : 0u;
+        if (ti + 1u < 5u) {
+            L.step += 1u << 12;
             flow_camera(S, I, L, true);
             return true;
         }
@@ -649,9 +672,10 @@ __device__ __forceinline__ bool flow_advance(const MetricShared& S, const Metric
     float angle = 0.0f, weight = 0.0f;
     int line = 0;  // the replay to set up next: 1 F, 2 C
     if (dline) {
-        L.d_exit = L.F.exit_facet;
-        L.d_bounces = L.F.d_bounces;
-        L.d_reason = L.F.reason;
+        if (cls != CLS_NONE && fl_sample(L) == 0) {  // the centre sample's record
+            L.centre = (uint32_t)(fl_entry(L) + 1) | ((uint32_t)(ff_exit(L.F.bits) + 1) << 9) | (L.R.b << 18) |
+                       (ff_reason(L.F.bits) << 22);
+        }
         L.d_T = L.F.transmittance;
         if (cls == CLS_RETURNED && S.want_fire) line = 1;
     } else if (task == FT_FLINE) {
@@ -662,10 +686,11 @@ __device__ __forceinline__ bool flow_advance(const MetricShared& S, const Metric
         has_fire = flow_fire(L, angle, weight);
     }
     while (line) {
-        if (trace_begin(L.hp0, L.dir0, L.nrm0, L.cos_i0, line == 1 ? I.nf : I.nc, L.F, L.R)) {
-            L.task = line == 1 ? FT_FLINE : FT_CLINE;
-            return true;
-        }
+        V3 nrm;
+        float cos_i;
+        flow_entry_geometry(sp, L, nrm, cos_i);
+        fl_set_task(L, line == 1 ? FT_FLINE : FT_CLINE);
+        if (trace_begin(L.hp0, L.dir0, nrm, cos_i, line == 1 ? I.nf : I.nc, L.F, L.R)) return true;
         // entry blocked at this index: the replay has ended at once
         if (line == 1) {
             flow_keep_fline(L);
@@ -678,41 +703,33 @@ __device__ __forceinline__ bool flow_advance(const MetricShared& S, const Metric
     }
     // the sample's tail (metric_pixel's sample loop body after classify)
     if (cls != CLS_NONE) {
-        L.hits++;
-        if (L.s == 0) {
-            L.c_entry = L.entry_facet;
-            L.c_exit = L.d_exit;
-            L.c_bounces = L.d_bounces;
-            L.c_reason = L.d_reason;
-        }
-        if (L.first >= 0) L.mixed |= (uint32_t)(L.entry_facet != L.first);
-        else L.first = L.entry_facet;
+        L.tally += 1u;  // hits
+        const uint32_t entry = (uint32_t)fl_entry(L);
+        const uint32_t first = L.facets & 511u;
+        if (first > 0u) L.tally |= (uint32_t)(entry + 1u != first) << 24;
+        else L.facets |= entry + 1u;
         if (cls == CLS_RETURNED && S.want_fire) {
             if (has_fire && __builtin_isfinite(angle) && __builtin_isfinite(weight)) {
                 L.fire_sum = fma_(I.zero_dispersion ? 0.0f : angle, weight, L.fire_sum);
-                L.fire_q++;
+                L.tally += 1u << 8;
             } else {
-                L.fire_r++;
+                L.tally += 1u << 16;
             }
         }
-        L.n0 += cls == 0 ? 1u : 0u;
-        L.n1 += cls == 1 ? 1u : 0u;
-        L.n2 += cls == 2 ? 1u : 0u;
-        L.n3 += cls == 3 ? 1u : 0u;
+        L.counts += 1u << (8 * cls);
     }
-    const bool more_samples = ++L.s < S.samples;
-    if (more_samples || (L.hits > 0 && S.want_scintillation)) {
-        L.ti = 0;
-        flow_camera(S, I, L, !more_samples);
+    const uint32_t s = fl_sample(L) + 1u;
+    L.step = (L.step & ~255u) | s;
+    const bool more_samples = s < S.samples;
+    if (more_samples || ((L.tally & 255u) > 0u && S.want_scintillation)) {
+        flow_camera(S, I, L, !more_samples);  // ti is 0
         return true;
     }
     return false;
 }
 
 // Occupancy target of the flow kernel (waves per SIMD).  Its lanes carry a
-// whole pixel's trace state; on the 512-design sweep (64x64 @ 32, 21 tilts)
-// 4 waves (128 VGPRs, some spilled) took 1.30 s, 3 waves (168, none) 1.34 s,
-// 5 waves 1.62 s and 7 waves 2.41 s.
+// whole pixel's trace state (packed: 128 VGPRs, nothing spilled).
 #ifndef IXG_METRIC_FLOW_WAVES_PER_EU
 #define IXG_METRIC_FLOW_WAVES_PER_EU 4
 #endif

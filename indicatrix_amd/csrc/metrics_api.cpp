@@ -310,15 +310,17 @@ hipError_t enqueue_sweep(SweepPlan* P, const MetricShared& S, size_t n_images, c
 
 // Times the candidate shapes at full residency on the first images of this
 // call -- about 2^26 sub-pixel rays' worth, at least 16 images -- and keeps
-// the fastest for this image size: the flow kernel at 64, 128 and 256 lanes
-// per block (items sized per call) and three work-queue shapes.  A one-image
+// the fastest for this image size: the flow kernel at 128 and 256 lanes per
+// block (items sized per call) and three work-queue shapes.  (The flow
+// kernel at 64 lanes wins only on small tuning samples cut into many items;
+// on whole images it ran 20-25 % behind.)  A one-image
 // pass over every candidate first loads each kernel's code, so the timed
 // passes measure only the work.
 hipError_t autotune(SweepPlan* P, const MetricShared& S, size_t n_images, hipStream_t st, Shape* best) {
     const size_t rays_per_image = std::max<size_t>(1, (size_t)S.width * S.height * S.samples);
     const size_t n = std::min<size_t>(n_images, std::max<size_t>(16, ((size_t)1 << 26) / rays_per_image));
     std::vector<Shape> cands;
-    for (int lanes : {64, 128, 256}) {
+    for (int lanes : {128, 256}) {
         int per_cu = 0;
         if (ixg_metric_flow_occupancy(lanes, &per_cu) != hipSuccess || per_cu <= 0) continue;
         cands.push_back(Shape{-lanes, 0, (uint32_t)(per_cu * P->cus)});
