@@ -172,8 +172,8 @@ __device__ __forceinline__ void camera_sample(const FilmParams& P, uint32_t pixe
 struct PathState {
     float4 st[NCH];           // Stokes per channel
     float path_pdf[NCH];
-    float split[NCH];         // exit-split staging radiance
-    float lambda[NCH];
+    float split[NCH];         // exit-split staging radiance; an escaped path's final radiance
+    float hero;               // the hero draw; channel c's wavelength is lambda_of(S, c)
     float n_o[NCH];
     float alpha_o[NCH], alpha_e[NCH];  // spectral_absorption, absorbing materials only
     uint64_t compat;          // byte c = MIS family of channel c
@@ -205,6 +205,15 @@ __device__ __forceinline__ void hero_comb(float hero, float lambda[NCH]) {
         r = r < 0.0f ? r + 400.0f : r;
         lambda[c] = 380.0f + r;
     }
+}
+
+// Channel c's wavelength of the comb (hero_comb's element c).
+__device__ __forceinline__ float comb_lambda(float hero, int c) {
+    const float lambda_hero = fma_(hero, 400.0f, 380.0f);
+    float off = fma_((float)c, 50.0f, lambda_hero - 380.0f);
+    float r = off >= 0.0f && off < 800.0f ? (off >= 400.0f ? off - 400.0f : off) : fmodf(off, 400.0f);
+    r = r < 0.0f ? r + 400.0f : r;
+    return 380.0f + r;
 }
 
 // The per-channel optics of a new path: n(lambda) and, for absorbing
@@ -256,8 +265,10 @@ template <bool ABSORB = false>
 __device__ __forceinline__ void path_init(const FilmParams& P, V3 org, V3 dir, uint32_t seed,
                                           float hero, PathState& S) {
     path_start(org, dir, seed, S);
-    hero_comb(hero, S.lambda);
-    path_optics<ABSORB>(P, S.lambda, S.n_o, S.alpha_o, S.alpha_e);
+    S.hero = hero;
+    float lambda[NCH];
+    hero_comb(hero, lambda);
+    path_optics<ABSORB>(P, lambda, S.n_o, S.alpha_o, S.alpha_e);
 }
 
 __device__ __forceinline__ float env_radiance(const EnvParams& E, V3 dir, V3 observer, float lambda) {
@@ -384,8 +395,8 @@ __device__ __forceinline__ void split_probe(const FilmParams& P, const float4* _
         if (EVENTS) ++S.n_probe_scan;
     }
     if (!escapes) return;
-    const float env = ISO ? lit_factor(P.env, dk, S.observer) * (spec_power(P.env, S.lambda[c]) * P.env.exposure)
-                          : env_radiance(P.env, dk, S.observer, S.lambda[c]);
+    const float env = ISO ? lit_factor(P.env, dk, S.observer) * (spec_power(P.env, comb_lambda(S.hero, c)) * P.env.exposure)
+                          : env_radiance(P.env, dk, S.observer, comb_lambda(S.hero, c));
     S.split[c] = fma_(fmaxf(transmitted_i, 0.0f) * 1.0f, env, S.split[c]);
 }
 
@@ -400,6 +411,46 @@ __device__ __forceinline__ V3 reflect_dir(V3 k, V3 normal) { return k - normal *
 // closed-form entry / internal arms where the wave is not along the optic
 // axis, the eigenmode-drawing scalar arms where it is, assigned-mode
 // absorption and the internal o<->e re-coupling draw.
+// The refracted directions of one entry event's channels, kept in LDS (a
+// column per lane of the 256-lane film blocks) from the channel loop to
+// narrow_compat, so 24 registers are not held across the loop (whose exit
+// probes run slab scans of their own).
+constexpr int kFilmLanes = 256;
+struct ChannelDirs {
+    float v[3][NCH][kFilmLanes];
+};
+__device__ __forceinline__ ChannelDirs& channel_dirs() {
+    __shared__ ChannelDirs d;
+    return d;
+}
+__device__ __forceinline__ void channel_dir_put(int c, V3 dir) {
+    ChannelDirs& d = channel_dirs();
+    d.v[0][c][threadIdx.x] = dir.x;
+    d.v[1][c][threadIdx.x] = dir.y;
+    d.v[2][c][threadIdx.x] = dir.z;
+}
+// narrow_compat (refraction/context.rs:219-242): channels whose refracted
+// directions disagree leave each other's MIS family; channel 0 against the
+// others by the hero-direction match, the rest pairwise from the stored
+// directions of the live channels.
+__device__ __forceinline__ void narrow_compat(uint64_t& compat, uint32_t match_mask, uint32_t live) {
+    const ChannelDirs& d = channel_dirs();
+    V3 dirs[NCH];
+#pragma unroll
+    for (int c = 0; c < NCH; ++c) dirs[c] = v3(d.v[0][c][threadIdx.x], d.v[1][c][threadIdx.x], d.v[2][c][threadIdx.x]);
+#pragma unroll
+    for (int a = 0; a < NCH; ++a)
+#pragma unroll
+        for (int b = a + 1; b < NCH; ++b) {
+            bool m;
+            if (a == 0) m = (match_mask >> b) & 1u;
+            else if (((live >> a) & 1u) && ((live >> b) & 1u))
+                m = dot(dirs[a], dirs[b]) >= DIRECTION_MATCH_COS_TOL;
+            else m = true;
+            if (!m) compat &= ~((1ull << (8 * a + b)) | (1ull << (8 * b + a)));
+        }
+}
+
 template <bool EVENTS, bool ISO, bool ABSORB, bool EXTRA = true>
 __device__ __forceinline__ int facet_aniso(const FilmParams& P, const float4* __restrict__ planes, PathState& S,
                                         float t_hit, int facet) {
@@ -430,7 +481,7 @@ __device__ __forceinline__ int facet_aniso(const FilmParams& P, const float4* __
     if (ABSORB && inside) {
         // apply_absorption with the assigned-mode coefficient (absorption.rs:224-249)
         const float n_o_h = S.n_o[0];
-        const float n_e_h = e_index_at(P, S.lambda[0], n_o_h);
+        const float n_e_h = e_index_at(P, comb_lambda(S.hero, 0), n_o_h);
         V3 e;
         if (!is_extra) {
             e = ordinary_eigen(k, cax);
@@ -454,7 +505,7 @@ __device__ __forceinline__ int facet_aniso(const FilmParams& P, const float4* __
     if (!inside) {
         // theta_c_for_bounce (geometry.rs:75-120): two fixed-point steps
         const float n_o_seed = S.n_o[0];
-        const float n_e_seed = e_index_at(P, S.lambda[0], n_o_seed);
+        const float n_e_seed = e_index_at(P, comb_lambda(S.hero, 0), n_o_seed);
         float n_guess = n_o_seed;
         theta_c = 0.0f;
         for (int it = 0; it < 2; ++it) {
@@ -472,7 +523,7 @@ __device__ __forceinline__ int facet_aniso(const FilmParams& P, const float4* __
     float n_e_ch[NCH], n_med[NCH];
 #pragma unroll
     for (int c = 0; c < NCH; ++c) {
-        n_e_ch[c] = e_index_at(P, S.lambda[c], S.n_o[c]);
+        n_e_ch[c] = e_index_at(P, comb_lambda(S.hero, c), S.n_o[c]);
         const float n_eff = effective_extraordinary_index(S.n_o[c], n_e_ch[c], theta_c);
         n_med[c] = (inside && !is_extra) ? S.n_o[c] : n_eff;
     }
@@ -576,7 +627,6 @@ __device__ __forceinline__ int facet_aniso(const FilmParams& P, const float4* __
                 S.fh = fnv(fnv(S.fh, (uint32_t)facet), 3u);
             }
             const float inv_t = div_rn(1.0f, 1.0f - r_branch);
-            V3 dirs[NCH];
             uint32_t live = 0u, match_mask = 0u;
 #pragma unroll
             for (int c = 0; c < NCH; ++c) {
@@ -584,7 +634,6 @@ __device__ __forceinline__ int facet_aniso(const FilmParams& P, const float4* __
                 const float n2k = use_extra ? n_med[c] : n_o_k;
                 const float r = div_rn(n1, n2k);
                 const float s2k = r * r * one_m_c2;
-                dirs[c] = v3(0.0f, 0.0f, 0.0f);
                 if (s2k > 1.0f) {
                     S.st[c] = scale4(S.st[c], 0.0f);
                     S.path_pdf[c] = 0.0f;
@@ -593,7 +642,7 @@ __device__ __forceinline__ int facet_aniso(const FilmParams& P, const float4* __
                 const float ek = div_rn(n1, n2k);
                 const V3 rk = refract_dir(k, normal, ek, cos_i, sqrt_rn(fmaxf(1.0f - s2k, 0.0f)));
                 const V3 fk = use_extra ? extraordinary_poynting_dir(rk, cax, n_o_k, n_o_k + P.an.biref) : rk;
-                dirs[c] = fk;
+                channel_dir_put(c, fk);
                 live |= 1u << c;
                 const bool match = dot(fk, final_refr) >= DIRECTION_MATCH_COS_TOL;
                 EntrySol sol;
@@ -618,18 +667,7 @@ __device__ __forceinline__ int facet_aniso(const FilmParams& P, const float4* __
                 S.st[c] = scale4(make_float4(dep, dep * c2, dep * s2, 0.0f), inv_t);
                 S.path_pdf[c] *= t_unpol;
             }
-            // narrow_compat (refraction/context.rs:219-242)
-#pragma unroll
-            for (int a = 0; a < NCH; ++a)
-#pragma unroll
-                for (int b = a + 1; b < NCH; ++b) {
-                    bool m;
-                    if (a == 0) m = (match_mask >> b) & 1u;
-                    else if (((live >> a) & 1u) && ((live >> b) & 1u))
-                        m = dot(dirs[a], dirs[b]) >= DIRECTION_MATCH_COS_TOL;
-                    else m = true;
-                    if (!m) S.compat &= ~((1ull << (8 * a + b)) | (1ull << (8 * b + a)));
-                }
+            narrow_compat(S.compat, match_mask, live);
             new_k = refr;
             new_s = final_refr;
             new_inside = true;
@@ -774,7 +812,6 @@ __device__ __forceinline__ int facet_aniso(const FilmParams& P, const float4* __
             const V3 final_refr =
                 (entering && use_extra) ? extraordinary_poynting_dir(refr, cax, n_o_hero, n_e_hero) : refr;
             const float inv_t = div_rn(1.0f, 1.0f - r_unpol);
-            V3 dirs[NCH];
             uint32_t live = 0u, match_mask = 0u;
 #pragma unroll
             for (int c = 0; c < NCH; ++c) {
@@ -783,7 +820,6 @@ __device__ __forceinline__ int facet_aniso(const FilmParams& P, const float4* __
                 const float n2k = (entering && !use_extra) ? S.n_o[c] : (inside ? 1.0f : n_med[c]);
                 const float r = div_rn(n1k, n2k);
                 const float s2k = r * r * one_m_c2;
-                dirs[c] = v3(0.0f, 0.0f, 0.0f);
                 if (s2k > 1.0f) {
                     S.st[c] = scale4(S.st[c], 0.0f);
                     S.path_pdf[c] = 0.0f;
@@ -820,22 +856,10 @@ __device__ __forceinline__ int facet_aniso(const FilmParams& P, const float4* __
                     if (inside && fmaxf(orig.x, 0.0f) > 0.0f)
                         split_probe<EVENTS, ISO>(P, planes, S, facet, hit, c, rk, fmaxf(tr.x, 0.0f));
                 }
-                dirs[c] = fk;
+                if (!inside) channel_dir_put(c, fk);
                 live |= 1u << c;
             }
-            if (!inside) {
-#pragma unroll
-                for (int a = 0; a < NCH; ++a)
-#pragma unroll
-                    for (int b = a + 1; b < NCH; ++b) {
-                        bool m;
-                        if (a == 0) m = (match_mask >> b) & 1u;
-                        else if (((live >> a) & 1u) && ((live >> b) & 1u))
-                            m = dot(dirs[a], dirs[b]) >= DIRECTION_MATCH_COS_TOL;
-                        else m = true;
-                        if (!m) S.compat &= ~((1ull << (8 * a + b)) | (1ull << (8 * b + a)));
-                    }
-            }
+            if (!inside) narrow_compat(S.compat, match_mask, live);
             new_k = refr;
             new_s = final_refr;
             new_inside = !inside;
@@ -883,7 +907,7 @@ __device__ __forceinline__ int facet_aniso(const FilmParams& P, const float4* __
 }
 
 // One iteration of the bounce loop (bounce_body.rs:361-707): intersection, and
-// either the escape lookup (written to `radiance`) or the facet event.
+// either the escape lookup (added to the split staging) or the facet event.
 // apply_russian_roulette (transport/bounce.rs:355-375): false when the path dies.
 __device__ __forceinline__ bool russian_roulette(PathState& S, uint32_t bounce) {
     float mx = 0.0f;
@@ -983,7 +1007,7 @@ __device__ __forceinline__ int scatter_step(const FilmParams& P, PathState& S, f
 // lean bounce.
 template <bool EVENTS, bool ISO = false, bool ABSORB = false, bool ANISO = false, bool EXTRA = true>
 __device__ __forceinline__ int path_bounce(const FilmParams& P, const float4* __restrict__ planes,
-                                           PathState& S, float radiance[NCH]) {
+                                           PathState& S) {
     const uint32_t bounce = S.bounce;
     float t_hit = 0.0f;
     int facet;
@@ -1000,30 +1024,30 @@ __device__ __forceinline__ int path_bounce(const FilmParams& P, const float4* __
             hdr_rgb(P.env, S.dir, rgb);
 #pragma unroll
             for (int c = 0; c < NCH; ++c) {
-                float env = rgb_to_spectral(rgb, S.lambda[c]);
-                radiance[c] = fma_(fmaxf(S.st[c].x, 0.0f) * 1.0f, env, 0.0f);
+                float env = rgb_to_spectral(rgb, comb_lambda(S.hero, c));
+                S.split[c] = fma_(fmaxf(S.st[c].x, 0.0f) * 1.0f, env, 0.0f) + S.split[c];
             }
         } else if (!ISO && P.env.model == ENV_ASET) {
             const V3 d = normalize(S.dir);
 #pragma unroll
             for (int c = 0; c < NCH; ++c) {
-                float env = aset_radiance(P.env, d, S.lambda[c], P.env.exposure);
-                radiance[c] = fma_(fmaxf(S.st[c].x, 0.0f) * 1.0f, env, 0.0f);
+                float env = aset_radiance(P.env, d, comb_lambda(S.hero, c), P.env.exposure);
+                S.split[c] = fma_(fmaxf(S.st[c].x, 0.0f) * 1.0f, env, 0.0f) + S.split[c];
             }
         } else if (ISO || P.env.model != ENV_STUDIO) {
             const float f = lit_factor(P.env, S.dir, S.observer);
 #pragma unroll
             for (int c = 0; c < NCH; ++c) {
-                float env = f * (spec_power(P.env, S.lambda[c]) * P.env.exposure);
-                radiance[c] = fma_(fmaxf(S.st[c].x, 0.0f) * 1.0f, env, 0.0f);
+                float env = f * (spec_power(P.env, comb_lambda(S.hero, c)) * P.env.exposure);
+                S.split[c] = fma_(fmaxf(S.st[c].x, 0.0f) * 1.0f, env, 0.0f) + S.split[c];
             }
         } else {
             DirLight L;
             direction_lighting(P.env, S.dir, S.observer, L);
 #pragma unroll
             for (int c = 0; c < NCH; ++c) {
-                float env = light_radiance(P.env, L, spec_power(P.env, S.lambda[c]));
-                radiance[c] = fma_(fmaxf(S.st[c].x, 0.0f) * 1.0f, env, 0.0f);
+                float env = light_radiance(P.env, L, spec_power(P.env, comb_lambda(S.hero, c)));
+                S.split[c] = fma_(fmaxf(S.st[c].x, 0.0f) * 1.0f, env, 0.0f) + S.split[c];
             }
         }
         return STEP_ESCAPED;
@@ -1152,7 +1176,6 @@ __device__ __forceinline__ int path_bounce(const FilmParams& P, const float4* __
         const float ct_dir = sqrt_rn(fmaxf(1.0f - s2d, 0.0f));
         const V3 refr = normalize(k * eta + normal * fma_(eta, cos_i, -ct_dir));
         const float inv_t = div_rn(1.0f, 1.0f - r_unpol);
-        V3 dirs[NCH];
         uint32_t live = 0u, match_mask = 0u;
 #pragma unroll
         for (int c = 0; c < NCH; ++c) {
@@ -1166,7 +1189,6 @@ __device__ __forceinline__ int path_bounce(const FilmParams& P, const float4* __
             if (s2k > 1.0f) {
                 S.st[c] = scale4(S.st[c], 0.0f);
                 S.path_pdf[c] = 0.0f;
-                dirs[c] = v3(0.0f, 0.0f, 0.0f);
                 continue;
             }
             float ctk = sqrt_rn(fmaxf(1.0f - s2k, 0.0f));
@@ -1211,29 +1233,16 @@ __device__ __forceinline__ int path_bounce(const FilmParams& P, const float4* __
                     }
                     if (escapes) {
                         float env = ISO ? lit_factor(P.env, dk, S.observer) *
-                                              (spec_power(P.env, S.lambda[c]) * P.env.exposure)
-                                        : env_radiance(P.env, dk, S.observer, S.lambda[c]);
+                                              (spec_power(P.env, comb_lambda(S.hero, c)) * P.env.exposure)
+                                        : env_radiance(P.env, dk, S.observer, comb_lambda(S.hero, c));
                         S.split[c] = fma_(fmaxf(fmaxf(tr.x, 0.0f), 0.0f) * 1.0f, env, S.split[c]);
                     }
                 }
             }
-            dirs[c] = dk;
+            if (!inside) channel_dir_put(c, dk);
             live |= 1u << c;
         }
-        if (!inside) {
-            // narrow_compat (refraction/context.rs:219-242)
-#pragma unroll
-            for (int a = 0; a < NCH; ++a)
-#pragma unroll
-                for (int b = a + 1; b < NCH; ++b) {
-                    bool m;
-                    if (a == 0) m = (match_mask >> b) & 1u;
-                    else if (((live >> a) & 1u) && ((live >> b) & 1u))
-                        m = dot(dirs[a], dirs[b]) >= DIRECTION_MATCH_COS_TOL;
-                    else m = true;
-                    if (!m) S.compat &= ~((1ull << (8 * a + b)) | (1ull << (8 * b + a)));
-                }
-        }
+        if (!inside) narrow_compat(S.compat, match_mask, live);
         new_k = refr;
         new_inside = !inside;
     }
@@ -1296,13 +1305,15 @@ __device__ __forceinline__ void finish_xyz(const FilmParams& P, uint64_t compat,
     xyz[2] = fmaxf(fma_(-0.008529f, L_, fma_(0.040043f, M_, 0.968487f * S_)), 0.0f);
 }
 
-__device__ __forceinline__ void path_finish(const FilmParams& P, const PathState& S, bool escaped,
-                                            float radiance[NCH], float xyz[3]) {
-    if (escaped) {
+// An escaped path's radiance is in S.split (the miss radiance added to the
+// exit-split staging when the path escaped); any other ending has none.
+__device__ __forceinline__ void path_finish(const FilmParams& P, const PathState& S, bool escaped, float xyz[3]) {
+    float radiance[NCH];
 #pragma unroll
-        for (int c = 0; c < NCH; ++c) radiance[c] += S.split[c];
-    }
-    finish_xyz(P, S.compat, S.path_pdf, S.lambda, radiance, xyz);
+    for (int c = 0; c < NCH; ++c) radiance[c] = escaped ? S.split[c] : 0.0f;
+    float lambda[NCH];
+    hero_comb(S.hero, lambda);
+    finish_xyz(P, S.compat, S.path_pdf, lambda, radiance, xyz);
 }
 
 // trace_spectral_ray (transport/mod.rs:118) for one explicit ray, start to end.
@@ -1312,14 +1323,11 @@ __device__ __forceinline__ void trace_path(const FilmParams& P, const float4* __
                                            PathOut& out, uint32_t* counts) {
     PathState S;
     path_init<ABSORB>(P, org, dir, seed, hero, S);
-    float radiance[NCH];
-#pragma unroll
-    for (int c = 0; c < NCH; ++c) radiance[c] = 0.0f;
     uint32_t bounces = P.max_bounces, term = 3u;
     bool escaped = false;
     for (uint32_t b = 0; b < P.max_bounces; ++b) {
         S.bounce = b;
-        int r = path_bounce<EVENTS, false, ABSORB, ANISO>(P, planes, S, radiance);
+        int r = path_bounce<EVENTS, false, ABSORB, ANISO>(P, planes, S);
         if (r != STEP_CONTINUE) {
             bounces = b + 1;
             term = r == STEP_ESCAPED ? 0u : (r == STEP_SCATTER_ABSORBED ? 4u : 2u);
@@ -1327,7 +1335,7 @@ __device__ __forceinline__ void trace_path(const FilmParams& P, const float4* __
             break;
         }
     }
-    path_finish(P, S, escaped, radiance, out.xyz);
+    path_finish(P, S, escaped, out.xyz);
     if (EVENTS) {
         out.ev.bounces = bounces;
         out.ev.termination = term;
@@ -1396,7 +1404,6 @@ __global__ __launch_bounds__(kBlock) IXG_WAVES_ATTR void film_persistent_kernel(
     const uint32_t n = (uint32_t)P.n_paths;
     const uint32_t lane = __lane_id();
     PathState S;
-    float radiance[NCH];
     uint32_t idx = 0;
     bool active = false, done = false;
     bool pending = false, pending_escaped = false;  // finished, XYZ not yet written
@@ -1411,7 +1418,7 @@ __global__ __launch_bounds__(kBlock) IXG_WAVES_ATTR void film_persistent_kernel(
         if (need_mask && (__popcll(need_mask) >= IXG_FILM_REGEN_MIN || active_mask == 0ull)) {
             if (pending) {
                 float xyz[3];
-                path_finish(P, S, pending_escaped, radiance, xyz);
+                path_finish(P, S, pending_escaped, xyz);
                 out_xyz[3 * (size_t)idx + 0] = xyz[0];
                 out_xyz[3 * (size_t)idx + 1] = xyz[1];
                 out_xyz[3 * (size_t)idx + 2] = xyz[2];
@@ -1431,8 +1438,6 @@ __global__ __launch_bounds__(kBlock) IXG_WAVES_ATTR void film_persistent_kernel(
                     camera_sample(P, (uint32_t)(p / P.spp), P.sample_offset + (uint32_t)(p % P.spp),
                                   org, dir, seed, hero);
                     path_init<ABSORB>(P, org, dir, seed, hero, S);
-#pragma unroll
-                    for (int c = 0; c < NCH; ++c) radiance[c] = 0.0f;
                     active = true;
                 } else {
                     done = true;
@@ -1442,7 +1447,7 @@ __global__ __launch_bounds__(kBlock) IXG_WAVES_ATTR void film_persistent_kernel(
         if (__ballot(active) == 0ull) break;
         if (active) {
             int r = STEP_CONTINUE;
-            if (S.bounce < P.max_bounces) r = path_bounce<false, ISO, ABSORB, ANISO, EXTRA>(P, sp, S, radiance);
+            if (S.bounce < P.max_bounces) r = path_bounce<false, ISO, ABSORB, ANISO, EXTRA>(P, sp, S);
             if (r != STEP_CONTINUE || ++S.bounce >= P.max_bounces) {
                 active = false;
                 pending = true;
