@@ -30,10 +30,13 @@ MI355X_BF16_PEAK_TFLOPS = 2500.0   # dense BF16 MFMA (the surrogate GEMMs)
 MI355X_HBM_PEAK_GBS = 8000.0
 LIBRARY_CHUNK_PATHS = 1 << 23      # paths per film launch (the library's default chunk)
 HOST_CPU_SHARE = 16                # host threads one GPU's share of the box allows
-# HBM bytes per film_persistent_kernel launch at this workload: FETCH_SIZE x 2.0
-# + WRITE_SIZE x 1.0 (scales from a 1 GiB copy) from two rocprofv3 --pmc runs,
-# recorded with their commands in profiles/film_persistent_kernel_c1.md.
-FILM_KERNEL_HBM_BYTES = (5333.05 * 2.0 + 29044.98) * 1024   # PMC FETCH_SIZE x 2 (calibrated) + WRITE_SIZE, per launch
+# HBM bytes per launch of each benchmark's dominant kernel: FETCH_SIZE x 2.0 +
+# WRITE_SIZE x 1.0 in KiB (the scales a 1 GiB device copy reads), from
+# rocprofv3 --pmc runs recorded with their commands under profiles/.
+FILM_KERNEL_HBM_BYTES = (618.0 * 2.0 + 13634.2) * 1024           # film C1, 1 048 576 paths
+FILM_C2_KERNEL_HBM_BYTES = (3077492.1 * 2.0 + 10327946.6) * 1024  # film C2, 2^23 paths (register spills)
+SWEEP_KERNEL_HBM_BYTES = (10527.94 * 2.0 + 2603463.28) * 1024     # the 512-design sweep launch
+MLP_STEP_HBM_BYTES = (53473.5 * 2.0 + 56344.8) * 1024            # one training step's four launches
 
 
 def parse():
@@ -132,7 +135,8 @@ def film_c2_rate(backend, steps):
         raise RuntimeError("flop_counts.json film_c2 does not describe this frame")
     totals = backend.count_events(sc, entry["sample_offset"], entry["spp"])
     px_per_launch = min(max(1, LIBRARY_CHUNK_PATHS // spp), sc.width * sc.height)
-    roof = film_roofline(entry, totals, sum(launch_ms) / len(launch_ms), px_per_launch * spp)
+    roof = film_roofline(entry, totals, sum(launch_ms) / len(launch_ms), px_per_launch * spp,
+                         round(FILM_C2_KERNEL_HBM_BYTES))
     return {"workload": "film 1024x1024 @ 256 spp, depth 24, 121-facet seeded oval (tier angles U[30,50]), "
                         "uniaxial n_e - n_o = 0.009, pleochroic Gaussian o/e bands, random optic axis",
             "steps": steps, "ms_per_step": round(dt * 1e3 / steps, 2),
@@ -162,7 +166,8 @@ def sweep_rate(backend, n_designs=512):
     from indicatrix_amd import designs, flops, metrics, tracer
     planes, ds, _ = designs.sweep_batch(n_designs)
     env = tracer.LightingPreset.studio("IsoHemisphere", 1.0, 0.0, 0.0)
-    metrics.metric_sweep(backend, planes, ds[:16], designs.TILTS_C3, 64, 64, 32, 2.4, env)  # warm + autotune
+    # warm-up and autotune on a call of the same size class (at least 8 pixels per resident lane)
+    metrics.metric_sweep(backend, planes, ds[:64], designs.TILTS_C3, 64, 64, 32, 2.4, env)
     t0 = time.perf_counter()
     out = metrics.metric_sweep(backend, planes, ds, designs.TILTS_C3, 64, 64, 32, 2.4, env)
     dt = time.perf_counter() - t0
@@ -179,7 +184,7 @@ def sweep_rate(backend, n_designs=512):
         result["roofline"] = {
             "bound": "valu", "pipe": "fp32 vector ALU", "achieved": round(achieved, 3),
             "peak": MI355X_FP32_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": round(achieved / MI355X_FP32_PEAK_TFLOPS, 4),
-            "traffic": None, "kernel": (f"metric_queue_kernel<{block},{tile}>" if block > 0 else
+            "traffic": round(SWEEP_KERNEL_HBM_BYTES), "kernel": (f"metric_queue_kernel<{block},{tile}>" if block > 0 else
                                         f"metric_flow_kernel<{-block}> ({tile} px per item)" if block < -1 else
                                         "metric_pixels_kernel"),
             "kernel_ms": round(dev_ms, 3), "flops_per_launch": fl,
@@ -292,7 +297,7 @@ def mlp_train_rate(backend, steps, batch=4096):
             "samples_per_s": round(batch / (ms * 1e-3)), "final_loss": round(float(loss), 5),
             "roofline": {"bound": "mfma", "pipe": "bf16 dense MFMA", "achieved": round(achieved, 3),
                          "peak": MI355X_BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
-                         "frac": round(achieved / MI355X_BF16_PEAK_TFLOPS, 4), "traffic": None,
+                         "frac": round(achieved / MI355X_BF16_PEAK_TFLOPS, 4), "traffic": round(MLP_STEP_HBM_BYTES),
                          "kernel": "mlp_forward/backward/dw/reduce (one training step)",
                          "kernel_ms": round(ms, 4), "flops_per_launch": fl,
                          "note": "flops = 2 M N K of the forward, backward-data and weight-gradient GEMMs "

@@ -25,6 +25,7 @@
 //   XYZ families            optics/raytracer/color.rs:145-172
 //   per-pixel reduction     renderer/shaders/reduce_xyz.wgsl:55-90, optics/raytracer/sampling.rs:237
 #pragma once
+#include <algorithm>
 #include "device_common.h"
 #include "uniaxial.h"
 
@@ -1522,27 +1523,42 @@ __global__ __launch_bounds__(kBlock) void film_rays_kernel(const FilmParams P,
     if (ev) ev[i] = o.ev;
 }
 
+// Resident blocks per CU of the lean isotropic persistent kernel (film_lean_iso.hip).
+extern "C" int ixg_lean_blocks_per_cu(void);
+
+// Launch one instantiation of the persistent kernel with as many blocks as
+// stay resident for it.  grid_blocks is the residency of the lean isotropic
+// instantiation (3 blocks per CU) times the CUs; the uniaxial ones fit 2 per
+// CU, and blocks beyond residency would only start once the path counter is
+// spent.  Residency per CU is a property of the compiled kernel, read once.
+template <bool ISO, bool ABSORB, bool ANISO, bool EXTRA>
+hipError_t launch_persistent(const FilmParams* P, const float4* d_planes, float* d_xyz, uint32_t* d_counter,
+                             uint32_t grid_blocks, hipStream_t stream) {
+    static const int per_cu = [] {
+        int n = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, film_persistent_kernel<ISO, ABSORB, ANISO, EXTRA>,
+                                                         kBlock, 0) != hipSuccess)
+            n = 0;
+        const int lean = ixg_lean_blocks_per_cu();
+        return (n > 0 && lean > 0) ? n * 1024 / lean : 1024;  // this kernel's residency relative to the lean one, x1024
+    }();
+    const uint32_t grid = (uint32_t)std::max<uint64_t>(1, (uint64_t)grid_blocks * (uint32_t)per_cu / 1024u);
+    hipLaunchKernelGGL((film_persistent_kernel<ISO, ABSORB, ANISO, EXTRA>), dim3(grid), dim3(kBlock), 0, stream, *P,
+                       d_planes, d_xyz, d_counter);
+    return hipGetLastError();
+}
+
 // The persistent kernel's instantiations for one (uniaxial, extra-features)
 // pair, chosen at launch by the environment class and the absorbing flag.
 template <bool ANISO, bool EXTRA>
 hipError_t launch_persistent_subset(const FilmParams* P, const float4* d_planes, float* d_xyz, uint32_t* d_counter,
                                     uint32_t grid_blocks, hipStream_t stream, bool iso) {
     if (iso) {
-        if (P->absorb)
-            hipLaunchKernelGGL((film_persistent_kernel<true, true, ANISO, EXTRA>), dim3(grid_blocks), dim3(kBlock), 0,
-                               stream, *P, d_planes, d_xyz, d_counter);
-        else
-            hipLaunchKernelGGL((film_persistent_kernel<true, false, ANISO, EXTRA>), dim3(grid_blocks), dim3(kBlock), 0,
-                               stream, *P, d_planes, d_xyz, d_counter);
-    } else {
-        if (P->absorb)
-            hipLaunchKernelGGL((film_persistent_kernel<false, true, ANISO, EXTRA>), dim3(grid_blocks), dim3(kBlock), 0,
-                               stream, *P, d_planes, d_xyz, d_counter);
-        else
-            hipLaunchKernelGGL((film_persistent_kernel<false, false, ANISO, EXTRA>), dim3(grid_blocks), dim3(kBlock), 0,
-                               stream, *P, d_planes, d_xyz, d_counter);
+        if (P->absorb) return launch_persistent<true, true, ANISO, EXTRA>(P, d_planes, d_xyz, d_counter, grid_blocks, stream);
+        return launch_persistent<true, false, ANISO, EXTRA>(P, d_planes, d_xyz, d_counter, grid_blocks, stream);
     }
-    return hipGetLastError();
+    if (P->absorb) return launch_persistent<false, true, ANISO, EXTRA>(P, d_planes, d_xyz, d_counter, grid_blocks, stream);
+    return launch_persistent<false, false, ANISO, EXTRA>(P, d_planes, d_xyz, d_counter, grid_blocks, stream);
 }
 
 // The diagnostic kernels (per-path events, explicit rays) for one material class.

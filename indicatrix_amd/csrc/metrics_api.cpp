@@ -230,7 +230,7 @@ struct SweepPlan {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     std::vector<uint8_t> graph_key;
-    std::map<std::array<uint32_t, 3>, Shape> tuned;
+    std::map<std::array<uint32_t, 4>, Shape> tuned;
     Shape last;
     int last_graphed = 0;
     std::vector<float> tune_ms;  // per candidate, last autotune
@@ -309,21 +309,26 @@ hipError_t enqueue_sweep(SweepPlan* P, const MetricShared& S, size_t n_images, c
 }
 
 // Times the candidate shapes at full residency on the first images of this
-// call -- about 2^26 sub-pixel rays' worth, at least 16 images -- and keeps
-// the fastest for this image size: the flow kernel at 128 and 256 lanes per
-// block (items sized per call) and three work-queue shapes.  (The flow
-// kernel at 64 lanes wins only on small tuning samples cut into many items;
-// on whole images it ran 20-25 % behind.)  A one-image
-// pass over every candidate first loads each kernel's code, so the timed
-// passes measure only the work.
+// call and keeps the fastest for this image size and call-size class: the
+// flow kernel at 128 and 256 lanes per block (items sized per call) and
+// three work-queue shapes.  The sample is about 2^26 sub-pixel rays' worth,
+// at least 16 images, and enough images to give every resident flow lane 16
+// pixels (or the whole call), so the flow kernel's items are as large as in
+// a full call and its measurement is not dominated by the end of its items.
+// A one-image pass over every candidate first loads each kernel's code, so
+// the timed passes measure only the work.  (The flow kernel at 64 lanes
+// wins only on samples cut into many small items; on whole images it ran
+// 20-25 % behind, so it is not a candidate.)
 hipError_t autotune(SweepPlan* P, const MetricShared& S, size_t n_images, hipStream_t st, Shape* best) {
-    const size_t rays_per_image = std::max<size_t>(1, (size_t)S.width * S.height * S.samples);
-    const size_t n = std::min<size_t>(n_images, std::max<size_t>(16, ((size_t)1 << 26) / rays_per_image));
+    const size_t npx = std::max<size_t>(1, (size_t)S.width * S.height);
+    const size_t rays_per_image = npx * std::max<uint32_t>(S.samples, 1);
     std::vector<Shape> cands;
+    size_t flow_lanes = 0;
     for (int lanes : {128, 256}) {
         int per_cu = 0;
         if (ixg_metric_flow_occupancy(lanes, &per_cu) != hipSuccess || per_cu <= 0) continue;
         cands.push_back(Shape{-lanes, 0, (uint32_t)(per_cu * P->cus)});
+        flow_lanes = std::max<size_t>(flow_lanes, (size_t)per_cu * P->cus * lanes);
     }
     const int queue[][2] = {{64, 64}, {128, 128}, {256, 1024}};
     for (const auto& q : queue) {
@@ -331,6 +336,8 @@ hipError_t autotune(SweepPlan* P, const MetricShared& S, size_t n_images, hipStr
         if (ixg_metric_queue_occupancy(q[0], q[1], &per_cu) != hipSuccess || per_cu <= 0) continue;
         cands.push_back(Shape{q[0], q[1], (uint32_t)(per_cu * P->cus)});
     }
+    const size_t n = std::min<size_t>(
+        n_images, std::max<size_t>({16, ((size_t)1 << 26) / rays_per_image, (16 * flow_lanes + npx - 1) / npx}));
     hipEvent_t a = nullptr, b = nullptr;
     hipError_t e = hipEventCreate(&a);
     if (e == hipSuccess) e = hipEventCreate(&b);
@@ -384,7 +391,12 @@ ixg_status run_sweep(ixg_ctx* ctx, const MetricShared& S, const std::vector<Metr
                 sh.grid = (uint32_t)std::max(1, per_cu) * (uint32_t)P->cus;
             }
         } else {
-            const std::array<uint32_t, 3> key{S.width, S.height, S.samples};
+            // the image size, and whether the call is large enough for the flow kernel alone
+            int per_cu = 0;
+            const size_t lanes = ixg_metric_flow_occupancy(128, &per_cu) == hipSuccess
+                                     ? (size_t)std::max(per_cu, 1) * P->cus * 128 : 1;
+            const uint32_t large = n * S.width * S.height >= 8 * lanes ? 1u : 0u;
+            const std::array<uint32_t, 4> key{S.width, S.height, S.samples, large};
             auto it = P->tuned.find(key);
             if (it != P->tuned.end()) {
                 sh = it->second;
