@@ -8,6 +8,7 @@
 #include <stdint.h>
 
 #include "../../include/ixg.h"
+#include "exp_ff.h"
 #include "transport_params.h"
 
 #define IX_TABLE_QUAL __constant__ static const
@@ -202,11 +203,34 @@ __device__ __forceinline__ float d65(float lambda) {
     return div_rn(fma_(frac, v1 - v0, v0), 100.0f);
 }
 
-// exp rounded once from double precision: the f32 nearest e^x, as glibc's
-// expf gives for the arguments here (the device library's f32 expf is up to
-// one ULP away).  Used where the value feeds every channel of a path: the
-// Planck illuminant and the absorption band sums.
-__device__ __forceinline__ float exp_rn(float x) { return (float)exp((double)x); }
+// e^x correctly rounded to f32 (to within the rare ties exp_ff.h states),
+// in single precision: what glibc's expf returns for the arguments here (the
+// device library's f32 expf is up to one ULP away).  Used where the value
+// feeds every channel of a path: the Planck illuminant and the absorption
+// band sums.
+__device__ __forceinline__ float exp_rn(float x) { return ixg_exp::exp_ff(x); }
+
+// fmodf for finite x and finite y != 0, exactly (the remainder is always an
+// f32), in single precision: the largest y 2^k not above the running
+// remainder is subtracted until it drops below |y|; each subtraction is
+// exact (Sterbenz: the operands lie within a factor 2).  Infinite or NaN x
+// gives NaN.
+__device__ __forceinline__ float fmod_exact(float x, float y) {
+    if (!(fabsf(x) < __builtin_inff())) return x - x;
+    const float ay = fabsf(y);
+    float r = fabsf(x);
+    while (r >= ay) {
+        float t = ay;
+        while (t <= 0.5f * r) t *= 2.0f;
+        r -= t;
+    }
+    return __builtin_copysignf(r, x);
+}
+
+// f32::midpoint: the f32 nearest (a + b) / 2.  For operands whose sum is
+// neither subnormal nor beyond the f32 range this is 0.5 * (a + b): halving
+// is exact there, and the f32 sum of two f32 values rounds as the exact sum.
+__device__ __forceinline__ float midpoint(float a, float b) { return 0.5f * (a + b); }
 
 __device__ __forceinline__ float spec_power(const EnvParams& E, float lambda) {
     if (E.d65) return d65(lambda);
@@ -269,7 +293,7 @@ __device__ __forceinline__ float lit_factor_unit(const EnvParams& E, V3 d, V3 ob
 // equirectangular (u, v), u wrapping and v clamped, bilinear over the four
 // neighbouring texels; the angles correctly rounded.
 __device__ __forceinline__ float rem_euclid1(float x, float m) {
-    const float r = fmodf(x, m);
+    const float r = fmod_exact(x, m);
     return r < 0.0f ? r + fabsf(m) : r;
 }
 __device__ __forceinline__ void hdr_rgb(const EnvParams& E, V3 dir, float out[3]) {

@@ -44,8 +44,6 @@ __device__ __forceinline__ float4 mueller_tir(float cd, float sd, float4 s) {
 __device__ __forceinline__ float4 scale4(float4 s, float f) {
     return make_float4(s.x * f, s.y * f, s.z * f, s.w * f);
 }
-// f32::midpoint for non-subnormal finite operands: round((a+b)/2) == 0.5*(a+b)
-__device__ __forceinline__ float midpoint(float a, float b) { return 0.5f * (a + b); }
 
 __device__ __forceinline__ void fresnel_reflection_coeffs(float rs, float rp, float& a, float& b,
                                                           float& c) {
@@ -201,8 +199,8 @@ __device__ __forceinline__ void hero_comb(float hero, float lambda[NCH]) {
         float off = fma_((float)c, 50.0f, lambda_hero - 380.0f);
         // rem_euclid(400): for 0 <= off < 800 this is off or off - 400, both
         // exact (Sterbenz), which is what fmodf returns; a hero draw outside
-        // [0, 1) (explicit rays) takes the library remainder
-        float r = off >= 0.0f && off < 800.0f ? (off >= 400.0f ? off - 400.0f : off) : fmodf(off, 400.0f);
+        // [0, 1) (explicit rays) takes the exact remainder loop
+        float r = off >= 0.0f && off < 800.0f ? (off >= 400.0f ? off - 400.0f : off) : fmod_exact(off, 400.0f);
         r = r < 0.0f ? r + 400.0f : r;
         lambda[c] = 380.0f + r;
     }
@@ -212,7 +210,7 @@ __device__ __forceinline__ void hero_comb(float hero, float lambda[NCH]) {
 __device__ __forceinline__ float comb_lambda(float hero, int c) {
     const float lambda_hero = fma_(hero, 400.0f, 380.0f);
     float off = fma_((float)c, 50.0f, lambda_hero - 380.0f);
-    float r = off >= 0.0f && off < 800.0f ? (off >= 400.0f ? off - 400.0f : off) : fmodf(off, 400.0f);
+    float r = off >= 0.0f && off < 800.0f ? (off >= 400.0f ? off - 400.0f : off) : fmod_exact(off, 400.0f);
     r = r < 0.0f ? r + 400.0f : r;
     return 380.0f + r;
 }
@@ -272,9 +270,13 @@ __device__ __forceinline__ void path_init(const FilmParams& P, V3 org, V3 dir, u
     path_optics<ABSORB>(P, lambda, S.n_o, S.alpha_o, S.alpha_e);
 }
 
+// The environment radiance seen along dir.  The HDR panorama is compiled only
+// into the EXTRA instantiations (its angles are correctly rounded through
+// double precision), which the launch chooses for a panorama.
+template <bool EXTRA>
 __device__ __forceinline__ float env_radiance(const EnvParams& E, V3 dir, V3 observer, float lambda) {
     if (E.model == ENV_ASET) return aset_radiance(E, normalize(dir), lambda, E.exposure);
-    if (E.model == ENV_HDR) {
+    if (EXTRA && E.model == ENV_HDR) {
         float rgb[3];
         hdr_rgb(E, dir, rgb);
         return rgb_to_spectral(rgb, lambda);
@@ -331,7 +333,7 @@ __device__ __forceinline__ void channel_alphas(const AbsorbParams& A, const Path
     for (int ch = 0; ch < NCH; ++ch) {
         const float qa = tensor_qf(A, S.alpha_o[ch], S.alpha_e[ch], ea);
         const float qb = tensor_qf(A, S.alpha_o[ch], S.alpha_e[ch], eb);
-        alpha[ch] = (float)(((double)qa + (double)qb) * 0.5);
+        alpha[ch] = midpoint(qa, qb);
     }
 }
 
@@ -352,7 +354,7 @@ __device__ __forceinline__ void apply_absorption(const AbsorbParams& A, PathStat
     for (int ch = 0; ch < NCH; ++ch) {
         const float qa = tensor_qf(A, S.alpha_o[ch], S.alpha_e[ch], ea);
         const float qb = tensor_qf(A, S.alpha_o[ch], S.alpha_e[ch], eb);
-        const float alpha = (float)(((double)qa + (double)qb) * 0.5);
+        const float alpha = midpoint(qa, qb);
         S.st[ch] = scale4(S.st[ch], exp_lane(-alpha * scaled));
     }
 }
@@ -384,7 +386,7 @@ __device__ __forceinline__ bool eigenmode_selection(V3 c, V3 cpn, V3 k, float4 h
 }
 
 // try_split_exit_channel (exit_split.rs:128-163) for channel c leaving along dk.
-template <bool EVENTS, bool ISO>
+template <bool EVENTS, bool ISO, bool EXTRA>
 __device__ __forceinline__ void split_probe(const FilmParams& P, const float4* __restrict__ planes, PathState& S,
                                             int facet, V3 hit, int c, V3 dk, float transmitted_i) {
     if (EVENTS) ++S.n_probe;
@@ -397,7 +399,7 @@ __device__ __forceinline__ void split_probe(const FilmParams& P, const float4* _
     }
     if (!escapes) return;
     const float env = ISO ? lit_factor(P.env, dk, S.observer) * (spec_power(P.env, comb_lambda(S.hero, c)) * P.env.exposure)
-                          : env_radiance(P.env, dk, S.observer, comb_lambda(S.hero, c));
+                          : env_radiance<EXTRA>(P.env, dk, S.observer, comb_lambda(S.hero, c));
     S.split[c] = fma_(fmaxf(transmitted_i, 0.0f) * 1.0f, env, S.split[c]);
 }
 
@@ -736,7 +738,7 @@ __device__ __forceinline__ int facet_aniso(const FilmParams& P, const float4* __
                     const float prefix = S.path_pdf[c];
                     S.st[c] = scale4(S.st[c], 0.0f);
                     S.path_pdf[c] = prefix * t_unpol;
-                    if (orig_i > 0.0f) split_probe<EVENTS, ISO>(P, planes, S, facet, hit, c, rk, fmaxf(tr.x, 0.0f));
+                    if (orig_i > 0.0f) split_probe<EVENTS, ISO, EXTRA>(P, planes, S, facet, hit, c, rk, fmaxf(tr.x, 0.0f));
                     continue;
                 }
                 S.st[c] = tr;
@@ -855,7 +857,7 @@ __device__ __forceinline__ int facet_aniso(const FilmParams& P, const float4* __
                     S.st[c] = scale4(S.st[c], 0.0f);
                     S.path_pdf[c] = prefix * (1.0f - ruk);
                     if (inside && fmaxf(orig.x, 0.0f) > 0.0f)
-                        split_probe<EVENTS, ISO>(P, planes, S, facet, hit, c, rk, fmaxf(tr.x, 0.0f));
+                        split_probe<EVENTS, ISO, EXTRA>(P, planes, S, facet, hit, c, rk, fmaxf(tr.x, 0.0f));
                 }
                 if (!inside) channel_dir_put(c, fk);
                 live |= 1u << c;
@@ -1020,7 +1022,7 @@ __device__ __forceinline__ int path_bounce(const FilmParams& P, const float4* __
     }
     if (facet < 0) {
         // handle_miss (bounce_body/miss.rs:33-54)
-        if (!ISO && P.env.model == ENV_HDR) {  // the panorama: one texel, per-channel spectrum
+        if (EXTRA && !ISO && P.env.model == ENV_HDR) {  // the panorama: one texel, per-channel spectrum
             float rgb[3];
             hdr_rgb(P.env, S.dir, rgb);
 #pragma unroll
@@ -1235,7 +1237,7 @@ __device__ __forceinline__ int path_bounce(const FilmParams& P, const float4* __
                     if (escapes) {
                         float env = ISO ? lit_factor(P.env, dk, S.observer) *
                                               (spec_power(P.env, comb_lambda(S.hero, c)) * P.env.exposure)
-                                        : env_radiance(P.env, dk, S.observer, comb_lambda(S.hero, c));
+                                        : env_radiance<EXTRA>(P.env, dk, S.observer, comb_lambda(S.hero, c));
                         S.split[c] = fma_(fmaxf(fmaxf(tr.x, 0.0f), 0.0f) * 1.0f, env, S.split[c]);
                     }
                 }

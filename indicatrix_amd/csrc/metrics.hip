@@ -214,24 +214,37 @@ __device__ __forceinline__ void cam_ray(const float* fwd, const float* right, co
                     v3(up[0], up[1], up[2]) * v);
 }
 
-// image.rs:21-36: sample 0 is the pixel centre, then base-2 / base-3 radical
-// inverses evaluated in double precision.
+// image.rs:21-36: sample 0 is the pixel centre, then the base-2 and base-3
+// radical inverses evaluated in double precision and rounded to f32.  A
+// request has at most 64 samples, so the offsets are a table the compiler
+// evaluates (its constant evaluation rounds as IEEE double arithmetic does):
+// the kernels read f32 values and compute nothing in double.
+struct SampleOffsets {
+    float jx[64], jy[64];
+    constexpr SampleOffsets() : jx(), jy() {
+        jx[0] = 0.5f;
+        jy[0] = 0.5f;
+        for (uint32_t s = 1; s < 64; ++s) {
+            uint32_t rev = 0;
+            for (int b = 0; b < 32; ++b) rev |= ((s >> b) & 1u) << (31 - b);
+            jx[s] = (float)((double)rev / 4294967296.0);
+            uint32_t n = s;
+            double scale = 1.0 / 3.0, y = 0.0;
+            while (n > 0) {
+                y += (double)(n % 3) * scale;
+                n /= 3;
+                scale /= 3.0;
+            }
+            jy[s] = (float)y;
+        }
+    }
+};
+__constant__ const SampleOffsets kSampleOffsets{};
+
 __device__ __forceinline__ void sample_offset(uint32_t s, float& jx, float& jy) {
-    if (s == 0) {
-        jx = 0.5f;
-        jy = 0.5f;
-        return;
-    }
-    double radical = (double)__builtin_bitreverse32(s) / 4294967296.0;
-    uint32_t n = s;
-    double scale = 1.0 / 3.0, y = 0.0;
-    while (n > 0) {
-        y += (double)(n % 3) * scale;
-        n /= 3;
-        scale /= 3.0;
-    }
-    jx = (float)radical;
-    jy = (float)y;
+    const uint32_t i = s < 64u ? s : 63u;  // requests hold at most 64 samples
+    jx = kSampleOffsets.jx[i];
+    jy = kSampleOffsets.jy[i];
 }
 
 // One pixel p of image I (sample_pixel, image.rs:110-175, plus the temporal

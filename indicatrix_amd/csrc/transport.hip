@@ -82,7 +82,8 @@ extern "C" hipError_t ixg_launch_film_persistent(const ixg::FilmParams* P, const
     hipLaunchKernelGGL(ixg::zero_counter_kernel, dim3(1), dim3(1), 0, stream, d_counter);
     // the lit models: one scaled factor per direction (ASET is per wavelength)
     const int iso = P->env.model != ixg::ENV_STUDIO && P->env.model != ixg::ENV_ASET && P->env.model != ixg::ENV_HDR;
-    const bool extra = P->frosted || P->sigma_s > 0.0f || P->edge_radius > 0.0f;
+    // the rarely used features, the HDR panorama among them, run the EXTRA kernels
+    const bool extra = P->frosted || P->sigma_s > 0.0f || P->edge_radius > 0.0f || P->env.model == ixg::ENV_HDR;
     if (extra)
         return P->an.on ? ixg_launch_persistent_extra_aniso(P, d_planes, d_xyz, d_counter, grid_blocks, stream, iso)
                         : ixg_launch_persistent_extra_iso(P, d_planes, d_xyz, d_counter, grid_blocks, stream, iso);

@@ -331,7 +331,7 @@ __device__ __forceinline__ void jones_to_mueller(Cx j_ss, Cx j_sp, Cx j_ps, Cx j
     M.m[0][2] = b_sp.re + b_pp.re;
     M.m[0][3] = -(b_sp.im + b_pp.im);
     M.m[1][0] = 0.5f * (((m_ss + m_ps) - m_sp) - m_pp);
-    M.m[1][1] = (float)(((double)((m_ss - m_ps) - m_sp) + (double)m_pp) * 0.5);
+    M.m[1][1] = midpoint((m_ss - m_ps) - m_sp, m_pp);
     M.m[1][2] = b_sp.re - b_pp.re;
     M.m[1][3] = -(b_sp.im) + b_pp.im;
     M.m[2][0] = a_plus.re;
