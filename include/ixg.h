@@ -282,6 +282,75 @@ double ixg_last_kernel_ms(const ixg_ctx* ctx);
 void* ixg_stream(ixg_ctx* ctx);
 uint32_t ixg_last_kernel_launches(const ixg_ctx* ctx);
 
+/* ---- per-function probes ----
+ * The reference's per-function GPU checks (renderer/gpu/ulp.rs with the
+ * *_check.rs modules under renderer/gpu/ and renderer/gpu/transport_check/;
+ * labels and budgets in crates/indicatrix/docs/gpu_harness_last_run.json:
+ * 24-73): ixg_probe evaluates the device function `fn` on n input records
+ * and writes n output records, with the scene supplying the camera,
+ * material, environment and facet set the function reads (the scene must
+ * route; the material's absorption bands are read only when it has some).
+ * Diagnostic: the film and metric paths never call it.  Records are floats;
+ * u32 / i32 fields carry their bits.  in -> out per record:
+ *   SAMPLE_DRAWS      pixel u32, sample u32 -> jx, jy, hero, seed u32, lambda[8]
+ *   CAMERA_RAY        pixel u32, sample u32 -> origin[3], dir[3]
+ *   CIE_CMF           lambda -> x, y, z
+ *   ILLUMINANT        lambda -> the environment's illuminant power
+ *   ENVIRONMENT       dir[3], observer[3], lambda -> radiance (exposure applied)
+ *   FRAME_ROTATION    cos_psi, sin_psi, stokes[4] -> stokes[4]
+ *   FRESNEL_REFLECTION n1, n2, cos_i, cos_t, stokes[4] -> stokes[4], clamped R_unpol
+ *   FRESNEL_TRANSMISSION n1, n2, cos_i, cos_t, stokes[4] -> stokes[4], r_s, r_p
+ *   TIR_RETARDATION   n1, cos_i, sin_i, stokes[4] -> stokes[4]
+ *   CHANNEL_OPTICS    lambda[8] -> n_o[8], alpha_o[8], alpha_e[8]
+ *   PLEOCHROIC_ALPHA  k[3], alpha_o[8], alpha_e[8] -> alpha[8]
+ *   ORDINARY_EIGEN    k[3], c[3] -> o[3]
+ *   EXTRAORDINARY_POYNTING w[3], c[3], n_o, n_e -> s[3]
+ *   THETA_C           lambda_hero, n_o, k[3], normal[3], inside -> theta
+ *   UNIAXIAL_INDICES  lambda, n_o, theta -> n_e, n_eff
+ *   ENTRY_SOLVE       n_o, n_e, c[3], k[3], normal[3] -> {r_s, r_p, t_o, t_e} (re, im)
+ *                     for s then p incidence, flux_o, flux_e
+ *   INTERNAL_SOLVE    n_inc, n_o, n_e, c[3], k[3], normal[3], ordinary ->
+ *                     r_o, r_e, t_s, t_p (re, im), flux_ro, flux_re, flux_ts, flux_tp, flux_inc
+ *   EXIT_TRANSMISSION INTERNAL_SOLVE's 13, inc_i, r_branch -> stokes[4], i_unit
+ *   HG_DIRECTION      u1, u2, g, fwd[3] -> dir[3]
+ *   COSINE_HEMISPHERE u1, u2, n[3] -> dir[3]
+ *   SHADING_NORMAL    hit[3], facet i32, normal[3], radius -> normal[3]
+ *   FINISH            compat lo u32, compat hi u32, pdf[8], lambda[8], radiance[8] -> xyz[3]
+ *   INTERSECT         origin[3], dir[3] -> facet i32 (-1 = miss), t
+ *   EXP_LANE          x -> exp_f32x8's lane value
+ *   EXP               x -> e^x correctly rounded */
+typedef enum ixg_probe_fn {
+    IXG_PROBE_SAMPLE_DRAWS = 0,
+    IXG_PROBE_CAMERA_RAY,
+    IXG_PROBE_CIE_CMF,
+    IXG_PROBE_ILLUMINANT,
+    IXG_PROBE_ENVIRONMENT,
+    IXG_PROBE_FRAME_ROTATION,
+    IXG_PROBE_FRESNEL_REFLECTION,
+    IXG_PROBE_FRESNEL_TRANSMISSION,
+    IXG_PROBE_TIR_RETARDATION,
+    IXG_PROBE_CHANNEL_OPTICS,
+    IXG_PROBE_PLEOCHROIC_ALPHA,
+    IXG_PROBE_ORDINARY_EIGEN,
+    IXG_PROBE_EXTRAORDINARY_POYNTING,
+    IXG_PROBE_THETA_C,
+    IXG_PROBE_UNIAXIAL_INDICES,
+    IXG_PROBE_ENTRY_SOLVE,
+    IXG_PROBE_INTERNAL_SOLVE,
+    IXG_PROBE_EXIT_TRANSMISSION,
+    IXG_PROBE_HG_DIRECTION,
+    IXG_PROBE_COSINE_HEMISPHERE,
+    IXG_PROBE_SHADING_NORMAL,
+    IXG_PROBE_FINISH,
+    IXG_PROBE_INTERSECT,
+    IXG_PROBE_EXP_LANE,
+    IXG_PROBE_EXP,
+    IXG_PROBE_COUNT
+} ixg_probe_fn;
+/* Floats per input and output record of fn; 0 for an unknown id. */
+int ixg_probe_shape(int32_t fn, uint32_t* in_floats, uint32_t* out_floats);
+ixg_status ixg_probe(ixg_ctx* ctx, const ixg_scene* scene, int32_t fn, const float* in, uint32_t n, float* out);
+
 /* ---- metrics ----
  * OpticalPixel (IX/color/metrics/image_types.rs:59-75): per-pixel class
  * counts {windowed, extinct, returned, entry-blocked/invalid} over the

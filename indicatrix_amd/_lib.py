@@ -48,6 +48,8 @@ def _declare(lib):
         "ixg_trace_rays": (st, [C.c_void_p, P(abi.Scene), P(C.c_float), C.c_uint32,
                                 P(C.c_float), P(abi.PathEvents)]),
         "ixg_stream": (C.c_void_p, [C.c_void_p]),
+        "ixg_probe_shape": (C.c_int, [C.c_int32, P(C.c_uint32), P(C.c_uint32)]),
+        "ixg_probe": (st, [C.c_void_p, P(abi.Scene), C.c_int32, P(C.c_float), C.c_uint32, P(C.c_float)]),
         "ixg_count_events": (st, [C.c_void_p, P(abi.Scene), C.c_uint32, C.c_uint32,
                                   P(C.c_uint64)]),
         "ixg_metric_image": (st, [C.c_void_p, P(abi.MetricRequest), P(abi.OpticalPixel),

@@ -13,6 +13,14 @@ ROUTE_OK, ROUTE_UNSUPPORTED_MATERIAL, ROUTE_UNSUPPORTED_ENVIRONMENT, ROUTE_DEVIC
     ROUTE_NO_DEVICE = range(5)
 
 MAX_PLANES = 256
+
+# ixg_probe_fn (include/ixg.h): per-function probe ids, in order.
+PROBE_FNS = ("sample_draws", "camera_ray", "cie_cmf", "illuminant", "environment", "frame_rotation",
+             "fresnel_reflection", "fresnel_transmission", "tir_retardation", "channel_optics",
+             "pleochroic_alpha", "ordinary_eigen", "extraordinary_poynting", "theta_c", "uniaxial_indices",
+             "entry_solve", "internal_solve", "exit_transmission", "hg_direction", "cosine_hemisphere",
+             "shading_normal", "finish", "intersect", "exp_lane", "exp")
+PROBE = {name: i for i, name in enumerate(PROBE_FNS)}
 NOALLOC_MAX_PATHS = 1 << 20
 NOALLOC_MAX_HDR_TEXELS = 1024 * 512
 MAX_BANDS = 8

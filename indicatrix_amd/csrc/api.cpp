@@ -29,6 +29,8 @@ extern "C" hipError_t ixg_launch_film_events(const ixg::FilmParams* P, const flo
 extern "C" hipError_t ixg_launch_rays(const ixg::FilmParams* P, const float4* d_planes, const float* d_rays,
                                       uint32_t n_rays, float* d_xyz, ixg_path_events* d_ev,
                                       hipStream_t stream);
+extern "C" hipError_t ixg_launch_probe(const ixg::FilmParams* P, const float4* d_planes, int32_t fn,
+                                       const float* d_in, uint32_t n, float* d_out, hipStream_t stream);
 extern "C" hipError_t ixg_launch_reduce(const float* d_xyz, uint32_t spp, uint32_t n_pixels,
                                         uint32_t first_pixel, double* d_accum, hipStream_t stream);
 
@@ -677,6 +679,32 @@ ixg_status ixg_trace_rays(ixg_ctx* ctx, const ixg_scene* s, const float* rays, u
     if (d_xyz) (void)hipFree(d_xyz);
     if (d_ev) (void)hipFree(d_ev);
     if (e != hipSuccess) return hip_fail(ctx, e, "trace_rays");
+    return IXG_OK;
+}
+
+ixg_status ixg_probe(ixg_ctx* ctx, const ixg_scene* s, int32_t fn, const float* in, uint32_t n, float* out) {
+    ixg_status st = begin_call(ctx, s);
+    if (st != IXG_OK) return st;
+    uint32_t in_w = 0, out_w = 0;
+    if (!ixg_probe_shape(fn, &in_w, &out_w)) return fail(ctx, IXG_INVALID, "unknown probe function");
+    if (!in || !out) return fail(ctx, IXG_INVALID, "null in/out");
+    if (n == 0) return IXG_OK;
+    if (n > (1u << 24)) return fail(ctx, IXG_RESOURCE_LIMIT, "at most 2^24 records per call");
+    ixg::FilmParams P;
+    st = build_params(ctx, s, 0, 1, &P);
+    if (st != IXG_OK) return st;
+    st = upload_planes(ctx, s);
+    if (st != IXG_OK) return st;
+    float *d_in = nullptr, *d_out = nullptr;
+    hipError_t e = hipMalloc(&d_in, (size_t)n * in_w * sizeof(float));
+    if (e == hipSuccess) e = hipMalloc(&d_out, (size_t)n * out_w * sizeof(float));
+    if (e == hipSuccess) e = hipMemcpy(d_in, in, (size_t)n * in_w * sizeof(float), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = ixg_launch_probe(&P, ctx->d_planes, fn, d_in, n, d_out, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e == hipSuccess) e = hipMemcpy(out, d_out, (size_t)n * out_w * sizeof(float), hipMemcpyDeviceToHost);
+    if (d_in) (void)hipFree(d_in);
+    if (d_out) (void)hipFree(d_out);
+    if (e != hipSuccess) return hip_fail(ctx, e, "probe");
     return IXG_OK;
 }
 

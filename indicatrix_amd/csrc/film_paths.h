@@ -89,6 +89,44 @@ __device__ __forceinline__ void fresnel_rs_rp(float n1, float n2, float cos_i, f
     rp = div_rn(fma_(n1, -cos_t, n2 * cos_i), fma_(n1, cos_t, n2 * cos_i));
 }
 
+// The Stokes frame rotation by the doubled angle (MuellerMatrix::
+// frame_rotation, polarization.rs:138, applied).
+__device__ __forceinline__ float4 frame_rotate(float c2, float s2, float4 s) {
+    return make_float4(s.x, c2 * s.y + s2 * s.z, (-s2) * s.y + c2 * s.z, s.w);
+}
+
+// One channel's partial reflection (apply_partial_reflect_bounce's per-channel
+// step, reflect_refract.rs:33-71): MuellerMatrix::fresnel_reflection
+// (polarization.rs:148-158) applied, and the clamped unpolarised reflectance
+// that the channel's path pdf takes.
+__device__ __forceinline__ float4 fresnel_reflect(float n1, float n2, float cos_i, float cos_t, float4 s,
+                                                  float& r_unpol) {
+    float rs, rp, a, b, c;
+    fresnel_rs_rp(n1, n2, cos_i, cos_t, rs, rp);
+    r_unpol = clampf(0.5f * fma_(rp, rp, rs * rs), R_UNPOL_PDF_MIN, R_UNPOL_PDF_MAX);
+    fresnel_reflection_coeffs(rs, rp, a, b, c);
+    return mueller_fresnel(a, b, c, s);
+}
+
+// compute_channel_transmission (exit_split.rs:69-101) with
+// MuellerMatrix::fresnel_transmission (polarization.rs:162-180) applied.  t_s
+// and r_s share the denominator n1 cos_i + n2 cos_t, t_p and r_p the other:
+// one reciprocal each, the four ratios within an ULP of the quotients; r_s and
+// r_p come back for the channel's reflectance.
+__device__ __forceinline__ float4 fresnel_transmit(float n1, float n2, float cos_i, float cos_t, float4 s,
+                                                   float& rs, float& rp) {
+    const float num = 2.0f * n1 * cos_i;
+    const float inv_s = div_rn(1.0f, fma_(n2, cos_t, n1 * cos_i));
+    const float inv_p = div_rn(1.0f, fma_(n1, cos_t, n2 * cos_i));
+    const float t_s = num * inv_s;
+    const float t_p = num * inv_p;
+    const float factor = div_rn(n2 * cos_t, fmaxf(n1 * cos_i, 1e-6f));
+    const float ts2 = t_s * t_s * factor, tp2 = t_p * t_p * factor;
+    rs = fma_(n2, -cos_t, n1 * cos_i) * inv_s;
+    rp = fma_(n1, -cos_t, n2 * cos_i) * inv_p;
+    return mueller_fresnel(midpoint(ts2, tp2), 0.5f * (ts2 - tp2), t_s * t_p * factor, s);
+}
+
 __device__ __forceinline__ uint32_t fnv(uint32_t h, uint32_t v) { return (h ^ v) * 16777619u; }
 
 // The frosted-facet bitmask rides after the facet set (IXG_MAX_PLANES float4,
@@ -144,17 +182,25 @@ struct PathOut {
     ixg_path_events ev;
 };
 
-// One (pixel, absolute sample): sample_draws (sampling.rs:176-218) and
-// Camera::generate_ray (camera.rs:149-167).
-__device__ __forceinline__ void camera_sample(const FilmParams& P, uint32_t pixel, uint32_t sample,
-                                              V3& org, V3& dir, uint32_t& seed, float& hero) {
+// sample_draws (sampling.rs:176-218): the path seed, the pixel jitter and the
+// hero draw of one (pixel, absolute sample).
+__device__ __forceinline__ void sample_draws(uint32_t pixel, uint32_t sample, uint32_t& seed, float& jx, float& jy,
+                                             float& hero) {
     seed = hash_u32((pixel * 0x9e3779b9u) ^ (sample * 0x85ebca6bu));
     float rx = ld_base2(hash_u32(pixel ^ 0xA511E9B3u));
     float ry = ld_base2(hash_u32(pixel ^ 0x63D81B23u));
     float rh = ld_base2(hash_u32(pixel ^ 0x1B873593u));
-    float jx = cp_rotate(ld_base2(sample), rx) - 0.5f;
-    float jy = cp_rotate(radical_inverse<3>(sample), ry) - 0.5f;
+    jx = cp_rotate(ld_base2(sample), rx) - 0.5f;
+    jy = cp_rotate(radical_inverse<3>(sample), ry) - 0.5f;
     hero = cp_rotate(radical_inverse<5>(sample), rh);
+}
+
+// One (pixel, absolute sample): sample_draws and Camera::generate_ray
+// (camera.rs:149-167).
+__device__ __forceinline__ void camera_sample(const FilmParams& P, uint32_t pixel, uint32_t sample,
+                                              V3& org, V3& dir, uint32_t& seed, float& hero) {
+    float jx, jy;
+    sample_draws(pixel, sample, seed, jx, jy, hero);
     uint32_t px = pixel % P.width, py = pixel / P.width;
     float aspect = div_rn(P.width_f, P.height_f);
     float u = (div_rn((float)px + jx, P.width_f) - 0.5f) * 2.0f * aspect * P.fov_tan;
@@ -371,6 +417,46 @@ __device__ __forceinline__ float e_index_at(const FilmParams& P, float lambda, f
     return P.an.has_e ? dispersion(P.an.e_kind, P.an.e_disp, lambda) : n_o + P.an.biref;
 }
 
+__device__ __forceinline__ V3 refract_dir(V3 k, V3 normal, float eta, float cos_i, float cos_t) {
+    return normalize(k * eta + normal * fma_(eta, cos_i, -cos_t));
+}
+
+// theta_c_for_bounce (refraction/geometry.rs:75-120): the wave's angle to the
+// optic axis; at entry two fixed-point steps on the refracted direction
+// (seeded with the hero's ordinary index), inside the wave normal's own.
+__device__ __forceinline__ float theta_c_for_bounce(const FilmParams& P, float lambda_hero, float n_o_seed, V3 k,
+                                                    V3 normal, V3 cax, float cos_i, bool inside) {
+    if (inside) return acosf(fabsf(clampf(dot(k, cax), -1.0f, 1.0f)));
+    const float one_m_c2 = fma_(cos_i, -cos_i, 1.0f);
+    const float n_e_seed = e_index_at(P, lambda_hero, n_o_seed);
+    float n_guess = n_o_seed;
+    float theta_c = 0.0f;
+    for (int it = 0; it < 2; ++it) {
+        const float eg = 1.0f / n_guess;
+        const float s2g = eg * eg * one_m_c2;
+        if (s2g > 1.0f) break;
+        const float ctg = sqrt_rn(fmaxf(1.0f - s2g, 0.0f));
+        const V3 wd = refract_dir(k, normal, eg, cos_i, ctg);
+        theta_c = acosf(fabsf(clampf(dot(wd, cax), -1.0f, 1.0f)));
+        n_guess = effective_extraordinary_index(n_o_seed, n_e_seed, theta_c);
+    }
+    return theta_c;
+}
+
+// compute_uniaxial_exit_transmission (refraction/uniaxial_internal.rs): the
+// transmitted Stokes vector of a unit-flux-normalised internal solve for an
+// incident intensity inc_i, scaled by inv_t = 1 / (1 - r_branch); i_unit is
+// the unit-intensity transmittance.
+__device__ __forceinline__ float4 exit_transmission(const InternalSol& sol, float inc_i, float inv_t, float& i_unit) {
+    const float flux_inc = fmaxf(sol.flux_inc, 1e-12f);
+    const Cx ts = cscale(sol.t_s, sqrt_rn(sol.flux_ts / flux_inc));
+    const Cx tp = cscale(sol.t_p, sqrt_rn(sol.flux_tp / flux_inc));
+    i_unit = cnorm2(ts) + cnorm2(tp);
+    const float qu = cnorm2(ts) - cnorm2(tp);
+    const Cx cr = cmul(ts, cconj(tp));
+    return scale4(make_float4(inc_i * i_unit, inc_i * qu, inc_i * (2.0f * cr.re), inc_i * (-2.0f * cr.im)), inv_t);
+}
+
 // entry_eigenmode_selection (refraction/uniaxial_entry.rs:50-75)
 __device__ __forceinline__ bool eigenmode_selection(V3 c, V3 cpn, V3 k, float4 h, float& p_o, float& c2,
                                                     float& s2) {
@@ -401,10 +487,6 @@ __device__ __forceinline__ void split_probe(const FilmParams& P, const float4* _
     const float env = ISO ? lit_factor(P.env, dk, S.observer) * (spec_power(P.env, comb_lambda(S.hero, c)) * P.env.exposure)
                           : env_radiance<EXTRA>(P.env, dk, S.observer, comb_lambda(S.hero, c));
     S.split[c] = fma_(fmaxf(transmitted_i, 0.0f) * 1.0f, env, S.split[c]);
-}
-
-__device__ __forceinline__ V3 refract_dir(V3 k, V3 normal, float eta, float cos_i, float cos_t) {
-    return normalize(k * eta + normal * fma_(eta, cos_i, -cos_t));
 }
 
 __device__ __forceinline__ V3 reflect_dir(V3 k, V3 normal) { return k - normal * (2.0f * dot(k, normal)); }
@@ -471,10 +553,7 @@ __device__ __forceinline__ int facet_aniso(const FilmParams& P, const float4* __
         float c2, s2;
         rotation_double_angle(cos_psi, sin_psi, c2, s2);
 #pragma unroll
-        for (int c = 0; c < NCH; ++c) {
-            const float4 s = S.st[c];
-            S.st[c] = make_float4(s.x, c2 * s.y + s2 * s.z, (-s2) * s.y + c2 * s.z, s.w);
-        }
+        for (int c = 0; c < NCH; ++c) S.st[c] = frame_rotate(c2, s2, S.st[c]);
         if (EVENTS) ++S.n_rot;
     }
     S.prev_plane = cur_plane;
@@ -504,25 +583,7 @@ __device__ __forceinline__ int facet_aniso(const FilmParams& P, const float4* __
     const float cos_i = clampf(dot(neg(k), normal), 0.0f, 1.0f);
     const float one_m_c2 = fma_(cos_i, -cos_i, 1.0f);
     const float sin_i = sqrt_rn(fmaxf(one_m_c2, 0.0f));
-    float theta_c;
-    if (!inside) {
-        // theta_c_for_bounce (geometry.rs:75-120): two fixed-point steps
-        const float n_o_seed = S.n_o[0];
-        const float n_e_seed = e_index_at(P, comb_lambda(S.hero, 0), n_o_seed);
-        float n_guess = n_o_seed;
-        theta_c = 0.0f;
-        for (int it = 0; it < 2; ++it) {
-            const float eg = 1.0f / n_guess;
-            const float s2g = eg * eg * one_m_c2;
-            if (s2g > 1.0f) break;
-            const float ctg = sqrt_rn(fmaxf(1.0f - s2g, 0.0f));
-            const V3 wd = refract_dir(k, normal, eg, cos_i, ctg);
-            theta_c = acosf(fabsf(clampf(dot(wd, cax), -1.0f, 1.0f)));
-            n_guess = effective_extraordinary_index(n_o_seed, n_e_seed, theta_c);
-        }
-    } else {
-        theta_c = acosf(fabsf(clampf(dot(k, cax), -1.0f, 1.0f)));
-    }
+    const float theta_c = theta_c_for_bounce(P, comb_lambda(S.hero, 0), S.n_o[0], k, normal, cax, cos_i, inside);
     float n_e_ch[NCH], n_med[NCH];
 #pragma unroll
     for (int c = 0; c < NCH; ++c) {
@@ -573,12 +634,9 @@ __device__ __forceinline__ int facet_aniso(const FilmParams& P, const float4* __
                     tir_retardation(n1k, cos_i, sin_i, cd, sd);
                     S.st[c] = mueller_tir(cd, sd, S.st[c]);
                 } else {
-                    const float ct = sqrt_rn(fmaxf(1.0f - s2k, 0.0f));
-                    float a_, b_, a, b, cc;
-                    fresnel_rs_rp(n1k, n2k, cos_i, ct, a_, b_);
-                    S.path_pdf[c] *= clampf(0.5f * fma_(b_, b_, a_ * a_), R_UNPOL_PDF_MIN, R_UNPOL_PDF_MAX);
-                    fresnel_reflection_coeffs(a_, b_, a, b, cc);
-                    S.st[c] = mueller_fresnel(a, b, cc, S.st[c]);
+                    float ru;
+                    S.st[c] = fresnel_reflect(n1k, n2k, cos_i, sqrt_rn(fmaxf(1.0f - s2k, 0.0f)), S.st[c], ru);
+                    S.path_pdf[c] *= ru;
                 }
             }
         }
@@ -724,15 +782,8 @@ __device__ __forceinline__ int facet_aniso(const FilmParams& P, const float4* __
                 const bool match = dot(rk, hero_refr) >= DIRECTION_MATCH_COS_TOL;
                 if (match) match_bits |= 1u << c;
                 const InternalSol sol = c == 0 ? sh : internal_solve(n_inc, S.n_o[c], n_e_ch[c], cax, F, !is_extra);
-                // compute_uniaxial_exit_transmission (uniaxial_internal.rs)
-                const float flux_inc = fmaxf(sol.flux_inc, 1e-12f);
-                const Cx ts = cscale(sol.t_s, sqrt_rn(sol.flux_ts / flux_inc));
-                const Cx tp = cscale(sol.t_p, sqrt_rn(sol.flux_tp / flux_inc));
-                const float iu = cnorm2(ts) + cnorm2(tp);
-                const float qu = cnorm2(ts) - cnorm2(tp);
-                const Cx cr = cmul(ts, cconj(tp));
-                const float4 tr =
-                    scale4(make_float4(orig_i * iu, orig_i * qu, orig_i * (2.0f * cr.re), orig_i * (-2.0f * cr.im)), inv_t);
+                float iu;
+                const float4 tr = exit_transmission(sol, orig_i, inv_t, iu);
                 const float t_unpol = clampf(iu, R_UNPOL_PDF_MIN, R_UNPOL_PDF_MAX);
                 if (!match) {
                     const float prefix = S.path_pdf[c];
@@ -792,12 +843,9 @@ __device__ __forceinline__ int facet_aniso(const FilmParams& P, const float4* __
                     tir_retardation(n1k, cos_i, sin_i, cd, sd);
                     st = mueller_tir(cd, sd, S.st[c]);
                 } else {
-                    const float ct = sqrt_rn(fmaxf(1.0f - s2k, 0.0f));
-                    float a_, b_, a, b, cc;
-                    fresnel_rs_rp(n1k, n2k, cos_i, ct, a_, b_);
-                    S.path_pdf[c] *= clampf(0.5f * fma_(b_, b_, a_ * a_), R_UNPOL_PDF_MIN, R_UNPOL_PDF_MAX);
-                    fresnel_reflection_coeffs(a_, b_, a, b, cc);
-                    st = mueller_fresnel(a, b, cc, S.st[c]);
+                    float ru;
+                    st = fresnel_reflect(n1k, n2k, cos_i, sqrt_rn(fmaxf(1.0f - s2k, 0.0f)), S.st[c], ru);
+                    S.path_pdf[c] *= ru;
                 }
                 S.st[c] = scale4(st, sc);
             }
@@ -1077,10 +1125,7 @@ __device__ __forceinline__ int path_bounce(const FilmParams& P, const float4* __
         float c2, s2;
         rotation_double_angle(cos_psi, sin_psi, c2, s2);
 #pragma unroll
-        for (int c = 0; c < NCH; ++c) {
-            float4 s = S.st[c];
-            S.st[c] = make_float4(s.x, c2 * s.y + s2 * s.z, (-s2) * s.y + c2 * s.z, s.w);
-        }
+        for (int c = 0; c < NCH; ++c) S.st[c] = frame_rotate(c2, s2, S.st[c]);
         if (EVENTS) ++S.n_rot;
     }
     S.prev_plane = cur_plane;
@@ -1159,12 +1204,9 @@ __device__ __forceinline__ int path_bounce(const FilmParams& P, const float4* __
                 tir_retardation(n1k, cos_i, sin_i, cd, sd);
                 s = mueller_tir(cd, sd, S.st[c]);
             } else {
-                float ct = sqrt_rn(fmaxf(1.0f - s2k, 0.0f));
-                float a_, b_, a, b, cc;
-                fresnel_rs_rp(n1k, n2k, cos_i, ct, a_, b_);
-                S.path_pdf[c] *= clampf(0.5f * fma_(b_, b_, a_ * a_), R_UNPOL_PDF_MIN, R_UNPOL_PDF_MAX);
-                fresnel_reflection_coeffs(a_, b_, a, b, cc);
-                s = mueller_fresnel(a, b, cc, S.st[c]);
+                float ru;
+                s = fresnel_reflect(n1k, n2k, cos_i, sqrt_rn(fmaxf(1.0f - s2k, 0.0f)), S.st[c], ru);
+                S.path_pdf[c] *= ru;
             }
             S.st[c] = scale4(s, scale);
         }
@@ -1197,21 +1239,8 @@ __device__ __forceinline__ int path_bounce(const FilmParams& P, const float4* __
             float ctk = sqrt_rn(fmaxf(1.0f - s2k, 0.0f));
             V3 dk = normalize(k * r + normal * fma_(r, cos_i, -ctk));
             bool match = dot(dk, refr) >= DIRECTION_MATCH_COS_TOL;
-            // compute_channel_transmission (exit_split.rs:69-101).  t_s and
-            // r_s share the denominator n1 cos_i + n2 cos_t, t_p and r_p the
-            // other: one reciprocal each, the four ratios within an ULP of
-            // the quotients.
-            float num = 2.0f * n1k * cos_i;
-            const float inv_s = div_rn(1.0f, fma_(n2k, ctk, n1k * cos_i));
-            const float inv_p = div_rn(1.0f, fma_(n1k, ctk, n2k * cos_i));
-            float t_s = num * inv_s;
-            float t_p = num * inv_p;
-            float factor = div_rn(n2k * ctk, fmaxf(n1k * cos_i, 1e-6f));
-            float ts2 = t_s * t_s * factor, tp2 = t_p * t_p * factor;
-            float4 tr = mueller_fresnel(midpoint(ts2, tp2), 0.5f * (ts2 - tp2), t_s * t_p * factor, orig);
-            tr = scale4(tr, inv_t);
-            const float a_ = fma_(n2k, -ctk, n1k * cos_i) * inv_s;
-            const float b_ = fma_(n1k, -ctk, n2k * cos_i) * inv_p;
+            float a_, b_;
+            const float4 tr = scale4(fresnel_transmit(n1k, n2k, cos_i, ctk, orig, a_, b_), inv_t);
             float ruk = clampf(0.5f * fma_(b_, b_, a_ * a_), R_UNPOL_PDF_MIN, R_UNPOL_PDF_MAX);
             if (match) {
                 S.st[c] = tr;

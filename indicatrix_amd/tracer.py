@@ -397,6 +397,21 @@ class GpuBackend:
             raise IxgError(st, self.last_error())
         return xyz, events_to_numpy(ev)
 
+    def probe(self, scene, fn, records):
+        """ixg_probe: the device function `fn` (a name of abi.PROBE_FNS or its
+        id) on float32 input records, shape (n, in_floats) -> (n, out_floats)."""
+        fid = abi.PROBE[fn] if isinstance(fn, str) else int(fn)
+        w_in, w_out = C.c_uint32(), C.c_uint32()
+        if not lib().ixg_probe_shape(fid, C.byref(w_in), C.byref(w_out)):
+            raise ValueError(f"unknown probe function {fn!r}")
+        rec = np.ascontiguousarray(records, np.float32).reshape(-1, w_in.value)
+        out = np.zeros((len(rec), w_out.value), np.float32)
+        st = lib().ixg_probe(self._ctx, scene.ref(), fid, rec.ctypes.data_as(C.POINTER(C.c_float)), len(rec),
+                             out.ctypes.data_as(C.POINTER(C.c_float)))
+        if st != abi.IXG_OK:
+            raise IxgError(st, self.last_error())
+        return out
+
     def count_events(self, scene, sample_offset, spp):
         tot = (C.c_uint64 * len(EVENT_SLOTS))()
         st = lib().ixg_count_events(self._ctx, scene.ref(), sample_offset, spp, tot)

@@ -2013,6 +2013,30 @@ static int scene_build(const ixg_scene* sc, scene_t* S) {
     return env_build(&sc->environment, &S->env);
 }
 
+/* PathState::finish's estimate (bounce_body.rs:724-764) from the path's final
+ * radiance (the exit-split staging already added for an escaped path): the
+ * balance-heuristic weight over each channel's MIS family (color.rs:145-172),
+ * the CIE integration and von Kries white balance. */
+static void path_finish_xyz(const scene_t* S, const path_t* P, float xyz[3]) {
+    const float norm_factor = (400.0f / 8.0f) / 106.856f;
+    v3 acc = mk(0, 0, 0);
+    for (int c = 0; c < NCH; ++c) { /* color.rs:145-172 */
+        float family = -0.0f;
+        for (int j = 0; j < NCH; ++j)
+            if (P->compat[c] & (1u << j)) family += P->path_pdf[j];
+        float w = family <= 1e-12f ? 1.0f : (8.0f * P->path_pdf[0]) / family;
+        float cmf[3];
+        ixo_cie_cmf(P->lambdas[c], cmf);
+        float s = (P->radiance[c] * w) * norm_factor;
+        acc = vadd(acc, mk(cmf[0] * s, cmf[1] * s, cmf[2] * s));
+    }
+    acc = vadd(acc, mk(0, 0, 0)); /* + nee_xyz (zero on this path) */
+    acc = von_kries(acc, S->env.wb);
+    xyz[0] = maxf(acc.x, 0.0f);
+    xyz[1] = maxf(acc.y, 0.0f);
+    xyz[2] = maxf(acc.z, 0.0f);
+}
+
 static void trace_core(const scene_t* S, v3 origin, v3 dir, uint32_t seed, float hero_rand,
                        float xyz[3], ixg_path_events* ev) {
     path_t P;
@@ -2078,23 +2102,7 @@ static void trace_core(const scene_t* S, v3 origin, v3 dir, uint32_t seed, float
     /* PathState::finish (bounce_body.rs:724-764) */
     if (P.escaped)
         for (int c = 0; c < NCH; ++c) P.radiance[c] += P.split[c];
-    const float norm_factor = (400.0f / 8.0f) / 106.856f;
-    v3 acc = mk(0, 0, 0);
-    for (int c = 0; c < NCH; ++c) { /* color.rs:145-172 */
-        float family = -0.0f;
-        for (int j = 0; j < NCH; ++j)
-            if (P.compat[c] & (1u << j)) family += P.path_pdf[j];
-        float w = family <= 1e-12f ? 1.0f : (8.0f * P.path_pdf[0]) / family;
-        float cmf[3];
-        ixo_cie_cmf(P.lambdas[c], cmf);
-        float s = (P.radiance[c] * w) * norm_factor;
-        acc = vadd(acc, mk(cmf[0] * s, cmf[1] * s, cmf[2] * s));
-    }
-    acc = vadd(acc, mk(0, 0, 0)); /* + nee_xyz (zero on this path) */
-    acc = von_kries(acc, S->env.wb);
-    xyz[0] = maxf(acc.x, 0.0f);
-    xyz[1] = maxf(acc.y, 0.0f);
-    xyz[2] = maxf(acc.z, 0.0f);
+    path_finish_xyz(S, &P, xyz);
 }
 
 void ixo_trace_ray(const ixg_scene* sc, const float origin[3], const float dir[3], uint32_t seed,
@@ -3165,3 +3173,210 @@ int ixo_pose_metrics(const ixg_facet_plane* planes, uint32_t n, const ixg_materi
     out[2] = clampf(fmaf(0.4f, temporal, 0.6f * spatial), 0.0f, 100.0f);
     return IXG_OK;
 }
+
+#ifndef IXO_NO_PROBE
+/* ------------------------------------------------------------------ */
+/* Per-function probes: the CPU side of the reference's tier-2 checks  */
+/* (renderer/gpu/ulp.rs and the *_check.rs modules under renderer/gpu/), */
+/* each case the oracle's own function for ixg_probe's id, on the same  */
+/* records (layouts with the ids in include/ixg.h).                     */
+/* ------------------------------------------------------------------ */
+static uint32_t f_bits(float x) { uint32_t u; memcpy(&u, &x, 4); return u; }
+static float bits_f(uint32_t u) { float x; memcpy(&x, &u, 4); return x; }
+static stokes st_in(const float* p) { stokes s = {p[0], p[1], p[2], p[3]}; return s; }
+static void st_out(float* p, stokes s) { p[0] = s.i; p[1] = s.q; p[2] = s.u; p[3] = s.v; }
+static void v_out(float* p, v3 v) { p[0] = v.x; p[1] = v.y; p[2] = v.z; }
+static void c_out(float* p, cplx c) { p[0] = c.re; p[1] = c.im; }
+
+/* compute_bounce_refraction_geometry's frame for an incident wave normal k on
+ * a normal facing the incident side (geometry.rs:306-425). */
+static uframe probe_frame(v3 k, v3 normal, v3 c) {
+    float cos_i = clampf(vdot(vneg(k), normal), 0.0f, 1.0f);
+    float sin_i = sqrtf(maxf(fmaf(cos_i, -cos_i, 1.0f), 0.0f));
+    return uframe_build(k, normal, c, cos_i, sin_i);
+}
+
+static const uint8_t kProbeIn[IXG_PROBE_COUNT] = {2, 2, 1, 1, 7, 6, 8, 8, 7, 8, 19, 6, 8, 9, 3, 11, 13, 15, 6, 5, 8, 26, 6, 1, 1};
+static const uint8_t kProbeOut[IXG_PROBE_COUNT] = {12, 6, 3, 1, 1, 4, 5, 6, 4, 24, 8, 3, 3, 1, 2, 18, 13, 5, 3, 3, 3, 3, 2, 1, 1};
+
+int ixo_probe_shape(int32_t fn, uint32_t* in_floats, uint32_t* out_floats) {
+    if (fn < 0 || fn >= IXG_PROBE_COUNT) return 0;
+    if (in_floats) *in_floats = kProbeIn[fn];
+    if (out_floats) *out_floats = kProbeOut[fn];
+    return 1;
+}
+
+int ixo_probe(const ixg_scene* sc, int32_t fn, const float* in, uint32_t n, float* out) {
+    if (fn < 0 || fn >= IXG_PROBE_COUNT) return 0;
+    scene_t S;
+    if (!scene_build(sc, &S)) return 0;
+    for (uint32_t r = 0; r < n; ++r) {
+        const float* a = in + (size_t)r * kProbeIn[fn];
+        float* o = out + (size_t)r * kProbeOut[fn];
+        switch (fn) {
+        case IXG_PROBE_SAMPLE_DRAWS: {
+            uint32_t seed;
+            float jx, jy, hero, lam[NCH];
+            ixo_sample_draws(f_bits(a[0]), f_bits(a[1]), &seed, &jx, &jy, &hero);
+            ixo_wrapped_hero_wavelengths(hero, lam);
+            o[0] = jx; o[1] = jy; o[2] = hero; o[3] = bits_f(seed);
+            for (int c = 0; c < NCH; ++c) o[4 + c] = lam[c];
+            break;
+        }
+        case IXG_PROBE_CAMERA_RAY: { /* sample_core's camera half (cpu_frame.rs:45-80) */
+            uint32_t seed, pixel = f_bits(a[0]);
+            float jx, jy, hero, org[3], dir[3];
+            ixo_sample_draws(pixel, f_bits(a[1]), &seed, &jx, &jy, &hero);
+            ixg_camera cam = sc->camera;
+            cam.width = ABI_F32((float)sc->width);
+            cam.height = ABI_F32((float)sc->height);
+            ixo_generate_ray(&cam, (float)(pixel % sc->width), (float)(pixel / sc->width), jx, jy, org, dir);
+            for (int j = 0; j < 3; ++j) { o[j] = org[j]; o[3 + j] = dir[j]; }
+            break;
+        }
+        case IXG_PROBE_CIE_CMF:
+            ixo_cie_cmf(a[0], o);
+            break;
+        case IXG_PROBE_ILLUMINANT:
+            o[0] = spec_power(&S.env, a[0]);
+            break;
+        case IXG_PROBE_ENVIRONMENT:
+            o[0] = env_channel(&S, vfromr(a), a[6], vfromr(a + 3));
+            break;
+        case IXG_PROBE_FRAME_ROTATION: { /* rotate_stokes_to_plane_of_incidence (absorption.rs:69-87) */
+            mat4 R = mueller_frame_rotation(atan2_cr(a[1], a[0]));
+            st_out(o, st_apply(&R, st_in(a + 2)));
+            break;
+        }
+        case IXG_PROBE_FRESNEL_REFLECTION: {
+            float rs, rp;
+            fresnel_rs_rp(a[0], a[1], a[2], a[3], &rs, &rp);
+            mat4 M = mueller_fresnel_reflection(rs, rp);
+            st_out(o, st_apply(&M, st_in(a + 4)));
+            o[4] = clampf(0.5f * fmaf(rp, rp, rs * rs), R_UNPOL_PDF_MIN, R_UNPOL_PDF_MAX);
+            break;
+        }
+        case IXG_PROBE_FRESNEL_TRANSMISSION: { /* exit_split.rs:69-101 */
+            float num = 2.0f * a[0] * a[2];
+            float t_s = num / fmaf(a[1], a[3], a[0] * a[2]);
+            float t_p = num / fmaf(a[0], a[3], a[1] * a[2]);
+            mat4 T = mueller_fresnel_transmission(a[0], a[1], a[2], a[3], t_s, t_p);
+            st_out(o, st_apply(&T, st_in(a + 4)));
+            fresnel_rs_rp(a[0], a[1], a[2], a[3], &o[4], &o[5]);
+            break;
+        }
+        case IXG_PROBE_TIR_RETARDATION: {
+            mat4 M = mueller_tir_retardation(tir_phase_delta(a[0], a[1], a[2]));
+            st_out(o, st_apply(&M, st_in(a + 3)));
+            break;
+        }
+        case IXG_PROBE_CHANNEL_OPTICS: /* trace_core's per-channel n(lambda) and spectral_absorption */
+            for (int c = 0; c < NCH; ++c) {
+                o[c] = ixo_dispersion_evaluate(&sc->material->dispersion, a[c]);
+                o[NCH + c] = band_sum(sc->material->bands_o, sc->material->n_bands_o, a[c]);
+                o[2 * NCH + c] = band_sum(sc->material->bands_e, sc->material->n_bands_e, a[c]);
+            }
+            break;
+        case IXG_PROBE_PLEOCHROIC_ALPHA: { /* apply_absorption's per-channel alpha */
+            v3 k = vfromr(a);
+            v3 ea = ordinary_eigen(k, S.c_raw);
+            v3 eb = vnormalize_or_zero(vcross(k, ea));
+            for (int c = 0; c < NCH; ++c)
+                o[c] = midpointf(tensor_qf(&S, a[3 + c], a[3 + NCH + c], ea), tensor_qf(&S, a[3 + c], a[3 + NCH + c], eb));
+            break;
+        }
+        case IXG_PROBE_ORDINARY_EIGEN:
+            v_out(o, ordinary_eigen(vfromr(a), vfromr(a + 3)));
+            break;
+        case IXG_PROBE_EXTRAORDINARY_POYNTING:
+            v_out(o, extraordinary_poynting_dir(vfromr(a), vfromr(a + 3), a[6], a[7]));
+            break;
+        case IXG_PROBE_THETA_C: {
+            path_t P;
+            memset(&P, 0, sizeof P);
+            P.lambdas[0] = a[0];
+            v3 k = vfromr(a + 2), normal = vfromr(a + 5);
+            float cos_i = clampf(vdot(vneg(k), normal), 0.0f, 1.0f);
+            o[0] = theta_c_for_bounce(&S, &P, normal, k, cos_i, a[8] != 0.0f, a[1]);
+            break;
+        }
+        case IXG_PROBE_UNIAXIAL_INDICES:
+            o[0] = e_index_at(&S, a[0], a[1]);
+            o[1] = effective_extraordinary_index(a[1], o[0], a[2]);
+            break;
+        case IXG_PROBE_ENTRY_SOLVE: {
+            v3 c = vfromr(a + 2);
+            uframe F = probe_frame(vfromr(a + 5), vfromr(a + 8), c);
+            entry_inc I = entry_incidence_frame(1.0f, &F);
+            entry_sol sol[2];
+            entry_solve_pair(&I, 1.0f, a[0], a[1], c, &F, &sol[0], &sol[1]);
+            for (int p = 0; p < 2; ++p) {
+                c_out(o + 8 * p, sol[p].r_s);
+                c_out(o + 8 * p + 2, sol[p].r_p);
+                c_out(o + 8 * p + 4, sol[p].t_o);
+                c_out(o + 8 * p + 6, sol[p].t_e);
+            }
+            o[16] = sol[0].flux_o;
+            o[17] = sol[0].flux_e;
+            break;
+        }
+        case IXG_PROBE_INTERNAL_SOLVE:
+        case IXG_PROBE_EXIT_TRANSMISSION: {
+            v3 c = vfromr(a + 3);
+            uframe F = probe_frame(vfromr(a + 6), vfromr(a + 9), c);
+            internal_sol sol = internal_solve(a[0], a[1], a[2], c, &F, a[12] != 0.0f);
+            if (fn == IXG_PROBE_EXIT_TRANSMISSION) {
+                float iu;
+                st_out(o, uniaxial_exit_transmission(&sol, a[14], a[13], &iu));
+                o[4] = iu;
+                break;
+            }
+            c_out(o, sol.r_o);
+            c_out(o + 2, sol.r_e);
+            c_out(o + 4, sol.t_s);
+            c_out(o + 6, sol.t_p);
+            o[8] = sol.flux_ro; o[9] = sol.flux_re; o[10] = sol.flux_ts; o[11] = sol.flux_tp; o[12] = sol.flux_inc;
+            break;
+        }
+        case IXG_PROBE_HG_DIRECTION:
+            v_out(o, hg_direction(a[0], a[1], a[2], vfromr(a + 3)));
+            break;
+        case IXG_PROBE_COSINE_HEMISPHERE:
+            v_out(o, cosine_weighted_hemisphere(a[0], a[1], vfromr(a + 2)));
+            break;
+        case IXG_PROBE_SHADING_NORMAL:
+            v_out(o, shading_normal_near_edge(sc->planes, sc->n_planes, vfromr(a), (int32_t)f_bits(a[3]), vfromr(a + 4), a[7]));
+            break;
+        case IXG_PROBE_FINISH: {
+            path_t P;
+            memset(&P, 0, sizeof P);
+            uint32_t lo = f_bits(a[0]), hi = f_bits(a[1]);
+            for (int c = 0; c < NCH; ++c) {
+                P.compat[c] = (uint8_t)((c < 4 ? lo >> (8 * c) : hi >> (8 * (c - 4))) & 0xFFu);
+                P.path_pdf[c] = a[2 + c];
+                P.lambdas[c] = a[2 + NCH + c];
+                P.radiance[c] = a[2 + 2 * NCH + c];
+            }
+            path_finish_xyz(&S, &P, o);
+            break;
+        }
+        case IXG_PROBE_INTERSECT: {
+            float t = 0.0f;
+            int32_t f = scene_intersect(&S, a, a + 3, &t);
+            o[0] = bits_f((uint32_t)f);
+            o[1] = f >= 0 ? t : 0.0f;
+            break;
+        }
+        case IXG_PROBE_EXP_LANE:
+            o[0] = ixo_exp_lane(a[0]);
+            break;
+        case IXG_PROBE_EXP: /* e^x in double precision, rounded to float */
+            o[0] = (float)exp((double)a[0]);
+            break;
+        default:
+            break;
+        }
+    }
+    return 1;
+}
+#endif

@@ -141,7 +141,14 @@ int ixo_pose_metrics(const ixg_facet_plane* planes, uint32_t n, const ixg_materi
                      const ixg_environment* env, float yaw, float pitch, int want_all,
                      float out[5]);
 
+/* Per-function probes (the CPU side of ixg_probe): the oracle's function for
+ * probe id fn on n records, layouts as in include/ixg.h.  Returns 0 for an
+ * unknown id or a scene the oracle cannot build. */
+int ixo_probe_shape(int32_t fn, uint32_t* in_floats, uint32_t* out_floats);
+int ixo_probe(const ixg_scene* scene, int32_t fn, const float* in, uint32_t n, float* out);
+
 #ifdef __cplusplus
 }
 #endif
+
 #endif

@@ -24,6 +24,7 @@
 #include "ixo.h"
 
 #define IXO_SCALAR_ONLY 1
+#define IXO_NO_PROBE 1
 #define IXO_FP64_TWIN 1
 #define float double
 #define fmaf fma

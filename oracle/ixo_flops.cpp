@@ -23,6 +23,7 @@ namespace {
 uint64_t g_flops = 0;
 }
 #define IXO_SCALAR_ONLY 1
+#define IXO_NO_PROBE 1
 #define IXO_FLOPS(n) (g_flops += (uint64_t)(n))
 
 struct CF {

@@ -119,8 +119,26 @@ def lib():
                                      C.c_int32, C.c_double, P(abi.FacetPlane), C.c_uint32]
         L.ixo_solid_outline.restype = C.c_int
         L.ixo_solid_outline.argtypes = [P(abi.FacetPlane), C.c_uint32, P(C.c_double), P(C.c_uint32)]
+        L.ixo_probe_shape.restype = C.c_int
+        L.ixo_probe_shape.argtypes = [C.c_int32, P(C.c_uint32), P(C.c_uint32)]
+        L.ixo_probe.restype = C.c_int
+        L.ixo_probe.argtypes = [P(abi.Scene), C.c_int32, P(C.c_float), C.c_uint32, P(C.c_float)]
         _lib = L
     return _lib
+
+
+def probe(scene, fn, records):
+    """ixo_probe: the oracle's function for probe `fn` (a name of
+    abi.PROBE_FNS) on float32 records (n, in_floats) -> (n, out_floats)."""
+    L = lib()
+    fid = abi.PROBE[fn]
+    w_in, w_out = C.c_uint32(), C.c_uint32()
+    assert L.ixo_probe_shape(fid, C.byref(w_in), C.byref(w_out))
+    rec = np.ascontiguousarray(records, np.float32).reshape(-1, w_in.value)
+    out = np.zeros((len(rec), w_out.value), np.float32)
+    assert L.ixo_probe(scene.ref(), fid, rec.ctypes.data_as(C.POINTER(C.c_float)), len(rec),
+                       out.ctypes.data_as(C.POINTER(C.c_float)))
+    return out
 
 
 def lib64():
