@@ -226,6 +226,30 @@ typedef void (*ixg_batch_done_fn)(void* user, uint32_t index, ixg_status outcome
  * arguments, else IXG_OK once every item has been reported. */
 ixg_status ixg_accumulate_batch(ixg_ctx* ctx, const ixg_batch_item* items, uint32_t n,
                                 const volatile uint8_t* cancel, ixg_batch_done_fn on_done, void* user);
+/* One chunk of a dispatch lane: WorkerLane::render_chunk (indicatrix-dispatch/
+ * src/lane.rs:113-142) as the coordinator's own lane runs it (coordinator/
+ * lanes/mod.rs:536-570 via stream_emit/tracer.rs:328 trace_range).  Traces
+ * [first_sample, first_sample + samples) of the scene in sub-batches sized
+ * toward 100 ms each (ixg_next_subbatch), adding each sub-batch only once it
+ * completed, and stops between (or inside, discarding that sub-batch)
+ * sub-batches once *cancel is non-zero.  sum_xyz (w*h*3 f32, the
+ * ChunkResult::sum Vec3 layout) is overwritten with the summed XYZ of the
+ * traced prefix [first_sample, first_sample + *done) -- always a prefix, never
+ * a gap.  *rate (may be NULL) = samples per second over the sub-batches after
+ * the first (the steady state without the scene upload), 0 with fewer than
+ * two; seed_subbatch = the first sub-batch (0 = 1, seeded_batch_size without
+ * a remembered size) and *subbatch_out (may be NULL) = the converged size to
+ * seed the next chunk of the same scene with.  Returns IXG_OK when the whole
+ * range was traced, IXG_CANCELLED when the flag ended it, or the failing call's
+ * status (ChunkResult::partial: the prefix before the failure is kept). */
+ixg_status ixg_render_chunk(ixg_ctx* ctx, const ixg_scene* scene, uint32_t first_sample, uint32_t samples,
+                            uint32_t seed_subbatch, float* sum_xyz, uint32_t* done, double* rate,
+                            uint32_t* subbatch_out, const volatile uint8_t* cancel);
+/* stream_emit/sizing.rs next_batch_size: the next sub-batch after one of
+ * prev samples took elapsed_s seconds -- prev scaled toward 100 ms, at most
+ * x4 per step and never above 2048 or below 1 (an unmeasurable time grows by
+ * x4). */
+uint32_t ixg_next_subbatch(uint32_t prev, double elapsed_s);
 /* Same, with accum_xyz a device pointer (inputs resident in HBM), on the
  * library's stream; no host copy of the result. */
 ixg_status ixg_accumulate_device(ixg_ctx* ctx, const ixg_scene* scene, uint32_t sample_offset,

@@ -48,6 +48,9 @@ def _declare(lib):
         "ixg_trace_rays": (st, [C.c_void_p, P(abi.Scene), P(C.c_float), C.c_uint32,
                                 P(C.c_float), P(abi.PathEvents)]),
         "ixg_stream": (C.c_void_p, [C.c_void_p]),
+        "ixg_render_chunk": (st, [C.c_void_p, P(abi.Scene), C.c_uint32, C.c_uint32, C.c_uint32, P(C.c_float),
+                                  P(C.c_uint32), P(C.c_double), P(C.c_uint32), P(C.c_uint8)]),
+        "ixg_next_subbatch": (C.c_uint32, [C.c_uint32, C.c_double]),
         "ixg_probe_shape": (C.c_int, [C.c_int32, P(C.c_uint32), P(C.c_uint32)]),
         "ixg_probe": (st, [C.c_void_p, P(abi.Scene), C.c_int32, P(C.c_float), C.c_uint32, P(C.c_float)]),
         "ixg_count_events": (st, [C.c_void_p, P(abi.Scene), C.c_uint32, C.c_uint32,

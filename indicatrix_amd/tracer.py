@@ -173,6 +173,70 @@ def round_brilliant(crown_deg, pavilion_deg, table_ratio):
 
 
 # --------------------------------------------------------------------------
+# dispatch lanes (indicatrix-dispatch/src/lane.rs)
+# --------------------------------------------------------------------------
+class ChunkResult:
+    """ChunkResult (lane.rs:67-111): the summed XYZ of the traced prefix
+    [first_sample, first_sample + done) as float32 (pixels, 3), the lane's
+    steady-state samples/s (None: let the pool time it) and why the chunk
+    ended short, if it did."""
+
+    def __init__(self, sum, done, rate=None, error=None):
+        self.sum = sum
+        self.done = done
+        self.rate = rate
+        self.error = error
+
+    @property
+    def complete(self):
+        return self.error is None
+
+
+class GpuLane:
+    """WorkerLane (lane.rs:113-142) backed by one libixg context: each chunk
+    is ixg_render_chunk, the adaptive sub-batch loop of the coordinator's own
+    lane (OwnLane, apps/indicatrix-worker/src/coordinator/lanes/mod.rs:536-570).
+    The converged sub-batch size is remembered per scene and seeds the next
+    chunk of that scene (stream_emit/sizing.rs remember_converged_subbatch)."""
+
+    def __init__(self, backend, name="local GPU"):
+        self.backend = backend
+        self._name = name
+        self._subbatch = {}
+
+    def name(self):
+        return self._name
+
+    def lost(self):
+        return self.backend.is_lost()
+
+    def render_chunk(self, scene, first_sample, samples, cancel=None):
+        n = scene.width * scene.height
+        out = np.zeros((n, 3), np.float32)
+        done, rate, nxt = C.c_uint32(), C.c_double(), C.c_uint32()
+        cflag = None if cancel is None else C.cast(cancel, C.POINTER(C.c_uint8))
+        st = lib().ixg_render_chunk(self.backend.handle, scene.ref(), first_sample, samples,
+                                    self._subbatch.get(id(scene), 0), out.ctypes.data_as(C.POINTER(C.c_float)),
+                                    C.byref(done), C.byref(rate), C.byref(nxt), cflag)
+        if done.value:
+            self._subbatch[id(scene)] = nxt.value
+        r = rate.value if rate.value > 0.0 else None
+        if st == abi.IXG_OK:
+            return ChunkResult(out, done.value, r)
+        if st == abi.IXG_CANCELLED:
+            return ChunkResult(out, done.value, None, "cancelled")
+        if st == abi.IXG_DECLINED and done.value == 0:
+            return ChunkResult(np.zeros((0, 3), np.float32), 0, None,
+                               "declined by the device route: " + self.backend.last_error())
+        return ChunkResult(out, done.value, None, self.backend.last_error() or f"ixg status {st}")
+
+
+def next_subbatch(prev, elapsed_s):
+    """stream_emit/sizing.rs next_batch_size, as ixg_next_subbatch."""
+    return lib().ixg_next_subbatch(prev, elapsed_s)
+
+
+# --------------------------------------------------------------------------
 # scenes and the backend
 # --------------------------------------------------------------------------
 class FrameScene:
