@@ -1052,6 +1052,44 @@ __device__ __forceinline__ int scatter_step(const FilmParams& P, PathState& S, f
     return -1;
 }
 
+// handle_miss (bounce_body/miss.rs:33-54): the environment seen along the
+// path's direction, weighted by each channel's intensity, added to its
+// exit-split staging.
+template <bool ISO, bool EXTRA>
+__device__ __forceinline__ void path_miss(const FilmParams& P, PathState& S) {
+    if (EXTRA && !ISO && P.env.model == ENV_HDR) {  // the panorama: one texel, per-channel spectrum
+        float rgb[3];
+        hdr_rgb(P.env, S.dir, rgb);
+#pragma unroll
+        for (int c = 0; c < NCH; ++c) {
+            float env = rgb_to_spectral(rgb, comb_lambda(S.hero, c));
+            S.split[c] = fma_(fmaxf(S.st[c].x, 0.0f) * 1.0f, env, 0.0f) + S.split[c];
+        }
+    } else if (!ISO && P.env.model == ENV_ASET) {
+        const V3 d = normalize(S.dir);
+#pragma unroll
+        for (int c = 0; c < NCH; ++c) {
+            float env = aset_radiance(P.env, d, comb_lambda(S.hero, c), P.env.exposure);
+            S.split[c] = fma_(fmaxf(S.st[c].x, 0.0f) * 1.0f, env, 0.0f) + S.split[c];
+        }
+    } else if (ISO || P.env.model != ENV_STUDIO) {
+        const float f = lit_factor(P.env, S.dir, S.observer);
+#pragma unroll
+        for (int c = 0; c < NCH; ++c) {
+            float env = f * (spec_power(P.env, comb_lambda(S.hero, c)) * P.env.exposure);
+            S.split[c] = fma_(fmaxf(S.st[c].x, 0.0f) * 1.0f, env, 0.0f) + S.split[c];
+        }
+    } else {
+        DirLight L;
+        direction_lighting(P.env, S.dir, S.observer, L);
+#pragma unroll
+        for (int c = 0; c < NCH; ++c) {
+            float env = light_radiance(P.env, L, spec_power(P.env, comb_lambda(S.hero, c)));
+            S.split[c] = fma_(fmaxf(S.st[c].x, 0.0f) * 1.0f, env, 0.0f) + S.split[c];
+        }
+    }
+}
+
 // EXTRA: the rarely used surface and medium features (edge-rounded normals,
 // frosted facets, volume scattering) are compiled in; the persistent film
 // kernel is also built without them, so a stone that uses none runs the
@@ -1069,38 +1107,7 @@ __device__ __forceinline__ int path_bounce(const FilmParams& P, const float4* __
         if (EVENTS) ++S.n_scan;
     }
     if (facet < 0) {
-        // handle_miss (bounce_body/miss.rs:33-54)
-        if (EXTRA && !ISO && P.env.model == ENV_HDR) {  // the panorama: one texel, per-channel spectrum
-            float rgb[3];
-            hdr_rgb(P.env, S.dir, rgb);
-#pragma unroll
-            for (int c = 0; c < NCH; ++c) {
-                float env = rgb_to_spectral(rgb, comb_lambda(S.hero, c));
-                S.split[c] = fma_(fmaxf(S.st[c].x, 0.0f) * 1.0f, env, 0.0f) + S.split[c];
-            }
-        } else if (!ISO && P.env.model == ENV_ASET) {
-            const V3 d = normalize(S.dir);
-#pragma unroll
-            for (int c = 0; c < NCH; ++c) {
-                float env = aset_radiance(P.env, d, comb_lambda(S.hero, c), P.env.exposure);
-                S.split[c] = fma_(fmaxf(S.st[c].x, 0.0f) * 1.0f, env, 0.0f) + S.split[c];
-            }
-        } else if (ISO || P.env.model != ENV_STUDIO) {
-            const float f = lit_factor(P.env, S.dir, S.observer);
-#pragma unroll
-            for (int c = 0; c < NCH; ++c) {
-                float env = f * (spec_power(P.env, comb_lambda(S.hero, c)) * P.env.exposure);
-                S.split[c] = fma_(fmaxf(S.st[c].x, 0.0f) * 1.0f, env, 0.0f) + S.split[c];
-            }
-        } else {
-            DirLight L;
-            direction_lighting(P.env, S.dir, S.observer, L);
-#pragma unroll
-            for (int c = 0; c < NCH; ++c) {
-                float env = light_radiance(P.env, L, spec_power(P.env, comb_lambda(S.hero, c)));
-                S.split[c] = fma_(fmaxf(S.st[c].x, 0.0f) * 1.0f, env, 0.0f) + S.split[c];
-            }
-        }
+        path_miss<ISO, EXTRA>(P, S);
         return STEP_ESCAPED;
     }
 
@@ -1410,7 +1417,7 @@ constexpr int kBlock = 256;
     __attribute__((amdgpu_waves_per_eu(ANISO ? IXG_FILM_WAVES_PER_EU_ANISO : IXG_FILM_WAVES_PER_EU)))
 // Idle lanes a wave collects before it finishes and regenerates paths.
 #ifndef IXG_FILM_REGEN_MIN
-#define IXG_FILM_REGEN_MIN 24
+#define IXG_FILM_REGEN_MIN 32
 #endif
 
 __device__ __forceinline__ void stage_planes(const FilmParams& P, const float4* __restrict__ g,
@@ -1480,7 +1487,16 @@ __global__ __launch_bounds__(kBlock) IXG_WAVES_ATTR void film_persistent_kernel(
         if (active) {
             int r = STEP_CONTINUE;
             if (S.bounce < P.max_bounces) r = path_bounce<false, ISO, ABSORB, ANISO, EXTRA>(P, sp, S);
-            if (r != STEP_CONTINUE || ++S.bounce >= P.max_bounces) {
+            // A path that now leaves the convex stone through the facet it
+            // just crossed would find no facet on its next bounce: that
+            // bounce's miss is taken here, so the lane does not sit out the
+            // other lanes' slab scans for an iteration.
+            if (r == STEP_CONTINUE && ++S.bounce < P.max_bounces && !S.inside && S.last_facet >= 0 &&
+                leaves_convex(sp, S.last_facet, S.org, S.dir)) {
+                path_miss<ISO, EXTRA>(P, S);
+                r = STEP_ESCAPED;
+            }
+            if (r != STEP_CONTINUE || S.bounce >= P.max_bounces) {
                 active = false;
                 pending = true;
                 pending_escaped = r == STEP_ESCAPED;
