@@ -99,8 +99,18 @@ typedef struct ixg_material {
     float absorption_path_scale;
     float edge_rounding_radius;           /* > 0: rounded meet edges (shading)     */
     float scattering_sigma_s;             /* > 0: Henyey-Greenstein volume scatter */
-    int32_t biaxial;                      /* GPU path: must be 0                   */
+    int32_t biaxial;                      /* 1: biaxial_delta_beta_alpha is Some   */
     float scattering_g;                   /* Henyey-Greenstein asymmetry           */
+    /* GemMaterial::biaxial_delta_beta_alpha (materials/mod.rs:146) when biaxial:
+     * n_beta = dispersion(lambda), n_alpha = n_beta - this, n_gamma = n_alpha +
+     * birefringence_delta, gamma along c_axis (materials/optics.rs:16-27). */
+    float biaxial_delta_beta_alpha;
+    /* AbsorptionTensor::beta_ray (optics/absorption.rs:337-356): 1 = Some (a
+     * trichroic tensor alpha/beta/gamma = o/beta/e bands), 0 = None. Read only
+     * for a biaxial material (absorption/wavelength_cache.rs:38-51). */
+    int32_t has_beta_bands;
+    int32_t n_bands_beta;
+    ixg_band bands_beta[IXG_MAX_BANDS];
 } ixg_material;
 
 /* LightingPreset ids, the reference's persisted ids

@@ -78,7 +78,10 @@ class Material(C.Structure):
                 ("absorption_path_scale", C.c_float),
                 ("edge_rounding_radius", C.c_float),
                 ("scattering_sigma_s", C.c_float),
-                ("biaxial", C.c_int32), ("scattering_g", C.c_float)]
+                ("biaxial", C.c_int32), ("scattering_g", C.c_float),
+                ("biaxial_delta_beta_alpha", C.c_float),
+                ("has_beta_bands", C.c_int32), ("n_bands_beta", C.c_int32),
+                ("bands_beta", Band * MAX_BANDS)]
 
 
 class Environment(C.Structure):

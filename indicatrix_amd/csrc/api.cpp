@@ -105,11 +105,18 @@ bool any_frosted(const ixg_scene* s) {
 }
 
 bool material_routable(const ixg_material& m) {
-    // The device path covers isotropic and uniaxial (RayMaterialContext::
-    // is_anisotropic, transport/inner.rs:181-198), polished, unzoned,
-    // non-scattering materials, with or without absorption bands; biaxial
-    // materials decline.
-    if (m.biaxial || !(m.edge_rounding_radius < 1e30f)) return false;
+    // The device path covers isotropic, uniaxial and biaxial
+    // (RayMaterialContext::is_anisotropic, transport/inner.rs:181-198),
+    // unzoned materials, with or without absorption bands.  A biaxial stone
+    // is anisotropic (a "biaxial" set on a stone that is not declines) and
+    // takes no volume scattering.
+    if (!(m.edge_rounding_radius < 1e30f)) return false;
+    if (m.biaxial) {
+        if (m.cubic || !(std::fabs(m.birefringence_delta) > 1e-4f) || !std::isfinite(m.biaxial_delta_beta_alpha))
+            return false;
+        if (m.scattering_sigma_s > 0.0f) return false;
+        if (m.has_beta_bands && (m.n_bands_beta < 0 || m.n_bands_beta > IXG_MAX_BANDS)) return false;
+    }
     // Henyey-Greenstein volume scattering: isotropic stones, finite parameters
     if (!(m.scattering_sigma_s <= 0.0f) &&
         (!(m.scattering_sigma_s < 1e30f) || !(std::fabs(m.scattering_g) < 1.0f) || !(m.absorption_path_scale > 0.0f) ||
@@ -161,6 +168,10 @@ void fill_absorption(const ixg_material& m, ixg::AbsorbParams* A) {
         A->o[i] = ixg::AbsBand{m.bands_o[i].center_nm, m.bands_o[i].width, m.bands_o[i].peak, m.bands_o[i].shape};
     for (int i = 0; i < m.n_bands_e; ++i)
         A->e[i] = ixg::AbsBand{m.bands_e[i].center_nm, m.bands_e[i].width, m.bands_e[i].peak, m.bands_e[i].shape};
+    A->n_b = (m.biaxial && m.has_beta_bands) ? m.n_bands_beta : 0;
+    for (int i = 0; i < A->n_b; ++i)
+        A->b[i] = ixg::AbsBand{m.bands_beta[i].center_nm, m.bands_beta[i].width, m.bands_beta[i].peak,
+                               m.bands_beta[i].shape};
 }
 
 // hdr_dst / hdr_cap (floats): where the panorama is uploaded; null = the
@@ -182,7 +193,10 @@ ixg_status build_params(ixg_ctx* ctx, const ixg_scene* s, uint32_t sample_offset
     P->n_planes = s->n_planes;
     P->max_bounces = s->max_bounces;
     P->disp_kind = s->material->dispersion.kind;
-    P->absorb = (s->material->n_bands_o > 0 || s->material->n_bands_e > 0) ? 1 : 0;
+    P->absorb = (s->material->n_bands_o > 0 || s->material->n_bands_e > 0 ||
+                 (s->material->biaxial && s->material->has_beta_bands && s->material->n_bands_beta > 0))
+                    ? 1
+                    : 0;
     if (P->absorb) fill_absorption(*s->material, &P->abs);
     P->abs.path_scale = s->material->absorption_path_scale;  // the scattering step reads it too
     const ixg_material& m = *s->material;
@@ -192,6 +206,18 @@ ixg_status build_params(ixg_ctx* ctx, const ixg_scene* s, uint32_t sample_offset
     P->an.e_kind = m.e_dispersion.kind;
     std::memcpy(P->an.e_disp, m.e_dispersion.coeff, sizeof P->an.e_disp);
     std::memcpy(P->an.c_axis, m.c_axis, sizeof P->an.c_axis);
+    if (m.biaxial) {
+        // BiaxialIndicatrix::from_gamma_axis (biaxial.rs:61-65): the frame the
+        // absorption tensor uses too
+        P->an.biax = 1;
+        P->an.trichroic = m.has_beta_bands ? 1 : 0;
+        P->an.dba = m.biaxial_delta_beta_alpha;
+        ixg::AbsorbParams F;
+        fill_absorption(m, &F);
+        std::memcpy(P->an.bx, F.a1, sizeof P->an.bx);
+        std::memcpy(P->an.by, F.a2, sizeof P->an.by);
+        std::memcpy(P->an.bz, F.cn, sizeof P->an.bz);
+    }
     std::memcpy(P->disp, s->material->dispersion.coeff, sizeof P->disp);
     P->edge_radius = s->material->edge_rounding_radius > 0.0f ? s->material->edge_rounding_radius : 0.0f;
     P->frosted = any_frosted(s) ? 1 : 0;

@@ -28,6 +28,7 @@
 #include <algorithm>
 #include "device_common.h"
 #include "uniaxial.h"
+#include "biaxial.h"
 
 namespace ixg {
 
@@ -221,6 +222,7 @@ struct PathState {
     float hero;               // the hero draw; channel c's wavelength is lambda_of(S, c)
     float n_o[NCH];
     float alpha_o[NCH], alpha_e[NCH];  // spectral_absorption, absorbing materials only
+    float alpha_b[NCH];               // of the beta bands (trichroic biaxial stones)
     uint64_t compat;          // byte c = MIS family of channel c
     V3 org, dir, observer, prev_plane;
     V3 k;                     // wave normal (uniaxial materials; dir is the Poynting direction)
@@ -306,7 +308,7 @@ __device__ __forceinline__ void path_start(V3 org, V3 dir, uint32_t seed, PathSt
 }
 
 // PathState::new (bounce_body.rs:258-356).
-template <bool ABSORB = false>
+template <bool ABSORB = false, int ANISO = 0>
 __device__ __forceinline__ void path_init(const FilmParams& P, V3 org, V3 dir, uint32_t seed,
                                           float hero, PathState& S) {
     path_start(org, dir, seed, S);
@@ -314,6 +316,14 @@ __device__ __forceinline__ void path_init(const FilmParams& P, V3 org, V3 dir, u
     float lambda[NCH];
     hero_comb(hero, lambda);
     path_optics<ABSORB>(P, lambda, S.n_o, S.alpha_o, S.alpha_e);
+    if (ABSORB && ANISO == 2) {  // the trichroic beta bands (wavelength_cache.rs:38-51)
+#pragma unroll
+        for (int c = 0; c < NCH; ++c) {
+            float sb = -0.0f;
+            for (int32_t b = 0; b < P.abs.n_b; ++b) sb += band_eval(P.abs.b[b], lambda[c]);
+            S.alpha_b[c] = sb;
+        }
+    }
 }
 
 // The environment radiance seen along dir.  The HDR panorama is compiled only
@@ -957,6 +967,244 @@ __device__ __forceinline__ int facet_aniso(const FilmParams& P, const float4* __
     return STEP_CONTINUE;
 }
 
+// One facet event of a biaxial material (biaxial_delta_beta_alpha is Some):
+// compute_bounce_refraction_geometry's biaxial block (geometry.rs:199-270,
+// 306-425: mode A = the fast root, mode B = the slow root, each channel's
+// indicatrix at the hero's wave normals, resolved at entry by two fixed-point
+// steps per mode), the scalar TIR and partial-Fresnel arms with a 50/50 mode
+// draw at entry (tir.rs, dispatch.rs:143-262), the refract arm's per-channel
+// biaxial walk-off (reflect_refract.rs:122-330), the assigned mode's
+// pleochroic absorption (absorption.rs:224-249) and the 50/50 internal
+// re-coupling draw (transport/bounce.rs:439-470).
+template <bool EVENTS, bool ISO, bool ABSORB, bool EXTRA = true>
+__device__ __forceinline__ int facet_biax(const FilmParams& P, const float4* __restrict__ planes, PathState& S,
+                                          float t_hit, int facet) {
+    const uint32_t bounce = S.bounce;
+    const BiaxFrame BF{v3(P.an.bx[0], P.an.bx[1], P.an.bx[2]), v3(P.an.by[0], P.an.by[1], P.an.by[2]),
+                       v3(P.an.bz[0], P.an.bz[1], P.an.bz[2])};
+    const V3 hit = S.org + S.dir * t_hit;
+    const float4 pf = planes[facet];
+    V3 normal = v3(pf.x, pf.y, pf.z);
+    if (EXTRA && P.edge_radius > 0.0f)
+        normal = shading_normal_near_edge(planes, P.n_planes, hit, facet, normal, P.edge_radius);
+    const V3 cur_plane = normalize_or_zero(cross(S.k, normal));
+    if (S.has_prev && dot(cur_plane, cur_plane) > 1e-6f && dot(S.prev_plane, S.prev_plane) > 1e-6f) {
+        const float cos_psi = clampf(dot(S.prev_plane, cur_plane), -1.0f, 1.0f);
+        const float sin_psi = dot(cross(S.prev_plane, cur_plane), normalize_or_zero(S.k));
+        float c2, s2;
+        rotation_double_angle(cos_psi, sin_psi, c2, s2);
+#pragma unroll
+        for (int c = 0; c < NCH; ++c) S.st[c] = frame_rotate(c2, s2, S.st[c]);
+        if (EVENTS) ++S.n_rot;
+    }
+    S.prev_plane = cur_plane;
+    S.has_prev = true;
+    const V3 k = S.k;
+    const bool inside = S.inside, is_extra = S.is_extra;
+    const Biax hero{S.n_o[0] - P.an.dba, S.n_o[0], (S.n_o[0] - P.an.dba) + P.an.biref};
+    if (ABSORB && inside) {
+        // channel_absorption_alphas_assigned: alpha = e . A . e for the assigned mode's E
+        const V3 e = biax_d_to_e(hero, BF, biax_eigen_d(hero, BF, k, is_extra));
+        const float l0 = dot(v3(P.abs.a1[0], P.abs.a1[1], P.abs.a1[2]), e);
+        const float l1 = dot(v3(P.abs.a2[0], P.abs.a2[1], P.abs.a2[2]), e);
+        const float l2 = dot(v3(P.abs.cn[0], P.abs.cn[1], P.abs.cn[2]), e);
+        const float scaled = t_hit * P.abs.path_scale;
+#pragma unroll
+        for (int c = 0; c < NCH; ++c) {
+            const float ab = P.an.trichroic ? S.alpha_b[c] : S.alpha_o[c];
+            const float q = (S.alpha_o[c] * (l0 * l0) + ab * (l1 * l1)) + S.alpha_e[c] * (l2 * l2);
+            S.st[c] = scale4(S.st[c], exp_lane(-q * scaled));
+        }
+    }
+    if (inside) normal = neg(normal);
+
+    const float cos_i = clampf(dot(neg(k), normal), 0.0f, 1.0f);
+    const float one_m_c2 = fma_(cos_i, -cos_i, 1.0f);
+    const float sin_i = sqrt_rn(fmaxf(one_m_c2, 0.0f));
+    // hero_biaxial_wave_dirs / per_channel_biaxial_indices (geometry.rs:216-270)
+    V3 dir_a = k, dir_b = k;
+    if (!inside) {
+        dir_a = biax_resolve_entry(hero, BF, k, normal, cos_i, S.n_o[0], false);
+        dir_b = biax_resolve_entry(hero, BF, k, normal, cos_i, S.n_o[0], true);
+    }
+    float n_a[NCH], n_b[NCH];
+#pragma unroll
+    for (int c = 0; c < NCH; ++c) {
+        const Biax B{S.n_o[c] - P.an.dba, S.n_o[c], (S.n_o[c] - P.an.dba) + P.an.biref};
+        float ns, nf;
+        biax_wave_indices(B, BF, dir_a, ns, nf);
+        n_a[c] = nf;
+        n_b[c] = ns;
+        if (!inside) {
+            biax_wave_indices(B, BF, dir_b, ns, nf);
+            n_b[c] = ns;
+        }
+    }
+    const bool mode_a = inside && !is_extra;
+    const float n_med0 = mode_a ? n_a[0] : n_b[0];
+    const float n1 = inside ? n_med0 : 1.0f;
+    const float n2 = inside ? 1.0f : n_med0;
+    const float eta = div_rn(n1, n2);
+    const float sin2_t = eta * eta * one_m_c2;
+
+    V3 new_k, new_s;
+    bool new_inside = inside;
+    int upd = -1;
+    uint32_t match_bits = 0u;
+    if (sin2_t > 1.0f) {
+        // apply_tir_bounce, scalar arm (tir.rs:56-160)
+        if (EVENTS) {
+            ++S.n_tir;
+            S.fh = fnv(fnv(S.fh, (uint32_t)facet), 1u);
+        }
+#pragma unroll
+        for (int c = 0; c < NCH; ++c) {
+            const float nm = mode_a ? n_a[c] : n_b[c];
+            const float n1k = inside ? nm : 1.0f, n2k = inside ? 1.0f : nm;
+            const float ek = div_rn(n1k, n2k);
+            const float s2k = ek * ek * one_m_c2;
+            if (s2k > 1.0f) {
+                float cd, sd;
+                tir_retardation(n1k, cos_i, sin_i, cd, sd);
+                S.st[c] = mueller_tir(cd, sd, S.st[c]);
+            } else {
+                float ru;
+                S.st[c] = fresnel_reflect(n1k, n2k, cos_i, sqrt_rn(fmaxf(1.0f - s2k, 0.0f)), S.st[c], ru);
+                S.path_pdf[c] *= ru;
+            }
+        }
+        new_k = reflect_dir(k, normal);
+        new_s = inside ? biax_mode_poynting(hero, BF, new_k, is_extra) : new_k;
+    } else {
+        // apply_partial_fresnel_bounce (dispatch.rs:143-262): no polarisation-
+        // weighted selection for a biaxial stone, so p_o = 0.5
+        const bool entering = !inside;
+        const float msr = unit_draw(S.seed, bounce, BIREFRINGENT_SPLIT_STREAM);
+        const bool use_extra = entering ? (msr < (1.0f - 0.5f)) : is_extra;
+        float rs_h, rp_h;
+        fresnel_rs_rp(n1, n2, cos_i, sqrt_rn(fmaxf(1.0f - sin2_t, 0.0f)), rs_h, rp_h);
+        const float r_unpol = clampf(0.5f * fma_(rp_h, rp_h, rs_h * rs_h), R_UNPOL_SELECT_MIN, R_UNPOL_SELECT_MAX);
+        if (unit_draw(S.seed, bounce, FRESNEL_BRANCH_STREAM) < r_unpol) {
+            if (EVENTS) {
+                ++S.n_refl;
+                S.fh = fnv(fnv(S.fh, (uint32_t)facet), 2u);
+            }
+            const float sc = div_rn(1.0f, r_unpol);
+#pragma unroll
+            for (int c = 0; c < NCH; ++c) {
+                const float nm = mode_a ? n_a[c] : n_b[c];
+                const float n1k = inside ? nm : 1.0f, n2k = inside ? 1.0f : nm;
+                const float ek = div_rn(n1k, n2k);
+                const float s2k = ek * ek * one_m_c2;
+                float4 st;
+                if (s2k > 1.0f) {
+                    float cd, sd;
+                    tir_retardation(n1k, cos_i, sin_i, cd, sd);
+                    st = mueller_tir(cd, sd, S.st[c]);
+                } else {
+                    float ru;
+                    st = fresnel_reflect(n1k, n2k, cos_i, sqrt_rn(fmaxf(1.0f - s2k, 0.0f)), S.st[c], ru);
+                    S.path_pdf[c] *= ru;
+                }
+                S.st[c] = scale4(st, sc);
+            }
+            new_k = reflect_dir(k, normal);
+            new_s = inside ? biax_mode_poynting(hero, BF, new_k, is_extra) : new_k;
+        } else {
+            if (EVENTS) {
+                ++S.n_refr;
+                S.fh = fnv(fnv(S.fh, (uint32_t)facet), 3u);
+            }
+            const float n2d = entering ? (use_extra ? n_b[0] : n_a[0]) : n2;
+            const float eta_dir = div_rn(n1, n2d);
+            const float s2d = fminf(eta_dir * eta_dir * one_m_c2, 1.0f);
+            const V3 refr = refract_dir(k, normal, eta_dir, cos_i, sqrt_rn(fmaxf(1.0f - s2d, 0.0f)));
+            const V3 final_refr = entering ? biax_mode_poynting(hero, BF, refr, use_extra) : refr;
+            const float inv_t = div_rn(1.0f, 1.0f - r_unpol);
+            uint32_t live = 0u, match_mask = 0u;
+#pragma unroll
+            for (int c = 0; c < NCH; ++c) {
+                const float4 orig = S.st[c];
+                const float nm = mode_a ? n_a[c] : n_b[c];
+                const float n1k = inside ? nm : 1.0f;
+                const float n2k = (entering && !use_extra) ? n_a[c] : (inside ? 1.0f : nm);
+                const float r = div_rn(n1k, n2k);
+                const float s2k = r * r * one_m_c2;
+                if (s2k > 1.0f) {
+                    S.st[c] = scale4(S.st[c], 0.0f);
+                    S.path_pdf[c] = 0.0f;
+                    continue;
+                }
+                const float ctk = sqrt_rn(fmaxf(1.0f - s2k, 0.0f));
+                const V3 rk = refract_dir(k, normal, div_rn(n1k, n2k), cos_i, ctk);
+                V3 fk = rk;
+                if (entering) {
+                    const Biax B{S.n_o[c] - P.an.dba, S.n_o[c], (S.n_o[c] - P.an.dba) + P.an.biref};
+                    fk = biax_mode_poynting(B, BF, rk, use_extra);
+                }
+                const bool match = dot(fk, final_refr) >= DIRECTION_MATCH_COS_TOL;
+                // compute_channel_transmission (exit_split.rs:69-101)
+                const float num = 2.0f * n1k * cos_i;
+                const float t_s = div_rn(num, fma_(n2k, ctk, n1k * cos_i));
+                const float t_p = div_rn(num, fma_(n1k, ctk, n2k * cos_i));
+                const float factor = div_rn(n2k * ctk, fmaxf(n1k * cos_i, 1e-6f));
+                const float ts2 = t_s * t_s * factor, tp2 = t_p * t_p * factor;
+                const float4 tr =
+                    scale4(mueller_fresnel(midpoint(ts2, tp2), 0.5f * (ts2 - tp2), t_s * t_p * factor, orig), inv_t);
+                float a_, b_;
+                fresnel_rs_rp(n1k, n2k, cos_i, ctk, a_, b_);
+                const float ruk = clampf(0.5f * fma_(b_, b_, a_ * a_), R_UNPOL_PDF_MIN, R_UNPOL_PDF_MAX);
+                if (match) {
+                    S.st[c] = tr;
+                    S.path_pdf[c] *= 1.0f - ruk;
+                    match_mask |= 1u << c;
+                    match_bits |= 1u << c;
+                } else {
+                    const float prefix = S.path_pdf[c];
+                    S.st[c] = scale4(S.st[c], 0.0f);
+                    S.path_pdf[c] = prefix * (1.0f - ruk);
+                    if (inside && fmaxf(orig.x, 0.0f) > 0.0f)
+                        split_probe<EVENTS, ISO, EXTRA>(P, planes, S, facet, hit, c, rk, fmaxf(tr.x, 0.0f));
+                }
+                if (!inside) channel_dir_put(c, fk);
+                live |= 1u << c;
+            }
+            if (!inside) narrow_compat(S.compat, match_mask, live);
+            new_k = refr;
+            new_s = final_refr;
+            new_inside = !inside;
+            if (entering) upd = use_extra ? 1 : 0;
+        }
+    }
+    // dispatch_bounce tail (transport/bounce.rs:200-238)
+    const bool was_internal_reflection = inside && upd < 0 && new_inside == inside;
+    S.dir = new_s;
+    S.org = hit + new_s * 1e-4f;
+    S.k = new_k;
+    S.inside = new_inside;
+    S.last_facet = facet;
+    if (upd >= 0) S.is_extra = upd == 1;
+    if (was_internal_reflection) S.is_extra = unit_draw(S.seed, bounce, MODE_COUPLING_STREAM) >= 0.5f;
+    if (EVENTS) S.fh = fnv(fnv(S.fh, 16u + (S.is_extra ? 1u : 0u)), 256u + match_bits);
+    if (bounce == 0 && !new_inside && P.surface_glare < 1.0f) {
+#pragma unroll
+        for (int c = 0; c < NCH; ++c) S.st[c] = scale4(S.st[c], P.surface_glare);
+    }
+    if (bounce > 4) {
+        float mx = 0.0f;
+#pragma unroll
+        for (int c = 0; c < NCH; ++c) mx = fmaxf(mx, fmaxf(S.st[c].x, 0.0f));
+        const float q = clampf(mx, 0.05f, 1.0f);
+        if (unit_draw(S.seed, bounce, RUSSIAN_ROULETTE_STREAM) > q) return STEP_ROULETTE;
+        const float inv_q = div_rn(1.0f, q);
+#pragma unroll
+        for (int c = 0; c < NCH; ++c) S.st[c] = scale4(S.st[c], inv_q);
+#pragma unroll
+        for (int c = 0; c < NCH; ++c) S.split[c] = div_rn(S.split[c], q);
+    }
+    return STEP_CONTINUE;
+}
+
 // One iteration of the bounce loop (bounce_body.rs:361-707): intersection, and
 // either the escape lookup (added to the split staging) or the facet event.
 // apply_russian_roulette (transport/bounce.rs:355-375): false when the path dies.
@@ -1094,7 +1342,7 @@ __device__ __forceinline__ void path_miss(const FilmParams& P, PathState& S) {
 // frosted facets, volume scattering) are compiled in; the persistent film
 // kernel is also built without them, so a stone that uses none runs the
 // lean bounce.
-template <bool EVENTS, bool ISO = false, bool ABSORB = false, bool ANISO = false, bool EXTRA = true>
+template <bool EVENTS, bool ISO = false, bool ABSORB = false, int ANISO = 0, bool EXTRA = true>
 __device__ __forceinline__ int path_bounce(const FilmParams& P, const float4* __restrict__ planes,
                                            PathState& S) {
     const uint32_t bounce = S.bounce;
@@ -1115,6 +1363,7 @@ __device__ __forceinline__ int path_bounce(const FilmParams& P, const float4* __
         const int r = scatter_step<EVENTS, ABSORB>(P, S, t_hit);
         if (r >= 0) return r;
     }
+    if (ANISO == 2) return facet_biax<EVENTS, ISO, ABSORB, EXTRA>(P, planes, S, t_hit, facet);
     if (ANISO) return facet_aniso<EVENTS, ISO, ABSORB, EXTRA>(P, planes, S, t_hit, facet);
     // facet_event (bounce_body.rs:611-707)
     const V3 k = S.dir;
@@ -1356,12 +1605,12 @@ __device__ __forceinline__ void path_finish(const FilmParams& P, const PathState
 }
 
 // trace_spectral_ray (transport/mod.rs:118) for one explicit ray, start to end.
-template <bool EVENTS, bool ABSORB, bool ANISO>
+template <bool EVENTS, bool ABSORB, int ANISO>
 __device__ __forceinline__ void trace_path(const FilmParams& P, const float4* __restrict__ planes,
                                            V3 org, V3 dir, const uint32_t seed, const float hero,
                                            PathOut& out, uint32_t* counts) {
     PathState S;
-    path_init<ABSORB>(P, org, dir, seed, hero, S);
+    path_init<ABSORB, ANISO>(P, org, dir, seed, hero, S);
     uint32_t bounces = P.max_bounces, term = 3u;
     bool escaped = false;
     for (uint32_t b = 0; b < P.max_bounces; ++b) {
@@ -1433,7 +1682,7 @@ __device__ __forceinline__ void stage_planes(const FilmParams& P, const float4* 
 // (paths run 1..max_bounces iterations, mostly 2-6), and the per-path results
 // are the same as film_transport_kernel's because a path is a pure function
 // of its (pixel, sample) index.
-template <bool ISO, bool ABSORB, bool ANISO, bool EXTRA>
+template <bool ISO, bool ABSORB, int ANISO, bool EXTRA>
 __global__ __launch_bounds__(kBlock) IXG_WAVES_ATTR void film_persistent_kernel(const FilmParams P,
                                                                  const float4* __restrict__ planes,
                                                                  float* __restrict__ out_xyz,
@@ -1476,7 +1725,7 @@ __global__ __launch_bounds__(kBlock) IXG_WAVES_ATTR void film_persistent_kernel(
                     float hero;
                     camera_sample(P, (uint32_t)(p / P.spp), P.sample_offset + (uint32_t)(p % P.spp),
                                   org, dir, seed, hero);
-                    path_init<ABSORB>(P, org, dir, seed, hero, S);
+                    path_init<ABSORB, ANISO>(P, org, dir, seed, hero, S);
                     active = true;
                 } else {
                     done = true;
@@ -1506,7 +1755,7 @@ __global__ __launch_bounds__(kBlock) IXG_WAVES_ATTR void film_persistent_kernel(
 }
 
 // Diagnostics: per-path XYZ + integer side channel.
-template <bool ABSORB, bool ANISO>
+template <bool ABSORB, int ANISO>
 __global__ __launch_bounds__(kBlock) void film_events_kernel(const FilmParams P,
                                                              const float4* __restrict__ planes,
                                                              float* __restrict__ out_xyz,
@@ -1549,7 +1798,7 @@ __global__ __launch_bounds__(kBlock) void film_events_kernel(const FilmParams P,
 
 // Explicit rays (the reference's trace_spectral_ray entry): ray i starts at
 // rays[i] = {ox, oy, oz, dx, dy, dz, hero_rand, seed bits}.
-template <bool ABSORB, bool ANISO>
+template <bool ABSORB, int ANISO>
 __global__ __launch_bounds__(kBlock) void film_rays_kernel(const FilmParams P,
                                                            const float4* __restrict__ planes,
                                                            const float* __restrict__ rays,
@@ -1578,7 +1827,7 @@ extern "C" int ixg_lean_blocks_per_cu(void);
 // instantiation (3 blocks per CU) times the CUs; the uniaxial ones fit 2 per
 // CU, and blocks beyond residency would only start once the path counter is
 // spent.  Residency per CU is a property of the compiled kernel, read once.
-template <bool ISO, bool ABSORB, bool ANISO, bool EXTRA>
+template <bool ISO, bool ABSORB, int ANISO, bool EXTRA>
 hipError_t launch_persistent(const FilmParams* P, const float4* d_planes, float* d_xyz, uint32_t* d_counter,
                              uint32_t grid_blocks, hipStream_t stream) {
     static const int per_cu = [] {
@@ -1597,7 +1846,7 @@ hipError_t launch_persistent(const FilmParams* P, const float4* d_planes, float*
 
 // The persistent kernel's instantiations for one (uniaxial, extra-features)
 // pair, chosen at launch by the environment class and the absorbing flag.
-template <bool ANISO, bool EXTRA>
+template <int ANISO, bool EXTRA>
 hipError_t launch_persistent_subset(const FilmParams* P, const float4* d_planes, float* d_xyz, uint32_t* d_counter,
                                     uint32_t grid_blocks, hipStream_t stream, bool iso) {
     if (iso) {
@@ -1609,7 +1858,7 @@ hipError_t launch_persistent_subset(const FilmParams* P, const float4* d_planes,
 }
 
 // The diagnostic kernels (per-path events, explicit rays) for one material class.
-template <bool ANISO>
+template <int ANISO>
 hipError_t launch_events_subset(const FilmParams* P, const float4* d_planes, float* d_xyz, ixg_path_events* d_ev,
                                 unsigned long long* d_totals, hipStream_t stream) {
     const uint64_t blocks = (P->n_paths + kBlock - 1) / kBlock;
@@ -1622,7 +1871,7 @@ hipError_t launch_events_subset(const FilmParams* P, const float4* d_planes, flo
     return hipGetLastError();
 }
 
-template <bool ANISO>
+template <int ANISO>
 hipError_t launch_rays_subset(const FilmParams* P, const float4* d_planes, const float* d_rays, uint32_t n_rays,
                               float* d_xyz, ixg_path_events* d_ev, hipStream_t stream) {
     const uint32_t blocks = (n_rays + kBlock - 1) / kBlock;

@@ -43,6 +43,12 @@ extern "C" hipError_t ixg_launch_persistent_lean_iso(IXG_PERSISTENT_ARGS);
 extern "C" hipError_t ixg_launch_persistent_lean_aniso(IXG_PERSISTENT_ARGS);
 extern "C" hipError_t ixg_launch_persistent_extra_iso(IXG_PERSISTENT_ARGS);
 extern "C" hipError_t ixg_launch_persistent_extra_aniso(IXG_PERSISTENT_ARGS);
+extern "C" hipError_t ixg_launch_persistent_lean_biax(IXG_PERSISTENT_ARGS);
+extern "C" hipError_t ixg_launch_persistent_extra_biax(IXG_PERSISTENT_ARGS);
+extern "C" hipError_t ixg_launch_events_biax(const ixg::FilmParams* P, const float4* d_planes, float* d_xyz,
+                                             ixg_path_events* d_ev, unsigned long long* d_totals, hipStream_t stream);
+extern "C" hipError_t ixg_launch_rays_biax(const ixg::FilmParams* P, const float4* d_planes, const float* d_rays,
+                                           uint32_t n_rays, float* d_xyz, ixg_path_events* d_ev, hipStream_t stream);
 extern "C" hipError_t ixg_launch_events_iso(const ixg::FilmParams* P, const float4* d_planes, float* d_xyz,
                                             ixg_path_events* d_ev, unsigned long long* d_totals, hipStream_t stream);
 extern "C" hipError_t ixg_launch_events_aniso(const ixg::FilmParams* P, const float4* d_planes, float* d_xyz,
@@ -57,6 +63,7 @@ extern "C" hipError_t ixg_launch_rays_aniso(const ixg::FilmParams* P, const floa
 extern "C" hipError_t ixg_launch_film_events(const ixg::FilmParams* P, const float4* d_planes,
                                              float* d_xyz, ixg_path_events* d_ev,
                                              unsigned long long* d_totals, hipStream_t stream) {
+    if (P->an.biax) return ixg_launch_events_biax(P, d_planes, d_xyz, d_ev, d_totals, stream);
     return P->an.on ? ixg_launch_events_aniso(P, d_planes, d_xyz, d_ev, d_totals, stream)
                     : ixg_launch_events_iso(P, d_planes, d_xyz, d_ev, d_totals, stream);
 }
@@ -72,6 +79,7 @@ extern "C" hipError_t ixg_launch_reduce(const float* d_xyz, uint32_t spp, uint32
 extern "C" hipError_t ixg_launch_rays(const ixg::FilmParams* P, const float4* d_planes, const float* d_rays,
                                       uint32_t n_rays, float* d_xyz, ixg_path_events* d_ev,
                                       hipStream_t stream) {
+    if (P->an.biax) return ixg_launch_rays_biax(P, d_planes, d_rays, n_rays, d_xyz, d_ev, stream);
     return P->an.on ? ixg_launch_rays_aniso(P, d_planes, d_rays, n_rays, d_xyz, d_ev, stream)
                     : ixg_launch_rays_iso(P, d_planes, d_rays, n_rays, d_xyz, d_ev, stream);
 }
@@ -84,6 +92,9 @@ extern "C" hipError_t ixg_launch_film_persistent(const ixg::FilmParams* P, const
     const int iso = P->env.model != ixg::ENV_STUDIO && P->env.model != ixg::ENV_ASET && P->env.model != ixg::ENV_HDR;
     // the rarely used features, the HDR panorama among them, run the EXTRA kernels
     const bool extra = P->frosted || P->sigma_s > 0.0f || P->edge_radius > 0.0f || P->env.model == ixg::ENV_HDR;
+    if (P->an.biax)
+        return extra ? ixg_launch_persistent_extra_biax(P, d_planes, d_xyz, d_counter, grid_blocks, stream, iso)
+                     : ixg_launch_persistent_lean_biax(P, d_planes, d_xyz, d_counter, grid_blocks, stream, iso);
     if (extra)
         return P->an.on ? ixg_launch_persistent_extra_aniso(P, d_planes, d_xyz, d_counter, grid_blocks, stream, iso)
                         : ixg_launch_persistent_extra_iso(P, d_planes, d_xyz, d_counter, grid_blocks, stream, iso);

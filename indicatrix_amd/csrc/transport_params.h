@@ -53,6 +53,8 @@ struct AbsorbParams {
     float a1[3], a2[3], cn[3];
     AbsBand o[8];
     AbsBand e[8];
+    int32_t n_b;   // trichroic biaxial stones: the beta bands
+    AbsBand b[8];
 };
 
 // A uniaxial material (RayMaterialContext::is_anisotropic): the ordinary
@@ -65,6 +67,13 @@ struct AnisoParams {
     int32_t e_kind;
     float e_disp[6];
     float c_axis[3];  // as given
+    // a biaxial stone (biaxial_delta_beta_alpha is Some): n_beta = the
+    // dispersion curve, n_alpha = n_beta - dba, n_gamma = n_alpha + biref;
+    // the principal frame bx / by / bz (gamma along the normalised axis)
+    int32_t biax;
+    int32_t trichroic;  // AbsorptionTensor::beta_ray is Some
+    float dba;
+    float bx[3], by[3], bz[3];
 };
 
 struct FilmParams {

@@ -43,6 +43,10 @@ class GemMaterial:
         self.absorption_path_scale = absorption_path_scale
         self.bands_o = []
         self.bands_e = []
+        # GemMaterial::biaxial_delta_beta_alpha (None = not biaxial) and
+        # AbsorptionTensor::beta_ray (None = no trichroic beta band set)
+        self.biaxial_delta_beta_alpha = None
+        self.bands_beta = None
 
     def to_abi(self):
         m = abi.Material()
@@ -60,6 +64,14 @@ class GemMaterial:
         for i, (c, w, p, s) in enumerate(self.bands_e):
             m.bands_e[i] = abi.Band(c, w, p, s)
         m.absorption_path_scale = self.absorption_path_scale
+        if self.biaxial_delta_beta_alpha is not None:
+            m.biaxial = 1
+            m.biaxial_delta_beta_alpha = self.biaxial_delta_beta_alpha
+        if self.bands_beta is not None:
+            m.has_beta_bands = 1
+            m.n_bands_beta = len(self.bands_beta)
+            for i, (c, w, p, s) in enumerate(self.bands_beta):
+                m.bands_beta[i] = abi.Band(c, w, p, s)
         return m
 
     @staticmethod
@@ -87,6 +99,32 @@ class GemMaterial:
                      energy(457.0, 12000.0, 0.433)]
         m.bands_e = [energy(612.0, 7466.0, 0.371), energy(729.0, 4955.0, 1.03),
                      energy(467.0, 11169.0, 0.524)]
+        return m
+
+    @staticmethod
+    def alexandrite():
+        """Alexandrite (IX/optics/materials/zircon_through_topaz.rs:105-137):
+        orthorhombic, biaxial positive; Sellmeier-3 beta curve, n_gamma - n_alpha
+        = 0.0076, n_beta - n_alpha = 0.001951, gamma along +Y, and the trichroic
+        band sets alpha / beta / gamma (AbsorptionTensor::biaxial)."""
+        m = GemMaterial("Alexandrite", abi.DISP_SELLMEIER3, [0.78522, 1.21202, 16.81, 0.0, 0.01262, 1000.0],
+                        birefringence_delta=0.0076, cubic=False)
+        m.bands_o = [(565.0, 40.0, 1.9, 0), (420.0, 32.0, 1.8, 0)]
+        m.bands_beta = [(560.0, 40.0, 0.65, 0), (422.0, 32.0, 3.5, 0)]
+        m.bands_e = [(595.0, 40.0, 3.1, 0), (430.0, 32.0, 1.5, 0)]
+        m.biaxial_delta_beta_alpha = 0.001951
+        return m
+
+    @staticmethod
+    def topaz():
+        """Topaz (IX/optics/materials/zircon_through_topaz.rs:211-233): Cauchy
+        beta curve, n_gamma - n_alpha = 0.008, n_beta - n_alpha = 0.0024,
+        gamma along +Y, one isotropic band (no beta_ray)."""
+        m = GemMaterial("Topaz", abi.DISP_CAUCHY, [1.614872, 0.004273, 0.0],
+                        birefringence_delta=0.0080, cubic=False)
+        m.bands_o = [(620.0, 70.0, 1.6, 0)]
+        m.bands_e = [(620.0, 70.0, 1.6, 0)]
+        m.biaxial_delta_beta_alpha = 0.0024
         return m
 
     @staticmethod

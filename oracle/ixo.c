@@ -813,7 +813,9 @@ typedef struct {
     env_t env;
     float surface_glare;
     int absorbing;      /* any absorption band (o or e) */
-    int aniso;          /* RayMaterialContext::is_anisotropic (uniaxial here) */
+    int aniso;          /* RayMaterialContext::is_anisotropic */
+    int biax;           /* genuinely biaxial (biaxial_delta_beta_alpha is Some) */
+    int trichroic;      /* biaxial with a beta_ray band set: tensor (o, beta, e) */
     v3 c_raw;           /* material c_axis as given */
     v3 ta1, ta2, tcn;   /* AbsorptionTensor3 axes: basis of normalize_or_zero(c_axis) */
     const uint8_t* finish; /* FacetFinish per plane, NULL = all polished */
@@ -909,6 +911,7 @@ typedef struct {
     float path_pdf[NCH];
     float n_o[NCH];
     float alpha_o[NCH], alpha_e[NCH]; /* spectral_absorption of o / e bands */
+    float alpha_b[NCH];               /* of the beta bands (trichroic biaxial) */
     uint8_t compat[NCH];
     v3 org, dir, k;     /* ray origin, ray (Poynting) direction S, wave normal k */
     int inside;
@@ -995,6 +998,7 @@ static void apply_absorption(const scene_t* S, path_t* P, float path_len) {
 }
 
 #include "ixo_uniaxial.h"
+#include "ixo_biaxial.h"
 
 /* sample_henyey_greenstein_direction (scattering/hg.rs:636-651); the
  * azimuth's sine and cosine correctly rounded. */
@@ -1173,6 +1177,8 @@ static void fresnel_rs_rp(float n1, float n2, float cos_i, float cos_t, float* r
 
 static void facet_event_aniso(const scene_t* S, path_t* P, uint32_t bounce, uint32_t seed, float t_hit,
                               int32_t facet, ixg_path_events* ev, int* stop);
+static void facet_event_biax(const scene_t* S, path_t* P, uint32_t bounce, uint32_t seed, float t_hit,
+                             int32_t facet, ixg_path_events* ev, int* stop);
 
 /* shading_normal_near_edge (RT/intersect.rs:196-233): within rounding_radius
  * of the nearest other plane the facet normal eases (smoothstep) toward the
@@ -1203,6 +1209,10 @@ void ixo_shading_normal_near_edge(const ixg_facet_plane* planes, uint32_t n, con
 
 static void facet_event(const scene_t* S, path_t* P, uint32_t bounce, uint32_t seed, float t_hit,
                         int32_t facet, ixg_path_events* ev, int* stop) {
+    if (S->biax) {
+        facet_event_biax(S, P, bounce, seed, t_hit, facet, ev, stop);
+        return;
+    }
     if (S->aniso) {
         facet_event_aniso(S, P, bounce, seed, t_hit, facet, ev, stop);
         return;
@@ -1938,6 +1948,271 @@ static void facet_event_aniso(const scene_t* S, path_t* P, uint32_t bounce, uint
     }
 }
 
+/* One facet event of a biaxial material (biaxial_delta_beta_alpha is Some):
+ * the bounce body's facet_event with compute_bounce_refraction_geometry's
+ * biaxial block (refraction/geometry.rs:199-270, 306-425: mode A = the fast
+ * root, mode B = the slow root, per channel at the hero's wave normals,
+ * resolved at entry by two fixed-point steps per mode), the scalar TIR and
+ * partial-Fresnel arms (tir.rs, dispatch.rs:143-262 with no
+ * polarisation-weighted selection: p_o = 0.5), the refract arm's biaxial
+ * walk-off per channel (reflect_refract.rs:122-330), the assigned mode's
+ * pleochroic absorption (absorption.rs:224-249 through
+ * assigned_mode_e_field) and the 50/50 internal re-coupling draw
+ * (transport/bounce.rs:439-470). */
+static void facet_event_biax(const scene_t* S, path_t* P, uint32_t bounce, uint32_t seed, float t_hit,
+                             int32_t facet, ixg_path_events* ev, int* stop) {
+    const ixg_material* M = S->mat;
+    v3 hit_point = vadd(P->org, vmul(P->dir, t_hit));
+    v3 normal = shading_normal_near_edge(S->planes, S->n_planes, hit_point, facet, vfrom(S->planes[facet].normal),
+                                         M->edge_rounding_radius);
+    v3 cur_plane = vnormalize_or_zero(vcross(P->k, normal));
+    if (P->has_prev && vlen_sq(cur_plane) > 1e-6f && vlen_sq(P->prev_plane) > 1e-6f) {
+        float cos_psi = clampf(vdot(P->prev_plane, cur_plane), -1.0f, 1.0f);
+        float sin_psi = vdot(vcross(P->prev_plane, cur_plane), vnormalize_or_zero(P->k));
+        mat4 R = mueller_frame_rotation(atan2_cr(sin_psi, cos_psi));
+        for (int c = 0; c < NCH; ++c) P->st[c] = st_apply(&R, P->st[c]);
+    }
+    P->prev_plane = cur_plane;
+    P->has_prev = 1;
+    const float dba = M->biaxial_delta_beta_alpha, bd = M->birefringence_delta;
+    biax_t ind[NCH]; /* GemMaterial::biaxial_indicatrix per channel (materials/optics.rs:16-27) */
+    for (int c = 0; c < NCH; ++c) {
+        float nb = P->n_o[c], na = nb - dba;
+        ind[c] = biax_from_gamma_axis(na, nb, na + bd, S->c_raw);
+    }
+    if (P->inside && S->absorbing) {
+        /* channel_absorption_alphas_assigned (absorption.rs:224-249) */
+        v3 e = biax_assigned_e(&ind[0], P->k, P->is_extra);
+        float l0 = vdot(S->ta1, e), l1 = vdot(S->ta2, e), l2 = vdot(S->tcn, e);
+        float scaled = t_hit * M->absorption_path_scale;
+        float arg[NCH], tr[NCH];
+        for (int c = 0; c < NCH; ++c) {
+            float ab = S->trichroic ? P->alpha_b[c] : P->alpha_o[c];
+            float q = (P->alpha_o[c] * (l0 * l0) + ab * (l1 * l1)) + P->alpha_e[c] * (l2 * l2);
+            arg[c] = -q * scaled;
+        }
+        exp_lane8(arg, tr);
+        for (int c = 0; c < NCH; ++c) P->st[c] = st_scale(P->st[c], tr[c]);
+    }
+    if (P->inside) normal = vneg(normal);
+
+    const v3 k = P->k;
+    const int inside = P->inside, is_extra = P->is_extra;
+    const float cos_i = clampf(vdot(vneg(k), normal), 0.0f, 1.0f);
+    const float one_m_c2 = fmaf(cos_i, -cos_i, 1.0f);
+    const float sin_i = sqrtf(maxf(one_m_c2, 0.0f));
+    /* hero_biaxial_wave_dirs / per_channel_biaxial_indices (geometry.rs:216-270) */
+    v3 dir_a = k, dir_b = k;
+    if (!inside) {
+        (void)biax_resolve_entry(&ind[0], k, normal, cos_i, P->n_o[0], 0, &dir_a);
+        (void)biax_resolve_entry(&ind[0], k, normal, cos_i, P->n_o[0], 1, &dir_b);
+    }
+    float n_a[NCH], n_b[NCH], n_med[NCH];
+    for (int c = 0; c < NCH; ++c) {
+        float ns, nf;
+        biax_wave_indices(&ind[c], dir_a, &ns, &nf);
+        n_a[c] = nf;
+        biax_wave_indices(&ind[c], dir_b, &ns, &nf);
+        n_b[c] = ns;
+        n_med[c] = (inside && !is_extra) ? n_a[c] : n_b[c];
+    }
+    const float n1 = inside ? n_med[0] : 1.0f, n2 = inside ? 1.0f : n_med[0];
+    const float eta = n1 / n2;
+    const float sin2_t = eta * eta * one_m_c2;
+    float n1_ch[NCH], n2_ch[NCH], sin2_t_ch[NCH];
+    for (int c = 0; c < NCH; ++c) {
+        n1_ch[c] = inside ? n_med[c] : 1.0f;
+        n2_ch[c] = inside ? 1.0f : n_med[c];
+        float ek = n1_ch[c] / n2_ch[c];
+        sin2_t_ch[c] = ek * ek * one_m_c2;
+    }
+
+    v3 new_k, new_s;
+    int new_inside = inside, upd = -1;
+    uint32_t match_bits = 0u;
+    if (sin2_t > 1.0f) {
+        /* apply_tir_bounce, scalar arm (refraction/tir.rs:56-160) */
+        if (ev) ev->n_tir++;
+        ev_facet(ev, facet, 1);
+        for (int c = 0; c < NCH; ++c) {
+            if (sin2_t_ch[c] > 1.0f) {
+                mat4 Mt = mueller_tir_retardation(tir_phase_delta(n1_ch[c], cos_i, sin_i));
+                P->st[c] = st_apply(&Mt, P->st[c]);
+            } else {
+                float ct = sqrtf(maxf(1.0f - sin2_t_ch[c], 0.0f));
+                float rs, rp;
+                fresnel_rs_rp(n1_ch[c], n2_ch[c], cos_i, ct, &rs, &rp);
+                mat4 Mr = mueller_fresnel_reflection(rs, rp);
+                P->st[c] = st_apply(&Mr, P->st[c]);
+                P->path_pdf[c] *= clampf(0.5f * fmaf(rp, rp, rs * rs), R_UNPOL_PDF_MIN, R_UNPOL_PDF_MAX);
+            }
+        }
+        new_k = vsub(k, vmul(normal, 2.0f * vdot(k, normal)));
+        new_s = inside ? biax_mode_poynting(&ind[0], new_k, is_extra) : new_k;
+    } else {
+        /* apply_partial_fresnel_bounce (refraction/dispatch.rs:143-262) */
+        const int entering = !inside;
+        float msr = unit_draw(seed, bounce, BIREFRINGENT_SPLIT_STREAM);
+        int use_extra = entering ? (msr < (1.0f - 0.5f)) : is_extra;
+        float cth = sqrtf(maxf(1.0f - sin2_t, 0.0f));
+        float rs, rp;
+        fresnel_rs_rp(n1, n2, cos_i, cth, &rs, &rp);
+        float r_unpol = clampf(0.5f * fmaf(rp, rp, rs * rs), R_UNPOL_SELECT_MIN, R_UNPOL_SELECT_MAX);
+        float rng = unit_draw(seed, bounce, FRESNEL_BRANCH_STREAM);
+        if (rng < r_unpol) {
+            if (ev) ev->n_reflect++;
+            ev_facet(ev, facet, 2);
+            for (int c = 0; c < NCH; ++c) {
+                mat4 Mr;
+                if (sin2_t_ch[c] > 1.0f) {
+                    Mr = mueller_tir_retardation(tir_phase_delta(n1_ch[c], cos_i, sin_i));
+                } else {
+                    float ct = sqrtf(maxf(1.0f - sin2_t_ch[c], 0.0f));
+                    float a, b;
+                    fresnel_rs_rp(n1_ch[c], n2_ch[c], cos_i, ct, &a, &b);
+                    P->path_pdf[c] *= clampf(0.5f * fmaf(b, b, a * a), R_UNPOL_PDF_MIN, R_UNPOL_PDF_MAX);
+                    Mr = mueller_fresnel_reflection(a, b);
+                }
+                P->st[c] = st_scale(st_apply(&Mr, P->st[c]), 1.0f / r_unpol);
+            }
+            new_k = vsub(k, vmul(normal, 2.0f * vdot(k, normal)));
+            new_s = inside ? biax_mode_poynting(&ind[0], new_k, is_extra) : new_k;
+        } else {
+            if (ev) ev->n_refract++;
+            ev_facet(ev, facet, 3);
+            float n2d = entering ? (use_extra ? n_b[0] : n_a[0]) : n2;
+            float eta_dir = n1 / n2d;
+            float s2d = minf(eta_dir * eta_dir * one_m_c2, 1.0f);
+            float ctd = sqrtf(maxf(1.0f - s2d, 0.0f));
+            v3 refr = refract_dir(k, normal, eta_dir, cos_i, ctd);
+            v3 final_refr = entering ? biax_mode_poynting(&ind[0], refr, use_extra) : refr;
+            v3 dirs[NCH];
+            int has_dir[NCH], hero_match[NCH];
+            for (int c = 0; c < NCH; ++c) {
+                stokes orig = P->st[c];
+                float n1k = n1_ch[c];
+                float n2k = (entering && !use_extra) ? n_a[c] : n2_ch[c];
+                float r = n1k / n2k;
+                float s2k = r * r * one_m_c2;
+                has_dir[c] = 0;
+                hero_match[c] = 0;
+                if (s2k > 1.0f) {
+                    P->st[c] = st_scale(P->st[c], 0.0f);
+                    P->path_pdf[c] = 0.0f;
+                    continue;
+                }
+                float ctk = sqrtf(maxf(1.0f - s2k, 0.0f));
+                float ek = n1k / n2k;
+                v3 rk = refract_dir(k, normal, ek, cos_i, ctk);
+                v3 fk = entering ? biax_mode_poynting(&ind[c], rk, use_extra) : rk;
+                int match = vdot(fk, final_refr) >= DIRECTION_MATCH_COS_TOL;
+                if (match) match_bits |= 1u << c;
+                float num = 2.0f * n1k * cos_i;
+                float t_s = num / fmaf(n2k, ctk, n1k * cos_i);
+                float t_p = num / fmaf(n1k, ctk, n2k * cos_i);
+                mat4 T = mueller_fresnel_transmission(n1k, n2k, cos_i, ctk, t_s, t_p);
+                stokes transmitted = st_scale(st_apply(&T, orig), 1.0f / (1.0f - r_unpol));
+                float a, b;
+                fresnel_rs_rp(n1k, n2k, cos_i, ctk, &a, &b);
+                float ruk = clampf(0.5f * fmaf(b, b, a * a), R_UNPOL_PDF_MIN, R_UNPOL_PDF_MAX);
+                if (match) {
+                    P->st[c] = transmitted;
+                    P->path_pdf[c] *= 1.0f - ruk;
+                } else {
+                    float prefix = P->path_pdf[c];
+                    P->st[c] = st_scale(P->st[c], 0.0f);
+                    P->path_pdf[c] = prefix * (1.0f - ruk);
+                    if (inside && st_intensity(orig) > 0.0f)
+                        split_probe(S, P, hit_point, c, rk, st_intensity(transmitted), ev);
+                }
+                dirs[c] = fk;
+                has_dir[c] = 1;
+                hero_match[c] = match;
+            }
+            if (!inside) narrow_compat_dirs(P, dirs, has_dir, hero_match);
+            new_k = refr;
+            new_s = final_refr;
+            new_inside = !inside;
+            if (entering) upd = use_extra;
+        }
+    }
+    /* dispatch_bounce tail (transport/bounce.rs:200-238) */
+    int was_internal_reflection = inside && upd < 0 && new_inside == inside;
+    P->dir = new_s;
+    P->org = vadd(hit_point, vmul(new_s, 1e-4f));
+    P->k = new_k;
+    P->inside = new_inside;
+    if (upd >= 0) P->is_extra = upd;
+    if (was_internal_reflection) /* apply_internal_mode_coupling, biaxial: p_o = 0.5 */
+        P->is_extra = unit_draw(seed, bounce, MODE_COUPLING_STREAM) >= 0.5f;
+    if (ev) {
+        ev->facet_hash = (ev->facet_hash ^ (16u + (uint32_t)P->is_extra)) * 16777619u;
+        ev->facet_hash = (ev->facet_hash ^ (256u + match_bits)) * 16777619u;
+    }
+    if (bounce == 0 && !P->inside && S->surface_glare < 1.0f)
+        for (int c = 0; c < NCH; ++c) P->st[c] = st_scale(P->st[c], S->surface_glare);
+    if (bounce > 4) {
+        float mx = 0.0f;
+        for (int c = 0; c < NCH; ++c) mx = maxf(mx, st_intensity(P->st[c]));
+        float q = clampf(mx, 0.05f, 1.0f);
+        float rr = unit_draw(seed, bounce, RUSSIAN_ROULETTE_STREAM);
+        if (rr > q) { *stop = 1; return; }
+        for (int c = 0; c < NCH; ++c) P->st[c] = st_scale(P->st[c], 1.0f / q);
+        for (int c = 0; c < NCH; ++c) P->split[c] /= q;
+    }
+}
+
+static biax_t biax_from_abi(float na, float nb, float ng, const float* axes) {
+    biax_t B;
+    B.na = na;
+    B.nb = nb;
+    B.ng = ng;
+    B.ax = vfromr(axes);
+    B.ay = vfromr(axes + 3);
+    B.az = vfromr(axes + 6);
+    return B;
+}
+static void v3_out(v3 v, float* o) {
+    o[0] = v.x;
+    o[1] = v.y;
+    o[2] = v.z;
+}
+void ixo_biax_from_gamma_axis(const float gamma_axis[3], float axes_out[9]) {
+    biax_t B = biax_from_gamma_axis(1.0f, 1.0f, 1.0f, vfromr(gamma_axis));
+    v3_out(B.ax, axes_out);
+    v3_out(B.ay, axes_out + 3);
+    v3_out(B.az, axes_out + 6);
+}
+void ixo_biax_wave_indices(float na, float nb, float ng, const float axes[9], const float k[3], float out[2]) {
+    biax_t B = biax_from_abi(na, nb, ng, axes);
+    biax_wave_indices(&B, vfromr(k), &out[0], &out[1]);
+}
+void ixo_biax_eigen_d(float na, float nb, float ng, const float axes[9], const float k[3], int want_slow,
+                      float out[3]) {
+    biax_t B = biax_from_abi(na, nb, ng, axes);
+    v3_out(biax_eigen_d(&B, vfromr(k), want_slow), out);
+}
+void ixo_biax_d_to_e(float na, float nb, float ng, const float axes[9], const float d[3], float out[3]) {
+    biax_t B = biax_from_abi(na, nb, ng, axes);
+    v3_out(biax_d_to_e(&B, vfromr(d)), out);
+}
+void ixo_biax_mode_poynting(float na, float nb, float ng, const float axes[9], const float k[3], int want_slow,
+                            float out[3]) {
+    biax_t B = biax_from_abi(na, nb, ng, axes);
+    v3_out(biax_mode_poynting(&B, vfromr(k), want_slow), out);
+}
+float ixo_biax_resolve_entry(float na, float nb, float ng, const float axes[9], const float incident[3],
+                             const float normal[3], float cos_i, float n_seed, int want_slow, float dir_out[3]) {
+    biax_t B = biax_from_abi(na, nb, ng, axes);
+    v3 d;
+    float n = biax_resolve_entry(&B, vfromr(incident), vfromr(normal), cos_i, n_seed, want_slow, &d);
+    v3_out(d, dir_out);
+    return n;
+}
+void ixo_poynting_direction(const float k[3], const float e[3], float out[3]) {
+    v3_out(poynting_direction(vfromr(k), vfromr(e)), out);
+}
+
 /* Energy bookkeeping of one uniaxial interface solve, for the tests:
  * out = {R, T} as fractions of the incident Poynting flux along the normal.
  * kind 0: air -> crystal entry, incident polarisation pol 0 = s, 1 = p;
@@ -1997,8 +2272,12 @@ static int scene_build(const ixg_scene* sc, scene_t* S) {
     S->max_bounces = sc->max_bounces;
     float g = sc->environment.surface_glare;
     S->surface_glare = (g >= 1.0f || isnan(g)) ? 1.0f : (g > 0.0f ? g : 0.0f);
-    S->absorbing = sc->material->n_bands_o > 0 || sc->material->n_bands_e > 0;
     S->aniso = !sc->material->cubic && fabsf(sc->material->birefringence_delta) > 1e-4f;
+    S->biax = sc->material->biaxial != 0;
+    S->trichroic = S->biax && sc->material->has_beta_bands;
+    if (S->biax && !S->aniso) return 0; /* a biaxial stone is anisotropic; this form is declined */
+    S->absorbing = sc->material->n_bands_o > 0 || sc->material->n_bands_e > 0 ||
+                   (S->trichroic && sc->material->n_bands_beta > 0);
     S->c_raw = vfrom(sc->material->c_axis);
     S->tcn = vnormalize_or_zero(S->c_raw);
     stable_basis(S->tcn, &S->ta1, &S->ta2);
@@ -2050,6 +2329,7 @@ static void trace_core(const scene_t* S, v3 origin, v3 dir, uint32_t seed, float
         if (S->absorbing) {
             P.alpha_o[c] = band_sum(S->mat->bands_o, S->mat->n_bands_o, P.lambdas[c]);
             P.alpha_e[c] = band_sum(S->mat->bands_e, S->mat->n_bands_e, P.lambdas[c]);
+            if (S->trichroic) P.alpha_b[c] = band_sum(S->mat->bands_beta, S->mat->n_bands_beta, P.lambdas[c]);
         }
     }
     P.org = origin;

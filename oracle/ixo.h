@@ -44,6 +44,20 @@ void ixo_uniaxial_energy(int kind, float n_inc, float n_o, float n_e, const floa
 void ixo_uniaxial_entry_r(float n_o, float n_e, const float c[3], const float k[3], const float normal[3], int pol,
                           float out[4]);
 
+/* The biaxial indicatrix (optics/birefringence/biaxial.rs), for the tests.
+ * axes = the alpha, beta, gamma principal directions as three columns
+ * (axes[0..2] alpha, [3..5] beta, [6..8] gamma). */
+void ixo_biax_from_gamma_axis(const float gamma_axis[3], float axes_out[9]);
+void ixo_biax_wave_indices(float na, float nb, float ng, const float axes[9], const float k[3], float out[2]);
+void ixo_biax_eigen_d(float na, float nb, float ng, const float axes[9], const float k[3], int want_slow,
+                      float out[3]);
+void ixo_biax_d_to_e(float na, float nb, float ng, const float axes[9], const float d[3], float out[3]);
+void ixo_biax_mode_poynting(float na, float nb, float ng, const float axes[9], const float k[3], int want_slow,
+                            float out[3]);
+float ixo_biax_resolve_entry(float na, float nb, float ng, const float axes[9], const float incident[3],
+                             const float normal[3], float cos_i, float n_seed, int want_slow, float dir_out[3]);
+void ixo_poynting_direction(const float k[3], const float e[3], float out[3]);
+
 /* sample_studio_environment (RT/environment/mod.rs:653): observer = 0. */
 float ixo_sample_studio_environment(const float dir[3], float lambda_nm, int32_t preset,
                                     float exposure, float light_yaw, float light_pitch);

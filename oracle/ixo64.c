@@ -48,6 +48,13 @@
 #define ixo_accumulate ixo64_accumulate
 #define ixo_asc_planes ixo64_asc_planes
 #define ixo_band_evaluate ixo64_band_evaluate
+#define ixo_biax_from_gamma_axis ixo64_biax_from_gamma_axis
+#define ixo_biax_wave_indices ixo64_biax_wave_indices
+#define ixo_biax_eigen_d ixo64_biax_eigen_d
+#define ixo_biax_d_to_e ixo64_biax_d_to_e
+#define ixo_biax_mode_poynting ixo64_biax_mode_poynting
+#define ixo_biax_resolve_entry ixo64_biax_resolve_entry
+#define ixo_poynting_direction ixo64_poynting_direction
 #define ixo_blackbody ixo64_blackbody
 #define ixo_camera_new ixo64_camera_new
 #define ixo_cie_cmf ixo64_cie_cmf

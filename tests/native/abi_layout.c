@@ -34,7 +34,9 @@ int main(void) {
     F(ixg_material, cubic); F(ixg_material, has_e_dispersion); F(ixg_material, e_dispersion);
     F(ixg_material, n_bands_o); F(ixg_material, n_bands_e); F(ixg_material, bands_o); F(ixg_material, bands_e);
     F(ixg_material, absorption_path_scale); F(ixg_material, edge_rounding_radius);
-    F(ixg_material, scattering_sigma_s); F(ixg_material, biaxial); F(ixg_material, scattering_g); end();
+    F(ixg_material, scattering_sigma_s); F(ixg_material, biaxial); F(ixg_material, scattering_g);
+    F(ixg_material, biaxial_delta_beta_alpha); F(ixg_material, has_beta_bands); F(ixg_material, n_bands_beta);
+    F(ixg_material, bands_beta); end();
     S(ixg_environment); F(ixg_environment, preset); F(ixg_environment, exposure); F(ixg_environment, light_yaw);
     F(ixg_environment, light_pitch); F(ixg_environment, backdrop); F(ixg_environment, surface_glare);
     F(ixg_environment, head_shadow_deg); end();

@@ -119,6 +119,16 @@ def lib():
                                      C.c_int32, C.c_double, P(abi.FacetPlane), C.c_uint32]
         L.ixo_solid_outline.restype = C.c_int
         L.ixo_solid_outline.argtypes = [P(abi.FacetPlane), C.c_uint32, P(C.c_double), P(C.c_uint32)]
+        f9 = C.c_float * 9
+        for name in ("ixo_biax_eigen_d", "ixo_biax_mode_poynting"):
+            getattr(L, name).argtypes = [C.c_float, C.c_float, C.c_float, f9, f3, C.c_int, P(C.c_float)]
+        L.ixo_biax_from_gamma_axis.argtypes = [f3, P(C.c_float)]
+        L.ixo_biax_wave_indices.argtypes = [C.c_float, C.c_float, C.c_float, f9, f3, P(C.c_float)]
+        L.ixo_biax_d_to_e.argtypes = [C.c_float, C.c_float, C.c_float, f9, f3, P(C.c_float)]
+        L.ixo_biax_resolve_entry.restype = C.c_float
+        L.ixo_biax_resolve_entry.argtypes = [C.c_float, C.c_float, C.c_float, f9, f3, f3, C.c_float, C.c_float,
+                                             C.c_int, P(C.c_float)]
+        L.ixo_poynting_direction.argtypes = [f3, f3, P(C.c_float)]
         L.ixo_probe_shape.restype = C.c_int
         L.ixo_probe_shape.argtypes = [C.c_int32, P(C.c_uint32), P(C.c_uint32)]
         L.ixo_probe.restype = C.c_int
@@ -447,3 +457,55 @@ def solid_outline(planes):
     bare = C.c_uint32()
     ok = lib().ixo_solid_outline(planes, len(planes), C.byref(lw), C.byref(bare))
     return (lw.value, bare.value) if ok else None
+
+
+# ---- the biaxial indicatrix (optics/birefringence/biaxial.rs) ----------
+class Biaxial:
+    """One BiaxialIndicatrix: principal indices and the alpha / beta / gamma
+    principal directions (the columns of `axes`, a 3x3 array)."""
+
+    def __init__(self, na, nb, ng, axes):
+        self.n = (float(na), float(nb), float(ng))
+        self.axes = np.asarray(axes, np.float32).reshape(3, 3)  # rows = columns of the Mat3
+
+    @staticmethod
+    def from_gamma_axis(na, nb, ng, gamma_axis):
+        out = (C.c_float * 9)()
+        lib().ixo_biax_from_gamma_axis((C.c_float * 3)(*gamma_axis), out)
+        return Biaxial(na, nb, ng, np.array(out[:], np.float32))
+
+    def _ax(self):
+        return (C.c_float * 9)(*[float(v) for v in self.axes.reshape(-1)])
+
+    def wave_indices(self, k):
+        out = (C.c_float * 2)()
+        lib().ixo_biax_wave_indices(*self.n, self._ax(), (C.c_float * 3)(*k), out)
+        return np.float32(out[0]), np.float32(out[1])
+
+    def eigen_d(self, k, want_slow):
+        out = (C.c_float * 3)()
+        lib().ixo_biax_eigen_d(*self.n, self._ax(), (C.c_float * 3)(*k), int(want_slow), out)
+        return np.array(out[:], np.float32)
+
+    def d_to_e(self, d):
+        out = (C.c_float * 3)()
+        lib().ixo_biax_d_to_e(*self.n, self._ax(), (C.c_float * 3)(*d), out)
+        return np.array(out[:], np.float32)
+
+    def mode_poynting(self, k, want_slow):
+        out = (C.c_float * 3)()
+        lib().ixo_biax_mode_poynting(*self.n, self._ax(), (C.c_float * 3)(*k), int(want_slow), out)
+        return np.array(out[:], np.float32)
+
+    def resolve_entry(self, incident, normal, cos_i, n_seed, want_slow):
+        out = (C.c_float * 3)()
+        n = lib().ixo_biax_resolve_entry(*self.n, self._ax(), (C.c_float * 3)(*incident),
+                                         (C.c_float * 3)(*normal), float(cos_i), float(n_seed),
+                                         int(want_slow), out)
+        return np.float32(n), np.array(out[:], np.float32)
+
+
+def poynting_direction(k, e):
+    out = (C.c_float * 3)()
+    lib().ixo_poynting_direction((C.c_float * 3)(*k), (C.c_float * 3)(*e), out)
+    return np.array(out[:], np.float32)

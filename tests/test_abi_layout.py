@@ -133,9 +133,9 @@ def test_integration_guide_rust_binding_matches_the_header(c_layout):
 
 
 def test_the_rust_check_catches_a_dropped_field(c_layout):
-    # a planted drift (the trailing field the guide once lacked) fails
+    # a planted drift (a dropped trailing field) fails
     with open(os.path.join(ROOT, "INTEGRATION.md")) as f:
         structs = rust_structs(f.read())
-    structs["IxgMaterial"] = [ft for ft in structs["IxgMaterial"] if ft[0] != "scattering_g"]
+    structs["IxgMaterial"] = [ft for ft in structs["IxgMaterial"] if ft[0] != "bands_beta"]
     size, _, fields = rust_layout(structs, "IxgMaterial", {})
     assert size != c_layout["ixg_material"]["size"] or list(fields) != list(c_layout["ixg_material"]["fields"])
