@@ -36,7 +36,7 @@ typedef enum ixg_status {
  * IX/renderer/gpu_backend/mod.rs:287-298). */
 typedef enum ixg_route_reason {
     IXG_ROUTE_OK = 0,
-    IXG_ROUTE_UNSUPPORTED_MATERIAL = 1,    /* biaxial, zoned, scattering uniaxial       */
+    IXG_ROUTE_UNSUPPORTED_MATERIAL = 1,    /* zoned, scattering anisotropic stones      */
     IXG_ROUTE_UNSUPPORTED_ENVIRONMENT = 2, /* UV lamps; frosted/scatter under sun or HDR */
     IXG_ROUTE_DEVICE_LIMITS = 3,           /* too many planes for the LDS facet set     */
     IXG_ROUTE_NO_DEVICE = 4
@@ -352,7 +352,15 @@ uint32_t ixg_last_kernel_launches(const ixg_ctx* ctx);
  *   FINISH            compat lo u32, compat hi u32, pdf[8], lambda[8], radiance[8] -> xyz[3]
  *   INTERSECT         origin[3], dir[3] -> facet i32 (-1 = miss), t
  *   EXP_LANE          x -> exp_f32x8's lane value
- *   EXP               x -> e^x correctly rounded */
+ *   EXP               x -> e^x correctly rounded
+ * The biaxial probes read the scene's biaxial material: lambda gives
+ * n_beta = dispersion(lambda), n_alpha = n_beta - delta_beta_alpha,
+ * n_gamma = n_alpha + birefringence_delta, gamma along the optic axis.
+ *   BIAX_WAVE_INDICES lambda, k[3] -> n_slow, n_fast
+ *   BIAX_EIGEN        lambda, k[3], want_slow -> D[3]
+ *   BIAX_MODE_POYNTING lambda, k[3], want_slow -> S[3]
+ *   BIAX_RESOLVE_ENTRY lambda, k[3], normal[3] (facing k), want_slow -> wave normal[3]
+ *   BIAX_ASSIGNED_ALPHA lambda, k[3], is_extra, alpha_o, alpha_beta, alpha_e -> alpha */
 typedef enum ixg_probe_fn {
     IXG_PROBE_SAMPLE_DRAWS = 0,
     IXG_PROBE_CAMERA_RAY,
@@ -379,6 +387,11 @@ typedef enum ixg_probe_fn {
     IXG_PROBE_INTERSECT,
     IXG_PROBE_EXP_LANE,
     IXG_PROBE_EXP,
+    IXG_PROBE_BIAX_WAVE_INDICES,
+    IXG_PROBE_BIAX_EIGEN,
+    IXG_PROBE_BIAX_MODE_POYNTING,
+    IXG_PROBE_BIAX_RESOLVE_ENTRY,
+    IXG_PROBE_BIAX_ASSIGNED_ALPHA,
     IXG_PROBE_COUNT
 } ixg_probe_fn;
 /* Floats per input and output record of fn; 0 for an unknown id. */

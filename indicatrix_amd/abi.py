@@ -19,7 +19,8 @@ PROBE_FNS = ("sample_draws", "camera_ray", "cie_cmf", "illuminant", "environment
              "fresnel_reflection", "fresnel_transmission", "tir_retardation", "channel_optics",
              "pleochroic_alpha", "ordinary_eigen", "extraordinary_poynting", "theta_c", "uniaxial_indices",
              "entry_solve", "internal_solve", "exit_transmission", "hg_direction", "cosine_hemisphere",
-             "shading_normal", "finish", "intersect", "exp_lane", "exp")
+             "shading_normal", "finish", "intersect", "exp_lane", "exp", "biax_wave_indices", "biax_eigen",
+             "biax_mode_poynting", "biax_resolve_entry", "biax_assigned_alpha")
 PROBE = {name: i for i, name in enumerate(PROBE_FNS)}
 NOALLOC_MAX_PATHS = 1 << 20
 NOALLOC_MAX_HDR_TEXELS = 1024 * 512

@@ -967,6 +967,31 @@ __device__ __forceinline__ int facet_aniso(const FilmParams& P, const float4* __
     return STEP_CONTINUE;
 }
 
+// The assigned biaxial mode's E field (assigned_mode_e_field, biaxial.rs:432-436:
+// mode B = the slow D-eigenvector) in the absorption tensor's frame
+// (axes^T e, absorption_tensor.rs:80-83).
+__device__ __forceinline__ V3 biax_assigned_locals(const AbsorbParams& A, const Biax& hero, const BiaxFrame& BF, V3 k,
+                                                   bool is_extra) {
+    const V3 e = biax_d_to_e(hero, BF, biax_eigen_d(hero, BF, k, is_extra));
+    return v3(dot(v3(A.a1[0], A.a1[1], A.a1[2]), e), dot(v3(A.a2[0], A.a2[1], A.a2[2]), e),
+              dot(v3(A.cn[0], A.cn[1], A.cn[2]), e));
+}
+// AbsorptionTensor3::quadratic_form with alpha = (o, beta, e) (assigned_mode_alpha,
+// absorption_tensor.rs:115-117): alpha . (l * l)
+__device__ __forceinline__ float biax_tensor_qf(V3 l, float ao, float ab, float ae) {
+    return (ao * (l.x * l.x) + ab * (l.y * l.y)) + ae * (l.z * l.z);
+}
+// The film's per-channel indicatrix: n_beta from the dispersion curve
+// (GemMaterial::biaxial_indicatrix, materials/optics.rs:16-27).
+__device__ __forceinline__ Biax biax_at(const FilmParams& P, float n_beta) {
+    const float na = n_beta - P.an.dba;
+    return Biax{na, n_beta, na + P.an.biref};
+}
+__device__ __forceinline__ BiaxFrame biax_frame(const FilmParams& P) {
+    return BiaxFrame{v3(P.an.bx[0], P.an.bx[1], P.an.bx[2]), v3(P.an.by[0], P.an.by[1], P.an.by[2]),
+                     v3(P.an.bz[0], P.an.bz[1], P.an.bz[2])};
+}
+
 // One facet event of a biaxial material (biaxial_delta_beta_alpha is Some):
 // compute_bounce_refraction_geometry's biaxial block (geometry.rs:199-270,
 // 306-425: mode A = the fast root, mode B = the slow root, each channel's
@@ -980,8 +1005,7 @@ template <bool EVENTS, bool ISO, bool ABSORB, bool EXTRA = true>
 __device__ __forceinline__ int facet_biax(const FilmParams& P, const float4* __restrict__ planes, PathState& S,
                                           float t_hit, int facet) {
     const uint32_t bounce = S.bounce;
-    const BiaxFrame BF{v3(P.an.bx[0], P.an.bx[1], P.an.bx[2]), v3(P.an.by[0], P.an.by[1], P.an.by[2]),
-                       v3(P.an.bz[0], P.an.bz[1], P.an.bz[2])};
+    const BiaxFrame BF = biax_frame(P);
     const V3 hit = S.org + S.dir * t_hit;
     const float4 pf = planes[facet];
     V3 normal = v3(pf.x, pf.y, pf.z);
@@ -1001,19 +1025,15 @@ __device__ __forceinline__ int facet_biax(const FilmParams& P, const float4* __r
     S.has_prev = true;
     const V3 k = S.k;
     const bool inside = S.inside, is_extra = S.is_extra;
-    const Biax hero{S.n_o[0] - P.an.dba, S.n_o[0], (S.n_o[0] - P.an.dba) + P.an.biref};
+    const Biax hero = biax_at(P, S.n_o[0]);
     if (ABSORB && inside) {
-        // channel_absorption_alphas_assigned: alpha = e . A . e for the assigned mode's E
-        const V3 e = biax_d_to_e(hero, BF, biax_eigen_d(hero, BF, k, is_extra));
-        const float l0 = dot(v3(P.abs.a1[0], P.abs.a1[1], P.abs.a1[2]), e);
-        const float l1 = dot(v3(P.abs.a2[0], P.abs.a2[1], P.abs.a2[2]), e);
-        const float l2 = dot(v3(P.abs.cn[0], P.abs.cn[1], P.abs.cn[2]), e);
+        // channel_absorption_alphas_assigned (absorption.rs:224-249)
+        const V3 l = biax_assigned_locals(P.abs, hero, BF, k, is_extra);
         const float scaled = t_hit * P.abs.path_scale;
 #pragma unroll
         for (int c = 0; c < NCH; ++c) {
             const float ab = P.an.trichroic ? S.alpha_b[c] : S.alpha_o[c];
-            const float q = (S.alpha_o[c] * (l0 * l0) + ab * (l1 * l1)) + S.alpha_e[c] * (l2 * l2);
-            S.st[c] = scale4(S.st[c], exp_lane(-q * scaled));
+            S.st[c] = scale4(S.st[c], exp_lane(-biax_tensor_qf(l, S.alpha_o[c], ab, S.alpha_e[c]) * scaled));
         }
     }
     if (inside) normal = neg(normal);
@@ -1030,7 +1050,7 @@ __device__ __forceinline__ int facet_biax(const FilmParams& P, const float4* __r
     float n_a[NCH], n_b[NCH];
 #pragma unroll
     for (int c = 0; c < NCH; ++c) {
-        const Biax B{S.n_o[c] - P.an.dba, S.n_o[c], (S.n_o[c] - P.an.dba) + P.an.biref};
+        const Biax B = biax_at(P, S.n_o[c]);
         float ns, nf;
         biax_wave_indices(B, BF, dir_a, ns, nf);
         n_a[c] = nf;
@@ -1139,7 +1159,7 @@ __device__ __forceinline__ int facet_biax(const FilmParams& P, const float4* __r
                 const V3 rk = refract_dir(k, normal, div_rn(n1k, n2k), cos_i, ctk);
                 V3 fk = rk;
                 if (entering) {
-                    const Biax B{S.n_o[c] - P.an.dba, S.n_o[c], (S.n_o[c] - P.an.dba) + P.an.biref};
+                    const Biax B = biax_at(P, S.n_o[c]);
                     fk = biax_mode_poynting(B, BF, rk, use_extra);
                 }
                 const bool match = dot(fk, final_refr) >= DIRECTION_MATCH_COS_TOL;

@@ -45,6 +45,11 @@ constexpr ProbeShape kProbeShapes[IXG_PROBE_COUNT] = {
     {6, 2},   // INTERSECT
     {1, 1},   // EXP_LANE
     {1, 1},   // EXP
+    {4, 2},   // BIAX_WAVE_INDICES
+    {5, 3},   // BIAX_EIGEN
+    {5, 3},   // BIAX_MODE_POYNTING
+    {8, 3},   // BIAX_RESOLVE_ENTRY
+    {8, 1},   // BIAX_ASSIGNED_ALPHA
 };
 
 __device__ __forceinline__ V3 ld3(const float* p) { return v3(p[0], p[1], p[2]); }
@@ -243,6 +248,32 @@ __global__ void __launch_bounds__(256) probe_kernel(FilmParams P, const float4* 
     case IXG_PROBE_EXP:
         o[0] = exp_rn(a[0]);
         break;
+    case IXG_PROBE_BIAX_WAVE_INDICES: {
+        const Biax B = biax_at(P, dispersion(P.disp_kind, P.disp, a[0]));
+        biax_wave_indices(B, biax_frame(P), ld3(a + 1), o[0], o[1]);
+        break;
+    }
+    case IXG_PROBE_BIAX_EIGEN:
+        st3(o, biax_eigen_d(biax_at(P, dispersion(P.disp_kind, P.disp, a[0])), biax_frame(P), ld3(a + 1),
+                            a[4] != 0.0f));
+        break;
+    case IXG_PROBE_BIAX_MODE_POYNTING:
+        st3(o, biax_mode_poynting(biax_at(P, dispersion(P.disp_kind, P.disp, a[0])), biax_frame(P), ld3(a + 1),
+                                  a[4] != 0.0f));
+        break;
+    case IXG_PROBE_BIAX_RESOLVE_ENTRY: {
+        const float nb = dispersion(P.disp_kind, P.disp, a[0]);
+        const V3 k = ld3(a + 1), normal = ld3(a + 4);
+        const float cos_i = clampf(dot(neg(k), normal), 0.0f, 1.0f);
+        st3(o, biax_resolve_entry(biax_at(P, nb), biax_frame(P), k, normal, cos_i, nb, a[7] != 0.0f));
+        break;
+    }
+    case IXG_PROBE_BIAX_ASSIGNED_ALPHA: {
+        const V3 l = biax_assigned_locals(P.abs, biax_at(P, dispersion(P.disp_kind, P.disp, a[0])), biax_frame(P),
+                                          ld3(a + 1), a[4] != 0.0f);
+        o[0] = biax_tensor_qf(l, a[5], a[6], a[7]);
+        break;
+    }
     default:
         break;
     }

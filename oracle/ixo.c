@@ -1959,6 +1959,18 @@ static void facet_event_aniso(const scene_t* S, path_t* P, uint32_t bounce, uint
  * pleochroic absorption (absorption.rs:224-249 through
  * assigned_mode_e_field) and the 50/50 internal re-coupling draw
  * (transport/bounce.rs:439-470). */
+/* The assigned mode's E field (assigned_mode_e_field, biaxial.rs:432-436)
+ * in the absorption tensor's frame (axes^T e, absorption_tensor.rs:80-83),
+ * and AbsorptionTensor3::quadratic_form with alpha = (o, beta, e)
+ * (assigned_mode_alpha, absorption_tensor.rs:115-117). */
+static v3 biax_assigned_locals(const scene_t* S, const biax_t* hero, v3 k, int is_extra) {
+    v3 e = biax_assigned_e(hero, k, is_extra);
+    return mk(vdot(S->ta1, e), vdot(S->ta2, e), vdot(S->tcn, e));
+}
+static float biax_tensor_qf(v3 l, float ao, float ab, float ae) {
+    return (ao * (l.x * l.x) + ab * (l.y * l.y)) + ae * (l.z * l.z);
+}
+
 static void facet_event_biax(const scene_t* S, path_t* P, uint32_t bounce, uint32_t seed, float t_hit,
                              int32_t facet, ixg_path_events* ev, int* stop) {
     const ixg_material* M = S->mat;
@@ -1982,14 +1994,12 @@ static void facet_event_biax(const scene_t* S, path_t* P, uint32_t bounce, uint3
     }
     if (P->inside && S->absorbing) {
         /* channel_absorption_alphas_assigned (absorption.rs:224-249) */
-        v3 e = biax_assigned_e(&ind[0], P->k, P->is_extra);
-        float l0 = vdot(S->ta1, e), l1 = vdot(S->ta2, e), l2 = vdot(S->tcn, e);
+        v3 l = biax_assigned_locals(S, &ind[0], P->k, P->is_extra);
         float scaled = t_hit * M->absorption_path_scale;
         float arg[NCH], tr[NCH];
         for (int c = 0; c < NCH; ++c) {
             float ab = S->trichroic ? P->alpha_b[c] : P->alpha_o[c];
-            float q = (P->alpha_o[c] * (l0 * l0) + ab * (l1 * l1)) + P->alpha_e[c] * (l2 * l2);
-            arg[c] = -q * scaled;
+            arg[c] = -biax_tensor_qf(l, P->alpha_o[c], ab, P->alpha_e[c]) * scaled;
         }
         exp_lane8(arg, tr);
         for (int c = 0; c < NCH; ++c) P->st[c] = st_scale(P->st[c], tr[c]);
@@ -3476,8 +3486,16 @@ static uframe probe_frame(v3 k, v3 normal, v3 c) {
     return uframe_build(k, normal, c, cos_i, sin_i);
 }
 
-static const uint8_t kProbeIn[IXG_PROBE_COUNT] = {2, 2, 1, 1, 7, 6, 8, 8, 7, 8, 19, 6, 8, 9, 3, 11, 13, 15, 6, 5, 8, 26, 6, 1, 1};
-static const uint8_t kProbeOut[IXG_PROBE_COUNT] = {12, 6, 3, 1, 1, 4, 5, 6, 4, 24, 8, 3, 3, 1, 2, 18, 13, 5, 3, 3, 3, 3, 2, 1, 1};
+static const uint8_t kProbeIn[IXG_PROBE_COUNT] = {2, 2, 1, 1, 7, 6, 8, 8, 7, 8, 19, 6, 8, 9, 3, 11, 13,
+                                                   15, 6, 5, 8, 26, 6, 1, 1, 4, 5, 5, 8, 8};
+static const uint8_t kProbeOut[IXG_PROBE_COUNT] = {12, 6, 3, 1, 1, 4, 5, 6, 4, 24, 8, 3, 3, 1, 2, 18, 13,
+                                                    5, 3, 3, 3, 3, 2, 1, 1, 2, 3, 3, 3, 1};
+
+/* the scene's biaxial indicatrix at lambda (materials/optics.rs:16-27) */
+static biax_t probe_biax(const scene_t* S, float lambda) {
+    float nb = ixo_dispersion_evaluate(&S->mat->dispersion, lambda), na = nb - S->mat->biaxial_delta_beta_alpha;
+    return biax_from_gamma_axis(na, nb, na + S->mat->birefringence_delta, S->c_raw);
+}
 
 int ixo_probe_shape(int32_t fn, uint32_t* in_floats, uint32_t* out_floats) {
     if (fn < 0 || fn >= IXG_PROBE_COUNT) return 0;
@@ -3653,6 +3671,34 @@ int ixo_probe(const ixg_scene* sc, int32_t fn, const float* in, uint32_t n, floa
         case IXG_PROBE_EXP: /* e^x in double precision, rounded to float */
             o[0] = (float)exp((double)a[0]);
             break;
+        case IXG_PROBE_BIAX_WAVE_INDICES: {
+            biax_t B = probe_biax(&S, a[0]);
+            biax_wave_indices(&B, vfromr(a + 1), &o[0], &o[1]);
+            break;
+        }
+        case IXG_PROBE_BIAX_EIGEN: {
+            biax_t B = probe_biax(&S, a[0]);
+            v_out(o, biax_eigen_d(&B, vfromr(a + 1), a[4] != 0.0f));
+            break;
+        }
+        case IXG_PROBE_BIAX_MODE_POYNTING: {
+            biax_t B = probe_biax(&S, a[0]);
+            v_out(o, biax_mode_poynting(&B, vfromr(a + 1), a[4] != 0.0f));
+            break;
+        }
+        case IXG_PROBE_BIAX_RESOLVE_ENTRY: {
+            biax_t B = probe_biax(&S, a[0]);
+            v3 k = vfromr(a + 1), normal = vfromr(a + 4), w;
+            float cos_i = clampf(vdot(vneg(k), normal), 0.0f, 1.0f);
+            (void)biax_resolve_entry(&B, k, normal, cos_i, B.nb, a[7] != 0.0f, &w);
+            v_out(o, w);
+            break;
+        }
+        case IXG_PROBE_BIAX_ASSIGNED_ALPHA: {
+            biax_t B = probe_biax(&S, a[0]);
+            o[0] = biax_tensor_qf(biax_assigned_locals(&S, &B, vfromr(a + 1), a[4] != 0.0f), a[5], a[6], a[7]);
+            break;
+        }
         default:
             break;
         }

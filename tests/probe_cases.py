@@ -328,6 +328,56 @@ def gen_exp(rng, sc):
                            [0.0, -0.0, 1e-30, -1e-30]]).astype(np.float32)[:, None]
 
 
+def scene_alexandrite():
+    return tracer.FrameScene(tracer.camera_new(0.35, 0.28, 5.0, 18.0, 16, 16, 1), 16, 16,
+                             tracer.standard_round_brilliant(), tracer.GemMaterial.alexandrite(), 16,
+                             tracer.LightingPreset.studio("IsoHemisphere", 1.0, 0.0, 0.0))
+
+
+def scene_strong_biaxial():
+    """A strongly biaxial stone (n_alpha ~ 1.60, n_beta ~ 1.65, n_gamma ~ 1.75)
+    with a tilted gamma axis, so the general quadratic and large walk-offs
+    are exercised away from the principal axes."""
+    m = tracer.GemMaterial("strong biaxial", abi.DISP_CAUCHY, [1.64, 0.004, 0.0], birefringence_delta=0.15,
+                           cubic=False, c_axis=(0.35, 0.82, -0.45))
+    m.biaxial_delta_beta_alpha = 0.05
+    m.bands_o = [(500.0, 40.0, 1.0, 0)]
+    m.bands_e = [(600.0, 40.0, 2.0, 0)]
+    m.bands_beta = [(450.0, 30.0, 1.5, 0)]
+    return tracer.FrameScene(tracer.camera_new(0.35, 0.28, 5.0, 18.0, 16, 16, 1), 16, 16,
+                             tracer.standard_round_brilliant(), m, 16,
+                             tracer.LightingPreset.studio("IsoHemisphere", 1.0, 0.0, 0.0))
+
+
+def _biax_k(rng, n):
+    lam = rng.uniform(380.0, 780.0, n).astype(np.float32)[:, None]
+    return lam, _unit(rng, n)
+
+
+def gen_biax_indices(rng, sc):
+    lam, k = _biax_k(rng, 4000)
+    return np.concatenate([lam, k], axis=1)
+
+
+def gen_biax_mode(rng, sc):
+    lam, k = _biax_k(rng, 4000)
+    slow = (rng.uniform(size=4000) < 0.5).astype(np.float32)[:, None]
+    return np.concatenate([lam, k, slow], axis=1)
+
+
+def gen_biax_entry(rng, sc):
+    lam, k = _biax_k(rng, 4000)
+    normal = _facing(k, _unit(rng, 4000))
+    slow = (rng.uniform(size=4000) < 0.5).astype(np.float32)[:, None]
+    return np.concatenate([lam, k, normal, slow], axis=1)
+
+
+def gen_biax_alpha(rng, sc):
+    rec = gen_biax_mode(rng, sc)
+    alphas = rng.uniform(0.0, 3.0, (len(rec), 3)).astype(np.float32)
+    return np.concatenate([rec, alphas], axis=1)
+
+
 def gen_exp_rn(rng, sc):
     return np.concatenate([np.linspace(-103.0, 88.0, 20001), rng.uniform(-20.0, 20.0, 20000),
                            [0.0, -0.0, 1e-30, -1e-30]]).astype(np.float32)[:, None]
@@ -390,6 +440,16 @@ CASES = [
          checks=[([1], 512, 1e-6)], exact=[0]),
     dict(label="exp_f32x8 lane", fn="exp_lane", scenes=[scene_c1], gen=gen_exp, checks=[(ALL, 2, 0.0)]),
     dict(label="exp (correctly rounded)", fn="exp", scenes=[scene_c1], gen=gen_exp_rn, checks=[(ALL, 1, 0.0)]),
+    dict(label="BiaxialIndicatrix::wave_indices", fn="biax_wave_indices",
+         scenes=[scene_alexandrite, scene_strong_biaxial], gen=gen_biax_indices, checks=[(ALL, 64, 1e-5)]),
+    dict(label="BiaxialIndicatrix::eigen_polarizations", fn="biax_eigen",
+         scenes=[scene_alexandrite, scene_strong_biaxial], gen=gen_biax_mode, checks=[(ALL, 48, 5e-5)]),
+    dict(label="BiaxialIndicatrix::mode_poynting_dir", fn="biax_mode_poynting",
+         scenes=[scene_alexandrite, scene_strong_biaxial], gen=gen_biax_mode, checks=[(ALL, 48, 1e-5)]),
+    dict(label="BiaxialIndicatrix::resolve_entry_mode", fn="biax_resolve_entry",
+         scenes=[scene_alexandrite, scene_strong_biaxial], gen=gen_biax_entry, checks=[(ALL, 4096, 1e-5)]),
+    dict(label="assigned_mode_alpha (biaxial, P1)", fn="biax_assigned_alpha",
+         scenes=[scene_alexandrite, scene_strong_biaxial], gen=gen_biax_alpha, checks=[(ALL, 1200, 1e-5)]),
 ]
 
 

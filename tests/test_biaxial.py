@@ -288,3 +288,30 @@ def test_route_for_biaxial_stones():
     sc = biaxial_scene(tracer.GemMaterial.topaz())
     sc.set_finishes([1] + [0] * (len(sc.planes) - 1))
     assert tracer.route(sc) != abi.ROUTE_OK
+
+
+def test_biaxial_probe_dispatch_is_the_exported_functions():
+    # the oracle's probe records go through the same functions the exported
+    # entry points call, at the scene's indicatrix for each record's lambda
+    import probe_cases
+    sc = probe_cases.scene_strong_biaxial()
+    rng = np.random.default_rng(5)
+    rec = probe_cases.gen_biax_entry(rng, sc)[:64]
+    m = sc.material_abi
+    d = m.dispersion
+    for r in rec:
+        nb = np.float32(oracle_lib.lib().ixo_dispersion_evaluate(d, float(r[0])))
+        na = np.float32(nb - np.float32(m.biaxial_delta_beta_alpha))
+        ind = Biaxial.from_gamma_axis(na, nb, np.float32(na + np.float32(m.birefringence_delta)), m.c_axis[:])
+        k, normal, slow = r[1:4], r[4:7], bool(r[7])
+        wi = oracle_lib.probe(sc, "biax_wave_indices", np.concatenate([r[:4]])[None, :])[0]
+        assert wi.tobytes() == np.array(ind.wave_indices(k), np.float32).tobytes()
+        ed = oracle_lib.probe(sc, "biax_eigen", np.concatenate([r[:4], [r[7]]])[None, :].astype(np.float32))[0]
+        assert ed.tobytes() == ind.eigen_d(k, slow).tobytes()
+        sp = oracle_lib.probe(sc, "biax_mode_poynting", np.concatenate([r[:4], [r[7]]])[None, :].astype(np.float32))[0]
+        assert sp.tobytes() == ind.mode_poynting(k, slow).tobytes()
+        w = oracle_lib.probe(sc, "biax_resolve_entry", r[None, :].astype(np.float32))[0]
+        dots = np.float32(np.float32(np.float32(-k[0]) * normal[0]) + np.float32(np.float32(-k[1]) * normal[1]))
+        cos_i = np.float32(np.clip(np.float32(dots + np.float32(np.float32(-k[2]) * normal[2])), 0.0, 1.0))
+        _, w2 = ind.resolve_entry(k, normal, cos_i, nb, slow)
+        assert w.tobytes() == w2.tobytes()

@@ -51,8 +51,10 @@ def test_biaxial_paths_match_oracle(backend, name, preset):
     assert 1.0 - same.mean() < 2e-3
     for f in ("n_tir", "n_reflect", "n_refract", "n_probe"):
         assert np.array_equal(g_ev[f][same], o_ev[f][same])
-    assert np.quantile(rel, 0.999) < 1e-3
-    assert (rel.max(axis=1) > 1e-3).mean() < 5e-4
+    # the indicatrix functions are bit-exact on the device (no transcendental
+    # in the general branch); what remains is the frame rotation, TIR
+    # retardation, exponentials and lighting (measured: flip 0, max 7e-7)
+    assert rel.max() < 1e-5
     assert (g_ev["n_refract"] > 0).mean() > 0.3
 
 

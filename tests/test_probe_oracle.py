@@ -56,7 +56,7 @@ def test_ulp_distance_reference_cases():
 def test_probe_layouts_agree():
     L = oracle_lib.lib()
     G = _lib.lib()
-    assert len(abi.PROBE_FNS) == 25
+    assert len(abi.PROBE_FNS) == 30
     for fid in range(len(abi.PROBE_FNS)):
         a, b, c, d = C.c_uint32(), C.c_uint32(), C.c_uint32(), C.c_uint32()
         assert L.ixo_probe_shape(fid, C.byref(a), C.byref(b)) == 1
