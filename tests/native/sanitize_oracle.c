@@ -81,6 +81,17 @@ int main(void) {
     uni.bands_e[0] = (ixg_band){600.0f, 40.0f, 1.2f, 0};
     sc.material = &uni;
     total += frame(&sc, 2);
+    /* biaxial trichroic, with a rounded edge and a panorama */
+    ixg_material bi = uni;
+    bi.biaxial = 1;
+    bi.biaxial_delta_beta_alpha = 0.004f;
+    bi.has_beta_bands = 1;
+    bi.n_bands_beta = 1;
+    bi.bands_beta[0] = (ixg_band){450.0f, 30.0f, 0.9f, 0};
+    sc.material = &bi;
+    total += frame(&sc, 2);
+    bi.edge_rounding_radius = 0.03f;
+    total += frame(&sc, 2);
     sc.material = &mat;
     /* HDR panorama */
     float rgb[8 * 4 * 3];
