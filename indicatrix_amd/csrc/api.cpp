@@ -107,20 +107,19 @@ bool any_frosted(const ixg_scene* s) {
 bool material_routable(const ixg_material& m) {
     // The device path covers isotropic, uniaxial and biaxial
     // (RayMaterialContext::is_anisotropic, transport/inner.rs:181-198),
-    // unzoned materials, with or without absorption bands.  A biaxial stone
-    // is anisotropic (a "biaxial" set on a stone that is not declines) and
-    // takes no volume scattering.
+    // unzoned materials, with or without absorption bands and volume
+    // scattering.  A biaxial stone is anisotropic (a "biaxial" set on a stone
+    // that is not declines).
     if (!(m.edge_rounding_radius < 1e30f)) return false;
     if (m.biaxial) {
         if (m.cubic || !(std::fabs(m.birefringence_delta) > 1e-4f) || !std::isfinite(m.biaxial_delta_beta_alpha))
             return false;
-        if (m.scattering_sigma_s > 0.0f) return false;
         if (m.has_beta_bands && (m.n_bands_beta < 0 || m.n_bands_beta > IXG_MAX_BANDS)) return false;
     }
-    // Henyey-Greenstein volume scattering: isotropic stones, finite parameters
+    // Henyey-Greenstein volume scattering: finite parameters (isotropic,
+    // uniaxial and biaxial stones)
     if (!(m.scattering_sigma_s <= 0.0f) &&
-        (!(m.scattering_sigma_s < 1e30f) || !(std::fabs(m.scattering_g) < 1.0f) || !(m.absorption_path_scale > 0.0f) ||
-         (!m.cubic && std::fabs(m.birefringence_delta) > 1e-4f)))
+        (!(m.scattering_sigma_s < 1e30f) || !(std::fabs(m.scattering_g) < 1.0f) || !(m.absorption_path_scale > 0.0f)))
         return false;
     if (m.has_e_dispersion && (m.e_dispersion.kind < IXG_DISP_SELLMEIER1 || m.e_dispersion.kind > IXG_DISP_CAUCHY))
         return false;

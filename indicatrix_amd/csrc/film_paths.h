@@ -546,6 +546,18 @@ __device__ __forceinline__ void narrow_compat(uint64_t& compat, uint32_t match_m
         }
 }
 
+// assigned_mode_e_field_uniaxial (birefringence/absorption_tensor.rs): the E
+// field of the path's assigned eigenmode at wave normal k.
+__device__ __forceinline__ V3 uniaxial_assigned_e(const FilmParams& P, const PathState& S, V3 k) {
+    const V3 cax = v3(P.an.c_axis[0], P.an.c_axis[1], P.an.c_axis[2]);
+    if (!S.is_extra) return ordinary_eigen(k, cax);
+    const float n_o_h = S.n_o[0];
+    const float n_e_h = e_index_at(P, comb_lambda(S.hero, 0), n_o_h);
+    const V3 kxc = cross(k, cax);
+    if (dot(kxc, kxc) < 1e-8f) return normalize_or_zero(cross(k, ordinary_eigen(k, cax)));
+    return normalize(cross(kxc, extraordinary_poynting_dir(k, cax, n_o_h, n_e_h)));
+}
+
 template <bool EVENTS, bool ISO, bool ABSORB, bool EXTRA = true>
 __device__ __forceinline__ int facet_aniso(const FilmParams& P, const float4* __restrict__ planes, PathState& S,
                                         float t_hit, int facet) {
@@ -570,18 +582,10 @@ __device__ __forceinline__ int facet_aniso(const FilmParams& P, const float4* __
     S.has_prev = true;
     const V3 k = S.k;
     const bool inside = S.inside, is_extra = S.is_extra;
-    if (ABSORB && inside) {
-        // apply_absorption with the assigned-mode coefficient (absorption.rs:224-249)
-        const float n_o_h = S.n_o[0];
-        const float n_e_h = e_index_at(P, comb_lambda(S.hero, 0), n_o_h);
-        V3 e;
-        if (!is_extra) {
-            e = ordinary_eigen(k, cax);
-        } else {
-            const V3 kxc = cross(k, cax);
-            if (dot(kxc, kxc) < 1e-8f) e = normalize_or_zero(cross(k, ordinary_eigen(k, cax)));
-            else e = normalize(cross(kxc, extraordinary_poynting_dir(k, cax, n_o_h, n_e_h)));
-        }
+    if (ABSORB && inside && !(EXTRA && P.sigma_s > 0.0f)) {
+        // apply_absorption with the assigned-mode coefficient (absorption.rs:224-249);
+        // a scattering stone's segment was attenuated by the scatter step
+        const V3 e = uniaxial_assigned_e(P, S, k);
         const float scaled = t_hit * P.abs.path_scale;
 #pragma unroll
         for (int c = 0; c < NCH; ++c)
@@ -1026,8 +1030,9 @@ __device__ __forceinline__ int facet_biax(const FilmParams& P, const float4* __r
     const V3 k = S.k;
     const bool inside = S.inside, is_extra = S.is_extra;
     const Biax hero = biax_at(P, S.n_o[0]);
-    if (ABSORB && inside) {
-        // channel_absorption_alphas_assigned (absorption.rs:224-249)
+    if (ABSORB && inside && !(EXTRA && P.sigma_s > 0.0f)) {
+        // channel_absorption_alphas_assigned (absorption.rs:224-249); a
+        // scattering stone's segment was attenuated by the scatter step
         const V3 l = biax_assigned_locals(P.abs, hero, BF, k, is_extra);
         const float scaled = t_hit * P.abs.path_scale;
 #pragma unroll
@@ -1271,11 +1276,22 @@ __device__ __forceinline__ V3 hg_direction(float u1, float u2, float g, V3 fwd) 
 // reweighted by its own transmittance) and the iteration ends there, else it
 // reaches the facet with each channel's survival relative to the hero's.
 // Returns a StepResult, or -1 for "reached the facet".
-template <bool EVENTS, bool ABSORB>
+template <bool EVENTS, bool ABSORB, int ANISO = 0>
 __device__ __forceinline__ int scatter_step(const FilmParams& P, PathState& S, float hit_t) {
     const uint32_t bounce = S.bounce;
     float alpha[NCH];
-    if (ABSORB) {
+    if (ABSORB && ANISO == 2) {
+        // channel_absorption_alphas_assigned (absorption.rs:224-249), biaxial
+        const V3 l = biax_assigned_locals(P.abs, biax_at(P, S.n_o[0]), biax_frame(P), S.k, S.is_extra);
+#pragma unroll
+        for (int c = 0; c < NCH; ++c)
+            alpha[c] = biax_tensor_qf(l, S.alpha_o[c], P.an.trichroic ? S.alpha_b[c] : S.alpha_o[c], S.alpha_e[c]);
+    } else if (ABSORB && ANISO) {
+        // the same, uniaxial: the assigned o / e mode's E field
+        const V3 e = uniaxial_assigned_e(P, S, S.k);
+#pragma unroll
+        for (int c = 0; c < NCH; ++c) alpha[c] = tensor_qf(P.abs, S.alpha_o[c], S.alpha_e[c], e);
+    } else if (ABSORB) {
         channel_alphas(P.abs, S, S.dir, alpha);
     } else {
 #pragma unroll
@@ -1303,6 +1319,7 @@ __device__ __forceinline__ int scatter_step(const FilmParams& P, PathState& S, f
         const V3 new_dir = hg_direction(u1, u2, P.scat_g, old_dir);
         S.org = S.org + old_dir * div_rn(t_free, path_scale);
         S.dir = new_dir;
+        S.k = new_dir;       // the wave normal of an anisotropic stone follows the new direction
         S.has_prev = false;  // depolarised: no previous plane of incidence
         if (EVENTS) S.fh = fnv(fnv(S.fh, 0xFFFFu), 7u);
         if (bounce > 4 && !russian_roulette(S, bounce)) return STEP_SCATTER_ABSORBED;
@@ -1379,8 +1396,8 @@ __device__ __forceinline__ int path_bounce(const FilmParams& P, const float4* __
         return STEP_ESCAPED;
     }
 
-    if (EXTRA && !ANISO && P.sigma_s > 0.0f && S.inside) {  // run_bounce (bounce_body.rs:410-416)
-        const int r = scatter_step<EVENTS, ABSORB>(P, S, t_hit);
+    if (EXTRA && P.sigma_s > 0.0f && S.inside) {  // run_bounce (bounce_body.rs:410-416)
+        const int r = scatter_step<EVENTS, ABSORB, ANISO>(P, S, t_hit);
         if (r >= 0) return r;
     }
     if (ANISO == 2) return facet_biax<EVENTS, ISO, ABSORB, EXTRA>(P, planes, S, t_hit, facet);

@@ -1032,17 +1032,23 @@ void ixo_hg_direction(float u1, float u2, float g, const float fwd[3], float out
  * 435-600), isotropic and unzoned, next-event estimation off.  Returns 0
  * scattered (the iteration ends at the scatter point), 3 killed by the
  * roulette at the scatter, -1 reached the facet. */
+static void scatter_alphas_assigned(const scene_t* S, path_t* P, float alpha[NCH]);
+
 static int scatter_step(const scene_t* S, path_t* P, uint32_t bounce, uint32_t seed, float hit_t,
                         ixg_path_events* ev) {
     float alpha[NCH];
-    v3 ea = ordinary_eigen(P->k, S->c_raw);   /* channel_absorption_alphas_assigned, absorption.rs:206-222 */
-    v3 eb = vnormalize_or_zero(vcross(P->k, ea));
-    for (int c = 0; c < NCH; ++c) {
-        if (!S->absorbing) { alpha[c] = 0.0f; continue; }
-        const float qa = tensor_qf(S, P->alpha_o[c], P->alpha_e[c], ea);
-        const float qb = tensor_qf(S, P->alpha_o[c], P->alpha_e[c], eb);
-        IXO_FLOPS(2);
-        alpha[c] = (float)(((double)qa + (double)qb) / 2.0);
+    if (S->aniso) { /* channel_absorption_alphas_assigned, the assigned mode (absorption.rs:224-249) */
+        scatter_alphas_assigned(S, P, alpha);
+    } else { /* its isotropic-by-symmetry arm (absorption.rs:206-222) */
+        v3 ea = ordinary_eigen(P->k, S->c_raw);
+        v3 eb = vnormalize_or_zero(vcross(P->k, ea));
+        for (int c = 0; c < NCH; ++c) {
+            if (!S->absorbing) { alpha[c] = 0.0f; continue; }
+            const float qa = tensor_qf(S, P->alpha_o[c], P->alpha_e[c], ea);
+            const float qb = tensor_qf(S, P->alpha_o[c], P->alpha_e[c], eb);
+            IXO_FLOPS(2);
+            alpha[c] = (float)(((double)qa + (double)qb) / 2.0);
+        }
     }
     const float path_scale = S->mat->absorption_path_scale;
     const float sigma_s = S->mat->scattering_sigma_s / path_scale;
@@ -1582,7 +1588,7 @@ static void facet_event_aniso(const scene_t* S, path_t* P, uint32_t bounce, uint
     }
     P->prev_plane = cur_plane;
     P->has_prev = 1;
-    if (P->inside && S->absorbing) apply_absorption_assigned(S, P, t_hit);
+    if (P->inside && S->absorbing && !(S->mat->scattering_sigma_s > 0.0f)) apply_absorption_assigned(S, P, t_hit);
     if (P->inside) normal = vneg(normal);
 
     const v3 k = P->k;
@@ -1971,6 +1977,29 @@ static float biax_tensor_qf(v3 l, float ao, float ab, float ae) {
     return (ao * (l.x * l.x) + ab * (l.y * l.y)) + ae * (l.z * l.z);
 }
 
+/* channel_absorption_alphas_assigned (absorption.rs:224-249) for the scatter
+ * step of an anisotropic stone: the assigned mode's E field at the current
+ * wave normal -- the uniaxial o / e field, or the biaxial slow / fast one of
+ * the hero's indicatrix -- through each channel's tensor. */
+static void scatter_alphas_assigned(const scene_t* S, path_t* P, float alpha[NCH]) {
+    if (!S->absorbing) {
+        for (int c = 0; c < NCH; ++c) alpha[c] = 0.0f;
+        return;
+    }
+    if (S->biax) {
+        float nb = P->n_o[0], na = nb - S->mat->biaxial_delta_beta_alpha;
+        biax_t hero = biax_from_gamma_axis(na, nb, na + S->mat->birefringence_delta, S->c_raw);
+        v3 l = biax_assigned_locals(S, &hero, P->k, P->is_extra);
+        for (int c = 0; c < NCH; ++c)
+            alpha[c] = biax_tensor_qf(l, P->alpha_o[c], S->trichroic ? P->alpha_b[c] : P->alpha_o[c], P->alpha_e[c]);
+        return;
+    }
+    float n_o_h = P->n_o[0];
+    float n_e_h = e_index_at(S, P->lambdas[0], n_o_h);
+    v3 e = assigned_mode_e_field_uniaxial(P->k, S->c_raw, P->is_extra, n_o_h, n_e_h);
+    for (int c = 0; c < NCH; ++c) alpha[c] = tensor_qf(S, P->alpha_o[c], P->alpha_e[c], e);
+}
+
 static void facet_event_biax(const scene_t* S, path_t* P, uint32_t bounce, uint32_t seed, float t_hit,
                              int32_t facet, ixg_path_events* ev, int* stop) {
     const ixg_material* M = S->mat;
@@ -1992,8 +2021,9 @@ static void facet_event_biax(const scene_t* S, path_t* P, uint32_t bounce, uint3
         float nb = P->n_o[c], na = nb - dba;
         ind[c] = biax_from_gamma_axis(na, nb, na + bd, S->c_raw);
     }
-    if (P->inside && S->absorbing) {
-        /* channel_absorption_alphas_assigned (absorption.rs:224-249) */
+    if (P->inside && S->absorbing && !(M->scattering_sigma_s > 0.0f)) {
+        /* channel_absorption_alphas_assigned (absorption.rs:224-249); a
+         * scattering stone's segment was attenuated by the scatter step */
         v3 l = biax_assigned_locals(S, &ind[0], P->k, P->is_extra);
         float scaled = t_hit * M->absorption_path_scale;
         float arg[NCH], tr[NCH];
@@ -2371,7 +2401,7 @@ static void trace_core(const scene_t* S, v3 origin, v3 dir, uint32_t seed, float
             stopped = 1;
             break;
         }
-        if (!S->aniso && S->mat->scattering_sigma_s > 0.0f && P.inside) { /* run_bounce, bounce_body.rs:410-416 */
+        if (S->mat->scattering_sigma_s > 0.0f && P.inside) { /* run_bounce, bounce_body.rs:410-416 */
             const int r = scatter_step(S, &P, bounce, seed, t, ev);
             if (r == 0) continue;
             if (r == 3) {

@@ -281,7 +281,7 @@ def test_route_for_biaxial_stones():
     assert tracer.route(biaxial_scene(tracer.GemMaterial.alexandrite())) == abi.ROUTE_OK
     m = tracer.GemMaterial.topaz().to_abi()
     m.scattering_sigma_s = 0.5
-    assert tracer.route(biaxial_scene(m)) == abi.ROUTE_UNSUPPORTED_MATERIAL
+    assert tracer.route(biaxial_scene(m)) == abi.ROUTE_OK        # scattering biaxial stones route
     m = tracer.GemMaterial.topaz()
     m.cubic = True
     assert tracer.route(biaxial_scene(m)) == abi.ROUTE_UNSUPPORTED_MATERIAL

@@ -75,5 +75,57 @@ def test_route_for_scattering_materials():
     m = tracer.GemMaterial.diamond().to_abi()
     m.cubic = 0
     m.birefringence_delta = 0.009
-    assert tracer.route(scattering_scene(2.0, material=m)) == abi.ROUTE_UNSUPPORTED_MATERIAL
+    assert tracer.route(scattering_scene(2.0, material=m)) == abi.ROUTE_OK          # uniaxial
+    assert tracer.route(scattering_scene(2.0, material=tracer.GemMaterial.alexandrite().to_abi())) == abi.ROUTE_OK
+    assert tracer.route(scattering_scene(2.0, material=m, preset="DaylightSun")) == abi.ROUTE_UNSUPPORTED_ENVIRONMENT
     assert tracer.route(workloads.film_absorbing_scene(8, 8)) == abi.ROUTE_OK
+
+
+def _aniso_material(kind, bands):
+    if kind == "uniaxial":
+        m = workloads.c2_material(bands=bands)
+    else:
+        m = tracer.GemMaterial.alexandrite()
+        if not bands:
+            m.bands_o, m.bands_e, m.bands_beta = [], [], None
+    return m.to_abi()
+
+
+@pytest.mark.parametrize("kind", ["uniaxial", "biaxial"])
+def test_vanishing_scattering_in_anisotropic_stones_is_the_clear_stone(kind):
+    # try_scatter_step runs for every interior segment of a scattering stone
+    # (bounce_body.rs:410-416, hg.rs:435-651), anisotropic ones included:
+    # without bands a vanishing coefficient leaves the frame bit for bit
+    clear = oracle_lib.accumulate(scattering_scene(0.0, material=_aniso_material(kind, False)), 0, 8)
+    faint = oracle_lib.accumulate(scattering_scene(1e-12, material=_aniso_material(kind, False)), 0, 8)
+    assert clear.tobytes() == faint.tobytes()
+
+
+@pytest.mark.parametrize("kind", ["uniaxial", "biaxial"])
+def test_anisotropic_scattering_white_furnace(kind):
+    # a non-absorbing scattering stone still returns the uniform panorama's
+    # spectrum: scattering only redistributes (albedo 1)
+    from test_hdr_env import furnace_target
+    m = _aniso_material(kind, False)
+    m.dispersion.kind = abi.DISP_CAUCHY
+    m.dispersion.coeff[0], m.dispersion.coeff[1], m.dispersion.coeff[2] = 1.5, 0.0, 0.0
+    m.scattering_sigma_s = 0.4
+    m.scattering_g = 0.5
+    cam = tracer.camera_new(0.35, 0.28, 5.0, 18.0, 12, 12, 96)
+    env = tracer.LightingPreset.studio("Daylight", 1.0, 0.4, 0.35)
+    sc = tracer.FrameScene(cam, 12, 12, tracer.standard_round_brilliant(), m, 48, env)
+    sc.set_hdr_map(np.full((1, 1, 3), 2.5, np.float32))
+    acc = oracle_lib.accumulate(sc, 0, 96)
+    mean = acc.sum(axis=0) / (12 * 12 * 96)
+    target = furnace_target(2.5)
+    assert (np.abs(mean - target) / target).max() < 0.015, (mean, target)
+
+
+@pytest.mark.parametrize("kind", ["uniaxial", "biaxial"])
+def test_anisotropic_scattering_absorbs_through_the_assigned_mode(kind):
+    # with bands, the scatter step's coefficients are the assigned mode's
+    # (absorption.rs:224-249): the frame differs from the clear stone's and
+    # from the band-free stone's
+    base = oracle_lib.accumulate(scattering_scene(0.8, material=_aniso_material(kind, False)), 0, 8)
+    banded = oracle_lib.accumulate(scattering_scene(0.8, material=_aniso_material(kind, True)), 0, 8)
+    assert np.all(np.isfinite(banded)) and banded.sum() < base.sum()
