@@ -455,13 +455,9 @@ ixg_route_reason ixg_route(const ixg_scene* s) {
     if (!std::isfinite(s->environment.backdrop)) return IXG_ROUTE_UNSUPPORTED_ENVIRONMENT;
     if (s->material->scattering_sigma_s > 0.0f && pi.model == ixg::ENV_DAYLIGHT_SUN)
         return IXG_ROUTE_UNSUPPORTED_ENVIRONMENT;  // the sun is sampled by next-event estimation at scatters
-    if (any_frosted(s)) {
-        // frosted facets: isotropic stones, and no environment whose sun the
-        // reference samples by next-event estimation at a frosted event
-        const ixg_material& m = *s->material;
-        if (!m.cubic && std::fabs(m.birefringence_delta) > 1e-4f) return IXG_ROUTE_UNSUPPORTED_MATERIAL;
-        if (pi.model == ixg::ENV_DAYLIGHT_SUN) return IXG_ROUTE_UNSUPPORTED_ENVIRONMENT;
-    }
+    // frosted facets: no environment whose sun the reference samples by
+    // next-event estimation at a frosted event
+    if (any_frosted(s) && pi.model == ixg::ENV_DAYLIGHT_SUN) return IXG_ROUTE_UNSUPPORTED_ENVIRONMENT;
     return IXG_ROUTE_OK;
 }
 

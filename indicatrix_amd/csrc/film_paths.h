@@ -546,6 +546,43 @@ __device__ __forceinline__ void narrow_compat(uint64_t& compat, uint32_t match_m
         }
 }
 
+// apply_frosted_bounce (scattering/frosted.rs:265-390), next-event estimation
+// off, for an anisotropic stone: the bounce geometry's hero indices decide TIR
+// and the Fresnel draw; a cosine-weighted hemisphere on the reflect or far
+// side with every channel depolarised; a transmission into the stone draws its
+// eigenmode 50/50 (upd).
+template <bool EVENTS>
+__device__ __forceinline__ V3 frosted_arm_aniso(PathState& S, uint32_t bounce, int facet, V3 normal, float cos_i,
+                                                float n1, float n2, float sin2_t, bool& new_inside, int& upd) {
+    const float u1 = unit_draw(S.seed, bounce, FROSTED_DIR_U_STREAM);
+    const float u2 = unit_draw(S.seed, bounce, FROSTED_DIR_V_STREAM);
+    uint32_t kind = 4u;
+    V3 dir;
+    float pdf = 1.0f;
+    if (sin2_t > 1.0f) {
+        dir = cosine_hemisphere(u1, u2, normal);
+    } else {
+        float rs, rp;
+        fresnel_rs_rp(n1, n2, cos_i, sqrt_rn(fmaxf(1.0f - sin2_t, 0.0f)), rs, rp);
+        const float r_unpol = clampf(0.5f * fma_(rp, rp, rs * rs), R_UNPOL_PDF_MIN, R_UNPOL_PDF_MAX);
+        const bool through = !(unit_draw(S.seed, bounce, FRESNEL_BRANCH_STREAM) < r_unpol);
+        dir = cosine_hemisphere(u1, u2, through ? neg(normal) : normal);
+        pdf = through ? 1.0f - r_unpol : r_unpol;
+        kind = through ? 6u : 5u;
+        if (through) {
+            if (!S.inside) upd = unit_draw(S.seed, bounce, BIREFRINGENT_SPLIT_STREAM) < 0.5f ? 1 : 0;
+            new_inside = !S.inside;
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < NCH; ++c) {
+        S.st[c] = make_float4(fmaxf(S.st[c].x, 0.0f), 0.0f, 0.0f, 0.0f);
+        if (kind != 4u) S.path_pdf[c] *= pdf;
+    }
+    if (EVENTS) S.fh = fnv(fnv(S.fh, (uint32_t)facet), kind);
+    return dir;
+}
+
 // assigned_mode_e_field_uniaxial (birefringence/absorption_tensor.rs): the E
 // field of the path's assigned eigenmode at wave normal k.
 __device__ __forceinline__ V3 uniaxial_assigned_e(const FilmParams& P, const PathState& S, V3 k) {
@@ -619,7 +656,12 @@ __device__ __forceinline__ int facet_aniso(const FilmParams& P, const float4* __
     int upd = -1;
     float exact_p_o = -1.0f;
     uint32_t match_bits = 0u;  // channels whose refracted direction matched the hero's
-    if (sin2_t > 1.0f) {
+    const bool frosted = EXTRA && P.frosted && facet_frosted(planes, facet);
+    if (frosted) {
+        // dispatch_bounce's frosted arm (transport/bounce.rs:137-160)
+        new_k = frosted_arm_aniso<EVENTS>(S, bounce, facet, normal, cos_i, n1, n2, sin2_t, new_inside, upd);
+        new_s = new_k;
+    } else if (sin2_t > 1.0f) {
         // apply_tir_bounce (tir.rs:56-160)
         if (EVENTS) {
             ++S.n_tir;
@@ -952,7 +994,7 @@ __device__ __forceinline__ int facet_aniso(const FilmParams& P, const float4* __
     }
     // the side channel also records the eigenmode after every event
     if (EVENTS) S.fh = fnv(fnv(S.fh, 16u + (S.is_extra ? 1u : 0u)), 256u + match_bits);
-    if (bounce == 0 && !new_inside && P.surface_glare < 1.0f) {
+    if (bounce == 0 && !new_inside && P.surface_glare < 1.0f && !frosted) {
 #pragma unroll
         for (int c = 0; c < NCH; ++c) S.st[c] = scale4(S.st[c], P.surface_glare);
     }
@@ -1076,7 +1118,12 @@ __device__ __forceinline__ int facet_biax(const FilmParams& P, const float4* __r
     bool new_inside = inside;
     int upd = -1;
     uint32_t match_bits = 0u;
-    if (sin2_t > 1.0f) {
+    const bool frosted = EXTRA && P.frosted && facet_frosted(planes, facet);
+    if (frosted) {
+        // dispatch_bounce's frosted arm (transport/bounce.rs:137-160)
+        new_k = frosted_arm_aniso<EVENTS>(S, bounce, facet, normal, cos_i, n1, n2, sin2_t, new_inside, upd);
+        new_s = new_k;
+    } else if (sin2_t > 1.0f) {
         // apply_tir_bounce, scalar arm (tir.rs:56-160)
         if (EVENTS) {
             ++S.n_tir;
@@ -1211,7 +1258,7 @@ __device__ __forceinline__ int facet_biax(const FilmParams& P, const float4* __r
     if (upd >= 0) S.is_extra = upd == 1;
     if (was_internal_reflection) S.is_extra = unit_draw(S.seed, bounce, MODE_COUPLING_STREAM) >= 0.5f;
     if (EVENTS) S.fh = fnv(fnv(S.fh, 16u + (S.is_extra ? 1u : 0u)), 256u + match_bits);
-    if (bounce == 0 && !new_inside && P.surface_glare < 1.0f) {
+    if (bounce == 0 && !new_inside && P.surface_glare < 1.0f && !frosted) {
 #pragma unroll
         for (int c = 0; c < NCH; ++c) S.st[c] = scale4(S.st[c], P.surface_glare);
     }

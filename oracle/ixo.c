@@ -41,6 +41,26 @@
 #ifndef IXO_FLOPS
 #define IXO_FLOPS(n) ((void)0)
 #endif
+/* A build for the tests only (-DIXO_ULP_JITTER) moves the uniaxial
+ * transport's transcendental results (acos, sin, cos, atan, atan2) by one
+ * ULP, a hash of the argument choosing down / none / up, as another
+ * library's functions may differ from glibc's, to measure how far such
+ * differences carry through the transport.  Off, TF(y, x) is y. */
+#ifdef IXO_ULP_JITTER
+static float ulp_jitter(float y, float x) {
+    uint32_t u;
+    memcpy(&u, &x, 4);
+    u = (u ^ (u >> 13)) * 0x9E3779B1u;
+    const uint32_t d = u >> 30;
+    if (d == 1u) return nextafterf(y, INFINITY);
+    if (d == 2u) return nextafterf(y, -INFINITY);
+    return y;
+}
+#define TF(y, x) ulp_jitter((y), (x))
+#else
+#define TF(y, x) (y)
+#endif
+
 /* a value stored into an ABI struct's f32 field */
 #define ABI_F32(x) ((ixo_abi_f32)(x))
 
@@ -1475,12 +1495,12 @@ static float theta_c_for_bounce(const scene_t* S, const path_t* P, v3 normal, v3
             if (s2g > 1.0f) break;
             float ctg = sqrtf(maxf(1.0f - s2g, 0.0f));
             v3 wd = vnormalize(vadd(vmul(k, eg), vmul(normal, fmaf(eg, cos_i, -ctg))));
-            theta = acosf(fabsf(clampf(vdot(wd, c), -1.0f, 1.0f)));
+            theta = TF(acosf(fabsf(clampf(vdot(wd, c), -1.0f, 1.0f))), vdot(wd, c));
             n_guess = effective_extraordinary_index(n_o_seed, n_e_seed, theta);
         }
         return theta;
     }
-    return acosf(fabsf(clampf(vdot(k, c), -1.0f, 1.0f)));
+    return TF(acosf(fabsf(clampf(vdot(k, c), -1.0f, 1.0f))), vdot(k, c));
 }
 
 /* entry_eigenmode_selection (refraction/uniaxial_entry.rs:50-75): the
@@ -1551,6 +1571,45 @@ static v3 refract_dir(v3 k, v3 normal, float eta, float cos_i, float cos_t) {
 static v3 poynting_for_mode(const scene_t* S, const ageo_t* G, v3 k, int inside, int is_extra) {
     if (!inside || !is_extra) return k;
     return extraordinary_poynting_dir(k, S->c_raw, G->n_o_hero, G->n_e_hero);
+}
+
+/* apply_frosted_bounce (scattering/frosted.rs:265-390), next-event estimation
+ * off, for an anisotropic stone: the bounce geometry's hero indices decide TIR
+ * and the Fresnel draw, the new direction is a cosine-weighted hemisphere on
+ * the reflect or far side with every channel depolarised, and a transmission
+ * into the stone draws its eigenmode 50/50.  *upd receives the entry's
+ * eigenmode (or stays -1). */
+static v3 frosted_arm_aniso(path_t* P, uint32_t bounce, uint32_t seed, int32_t facet, v3 normal, float cos_i,
+                            float n1, float n2, float sin2_t, int* new_inside, int* upd, ixg_path_events* ev) {
+    const float u1 = unit_draw(seed, bounce, FROSTED_DIR_U_STREAM);
+    const float u2 = unit_draw(seed, bounce, FROSTED_DIR_V_STREAM);
+    if (sin2_t > 1.0f) {
+        for (int c = 0; c < NCH; ++c) P->st[c] = st_unpolarized(P->st[c]);
+        ev_facet(ev, facet, 4);
+        return cosine_weighted_hemisphere(u1, u2, normal);
+    }
+    const float cos_t = sqrtf(maxf(1.0f - sin2_t, 0.0f));
+    float rs, rp;
+    fresnel_rs_rp(n1, n2, cos_i, cos_t, &rs, &rp);
+    const float r_unpol = clampf(0.5f * fmaf(rp, rp, rs * rs), R_UNPOL_PDF_MIN, R_UNPOL_PDF_MAX);
+    if (unit_draw(seed, bounce, FRESNEL_BRANCH_STREAM) < r_unpol) {
+        for (int c = 0; c < NCH; ++c) {
+            P->st[c] = st_unpolarized(P->st[c]);
+            P->path_pdf[c] *= r_unpol;
+        }
+        ev_facet(ev, facet, 5);
+        return cosine_weighted_hemisphere(u1, u2, normal);
+    }
+    const v3 dir = cosine_weighted_hemisphere(u1, u2, vneg(normal));
+    const float t_unpol = 1.0f - r_unpol;
+    for (int c = 0; c < NCH; ++c) {
+        P->st[c] = st_unpolarized(P->st[c]);
+        P->path_pdf[c] *= t_unpol;
+    }
+    if (!P->inside) *upd = unit_draw(seed, bounce, BIREFRINGENT_SPLIT_STREAM) < 0.5f;
+    *new_inside = !P->inside;
+    ev_facet(ev, facet, 6);
+    return dir;
 }
 
 /* compute_uniaxial_exit_transmission (refraction/uniaxial_internal.rs) */
@@ -1624,7 +1683,11 @@ static void facet_event_aniso(const scene_t* S, path_t* P, uint32_t bounce, uint
     int new_inside = inside, upd = -1;
     float exact_p_o = -1.0f;
     uint32_t match_bits = 0u; /* channels whose refracted direction matched the hero's */
-    if (G.sin2_t > 1.0f) {
+    const int frosted = S->finish != NULL && S->finish[facet] != 0;
+    if (frosted) { /* dispatch_bounce's frosted arm (transport/bounce.rs:137-160) */
+        new_k = frosted_arm_aniso(P, bounce, seed, facet, normal, G.cos_i, G.n1, G.n2, G.sin2_t, &new_inside, &upd, ev);
+        new_s = new_k;
+    } else if (G.sin2_t > 1.0f) {
         /* apply_tir_bounce (refraction/tir.rs:56-160) */
         if (ev) ev->n_tir++;
         ev_facet(ev, facet, 1);
@@ -1941,7 +2004,7 @@ static void facet_event_aniso(const scene_t* S, path_t* P, uint32_t bounce, uint
         ev->facet_hash = (ev->facet_hash ^ (16u + (uint32_t)P->is_extra)) * 16777619u;
         ev->facet_hash = (ev->facet_hash ^ (256u + match_bits)) * 16777619u;
     }
-    if (bounce == 0 && !P->inside && S->surface_glare < 1.0f)
+    if (bounce == 0 && !P->inside && S->surface_glare < 1.0f && !frosted)
         for (int c = 0; c < NCH; ++c) P->st[c] = st_scale(P->st[c], S->surface_glare);
     if (bounce > 4) {
         float mx = 0.0f;
@@ -2070,7 +2133,11 @@ static void facet_event_biax(const scene_t* S, path_t* P, uint32_t bounce, uint3
     v3 new_k, new_s;
     int new_inside = inside, upd = -1;
     uint32_t match_bits = 0u;
-    if (sin2_t > 1.0f) {
+    const int frosted = S->finish != NULL && S->finish[facet] != 0;
+    if (frosted) { /* dispatch_bounce's frosted arm (transport/bounce.rs:137-160) */
+        new_k = frosted_arm_aniso(P, bounce, seed, facet, normal, cos_i, n1, n2, sin2_t, &new_inside, &upd, ev);
+        new_s = new_k;
+    } else if (sin2_t > 1.0f) {
         /* apply_tir_bounce, scalar arm (refraction/tir.rs:56-160) */
         if (ev) ev->n_tir++;
         ev_facet(ev, facet, 1);
@@ -2189,7 +2256,7 @@ static void facet_event_biax(const scene_t* S, path_t* P, uint32_t bounce, uint3
         ev->facet_hash = (ev->facet_hash ^ (16u + (uint32_t)P->is_extra)) * 16777619u;
         ev->facet_hash = (ev->facet_hash ^ (256u + match_bits)) * 16777619u;
     }
-    if (bounce == 0 && !P->inside && S->surface_glare < 1.0f)
+    if (bounce == 0 && !P->inside && S->surface_glare < 1.0f && !frosted)
         for (int c = 0; c < NCH; ++c) P->st[c] = st_scale(P->st[c], S->surface_glare);
     if (bounce > 4) {
         float mx = 0.0f;

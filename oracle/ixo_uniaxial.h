@@ -40,8 +40,8 @@ static inline cplx cdiv(cplx a, cplx o) {
 static cplx csqrt_fwd(cplx z) {
     if (z.im == 0.0f && z.re < 0.0f) return cx(0.0f, sqrtf(-z.re));
     float r = sqrtf(sqrtf(cnorm2(z)));
-    float th = atan2f(z.im, z.re) * 0.5f;
-    cplx o = cx(r * cosf(th), r * sinf(th));
+    float th = TF(atan2f(z.im, z.re), z.re) * 0.5f;
+    cplx o = cx(r * TF(cosf(th), th + 1.0f), r * TF(sinf(th), th));
     if (o.re < 0.0f) o = cx(-o.re, -o.im);
     if (fabsf(o.re) < 1e-9f && o.im < 0.0f) o = cx(-o.re, -o.im);
     return o;
@@ -352,7 +352,7 @@ static void azimuth2_in_frame(v3 dir, v3 s_axis, v3 p_axis, float* c2, float* s2
 /* ---- BirefringenceParams (birefringence/uniaxial.rs) ---- */
 static float effective_extraordinary_index(float n_o, float n_e, float theta) {
     if (fabsf(n_o - n_e) < 1e-5f) return n_o;
-    float st = sinf(theta), ct = cosf(theta);
+    float st = TF(sinf(theta), theta), ct = TF(cosf(theta), theta + 1.0f);
     float nos = n_o * st;
     float denom_sq = fmaf(n_e * ct, n_e * ct, nos * nos);
     if (denom_sq <= 1e-8f) return n_o;
@@ -363,7 +363,7 @@ static float walk_off_from_sin_cos(float n_o, float n_e, float st, float ct) {
     if (fabsf(n_o - n_e) < 1e-5f) return 0.0f;
     float n_o2 = n_o * n_o, n_e2 = n_e * n_e;
     float tan_rho = ((n_o2 - n_e2) * st * ct) / maxf(fmaf(n_e2 * ct, ct, n_o2 * st * st), 1e-6f);
-    return atanf(tan_rho);
+    return TF(atanf(tan_rho), tan_rho);
 }
 
 static v3 extraordinary_poynting_dir(v3 w, v3 c, float n_o, float n_e) {
@@ -375,7 +375,7 @@ static v3 extraordinary_poynting_dir(v3 w, v3 c, float n_o, float n_e) {
     v3 c_proj = vnormalize_or_zero(vsub(c, vmul(w, cos_theta)));
     if (vlen_sq(c_proj) < 1e-6f) return w;
     float sign = cos_theta >= 0.0f ? 1.0f : -1.0f;
-    return vnormalize(vsub(vmul(w, cosf(delta)), vmul(vmul(c_proj, sign), sinf(delta))));
+    return vnormalize(vsub(vmul(w, TF(cosf(delta), delta + 1.0f)), vmul(vmul(c_proj, sign), TF(sinf(delta), delta))));
 }
 
 /* assigned_mode_e_field_uniaxial (birefringence/absorption_tensor.rs) */

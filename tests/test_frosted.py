@@ -7,13 +7,17 @@ and the cosine-law moments (E[cos] = 2/3, E[cos^2] = 1/2, no azimuthal
 bias).  Polished finishes given explicitly equal no finishes bit for bit; a
 frosted girdle changes the image.  The device follows the oracle path for
 path in tests/test_frosted_gpu.py.  Next-event estimation at frosted events
-is not carried, so the route declines a frosted stone under DaylightSun, and
-frosted uniaxial stones, for which the transport has no frosted arm.
+is not carried, so the route declines a frosted stone under DaylightSun.
+Uniaxial and biaxial stones take the same arm from their own bounce geometry
+(the eigenmode of a transmission into the stone drawn 50/50,
+scattering/frosted.rs:340-390); a non-absorbing frosted stone of either kind
+conserves energy in the analytic white furnace.
 """
 import ctypes as C
 import math
 
 import numpy as np
+import pytest
 
 import oracle_lib
 from indicatrix_amd import abi, tracer
@@ -77,5 +81,44 @@ def test_route_for_frosted_stones():
     m = tracer.GemMaterial.diamond().to_abi()
     m.cubic = 0
     m.birefringence_delta = 0.009
-    assert tracer.route(_scene(fr, material=m)) == abi.ROUTE_UNSUPPORTED_MATERIAL
+    assert tracer.route(_scene(fr, material=m)) == abi.ROUTE_OK           # frosted uniaxial
+    assert tracer.route(_scene(fr, material=tracer.GemMaterial.alexandrite())) == abi.ROUTE_OK
+    assert tracer.route(_scene(fr, "DaylightSun", material=m)) == abi.ROUTE_UNSUPPORTED_ENVIRONMENT
     assert tracer.route(_scene(None, material=m)) == abi.ROUTE_OK
+
+
+def _aniso(kind):
+    from indicatrix_amd import workloads
+    return (workloads.c2_material(bands=True) if kind == "uniaxial" else tracer.GemMaterial.alexandrite()).to_abi()
+
+
+@pytest.mark.parametrize("kind", ["uniaxial", "biaxial"])
+def test_frosted_anisotropic_stones(kind):
+    fr = [1 if i in GIRDLE else 0 for i in range(74)]
+    a = oracle_lib.accumulate(_scene(None, material=_aniso(kind)), 0, 8)
+    b = oracle_lib.accumulate(_scene([0] * 74, material=_aniso(kind)), 0, 8)
+    assert a.tobytes() == b.tobytes()
+    c = oracle_lib.accumulate(_scene(fr, material=_aniso(kind)), 0, 8)
+    assert not np.array_equal(a, c) and np.isfinite(c).all() and c.sum() > 0
+
+
+@pytest.mark.parametrize("kind", ["isotropic", "uniaxial", "biaxial"])
+def test_frosted_white_furnace(kind):
+    # the frosted arm's weights (cosine-weighted direction, Fresnel-chosen side,
+    # pdf by the clamped reflectance) conserve energy: a frosted, non-absorbing
+    # stone in a uniform panorama returns its spectrum within the truncation
+    from test_hdr_env import furnace_target
+    m = tracer.GemMaterial("furnace probe", abi.DISP_CAUCHY, [1.5, 0.0, 0.0],
+                           birefringence_delta=0.0 if kind == "isotropic" else 0.02, cubic=kind == "isotropic",
+                           c_axis=(0.3, 0.8, 0.52))
+    if kind == "biaxial":
+        m.biaxial_delta_beta_alpha = 0.008
+    fr = [1 if (i in GIRDLE or i >= 49) else 0 for i in range(74)]
+    cam = tracer.camera_new(0.35, 0.28, 5.0, 18.0, 12, 12, 96)
+    env = tracer.LightingPreset.studio("Daylight", 1.0, 0.4, 0.35)
+    sc = tracer.FrameScene(cam, 12, 12, tracer.standard_round_brilliant(), m, 48, env, facet_finishes=fr)
+    sc.set_hdr_map(np.full((1, 1, 3), 2.5, np.float32))
+    acc = oracle_lib.accumulate(sc, 0, 96)
+    mean = acc.sum(axis=0) / (12 * 12 * 96)
+    target = furnace_target(2.5)
+    assert (np.abs(mean - target) / target).max() < 0.015, (mean, target)
