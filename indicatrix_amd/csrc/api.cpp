@@ -444,7 +444,8 @@ ixg_route_reason ixg_route(const ixg_scene* s) {
     if (!material_routable(*s->material)) return IXG_ROUTE_UNSUPPORTED_MATERIAL;
     ixg::PresetInfo pi;
     if (s->hdr_map) {
-        // a panorama: frosted facets and scatters would sample it by next-event estimation
+        // a panorama: frosted facets and scatters would light-sample it by its
+        // importance distribution, which is not carried
         const ixg_hdr_map& h = *s->hdr_map;
         if (!h.rgb || h.width == 0 || h.height == 0 || (uint64_t)h.width * h.height > IXG_MAX_HDR_TEXELS)
             return IXG_ROUTE_UNSUPPORTED_ENVIRONMENT;
@@ -453,11 +454,8 @@ ixg_route_reason ixg_route(const ixg_scene* s) {
     }
     if (!ixg::preset_info(s->environment.preset, &pi)) return IXG_ROUTE_UNSUPPORTED_ENVIRONMENT;
     if (!std::isfinite(s->environment.backdrop)) return IXG_ROUTE_UNSUPPORTED_ENVIRONMENT;
-    if (s->material->scattering_sigma_s > 0.0f && pi.model == ixg::ENV_DAYLIGHT_SUN)
-        return IXG_ROUTE_UNSUPPORTED_ENVIRONMENT;  // the sun is sampled by next-event estimation at scatters
-    // frosted facets: no environment whose sun the reference samples by
-    // next-event estimation at a frosted event
-    if (any_frosted(s) && pi.model == ixg::ENV_DAYLIGHT_SUN) return IXG_ROUTE_UNSUPPORTED_ENVIRONMENT;
+    // DaylightSun: the sun disc is light-sampled at exterior frosted events
+    // (built); scatters light-sample only a panorama (hg_nee.rs:160-162)
     return IXG_ROUTE_OK;
 }
 

@@ -223,6 +223,9 @@ struct PathState {
     float n_o[NCH];
     float alpha_o[NCH], alpha_e[NCH];  // spectral_absorption, absorbing materials only
     float alpha_b[NCH];               // of the beta bands (trichroic biaxial stones)
+    float nee[NCH];                   // light-sample deposits (EXTRA kernels: frosted exterior events)
+    bool has_mis;                     // the light-sample MIS carry of an exterior frosted bounce ...
+    float mis_phase;                  // ... and its BSDF density, for the next bounce's miss
     uint64_t compat;          // byte c = MIS family of channel c
     V3 org, dir, observer, prev_plane;
     V3 k;                     // wave normal (uniaxial materials; dir is the Poynting direction)
@@ -302,6 +305,10 @@ __device__ __forceinline__ void path_start(V3 org, V3 dir, uint32_t seed, PathSt
         S.split[c] = 0.0f;
         S.st[c] = make_float4(1.0f, 0.0f, 0.0f, 0.0f);
     }
+    S.has_mis = false;
+    S.mis_phase = 0.0f;
+#pragma unroll
+    for (int c = 0; c < NCH; ++c) S.nee[c] = 0.0f;
     S.fh = 2166136261u;
     S.n_tir = S.n_refl = S.n_refr = S.n_probe = S.n_rot = 0;
     S.n_scan = S.n_probe_scan = 0;
@@ -342,6 +349,65 @@ __device__ __forceinline__ float env_radiance(const EnvParams& E, V3 dir, V3 obs
     DirLight L;
     direction_lighting(E, dir, observer, L);
     return light_radiance(E, L, spec_power(E, lambda));
+}
+
+// The analytic sun (environment/sun.rs): cone sampling of its disc, its pdf,
+// and next-event estimation at an exterior frosted event
+// (nee_contribution_frosted_exterior, scattering/frosted.rs:98-182) with
+// sample_environment_for_nee's DaylightSun light sample (environment/nee.rs:81-120).
+#define FROSTED_NEE_ENV_DIR_U_STREAM 0x6A09E667u
+#define FROSTED_NEE_ENV_DIR_V_STREAM 0xBB67AE85u
+#define SUN_DISC_COS 0.9999889f
+#define SUN_ONE_MINUS_COS 1.1086464e-5f
+#define SUN_SOLID_ANGLE 6.965831e-5f
+
+__device__ __forceinline__ V3 sun_cone_direction(V3 key, float u0, float u1) {
+    const float omc = u0 * SUN_ONE_MINUS_COS;
+    const float cos_t = 1.0f - omc;
+    const float sin_t = sqrt_rn(fmaxf(omc * (2.0f - omc), 0.0f));
+    const float phi = (2.0f * 3.14159265358979323846f) * u1;
+    const float sp = (float)sin((double)phi), cp = (float)cos((double)phi);
+    const V3 a = fabsf(key.x) > 0.9f ? v3(0.0f, 1.0f, 0.0f) : v3(1.0f, 0.0f, 0.0f);
+    const V3 t = normalize_or_zero(a - key * dot(key, a));
+    const V3 b = cross(key, t);
+    return normalize_or_zero((t * (sin_t * cp) + b * (sin_t * sp)) + key * cos_t);
+}
+
+// balance_heuristic (scattering/mod.rs:102-105)
+__device__ __forceinline__ float balance_heuristic(float a, float b) {
+    const float denom = a + b;
+    return denom > 1e-12f ? div_rn(a, denom) : 0.0f;
+}
+
+// environment_nee_pdf (environment/nee.rs:144-157): the sun disc's 1 / solid
+// angle under DaylightSun, 0 for every other studio model
+__device__ __forceinline__ float environment_nee_pdf(const EnvParams& E, V3 dir) {
+    if (E.model != ENV_DAYLIGHT_SUN) return 0.0f;
+    return dot(dir, v3(E.key[0], E.key[1], E.key[2])) >= SUN_DISC_COS ? div_rn(1.0f, SUN_SOLID_ANGLE) : 0.0f;
+}
+
+// apply_frosted_bounce's exterior side (frosted.rs:331-341, 372-382): the sun
+// light sample deposited into S.nee, and the MIS carry for the next miss.
+__device__ __noinline__ void frosted_exterior_light(const FilmParams& P, PathState& S, V3 ext_normal, V3 new_dir,
+                                                    uint32_t bounce) {
+    S.has_mis = true;
+    S.mis_phase = div_rn(dot(new_dir, ext_normal), 3.14159265358979323846f);
+    if (P.env.model != ENV_DAYLIGHT_SUN) return;  // no light sample
+    const float u0 = unit_draw(S.seed, bounce, FROSTED_NEE_ENV_DIR_U_STREAM);
+    const float u1 = unit_draw(S.seed, bounce, FROSTED_NEE_ENV_DIR_V_STREAM);
+    const V3 dir = sun_cone_direction(v3(P.env.key[0], P.env.key[1], P.env.key[2]), u0, u1);
+    const float pdf = div_rn(1.0f, SUN_SOLID_ANGLE);
+    const float cos_light = dot(dir, ext_normal);
+    if (cos_light <= 0.0f) return;
+    const float brdf_pdf = div_rn(cos_light, 3.14159265358979323846f);
+    const float mis = balance_heuristic(pdf, brdf_pdf);
+    if (mis <= 0.0f) return;
+    const float common = div_rn(brdf_pdf * mis, pdf);
+#pragma unroll
+    for (int c = 0; c < NCH; ++c) {
+        const float env = P.env.sun * (spec_power(P.env, comb_lambda(S.hero, c)) * P.env.exposure);
+        S.nee[c] = fma_(fmaxf(S.st[c].x, 0.0f) * common * env, 1.0f, S.nee[c]);
+    }
 }
 
 // A ray leaving facet f of a convex polyhedron outward cannot meet it again:
@@ -552,8 +618,9 @@ __device__ __forceinline__ void narrow_compat(uint64_t& compat, uint32_t match_m
 // side with every channel depolarised; a transmission into the stone draws its
 // eigenmode 50/50 (upd).
 template <bool EVENTS>
-__device__ __forceinline__ V3 frosted_arm_aniso(PathState& S, uint32_t bounce, int facet, V3 normal, float cos_i,
-                                                float n1, float n2, float sin2_t, bool& new_inside, int& upd) {
+__device__ __forceinline__ V3 frosted_arm_aniso(const FilmParams& P, PathState& S, uint32_t bounce, int facet,
+                                                V3 normal, float cos_i, float n1, float n2, float sin2_t,
+                                                bool& new_inside, int& upd) {
     const float u1 = unit_draw(S.seed, bounce, FROSTED_DIR_U_STREAM);
     const float u2 = unit_draw(S.seed, bounce, FROSTED_DIR_V_STREAM);
     uint32_t kind = 4u;
@@ -579,6 +646,9 @@ __device__ __forceinline__ V3 frosted_arm_aniso(PathState& S, uint32_t bounce, i
         S.st[c] = make_float4(fmaxf(S.st[c].x, 0.0f), 0.0f, 0.0f, 0.0f);
         if (kind != 4u) S.path_pdf[c] *= pdf;
     }
+    // the exterior side: reflected while outside, or transmitted out
+    if (kind == 5u && !S.inside) frosted_exterior_light(P, S, normal, dir, bounce);
+    if (kind == 6u && S.inside) frosted_exterior_light(P, S, neg(normal), dir, bounce);
     if (EVENTS) S.fh = fnv(fnv(S.fh, (uint32_t)facet), kind);
     return dir;
 }
@@ -659,7 +729,7 @@ __device__ __forceinline__ int facet_aniso(const FilmParams& P, const float4* __
     const bool frosted = EXTRA && P.frosted && facet_frosted(planes, facet);
     if (frosted) {
         // dispatch_bounce's frosted arm (transport/bounce.rs:137-160)
-        new_k = frosted_arm_aniso<EVENTS>(S, bounce, facet, normal, cos_i, n1, n2, sin2_t, new_inside, upd);
+        new_k = frosted_arm_aniso<EVENTS>(P, S, bounce, facet, normal, cos_i, n1, n2, sin2_t, new_inside, upd);
         new_s = new_k;
     } else if (sin2_t > 1.0f) {
         // apply_tir_bounce (tir.rs:56-160)
@@ -1121,7 +1191,7 @@ __device__ __forceinline__ int facet_biax(const FilmParams& P, const float4* __r
     const bool frosted = EXTRA && P.frosted && facet_frosted(planes, facet);
     if (frosted) {
         // dispatch_bounce's frosted arm (transport/bounce.rs:137-160)
-        new_k = frosted_arm_aniso<EVENTS>(S, bounce, facet, normal, cos_i, n1, n2, sin2_t, new_inside, upd);
+        new_k = frosted_arm_aniso<EVENTS>(P, S, bounce, facet, normal, cos_i, n1, n2, sin2_t, new_inside, upd);
         new_s = new_k;
     } else if (sin2_t > 1.0f) {
         // apply_tir_bounce, scalar arm (tir.rs:56-160)
@@ -1389,27 +1459,32 @@ __device__ __forceinline__ int scatter_step(const FilmParams& P, PathState& S, f
 // exit-split staging.
 template <bool ISO, bool EXTRA>
 __device__ __forceinline__ void path_miss(const FilmParams& P, PathState& S) {
+    // the light-sample MIS carry of an exterior frosted bounce (bounce_body/miss.rs:22-26):
+    // the balance heuristic against the environment's light-sample density
+    float w = 1.0f;
+    if (EXTRA && S.has_mis) w = balance_heuristic(S.mis_phase, environment_nee_pdf(P.env, S.dir));
+    if (EXTRA) S.has_mis = false;
     if (EXTRA && !ISO && P.env.model == ENV_HDR) {  // the panorama: one texel, per-channel spectrum
         float rgb[3];
         hdr_rgb(P.env, S.dir, rgb);
 #pragma unroll
         for (int c = 0; c < NCH; ++c) {
             float env = rgb_to_spectral(rgb, comb_lambda(S.hero, c));
-            S.split[c] = fma_(fmaxf(S.st[c].x, 0.0f) * 1.0f, env, 0.0f) + S.split[c];
+            S.split[c] = fma_(fmaxf(S.st[c].x, 0.0f) * (EXTRA ? w : 1.0f), env, EXTRA ? S.nee[c] : 0.0f) + S.split[c];
         }
     } else if (!ISO && P.env.model == ENV_ASET) {
         const V3 d = normalize(S.dir);
 #pragma unroll
         for (int c = 0; c < NCH; ++c) {
             float env = aset_radiance(P.env, d, comb_lambda(S.hero, c), P.env.exposure);
-            S.split[c] = fma_(fmaxf(S.st[c].x, 0.0f) * 1.0f, env, 0.0f) + S.split[c];
+            S.split[c] = fma_(fmaxf(S.st[c].x, 0.0f) * (EXTRA ? w : 1.0f), env, EXTRA ? S.nee[c] : 0.0f) + S.split[c];
         }
     } else if (ISO || P.env.model != ENV_STUDIO) {
         const float f = lit_factor(P.env, S.dir, S.observer);
 #pragma unroll
         for (int c = 0; c < NCH; ++c) {
             float env = f * (spec_power(P.env, comb_lambda(S.hero, c)) * P.env.exposure);
-            S.split[c] = fma_(fmaxf(S.st[c].x, 0.0f) * 1.0f, env, 0.0f) + S.split[c];
+            S.split[c] = fma_(fmaxf(S.st[c].x, 0.0f) * (EXTRA ? w : 1.0f), env, EXTRA ? S.nee[c] : 0.0f) + S.split[c];
         }
     } else {
         DirLight L;
@@ -1417,7 +1492,7 @@ __device__ __forceinline__ void path_miss(const FilmParams& P, PathState& S) {
 #pragma unroll
         for (int c = 0; c < NCH; ++c) {
             float env = light_radiance(P.env, L, spec_power(P.env, comb_lambda(S.hero, c)));
-            S.split[c] = fma_(fmaxf(S.st[c].x, 0.0f) * 1.0f, env, 0.0f) + S.split[c];
+            S.split[c] = fma_(fmaxf(S.st[c].x, 0.0f) * (EXTRA ? w : 1.0f), env, EXTRA ? S.nee[c] : 0.0f) + S.split[c];
         }
     }
 }
@@ -1442,6 +1517,7 @@ __device__ __forceinline__ int path_bounce(const FilmParams& P, const float4* __
         path_miss<ISO, EXTRA>(P, S);
         return STEP_ESCAPED;
     }
+    if (EXTRA) S.has_mis = false;  // the carry is this bounce's only (bounce_body.rs:372)
 
     if (EXTRA && P.sigma_s > 0.0f && S.inside) {  // run_bounce (bounce_body.rs:410-416)
         const int r = scatter_step<EVENTS, ABSORB, ANISO>(P, S, t_hit);
@@ -1518,6 +1594,8 @@ __device__ __forceinline__ int path_bounce(const FilmParams& P, const float4* __
             S.st[c] = make_float4(fmaxf(S.st[c].x, 0.0f), 0.0f, 0.0f, 0.0f);
             if (!hero_tir) S.path_pdf[c] *= pdf;
         }
+        // the exterior side: reflected while outside, or transmitted out
+        if (!hero_tir && through == inside) frosted_exterior_light(P, S, through ? neg(normal) : normal, new_k, bounce);
         if (through) new_inside = !inside;
         if (EVENTS) S.fh = fnv(fnv(S.fh, (uint32_t)facet), hero_tir ? 4u : (through ? 6u : 5u));
     } else if (reflect) {
@@ -1682,7 +1760,7 @@ __device__ __forceinline__ void finish_xyz(const FilmParams& P, uint64_t compat,
 __device__ __forceinline__ void path_finish(const FilmParams& P, const PathState& S, bool escaped, float xyz[3]) {
     float radiance[NCH];
 #pragma unroll
-    for (int c = 0; c < NCH; ++c) radiance[c] = escaped ? S.split[c] : 0.0f;
+    for (int c = 0; c < NCH; ++c) radiance[c] = escaped ? S.split[c] : S.nee[c];
     float lambda[NCH];
     hero_comb(S.hero, lambda);
     finish_xyz(P, S.compat, S.path_pdf, lambda, radiance, xyz);

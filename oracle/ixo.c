@@ -192,6 +192,8 @@ static mat4 mueller_tir_retardation(float delta) {
 #define PHASE_DIR_V_STREAM 0x1CE4E5B9u
 #define FROSTED_DIR_V_STREAM 0x165667B1u
 #define RUSSIAN_ROULETTE_STREAM 0x517CC1B7u
+#define FROSTED_NEE_ENV_DIR_U_STREAM 0x6A09E667u /* sampling.rs:52-53 */
+#define FROSTED_NEE_ENV_DIR_V_STREAM 0xBB67AE85u
 #define PIXEL_JITTER_X_ROTATION_STREAM 0xA511E9B3u
 #define PIXEL_JITTER_Y_ROTATION_STREAM 0x63D81B23u
 #define HERO_WAVELENGTH_ROTATION_STREAM 0x1B873593u
@@ -932,6 +934,8 @@ typedef struct {
     float n_o[NCH];
     float alpha_o[NCH], alpha_e[NCH]; /* spectral_absorption of o / e bands */
     float alpha_b[NCH];               /* of the beta bands (trichroic biaxial) */
+    int has_mis;        /* the light-sample MIS carry of an exterior frosted bounce ... */
+    float mis_phase;    /* ... and its BSDF density, for the next bounce's miss */
     uint8_t compat[NCH];
     v3 org, dir, k;     /* ray origin, ray (Poynting) direction S, wave normal k */
     int inside;
@@ -954,6 +958,79 @@ static v3 cosine_weighted_hemisphere(float u1, float u2, v3 n) {
     const v3 t = vnormalize_or_zero(vsub(a, vmul(n, vdot(n, a))));
     const v3 b = vcross(n, t);
     return vnormalize_or_zero(vadd(vadd(vmul(t, r * cos_t), vmul(b, r * sin_t)), vmul(n, sqrtf(maxf(1.0f - u1, 0.0f)))));
+}
+
+/* The analytic sun (environment/sun.rs): its disc, cone sampling and pdf,
+ * and next-event estimation at an exterior frosted event
+ * (nee_contribution_frosted_exterior, scattering/frosted.rs:98-182) with the
+ * light sample of sample_environment_for_nee (environment/nee.rs:81-120) --
+ * DaylightSun's disc; the panorama's importance sampling is not carried. */
+#define SUN_DISC_COS 0.9999889f
+#define SUN_ONE_MINUS_COS 1.1086464e-5f
+#define SUN_SOLID_ANGLE 6.965831e-5f
+
+static v3 sun_cone_direction(v3 key, float u0, float u1) {
+    const float omc = u0 * SUN_ONE_MINUS_COS;
+    const float cos_t = 1.0f - omc;
+    const float sin_t = sqrtf(maxf(omc * (2.0f - omc), 0.0f));
+    const float phi = 2.0f * F32_PI * u1;
+    const float sp = sin_cr(phi), cp = cos_cr(phi);
+    const v3 a = fabsf(key.x) > 0.9f ? mk(0.0f, 1.0f, 0.0f) : mk(1.0f, 0.0f, 0.0f);
+    const v3 t = vnormalize_or_zero(vsub(a, vmul(key, vdot(key, a))));
+    const v3 b = vcross(key, t);
+    return vnormalize_or_zero(vadd(vadd(vmul(t, sin_t * cp), vmul(b, sin_t * sp)), vmul(key, cos_t)));
+}
+
+void ixo_sun_cone_direction(const float key[3], float u0, float u1, float out[3]) {
+    const v3 d = sun_cone_direction(vfromr(key), u0, u1);
+    out[0] = d.x;
+    out[1] = d.y;
+    out[2] = d.z;
+}
+
+/* balance_heuristic (scattering/mod.rs:102-105) */
+static float balance_heuristic(float a, float b) {
+    const float denom = a + b;
+    return denom > 1e-12f ? a / denom : 0.0f;
+}
+
+/* environment_nee_pdf (environment/nee.rs:144-157): the sun disc's 1 / solid
+ * angle under DaylightSun, 0 for every other studio model */
+static float environment_nee_pdf(const scene_t* S, v3 dir) {
+    if (S->hdr || S->env.model != MODEL_SUN) return 0.0f;
+    return vdot(dir, S->env.key_dir) >= SUN_DISC_COS ? 1.0f / SUN_SOLID_ANGLE : 0.0f;
+}
+
+static void nee_frosted_exterior(const scene_t* S, path_t* P, v3 ext_normal, uint32_t seed, uint32_t bounce) {
+    if (S->hdr || S->env.model != MODEL_SUN) return; /* no light sample */
+    const float u0 = unit_draw(seed, bounce, FROSTED_NEE_ENV_DIR_U_STREAM);
+    const float u1 = unit_draw(seed, bounce, FROSTED_NEE_ENV_DIR_V_STREAM);
+    const v3 dir = sun_cone_direction(S->env.key_dir, u0, u1);
+    const float pdf = 1.0f / SUN_SOLID_ANGLE;
+    const float cos_light = vdot(dir, ext_normal);
+    if (cos_light <= 0.0f) return;
+    const float brdf_pdf = cos_light / F32_PI;
+    const float mis = balance_heuristic(pdf, brdf_pdf);
+    if (mis <= 0.0f) return;
+    const float common = brdf_pdf * mis / pdf;
+    for (int c = 0; c < NCH; ++c) {
+        const float env = S->env.sun * (spec_power(&S->env, P->lambdas[c]) * S->env.exposure);
+        P->radiance[c] = fmaf(st_intensity(P->st[c]) * common * env, 1.0f, P->radiance[c]);
+    }
+}
+
+/* apply_frosted_bounce's exterior side (frosted.rs:331-341, 372-382): the
+ * light sample, and the MIS carry the next bounce's miss weighs against */
+/* The reference's enable_nee switch (transport/inner.rs:240-290), on in every
+ * render; the tests turn it off to check the estimator without light samples. */
+int ixo_enable_nee = 1;
+
+static void frosted_exterior_light(const scene_t* S, path_t* P, v3 ext_normal, v3 new_dir, uint32_t seed,
+                                   uint32_t bounce) {
+    if (!ixo_enable_nee) return;
+    nee_frosted_exterior(S, P, ext_normal, seed, bounce);
+    P->has_mis = 1;
+    P->mis_phase = vdot(new_dir, ext_normal) / F32_PI;
 }
 
 void ixo_cosine_hemisphere(float u1, float u2, const float n[3], float out[3]) {
@@ -1305,6 +1382,7 @@ static void facet_event(const scene_t* S, path_t* P, uint32_t bounce, uint32_t s
                     P->st[c] = st_unpolarized(P->st[c]);
                     P->path_pdf[c] *= r_unpol;
                 }
+                if (!P->inside) frosted_exterior_light(S, P, normal, new_k, seed, bounce);
                 ev_facet(ev, facet, 5);
             } else {
                 new_k = cosine_weighted_hemisphere(u1, u2, vneg(normal));
@@ -1313,6 +1391,7 @@ static void facet_event(const scene_t* S, path_t* P, uint32_t bounce, uint32_t s
                     P->st[c] = st_unpolarized(P->st[c]);
                     P->path_pdf[c] *= t_unpol;
                 }
+                if (P->inside) frosted_exterior_light(S, P, vneg(normal), new_k, seed, bounce);
                 new_inside = !P->inside;
                 ev_facet(ev, facet, 6);
             }
@@ -1579,8 +1658,9 @@ static v3 poynting_for_mode(const scene_t* S, const ageo_t* G, v3 k, int inside,
  * the reflect or far side with every channel depolarised, and a transmission
  * into the stone draws its eigenmode 50/50.  *upd receives the entry's
  * eigenmode (or stays -1). */
-static v3 frosted_arm_aniso(path_t* P, uint32_t bounce, uint32_t seed, int32_t facet, v3 normal, float cos_i,
-                            float n1, float n2, float sin2_t, int* new_inside, int* upd, ixg_path_events* ev) {
+static v3 frosted_arm_aniso(const scene_t* S, path_t* P, uint32_t bounce, uint32_t seed, int32_t facet, v3 normal,
+                            float cos_i, float n1, float n2, float sin2_t, int* new_inside, int* upd,
+                            ixg_path_events* ev) {
     const float u1 = unit_draw(seed, bounce, FROSTED_DIR_U_STREAM);
     const float u2 = unit_draw(seed, bounce, FROSTED_DIR_V_STREAM);
     if (sin2_t > 1.0f) {
@@ -1593,12 +1673,14 @@ static v3 frosted_arm_aniso(path_t* P, uint32_t bounce, uint32_t seed, int32_t f
     fresnel_rs_rp(n1, n2, cos_i, cos_t, &rs, &rp);
     const float r_unpol = clampf(0.5f * fmaf(rp, rp, rs * rs), R_UNPOL_PDF_MIN, R_UNPOL_PDF_MAX);
     if (unit_draw(seed, bounce, FRESNEL_BRANCH_STREAM) < r_unpol) {
+        const v3 dir = cosine_weighted_hemisphere(u1, u2, normal);
         for (int c = 0; c < NCH; ++c) {
             P->st[c] = st_unpolarized(P->st[c]);
             P->path_pdf[c] *= r_unpol;
         }
+        if (!P->inside) frosted_exterior_light(S, P, normal, dir, seed, bounce);
         ev_facet(ev, facet, 5);
-        return cosine_weighted_hemisphere(u1, u2, normal);
+        return dir;
     }
     const v3 dir = cosine_weighted_hemisphere(u1, u2, vneg(normal));
     const float t_unpol = 1.0f - r_unpol;
@@ -1606,6 +1688,7 @@ static v3 frosted_arm_aniso(path_t* P, uint32_t bounce, uint32_t seed, int32_t f
         P->st[c] = st_unpolarized(P->st[c]);
         P->path_pdf[c] *= t_unpol;
     }
+    if (P->inside) frosted_exterior_light(S, P, vneg(normal), dir, seed, bounce);
     if (!P->inside) *upd = unit_draw(seed, bounce, BIREFRINGENT_SPLIT_STREAM) < 0.5f;
     *new_inside = !P->inside;
     ev_facet(ev, facet, 6);
@@ -1685,7 +1768,7 @@ static void facet_event_aniso(const scene_t* S, path_t* P, uint32_t bounce, uint
     uint32_t match_bits = 0u; /* channels whose refracted direction matched the hero's */
     const int frosted = S->finish != NULL && S->finish[facet] != 0;
     if (frosted) { /* dispatch_bounce's frosted arm (transport/bounce.rs:137-160) */
-        new_k = frosted_arm_aniso(P, bounce, seed, facet, normal, G.cos_i, G.n1, G.n2, G.sin2_t, &new_inside, &upd, ev);
+        new_k = frosted_arm_aniso(S, P, bounce, seed, facet, normal, G.cos_i, G.n1, G.n2, G.sin2_t, &new_inside, &upd, ev);
         new_s = new_k;
     } else if (G.sin2_t > 1.0f) {
         /* apply_tir_bounce (refraction/tir.rs:56-160) */
@@ -2135,7 +2218,7 @@ static void facet_event_biax(const scene_t* S, path_t* P, uint32_t bounce, uint3
     uint32_t match_bits = 0u;
     const int frosted = S->finish != NULL && S->finish[facet] != 0;
     if (frosted) { /* dispatch_bounce's frosted arm (transport/bounce.rs:137-160) */
-        new_k = frosted_arm_aniso(P, bounce, seed, facet, normal, cos_i, n1, n2, sin2_t, &new_inside, &upd, ev);
+        new_k = frosted_arm_aniso(S, P, bounce, seed, facet, normal, cos_i, n1, n2, sin2_t, &new_inside, &upd, ev);
         new_s = new_k;
     } else if (sin2_t > 1.0f) {
         /* apply_tir_bounce, scalar arm (refraction/tir.rs:56-160) */
@@ -2448,19 +2531,23 @@ static void trace_core(const scene_t* S, v3 origin, v3 dir, uint32_t seed, float
     int stopped = 0;
     for (bounce = 0; bounce < S->max_bounces; ++bounce) {
         float o[3] = {P.org.x, P.org.y, P.org.z}, d[3] = {P.dir.x, P.dir.y, P.dir.z}, t;
+        const int has_mis = P.has_mis; /* pending_light_mis.take() (bounce_body.rs:372) */
+        const float mis_phase = P.mis_phase;
+        P.has_mis = 0;
         int32_t f = scene_intersect(S, o, d, &t);
         if (f < 0) { /* handle_miss (bounce_body/miss.rs:33-54) */
+            const float w = has_mis ? balance_heuristic(mis_phase, environment_nee_pdf(S, P.dir)) : 1.0f;
             if (S->hdr) { /* sample_environment_channels' HdrMap arm (rig.rs:756-765) */
                 float rgb[3];
                 hdr_rgb(S->hdr, P.dir, rgb);
                 for (int c = 0; c < NCH; ++c)
-                    P.radiance[c] = fmaf(st_intensity(P.st[c]) * 1.0f, ixo_rgb_to_spectral(rgb, P.lambdas[c]),
+                    P.radiance[c] = fmaf(st_intensity(P.st[c]) * w, ixo_rgb_to_spectral(rgb, P.lambdas[c]),
                                          P.radiance[c]);
             } else {
                 dir_light L = direction_lighting(&S->env, P.dir, P.observer);
                 for (int c = 0; c < NCH; ++c) {
                     float env = light_radiance(&L, spec_power(&S->env, P.lambdas[c]), S->env.exposure);
-                    P.radiance[c] = fmaf(st_intensity(P.st[c]) * 1.0f, env, P.radiance[c]);
+                    P.radiance[c] = fmaf(st_intensity(P.st[c]) * w, env, P.radiance[c]);
                 }
             }
             P.escaped = 1;

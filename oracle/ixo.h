@@ -58,6 +58,11 @@ float ixo_biax_resolve_entry(float na, float nb, float ng, const float axes[9], 
                              const float normal[3], float cos_i, float n_seed, int want_slow, float dir_out[3]);
 void ixo_poynting_direction(const float k[3], const float e[3], float out[3]);
 
+/* 1 (default): next-event estimation at exterior frosted events (DaylightSun's disc). */
+extern int ixo_enable_nee;
+/* sun_cone_direction (environment/sun.rs): a direction in the sun disc around key. */
+void ixo_sun_cone_direction(const float key[3], float u0, float u1, float out[3]);
+
 /* sample_studio_environment (RT/environment/mod.rs:653): observer = 0. */
 float ixo_sample_studio_environment(const float dir[3], float lambda_nm, int32_t preset,
                                     float exposure, float light_yaw, float light_pitch);

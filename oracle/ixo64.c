@@ -56,6 +56,8 @@
 #define ixo_biax_resolve_entry ixo64_biax_resolve_entry
 #define ixo_poynting_direction ixo64_poynting_direction
 #define ixo_blackbody ixo64_blackbody
+#define ixo_enable_nee ixo64_enable_nee
+#define ixo_sun_cone_direction ixo64_sun_cone_direction
 #define ixo_camera_new ixo64_camera_new
 #define ixo_cie_cmf ixo64_cie_cmf
 #define ixo_classify_ray ixo64_classify_ray

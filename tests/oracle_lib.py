@@ -129,6 +129,7 @@ def lib():
         L.ixo_biax_resolve_entry.argtypes = [C.c_float, C.c_float, C.c_float, f9, f3, f3, C.c_float, C.c_float,
                                              C.c_int, P(C.c_float)]
         L.ixo_poynting_direction.argtypes = [f3, f3, P(C.c_float)]
+        L.ixo_sun_cone_direction.argtypes = [f3, C.c_float, C.c_float, P(C.c_float)]
         L.ixo_probe_shape.restype = C.c_int
         L.ixo_probe_shape.argtypes = [C.c_int32, P(C.c_uint32), P(C.c_uint32)]
         L.ixo_probe.restype = C.c_int

@@ -71,7 +71,9 @@ def test_route_predicate():
     sc2 = tracer.FrameScene(sc.scene.camera, 8, 8, sc.planes, m, 16, sc.scene.environment)
     assert tracer.route(sc2) == abi.ROUTE_OK                  # uniaxial
     sc2.material_abi.biaxial = 1
-    assert tracer.route(sc2) == abi.ROUTE_UNSUPPORTED_MATERIAL
+    assert tracer.route(sc2) == abi.ROUTE_OK                  # biaxial
+    sc2.material_abi.cubic = 1
+    assert tracer.route(sc2) == abi.ROUTE_UNSUPPORTED_MATERIAL  # "biaxial" on a cubic stone
     for name in ("LightTent", "WhiteTray", "ShopLights", "WindowDaylight", "DaylightDome", "DaylightSun", "Aset"):
         env = tracer.LightingPreset.studio(name)
         sc3 = tracer.FrameScene(sc.scene.camera, 8, 8, sc.planes, tracer.GemMaterial.diamond(), 16, env)

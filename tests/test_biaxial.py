@@ -287,7 +287,9 @@ def test_route_for_biaxial_stones():
     assert tracer.route(biaxial_scene(m)) == abi.ROUTE_UNSUPPORTED_MATERIAL
     sc = biaxial_scene(tracer.GemMaterial.topaz())
     sc.set_finishes([1] + [0] * (len(sc.planes) - 1))
-    assert tracer.route(sc) != abi.ROUTE_OK
+    assert tracer.route(sc) == abi.ROUTE_OK                      # frosted biaxial stones route
+    sc.set_hdr_map(np.full((2, 4, 3), 1.0, np.float32))
+    assert tracer.route(sc) == abi.ROUTE_UNSUPPORTED_ENVIRONMENT  # ... but not under a panorama
 
 
 def test_biaxial_probe_dispatch_is_the_exported_functions():

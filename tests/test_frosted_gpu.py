@@ -81,3 +81,47 @@ def test_frosted_anisotropic_paths_match_oracle(backend, kind, preset):
         assert ok, res
     f_xyz = backend.film_samples(sc, 0, 8)   # the timed EXTRA instantiation, path for path
     assert f_xyz.tobytes() == g_xyz.tobytes()
+
+
+@pytest.mark.parametrize("kind", ["isotropic", "uniaxial", "biaxial"])
+def test_sun_light_samples_match_oracle(backend, kind):
+    """Under DaylightSun an exterior frosted event light-samples the sun disc
+    (nee_contribution_frosted_exterior) and the next miss weighs the
+    BSDF-sampled continuation by the balance heuristic: the deposits and the
+    weights path for path against the oracle."""
+    from indicatrix_amd import workloads
+    m = {"isotropic": tracer.GemMaterial.diamond, "uniaxial": lambda: workloads.c2_material(bands=True),
+         "biaxial": tracer.GemMaterial.alexandrite}[kind]()
+    fr = [1 if (i in GIRDLE or i >= 49) else 0 for i in range(74)]
+    cam = tracer.camera_new(0.35, 0.28, 5.0, 18.0, 32, 24, 8)
+    env = tracer.LightingPreset.studio("DaylightSun", 1.0, 0.4, 0.9)
+    sc = tracer.FrameScene(cam, 32, 24, tracer.standard_round_brilliant(), m, 12, env, facet_finishes=fr)
+    print(f"sun light samples {kind} 32x24x8", flush=True)
+    g_xyz, g_ev = backend.trace_paths(sc, 0, 8)
+    o_xyz, o_ev = oracle_lib.trace_paths(sc, 0, 8)
+    same = ((g_ev["bounces"] == o_ev["bounces"]) & (g_ev["termination"] == o_ev["termination"])
+            & (g_ev["facet_hash"] == o_ev["facet_hash"]))
+    a, b = g_xyz[same].astype(np.float64), o_xyz[same].astype(np.float64)
+    rel = np.abs(a - b) / np.maximum(np.abs(b).max(axis=1, keepdims=True), 1e-6)
+    print(f"  flip {1 - same.mean():.2e} q999 {np.quantile(rel, 0.999):.2e} max {rel.max():.2e}", flush=True)
+    if kind == "uniaxial":
+        assert 1.0 - same.mean() < 1e-2 and (rel.max(axis=1) > 1e-3).mean() < 1e-2
+    else:
+        assert 1.0 - same.mean() < 3e-3 and np.quantile(rel, 0.999) < 1e-4
+    f_xyz = backend.film_samples(sc, 0, 8)   # the timed EXTRA instantiation
+    assert f_xyz.tobytes() == g_xyz.tobytes()
+
+
+def test_scattering_under_the_sun_matches_oracle(backend):
+    """Scatters light-sample only a panorama: under DaylightSun a scattering
+    stone is the plain estimator, path for path."""
+    from test_scattering import scattering_scene
+    sc = scattering_scene(1.5, 0.6, preset="DaylightSun", size=32, spp=8)
+    g_xyz, g_ev = backend.trace_paths(sc, 0, 8)
+    o_xyz, o_ev = oracle_lib.trace_paths(sc, 0, 8)
+    same = ((g_ev["bounces"] == o_ev["bounces"]) & (g_ev["termination"] == o_ev["termination"])
+            & (g_ev["facet_hash"] == o_ev["facet_hash"]))
+    a, b = g_xyz[same].astype(np.float64), o_xyz[same].astype(np.float64)
+    rel = np.abs(a - b) / np.maximum(np.abs(b).max(axis=1, keepdims=True), 1e-6)
+    print(f"scattering under the sun: flip {1 - same.mean():.2e} q999 {np.quantile(rel, 0.999):.2e}", flush=True)
+    assert 1.0 - same.mean() < 3e-3 and np.quantile(rel, 0.999) < 1e-4

@@ -70,14 +70,14 @@ def test_dense_medium_changes_the_image():
 
 def test_route_for_scattering_materials():
     assert tracer.route(scattering_scene(2.0)) == abi.ROUTE_OK
-    assert tracer.route(scattering_scene(2.0, preset="DaylightSun")) == abi.ROUTE_UNSUPPORTED_ENVIRONMENT
+    assert tracer.route(scattering_scene(2.0, preset="DaylightSun")) == abi.ROUTE_OK   # no sun sample at scatters
     assert tracer.route(scattering_scene(2.0, g=1.0)) == abi.ROUTE_UNSUPPORTED_MATERIAL
     m = tracer.GemMaterial.diamond().to_abi()
     m.cubic = 0
     m.birefringence_delta = 0.009
     assert tracer.route(scattering_scene(2.0, material=m)) == abi.ROUTE_OK          # uniaxial
     assert tracer.route(scattering_scene(2.0, material=tracer.GemMaterial.alexandrite().to_abi())) == abi.ROUTE_OK
-    assert tracer.route(scattering_scene(2.0, material=m, preset="DaylightSun")) == abi.ROUTE_UNSUPPORTED_ENVIRONMENT
+    assert tracer.route(scattering_scene(2.0, material=m, preset="DaylightSun")) == abi.ROUTE_OK
     assert tracer.route(workloads.film_absorbing_scene(8, 8)) == abi.ROUTE_OK
 
 
