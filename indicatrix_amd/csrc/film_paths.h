@@ -1757,10 +1757,11 @@ __device__ __forceinline__ void finish_xyz(const FilmParams& P, uint64_t compat,
 
 // An escaped path's radiance is in S.split (the miss radiance added to the
 // exit-split staging when the path escaped); any other ending has none.
+template <bool EXTRA = true>
 __device__ __forceinline__ void path_finish(const FilmParams& P, const PathState& S, bool escaped, float xyz[3]) {
     float radiance[NCH];
 #pragma unroll
-    for (int c = 0; c < NCH; ++c) radiance[c] = escaped ? S.split[c] : S.nee[c];
+    for (int c = 0; c < NCH; ++c) radiance[c] = escaped ? S.split[c] : (EXTRA ? S.nee[c] : 0.0f);
     float lambda[NCH];
     hero_comb(S.hero, lambda);
     finish_xyz(P, S.compat, S.path_pdf, lambda, radiance, xyz);
@@ -1868,7 +1869,7 @@ __global__ __launch_bounds__(kBlock) IXG_WAVES_ATTR void film_persistent_kernel(
         if (need_mask && (__popcll(need_mask) >= IXG_FILM_REGEN_MIN || active_mask == 0ull)) {
             if (pending) {
                 float xyz[3];
-                path_finish(P, S, pending_escaped, xyz);
+                path_finish<EXTRA>(P, S, pending_escaped, xyz);
                 out_xyz[3 * (size_t)idx + 0] = xyz[0];
                 out_xyz[3 * (size_t)idx + 1] = xyz[1];
                 out_xyz[3 * (size_t)idx + 2] = xyz[2];
