@@ -550,7 +550,7 @@ __device__ __forceinline__ bool eigenmode_selection(V3 c, V3 cpn, V3 k, float4 h
 // try_split_exit_channel (exit_split.rs:128-163) for channel c leaving along dk.
 template <bool EVENTS, bool ISO, bool EXTRA>
 __device__ __forceinline__ void split_probe(const FilmParams& P, const float4* __restrict__ planes, PathState& S,
-                                            int facet, V3 hit, int c, V3 dk, float transmitted_i) {
+                                            float* split, int facet, V3 hit, int c, V3 dk, float transmitted_i) {
     if (EVENTS) ++S.n_probe;
     const V3 po = hit + dk * 1e-4f;
     bool escapes = leaves_convex(planes, facet, po, dk);
@@ -562,7 +562,7 @@ __device__ __forceinline__ void split_probe(const FilmParams& P, const float4* _
     if (!escapes) return;
     const float env = ISO ? lit_factor(P.env, dk, S.observer) * (spec_power(P.env, comb_lambda(S.hero, c)) * P.env.exposure)
                           : env_radiance<EXTRA>(P.env, dk, S.observer, comb_lambda(S.hero, c));
-    S.split[c] = fma_(fmaxf(transmitted_i, 0.0f) * 1.0f, env, S.split[c]);
+    split[c] = fma_(fmaxf(transmitted_i, 0.0f) * 1.0f, env, split[c]);
 }
 
 __device__ __forceinline__ V3 reflect_dir(V3 k, V3 normal) { return k - normal * (2.0f * dot(k, normal)); }
@@ -669,6 +669,18 @@ template <bool EVENTS, bool ISO, bool ABSORB, bool EXTRA = true>
 __device__ __forceinline__ int facet_aniso(const FilmParams& P, const float4* __restrict__ planes, PathState& S,
                                         float t_hit, int facet) {
     const uint32_t bounce = S.bounce;
+    // The per-channel state in local arrays for the event: the channel loops
+    // around the boundary solves stay rolled and index it at run time, which
+    // would otherwise pin the whole path state to private memory.
+    float4 st_l[NCH];
+    float pdf_l[NCH], n_o_l[NCH], split_l[NCH];
+#pragma unroll
+    for (int c = 0; c < NCH; ++c) {
+        st_l[c] = S.st[c];
+        pdf_l[c] = S.path_pdf[c];
+        n_o_l[c] = S.n_o[c];
+        split_l[c] = S.split[c];
+    }
     const V3 cax = v3(P.an.c_axis[0], P.an.c_axis[1], P.an.c_axis[2]);
     const V3 hit = S.org + S.dir * t_hit;
     const float4 pf = planes[facet];
@@ -682,7 +694,7 @@ __device__ __forceinline__ int facet_aniso(const FilmParams& P, const float4* __
         float c2, s2;
         rotation_double_angle(cos_psi, sin_psi, c2, s2);
 #pragma unroll
-        for (int c = 0; c < NCH; ++c) S.st[c] = frame_rotate(c2, s2, S.st[c]);
+        for (int c = 0; c < NCH; ++c) st_l[c] = frame_rotate(c2, s2, st_l[c]);
         if (EVENTS) ++S.n_rot;
     }
     S.prev_plane = cur_plane;
@@ -696,7 +708,7 @@ __device__ __forceinline__ int facet_aniso(const FilmParams& P, const float4* __
         const float scaled = t_hit * P.abs.path_scale;
 #pragma unroll
         for (int c = 0; c < NCH; ++c)
-            S.st[c] = scale4(S.st[c], exp_lane(-tensor_qf(P.abs, S.alpha_o[c], S.alpha_e[c], e) * scaled));
+            st_l[c] = scale4(st_l[c], exp_lane(-tensor_qf(P.abs, S.alpha_o[c], S.alpha_e[c], e) * scaled));
     }
     if (inside) normal = neg(normal);
 
@@ -704,15 +716,15 @@ __device__ __forceinline__ int facet_aniso(const FilmParams& P, const float4* __
     const float cos_i = clampf(dot(neg(k), normal), 0.0f, 1.0f);
     const float one_m_c2 = fma_(cos_i, -cos_i, 1.0f);
     const float sin_i = sqrt_rn(fmaxf(one_m_c2, 0.0f));
-    const float theta_c = theta_c_for_bounce(P, comb_lambda(S.hero, 0), S.n_o[0], k, normal, cax, cos_i, inside);
+    const float theta_c = theta_c_for_bounce(P, comb_lambda(S.hero, 0), n_o_l[0], k, normal, cax, cos_i, inside);
     float n_e_ch[NCH], n_med[NCH];
 #pragma unroll
     for (int c = 0; c < NCH; ++c) {
-        n_e_ch[c] = e_index_at(P, comb_lambda(S.hero, c), S.n_o[c]);
-        const float n_eff = effective_extraordinary_index(S.n_o[c], n_e_ch[c], theta_c);
-        n_med[c] = (inside && !is_extra) ? S.n_o[c] : n_eff;
+        n_e_ch[c] = e_index_at(P, comb_lambda(S.hero, c), n_o_l[c]);
+        const float n_eff = effective_extraordinary_index(n_o_l[c], n_e_ch[c], theta_c);
+        n_med[c] = (inside && !is_extra) ? n_o_l[c] : n_eff;
     }
-    const float n_o_hero = S.n_o[0];
+    const float n_o_hero = n_o_l[0];
     const float n_e_hero = n_o_hero + P.an.biref;
     const float n1 = inside ? n_med[0] : 1.0f;
     const float n2 = inside ? 1.0f : n_med[0];
@@ -729,7 +741,17 @@ __device__ __forceinline__ int facet_aniso(const FilmParams& P, const float4* __
     const bool frosted = EXTRA && P.frosted && facet_frosted(planes, facet);
     if (frosted) {
         // dispatch_bounce's frosted arm (transport/bounce.rs:137-160)
+#pragma unroll
+        for (int c = 0; c < NCH; ++c) {
+            S.st[c] = st_l[c];
+            S.path_pdf[c] = pdf_l[c];
+        }
         new_k = frosted_arm_aniso<EVENTS>(P, S, bounce, facet, normal, cos_i, n1, n2, sin2_t, new_inside, upd);
+#pragma unroll
+        for (int c = 0; c < NCH; ++c) {
+            st_l[c] = S.st[c];
+            pdf_l[c] = S.path_pdf[c];
+        }
         new_s = new_k;
     } else if (sin2_t > 1.0f) {
         // apply_tir_bounce (tir.rs:56-160)
@@ -738,16 +760,22 @@ __device__ __forceinline__ int facet_aniso(const FilmParams& P, const float4* __
             S.fh = fnv(fnv(S.fh, (uint32_t)facet), 1u);
         }
         if (nondegenerate) {
-#pragma unroll
+            // the solves over small local arrays (a loop the compiler may keep
+            // rolled), then the path state updated with constant channel indices,
+            // so the state need not live in indexable private memory
+            float r_tot[NCH];
             for (int c = 0; c < NCH; ++c) {
-                const InternalSol sol = internal_solve(n_med[c], S.n_o[c], n_e_ch[c], cax, F, !is_extra);
+                const InternalSol sol = internal_solve(n_med[c], n_o_l[c], n_e_ch[c], cax, F, !is_extra);
                 const float flux_inc = fmaxf(sol.flux_inc, 1e-12f);
                 const float ro = cnorm2(sol.r_o) * sol.flux_ro, re = cnorm2(sol.r_e) * sol.flux_re;
-                const float r_total = fminf((ro + re) / flux_inc, 1.0f);
-                S.st[c] = scale4(S.st[c], r_total);
-                const float ek = n_med[c] / 1.0f;
-                if (ek * ek * one_m_c2 <= 1.0f) S.path_pdf[c] *= clampf(r_total, R_UNPOL_PDF_MIN, R_UNPOL_PDF_MAX);
+                r_tot[c] = fminf((ro + re) / flux_inc, 1.0f);
                 if (c == 0) exact_p_o = ro / fmaxf(ro + re, 1e-12f);
+            }
+#pragma unroll
+            for (int c = 0; c < NCH; ++c) {
+                st_l[c] = scale4(st_l[c], r_tot[c]);
+                const float ek = n_med[c] / 1.0f;
+                if (ek * ek * one_m_c2 <= 1.0f) pdf_l[c] *= clampf(r_tot[c], R_UNPOL_PDF_MIN, R_UNPOL_PDF_MAX);
             }
         } else {
 #pragma unroll
@@ -758,11 +786,11 @@ __device__ __forceinline__ int facet_aniso(const FilmParams& P, const float4* __
                 if (s2k > 1.0f) {
                     float cd, sd;
                     tir_retardation(n1k, cos_i, sin_i, cd, sd);
-                    S.st[c] = mueller_tir(cd, sd, S.st[c]);
+                    st_l[c] = mueller_tir(cd, sd, st_l[c]);
                 } else {
                     float ru;
-                    S.st[c] = fresnel_reflect(n1k, n2k, cos_i, sqrt_rn(fmaxf(1.0f - s2k, 0.0f)), S.st[c], ru);
-                    S.path_pdf[c] *= ru;
+                    st_l[c] = fresnel_reflect(n1k, n2k, cos_i, sqrt_rn(fmaxf(1.0f - s2k, 0.0f)), st_l[c], ru);
+                    pdf_l[c] *= ru;
                 }
             }
         }
@@ -773,13 +801,13 @@ __device__ __forceinline__ int facet_aniso(const FilmParams& P, const float4* __
         EntryInc I;
         entry_incidence(F, I);
         EntrySol hsol;
-        entry_solve(I, S.n_o[0], n_e_ch[0], cax, F, hsol, false);
+        entry_solve(I, n_o_l[0], n_e_ch[0], cax, F, hsol, false);
         M4 Mh;
         jones_to_mueller(hsol.x[0][0], hsol.x[1][0], hsol.x[0][1], hsol.x[1][1], Mh);
         const float r_branch = clampf(Mh.m[0][0], R_UNPOL_SELECT_MIN, R_UNPOL_SELECT_MAX);
         const float inc_flux = 0.5f * F.cos_i;
-        const float p_o_raw = mode_power(hsol.x[0][2], hsol.x[1][2], hsol.flux_o, inc_flux, S.st[0]);
-        const float p_e_raw = mode_power(hsol.x[0][3], hsol.x[1][3], hsol.flux_e, inc_flux, S.st[0]);
+        const float p_o_raw = mode_power(hsol.x[0][2], hsol.x[1][2], hsol.flux_o, inc_flux, st_l[0]);
+        const float p_e_raw = mode_power(hsol.x[0][3], hsol.x[1][3], hsol.flux_e, inc_flux, st_l[0]);
         const float p_o_hero =
             clampf(p_o_raw / fmaxf(p_o_raw + p_e_raw, 1e-12f), R_UNPOL_SELECT_MIN, R_UNPOL_SELECT_MAX);
         const bool use_extra = unit_draw(S.seed, bounce, BIREFRINGENT_SPLIT_STREAM) < (1.0f - p_o_hero);
@@ -799,12 +827,12 @@ __device__ __forceinline__ int facet_aniso(const FilmParams& P, const float4* __
 #pragma unroll
             for (int c = 0; c < NCH; ++c) {
                 EntrySol sol;
-                entry_solve(I, S.n_o[c], n_e_ch[c], cax, F, sol, false);
+                entry_solve(I, n_o_l[c], n_e_ch[c], cax, F, sol, false);
                 M4 Mk;
                 jones_to_mueller(sol.x[0][0], sol.x[1][0], sol.x[0][1], sol.x[1][1], Mk);
                 const float ru = clampf(Mk.m[0][0], R_UNPOL_PDF_MIN, R_UNPOL_PDF_MAX);
-                S.st[c] = scale4(mat_apply(Mk, S.st[c]), inv_r);
-                S.path_pdf[c] *= ru;
+                st_l[c] = scale4(mat_apply(Mk, st_l[c]), inv_r);
+                pdf_l[c] *= ru;
             }
             new_k = reflect_dir(k, normal);
             new_s = new_k;
@@ -817,13 +845,13 @@ __device__ __forceinline__ int facet_aniso(const FilmParams& P, const float4* __
             uint32_t live = 0u, match_mask = 0u;
 #pragma unroll
             for (int c = 0; c < NCH; ++c) {
-                const float n_o_k = S.n_o[c];
+                const float n_o_k = n_o_l[c];
                 const float n2k = use_extra ? n_med[c] : n_o_k;
                 const float r = div_rn(n1, n2k);
                 const float s2k = r * r * one_m_c2;
                 if (s2k > 1.0f) {
-                    S.st[c] = scale4(S.st[c], 0.0f);
-                    S.path_pdf[c] = 0.0f;
+                    st_l[c] = scale4(st_l[c], 0.0f);
+                    pdf_l[c] = 0.0f;
                     continue;
                 }
                 const float ek = div_rn(n1, n2k);
@@ -838,8 +866,8 @@ __device__ __forceinline__ int facet_aniso(const FilmParams& P, const float4* __
                 jones_to_mueller(sol.x[0][0], sol.x[1][0], sol.x[0][1], sol.x[1][1], Mk);
                 const float t_unpol = clampf(1.0f - Mk.m[0][0], R_UNPOL_PDF_MIN, R_UNPOL_PDF_MAX);
                 if (!match) {
-                    S.st[c] = scale4(S.st[c], 0.0f);
-                    S.path_pdf[c] *= t_unpol;
+                    st_l[c] = scale4(st_l[c], 0.0f);
+                    pdf_l[c] *= t_unpol;
                     continue;
                 }
                 match_mask |= 1u << c;
@@ -848,11 +876,11 @@ __device__ __forceinline__ int facet_aniso(const FilmParams& P, const float4* __
                 const Cx tpr = use_extra ? sol.x[1][3] : sol.x[1][2];
                 const float flux_k = use_extra ? sol.flux_e : sol.flux_o;
                 const V3 mode_dir = use_extra ? sol.e_hat : sol.o_hat;
-                const float dep = mode_power(tsr, tpr, flux_k, inc_flux, S.st[c]) / p_mode_frac;
+                const float dep = mode_power(tsr, tpr, flux_k, inc_flux, st_l[c]) / p_mode_frac;
                 float c2, s2;
                 azimuth2_in_frame(mode_dir, F.s_axis, p_axis_t, c2, s2);
-                S.st[c] = scale4(make_float4(dep, dep * c2, dep * s2, 0.0f), inv_t);
-                S.path_pdf[c] *= t_unpol;
+                st_l[c] = scale4(make_float4(dep, dep * c2, dep * s2, 0.0f), inv_t);
+                pdf_l[c] *= t_unpol;
             }
             narrow_compat(S.compat, match_mask, live);
             new_k = refr;
@@ -862,7 +890,7 @@ __device__ __forceinline__ int facet_aniso(const FilmParams& P, const float4* __
         }
     } else if (inside && nondegenerate) {
         // apply_uniaxial_internal_bounce (uniaxial_internal.rs)
-        const InternalSol sh = internal_solve(n_med[0], S.n_o[0], n_e_ch[0], cax, F, !is_extra);
+        const InternalSol sh = internal_solve(n_med[0], n_o_l[0], n_e_ch[0], cax, F, !is_extra);
         const float flux_inc_h = fmaxf(sh.flux_inc, 1e-12f);
         const float ro = cnorm2(sh.r_o) * sh.flux_ro, re = cnorm2(sh.r_e) * sh.flux_re;
         const float r_branch = clampf((ro + re) / flux_inc_h, R_UNPOL_SELECT_MIN, R_UNPOL_SELECT_MAX);
@@ -872,14 +900,16 @@ __device__ __forceinline__ int facet_aniso(const FilmParams& P, const float4* __
                 ++S.n_refl;
                 S.fh = fnv(fnv(S.fh, (uint32_t)facet), 2u);
             }
+            float r_tot[NCH];
+            for (int c = 0; c < NCH; ++c) {  // the solves, then the state (as in the TIR arm)
+                const InternalSol sol = c == 0 ? sh : internal_solve(n_med[c], n_o_l[c], n_e_ch[c], cax, F, !is_extra);
+                const float flux_inc = fmaxf(sol.flux_inc, 1e-12f);
+                r_tot[c] = fminf(fma_(cnorm2(sol.r_e), sol.flux_re, cnorm2(sol.r_o) * sol.flux_ro) / flux_inc, 1.0f);
+            }
 #pragma unroll
             for (int c = 0; c < NCH; ++c) {
-                const InternalSol sol = c == 0 ? sh : internal_solve(n_med[c], S.n_o[c], n_e_ch[c], cax, F, !is_extra);
-                const float flux_inc = fmaxf(sol.flux_inc, 1e-12f);
-                const float r_total =
-                    fminf(fma_(cnorm2(sol.r_e), sol.flux_re, cnorm2(sol.r_o) * sol.flux_ro) / flux_inc, 1.0f);
-                S.st[c] = scale4(S.st[c], r_total / r_branch);
-                S.path_pdf[c] *= clampf(r_total, R_UNPOL_PDF_MIN, R_UNPOL_PDF_MAX);
+                st_l[c] = scale4(st_l[c], r_tot[c] / r_branch);
+                pdf_l[c] *= clampf(r_tot[c], R_UNPOL_PDF_MIN, R_UNPOL_PDF_MAX);
             }
             new_k = reflect_dir(k, normal);
             new_s = new_k;
@@ -894,32 +924,32 @@ __device__ __forceinline__ int facet_aniso(const FilmParams& P, const float4* __
             const float inv_t = div_rn(1.0f, 1.0f - r_branch);
 #pragma unroll
             for (int c = 0; c < NCH; ++c) {
-                const float orig_i = S.st[c].x;
+                const float orig_i = st_l[c].x;
                 const float n_inc = n_med[c];
                 const float r = n_inc / 1.0f;
                 const float s2k = r * r * one_m_c2;
                 if (s2k > 1.0f) {
-                    S.st[c] = scale4(S.st[c], 0.0f);
-                    S.path_pdf[c] = 0.0f;
+                    st_l[c] = scale4(st_l[c], 0.0f);
+                    pdf_l[c] = 0.0f;
                     continue;
                 }
                 const float ek = n_inc / 1.0f;
                 const V3 rk = refract_dir(k, normal, ek, cos_i, sqrt_rn(fmaxf(1.0f - s2k, 0.0f)));
                 const bool match = dot(rk, hero_refr) >= DIRECTION_MATCH_COS_TOL;
                 if (match) match_bits |= 1u << c;
-                const InternalSol sol = c == 0 ? sh : internal_solve(n_inc, S.n_o[c], n_e_ch[c], cax, F, !is_extra);
+                const InternalSol sol = c == 0 ? sh : internal_solve(n_inc, n_o_l[c], n_e_ch[c], cax, F, !is_extra);
                 float iu;
                 const float4 tr = exit_transmission(sol, orig_i, inv_t, iu);
                 const float t_unpol = clampf(iu, R_UNPOL_PDF_MIN, R_UNPOL_PDF_MAX);
                 if (!match) {
-                    const float prefix = S.path_pdf[c];
-                    S.st[c] = scale4(S.st[c], 0.0f);
-                    S.path_pdf[c] = prefix * t_unpol;
-                    if (orig_i > 0.0f) split_probe<EVENTS, ISO, EXTRA>(P, planes, S, facet, hit, c, rk, fmaxf(tr.x, 0.0f));
+                    const float prefix = pdf_l[c];
+                    st_l[c] = scale4(st_l[c], 0.0f);
+                    pdf_l[c] = prefix * t_unpol;
+                    if (orig_i > 0.0f) split_probe<EVENTS, ISO, EXTRA>(P, planes, S, split_l, facet, hit, c, rk, fmaxf(tr.x, 0.0f));
                     continue;
                 }
-                S.st[c] = tr;
-                S.path_pdf[c] *= t_unpol;
+                st_l[c] = tr;
+                pdf_l[c] *= t_unpol;
             }
             new_k = hero_refr;
             new_s = hero_refr;
@@ -933,7 +963,7 @@ __device__ __forceinline__ int facet_aniso(const FilmParams& P, const float4* __
         bool have_sel = false;
         if (entering) {
             float p;
-            have_sel = eigenmode_selection(cax, cur_plane, k, S.st[0], p, c2o, s2o);
+            have_sel = eigenmode_selection(cax, cur_plane, k, st_l[0], p, c2o, s2o);
             if (have_sel) p_o = p;
         }
         const float msr = unit_draw(S.seed, bounce, BIREFRINGENT_SPLIT_STREAM);
@@ -967,13 +997,13 @@ __device__ __forceinline__ int facet_aniso(const FilmParams& P, const float4* __
                 if (s2k > 1.0f) {
                     float cd, sd;
                     tir_retardation(n1k, cos_i, sin_i, cd, sd);
-                    st = mueller_tir(cd, sd, S.st[c]);
+                    st = mueller_tir(cd, sd, st_l[c]);
                 } else {
                     float ru;
-                    st = fresnel_reflect(n1k, n2k, cos_i, sqrt_rn(fmaxf(1.0f - s2k, 0.0f)), S.st[c], ru);
-                    S.path_pdf[c] *= ru;
+                    st = fresnel_reflect(n1k, n2k, cos_i, sqrt_rn(fmaxf(1.0f - s2k, 0.0f)), st_l[c], ru);
+                    pdf_l[c] *= ru;
                 }
-                S.st[c] = scale4(st, sc);
+                st_l[c] = scale4(st, sc);
             }
             new_k = reflect_dir(k, normal);
             new_s = (inside && is_extra) ? extraordinary_poynting_dir(new_k, cax, n_o_hero, n_e_hero) : new_k;
@@ -992,20 +1022,20 @@ __device__ __forceinline__ int facet_aniso(const FilmParams& P, const float4* __
             uint32_t live = 0u, match_mask = 0u;
 #pragma unroll
             for (int c = 0; c < NCH; ++c) {
-                const float4 orig = S.st[c];
+                const float4 orig = st_l[c];
                 const float n1k = inside ? n_med[c] : 1.0f;
-                const float n2k = (entering && !use_extra) ? S.n_o[c] : (inside ? 1.0f : n_med[c]);
+                const float n2k = (entering && !use_extra) ? n_o_l[c] : (inside ? 1.0f : n_med[c]);
                 const float r = div_rn(n1k, n2k);
                 const float s2k = r * r * one_m_c2;
                 if (s2k > 1.0f) {
-                    S.st[c] = scale4(S.st[c], 0.0f);
-                    S.path_pdf[c] = 0.0f;
+                    st_l[c] = scale4(st_l[c], 0.0f);
+                    pdf_l[c] = 0.0f;
                     continue;
                 }
                 const float ctk = sqrt_rn(fmaxf(1.0f - s2k, 0.0f));
                 const V3 rk = refract_dir(k, normal, div_rn(n1k, n2k), cos_i, ctk);
                 const V3 fk = (entering && use_extra)
-                                  ? extraordinary_poynting_dir(rk, cax, S.n_o[c], S.n_o[c] + P.an.biref)
+                                  ? extraordinary_poynting_dir(rk, cax, n_o_l[c], n_o_l[c] + P.an.biref)
                                   : rk;
                 const bool match = dot(fk, final_refr) >= DIRECTION_MATCH_COS_TOL;
                 // compute_channel_transmission (exit_split.rs:69-101)
@@ -1022,16 +1052,16 @@ __device__ __forceinline__ int facet_aniso(const FilmParams& P, const float4* __
                 fresnel_rs_rp(n1k, n2k, cos_i, ctk, a_, b_);
                 const float ruk = clampf(0.5f * fma_(b_, b_, a_ * a_), R_UNPOL_PDF_MIN, R_UNPOL_PDF_MAX);
                 if (match) {
-                    S.st[c] = tr;
-                    S.path_pdf[c] *= 1.0f - ruk;
+                    st_l[c] = tr;
+                    pdf_l[c] *= 1.0f - ruk;
                     match_mask |= 1u << c;
                     match_bits |= 1u << c;
                 } else {
-                    const float prefix = S.path_pdf[c];
-                    S.st[c] = scale4(S.st[c], 0.0f);
-                    S.path_pdf[c] = prefix * (1.0f - ruk);
+                    const float prefix = pdf_l[c];
+                    st_l[c] = scale4(st_l[c], 0.0f);
+                    pdf_l[c] = prefix * (1.0f - ruk);
                     if (inside && fmaxf(orig.x, 0.0f) > 0.0f)
-                        split_probe<EVENTS, ISO, EXTRA>(P, planes, S, facet, hit, c, rk, fmaxf(tr.x, 0.0f));
+                        split_probe<EVENTS, ISO, EXTRA>(P, planes, S, split_l, facet, hit, c, rk, fmaxf(tr.x, 0.0f));
                 }
                 if (!inside) channel_dir_put(c, fk);
                 live |= 1u << c;
@@ -1058,7 +1088,7 @@ __device__ __forceinline__ int facet_aniso(const FilmParams& P, const float4* __
             p_o = exact_p_o;
         } else {
             float p, c2, s2;
-            if (eigenmode_selection(cax, cur_plane, new_k, S.st[0], p, c2, s2)) p_o = p;
+            if (eigenmode_selection(cax, cur_plane, new_k, st_l[0], p, c2, s2)) p_o = p;
         }
         S.is_extra = unit_draw(S.seed, bounce, MODE_COUPLING_STREAM) >= p_o;
     }
@@ -1066,19 +1096,33 @@ __device__ __forceinline__ int facet_aniso(const FilmParams& P, const float4* __
     if (EVENTS) S.fh = fnv(fnv(S.fh, 16u + (S.is_extra ? 1u : 0u)), 256u + match_bits);
     if (bounce == 0 && !new_inside && P.surface_glare < 1.0f && !frosted) {
 #pragma unroll
-        for (int c = 0; c < NCH; ++c) S.st[c] = scale4(S.st[c], P.surface_glare);
+        for (int c = 0; c < NCH; ++c) st_l[c] = scale4(st_l[c], P.surface_glare);
     }
     if (bounce > 4) {
         float mx = 0.0f;
 #pragma unroll
-        for (int c = 0; c < NCH; ++c) mx = fmaxf(mx, fmaxf(S.st[c].x, 0.0f));
+        for (int c = 0; c < NCH; ++c) mx = fmaxf(mx, fmaxf(st_l[c].x, 0.0f));
         const float q = clampf(mx, 0.05f, 1.0f);
-        if (unit_draw(S.seed, bounce, RUSSIAN_ROULETTE_STREAM) > q) return STEP_ROULETTE;
+        if (unit_draw(S.seed, bounce, RUSSIAN_ROULETTE_STREAM) > q) {
+    #pragma unroll
+        for (int c = 0; c < NCH; ++c) {
+            S.st[c] = st_l[c];
+            S.path_pdf[c] = pdf_l[c];
+            S.split[c] = split_l[c];
+        }
+            return STEP_ROULETTE;
+        }
         const float inv_q = div_rn(1.0f, q);
 #pragma unroll
-        for (int c = 0; c < NCH; ++c) S.st[c] = scale4(S.st[c], inv_q);
+        for (int c = 0; c < NCH; ++c) st_l[c] = scale4(st_l[c], inv_q);
 #pragma unroll
-        for (int c = 0; c < NCH; ++c) S.split[c] = div_rn(S.split[c], q);
+        for (int c = 0; c < NCH; ++c) split_l[c] = div_rn(split_l[c], q);
+    }
+#pragma unroll
+    for (int c = 0; c < NCH; ++c) {
+        S.st[c] = st_l[c];
+        S.path_pdf[c] = pdf_l[c];
+        S.split[c] = split_l[c];
     }
     return STEP_CONTINUE;
 }
@@ -1306,7 +1350,7 @@ __device__ __forceinline__ int facet_biax(const FilmParams& P, const float4* __r
                     S.st[c] = scale4(S.st[c], 0.0f);
                     S.path_pdf[c] = prefix * (1.0f - ruk);
                     if (inside && fmaxf(orig.x, 0.0f) > 0.0f)
-                        split_probe<EVENTS, ISO, EXTRA>(P, planes, S, facet, hit, c, rk, fmaxf(tr.x, 0.0f));
+                        split_probe<EVENTS, ISO, EXTRA>(P, planes, S, S.split, facet, hit, c, rk, fmaxf(tr.x, 0.0f));
                 }
                 if (!inside) channel_dir_put(c, fk);
                 live |= 1u << c;
