@@ -34,7 +34,7 @@ HOST_CPU_SHARE = 16                # host threads one GPU's share of the box all
 # WRITE_SIZE x 1.0 in KiB (the scales a 1 GiB device copy reads), from
 # rocprofv3 --pmc runs recorded with their commands under profiles/.
 FILM_KERNEL_HBM_BYTES = (667.3 * 2.0 + 13433.8) * 1024           # film C1, 1 048 576 paths
-FILM_C2_KERNEL_HBM_BYTES = (3077492.1 * 2.0 + 10327946.6) * 1024  # film C2, 2^23 paths (register spills)
+FILM_C2_KERNEL_HBM_BYTES = (1033222.0 * 2.0 + 6310438.3) * 1024   # film C2, 2^23 paths (mostly private memory)
 SWEEP_KERNEL_HBM_BYTES = (10527.94 * 2.0 + 2603463.28) * 1024     # the 512-design sweep launch
 MLP_STEP_HBM_BYTES = (53473.5 * 2.0 + 56344.8) * 1024            # one training step's four launches
 
